@@ -1,0 +1,124 @@
+"""ctypes binding of cugmres_amd/libcugmres.so (the C ABI in include/cugmres.h).
+
+The library is built in-tree by ``python __graft_entry__.py`` / ``make -C
+cugmres_amd``.  There is no fallback: if the shared object is missing every
+entry point raises, so a GPU run can never silently use another path.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcugmres.so")
+
+_lib = None
+
+c_float_p = ctypes.POINTER(ctypes.c_float)
+c_double_p = ctypes.POINTER(ctypes.c_double)
+c_void_p = ctypes.c_void_p
+
+SUCCESS = 0
+ERROR_INVALID_VALUE = 1
+ERROR_DEVICE = 2
+ERROR_ALLOC = 3
+ERROR_NOT_CONVERGED = 4
+ERROR_NONFINITE = 5
+ERROR_UNSUPPORTED = 7
+ERROR_NOT_SYMMETRIC = 8
+
+OP_FORWARD = 0
+OP_ADJOINT = 1
+OP_GREEN = 2
+
+
+class GmresParams(ctypes.Structure):
+    _fields_ = [("restart", ctypes.c_int), ("max_cycles", ctypes.c_int),
+                ("tol", ctypes.c_double), ("adjoint", ctypes.c_int), ("x0_zero", ctypes.c_int)]
+
+
+class GmresResult(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("cycles", ctypes.c_int),
+                ("converged", ctypes.c_int), ("rel_residual", ctypes.c_double),
+                ("beta0", ctypes.c_double), ("bnorm", ctypes.c_double)]
+
+
+class RefineParams(ctypes.Structure):
+    _fields_ = [("inner_restart", ctypes.c_int), ("inner_max_cycles", ctypes.c_int),
+                ("inner_tol", ctypes.c_double), ("outer_tol", ctypes.c_double),
+                ("max_outer", ctypes.c_int)]
+
+
+class RefineResult(ctypes.Structure):
+    _fields_ = [("outer", ctypes.c_int), ("inner_iterations", ctypes.c_int),
+                ("converged", ctypes.c_int), ("rel_residual", ctypes.c_double)]
+
+
+# name -> (restype, argtypes); every symbol declared in include/cugmres.h
+SIGNATURES = {
+    "cgm_status_string": (ctypes.c_char_p, [ctypes.c_int]),
+    "cgm_last_error": (ctypes.c_char_p, []),
+    "cgm_version": (None, [ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
+    "cgm_create": (ctypes.c_int, [ctypes.POINTER(c_void_p), ctypes.c_int, ctypes.c_int]),
+    "cgm_destroy": (ctypes.c_int, [c_void_p]),
+    "cgm_set_stream": (ctypes.c_int, [c_void_p, c_void_p]),
+    "cgm_get_embedding": (ctypes.c_int, [c_void_p, ctypes.POINTER(ctypes.c_int)]),
+    "cgm_spectrum_size": (ctypes.c_int, [c_void_p, ctypes.POINTER(ctypes.c_size_t)]),
+    "cgm_spectrum_from_gprep": (ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
+    "cgm_spectrum_from_block": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "cgm_spectrum_reference": (ctypes.c_int, [c_void_p, ctypes.c_double, ctypes.c_double,
+                                              ctypes.c_double, c_void_p, c_void_p]),
+    "cgm_spectrum_helmholtz": (ctypes.c_int, [c_void_p, ctypes.c_double, c_void_p, c_void_p]),
+    "cgm_matvec": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int]),
+    "cgm_matvec_c128": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int]),
+    "cgm_gmres": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                 ctypes.POINTER(GmresParams), ctypes.POINTER(GmresResult), c_void_p]),
+    "cgm_gmres_refine": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                        ctypes.POINTER(RefineParams), ctypes.POINTER(RefineResult)]),
+    "cgm_reference_plane_wave": (ctypes.c_int, [c_void_p, c_void_p]),
+    "cgm_reference_chi": (ctypes.c_float, [ctypes.c_int]),
+    "cgm_set_mask_contrast": (ctypes.c_int, [c_void_p, ctypes.c_float]),
+    "cgm_n_timestamps": (ctypes.c_uint, [ctypes.c_uint]),
+    "cgm_Fast_GMRES_with_CUDA": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, ctypes.POINTER(c_void_p),
+                                                ctypes.POINTER(ctypes.c_uint), c_void_p, ctypes.c_float,
+                                                ctypes.c_bool, ctypes.c_uint, ctypes.POINTER(ctypes.c_bool),
+                                                ctypes.c_uint, c_void_p, ctypes.c_uint]),
+    "cgm_G_x_fft_matvec": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_uint]),
+    "cgm_G_x_fft_matvec_T": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_uint]),
+    "cgm_cuSolve_LES": (ctypes.c_int, [c_void_p, c_void_p, ctypes.c_int, c_void_p, c_void_p, c_void_p,
+                                       c_void_p]),
+    "cgm_malloc": (ctypes.c_int, [ctypes.POINTER(c_void_p), ctypes.c_size_t]),
+    "cgm_free": (ctypes.c_int, [c_void_p]),
+    "cgm_memcpy_h2d": (ctypes.c_int, [c_void_p, c_void_p, ctypes.c_size_t]),
+    "cgm_memcpy_d2h": (ctypes.c_int, [c_void_p, c_void_p, ctypes.c_size_t]),
+    "cgm_synchronize": (ctypes.c_int, [c_void_p]),
+    "cgm_bessel01": (None, [ctypes.c_double] + [c_double_p] * 4),
+}
+
+
+class CgmError(RuntimeError):
+    def __init__(self, status, where):
+        self.status = status
+        msg = lib().cgm_last_error().decode(errors="replace")
+        name = lib().cgm_status_string(status).decode()
+        super().__init__(f"{where}: {name} ({status}) {msg}")
+
+
+def lib():
+    """Load the shared library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `make -C cugmres_amd` "
+                              "or `python __graft_entry__.py`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status, where, ok=(SUCCESS,)):
+    if status not in ok:
+        raise CgmError(status, where)
+    return status

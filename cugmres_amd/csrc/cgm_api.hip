@@ -1,0 +1,458 @@
+// C ABI of the solver: contexts, spectra, operator, restarted GMRES, FP64
+// refinement and the reference-compatible entry points.  Host code; every
+// device operation is enqueued on the context stream.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+#include <time.h>
+
+#include <vector>
+
+#include "cgm_internal.h"
+#include "cgm_special.h"
+
+using namespace cgm;
+
+static thread_local char g_last_error[512] = "";
+
+void cgm_set_last_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_last_error, sizeof(g_last_error), fmt, ap);
+  va_end(ap);
+}
+
+extern "C" const char* cgm_last_error(void) { return g_last_error; }
+
+extern "C" const char* cgm_status_string(int s) {
+  switch (s) {
+    case CGM_SUCCESS: return "success";
+    case CGM_ERROR_INVALID_VALUE: return "invalid value";
+    case CGM_ERROR_DEVICE: return "device error";
+    case CGM_ERROR_ALLOC: return "allocation failed";
+    case CGM_ERROR_NOT_CONVERGED: return "not converged";
+    case CGM_ERROR_NONFINITE: return "non-finite value";
+    case CGM_ERROR_UNSUPPORTED: return "unsupported size";
+    case CGM_ERROR_NOT_SYMMETRIC: return "kernel array not even-symmetric";
+  }
+  return "unknown status";
+}
+
+extern "C" void cgm_version(int* major, int* minor) {
+  if (major) *major = CGM_VERSION_MAJOR;
+  if (minor) *minor = CGM_VERSION_MINOR;
+}
+
+extern "C" void cgm_bessel01(double x, double* j0, double* j1, double* y0, double* y1) {
+  double a, b, c, d;
+  bessel01(x, a, b, c, d);
+  if (j0) *j0 = a;
+  if (j1) *j1 = b;
+  if (y0) *y0 = c;
+  if (y1) *y1 = d;
+}
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+static int upload_twiddles(cgm_context* c) {
+  const int M = c->M;
+  std::vector<float2> t32(M);
+  std::vector<double2> t64(M);
+  for (int q = 0; q < M; ++q) {
+    const double a = -2.0 * CGM_PI * (double)q / (double)M;
+    t64[q] = make_double2(cos(a), sin(a));
+    t32[q] = make_float2((float)t64[q].x, (float)t64[q].y);
+  }
+  CGM_HIP(hipMalloc((void**)&c->tw32, sizeof(float2) * M));
+  CGM_HIP(hipMalloc((void**)&c->tw64, sizeof(double2) * M));
+  CGM_HIP(hipMemcpy(c->tw32, t32.data(), sizeof(float2) * M, hipMemcpyHostToDevice));
+  CGM_HIP(hipMemcpy(c->tw64, t64.data(), sizeof(double2) * M, hipMemcpyHostToDevice));
+  return CGM_SUCCESS;
+}
+
+extern "C" int cgm_destroy(cgm_ctx_t c) {
+  if (!c) return CGM_SUCCESS;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  krylov_free(c->ws);
+  void* ptrs[] = {c->tw32, c->tw64, c->T32, c->T64, c->st, c->chi_tmp, c->vec_tmp, c->v64a, c->chi32};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (c->hst) (void)hipHostFree(c->hst);
+  delete c;
+  return CGM_SUCCESS;
+}
+
+extern "C" int cgm_create(cgm_ctx_t* out, int N, int device) {
+  CGM_CHECK(out != nullptr, CGM_ERROR_INVALID_VALUE, "cgm_create: null output");
+  *out = nullptr;
+  CGM_CHECK(N >= 1 && N <= 2048, CGM_ERROR_UNSUPPORTED, "cgm_create: N=%d outside [1, 2048]", N);
+  int ndev = 0;
+  CGM_HIP(hipGetDeviceCount(&ndev));
+  CGM_CHECK(device >= 0 && device < ndev, CGM_ERROR_INVALID_VALUE, "cgm_create: device %d of %d",
+            device, ndev);
+  CGM_HIP(hipSetDevice(device));
+  cgm_context* c = new cgm_context();
+  c->N = N;
+  c->device = device;
+  int rc = plan_init(c->plan, N);
+  if (rc) { delete c; cgm_set_last_error("cgm_create: no embedding for N=%d", N); return rc; }
+  c->M = c->plan.M;
+  c->mask_chi = cgm_reference_chi(N);
+  if ((rc = upload_twiddles(c))) { cgm_destroy(c); return rc; }
+  const size_t nm = (size_t)N * c->M;
+  if (hipMalloc((void**)&c->T32, sizeof(float2) * nm) != hipSuccess ||
+      hipMalloc((void**)&c->st, sizeof(DevState)) != hipSuccess ||
+      hipHostMalloc((void**)&c->hst, sizeof(DevState), hipHostMallocDefault) != hipSuccess) {
+    cgm_destroy(c);
+    cgm_set_last_error("cgm_create: device allocation failed");
+    return CGM_ERROR_ALLOC;
+  }
+  CGM_HIP(hipMemset(c->st, 0, sizeof(DevState)));
+  *out = c;
+  return CGM_SUCCESS;
+}
+
+extern "C" int cgm_set_stream(cgm_ctx_t c, void* stream) {
+  CGM_CHECK(c, CGM_ERROR_INVALID_VALUE, "null context");
+  c->stream = (hipStream_t)stream;
+  return CGM_SUCCESS;
+}
+
+extern "C" int cgm_get_embedding(cgm_ctx_t c, int* M) {
+  CGM_CHECK(c && M, CGM_ERROR_INVALID_VALUE, "null argument");
+  *M = c->M;
+  return CGM_SUCCESS;
+}
+
+extern "C" int cgm_spectrum_size(cgm_ctx_t c, size_t* n) {
+  CGM_CHECK(c && n, CGM_ERROR_INVALID_VALUE, "null argument");
+  const size_t H = (size_t)c->M / 2 + 1;
+  *n = H * H;
+  return CGM_SUCCESS;
+}
+
+extern "C" int cgm_synchronize(cgm_ctx_t c) {
+  CGM_CHECK(c, CGM_ERROR_INVALID_VALUE, "null context");
+  CGM_HIP(hipStreamSynchronize(c->stream));
+  return CGM_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------
+// spectra
+// ---------------------------------------------------------------------------
+static int spectrum_common(cgm_context* c, const double2* blk, cgm_complex64* g32,
+                           cgm_complex128* g64) {
+  return spectrum_from_block(c->plan, c->stream, blk, c->tw64, (float2*)g32, (double2*)g64);
+}
+
+extern "C" int cgm_spectrum_from_block(cgm_ctx_t c, const cgm_complex128* blk, cgm_complex64* g32,
+                                       cgm_complex128* g64) {
+  CGM_CHECK(c && blk && (g32 || g64), CGM_ERROR_INVALID_VALUE, "null argument");
+  CGM_HIP(hipSetDevice(c->device));
+  int rc = spectrum_common(c, (const double2*)blk, g32, g64);
+  if (rc) return rc;
+  CGM_HIP(hipStreamSynchronize(c->stream));
+  return CGM_SUCCESS;
+}
+
+extern "C" int cgm_spectrum_from_gprep(cgm_ctx_t c, const cgm_complex64* gprep, cgm_complex64* g32) {
+  CGM_CHECK(c && gprep && g32, CGM_ERROR_INVALID_VALUE, "null argument");
+  CGM_HIP(hipSetDevice(c->device));
+  const int N = c->N, Mp = 2 * N - 1;
+  // symmetry check on the host copy (one-off setup)
+  std::vector<cgm_complex64> h((size_t)Mp * Mp);
+  CGM_HIP(hipMemcpy(h.data(), gprep, sizeof(cgm_complex64) * h.size(), hipMemcpyDeviceToHost));
+  for (int i = 0; i < N; ++i)
+    for (int j = 0; j < N; ++j) {
+      const cgm_complex64 a = h[(size_t)i * Mp + j];
+      const size_t mir[3] = {(size_t)((Mp - i) % Mp) * Mp + j, (size_t)i * Mp + (Mp - j) % Mp,
+                             (size_t)((Mp - i) % Mp) * Mp + (Mp - j) % Mp};
+      for (size_t q : mir) {
+        const cgm_complex64 b = h[q];
+        if (b.re != a.re || b.im != a.im) {
+          cgm_set_last_error("cgm_spectrum_from_gprep: G_prep not even at (%d,%d)", i, j);
+          return CGM_ERROR_NOT_SYMMETRIC;
+        }
+      }
+    }
+  double2* blk = nullptr;
+  CGM_HIP(hipMalloc((void**)&blk, sizeof(double2) * (size_t)N * N));
+  int rc = gprep_block(c->stream, (const float2*)gprep, blk, N);
+  if (!rc) rc = spectrum_common(c, blk, g32, nullptr);
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(blk);
+  return rc;
+}
+
+static int spectrum_generated(cgm_context* c, int kind, double k, double sr, double sc,
+                              cgm_complex64* g32, cgm_complex128* g64) {
+  CGM_CHECK(c && (g32 || g64), CGM_ERROR_INVALID_VALUE, "null argument");
+  CGM_CHECK(k > 0 && isfinite(k), CGM_ERROR_INVALID_VALUE, "wave number must be positive");
+  CGM_HIP(hipSetDevice(c->device));
+  const int N = c->N;
+  double2* blk = nullptr;
+  CGM_HIP(hipMalloc((void**)&blk, sizeof(double2) * (size_t)N * N));
+  int rc = green_block(c->stream, blk, N, kind, k, sr, sc);
+  if (!rc) rc = spectrum_common(c, blk, g32, g64);
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(blk);
+  return rc;
+}
+
+extern "C" int cgm_spectrum_reference(cgm_ctx_t c, double k, double sr, double sc,
+                                      cgm_complex64* g32, cgm_complex128* g64) {
+  return spectrum_generated(c, 1, k, sr, sc, g32, g64);
+}
+
+extern "C" int cgm_spectrum_helmholtz(cgm_ctx_t c, double kh, cgm_complex64* g32,
+                                      cgm_complex128* g64) {
+  return spectrum_generated(c, 0, kh, 0.0, 0.0, g32, g64);
+}
+
+// ---------------------------------------------------------------------------
+// operator
+// ---------------------------------------------------------------------------
+static int matvec32(cgm_context* c, const float2* gq, const float* chi, const float2* x,
+                    const float2* b, float2* y, int mode, const double* norm_ptr, float2* xw,
+                    const int* done) {
+  int rc = launch_pass1<float>(c->plan, c->stream, x, xw, chi, norm_ptr, mode, c->T32, c->tw32, done);
+  if (!rc) rc = launch_pass2<float, float2>(c->plan, c->stream, c->T32, gq, c->tw32, done);
+  // pass 3 reads the (possibly rescaled) input written back by pass 1
+  const float2* xin = xw ? xw : x;
+  if (!rc) rc = launch_pass3<float>(c->plan, c->stream, c->T32, xin, chi, b, y, mode, c->tw32, done);
+  return rc;
+}
+
+static int ensure_T64(cgm_context* c) {
+  if (!c->T64) CGM_HIP(hipMalloc((void**)&c->T64, sizeof(double2) * (size_t)c->N * c->M));
+  return CGM_SUCCESS;
+}
+
+static int matvec64(cgm_context* c, const double2* gq, const double* chi, const double2* x,
+                    const double2* b, double2* y, int mode) {
+  int rc = ensure_T64(c);
+  if (!rc) rc = launch_pass1<double>(c->plan, c->stream, x, nullptr, chi, nullptr, mode, c->T64, c->tw64, nullptr);
+  if (!rc) rc = launch_pass2<double, double2>(c->plan, c->stream, c->T64, gq, c->tw64, nullptr);
+  if (!rc) rc = launch_pass3<double>(c->plan, c->stream, c->T64, x, chi, b, y, mode, c->tw64, nullptr);
+  return rc;
+}
+
+extern "C" int cgm_matvec(cgm_ctx_t c, const cgm_complex64* gamma, const float* chi,
+                          const cgm_complex64* x, cgm_complex64* y, int mode) {
+  CGM_CHECK(c && gamma && x && y, CGM_ERROR_INVALID_VALUE, "null argument");
+  CGM_CHECK(mode >= 0 && mode <= 2, CGM_ERROR_INVALID_VALUE, "mode %d", mode);
+  CGM_CHECK(mode == CGM_OP_GREEN || chi, CGM_ERROR_INVALID_VALUE, "null contrast");
+  CGM_HIP(hipSetDevice(c->device));
+  return matvec32(c, (const float2*)gamma, chi, (const float2*)x, nullptr, (float2*)y, mode,
+                  nullptr, nullptr, nullptr);
+}
+
+extern "C" int cgm_matvec_c128(cgm_ctx_t c, const cgm_complex128* gamma, const double* chi,
+                               const cgm_complex128* x, cgm_complex128* y, int mode) {
+  CGM_CHECK(c && gamma && x && y, CGM_ERROR_INVALID_VALUE, "null argument");
+  CGM_CHECK(mode >= 0 && mode <= 2, CGM_ERROR_INVALID_VALUE, "mode %d", mode);
+  CGM_CHECK(mode == CGM_OP_GREEN || chi, CGM_ERROR_INVALID_VALUE, "null contrast");
+  CGM_HIP(hipSetDevice(c->device));
+  return matvec64(c, (const double2*)gamma, chi, (const double2*)x, nullptr, (double2*)y, mode);
+}
+
+// ---------------------------------------------------------------------------
+// GMRES
+// ---------------------------------------------------------------------------
+static int read_state(cgm_context* c) {
+  CGM_HIP(hipMemcpyAsync(c->hst, c->st, sizeof(DevState), hipMemcpyDeviceToHost, c->stream));
+  CGM_HIP(hipStreamSynchronize(c->stream));
+  return CGM_SUCCESS;
+}
+
+// Core solver on float2 device vectors.  hist: optional device array,
+// hist[0] = beta0, hist[t] = estimate after t steps.
+static int gmres_core(cgm_context* c, const float2* gq, const float* chi, const float2* b,
+                      float2* x, const cgm_gmres_params* p, cgm_gmres_result* res, double* hist) {
+  const long n = (long)c->N * c->N;
+  const int m = p->restart;
+  int rc = krylov_alloc(c->ws, n, m, c->stream);
+  if (rc) return rc;
+  KrylovWs& ws = c->ws;
+  const int fwd = p->adjoint ? kModeAdjoint : kModeForward;
+  const int resmode = p->adjoint ? kModeResidualAdj : kModeResidual;
+  // ||b|| and the absolute tolerance
+  CGM_HIP(hipMemsetAsync(&c->st->status, 0, sizeof(int), c->stream));
+  if ((rc = launch_norm_init(c->stream, ws, b, c->st, 1, p->tol))) return rc;
+  if (p->x0_zero) CGM_HIP(hipMemsetAsync(x, 0, sizeof(float2) * n, c->stream));
+  int total = 0, cycles = 0;
+  res->converged = 0;
+  for (;;) {
+    // true residual r = b - A x into V_0
+    if (p->x0_zero && cycles == 0) {
+      CGM_HIP(hipMemcpyAsync(ws.V, b, sizeof(float2) * n, hipMemcpyDeviceToDevice, c->stream));
+    } else {
+      rc = matvec32(c, gq, chi, x, b, ws.V, resmode, nullptr, nullptr, nullptr);
+      if (rc) return rc;
+    }
+    if ((rc = launch_norm_init(c->stream, ws, ws.V, c->st, 0, 0.0))) return rc;
+    if ((rc = read_state(c))) return rc;
+    const DevState& h = *c->hst;
+    if (cycles == 0) {
+      res->beta0 = h.beta;
+      res->bnorm = h.bnorm;
+      if (hist) CGM_HIP(hipMemcpyAsync(hist, &h.beta, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    }
+    res->rel_residual = h.bnorm > 0 ? h.beta / h.bnorm : 0.0;
+    if (h.status) {
+      cgm_set_last_error("cgm_gmres: non-finite residual");
+      res->iterations = total;
+      res->cycles = cycles;
+      return CGM_ERROR_NONFINITE;
+    }
+    if (h.bnorm == 0.0 || h.beta <= p->tol * h.bnorm) { res->converged = 1; break; }
+    if (cycles >= p->max_cycles) break;
+    // one restart cycle, enqueued without host synchronisation
+    for (int i = 0; i < m; ++i) {
+      float2* vi = ws.V + (long)i * n;
+      const double* nrm = (i == 0) ? &c->st->beta : &c->st->hn;
+      rc = matvec32(c, gq, chi, vi, nullptr, vi + n, fwd, nrm, vi, &c->st->done);
+      if (rc) return rc;
+      rc = launch_cgs2_step(c->stream, ws, i, c->st, hist ? hist + 1 + total : nullptr);
+      if (rc) return rc;
+    }
+    if ((rc = launch_backsub_update(c->stream, ws, c->st, x))) return rc;
+    if ((rc = read_state(c))) return rc;
+    total += c->hst->k;
+    ++cycles;
+    if (c->hst->status) {
+      cgm_set_last_error("cgm_gmres: non-finite value in the Arnoldi process");
+      res->iterations = total;
+      res->cycles = cycles;
+      return CGM_ERROR_NONFINITE;
+    }
+  }
+  res->iterations = total;
+  res->cycles = cycles;
+  return res->converged ? CGM_SUCCESS : CGM_ERROR_NOT_CONVERGED;
+}
+
+static int check_params(const cgm_gmres_params* p) {
+  CGM_CHECK(p, CGM_ERROR_INVALID_VALUE, "null params");
+  CGM_CHECK(p->restart >= 1 && p->restart <= kMaxRestart, CGM_ERROR_INVALID_VALUE,
+            "restart %d outside [1, %d]", p->restart, kMaxRestart);
+  CGM_CHECK(p->max_cycles >= 0, CGM_ERROR_INVALID_VALUE, "max_cycles < 0");
+  CGM_CHECK(p->tol >= 0 && isfinite(p->tol), CGM_ERROR_INVALID_VALUE, "bad tolerance");
+  return CGM_SUCCESS;
+}
+
+extern "C" int cgm_gmres(cgm_ctx_t c, const cgm_complex64* gamma, const float* chi,
+                         const cgm_complex64* b, cgm_complex64* x, const cgm_gmres_params* p,
+                         cgm_gmres_result* res, double* hist) {
+  CGM_CHECK(c && gamma && chi && b && x && res, CGM_ERROR_INVALID_VALUE, "null argument");
+  int rc = check_params(p);
+  if (rc) return rc;
+  CGM_HIP(hipSetDevice(c->device));
+  memset(res, 0, sizeof(*res));
+  return gmres_core(c, (const float2*)gamma, chi, (const float2*)b, (float2*)x, p, res, hist);
+}
+
+// ---------------------------------------------------------------------------
+// FP64 refinement
+// ---------------------------------------------------------------------------
+__global__ void k_to_float_scaled(const double2* __restrict__ r, float2* __restrict__ o, long n,
+                                  const DevState* st) {
+  const double s = st->norm_out > 0 ? 1.0 / st->norm_out : 0.0;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x)
+    o[e] = make_float2((float)(r[e].x * s), (float)(r[e].y * s));
+}
+
+__global__ void k_axpy_mixed(double2* __restrict__ x, const float2* __restrict__ d, long n,
+                             const DevState* st) {
+  const double s = st->norm_out;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    x[e].x += s * (double)d[e].x;
+    x[e].y += s * (double)d[e].y;
+  }
+}
+
+__global__ void k_d2f(const double* __restrict__ a, float* __restrict__ o, long n) {
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x)
+    o[e] = (float)a[e];
+}
+
+extern "C" int cgm_gmres_refine(cgm_ctx_t c, const cgm_complex64* g32, const cgm_complex128* g64,
+                                const double* chi, const cgm_complex128* b, cgm_complex128* x,
+                                const cgm_refine_params* p, cgm_refine_result* res) {
+  CGM_CHECK(c && g32 && g64 && chi && b && x && p && res, CGM_ERROR_INVALID_VALUE, "null argument");
+  CGM_CHECK(p->inner_restart >= 1 && p->inner_restart <= kMaxRestart && p->max_outer >= 0,
+            CGM_ERROR_INVALID_VALUE, "bad refinement parameters");
+  CGM_HIP(hipSetDevice(c->device));
+  memset(res, 0, sizeof(*res));
+  const long n = (long)c->N * c->N;
+  if (!c->v64a) CGM_HIP(hipMalloc((void**)&c->v64a, sizeof(double2) * n));
+  if (!c->chi32) CGM_HIP(hipMalloc((void**)&c->chi32, sizeof(float) * n));
+  if (!c->vec_tmp) CGM_HIP(hipMalloc((void**)&c->vec_tmp, sizeof(float2) * n));
+  float2* d32 = nullptr;
+  CGM_HIP(hipMalloc((void**)&d32, sizeof(float2) * n));
+  int rc = krylov_alloc(c->ws, n, p->inner_restart, c->stream);
+  if (rc) { (void)hipFree(d32); return rc; }
+  const int blocks = 1024;
+  hipLaunchKernelGGL(k_d2f, dim3(blocks), dim3(256), 0, c->stream, chi, c->chi32, n);
+  // ||b|| in FP64
+  rc = launch_norm64(c->stream, (const double2*)b, n, c->ws.partn, c->ws.G, c->st);
+  if (!rc) rc = read_state(c);
+  const double bn = c->hst->norm_out;
+  cgm_gmres_params ip{p->inner_restart, p->inner_max_cycles, p->inner_tol, 0, 1};
+  for (int it = 0; !rc && it <= p->max_outer; ++it) {
+    rc = matvec64(c, (const double2*)g64, chi, (const double2*)x, (const double2*)b, c->v64a,
+                  kModeResidual);
+    if (rc) break;
+    if ((rc = launch_norm64(c->stream, c->v64a, n, c->ws.partn, c->ws.G, c->st))) break;
+    if ((rc = read_state(c))) break;
+    const double rn = c->hst->norm_out;
+    res->rel_residual = bn > 0 ? rn / bn : 0.0;
+    if (!isfinite(rn)) { rc = CGM_ERROR_NONFINITE; cgm_set_last_error("refine: non-finite residual"); break; }
+    if (rn <= p->outer_tol * bn) { res->converged = 1; break; }
+    if (it == p->max_outer) break;
+    hipLaunchKernelGGL(k_to_float_scaled, dim3(blocks), dim3(256), 0, c->stream, c->v64a,
+                       c->vec_tmp, n, c->st);
+    cgm_gmres_result ir;
+    memset(&ir, 0, sizeof(ir));
+    int irc = gmres_core(c, (const float2*)g32, c->chi32, c->vec_tmp, d32, &ip, &ir, nullptr);
+    if (irc && irc != CGM_ERROR_NOT_CONVERGED) { rc = irc; break; }
+    res->inner_iterations += ir.iterations;
+    res->outer += 1;
+    // the inner solve overwrote norm_out; recompute ||r|| for the scale
+    if ((rc = launch_norm64(c->stream, c->v64a, n, c->ws.partn, c->ws.G, c->st))) break;
+    hipLaunchKernelGGL(k_axpy_mixed, dim3(blocks), dim3(256), 0, c->stream, (double2*)x, d32, n,
+                       c->st);
+  }
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(d32);
+  if (rc) return rc;
+  return res->converged ? CGM_SUCCESS : CGM_ERROR_NOT_CONVERGED;
+}
+
+// ---------------------------------------------------------------------------
+// memory helpers
+// ---------------------------------------------------------------------------
+extern "C" int cgm_malloc(void** p, size_t bytes) {
+  CGM_CHECK(p, CGM_ERROR_INVALID_VALUE, "null pointer");
+  if (hipMalloc(p, bytes) != hipSuccess) {
+    cgm_set_last_error("hipMalloc of %zu bytes failed", bytes);
+    return CGM_ERROR_ALLOC;
+  }
+  return CGM_SUCCESS;
+}
+extern "C" int cgm_free(void* p) {
+  CGM_HIP(hipFree(p));
+  return CGM_SUCCESS;
+}
+extern "C" int cgm_memcpy_h2d(void* d, const void* s, size_t bytes) {
+  CGM_HIP(hipMemcpy(d, s, bytes, hipMemcpyHostToDevice));
+  return CGM_SUCCESS;
+}
+extern "C" int cgm_memcpy_d2h(void* d, const void* s, size_t bytes) {
+  CGM_HIP(hipMemcpy(d, s, bytes, hipMemcpyDeviceToHost));
+  return CGM_SUCCESS;
+}

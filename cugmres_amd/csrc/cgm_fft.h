@@ -1,0 +1,172 @@
+// LDS-resident self-sorting Stockham FFT for one transform of compile-time
+// length M, executed by NT = M / EPT threads that each hold EPT elements.
+//
+// Stage s with radix R over span Ns (product of earlier radices): butterfly
+// j in [0, M/R) reads x[j + r*M/R], multiplies by exp(sign*2*pi*i*r*(j%Ns)/(Ns*R)),
+// applies the R-point DFT and writes x[(j/Ns)*Ns*R + j%Ns + r*Ns].  Twiddles
+// come from a table tw[q] = exp(-2*pi*i*q/M) built on the host in FP64.
+// The unnormalised transform is the one cuFFT computes for the reference
+// (B/CUDA_GMRES/Fast_matvec.cuh:13-26).
+#pragma once
+#include "cgm_common.h"
+
+namespace cgm {
+
+// ---- radix plans ------------------------------------------------------------
+// Elements per thread: 8 for powers of two (radices 8/4/2), 12 for 3*2^k
+// (radices 3/4/2).  Radices must divide EPT so that every thread owns exactly
+// EPT/R butterflies per stage.
+template <int M> struct FftShape {
+  static constexpr bool pow2 = (M & (M - 1)) == 0;
+  static constexpr int EPT = pow2 ? 8 : 12;
+  static constexpr int NT = M / EPT;
+  // work-group size bound used for launch_bounds: groups of transforms fill 256 threads
+  static constexpr int WG = NT > 256 ? NT : 256;
+};
+
+template <int M> __host__ __device__ constexpr int fft_num_stages() {
+  int n = M, k = 0;
+  if ((M & (M - 1)) == 0) {
+    while (n % 8 == 0 && n > 1) { n /= 8; ++k; }
+    if (n > 1) ++k;
+  } else {
+    while (n % 3 == 0) { n /= 3; ++k; }
+    while (n % 4 == 0) { n /= 4; ++k; }
+    if (n > 1) ++k;
+  }
+  return k;
+}
+
+template <int M> __host__ __device__ constexpr int fft_radix(int s) {
+  int n = M, k = 0;
+  if ((M & (M - 1)) == 0) {
+    while (n % 8 == 0 && n > 1) { if (k == s) return 8; n /= 8; ++k; }
+    return n;  // remainder 2 or 4
+  } else {
+    while (n % 3 == 0) { if (k == s) return 3; n /= 3; ++k; }
+    while (n % 4 == 0) { if (k == s) return 4; n /= 4; ++k; }
+    return n;  // remainder 2
+  }
+}
+
+// ---- small DFTs in registers -------------------------------------------------
+template <typename T, int SIGN>
+__device__ inline void dft2(cplx<T>& a, cplx<T>& b) {
+  cplx<T> t = a;
+  a = cadd(t, b);
+  b = csub(t, b);
+}
+
+// multiply by SIGN*i
+template <typename T, int SIGN>
+__device__ inline cplx<T> mul_si(cplx<T> a) {
+  return SIGN < 0 ? mk(a.y, -a.x) : mk(-a.y, a.x);
+}
+
+template <typename T, int SIGN>
+__device__ inline void dft4(cplx<T>* v) {
+  cplx<T> a = cadd(v[0], v[2]), b = csub(v[0], v[2]);
+  cplx<T> c = cadd(v[1], v[3]), d = mul_si<T, SIGN>(csub(v[1], v[3]));
+  v[0] = cadd(a, c);
+  v[2] = csub(a, c);
+  v[1] = cadd(b, d);
+  v[3] = csub(b, d);
+}
+
+template <typename T, int SIGN>
+__device__ inline void dft3(cplx<T>* v) {
+  const T c1 = T(-0.5);
+  const T s1 = T(SIGN) * T(0.86602540378443864676);  // sin(2pi/3) * sign
+  cplx<T> a = v[0], b = v[1], c = v[2];
+  cplx<T> s = cadd(b, c), d = csub(b, c);
+  v[0] = cadd(a, s);
+  cplx<T> m = mk(a.x + c1 * s.x, a.y + c1 * s.y);
+  // i*s1*d
+  cplx<T> r = mk(-s1 * d.y, s1 * d.x);
+  v[1] = cadd(m, r);
+  v[2] = csub(m, r);
+}
+
+template <typename T, int SIGN>
+__device__ inline void dft8(cplx<T>* v) {
+  // radix-2 split: even/odd 4-point DFTs, then combine with w8^k
+  cplx<T> e[4] = {v[0], v[2], v[4], v[6]};
+  cplx<T> o[4] = {v[1], v[3], v[5], v[7]};
+  dft4<T, SIGN>(e);
+  dft4<T, SIGN>(o);
+  const T h = T(0.70710678118654752440);
+  // w8^1 = (1 + SIGN i)/sqrt2, w8^2 = SIGN i, w8^3 = (-1 + SIGN i)/sqrt2
+  cplx<T> o1 = cmul(o[1], mk(h, T(SIGN) * h));
+  cplx<T> o2 = mul_si<T, SIGN>(o[2]);
+  cplx<T> o3 = cmul(o[3], mk(-h, T(SIGN) * h));
+  v[0] = cadd(e[0], o[0]);
+  v[4] = csub(e[0], o[0]);
+  v[1] = cadd(e[1], o1);
+  v[5] = csub(e[1], o1);
+  v[2] = cadd(e[2], o2);
+  v[6] = csub(e[2], o2);
+  v[3] = cadd(e[3], o3);
+  v[7] = csub(e[3], o3);
+}
+
+template <typename T, int SIGN, int R>
+__device__ inline void dft(cplx<T>* v) {
+  if constexpr (R == 2) dft2<T, SIGN>(v[0], v[1]);
+  else if constexpr (R == 3) dft3<T, SIGN>(v);
+  else if constexpr (R == 4) dft4<T, SIGN>(v);
+  else if constexpr (R == 8) dft8<T, SIGN>(v);
+}
+
+// One Stockham stage for the butterflies owned by thread t.
+template <typename T, int M, int SIGN, int R, int NS>
+__device__ inline void fft_stage(cplx<T>* buf, const cplx<T>* __restrict__ tw, int t) {
+  constexpr int EPT = FftShape<M>::EPT;
+  constexpr int NT = FftShape<M>::NT;
+  constexpr int NB = EPT / R;        // butterflies per thread
+  constexpr int MR = M / R;
+  cplx<T> v[NB][R];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int j = t + b * NT;
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[b][r] = buf[j + r * MR];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int j = t + b * NT;
+    const int k = j % NS;
+    if constexpr (NS > 1) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        cplx<T> w = tw[(r * k) * (M / (NS * R))];
+        if (SIGN > 0) w.y = -w.y;
+        v[b][r] = cmul(v[b][r], w);
+      }
+    }
+    dft<T, SIGN, R>(v[b]);
+    const int base = (j / NS) * NS * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) buf[base + r * NS] = v[b][r];
+  }
+  __syncthreads();
+}
+
+template <typename T, int M, int SIGN, int S, int NS>
+__device__ inline void fft_stages(cplx<T>* buf, const cplx<T>* __restrict__ tw, int t) {
+  if constexpr (S < fft_num_stages<M>()) {
+    constexpr int R = fft_radix<M>(S);
+    fft_stage<T, M, SIGN, R, NS>(buf, tw, t);
+    fft_stages<T, M, SIGN, S + 1, NS * R>(buf, tw, t);
+  }
+}
+
+// Full transform of buf[0..M) (LDS) by threads t in [0, NT).  The caller
+// must have synchronised the work-group after filling buf; on return buf
+// holds the result and the group is synchronised.
+template <typename T, int M, int SIGN>
+__device__ inline void lds_fft(cplx<T>* buf, const cplx<T>* __restrict__ tw, int t) {
+  fft_stages<T, M, SIGN, 0, 1>(buf, tw, t);
+}
+
+}  // namespace cgm

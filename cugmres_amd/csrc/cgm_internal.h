@@ -1,0 +1,103 @@
+// Internal declarations shared by the operator, Krylov and API translation units.
+#pragma once
+#include "cgm_common.h"
+
+namespace cgm {
+
+enum {
+  kModeForward = 0,      // y = x - G(chi x)
+  kModeAdjoint = 1,      // y = x - chi G(x)
+  kModeGreen = 2,        // y = G(x)
+  kModeResidual = 3,     // y = b - (x - G(chi x))
+  kModeResidualAdj = 4   // y = b - (x - chi G(x))
+};
+
+struct Plan {
+  int N, M;
+  int rows1, rows3;  // rows per work-group in passes 1 and 3
+  int cols2;         // columns per work-group in pass 2
+};
+
+int plan_init(Plan& P, int N);
+int fft_threads(int M);
+int choose_embedding(int N);
+
+template <typename T>
+int launch_pass1(const Plan& P, hipStream_t st, const cplx<T>* x, cplx<T>* xw, const T* chi,
+                 const double* norm_ptr, int mode, cplx<T>* Tm, const cplx<T>* tw, const int* done);
+template <typename T, typename G>
+int launch_pass2(const Plan& P, hipStream_t st, cplx<T>* Tm, const G* gq, const cplx<T>* tw,
+                 const int* done);
+template <typename T>
+int launch_pass3(const Plan& P, hipStream_t st, const cplx<T>* Tm, const cplx<T>* x, const T* chi,
+                 const cplx<T>* b, cplx<T>* y, int mode, const cplx<T>* tw, const int* done);
+
+int spectrum_from_block(const Plan& P, hipStream_t st, const double2* blk, const double2* tw64,
+                        float2* gq32, double2* gq64);
+int green_block(hipStream_t st, double2* blk, int N, int kind, double k, double sr, double sc);
+int gprep_block(hipStream_t st, const float2* gprep, double2* blk, int N);
+
+// Solver state in device memory (one per context).
+struct DevState {
+  double beta;     // ||r|| at the start of the current cycle
+  double bnorm;    // ||b||
+  double hn;       // last ||w|| (H[i+1, i])
+  double tol_abs;  // tol * ||b||
+  double res_est;  // latest |g_{i+1}|
+  double norm_out; // result slot of the stand-alone norm kernel
+  int done;        // 1: remaining launches of this cycle are no-ops
+  int k;           // valid Arnoldi columns of this cycle
+  int status;      // 0 or CGM_ERROR_NONFINITE
+  int pad;
+  unsigned tickets[8];
+};
+
+enum { kTicketDots1 = 0, kTicketDots2 = 1, kTicketNorm = 2, kTicketInit = 3, kTicketMisc = 4 };
+
+constexpr int kMaxRestart = 512;
+constexpr int kKB = 256;  // threads per block of the Krylov kernels
+
+struct KrylovWs {
+  int m = 0;              // restart this workspace was sized for
+  int G = 0;              // blocks of the reduction kernels
+  long n = 0;
+  float2* V = nullptr;    // (m+1) * n basis
+  double2* h1 = nullptr;  // first CGS pass coefficients (m+1)
+  double2* h2 = nullptr;  // second CGS pass coefficients (m+1)
+  double* cs = nullptr;   // Givens cosines (m)
+  double2* sn = nullptr;  // Givens sines (m)
+  double2* g = nullptr;   // rotated right-hand side (m+1)
+  double2* R = nullptr;   // upper triangular factor, row-major m x m
+  double2* y = nullptr;   // least-squares solution (m)
+  double2* part = nullptr;  // reduction partials (m+1) * G * 2
+  double* partn = nullptr;  // norm partials G
+};
+
+// Krylov launches (cgm_krylov.hip)
+int krylov_alloc(KrylovWs& ws, long n, int m, hipStream_t st);
+void krylov_free(KrylovWs& ws);
+int launch_norm_init(hipStream_t st, const KrylovWs& ws, const float2* r, DevState* s, int what,
+                     double tol);
+int launch_cgs2_step(hipStream_t st, const KrylovWs& ws, int i, DevState* s, double* hist);
+int launch_backsub_update(hipStream_t st, const KrylovWs& ws, DevState* s, float2* x);
+int launch_norm64(hipStream_t st, const double2* v, long n, double* partn, int G, DevState* s);
+
+}  // namespace cgm
+
+struct cgm_context {
+  int N = 0, M = 0, device = 0;
+  hipStream_t stream = nullptr;
+  cgm::Plan plan{};
+  float2* tw32 = nullptr;
+  double2* tw64 = nullptr;
+  float2* T32 = nullptr;   // N x M scratch of one complex64 matvec
+  double2* T64 = nullptr;  // N x M scratch of one complex128 matvec (lazy)
+  float mask_chi = 0.f;
+  cgm::KrylovWs ws;
+  cgm::DevState* st = nullptr;   // device
+  cgm::DevState* hst = nullptr;  // pinned host mirror
+  float* chi_tmp = nullptr;      // mask -> contrast conversions (N*N)
+  float2* vec_tmp = nullptr;     // right-hand sides built on the device (N*N)
+  double2* v64a = nullptr;       // refinement residual (N*N, lazy)
+  float* chi32 = nullptr;        // refinement contrast in float (N*N, lazy)
+};

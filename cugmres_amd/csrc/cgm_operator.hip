@@ -1,0 +1,430 @@
+// Circulant-embedded BTTB operator on gfx950: kernel-spectrum precompute
+// (FP64) and the three fused FFT passes of the matvec.
+//
+// Reference counterparts: Ghat = FFT(G_prep) at B/CUDA_GMRES/launch_GMRES.cuh:138-142;
+// G_x_fft_matvec at B/CUDA_GMRES/Fast_matvec.cuh:1-56, i.e. the chain
+// extend_by_zeros_kernel -> cufftExecC2C -> MatMul_ElemWise_kernel ->
+// cufftExecC2C -> _2D_to_1D_kernel (B/CUDA_GMRES/GMRES_kernels.cuh:98-167,286-326).
+//
+// Here the embedding is M x M with M >= 2N-1 chosen FFT-friendly (2^a or
+// 3*2^a), and the five reference launches collapse into three passes:
+//   pass 1  row i < N:  u = chi .* (s x)[i, :] zero-padded to M, forward FFT
+//                       along the row -> T[i, :]      (optionally writes s x back)
+//   pass 2  column c:   forward FFT of T[:, c] (rows >= N are zero), multiply
+//                       by Ghat[:, c], inverse FFT, keep rows < N -> T[:, c]
+//   pass 3  row i < N:  inverse FFT of T[i, :], crop to N, scale 1/M^2 and
+//                       combine with x (and b for a residual) -> y[i, :]
+// T is N x M row-major (complex), so only N of the M rows are ever stored.
+#include <math.h>
+#include <string.h>
+
+#include "cgm_fft.h"
+#include "cgm_internal.h"
+#include "cgm_special.h"
+
+namespace cgm {
+
+// ---------------------------------------------------------------------------
+// pass kernels
+// ---------------------------------------------------------------------------
+template <typename T, int M>
+__global__ void __launch_bounds__(FftShape<M>::WG)
+k_pass1(const cplx<T>* __restrict__ x, cplx<T>* xw, const T* __restrict__ chi,
+        const double* __restrict__ norm_ptr, int mode, cplx<T>* __restrict__ Tm, int N,
+        int rows_per_wg, const cplx<T>* __restrict__ tw, const int* __restrict__ done) {
+  if (done && *done) return;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<T>* smem = reinterpret_cast<cplx<T>*>(smem_raw);
+  constexpr int NT = FftShape<M>::NT;
+  const int t = threadIdx.x % NT;
+  const int rl = threadIdx.x / NT;
+  const int row = blockIdx.x * rows_per_wg + rl;
+  cplx<T>* buf = smem + rl * M;
+  T s = T(1);
+  if (norm_ptr) {
+    const double nv = *norm_ptr;
+    s = nv > 0.0 ? T(1.0 / nv) : T(0);
+  }
+  const bool live = row < N;
+  const long rb = (long)row * N;
+  for (int e = t; e < M; e += NT) {
+    cplx<T> u = mk(T(0), T(0));
+    if (live && e < N) {
+      cplx<T> v = x[rb + e];
+      if (norm_ptr) {
+        v = cscale(v, s);
+        if (xw) xw[rb + e] = v;
+      }
+      u = (mode == kModeForward || mode == kModeResidual) ? cscale(v, chi[rb + e]) : v;
+    }
+    buf[e] = u;
+  }
+  __syncthreads();
+  lds_fft<T, M, -1>(buf, tw, t);
+  if (live) {
+    cplx<T>* out = Tm + (long)row * M;
+    for (int e = t; e < M; e += NT) out[e] = buf[e];
+  }
+}
+
+template <typename T, typename G, int M>
+__global__ void __launch_bounds__(FftShape<M>::WG)
+k_pass2(cplx<T>* __restrict__ Tm, const G* __restrict__ gq, int N, int cols_per_wg,
+        const cplx<T>* __restrict__ tw, const int* __restrict__ done) {
+  if (done && *done) return;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<T>* smem = reinterpret_cast<cplx<T>*>(smem_raw);
+  constexpr int NT = FftShape<M>::NT;
+  constexpr int H = M / 2 + 1;
+  const int grp = xcd_tile(blockIdx.x, gridDim.x);
+  const int c0 = grp * cols_per_wg;
+  const int nthr = blockDim.x;
+  // gather the column tile: consecutive threads read consecutive columns
+  for (int idx = threadIdx.x; idx < N * cols_per_wg; idx += nthr) {
+    const int r = idx / cols_per_wg, c = idx - r * cols_per_wg;
+    smem[c * M + r] = Tm[(long)r * M + c0 + c];
+  }
+  for (int idx = threadIdx.x; idx < (M - N) * cols_per_wg; idx += nthr) {
+    const int r = N + idx / cols_per_wg, c = idx % cols_per_wg;
+    smem[c * M + r] = mk(T(0), T(0));
+  }
+  __syncthreads();
+  const int t = threadIdx.x % NT;
+  const int cl = threadIdx.x / NT;
+  cplx<T>* buf = smem + cl * M;
+  lds_fft<T, M, -1>(buf, tw, t);
+  const int col = c0 + cl;
+  const int cq = col <= M - col ? col : M - col;
+  const G* grow = gq + (long)cq * H;
+  for (int e = t; e < M; e += NT) {
+    const int fr = e <= M - e ? e : M - e;
+    const G g = grow[fr];
+    buf[e] = cmul(buf[e], mk(T(g.x), T(g.y)));
+  }
+  __syncthreads();
+  lds_fft<T, M, +1>(buf, tw, t);
+  for (int idx = threadIdx.x; idx < N * cols_per_wg; idx += nthr) {
+    const int r = idx / cols_per_wg, c = idx - r * cols_per_wg;
+    Tm[(long)r * M + c0 + c] = smem[c * M + r];
+  }
+}
+
+template <typename T, int M>
+__global__ void __launch_bounds__(FftShape<M>::WG)
+k_pass3(const cplx<T>* __restrict__ Tm, const cplx<T>* __restrict__ x, const T* __restrict__ chi,
+        const cplx<T>* __restrict__ b, cplx<T>* __restrict__ y, int mode, int N, int rows_per_wg,
+        T inv, const cplx<T>* __restrict__ tw, const int* __restrict__ done) {
+  if (done && *done) return;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<T>* smem = reinterpret_cast<cplx<T>*>(smem_raw);
+  constexpr int NT = FftShape<M>::NT;
+  const int t = threadIdx.x % NT;
+  const int rl = threadIdx.x / NT;
+  const int row = blockIdx.x * rows_per_wg + rl;
+  cplx<T>* buf = smem + rl * M;
+  const bool live = row < N;
+  const cplx<T>* in = Tm + (long)row * M;
+  for (int e = t; e < M; e += NT) buf[e] = live ? in[e] : mk(T(0), T(0));
+  __syncthreads();
+  lds_fft<T, M, +1>(buf, tw, t);
+  if (!live) return;
+  const long rb = (long)row * N;
+  for (int e = t; e < N; e += NT) {
+    const cplx<T> g = cscale(buf[e], inv);
+    cplx<T> o;
+    switch (mode) {
+      case kModeForward: o = csub(x[rb + e], g); break;
+      case kModeAdjoint: o = csub(x[rb + e], cscale(g, chi[rb + e])); break;
+      case kModeGreen: o = g; break;
+      case kModeResidual: o = csub(b[rb + e], csub(x[rb + e], g)); break;
+      default: o = csub(b[rb + e], csub(x[rb + e], cscale(g, chi[rb + e]))); break;
+    }
+    y[rb + e] = o;
+  }
+}
+
+// Full-size 2-D FFT helpers for the one-off spectrum precompute (FP64).
+template <typename T, int M, int SIGN>
+__global__ void __launch_bounds__(FftShape<M>::WG)
+k_fft_rows(cplx<T>* __restrict__ data, int rows, int rows_per_wg, const cplx<T>* __restrict__ tw) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<T>* smem = reinterpret_cast<cplx<T>*>(smem_raw);
+  constexpr int NT = FftShape<M>::NT;
+  const int t = threadIdx.x % NT, rl = threadIdx.x / NT;
+  const int row = blockIdx.x * rows_per_wg + rl;
+  cplx<T>* buf = smem + rl * M;
+  const bool live = row < rows;
+  for (int e = t; e < M; e += NT) buf[e] = live ? data[(long)row * M + e] : mk(T(0), T(0));
+  __syncthreads();
+  lds_fft<T, M, SIGN>(buf, tw, t);
+  if (live)
+    for (int e = t; e < M; e += NT) data[(long)row * M + e] = buf[e];
+}
+
+template <typename T, int M, int SIGN>
+__global__ void __launch_bounds__(FftShape<M>::WG)
+k_fft_cols(cplx<T>* __restrict__ data, int cols_per_wg, const cplx<T>* __restrict__ tw) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<T>* smem = reinterpret_cast<cplx<T>*>(smem_raw);
+  constexpr int NT = FftShape<M>::NT;
+  const int c0 = blockIdx.x * cols_per_wg;
+  for (int idx = threadIdx.x; idx < M * cols_per_wg; idx += blockDim.x) {
+    const int r = idx / cols_per_wg, c = idx % cols_per_wg;
+    smem[c * M + r] = data[(long)r * M + c0 + c];
+  }
+  __syncthreads();
+  const int t = threadIdx.x % NT, cl = threadIdx.x / NT;
+  lds_fft<T, M, SIGN>(smem + cl * M, tw, t);
+  for (int idx = threadIdx.x; idx < M * cols_per_wg; idx += blockDim.x) {
+    const int r = idx / cols_per_wg, c = idx % cols_per_wg;
+    data[(long)r * M + c0 + c] = smem[c * M + r];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Green's-function generators (FP64)
+// ---------------------------------------------------------------------------
+// Block g[i, j] (i, j < N) of the BTTB generator.
+//  kind 0 (north-star operator, k0^2 G): off-diagonal kh^2 (i/4) H0(kh rho),
+//         self term (i pi ka/2) H1(ka) - 1, ka = kh / sqrt(pi).
+//  kind 1 (reference operator, B/MKL_GMRES_test_09122019/source/loop_functions.h:139-142):
+//         (i/4) H0(k sqrt((i - sr)^2 + (j - sc)^2)) for every (i, j), the
+//         half-cell shift sr = 0.5 (C++ orientation) or sc = 0.5 (Python
+//         orientation, B/PyGMRES/GMRES.py:23-27).
+__global__ void k_green_block(double2* __restrict__ blk, int N, int kind, double k, double sr,
+                              double sc) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= (long)N * N) return;
+  const int i = (int)(p / N), j = (int)(p % N);
+  double re, im, J0, J1, Y0, Y1;
+  if (kind == 0) {
+    if (i == 0 && j == 0) {
+      const double ka = k / sqrt(CGM_PI);
+      bessel01(ka, J0, J1, Y0, Y1);
+      re = -0.5 * CGM_PI * ka * Y1 - 1.0;
+      im = 0.5 * CGM_PI * ka * J1;
+    } else {
+      const double rho = sqrt((double)i * i + (double)j * j);
+      bessel01(k * rho, J0, J1, Y0, Y1);
+      re = -0.25 * k * k * Y0;
+      im = 0.25 * k * k * J0;
+    }
+  } else {
+    const double di = i - sr, dj = j - sc;
+    bessel01(k * sqrt(di * di + dj * dj), J0, J1, Y0, Y1);
+    re = -0.25 * Y0;
+    im = 0.25 * J0;
+  }
+  blk[p] = make_double2(re, im);
+}
+
+// Reads the generator out of a (2N-1)^2 embedded array laid out as the
+// reference's G_prep (B/MKL_GMRES_test_09122019/source/loop_functions.h:150-163).
+__global__ void k_gprep_block(const float2* __restrict__ gprep, double2* __restrict__ blk, int N) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= (long)N * N) return;
+  const int i = (int)(p / N), j = (int)(p % N);
+  const float2 v = gprep[(long)i * (2 * N - 1) + j];
+  blk[p] = make_double2(v.x, v.y);
+}
+
+// E[(di mod M), (dj mod M)] = blk[|di|, |dj|] for |di|, |dj| < N, else 0.
+__global__ void k_embed(const double2* __restrict__ blk, double2* __restrict__ E, int N, int M) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= (long)M * M) return;
+  const int r = (int)(p / M), c = (int)(p % M);
+  const int ai = r < N ? r : (r > M - N ? M - r : -1);
+  const int aj = c < N ? c : (c > M - N ? M - c : -1);
+  E[p] = (ai >= 0 && aj >= 0) ? blk[(long)ai * N + aj] : make_double2(0.0, 0.0);
+}
+
+// Quarter of the even-symmetric spectrum, stored column-frequency major:
+// gq[c' * (M/2+1) + r'] = Ghat[r', c'].
+template <typename G>
+__global__ void k_quarter(const double2* __restrict__ E, G* __restrict__ gq, int M) {
+  const int H = M / 2 + 1;
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= (long)H * H) return;
+  const int cq = (int)(p / H), rq = (int)(p % H);
+  const double2 v = E[(long)rq * M + cq];
+  G o;
+  o.x = v.x;
+  o.y = v.y;
+  gq[p] = o;
+}
+
+// ---------------------------------------------------------------------------
+// host-side dispatch over the supported embedding sizes
+// ---------------------------------------------------------------------------
+#define CGM_FOR_EACH_M(X) \
+  X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) \
+  X(1024) X(1536) X(2048) X(3072) X(4096)
+
+int fft_threads(int M) {
+  switch (M) {
+#define X(m) case m: return FftShape<m>::NT;
+    CGM_FOR_EACH_M(X)
+#undef X
+  }
+  return 0;
+}
+
+int choose_embedding(int N) {
+  static const int sizes[] = {16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512, 768,
+                              1024, 1536, 2048, 3072, 4096};
+  for (int s : sizes)
+    if (s >= 2 * N - 1) return s;
+  return 0;
+}
+
+static int groups_for(int M, int target_threads) {
+  const int nt = fft_threads(M);
+  int g = target_threads / nt;
+  return g < 1 ? 1 : g;
+}
+
+template <typename T>
+int launch_pass1(const Plan& P, hipStream_t st, const cplx<T>* x, cplx<T>* xw, const T* chi,
+                 const double* norm_ptr, int mode, cplx<T>* Tm, const cplx<T>* tw, const int* done) {
+  const int M = P.M, N = P.N;
+  const int rows = P.rows1;
+  const int grid = (N + rows - 1) / rows;
+  const int threads = rows * fft_threads(M);
+  const size_t lds = (size_t)rows * M * sizeof(cplx<T>);
+  switch (M) {
+#define X(m)                                                                                 \
+  case m:                                                                                    \
+    hipLaunchKernelGGL((k_pass1<T, m>), dim3(grid), dim3(threads), lds, st, x, xw, chi,      \
+                       norm_ptr, mode, Tm, N, rows, tw, done);                                \
+    break;
+    CGM_FOR_EACH_M(X)
+#undef X
+    default: return CGM_ERROR_UNSUPPORTED;
+  }
+  return hipGetLastError() == hipSuccess ? CGM_SUCCESS : CGM_ERROR_DEVICE;
+}
+
+template <typename T, typename G>
+int launch_pass2(const Plan& P, hipStream_t st, cplx<T>* Tm, const G* gq, const cplx<T>* tw,
+                 const int* done) {
+  const int M = P.M, N = P.N;
+  const int cols = P.cols2;
+  const int grid = M / cols;
+  const int threads = cols * fft_threads(M);
+  const size_t lds = (size_t)cols * M * sizeof(cplx<T>);
+  switch (M) {
+#define X(m)                                                                                  \
+  case m:                                                                                     \
+    hipLaunchKernelGGL((k_pass2<T, G, m>), dim3(grid), dim3(threads), lds, st, Tm, gq, N, cols, \
+                       tw, done);                                                             \
+    break;
+    CGM_FOR_EACH_M(X)
+#undef X
+    default: return CGM_ERROR_UNSUPPORTED;
+  }
+  return hipGetLastError() == hipSuccess ? CGM_SUCCESS : CGM_ERROR_DEVICE;
+}
+
+template <typename T>
+int launch_pass3(const Plan& P, hipStream_t st, const cplx<T>* Tm, const cplx<T>* x, const T* chi,
+                 const cplx<T>* b, cplx<T>* y, int mode, const cplx<T>* tw, const int* done) {
+  const int M = P.M, N = P.N;
+  const int rows = P.rows3;
+  const int grid = (N + rows - 1) / rows;
+  const int threads = rows * fft_threads(M);
+  const size_t lds = (size_t)rows * M * sizeof(cplx<T>);
+  const T inv = T(1.0 / ((double)M * (double)M));
+  switch (M) {
+#define X(m)                                                                                   \
+  case m:                                                                                      \
+    hipLaunchKernelGGL((k_pass3<T, m>), dim3(grid), dim3(threads), lds, st, Tm, x, chi, b, y,   \
+                       mode, N, rows, inv, tw, done);                                          \
+    break;
+    CGM_FOR_EACH_M(X)
+#undef X
+    default: return CGM_ERROR_UNSUPPORTED;
+  }
+  return hipGetLastError() == hipSuccess ? CGM_SUCCESS : CGM_ERROR_DEVICE;
+}
+
+template int launch_pass1<float>(const Plan&, hipStream_t, const float2*, float2*, const float*,
+                                 const double*, int, float2*, const float2*, const int*);
+template int launch_pass1<double>(const Plan&, hipStream_t, const double2*, double2*,
+                                  const double*, const double*, int, double2*, const double2*,
+                                  const int*);
+template int launch_pass2<float, float2>(const Plan&, hipStream_t, float2*, const float2*,
+                                         const float2*, const int*);
+template int launch_pass2<double, double2>(const Plan&, hipStream_t, double2*, const double2*,
+                                           const double2*, const int*);
+template int launch_pass3<float>(const Plan&, hipStream_t, const float2*, const float2*,
+                                 const float*, const float2*, float2*, int, const float2*,
+                                 const int*);
+template int launch_pass3<double>(const Plan&, hipStream_t, const double2*, const double2*,
+                                  const double*, const double2*, double2*, int, const double2*,
+                                  const int*);
+
+int plan_init(Plan& P, int N) {
+  P.N = N;
+  P.M = choose_embedding(N);
+  if (P.M == 0) return CGM_ERROR_UNSUPPORTED;
+  const int nt = fft_threads(P.M);
+  P.rows1 = groups_for(P.M, 256);
+  P.rows3 = P.rows1;
+  P.cols2 = groups_for(P.M, 256);
+  while (P.M % P.cols2) P.cols2 /= 2;
+  if (P.cols2 < 1) P.cols2 = 1;
+  (void)nt;
+  return CGM_SUCCESS;
+}
+
+// FP64 spectrum of an N x N generator block in device memory.  Writes the
+// complex64 quarter (gq32, may be null) and/or the complex128 quarter
+// (gq64, may be null).
+int spectrum_from_block(const Plan& P, hipStream_t st, const double2* blk, const double2* tw64,
+                        float2* gq32, double2* gq64) {
+  const int N = P.N, M = P.M;
+  double2* E = nullptr;
+  CGM_HIP(hipMallocAsync((void**)&E, sizeof(double2) * (size_t)M * M, st));
+  const long MM = (long)M * M;
+  hipLaunchKernelGGL(k_embed, dim3((MM + 255) / 256), dim3(256), 0, st, blk, E, N, M);
+  const int nt = fft_threads(M);
+  const int rows = groups_for(M, 256);
+  const size_t lds = (size_t)rows * M * sizeof(double2);
+  switch (M) {
+#define X(m)                                                                                   \
+  case m:                                                                                      \
+    hipLaunchKernelGGL((k_fft_rows<double, m, -1>), dim3((M + rows - 1) / rows),                \
+                       dim3(rows * nt), lds, st, E, M, rows, tw64);                             \
+    hipLaunchKernelGGL((k_fft_cols<double, m, -1>), dim3(M / P.cols2), dim3(P.cols2 * nt),      \
+                       (size_t)P.cols2 * M * sizeof(double2), st, E, P.cols2, tw64);            \
+    break;
+    CGM_FOR_EACH_M(X)
+#undef X
+    default:
+      (void)hipFreeAsync(E, st);
+      return CGM_ERROR_UNSUPPORTED;
+  }
+  const int H = M / 2 + 1;
+  const long HH = (long)H * H;
+  if (gq32) hipLaunchKernelGGL((k_quarter<float2>), dim3((HH + 255) / 256), dim3(256), 0, st, E, gq32, M);
+  if (gq64) hipLaunchKernelGGL((k_quarter<double2>), dim3((HH + 255) / 256), dim3(256), 0, st, E, gq64, M);
+  CGM_HIP(hipGetLastError());
+  CGM_HIP(hipFreeAsync(E, st));
+  return CGM_SUCCESS;
+}
+
+int green_block(hipStream_t st, double2* blk, int N, int kind, double k, double sr, double sc) {
+  const long n = (long)N * N;
+  hipLaunchKernelGGL(k_green_block, dim3((n + 255) / 256), dim3(256), 0, st, blk, N, kind, k, sr, sc);
+  CGM_HIP(hipGetLastError());
+  return CGM_SUCCESS;
+}
+
+int gprep_block(hipStream_t st, const float2* gprep, double2* blk, int N) {
+  const long n = (long)N * N;
+  hipLaunchKernelGGL(k_gprep_block, dim3((n + 255) / 256), dim3(256), 0, st, gprep, blk, N);
+  CGM_HIP(hipGetLastError());
+  return CGM_SUCCESS;
+}
+
+}  // namespace cgm

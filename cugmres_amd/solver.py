@@ -1,0 +1,164 @@
+"""Device-level Python front end of the C ABI.
+
+PyTorch-ROCm is used only as the device-memory and stream plumbing: vectors
+are ``torch`` tensors on ``cuda:<device>`` whose ``data_ptr()`` is handed to
+the C ABI, and the context is bound to torch's current stream.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+
+_c = ctypes
+
+
+def _ptr(t):
+    return _c.c_void_p(t.data_ptr()) if t is not None else None
+
+
+class Context:
+    """One solver context: grid size N, embedding M, one device, one stream."""
+
+    def __init__(self, N, device=0):
+        import torch
+        self.torch = torch
+        self.N = int(N)
+        self.device = torch.device("cuda", device)
+        p = _c.c_void_p()
+        check(lib().cgm_create(_c.byref(p), self.N, device), "cgm_create")
+        self._ptr = p
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        check(lib().cgm_set_stream(self._ptr, _c.c_void_p(stream)), "cgm_set_stream")
+        m = _c.c_int()
+        check(lib().cgm_get_embedding(self._ptr, _c.byref(m)), "cgm_get_embedding")
+        self.M = m.value
+        s = _c.c_size_t()
+        check(lib().cgm_spectrum_size(self._ptr, _c.byref(s)), "cgm_spectrum_size")
+        self.spectrum_len = s.value
+
+    def close(self):
+        if getattr(self, "_ptr", None) is not None:
+            lib().cgm_destroy(self._ptr)
+            self._ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._ptr
+
+    # ---- tensors --------------------------------------------------------
+    def empty(self, dtype="c64"):
+        t = self.torch
+        dt = {"c64": t.complex64, "c128": t.complex128, "f32": t.float32, "f64": t.float64}[dtype]
+        return t.empty(self.N * self.N, dtype=dt, device=self.device)
+
+    def to_device(self, a, dtype="c64"):
+        t = self.torch
+        dt = {"c64": t.complex64, "c128": t.complex128, "f32": t.float32, "f64": t.float64}[dtype]
+        return t.as_tensor(np.ascontiguousarray(a).reshape(-1)).to(device=self.device, dtype=dt).contiguous()
+
+    def _spectra(self, want32, want64):
+        t = self.torch
+        g32 = t.empty(self.spectrum_len, dtype=t.complex64, device=self.device) if want32 else None
+        g64 = t.empty(self.spectrum_len, dtype=t.complex128, device=self.device) if want64 else None
+        return g32, g64
+
+    # ---- spectra ----------------------------------------------------------
+    def spectrum_helmholtz(self, kh, fp64=False):
+        g32, g64 = self._spectra(True, fp64)
+        check(lib().cgm_spectrum_helmholtz(self._ptr, float(kh), _ptr(g32), _ptr(g64)),
+              "cgm_spectrum_helmholtz")
+        return (g32, g64) if fp64 else g32
+
+    def spectrum_reference(self, k, row_shift=0.5, col_shift=0.0, fp64=False):
+        g32, g64 = self._spectra(True, fp64)
+        check(lib().cgm_spectrum_reference(self._ptr, float(k), float(row_shift), float(col_shift),
+                                           _ptr(g32), _ptr(g64)), "cgm_spectrum_reference")
+        return (g32, g64) if fp64 else g32
+
+    def spectrum_from_gprep(self, gprep):
+        g = self.to_device(np.asarray(gprep, np.complex64), "c64")
+        g32, _ = self._spectra(True, False)
+        check(lib().cgm_spectrum_from_gprep(self._ptr, _ptr(g), _ptr(g32)), "cgm_spectrum_from_gprep")
+        return g32
+
+    def spectrum_from_block(self, block, fp64=False):
+        b = self.to_device(np.asarray(block, np.complex128), "c128")
+        g32, g64 = self._spectra(True, fp64)
+        check(lib().cgm_spectrum_from_block(self._ptr, _ptr(b), _ptr(g32), _ptr(g64)),
+              "cgm_spectrum_from_block")
+        return (g32, g64) if fp64 else g32
+
+    # ---- operator ---------------------------------------------------------
+    def matvec(self, gamma, chi, x, mode=_lib.OP_FORWARD, out=None):
+        y = out if out is not None else self.torch.empty_like(x)
+        check(lib().cgm_matvec(self._ptr, _ptr(gamma), _ptr(chi), _ptr(x), _ptr(y), int(mode)),
+              "cgm_matvec")
+        return y
+
+    def matvec_c128(self, gamma64, chi64, x, mode=_lib.OP_FORWARD, out=None):
+        y = out if out is not None else self.torch.empty_like(x)
+        check(lib().cgm_matvec_c128(self._ptr, _ptr(gamma64), _ptr(chi64), _ptr(x), _ptr(y), int(mode)),
+              "cgm_matvec_c128")
+        return y
+
+    def synchronize(self):
+        check(lib().cgm_synchronize(self._ptr), "cgm_synchronize")
+
+    # ---- solvers ----------------------------------------------------------
+    def gmres(self, gamma, chi, b, x=None, restart=30, tol=1e-6, max_cycles=100, adjoint=False,
+              history=False):
+        """Restarted GMRES(restart).  Returns (x, info dict).  x (if given) is
+        the initial guess and is updated in place."""
+        t = self.torch
+        x0_zero = x is None
+        if x is None:
+            x = t.zeros_like(b)
+        hist = None
+        if history:
+            hist = t.zeros(restart * max(max_cycles, 1) + 1, dtype=t.float64, device=self.device)
+        p = _lib.GmresParams(int(restart), int(max_cycles), float(tol), int(bool(adjoint)), int(x0_zero))
+        r = _lib.GmresResult()
+        st = lib().cgm_gmres(self._ptr, _ptr(gamma), _ptr(chi), _ptr(b), _ptr(x), _c.byref(p),
+                             _c.byref(r), _ptr(hist))
+        check(st, "cgm_gmres", ok=(_lib.SUCCESS, _lib.ERROR_NOT_CONVERGED))
+        info = dict(iterations=r.iterations, cycles=r.cycles, converged=bool(r.converged),
+                    rel_residual=r.rel_residual, beta0=r.beta0, bnorm=r.bnorm)
+        if history:
+            info["history"] = hist[: r.iterations + 1].cpu().numpy()
+        return x, info
+
+    def gmres_refine(self, gamma32, gamma64, chi64, b, x=None, inner_restart=50, inner_tol=1e-4,
+                     inner_max_cycles=40, outer_tol=1e-12, max_outer=30):
+        t = self.torch
+        if x is None:
+            x = t.zeros_like(b)
+        p = _lib.RefineParams(int(inner_restart), int(inner_max_cycles), float(inner_tol),
+                              float(outer_tol), int(max_outer))
+        r = _lib.RefineResult()
+        st = lib().cgm_gmres_refine(self._ptr, _ptr(gamma32), _ptr(gamma64), _ptr(chi64), _ptr(b),
+                                    _ptr(x), _c.byref(p), _c.byref(r))
+        check(st, "cgm_gmres_refine", ok=(_lib.SUCCESS, _lib.ERROR_NOT_CONVERGED))
+        return x, dict(outer=r.outer, inner_iterations=r.inner_iterations,
+                       converged=bool(r.converged), rel_residual=r.rel_residual)
+
+
+def reference_chi(N):
+    return float(lib().cgm_reference_chi(int(N)))
+
+
+def n_timestamps(maxiter):
+    return int(lib().cgm_n_timestamps(int(maxiter)))
+
+
+def bessel01(x):
+    out = [_c.c_double() for _ in range(4)]
+    lib().cgm_bessel01(float(x), *[_c.byref(v) for v in out])
+    return tuple(v.value for v in out)

@@ -1,0 +1,247 @@
+/*
+ * cugmres.h -- C ABI of the gfx950 GMRES solver for the 2-D Helmholtz
+ * Green's-function integral equation
+ *
+ *     A x = x - G (chi .* x),      G block-Toeplitz with Toeplitz blocks,
+ *
+ * applied through a zero-padded M x M circulant embedding (M >= 2N-1).
+ *
+ * Every entry point takes plain pointers and sizes.  Vectors are N*N
+ * interleaved complex values, row-major, index i*N + j with i the row (the
+ * direction of the reference's incident wave).  Device pointers are HIP
+ * device memory on the context's device.  Functions return a cgm_status;
+ * none of them exits the process.
+ *
+ * Reference interfaces replaced (paths relative to the reference checkout,
+ * B/ = "Benchmarks of CUDA-based GMRES Solver for Toeplitz and Hankel
+ * Matrices and Applications to Topology Optimization of Photonic Devices/"):
+ *   cufftHandle + cublasHandle_t + cusolverDnHandle_t  -> cgm_ctx_t
+ *        (B/CUDA_GMRES/launch_GMRES.cuh:93-101)
+ *   cufftExecC2C(plan, G_prep, FORWARD)                 -> cgm_spectrum_from_gprep
+ *        (B/CUDA_GMRES/launch_GMRES.cuh:138-142)
+ *   Green_matrix_create_MKL + get_gamma_array           -> cgm_spectrum_reference
+ *        (B/MKL_GMRES_test_09122019/source/loop_functions.h:128-168,
+ *         B/MKL_GMRES_test_09122019/source/mkl_based_functions.h:19-34)
+ *   G_x_fft_matvec (+ _2D_to_1D_kernel)                 -> cgm_matvec, cgm_G_x_fft_matvec
+ *        (B/CUDA_GMRES/Fast_matvec.cuh:1-56, B/CUDA_GMRES/GMRES_kernels.cuh:286-326)
+ *   Fast_GMRES_with_CUDA                                -> cgm_gmres, cgm_Fast_GMRES_with_CUDA
+ *        (B/CUDA_GMRES/GMRES.cuh:22-446)
+ *   cuSolve_LES                                         -> cgm_cuSolve_LES
+ *        (B/CUDA_GMRES/GMRES.cuh:450-511)
+ *   init_x0_kernel                                      -> cgm_reference_plane_wave
+ *        (B/CUDA_GMRES/GMRES_kernels.cuh:557-572)
+ */
+#ifndef CUGMRES_H
+#define CUGMRES_H
+
+#include <stddef.h>
+#include <stdbool.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CGM_VERSION_MAJOR 0
+#define CGM_VERSION_MINOR 1
+
+/* Layout-identical to cuComplex/hipFloatComplex ({float x, y}) and to the
+ * reference's complex8 (B/MKL_GMRES_test_09122019/source/parameters.h:50). */
+typedef struct { float re, im; } cgm_complex64;
+typedef struct { double re, im; } cgm_complex128;
+
+typedef struct cgm_context* cgm_ctx_t;
+
+typedef enum {
+  CGM_SUCCESS = 0,
+  CGM_ERROR_INVALID_VALUE = 1,
+  CGM_ERROR_DEVICE = 2,
+  CGM_ERROR_ALLOC = 3,
+  CGM_ERROR_NOT_CONVERGED = 4,
+  CGM_ERROR_NONFINITE = 5,
+  CGM_ERROR_UNSUPPORTED = 7,
+  CGM_ERROR_NOT_SYMMETRIC = 8
+} cgm_status;
+
+/* Operator modes (cgm_matvec). */
+typedef enum {
+  CGM_OP_FORWARD = 0,  /* y = x - G(chi x)          (B/CUDA_GMRES/Fast_matvec.cuh:1-28)  */
+  CGM_OP_ADJOINT = 1,  /* y = x - chi G(x) = A^T x  (B/CUDA_GMRES/Fast_matvec.cuh:31-56) */
+  CGM_OP_GREEN = 2     /* y = G(x)                  (B/PyGMRES/GMRES.py:30-44)          */
+} cgm_op_mode;
+
+const char* cgm_status_string(int status);
+/* Message of the last error raised in this thread ("" if none). */
+const char* cgm_last_error(void);
+void cgm_version(int* major, int* minor);
+
+/* ---- context ------------------------------------------------------------ */
+/* One context per (grid size N, device, host thread).  Chooses the embedding
+ * size M (smallest of 2^a, 3*2^a >= 2N-1; N <= 2048) and owns the FFT
+ * twiddle tables and the scratch of one matvec.  Replaces the reference's
+ * cufftPlan2d/cublasCreate/cusolverDnCreate (B/CUDA_GMRES/launch_GMRES.cuh:95-101). */
+int cgm_create(cgm_ctx_t* ctx, int N, int device);
+int cgm_destroy(cgm_ctx_t ctx);
+/* All work of the context is enqueued on this HIP stream (hipStream_t passed
+ * as void*; NULL = the default stream). */
+int cgm_set_stream(cgm_ctx_t ctx, void* stream);
+int cgm_get_embedding(cgm_ctx_t ctx, int* M);
+/* Number of complex entries of a kernel spectrum: (M/2+1)^2.  The spectrum of
+ * an even circulant embedding is even in both frequencies, so only this
+ * quarter is stored. */
+int cgm_spectrum_size(cgm_ctx_t ctx, size_t* n_complex);
+
+/* ---- kernel spectrum (FP64 on the device, stored complex64/complex128) ---- */
+/* From the reference's embedded spatial array G_prep ((2N-1)^2 complex64,
+ * device memory, layout of B/MKL_GMRES_test_09122019/source/loop_functions.h:150-163).
+ * Returns CGM_ERROR_NOT_SYMMETRIC if G_prep is not even in both offsets. */
+int cgm_spectrum_from_gprep(cgm_ctx_t ctx, const cgm_complex64* dev_gprep,
+                            cgm_complex64* dev_gamma);
+/* From an N x N generator block g[|di|, |dj|] (complex128, device memory). */
+int cgm_spectrum_from_block(cgm_ctx_t ctx, const cgm_complex128* dev_block,
+                            cgm_complex64* dev_gamma, cgm_complex128* dev_gamma64);
+/* The reference's operator generated on the device in FP64:
+ * g = (i/4) H0(k sqrt((i - row_shift)^2 + (j - col_shift)^2)); row_shift = 0.5,
+ * col_shift = 0 is the C++ orientation, the reverse the Python one. */
+int cgm_spectrum_reference(cgm_ctx_t ctx, double wave_number, double row_shift, double col_shift,
+                           cgm_complex64* dev_gamma, cgm_complex128* dev_gamma64);
+/* The north-star operator k0^2 G for cells of size h, kh = k0 h:
+ * off-diagonal kh^2 (i/4) H0(kh |r_p - r_q|/h), self term (i pi ka/2) H1(ka) - 1,
+ * ka = kh / sqrt(pi).  Either output may be NULL. */
+int cgm_spectrum_helmholtz(cgm_ctx_t ctx, double kh, cgm_complex64* dev_gamma,
+                           cgm_complex128* dev_gamma64);
+
+/* ---- operator ------------------------------------------------------------ */
+/* y = A x (mode FORWARD), A^T x (ADJOINT) or G x (GREEN); chi is the real
+ * contrast per cell (N*N float).  Asynchronous on the context stream. */
+int cgm_matvec(cgm_ctx_t ctx, const cgm_complex64* dev_gamma, const float* dev_chi,
+               const cgm_complex64* dev_x, cgm_complex64* dev_y, int mode);
+/* FP64 variant (spectrum from dev_gamma64, chi as double). */
+int cgm_matvec_c128(cgm_ctx_t ctx, const cgm_complex128* dev_gamma64, const double* dev_chi,
+                    const cgm_complex128* dev_x, cgm_complex128* dev_y, int mode);
+
+/* ---- solver --------------------------------------------------------------- */
+typedef struct {
+  int restart;        /* Krylov dimension per cycle m (1..512)                    */
+  int max_cycles;     /* restart cap (>= 1)                                      */
+  double tol;         /* stop when the true ||b - A x|| <= tol ||b||             */
+  int adjoint;        /* 1: solve A^T x = b                                      */
+  int x0_zero;        /* 1: ignore dev_x on entry and start from 0               */
+} cgm_gmres_params;
+
+typedef struct {
+  int iterations;          /* Arnoldi steps in total                            */
+  int cycles;              /* restart cycles run                                */
+  int converged;           /* 1 when the true relative residual <= tol          */
+  double rel_residual;     /* true ||b - A x|| / ||b|| at exit (complex64 math) */
+  double beta0;            /* ||b - A x0||                                      */
+  double bnorm;            /* ||b||                                             */
+} cgm_gmres_result;
+
+/* Restarted GMRES(m): CGS2 orthogonalisation, unitary Givens rotations and
+ * back substitution in FP64, everything device resident; one host
+ * synchronisation per restart cycle.  dev_hist (optional, device, length
+ * restart*max_cycles + 1) receives the absolute Givens residual estimates.
+ * Synchronous on return.  Returns CGM_SUCCESS or CGM_ERROR_NOT_CONVERGED
+ * (x then holds the last iterate). */
+int cgm_gmres(cgm_ctx_t ctx, const cgm_complex64* dev_gamma, const float* dev_chi,
+              const cgm_complex64* dev_b, cgm_complex64* dev_x, const cgm_gmres_params* params,
+              cgm_gmres_result* result, double* dev_hist);
+
+/* FP64 iterative refinement: the true residual r = b - A x is formed with the
+ * complex128 operator, the correction A d = r is solved by complex64
+ * GMRES(inner_restart) to inner_tol, x += d, until ||r|| <= outer_tol ||b||. */
+typedef struct {
+  int inner_restart;
+  int inner_max_cycles;
+  double inner_tol;
+  double outer_tol;
+  int max_outer;
+} cgm_refine_params;
+
+typedef struct {
+  int outer;               /* refinement steps                                  */
+  int inner_iterations;    /* Arnoldi steps over all inner solves               */
+  int converged;
+  double rel_residual;     /* FP64 true relative residual at exit               */
+} cgm_refine_result;
+
+int cgm_gmres_refine(cgm_ctx_t ctx, const cgm_complex64* dev_gamma,
+                     const cgm_complex128* dev_gamma64, const double* dev_chi,
+                     const cgm_complex128* dev_b, cgm_complex128* dev_x,
+                     const cgm_refine_params* params, cgm_refine_result* result);
+
+/* ---- reference-compatible entry points ------------------------------------ */
+/* The reference's plane wave exp(-i k i) with k = 2*3.14f/(N/6.f)
+ * (B/CUDA_GMRES/GMRES_kernels.cuh:12,557-572), on the device. */
+int cgm_reference_plane_wave(cgm_ctx_t ctx, cgm_complex64* dev_x);
+/* CHI = (2.25 - 1) k^2 of B/CUDA_GMRES/GMRES_kernels.cuh:18-19, float. */
+float cgm_reference_chi(int N);
+/* Contrast used by the two entry points below for mask cells (default
+ * cgm_reference_chi(N)). */
+int cgm_set_mask_contrast(cgm_ctx_t ctx, float chi);
+/* Number of timestamps Fast_GMRES_with_CUDA writes for maxiter
+ * (B/CUDA_GMRES/launch_GMRES.cuh:245-307). */
+unsigned cgm_n_timestamps(unsigned maxiter);
+
+/* Drop-in for Fast_GMRES_with_CUDA (B/CUDA_GMRES/GMRES.cuh:22-35), same
+ * parameter order and meaning:
+ *   dev_gamma_array   spectrum from cgm_spectrum_* (caller owned)
+ *   dev_mask          bool[N*N]; the contrast is cgm_set_mask_contrast * mask
+ *   dev_solution      in: x0, out: x
+ *   dev_actual_residual  callee-allocated device float[maxiter+1], caller
+ *                     frees with cgm_free: [0] = ||r0||, [i] = residual
+ *                     estimate after i steps
+ *   GMRES_n           Krylov steps done
+ *   plan              the context (in place of cufftHandle + cuBLAS handle)
+ *   tolerance         absolute; no iteration when ||r0|| <= tolerance
+ *   for_gradient      1: solve A^T x = e_{h_index_of_max}, else A x = b with
+ *                     b the reference plane wave
+ *   h_res_vs_tol_p    final estimate > tolerance
+ *   maxiter           Krylov dimension (fixed, no restart)
+ *   h_computation_times  caller array of cgm_n_timestamps(maxiter) entries,
+ *                     stamped per phase (may be NULL)
+ * The Givens rotations are unitary, so the estimates are the true GMRES
+ * residual norms in exact arithmetic.  Returns a cgm_status. */
+struct timespec;
+int cgm_Fast_GMRES_with_CUDA(const cgm_complex64* dev_gamma_array, const bool* dev_mask,
+                             cgm_complex64* dev_solution, float** dev_actual_residual,
+                             unsigned int* GMRES_n, cgm_ctx_t plan, float tolerance,
+                             bool for_gradient, unsigned int h_index_of_max,
+                             bool* h_res_vs_tol_p, unsigned int maxiter,
+                             struct timespec* h_computation_times, unsigned int N);
+
+/* Drop-in for G_x_fft_matvec followed by _2D_to_1D_kernel
+ * (B/CUDA_GMRES/Fast_matvec.cuh:1-28, GMRES_kernels.cuh:286-326): writes
+ * dev_out = x - G(chi mask x) on the N x N grid (the (2N-1)^2 intermediate of
+ * the reference stays internal). */
+int cgm_G_x_fft_matvec(const cgm_complex64* dev_gamma_array, const bool* dev_mask,
+                       const cgm_complex64* dev_solution, cgm_complex64* dev_out,
+                       cgm_ctx_t plan, unsigned int N);
+/* The gradient pairing: the unmasked G_x_fft_matvec overload
+ * (B/CUDA_GMRES/Fast_matvec.cuh:31-56) followed by the masked crop
+ * _2D_to_1D_kernel(dev_mask, ...) (B/CUDA_GMRES/GMRES_kernels.cuh:305-326),
+ * i.e. dev_out = x - chi mask G(x) = A^T x. */
+int cgm_G_x_fft_matvec_T(const cgm_complex64* dev_gamma_array, const bool* dev_mask,
+                         const cgm_complex64* dev_solution, cgm_complex64* dev_out,
+                         cgm_ctx_t plan, unsigned int N);
+
+/* Drop-in for cuSolve_LES (B/CUDA_GMRES/GMRES.cuh:450-511): solves the m x m
+ * column-major system A y = B in place (B <- y) by LU with partial pivoting,
+ * on the device in FP64.  dev_work, dev_Ipiv and dev_info as in the reference
+ * (dev_work may be NULL); *dev_info = 0 on success or k if U(k,k) = 0. */
+int cgm_cuSolve_LES(cgm_complex64* dev_A, cgm_complex64* dev_B, int m, cgm_ctx_t ctx,
+                    cgm_complex64* dev_work, int* dev_Ipiv, int* dev_info);
+
+/* ---- memory and host helpers ---------------------------------------------- */
+int cgm_malloc(void** ptr, size_t bytes);
+int cgm_free(void* ptr);
+int cgm_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int cgm_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int cgm_synchronize(cgm_ctx_t ctx);
+/* FP64 J0, J1, Y0, Y1 (host); H_n^(1) = J_n + i Y_n. */
+void cgm_bessel01(double x, double* j0, double* j1, double* y0, double* y1);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CUGMRES_H */
