@@ -91,6 +91,8 @@ SIGNATURES = {
     "cgm_memcpy_d2h": (ctypes.c_int, [c_void_p, c_void_p, ctypes.c_size_t]),
     "cgm_synchronize": (ctypes.c_int, [c_void_p]),
     "cgm_bessel01": (None, [ctypes.c_double] + [c_double_p] * 4),
+    "cgm_profile_step": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, ctypes.c_int,
+                                        c_float_p]),
 }
 
 

@@ -205,6 +205,9 @@ static int spectrum_generated(cgm_context* c, int kind, double k, double sr, dou
 
 extern "C" int cgm_spectrum_reference(cgm_ctx_t c, double k, double sr, double sc,
                                       cgm_complex64* g32, cgm_complex128* g64) {
+  // the shifted distance must stay positive at every offset (H0 is singular at 0)
+  CGM_CHECK(isfinite(sr) && isfinite(sc) && (fabs(sr - rint(sr)) > 0 || fabs(sc - rint(sc)) > 0),
+            CGM_ERROR_INVALID_VALUE, "cgm_spectrum_reference: shifts must not both be integers");
   return spectrum_generated(c, 1, k, sr, sc, g32, g64);
 }
 
@@ -431,6 +434,61 @@ extern "C" int cgm_gmres_refine(cgm_ctx_t c, const cgm_complex64* g32, const cgm
   (void)hipFree(d32);
   if (rc) return rc;
   return res->converged ? CGM_SUCCESS : CGM_ERROR_NOT_CONVERGED;
+}
+
+// ---------------------------------------------------------------------------
+// diagnostics: per-kernel durations of one Arnoldi step
+// ---------------------------------------------------------------------------
+extern "C" int cgm_profile_step(cgm_ctx_t c, const cgm_complex64* gamma, const float* chi,
+                                const cgm_complex64* x, int restart, int reps, float* ms) {
+  CGM_CHECK(c && gamma && chi && x && ms && reps > 0, CGM_ERROR_INVALID_VALUE, "null argument");
+  CGM_CHECK(restart >= 1 && restart <= kMaxRestart, CGM_ERROR_INVALID_VALUE, "bad restart");
+  CGM_HIP(hipSetDevice(c->device));
+  const long n = (long)c->N * c->N;
+  int rc = krylov_alloc(c->ws, n, restart, c->stream);
+  if (rc) return rc;
+  KrylovWs& ws = c->ws;
+  const int i = restart / 2;  // mid-cycle step: basis of restart/2 + 1 vectors
+  // orthonormal basis e_{j*stride} and w = e_{(i+1)*stride}: every projection
+  // is zero, so repeated launches leave the data unchanged (the traffic does
+  // not depend on the values)
+  CGM_HIP(hipMemsetAsync(ws.V, 0, sizeof(float2) * n * (i + 2), c->stream));
+  const float2 one = make_float2(1.f, 0.f);
+  const long stride = n / (i + 2);
+  for (int j = 0; j <= i + 1; ++j)
+    CGM_HIP(hipMemcpy(ws.V + (long)j * n + j * stride, &one, sizeof(float2), hipMemcpyHostToDevice));
+  // a state whose tolerance can never be met keeps every launch live
+  DevState init{};
+  init.tol_abs = -1.0;
+  init.hn = 1.0;
+  init.beta = 1.0;
+  CGM_HIP(hipMemcpyAsync(c->st, &init, sizeof(DevState), hipMemcpyHostToDevice, c->stream));
+  std::vector<double2> g1(ws.m + 1, make_double2(1.0, 0.0));
+  CGM_HIP(hipMemcpyAsync(ws.g, g1.data(), sizeof(double2) * (ws.m + 1), hipMemcpyHostToDevice, c->stream));
+  hipEvent_t e0, e1;
+  CGM_HIP(hipEventCreate(&e0));
+  CGM_HIP(hipEventCreate(&e1));
+  if (!c->vec_tmp) CGM_HIP(hipMalloc((void**)&c->vec_tmp, sizeof(float2) * n));
+  for (int k = 0; k < 6 && !rc; ++k) {
+    for (int r = -2; r < reps && !rc; ++r) {  // two untimed warm-up launches
+      if (r == 0) CGM_HIP(hipEventRecord(e0, c->stream));
+      switch (k) {
+        case 0: rc = launch_pass1<float>(c->plan, c->stream, (const float2*)x, nullptr, chi, nullptr, kModeForward, c->T32, c->tw32, nullptr); break;
+        case 1: rc = launch_pass2<float, float2>(c->plan, c->stream, c->T32, (const float2*)gamma, c->tw32, nullptr); break;
+        case 2: rc = launch_pass3<float>(c->plan, c->stream, c->T32, (const float2*)x, chi, nullptr, c->vec_tmp, kModeForward, c->tw32, nullptr); break;
+        default: rc = launch_cgs2_step(c->stream, ws, i, c->st, nullptr, k - 3); break;
+      }
+    }
+    CGM_HIP(hipEventRecord(e1, c->stream));
+    CGM_HIP(hipEventSynchronize(e1));
+    CGM_HIP(hipMemcpyAsync(c->st, &init, sizeof(DevState), hipMemcpyHostToDevice, c->stream));
+    float t = 0.f;
+    CGM_HIP(hipEventElapsedTime(&t, e0, e1));
+    ms[k] = t / reps;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
 }
 
 // ---------------------------------------------------------------------------

@@ -117,9 +117,46 @@ __device__ inline void dft(cplx<T>* v) {
   else if constexpr (R == 8) dft8<T, SIGN>(v);
 }
 
+// Twiddles of one thread for every stage with span NS > 1, flattened:
+// stage S, butterfly b, r = 1..R-1 at tw_offset<M>(S) + b*(R-1) + r-1.
+template <int M> __host__ __device__ constexpr int fft_stage_span(int S) {
+  int ns = 1;
+  for (int s = 0; s < S; ++s) ns *= fft_radix<M>(s);
+  return ns;
+}
+template <int M> __host__ __device__ constexpr int fft_tw_offset(int S) {
+  int off = 0;
+  for (int s = 0; s < S; ++s)
+    if (fft_stage_span<M>(s) > 1) off += (FftShape<M>::EPT / fft_radix<M>(s)) * (fft_radix<M>(s) - 1);
+  return off;
+}
+template <int M> __host__ __device__ constexpr int fft_tw_count() {
+  return fft_tw_offset<M>(fft_num_stages<M>());
+}
+
+template <typename T, int M, int S>
+__device__ inline void fft_load_tw(cplx<T>* reg, const cplx<T>* __restrict__ tw, int t) {
+  if constexpr (S < fft_num_stages<M>()) {
+    constexpr int R = fft_radix<M>(S);
+    constexpr int NS = fft_stage_span<M>(S);
+    if constexpr (NS > 1) {
+      constexpr int NB = FftShape<M>::EPT / R;
+      constexpr int NT = FftShape<M>::NT;
+      constexpr int OFF = fft_tw_offset<M>(S);
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        const int k = (t + b * NT) % NS;
+#pragma unroll
+        for (int r = 1; r < R; ++r) reg[OFF + b * (R - 1) + r - 1] = tw[(r * k) * (M / (NS * R))];
+      }
+    }
+    fft_load_tw<T, M, S + 1>(reg, tw, t);
+  }
+}
+
 // One Stockham stage for the butterflies owned by thread t.
-template <typename T, int M, int SIGN, int R, int NS>
-__device__ inline void fft_stage(cplx<T>* buf, const cplx<T>* __restrict__ tw, int t) {
+template <typename T, int M, int SIGN, int R, int NS, int OFF>
+__device__ inline void fft_stage(cplx<T>* buf, const cplx<T>* reg, int t) {
   constexpr int EPT = FftShape<M>::EPT;
   constexpr int NT = FftShape<M>::NT;
   constexpr int NB = EPT / R;        // butterflies per thread
@@ -139,7 +176,7 @@ __device__ inline void fft_stage(cplx<T>* buf, const cplx<T>* __restrict__ tw, i
     if constexpr (NS > 1) {
 #pragma unroll
       for (int r = 1; r < R; ++r) {
-        cplx<T> w = tw[(r * k) * (M / (NS * R))];
+        cplx<T> w = reg[OFF + b * (R - 1) + r - 1];
         if (SIGN > 0) w.y = -w.y;
         v[b][r] = cmul(v[b][r], w);
       }
@@ -153,20 +190,29 @@ __device__ inline void fft_stage(cplx<T>* buf, const cplx<T>* __restrict__ tw, i
 }
 
 template <typename T, int M, int SIGN, int S, int NS>
-__device__ inline void fft_stages(cplx<T>* buf, const cplx<T>* __restrict__ tw, int t) {
+__device__ inline void fft_stages(cplx<T>* buf, const cplx<T>* reg, int t) {
   if constexpr (S < fft_num_stages<M>()) {
     constexpr int R = fft_radix<M>(S);
-    fft_stage<T, M, SIGN, R, NS>(buf, tw, t);
-    fft_stages<T, M, SIGN, S + 1, NS * R>(buf, tw, t);
+    fft_stage<T, M, SIGN, R, NS, fft_tw_offset<M>(S)>(buf, reg, t);
+    fft_stages<T, M, SIGN, S + 1, NS * R>(buf, reg, t);
   }
 }
+
+// Per-thread twiddle registers, loaded once per kernel (before the data, so
+// the table reads overlap the first global loads) and reused by every
+// forward and inverse transform the thread takes part in.
+template <typename T, int M> struct FftTwiddles {
+  static constexpr int K = fft_tw_count<M>() > 0 ? fft_tw_count<M>() : 1;
+  cplx<T> reg[K];
+  __device__ inline void load(const cplx<T>* __restrict__ tw, int t) { fft_load_tw<T, M, 0>(reg, tw, t); }
+};
 
 // Full transform of buf[0..M) (LDS) by threads t in [0, NT).  The caller
 // must have synchronised the work-group after filling buf; on return buf
 // holds the result and the group is synchronised.
 template <typename T, int M, int SIGN>
-__device__ inline void lds_fft(cplx<T>* buf, const cplx<T>* __restrict__ tw, int t) {
-  fft_stages<T, M, SIGN, 0, 1>(buf, tw, t);
+__device__ inline void lds_fft(cplx<T>* buf, const FftTwiddles<T, M>& tws, int t) {
+  fft_stages<T, M, SIGN, 0, 1>(buf, tws.reg, t);
 }
 
 }  // namespace cgm

@@ -55,7 +55,7 @@ struct DevState {
 enum { kTicketDots1 = 0, kTicketDots2 = 1, kTicketNorm = 2, kTicketInit = 3, kTicketMisc = 4 };
 
 constexpr int kMaxRestart = 512;
-constexpr int kKB = 256;  // threads per block of the Krylov kernels
+constexpr int kKB = 1024;  // threads per block of the streaming Krylov kernels
 
 struct KrylovWs {
   int m = 0;              // restart this workspace was sized for
@@ -78,7 +78,8 @@ int krylov_alloc(KrylovWs& ws, long n, int m, hipStream_t st);
 void krylov_free(KrylovWs& ws);
 int launch_norm_init(hipStream_t st, const KrylovWs& ws, const float2* r, DevState* s, int what,
                      double tol);
-int launch_cgs2_step(hipStream_t st, const KrylovWs& ws, int i, DevState* s, double* hist);
+int launch_cgs2_step(hipStream_t st, const KrylovWs& ws, int i, DevState* s, double* hist,
+                     int part = -1);
 int launch_backsub_update(hipStream_t st, const KrylovWs& ws, DevState* s, float2* x);
 int launch_norm64(hipStream_t st, const double2* v, long n, double* partn, int G, DevState* s);
 

@@ -12,7 +12,11 @@
 // Reductions are deterministic: every block writes its partial sums, the
 // last block to finish (ticket counter) combines them in a fixed order and
 // resets the counter, so repeated runs are bitwise identical and nothing has
-// to be cleared between launches.
+// to be cleared between launches.  The partials are the only data that
+// crosses blocks inside a kernel; they are written and read with agent-scope
+// atomic stores/loads (coherent across the XCDs' L2 caches) and the writer
+// waits for its stores before taking a ticket.  A device-wide fence would
+// instead write back and invalidate the whole L2 in every block.
 #include <math.h>
 
 #include "cgm_internal.h"
@@ -25,32 +29,68 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
-// True in the last block to arrive; partial sums written by every block
-// before the call are then visible to it.
+// Sum of a float over the 64 lanes of a wave with DPP moves (VALU, no LDS
+// traffic): rotate-by-8 and rotate-by-4 inside each row of 16 lanes, xor-2 and
+// xor-1 quad permutes, then the four row sums are read out with readlane.
+// Every lane gets the same value; the summation order is fixed.
+template <int CTRL>
+__device__ inline float dpp_mov(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
+}
+__device__ inline float wave_sum_f(float v) {
+  v += dpp_mov<0x128>(v);  // row_ror:8
+  v += dpp_mov<0x124>(v);  // row_ror:4
+  v += dpp_mov<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_mov<0xB1>(v);   // quad_perm [1,0,3,2]
+  const float r0 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 0));
+  const float r1 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 16));
+  const float r2 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 32));
+  const float r3 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 48));
+  return (r0 + r1) + (r2 + r3);
+}
+
+__device__ inline void put(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline double get(const double* p) {
+  return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void put2(double2* p, double re, double im) {
+  put(&p->x, re);
+  put(&p->y, im);
+}
+__device__ inline double2 get2(const double2* p) { return make_double2(get(&p->x), get(&p->y)); }
+
+// True in the last block to arrive.  Every thread first waits for its own
+// outstanding memory operations (the partial stores above), so the partials
+// of all blocks are complete when the last ticket is drawn.
 __device__ inline bool arrive_last(unsigned* ticket) {
   __shared__ int s_last;
-  __threadfence();
+  __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   if (threadIdx.x == 0) {
-    const unsigned t = atomicAdd(ticket, 1u);
+    const unsigned t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     s_last = (t == gridDim.x - 1) ? 1 : 0;
   }
   __syncthreads();
-  if (s_last) __threadfence();
   return s_last != 0;
+}
+
+__device__ inline void reset_ticket(unsigned* ticket) {
+  __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Reduce CH complex float accumulators across the block and store them as
 // part[(c0 + jj) * G + blockIdx.x] (FP64).
-template <int CH>
+template <int BS, int CH>
 __device__ inline void block_store(const float2 (&acc)[CH], double2* part, int G, int c0, int nvec) {
-  __shared__ double red[kKB / 64][CH][2];
+  __shared__ double red[BS / 64][CH][2];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
   for (int jj = 0; jj < CH; ++jj) {
     if (c0 + jj < nvec) {
-      const double re = wave_sum((double)acc[jj].x);
-      const double im = wave_sum((double)acc[jj].y);
+      const float re = wave_sum_f(acc[jj].x);
+      const float im = wave_sum_f(acc[jj].y);
       if (lane == 0) { red[wave][jj][0] = re; red[wave][jj][1] = im; }
     }
   }
@@ -58,27 +98,42 @@ __device__ inline void block_store(const float2 (&acc)[CH], double2* part, int G
   if (threadIdx.x < CH && c0 + (int)threadIdx.x < nvec) {
     double re = 0, im = 0;
 #pragma unroll
-    for (int w = 0; w < kKB / 64; ++w) { re += red[w][threadIdx.x][0]; im += red[w][threadIdx.x][1]; }
-    part[(long)(c0 + threadIdx.x) * G + blockIdx.x] = make_double2(re, im);
+    for (int w = 0; w < BS / 64; ++w) { re += red[w][threadIdx.x][0]; im += red[w][threadIdx.x][1]; }
+    put2(&part[(long)(c0 + threadIdx.x) * G + blockIdx.x], re, im);
   }
   __syncthreads();
 }
 
-// out[j] = sum_g part[j * G + g] in a fixed order (last block only).
+// out[j] = sum_g part[j * G + g] in a fixed order (last block only).  Each
+// j gets a group of L lanes (the largest power of two <= 64 with
+// L * nvec <= BS); a lane
+// issues all of its loads before summing, then the group combines its lanes
+// with butterfly shuffles.
+template <int BS>
 __device__ inline void finish_partials(const double2* part, int G, int nvec, double2* out) {
-  const int sub = threadIdx.x & 7;
-  for (int j0 = 0; j0 < nvec; j0 += kKB / 8) {
-    const int j = j0 + (threadIdx.x >> 3);
+  int np2 = 1;
+  while (np2 < nvec) np2 <<= 1;
+  int L = BS / np2;
+  L = L > 64 ? 64 : (L < 1 ? 1 : L);
+  const int lanes_total = BS / L;  // j values handled per round
+  const int sub = threadIdx.x % L;
+  for (int j0 = 0; j0 < nvec; j0 += lanes_total) {
+    const int j = j0 + threadIdx.x / L;
     double re = 0, im = 0;
     if (j < nvec) {
-      for (int g = sub; g < G; g += 8) {
-        const double2 v = part[(long)j * G + g];
-        re += v.x;
-        im += v.y;
+      constexpr int U = 8;
+      for (int g0 = sub; g0 < G; g0 += U * L) {
+        double2 v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int g = g0 + u * L;
+          v[u] = g < G ? get2(&part[(long)j * G + g]) : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) { re += v[u].x; im += v[u].y; }
       }
     }
-#pragma unroll
-    for (int off = 1; off < 8; off <<= 1) {
+    for (int off = 1; off < L; off <<= 1) {
       re += __shfl_xor(re, off, 64);
       im += __shfl_xor(im, off, 64);
     }
@@ -86,9 +141,26 @@ __device__ inline void finish_partials(const double2* part, int G, int nvec, dou
   }
 }
 
+// sum of G doubles (agent-scope partials) by the first wave, fixed order.
+__device__ inline double sum_partials(const double* partn, int G) {
+  double s = 0.0;
+  if (threadIdx.x < 64) {
+    constexpr int U = 8;
+    for (int g0 = threadIdx.x; g0 < G; g0 += U * 64) {
+      double v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) v[u] = (g0 + u * 64 < G) ? get(&partn[g0 + u * 64]) : 0.0;
+#pragma unroll
+      for (int u = 0; u < U; ++u) s += v[u];
+    }
+    s = wave_sum(s);
+  }
+  return s;
+}
+
 // h[j] = sum_e conj(V_j[e]) w[e], j < nvec.
-template <int CH>
-__global__ void __launch_bounds__(kKB)
+template <int BS, int CH>
+__global__ void __launch_bounds__(BS)
 k_dots(const float2* __restrict__ V, long n, int nvec, const float2* __restrict__ w,
        double2* __restrict__ part, int G, double2* __restrict__ out, unsigned* ticket,
        const int* __restrict__ done) {
@@ -97,7 +169,7 @@ k_dots(const float2* __restrict__ V, long n, int nvec, const float2* __restrict_
     float2 acc[CH];
 #pragma unroll
     for (int jj = 0; jj < CH; ++jj) acc[jj] = make_float2(0.f, 0.f);
-    for (long e = (long)blockIdx.x * kKB + threadIdx.x; e < n; e += (long)G * kKB) {
+    for (long e = (long)blockIdx.x * BS + threadIdx.x; e < n; e += (long)G * BS) {
       const float2 wv = w[e];
 #pragma unroll
       for (int jj = 0; jj < CH; ++jj) {
@@ -108,18 +180,18 @@ k_dots(const float2* __restrict__ V, long n, int nvec, const float2* __restrict_
         }
       }
     }
-    block_store<CH>(acc, part, G, c0, nvec);
+    block_store<BS, CH>(acc, part, G, c0, nvec);
   }
   if (arrive_last(ticket)) {
-    finish_partials(part, G, nvec, out);
-    if (threadIdx.x == 0) *ticket = 0u;
+    finish_partials<BS>(part, G, nvec, out);
+    if (threadIdx.x == 0) reset_ticket(ticket);
   }
 }
 
 // w -= sum_j h_j V_j, then h2[j] = sum_e conj(V_j[e]) w[e] from the same
 // registers (one read of V for the update and the second projection).
-template <int MAXV>
-__global__ void __launch_bounds__(kKB)
+template <int BS, int MAXV>
+__global__ void __launch_bounds__(BS)
 k_upd_dots(const float2* __restrict__ V, long n, int nvec, float2* __restrict__ w,
            const double2* __restrict__ h, double2* __restrict__ part, int G,
            double2* __restrict__ out, unsigned* ticket, const int* __restrict__ done) {
@@ -133,7 +205,7 @@ k_upd_dots(const float2* __restrict__ V, long n, int nvec, float2* __restrict__ 
   float2 acc[MAXV];
 #pragma unroll
   for (int j = 0; j < MAXV; ++j) acc[j] = make_float2(0.f, 0.f);
-  for (long e = (long)blockIdx.x * kKB + threadIdx.x; e < n; e += (long)G * kKB) {
+  for (long e = (long)blockIdx.x * BS + threadIdx.x; e < n; e += (long)G * BS) {
     float2 v[MAXV];
 #pragma unroll
     for (int j = 0; j < MAXV; ++j) v[j] = j < nvec ? V[(long)j * n + e] : make_float2(0.f, 0.f);
@@ -150,19 +222,20 @@ k_upd_dots(const float2* __restrict__ V, long n, int nvec, float2* __restrict__ 
       acc[j].y += v[j].x * wv.y - v[j].y * wv.x;
     }
   }
-  block_store<MAXV>(acc, part, G, 0, nvec);
+  block_store<BS, MAXV>(acc, part, G, 0, nvec);
   if (arrive_last(ticket)) {
-    finish_partials(part, G, nvec, out);
-    if (threadIdx.x == 0) *ticket = 0u;
+    finish_partials<BS>(part, G, nvec, out);
+    if (threadIdx.x == 0) reset_ticket(ticket);
   }
 }
 
 // w -= sum_j h_j V_j (any nvec).
-__global__ void __launch_bounds__(kKB)
+template <int BS>
+__global__ void __launch_bounds__(BS)
 k_upd(const float2* __restrict__ V, long n, int nvec, float2* __restrict__ w,
       const double2* __restrict__ h, int G, const int* __restrict__ done) {
   if (*done) return;
-  for (long e = (long)blockIdx.x * kKB + threadIdx.x; e < n; e += (long)G * kKB) {
+  for (long e = (long)blockIdx.x * BS + threadIdx.x; e < n; e += (long)G * BS) {
     float2 wv = w[e];
     for (int j = 0; j < nvec; ++j) {
       const double2 hv = h[j];
@@ -180,13 +253,14 @@ k_upd(const float2* __restrict__ V, long n, int nvec, float2* __restrict__ w,
 // applies the previous rotations, builds rotation i and updates g, all in
 // FP64.  The rotation is the unitary [[c, s], [-conj(s), c]] with real c,
 // chosen so that it annihilates H[i+1, i].
-__global__ void __launch_bounds__(kKB)
+template <int BS>
+__global__ void __launch_bounds__(BS)
 k_upd_norm_givens(const float2* __restrict__ V, long n, int i, float2* __restrict__ w,
                   KrylovWs ws, DevState* st, double* hist, int m, const int* __restrict__ done) {
   if (*done) return;
   const int nvec = i + 1;
   double ss = 0.0;
-  for (long e = (long)blockIdx.x * kKB + threadIdx.x; e < n; e += (long)ws.G * kKB) {
+  for (long e = (long)blockIdx.x * BS + threadIdx.x; e < n; e += (long)ws.G * BS) {
     float2 wv = w[e];
     for (int j = 0; j < nvec; ++j) {
       const double2 hv = ws.h2[j];
@@ -198,36 +272,40 @@ k_upd_norm_givens(const float2* __restrict__ V, long n, int i, float2* __restric
     w[e] = wv;
     ss += (double)wv.x * wv.x + (double)wv.y * wv.y;
   }
-  __shared__ double red[kKB / 64];
+  __shared__ double red[BS / 64];
   ss = wave_sum(ss);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ss;
   __syncthreads();
   if (threadIdx.x == 0) {
     double s = 0;
-    for (int k = 0; k < kKB / 64; ++k) s += red[k];
-    ws.partn[blockIdx.x] = s;
+    for (int k = 0; k < BS / 64; ++k) s += red[k];
+    put(&ws.partn[blockIdx.x], s);
   }
   if (!arrive_last(&st->tickets[kTicketNorm])) return;
 
   __shared__ double2 col[kMaxRestart + 1];
+  __shared__ double2 lsn[kMaxRestart];
+  __shared__ double lcs[kMaxRestart];
   __shared__ double s_norm;
-  if (threadIdx.x < 64) {
-    double s = 0;
-    for (int g = threadIdx.x; g < ws.G; g += 64) s += ws.partn[g];
-    s = wave_sum(s);
-    if (threadIdx.x == 0) s_norm = sqrt(s);
+  {
+    const double sq = sum_partials(ws.partn, ws.G);
+    if (threadIdx.x == 0) s_norm = sqrt(sq);
   }
-  for (int j = threadIdx.x; j <= i; j += kKB) {
+  for (int j = threadIdx.x; j <= i; j += BS) {
     const double2 a = ws.h1[j], b = ws.h2[j];
     col[j] = make_double2(a.x + b.x, a.y + b.y);
+    if (j < i) {
+      lcs[j] = ws.cs[j];
+      lsn[j] = ws.sn[j];
+    }
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     const double hn = s_norm;
     col[i + 1] = make_double2(hn, 0.0);
     for (int j = 0; j < i; ++j) {
-      const double c = ws.cs[j];
-      const double2 s = ws.sn[j];
+      const double c = lcs[j];
+      const double2 s = lsn[j];
       const double2 a = col[j], b = col[j + 1];
       // [a; b] <- [c a + s b; -conj(s) a + c b]
       col[j] = make_double2(c * a.x + (s.x * b.x - s.y * b.y), c * a.y + (s.x * b.y + s.y * b.x));
@@ -265,8 +343,8 @@ k_upd_norm_givens(const float2* __restrict__ V, long n, int i, float2* __restric
     if (bad || !(hn > 0.0) || est <= st->tol_abs) st->done = 1;
   }
   __syncthreads();
-  for (int j = threadIdx.x; j <= i; j += kKB) ws.R[(long)j * m + i] = col[j];
-  if (threadIdx.x == 0) st->tickets[kTicketNorm] = 0u;
+  for (int j = threadIdx.x; j <= i; j += BS) ws.R[(long)j * m + i] = col[j];
+  if (threadIdx.x == 0) reset_ticket(&st->tickets[kTicketNorm]);
 }
 
 // ||r|| of a complex64 vector.  what = 0: start of a cycle (beta = ||r||,
@@ -287,13 +365,11 @@ k_norm_init(const float2* __restrict__ r, long n, double* partn, int G, DevState
   if (threadIdx.x == 0) {
     double s = 0;
     for (int k = 0; k < kKB / 64; ++k) s += red[k];
-    partn[blockIdx.x] = s;
+    put(&partn[blockIdx.x], s);
   }
   if (!arrive_last(&st->tickets[kTicketInit])) return;
-  if (threadIdx.x < 64) {
-    double s = 0;
-    for (int q = threadIdx.x; q < G; q += 64) s += partn[q];
-    s = wave_sum(s);
+  {
+    const double s = sum_partials(partn, G);
     if (threadIdx.x == 0) {
       const double nrm = sqrt(s);
       if (what == 1) {
@@ -311,7 +387,7 @@ k_norm_init(const float2* __restrict__ r, long n, double* partn, int G, DevState
       } else {
         st->norm_out = nrm;
       }
-      st->tickets[kTicketInit] = 0u;
+      reset_ticket(&st->tickets[kTicketInit]);
     }
   }
 }
@@ -331,16 +407,14 @@ k_norm64(const double2* __restrict__ v, long n, double* partn, int G, DevState* 
   if (threadIdx.x == 0) {
     double s = 0;
     for (int k = 0; k < kKB / 64; ++k) s += red[k];
-    partn[blockIdx.x] = s;
+    put(&partn[blockIdx.x], s);
   }
   if (!arrive_last(&st->tickets[kTicketMisc])) return;
-  if (threadIdx.x < 64) {
-    double s = 0;
-    for (int q = threadIdx.x; q < G; q += 64) s += partn[q];
-    s = wave_sum(s);
+  {
+    const double s = sum_partials(partn, G);
     if (threadIdx.x == 0) {
       st->norm_out = sqrt(s);
-      st->tickets[kTicketMisc] = 0u;
+      reset_ticket(&st->tickets[kTicketMisc]);
     }
   }
 }
@@ -434,26 +508,34 @@ int launch_norm64(hipStream_t st, const double2* v, long n, double* partn, int G
 }
 
 // CGS2 for step i: w = V_{i+1} against V_0..V_i, then ||w|| and rotation i.
-int launch_cgs2_step(hipStream_t st, const KrylovWs& ws, int i, DevState* s, double* hist) {
+// part: 0 first projection, 1 update + second projection, 2 update + norm +
+// Givens, -1 all three.
+int launch_cgs2_step(hipStream_t st, const KrylovWs& ws, int i, DevState* s, double* hist,
+                     int part) {
   const int nvec = i + 1;
   const long n = ws.n;
   float2* w = ws.V + (long)(i + 1) * n;
   const int* done = &s->done;
-  hipLaunchKernelGGL((k_dots<16>), dim3(ws.G), dim3(kKB), 0, st, ws.V, n, nvec, w, ws.part, ws.G,
-                     ws.h1, &s->tickets[kTicketDots1], done);
-  if (nvec <= 16) {
-    hipLaunchKernelGGL((k_upd_dots<16>), dim3(ws.G), dim3(kKB), 0, st, ws.V, n, nvec, w, ws.h1,
-                       ws.part, ws.G, ws.h2, &s->tickets[kTicketDots2], done);
-  } else if (nvec <= 32) {
-    hipLaunchKernelGGL((k_upd_dots<32>), dim3(ws.G), dim3(kKB), 0, st, ws.V, n, nvec, w, ws.h1,
-                       ws.part, ws.G, ws.h2, &s->tickets[kTicketDots2], done);
-  } else {
-    hipLaunchKernelGGL(k_upd, dim3(ws.G), dim3(kKB), 0, st, ws.V, n, nvec, w, ws.h1, ws.G, done);
-    hipLaunchKernelGGL((k_dots<16>), dim3(ws.G), dim3(kKB), 0, st, ws.V, n, nvec, w, ws.part,
-                       ws.G, ws.h2, &s->tickets[kTicketDots2], done);
+  if (part < 0 || part == 0)
+    hipLaunchKernelGGL((k_dots<kKB, 16>), dim3(ws.G), dim3(kKB), 0, st, ws.V, n, nvec, w, ws.part,
+                       ws.G, ws.h1, &s->tickets[kTicketDots1], done);
+  if (part < 0 || part == 1) {
+    if (nvec <= 16) {
+      hipLaunchKernelGGL((k_upd_dots<kKB, 16>), dim3(ws.G), dim3(kKB), 0, st, ws.V, n, nvec, w,
+                         ws.h1, ws.part, ws.G, ws.h2, &s->tickets[kTicketDots2], done);
+    } else if (nvec <= 32) {
+      hipLaunchKernelGGL((k_upd_dots<kKB / 2, 32>), dim3(ws.G), dim3(kKB / 2), 0, st, ws.V, n, nvec,
+                         w, ws.h1, ws.part, ws.G, ws.h2, &s->tickets[kTicketDots2], done);
+    } else {
+      hipLaunchKernelGGL((k_upd<kKB>), dim3(ws.G), dim3(kKB), 0, st, ws.V, n, nvec, w, ws.h1, ws.G,
+                         done);
+      hipLaunchKernelGGL((k_dots<kKB, 16>), dim3(ws.G), dim3(kKB), 0, st, ws.V, n, nvec, w,
+                         ws.part, ws.G, ws.h2, &s->tickets[kTicketDots2], done);
+    }
   }
-  hipLaunchKernelGGL(k_upd_norm_givens, dim3(ws.G), dim3(kKB), 0, st, ws.V, n, i, w, ws, s,
-                     hist, ws.m, done);
+  if (part < 0 || part == 2)
+    hipLaunchKernelGGL((k_upd_norm_givens<kKB>), dim3(ws.G), dim3(kKB), 0, st, ws.V, n, i, w, ws,
+                       s, hist, ws.m, done);
   CGM_HIP(hipGetLastError());
   return CGM_SUCCESS;
 }

@@ -47,6 +47,8 @@ k_pass1(const cplx<T>* __restrict__ x, cplx<T>* xw, const T* __restrict__ chi,
   }
   const bool live = row < N;
   const long rb = (long)row * N;
+  FftTwiddles<T, M> tws;
+  tws.load(tw, t);
   for (int e = t; e < M; e += NT) {
     cplx<T> u = mk(T(0), T(0));
     if (live && e < N) {
@@ -60,7 +62,7 @@ k_pass1(const cplx<T>* __restrict__ x, cplx<T>* xw, const T* __restrict__ chi,
     buf[e] = u;
   }
   __syncthreads();
-  lds_fft<T, M, -1>(buf, tw, t);
+  lds_fft<T, M, -1>(buf, tws, t);
   if (live) {
     cplx<T>* out = Tm + (long)row * M;
     for (int e = t; e < M; e += NT) out[e] = buf[e];
@@ -79,6 +81,8 @@ k_pass2(cplx<T>* __restrict__ Tm, const G* __restrict__ gq, int N, int cols_per_
   const int grp = xcd_tile(blockIdx.x, gridDim.x);
   const int c0 = grp * cols_per_wg;
   const int nthr = blockDim.x;
+  FftTwiddles<T, M> tws;
+  tws.load(tw, threadIdx.x % NT);
   // gather the column tile: consecutive threads read consecutive columns
   for (int idx = threadIdx.x; idx < N * cols_per_wg; idx += nthr) {
     const int r = idx / cols_per_wg, c = idx - r * cols_per_wg;
@@ -92,7 +96,7 @@ k_pass2(cplx<T>* __restrict__ Tm, const G* __restrict__ gq, int N, int cols_per_
   const int t = threadIdx.x % NT;
   const int cl = threadIdx.x / NT;
   cplx<T>* buf = smem + cl * M;
-  lds_fft<T, M, -1>(buf, tw, t);
+  lds_fft<T, M, -1>(buf, tws, t);
   const int col = c0 + cl;
   const int cq = col <= M - col ? col : M - col;
   const G* grow = gq + (long)cq * H;
@@ -102,7 +106,7 @@ k_pass2(cplx<T>* __restrict__ Tm, const G* __restrict__ gq, int N, int cols_per_
     buf[e] = cmul(buf[e], mk(T(g.x), T(g.y)));
   }
   __syncthreads();
-  lds_fft<T, M, +1>(buf, tw, t);
+  lds_fft<T, M, +1>(buf, tws, t);
   for (int idx = threadIdx.x; idx < N * cols_per_wg; idx += nthr) {
     const int r = idx / cols_per_wg, c = idx - r * cols_per_wg;
     Tm[(long)r * M + c0 + c] = smem[c * M + r];
@@ -123,10 +127,12 @@ k_pass3(const cplx<T>* __restrict__ Tm, const cplx<T>* __restrict__ x, const T* 
   const int row = blockIdx.x * rows_per_wg + rl;
   cplx<T>* buf = smem + rl * M;
   const bool live = row < N;
+  FftTwiddles<T, M> tws;
+  tws.load(tw, t);
   const cplx<T>* in = Tm + (long)row * M;
   for (int e = t; e < M; e += NT) buf[e] = live ? in[e] : mk(T(0), T(0));
   __syncthreads();
-  lds_fft<T, M, +1>(buf, tw, t);
+  lds_fft<T, M, +1>(buf, tws, t);
   if (!live) return;
   const long rb = (long)row * N;
   for (int e = t; e < N; e += NT) {
@@ -154,9 +160,11 @@ k_fft_rows(cplx<T>* __restrict__ data, int rows, int rows_per_wg, const cplx<T>*
   const int row = blockIdx.x * rows_per_wg + rl;
   cplx<T>* buf = smem + rl * M;
   const bool live = row < rows;
+  FftTwiddles<T, M> tws;
+  tws.load(tw, t);
   for (int e = t; e < M; e += NT) buf[e] = live ? data[(long)row * M + e] : mk(T(0), T(0));
   __syncthreads();
-  lds_fft<T, M, SIGN>(buf, tw, t);
+  lds_fft<T, M, SIGN>(buf, tws, t);
   if (live)
     for (int e = t; e < M; e += NT) data[(long)row * M + e] = buf[e];
 }
@@ -174,7 +182,9 @@ k_fft_cols(cplx<T>* __restrict__ data, int cols_per_wg, const cplx<T>* __restric
   }
   __syncthreads();
   const int t = threadIdx.x % NT, cl = threadIdx.x / NT;
-  lds_fft<T, M, SIGN>(smem + cl * M, tw, t);
+  FftTwiddles<T, M> tws;
+  tws.load(tw, t);
+  lds_fft<T, M, SIGN>(smem + cl * M, tws, t);
   for (int idx = threadIdx.x; idx < M * cols_per_wg; idx += blockDim.x) {
     const int r = idx / cols_per_wg, c = idx % cols_per_wg;
     data[(long)r * M + c0 + c] = smem[c * M + r];

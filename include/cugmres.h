@@ -232,6 +232,15 @@ int cgm_G_x_fft_matvec_T(const cgm_complex64* dev_gamma_array, const bool* dev_m
 int cgm_cuSolve_LES(cgm_complex64* dev_A, cgm_complex64* dev_B, int m, cgm_ctx_t ctx,
                     cgm_complex64* dev_work, int* dev_Ipiv, int* dev_info);
 
+/* ---- diagnostics ------------------------------------------------------------ */
+/* Average device time (ms, HIP events on the context stream, reps launches
+ * each) of the kernels of one Arnoldi step at basis size restart/2 + 1:
+ * ms[0..2] the three FFT passes of the matvec, ms[3] first CGS projection,
+ * ms[4] update + second projection, ms[5] update + norm + Givens.  Uses
+ * dev_x as every basis vector; overwrites the context's solver workspace. */
+int cgm_profile_step(cgm_ctx_t ctx, const cgm_complex64* dev_gamma, const float* dev_chi,
+                     const cgm_complex64* dev_x, int restart, int reps, float* ms);
+
 /* ---- memory and host helpers ---------------------------------------------- */
 int cgm_malloc(void** ptr, size_t bytes);
 int cgm_free(void* ptr);
