@@ -43,6 +43,9 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=15.0,
                    help="budget of the CPU-baseline sample")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--mode", choices=["bench", "step", "copy"], default="bench",
+                   help="bench: the benchmark; step: only the per-kernel profile of one Arnoldi "
+                        "step (for rocprofv3 --pmc); copy: 3 copies of 256 MiB (counter calibration)")
     return p.parse_args()
 
 
@@ -58,8 +61,23 @@ def main():
     from cugmres_amd import problems, solver
     from cugmres_amd._lib import lib
 
+    if a.mode == "copy":
+        src = torch.ones(32 * 1024 * 1024, dtype=torch.complex64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": src.numel() * 8, "copies": 3}))
+        return
     N = a.N
     ctx = solver.Context(N, 0)
+    if a.mode == "step":
+        chi, b, _ = problems.config_plain(N, seed=0)
+        prof = profile_kernels(ctx, ctx.spectrum_helmholtz(problems.KH),
+                               ctx.to_device(chi.reshape(-1), "f32"), ctx.to_device(b.reshape(-1)),
+                               a.restart)
+        print(json.dumps(prof))
+        return
     stream = torch.cuda.current_stream()
     g = ctx.spectrum_helmholtz(problems.KH)
     seeds = list(range(a.warmup + a.steps))
@@ -93,6 +111,9 @@ def main():
     # ---- roofline of the dominant kernel ---------------------------------
     prof = profile_kernels(ctx, g, inputs[0][0], inputs[0][1], a.restart)
 
+    # ---- the reference's own problem at the same N ------------------------
+    refp = reference_problem(ctx, a.restart) if N == 512 else None
+
     # ---- CPU baseline ------------------------------------------------------
     cpu = None
     if not a.no_cpu:
@@ -121,6 +142,7 @@ def main():
         "roofline": prof["roofline"],
         "kernels": prof["kernels"],
         "cpu_baseline": cpu,
+        "reference_problem": refp,
     }
     print(json.dumps(out), flush=True)
 
@@ -167,6 +189,37 @@ def profile_kernels(ctx, g, dchi, db, restart):
             "frac": a / HBM_PEAK_GBS, "traffic": None,
             "per_unit": f"{kernels[dom]['bytes']} B per launch at N={N}, M={M}, basis {nv}"}
     return {"roofline": roof, "kernels": kernels}
+
+
+def reference_problem(ctx, restart, reps=5):
+    """The reference's benchmark problem at N = 512 (eps = 2.25 disk of
+    radius N/6, k = 2*3.14/(N/6) per cell, x0 = b = plane wave along the row
+    index, B/CUDA_GMRES/GMRES_kernels.cuh:12-19,557-572), solved to a true
+    relative residual of 1e-6 with the same GMRES(restart).  The reference
+    published 44.38 ms on a V100 for its m = 37 solve (Results/V100_mean_times.npy)."""
+    import numpy as np
+    import torch
+    from cugmres_amd import solver
+    N = ctx.N
+    k = 2 * 3.14 / (N / 6)
+    g = ctx.spectrum_reference(k, 0.5, 0.0)
+    i, j = np.mgrid[0:N, 0:N]
+    mask = ((i - N / 3) ** 2 + (j - N / 2) ** 2 <= (N / 6) ** 2)
+    chi = ctx.to_device((solver.reference_chi(N) * mask).reshape(-1), "f32")
+    b = ctx.to_device(np.repeat(np.exp(-1j * k * np.arange(N))[:, None], N, axis=1).reshape(-1))
+    x = b.clone()
+    ctx.gmres(g, chi, b, x, restart=restart, tol=1e-6, max_cycles=20)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        x.copy_(b)
+        _, info = ctx.gmres(g, chi, b, x, restart=restart, tol=1e-6, max_cycles=20)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / reps
+    return {"problem": "reference eps=2.25 disk, N=512, x0=b, tol 1e-6", "restart": restart,
+            "iterations": info["iterations"], "converged": info["converged"],
+            "rel_residual": info["rel_residual"], "ms_per_solve": dt * 1e3, "solves_per_s": 1.0 / dt,
+            "V100_published_ms": 44.38, "speedup_vs_V100": 44.38e-3 / dt}
 
 
 def cpu_baseline(N, restart, budget_s, cycles_per_solve):

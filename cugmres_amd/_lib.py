@@ -61,6 +61,7 @@ SIGNATURES = {
     "cgm_destroy": (ctypes.c_int, [c_void_p]),
     "cgm_set_stream": (ctypes.c_int, [c_void_p, c_void_p]),
     "cgm_get_embedding": (ctypes.c_int, [c_void_p, ctypes.POINTER(ctypes.c_int)]),
+    "cgm_set_arnoldi_path": (ctypes.c_int, [c_void_p, ctypes.c_int]),
     "cgm_spectrum_size": (ctypes.c_int, [c_void_p, ctypes.POINTER(ctypes.c_size_t)]),
     "cgm_spectrum_from_gprep": (ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
     "cgm_spectrum_from_block": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
@@ -91,6 +92,7 @@ SIGNATURES = {
     "cgm_memcpy_d2h": (ctypes.c_int, [c_void_p, c_void_p, ctypes.c_size_t]),
     "cgm_synchronize": (ctypes.c_int, [c_void_p]),
     "cgm_bessel01": (None, [ctypes.c_double] + [c_double_p] * 4),
+    "cgm_green_values": (ctypes.c_int, [c_void_p, ctypes.c_size_t, ctypes.c_double, c_void_p]),
     "cgm_profile_step": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, ctypes.c_int,
                                         c_float_p]),
 }

@@ -54,6 +54,22 @@ extern "C" void cgm_bessel01(double x, double* j0, double* j1, double* y0, doubl
   if (y1) *y1 = d;
 }
 
+extern "C" int cgm_green_values(const double* r, size_t n, double k, cgm_complex128* out) {
+  CGM_CHECK((r && out) || n == 0, CGM_ERROR_INVALID_VALUE, "null argument");
+  for (size_t q = 0; q < n; ++q) {
+    const double x = k * r[q];
+    if (!(x > 0.0) || !isfinite(x)) {
+      out[q].re = out[q].im = NAN;  // H0 is singular at 0
+      continue;
+    }
+    double j0, j1, y0, y1;
+    bessel01(x, j0, j1, y0, y1);
+    out[q].re = -0.25 * y0;  // (i/4)(J0 + i Y0)
+    out[q].im = 0.25 * j0;
+  }
+  return CGM_SUCCESS;
+}
+
 // ---------------------------------------------------------------------------
 // context
 // ---------------------------------------------------------------------------
@@ -119,6 +135,12 @@ extern "C" int cgm_create(cgm_ctx_t* out, int N, int device) {
 extern "C" int cgm_set_stream(cgm_ctx_t c, void* stream) {
   CGM_CHECK(c, CGM_ERROR_INVALID_VALUE, "null context");
   c->stream = (hipStream_t)stream;
+  return CGM_SUCCESS;
+}
+
+extern "C" int cgm_set_arnoldi_path(cgm_ctx_t c, int path) {
+  CGM_CHECK(c && (path == 0 || path == 1), CGM_ERROR_INVALID_VALUE, "path must be 0 or 1");
+  c->path = path;
   return CGM_SUCCESS;
 }
 
@@ -283,6 +305,7 @@ static int gmres_core(cgm_context* c, const float2* gq, const float* chi, const 
   KrylovWs& ws = c->ws;
   const int fwd = p->adjoint ? kModeAdjoint : kModeForward;
   const int resmode = p->adjoint ? kModeResidualAdj : kModeResidual;
+  const bool fused = c->path != 1 && fused_supported(c->N, m);
   // ||b|| and the absolute tolerance
   CGM_HIP(hipMemsetAsync(&c->st->status, 0, sizeof(int), c->stream));
   if ((rc = launch_norm_init(c->stream, ws, b, c->st, 1, p->tol))) return rc;
@@ -315,13 +338,36 @@ static int gmres_core(cgm_context* c, const float2* gq, const float* chi, const 
     if (h.bnorm == 0.0 || h.beta <= p->tol * h.bnorm) { res->converged = 1; break; }
     if (cycles >= p->max_cycles) break;
     // one restart cycle, enqueued without host synchronisation
-    for (int i = 0; i < m; ++i) {
-      float2* vi = ws.V + (long)i * n;
-      const double* nrm = (i == 0) ? &c->st->beta : &c->st->hn;
-      rc = matvec32(c, gq, chi, vi, nullptr, vi + n, fwd, nrm, vi, &c->st->done);
+    if (fused) {
+      // r = b - A x is in V_0; u_0 = r, first matvec's passes 1-2
+      float2* U0 = ws.U;
+      CGM_HIP(hipMemcpyAsync(U0, ws.V, sizeof(float2) * n, hipMemcpyDeviceToDevice, c->stream));
+      rc = launch_pass1<float>(c->plan, c->stream, U0, nullptr, chi, nullptr, fwd, c->T32, c->tw32, &c->st->done);
+      if (!rc) rc = launch_pass2<float, float2>(c->plan, c->stream, c->T32, gq, c->tw32, &c->st->done);
       if (rc) return rc;
-      rc = launch_cgs2_step(c->stream, ws, i, c->st, hist ? hist + 1 + total : nullptr);
-      if (rc) return rc;
+      FusedLaunch L{};
+      L.N = c->N; L.m = m; L.ld = ws.m; L.mode = fwd; L.chi = chi; L.V = ws.V; L.Z = ws.Z;
+      L.T = c->T32; L.tw = c->tw32; L.part = ws.part; L.H = ws.H; L.Rm = ws.R; L.cs = ws.cs;
+      L.sn = ws.sn; L.g = ws.g; L.h1p = ws.h1p; L.a1 = ws.a1; L.a2 = ws.a2; L.scal = ws.scal;
+      L.st = c->st; L.hist = hist ? hist + total : nullptr;
+      for (int j = 0; j <= m; ++j) {
+        L.j = j;
+        L.Uj = ws.U + (long)(j & 1) * n;
+        L.Unext = ws.U + (long)((j + 1) & 1) * n;
+        if ((rc = launch_fused(c->stream, 0, L))) return rc;
+        if (j == m) break;
+        if ((rc = launch_fused(c->stream, 1, L))) return rc;
+        if ((rc = launch_pass2<float, float2>(c->plan, c->stream, c->T32, gq, c->tw32, &c->st->done))) return rc;
+      }
+    } else {
+      for (int i = 0; i < m; ++i) {
+        float2* vi = ws.V + (long)i * n;
+        const double* nrm = (i == 0) ? &c->st->beta : &c->st->hn;
+        rc = matvec32(c, gq, chi, vi, nullptr, vi + n, fwd, nrm, vi, &c->st->done);
+        if (rc) return rc;
+        rc = launch_cgs2_step(c->stream, ws, i, c->st, hist ? hist + 1 + total : nullptr);
+        if (rc) return rc;
+      }
     }
     if ((rc = launch_backsub_update(c->stream, ws, c->st, x))) return rc;
     if ((rc = read_state(c))) return rc;

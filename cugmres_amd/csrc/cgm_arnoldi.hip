@@ -1,0 +1,461 @@
+// Fused Arnoldi steps with lagged CGS2 (delayed reorthogonalisation) for
+// power-of-two grids 64 <= N <= 1024 and restart <= 64.
+//
+// CGS2 (two classical Gram-Schmidt passes per step) replaces the reference's
+// modified Gram-Schmidt (B/CUDA_GMRES/GMRES.cuh:289-303).  Here the second
+// pass of step j-1 is delayed and merged into the single global reduction of
+// step j (the DCGS2 scheme of Swirydowicz, Langou, Ananthan, Yang and Thomas,
+// "Low synchronization Gram-Schmidt and generalized minimal residual
+// algorithms", Numer. Linear Algebra Appl. 28 (2021) e2343).  With u_j the
+// once-orthogonalised, unnormalised vector of step j and Q_j = [q_0..q_{j-1}]:
+//
+//   z = A u_j                                          (FFT matvec)
+//   s = Q_j^H u_j,  t = Q_j^H z,  alpha = u_j^H u_j,  gamma = u_j^H z
+//                                                      (one pass over Q_j)
+//   nu = sqrt(alpha - |s|^2),  q_j = (u_j - Q_j s)/nu   (second pass of u_j)
+//   H[0:j, j-1] += s,  H[j, j-1] = nu                  (column j-1 complete)
+//   p = H[0:j+1, 0:j] s                                 (A Q_j s = Q_{j+1} p)
+//   h[0:j] = (t - p[0:j])/nu,  h[j] = ((gamma - s^H t)/nu - p[j])/nu
+//                                                      (first pass of A q_j)
+//   u_{j+1} = z/nu - (d/nu) u_j - Q_j (c - s d/nu),
+//        c = p[0:j]/nu + h[0:j],  d = p[j]/nu + h[j]   (one pass over Q_j)
+//
+// Each step is three kernels over the N x M scratch T of the matvec:
+//   A(j): pass 3 of z = A u_j (inverse row FFTs) + all inner products; the
+//         last block completes column j-1, applies the FP64 Givens rotations
+//         and prepares the coefficients of B(j).
+//   B(j): q_j and u_{j+1} in one pass over Q_j + pass 1 of A u_{j+1}
+//         (row FFTs of chi .* u_{j+1}).
+//   pass 2 (column FFTs, spectrum multiply), shared with the plain matvec.
+// A block owns R whole rows (R*N = 1024*EPD elements) so the row FFTs and the
+// element-wise Krylov work run in the same kernel.
+#include <math.h>
+
+#include "cgm_fft.h"
+#include "cgm_reduce.h"
+
+namespace cgm {
+
+constexpr int kFBS = 1024;
+constexpr int kCH = 8;  // basis vectors per accumulation chunk
+
+struct FusedArgs {
+  int N, R, m, ld, j, mode;  // m: steps of the cycle; ld: leading dimension of H, Rm
+  int G;                 // blocks
+  long n;
+  float inv_mm;          // 1 / M^2
+  const float* chi;
+  float2* V;             // q_0 .. q_m
+  const float2* Uj;      // u_j
+  float2* Unext;         // u_{j+1}
+  float2* Z;             // z = A u_j
+  float2* T;             // N x M FFT scratch
+  const float2* tw;      // twiddles
+  double2* part;         // partials: s [0, m+1), t [m+1, 2m+2), (alpha, gamma) [2m+2, 2m+4)
+  double2* H;            // (m+1) x m Hessenberg, row-major, ld m
+  double2* Rm;           // m x m rotated factor, row-major, ld m
+  double* cs;
+  double2* sn;
+  double2* g;
+  double2* h1p;          // first-pass coefficients of the previous step (m+1)
+  double2* a1;           // coefficients of q_j:      Q_j s
+  double2* a2;           // coefficients of u_{j+1}:  Q_j (c - s d/nu)
+  double2* scal;         // [0] = (1/nu, 0), [1] = d/nu
+  DevState* st;
+  double* hist;          // estimates, hist[j] after j steps of this cycle (may be null)
+};
+
+__device__ inline float2 cmulf(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// conj(a) * b
+__device__ inline float2 cjmulf(float2 a, float2 b) {
+  return make_float2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+__device__ inline double2 dmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ inline double2 dcjmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+__device__ inline double2 dsub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ inline double2 dadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ inline double2 dscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+
+template <int M, int EPD>
+__global__ void __launch_bounds__(kFBS) k_arn_a(FusedArgs a) {
+  if (a.st->done) return;
+  constexpr int NT = FftShape<M>::NT;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  float2* smem = reinterpret_cast<float2*>(smem_raw);
+  const int N = a.N, R = a.R, j = a.j;
+  const long n = a.n;
+  const int r0 = blockIdx.x * R;
+  const int tid = threadIdx.x;
+  // ---- pass 3: inverse row FFTs --------------------------------------------
+  const bool fa = tid < R * NT;
+  const int t = tid % NT, rl = fa ? tid / NT : 0;
+  FftTwiddles<float, M> tws;
+  if (fa) tws.load(a.tw, t);
+  float2* buf = smem + rl * M;
+  if (fa) {
+    const int row = r0 + rl;
+    const float2* in = a.T + (long)row * M;
+    for (int e = t; e < M; e += NT) buf[e] = row < N ? in[e] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  lds_fft<float, M, +1>(buf, tws, t, fa);
+  // ---- z = u - G(.) ----------------------------------------------------------
+  float2 uu[EPD], zz[EPD];
+  long idx[EPD];
+  bool live[EPD];
+#pragma unroll
+  for (int k = 0; k < EPD; ++k) {
+    const int e = tid + k * kFBS;
+    const int r = e / N, c = e - r * N;
+    const int row = r0 + r;
+    live[k] = row < N;
+    idx[k] = (long)row * N + c;
+    uu[k] = make_float2(0.f, 0.f);
+    zz[k] = make_float2(0.f, 0.f);
+    if (live[k]) {
+      const float2 u = a.Uj[idx[k]];
+      float2 tv = smem[r * M + c];
+      tv.x *= a.inv_mm;
+      tv.y *= a.inv_mm;
+      if (a.mode == kModeAdjoint) {
+        const float ch = a.chi[idx[k]];
+        tv.x *= ch;
+        tv.y *= ch;
+      }
+      const float2 z = make_float2(u.x - tv.x, u.y - tv.y);
+      a.Z[idx[k]] = z;
+      uu[k] = u;
+      zz[k] = z;
+    }
+  }
+  // ---- inner products --------------------------------------------------------
+  const int G = a.G;
+  double2* part_s = a.part;
+  double2* part_t = a.part + (long)(a.ld + 1) * G;
+  double2* part_ag = a.part + (long)(2 * a.ld + 2) * G;
+  for (int c0 = 0; c0 < j; c0 += kCH) {
+    float2 acs[kCH], act[kCH];
+#pragma unroll
+    for (int q = 0; q < kCH; ++q) acs[q] = act[q] = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int k = 0; k < EPD; ++k) {
+      if (!live[k]) continue;
+#pragma unroll
+      for (int q = 0; q < kCH; ++q) {
+        if (c0 + q < j) {
+          const float2 v = a.V[(long)(c0 + q) * n + idx[k]];
+          const float2 ps = cjmulf(v, uu[k]), pt = cjmulf(v, zz[k]);
+          acs[q].x += ps.x; acs[q].y += ps.y;
+          act[q].x += pt.x; act[q].y += pt.y;
+        }
+      }
+    }
+    block_store<kFBS, kCH>(acs, part_s, G, c0, j);
+    block_store<kFBS, kCH>(act, part_t, G, c0, j);
+  }
+  {
+    float2 ag[2] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
+#pragma unroll
+    for (int k = 0; k < EPD; ++k) {
+      ag[0].x += uu[k].x * uu[k].x + uu[k].y * uu[k].y;
+      const float2 pg = cjmulf(uu[k], zz[k]);
+      ag[1].x += pg.x;
+      ag[1].y += pg.y;
+    }
+    block_store<kFBS, 2>(ag, part_ag, G, 0, 2);
+  }
+  if (!arrive_last(&a.st->tickets[kTicketDots1])) return;
+
+  // ---- last block: FP64 small math ------------------------------------------
+  constexpr int LM = 65;  // max restart + 1
+  __shared__ double2 ls[LM], lt[LM], lp[LM + 1], lcol[LM + 1], lsn[LM];
+  __shared__ double lcs[LM];
+  __shared__ double2 lag[2];
+  __shared__ double s_nu;
+  __shared__ int s_stop;
+  DevState* st = a.st;
+  finish_partials<kFBS>(part_s, G, j, ls);
+  finish_partials<kFBS>(part_t, G, j, lt);
+  finish_partials<kFBS>(part_ag, G, 2, lag);
+  __syncthreads();
+  if (tid == 0) {
+    double ss = 0.0;
+    for (int i = 0; i < j; ++i) ss += ls[i].x * ls[i].x + ls[i].y * ls[i].y;
+    const double nu2 = lag[0].x - ss;
+    const double nu = nu2 > 0.0 ? sqrt(nu2) : 0.0;
+    s_nu = nu;
+    int stop = 0;
+    const bool bad = !isfinite(lag[0].x) || !isfinite(lag[1].x) || !isfinite(lag[1].y) || !isfinite(nu);
+    if (bad) {
+      st->status = CGM_ERROR_NONFINITE;
+      stop = 1;
+    }
+    if (j == 0) {
+      st->beta = nu;
+      a.g[0] = make_double2(nu, 0.0);
+      st->k = 0;
+      if (!bad && !(nu > st->tol_abs)) stop = 1;
+    }
+    s_stop = stop;
+  }
+  __syncthreads();
+  const double nu = s_nu;
+  if (j > 0 && !s_stop) {
+    // complete column j-1 of H and rotate it
+    for (int i = tid; i <= j; i += kFBS) {
+      double2 hv;
+      if (i < j) hv = dadd(a.h1p[i], ls[i]);
+      else hv = make_double2(nu, 0.0);
+      lcol[i] = hv;
+      a.H[(long)i * a.ld + (j - 1)] = hv;
+      if (i < j - 1) {
+        lcs[i] = a.cs[i];
+        lsn[i] = a.sn[i];
+      }
+    }
+    __syncthreads();
+    if (tid == 0) {
+      // rotated copy of the column in lp (lcol keeps the original for p below)
+      for (int i = 0; i <= j; ++i) lp[i] = lcol[i];
+      for (int i = 0; i < j - 1; ++i) {
+        const double c = lcs[i];
+        const double2 s = lsn[i];
+        const double2 x0 = lp[i], x1 = lp[i + 1];
+        lp[i] = make_double2(c * x0.x + (s.x * x1.x - s.y * x1.y), c * x0.y + (s.x * x1.y + s.y * x1.x));
+        lp[i + 1] = make_double2(-(s.x * x0.x + s.y * x0.y) + c * x1.x, -(s.x * x0.y - s.y * x0.x) + c * x1.y);
+      }
+      const int i = j - 1;
+      const double2 x0 = lp[i];
+      const double aa = hypot(x0.x, x0.y);
+      const double d = hypot(aa, nu);
+      double c;
+      double2 s, r;
+      if (aa == 0.0) {
+        c = 0.0;
+        s = make_double2(1.0, 0.0);
+        r = make_double2(nu, 0.0);
+      } else {
+        c = aa / d;
+        s = make_double2(x0.x / aa * nu / d, x0.y / aa * nu / d);
+        r = make_double2(x0.x / aa * d, x0.y / aa * d);
+      }
+      lp[i] = r;
+      a.cs[i] = c;
+      a.sn[i] = s;
+      const double2 gi = a.g[i];
+      const double2 gn = make_double2(-(s.x * gi.x + s.y * gi.y), -(s.x * gi.y - s.y * gi.x));
+      a.g[i + 1] = gn;
+      a.g[i] = make_double2(c * gi.x, c * gi.y);
+      const double est = hypot(gn.x, gn.y);
+      st->res_est = est;
+      st->hn = nu;
+      st->k = j;
+      if (a.hist) a.hist[j] = est;
+      if (!isfinite(est)) st->status = CGM_ERROR_NONFINITE;
+      if (!isfinite(est) || est <= st->tol_abs || !(nu > 0.0) || j == a.m) s_stop = 1;
+    }
+    __syncthreads();
+    for (int i = tid; i < j; i += kFBS) a.Rm[(long)i * a.ld + (j - 1)] = lp[i];
+  }
+  __syncthreads();
+  if (s_stop || j == a.m) {
+    if (tid == 0) {
+      st->done = 1;
+      reset_ticket(&st->tickets[kTicketDots1]);
+    }
+    return;
+  }
+  // p = H[0:j+1, 0:j] s: one wave per row, lane c multiplies column c
+  {
+    const int wave = tid >> 6, lane = tid & 63;
+    for (int r = wave; r <= j; r += kFBS / 64) {
+      double2 v = make_double2(0.0, 0.0);
+      if (lane < j && lane >= r - 1) {
+        const double2 h = (lane == j - 1) ? lcol[r] : a.H[(long)r * a.ld + lane];
+        v = dmul(h, ls[lane]);
+      }
+      v.x = wave_sum(v.x);
+      v.y = wave_sum(v.y);
+      if (lane == 0) lp[r] = v;
+    }
+  }
+  __syncthreads();
+  // first-pass coefficients of A q_j and the update coefficients
+  __shared__ double2 s_dd;
+  if (tid == 0) {
+    double2 st_ = make_double2(0.0, 0.0);
+    for (int i = 0; i < j; ++i) st_ = dadd(st_, dcjmul(ls[i], lt[i]));
+    const double2 qz = dscale(dsub(lag[1], st_), 1.0 / nu);
+    const double2 hj = dscale(dsub(qz, lp[j]), 1.0 / nu);
+    a.h1p[j] = hj;
+    const double2 dd = dadd(dscale(lp[j], 1.0 / nu), hj);
+    s_dd = dd;
+    a.scal[0] = make_double2(1.0 / nu, 0.0);
+    a.scal[1] = dscale(dd, 1.0 / nu);
+  }
+  __syncthreads();
+  const double2 ddnu = dscale(s_dd, 1.0 / nu);
+  for (int i = tid; i < j; i += kFBS) {
+    const double2 hi = dscale(dsub(lt[i], lp[i]), 1.0 / nu);
+    a.h1p[i] = hi;
+    const double2 ci = dadd(dscale(lp[i], 1.0 / nu), hi);
+    a.a1[i] = ls[i];
+    a.a2[i] = dsub(ci, dmul(ls[i], ddnu));
+  }
+  if (tid == 0) reset_ticket(&st->tickets[kTicketDots1]);
+}
+
+template <int M, int EPD>
+__global__ void __launch_bounds__(kFBS) k_arn_b(FusedArgs a) {
+  if (a.st->done) return;
+  constexpr int NT = FftShape<M>::NT;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  float2* smem = reinterpret_cast<float2*>(smem_raw);
+  const int N = a.N, R = a.R, j = a.j;
+  const long n = a.n;
+  const int r0 = blockIdx.x * R;
+  const int tid = threadIdx.x;
+  const bool fa = tid < R * NT;
+  const int t = tid % NT, rl = fa ? tid / NT : 0;
+  FftTwiddles<float, M> tws;
+  if (fa) tws.load(a.tw, t);
+  const double2 sc0 = a.scal[0], sc1 = a.scal[1];
+  const float invnu = (float)sc0.x;
+  const float2 ddnu = make_float2((float)sc1.x, (float)sc1.y);
+#pragma unroll
+  for (int k = 0; k < EPD; ++k) {
+    const int e = tid + k * kFBS;
+    const int r = e / N, c = e - r * N;
+    const int row = r0 + r;
+    float2 w = make_float2(0.f, 0.f);
+    if (row < N) {
+      const long id = (long)row * N + c;
+      const float2 u = a.Uj[id], z = a.Z[id];
+      float2 qa = make_float2(0.f, 0.f), qb = make_float2(0.f, 0.f);
+      int i = 0;
+      for (; i + 4 <= j; i += 4) {
+        float2 v[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[q] = a.V[(long)(i + q) * n + id];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const double2 c1 = a.a1[i + q], c2 = a.a2[i + q];
+          const float2 f1 = make_float2((float)c1.x, (float)c1.y), f2 = make_float2((float)c2.x, (float)c2.y);
+          const float2 p1 = cmulf(f1, v[q]), p2 = cmulf(f2, v[q]);
+          qa.x += p1.x; qa.y += p1.y;
+          qb.x += p2.x; qb.y += p2.y;
+        }
+      }
+      for (; i < j; ++i) {
+        const float2 v = a.V[(long)i * n + id];
+        const double2 c1 = a.a1[i], c2 = a.a2[i];
+        const float2 p1 = cmulf(make_float2((float)c1.x, (float)c1.y), v);
+        const float2 p2 = cmulf(make_float2((float)c2.x, (float)c2.y), v);
+        qa.x += p1.x; qa.y += p1.y;
+        qb.x += p2.x; qb.y += p2.y;
+      }
+      const float2 q = make_float2((u.x - qa.x) * invnu, (u.y - qa.y) * invnu);
+      a.V[(long)j * n + id] = q;
+      const float2 du = cmulf(ddnu, u);
+      const float2 un = make_float2(z.x * invnu - du.x - qb.x, z.y * invnu - du.y - qb.y);
+      a.Unext[id] = un;
+      w = un;
+      if (a.mode == kModeForward) {
+        const float ch = a.chi[id];
+        w.x *= ch;
+        w.y *= ch;
+      }
+    }
+    smem[r * M + c] = w;
+  }
+  for (int e = tid; e < R * (M - N); e += kFBS) {
+    const int r = e / (M - N), c = N + e % (M - N);
+    smem[r * M + c] = make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  float2* buf = smem + rl * M;
+  lds_fft<float, M, -1>(buf, tws, t, fa);
+  if (fa) {
+    const int row = r0 + rl;
+    if (row < N) {
+      float2* out = a.T + (long)row * M;
+      for (int e = t; e < M; e += NT) out[e] = buf[e];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+bool fused_supported(int N, int restart) {
+  const bool pow2 = N >= 64 && N <= 1024 && (N & (N - 1)) == 0;
+  return pow2 && restart >= 1 && restart <= 64;
+}
+
+static int fused_epd(int N) { return N >= 1024 ? 2 : 1; }
+
+int fused_blocks(int N) {
+  const int R = kFBS * fused_epd(N) / N;
+  return (N + R - 1) / R;
+}
+
+// One launch of A(j) (which = 0) or B(j) (which = 1).
+int launch_fused(hipStream_t st, int which, const FusedLaunch& L) {
+  FusedArgs a;
+  a.N = L.N;
+  const int M = 2 * L.N;
+  const int epd = fused_epd(L.N);
+  a.R = kFBS * epd / L.N;
+  a.m = L.m;
+  a.ld = L.ld;
+  a.j = L.j;
+  a.mode = L.mode;
+  a.G = fused_blocks(L.N);
+  a.n = (long)L.N * L.N;
+  a.inv_mm = (float)(1.0 / ((double)M * (double)M));
+  a.chi = L.chi;
+  a.V = L.V;
+  a.Uj = L.Uj;
+  a.Unext = L.Unext;
+  a.Z = L.Z;
+  a.T = L.T;
+  a.tw = L.tw;
+  a.part = L.part;
+  a.H = L.H;
+  a.Rm = L.Rm;
+  a.cs = L.cs;
+  a.sn = L.sn;
+  a.g = L.g;
+  a.h1p = L.h1p;
+  a.a1 = L.a1;
+  a.a2 = L.a2;
+  a.scal = L.scal;
+  a.st = L.st;
+  a.hist = L.hist;
+  const size_t lds = (size_t)a.R * M * sizeof(float2);
+  switch (L.N) {
+#define CGM_FUSED_CASE(NN, EPD)                                                                 \
+  case NN:                                                                                      \
+    if (which == 0)                                                                             \
+      hipLaunchKernelGGL((k_arn_a<2 * NN, EPD>), dim3(a.G), dim3(kFBS), lds, st, a);            \
+    else                                                                                        \
+      hipLaunchKernelGGL((k_arn_b<2 * NN, EPD>), dim3(a.G), dim3(kFBS), lds, st, a);            \
+    break;
+    CGM_FUSED_CASE(64, 1)
+    CGM_FUSED_CASE(128, 1)
+    CGM_FUSED_CASE(256, 1)
+    CGM_FUSED_CASE(512, 1)
+    CGM_FUSED_CASE(1024, 2)
+#undef CGM_FUSED_CASE
+    default:
+      return CGM_ERROR_UNSUPPORTED;
+  }
+  CGM_HIP(hipGetLastError());
+  return CGM_SUCCESS;
+}
+
+}  // namespace cgm

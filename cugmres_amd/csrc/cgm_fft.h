@@ -156,11 +156,16 @@ __device__ inline void fft_load_tw(cplx<T>* reg, const cplx<T>* __restrict__ tw,
 
 // One Stockham stage for the butterflies owned by thread t.
 template <typename T, int M, int SIGN, int R, int NS, int OFF>
-__device__ inline void fft_stage(cplx<T>* buf, const cplx<T>* reg, int t) {
+__device__ inline void fft_stage(cplx<T>* buf, const cplx<T>* reg, int t, bool active = true) {
   constexpr int EPT = FftShape<M>::EPT;
   constexpr int NT = FftShape<M>::NT;
   constexpr int NB = EPT / R;        // butterflies per thread
   constexpr int MR = M / R;
+  if (!active) {  // threads outside the transform still take part in the barriers
+    __syncthreads();
+    __syncthreads();
+    return;
+  }
   cplx<T> v[NB][R];
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
@@ -190,11 +195,11 @@ __device__ inline void fft_stage(cplx<T>* buf, const cplx<T>* reg, int t) {
 }
 
 template <typename T, int M, int SIGN, int S, int NS>
-__device__ inline void fft_stages(cplx<T>* buf, const cplx<T>* reg, int t) {
+__device__ inline void fft_stages(cplx<T>* buf, const cplx<T>* reg, int t, bool active) {
   if constexpr (S < fft_num_stages<M>()) {
     constexpr int R = fft_radix<M>(S);
-    fft_stage<T, M, SIGN, R, NS, fft_tw_offset<M>(S)>(buf, reg, t);
-    fft_stages<T, M, SIGN, S + 1, NS * R>(buf, reg, t);
+    fft_stage<T, M, SIGN, R, NS, fft_tw_offset<M>(S)>(buf, reg, t, active);
+    fft_stages<T, M, SIGN, S + 1, NS * R>(buf, reg, t, active);
   }
 }
 
@@ -211,8 +216,8 @@ template <typename T, int M> struct FftTwiddles {
 // must have synchronised the work-group after filling buf; on return buf
 // holds the result and the group is synchronised.
 template <typename T, int M, int SIGN>
-__device__ inline void lds_fft(cplx<T>* buf, const FftTwiddles<T, M>& tws, int t) {
-  fft_stages<T, M, SIGN, 0, 1>(buf, tws.reg, t);
+__device__ inline void lds_fft(cplx<T>* buf, const FftTwiddles<T, M>& tws, int t, bool active = true) {
+  fft_stages<T, M, SIGN, 0, 1>(buf, tws.reg, t, active);
 }
 
 }  // namespace cgm

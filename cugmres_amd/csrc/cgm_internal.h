@@ -69,9 +69,43 @@ struct KrylovWs {
   double2* g = nullptr;   // rotated right-hand side (m+1)
   double2* R = nullptr;   // upper triangular factor, row-major m x m
   double2* y = nullptr;   // least-squares solution (m)
-  double2* part = nullptr;  // reduction partials (m+1) * G * 2
+  double2* part = nullptr;  // reduction partials (2m+4) * max(G, fused blocks)
   double* partn = nullptr;  // norm partials G
+  // lagged-CGS2 (fused) path
+  double2* H = nullptr;     // (m+1) x m Hessenberg, row-major
+  double2* h1p = nullptr;   // (m+1)
+  double2* a1 = nullptr;    // (m+1)
+  double2* a2 = nullptr;    // (m+1)
+  double2* scal = nullptr;  // 4
+  float2* U = nullptr;      // 2 * n (u ping-pong)
+  float2* Z = nullptr;      // n
 };
+
+struct FusedLaunch {
+  int N, m, ld, j, mode;  // m: steps of this cycle; ld: leading dimension of H and R
+  const float* chi;
+  float2* V;
+  const float2* Uj;
+  float2* Unext;
+  float2* Z;
+  float2* T;
+  const float2* tw;
+  double2* part;
+  double2* H;
+  double2* Rm;
+  double* cs;
+  double2* sn;
+  double2* g;
+  double2* h1p;
+  double2* a1;
+  double2* a2;
+  double2* scal;
+  DevState* st;
+  double* hist;
+};
+bool fused_supported(int N, int restart);
+int fused_blocks(int N);
+int launch_fused(hipStream_t st, int which, const FusedLaunch& L);
 
 // Krylov launches (cgm_krylov.hip)
 int krylov_alloc(KrylovWs& ws, long n, int m, hipStream_t st);
@@ -94,6 +128,7 @@ struct cgm_context {
   float2* T32 = nullptr;   // N x M scratch of one complex64 matvec
   double2* T64 = nullptr;  // N x M scratch of one complex128 matvec (lazy)
   float mask_chi = 0.f;
+  int path = 0;  // Arnoldi kernels: 0 = fused lagged CGS2 where supported, 1 = two-pass CGS2
   cgm::KrylovWs ws;
   cgm::DevState* st = nullptr;   // device
   cgm::DevState* hst = nullptr;  // pinned host mirror

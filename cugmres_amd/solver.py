@@ -53,6 +53,10 @@ class Context:
     def handle(self):
         return self._ptr
 
+    def set_arnoldi_path(self, path):
+        """0: fused lagged-CGS2 steps (default), 1: separate two-pass CGS2 kernels."""
+        check(lib().cgm_set_arnoldi_path(self._ptr, int(path)), "cgm_set_arnoldi_path")
+
     # ---- tensors --------------------------------------------------------
     def empty(self, dtype="c64"):
         t = self.torch

@@ -85,6 +85,11 @@ int cgm_destroy(cgm_ctx_t ctx);
  * as void*; NULL = the default stream). */
 int cgm_set_stream(cgm_ctx_t ctx, void* stream);
 int cgm_get_embedding(cgm_ctx_t ctx, int* M);
+/* Arnoldi kernels of cgm_gmres: 0 (default) = fused steps with lagged CGS2
+ * (one global reduction and two basis passes per step; power-of-two N in
+ * [64, 1024], restart <= 64, other cases fall back to 1); 1 = separate
+ * kernels with two-pass CGS2 (three basis passes per step). */
+int cgm_set_arnoldi_path(cgm_ctx_t ctx, int path);
 /* Number of complex entries of a kernel spectrum: (M/2+1)^2.  The spectrum of
  * an even circulant embedding is even in both frequencies, so only this
  * quarter is stored. */
@@ -249,6 +254,9 @@ int cgm_memcpy_d2h(void* dst, const void* src, size_t bytes);
 int cgm_synchronize(cgm_ctx_t ctx);
 /* FP64 J0, J1, Y0, Y1 (host); H_n^(1) = J_n + i Y_n. */
 void cgm_bessel01(double x, double* j0, double* j1, double* y0, double* y1);
+/* out[q] = (i/4) H0^(1)(k r[q]) on the host in FP64 (get_greenfun,
+ * B/PyGMRES/GMRES.py:20-21); NaN where k r[q] <= 0. */
+int cgm_green_values(const double* r, size_t n, double k, cgm_complex128* out);
 
 #ifdef __cplusplus
 }

@@ -321,3 +321,37 @@ def test_plain_config_full_size_invariants(sol, orc):
     op = orc.Op(N, orc.baseline_green_block(N, problems.KH), c.M)
     r = b - op.matvec(chi, np_(x).astype(np.complex128))
     assert np.linalg.norm(r) / np.linalg.norm(b) == pytest.approx(last, rel=1e-3)
+
+
+@pytest.mark.parametrize("N,restart,cycles", [(64, 30, 20), (128, 20, 20), (256, 30, 20), (512, 30, 3),
+                                              (1024, 30, 3)])
+def test_fused_and_classic_arnoldi_agree(sol, orc, N, restart, cycles):
+    """Fused lagged-CGS2 steps (default) and the two-pass CGS2 kernels give
+    the same iterates, iteration counts and Givens estimates (converged
+    solves at the small sizes, a fixed number of cycles at the large ones)."""
+    from cugmres_amd import problems
+    c = ctx_for(sol, N)
+    g = c.spectrum_helmholtz(problems.KH)
+    rng = problems.SplitMix64(N)
+    th = 2 * math.pi * rng.uniform()
+    eps = problems.block_eps_map(N, 4, 0.5, 1.0, rng=rng, eps_hi=1.6)
+    chi = c.to_device((eps - 1).reshape(-1), "f32")
+    b = c.to_device(problems.plane_wave(N, problems.KH, th).reshape(-1))
+    out = []
+    for path in (0, 1):
+        c.set_arnoldi_path(path)
+        x, info = c.gmres(g, chi, b, restart=restart, tol=1e-6, max_cycles=cycles, history=True)
+        out.append((np_(x), info))
+    c.set_arnoldi_path(0)
+    (x0, i0), (x1, i1) = out
+    if N <= 256:
+        assert i0["converged"] and i1["converged"]
+        assert rel(x0, x1) < 2e-5
+    else:
+        assert i0["iterations"] == i1["iterations"] == restart * cycles
+        assert rel(x0, x1) < 1e-4
+        assert i0["rel_residual"] == pytest.approx(i1["rel_residual"], rel=1e-3)
+    assert abs(i0["iterations"] - i1["iterations"]) <= 1
+    k = min(len(i0["history"]), len(i1["history"]))
+    h0 = i0["history"][0]
+    assert np.allclose(i0["history"][:k], i1["history"][:k], rtol=1e-3, atol=1e-5 * h0)
