@@ -117,6 +117,7 @@ extern "C" int cgm_create(cgm_ctx_t* out, int N, int device) {
   int rc = plan_init(c->plan, N);
   if (rc) { delete c; cgm_set_last_error("cgm_create: no embedding for N=%d", N); return rc; }
   c->M = c->plan.M;
+  c->plan64 = c->plan;
   c->mask_chi = cgm_reference_chi(N);
   if ((rc = upload_twiddles(c))) { cgm_destroy(c); return rc; }
   const size_t nm = (size_t)N * c->M;
@@ -128,7 +129,88 @@ extern "C" int cgm_create(cgm_ctx_t* out, int N, int device) {
     return CGM_ERROR_ALLOC;
   }
   CGM_HIP(hipMemset(c->st, 0, sizeof(DevState)));
+  if ((rc = cgm_autotune(c))) { cgm_destroy(c); return rc; }
   *out = c;
+  return CGM_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------
+// autotuner: rows per work-group of passes 1/3 and columns per work-group of
+// pass 2, timed with HIP events on zero data (the kernels' cost does not
+// depend on the values)
+// ---------------------------------------------------------------------------
+extern "C" int cgm_autotune(cgm_ctx_t c) {
+  CGM_CHECK(c, CGM_ERROR_INVALID_VALUE, "null context");
+  CGM_HIP(hipSetDevice(c->device));
+  const int N = c->N, M = c->M;
+  const int nt = fft_threads(M);
+  const long n = (long)N * N;
+  const size_t H = (size_t)M / 2 + 1;
+  float2 *x = nullptr, *gq = nullptr;
+  float* chi = nullptr;
+  CGM_HIP(hipMalloc((void**)&x, sizeof(float2) * n));
+  CGM_HIP(hipMalloc((void**)&gq, sizeof(float2) * H * H));
+  CGM_HIP(hipMalloc((void**)&chi, sizeof(float) * n));
+  CGM_HIP(hipMemsetAsync(x, 0, sizeof(float2) * n, c->stream));
+  CGM_HIP(hipMemsetAsync(gq, 0, sizeof(float2) * H * H, c->stream));
+  CGM_HIP(hipMemsetAsync(chi, 0, sizeof(float) * n, c->stream));
+  CGM_HIP(hipMemsetAsync(c->T32, 0, sizeof(float2) * (size_t)N * M, c->stream));
+  hipEvent_t e0, e1;
+  CGM_HIP(hipEventCreate(&e0));
+  CGM_HIP(hipEventCreate(&e1));
+  Plan P = c->plan;
+  const int reps = 20;
+  auto time_it = [&](auto&& launch) -> float {
+    for (int r = 0; r < 3; ++r) launch();
+    (void)hipEventRecord(e0, c->stream);
+    for (int r = 0; r < reps; ++r) launch();
+    (void)hipEventRecord(e1, c->stream);
+    (void)hipEventSynchronize(e1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    return ms / reps;
+  };
+  int rc = CGM_SUCCESS;
+  const int cand[] = {1, 2, 4, 8};
+  float best1 = 1e30f, best2 = 1e30f, best3 = 1e30f;
+  int r1 = P.rows1, c2 = P.cols2, r3 = P.rows3;
+  for (int k : cand) {
+    if (k * nt > pass_max_threads(M, false) || (size_t)k * M * sizeof(float2) > 65536) continue;
+    Plan Q = P;
+    Q.rows1 = Q.rows3 = k;
+    const float t1 = time_it([&] { rc |= launch_pass1<float>(Q, c->stream, x, nullptr, chi, nullptr, kModeForward, c->T32, c->tw32, nullptr); });
+    const float t3 = time_it([&] { rc |= launch_pass3<float>(Q, c->stream, c->T32, x, chi, nullptr, c->vec_tmp ? c->vec_tmp : x, kModeForward, c->tw32, nullptr); });
+    if (t1 < best1) { best1 = t1; r1 = k; }
+    if (t3 < best3) { best3 = t3; r3 = k; }
+    if (M % k == 0) {
+      Q.cols2 = k;
+      const float t2 = time_it([&] { rc |= launch_pass2<float, float2>(Q, c->stream, c->T32, gq, c->tw32, nullptr); });
+      if (t2 < best2) { best2 = t2; c2 = k; }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(x);
+  (void)hipFree(gq);
+  (void)hipFree(chi);
+  if (rc) return rc;
+  c->plan.rows1 = r1;
+  c->plan.rows3 = r3;
+  c->plan.cols2 = c2;
+  c->tuned_us[0] = best1 * 1e3f;
+  c->tuned_us[1] = best2 * 1e3f;
+  c->tuned_us[2] = best3 * 1e3f;
+  return CGM_SUCCESS;
+}
+
+extern "C" int cgm_get_tiles(cgm_ctx_t c, int* tiles, float* us) {
+  CGM_CHECK(c && tiles, CGM_ERROR_INVALID_VALUE, "null argument");
+  tiles[0] = c->plan.rows1;
+  tiles[1] = c->plan.cols2;
+  tiles[2] = c->plan.rows3;
+  if (us)
+    for (int k = 0; k < 3; ++k) us[k] = c->tuned_us[k];
   return CGM_SUCCESS;
 }
 
@@ -168,7 +250,7 @@ extern "C" int cgm_synchronize(cgm_ctx_t c) {
 // ---------------------------------------------------------------------------
 static int spectrum_common(cgm_context* c, const double2* blk, cgm_complex64* g32,
                            cgm_complex128* g64) {
-  return spectrum_from_block(c->plan, c->stream, blk, c->tw64, (float2*)g32, (double2*)g64);
+  return spectrum_from_block(c->plan64, c->stream, blk, c->tw64, (float2*)g32, (double2*)g64);
 }
 
 extern "C" int cgm_spectrum_from_block(cgm_ctx_t c, const cgm_complex128* blk, cgm_complex64* g32,
@@ -260,9 +342,9 @@ static int ensure_T64(cgm_context* c) {
 static int matvec64(cgm_context* c, const double2* gq, const double* chi, const double2* x,
                     const double2* b, double2* y, int mode) {
   int rc = ensure_T64(c);
-  if (!rc) rc = launch_pass1<double>(c->plan, c->stream, x, nullptr, chi, nullptr, mode, c->T64, c->tw64, nullptr);
-  if (!rc) rc = launch_pass2<double, double2>(c->plan, c->stream, c->T64, gq, c->tw64, nullptr);
-  if (!rc) rc = launch_pass3<double>(c->plan, c->stream, c->T64, x, chi, b, y, mode, c->tw64, nullptr);
+  if (!rc) rc = launch_pass1<double>(c->plan64, c->stream, x, nullptr, chi, nullptr, mode, c->T64, c->tw64, nullptr);
+  if (!rc) rc = launch_pass2<double, double2>(c->plan64, c->stream, c->T64, gq, c->tw64, nullptr);
+  if (!rc) rc = launch_pass3<double>(c->plan64, c->stream, c->T64, x, chi, b, y, mode, c->tw64, nullptr);
   return rc;
 }
 

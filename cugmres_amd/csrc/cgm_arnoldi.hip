@@ -105,70 +105,91 @@ __global__ void __launch_bounds__(kFBS) k_arn_a(FusedArgs a) {
   }
   __syncthreads();
   lds_fft<float, M, +1>(buf, tws, t, fa);
-  // ---- z = u - G(.) ----------------------------------------------------------
+  // ---- z = u - G(.), u and z of the block's R*N elements staged in LDS -------
+  // (the FFT rows are no longer needed once every thread has read its values)
   float2 uu[EPD], zz[EPD];
-  long idx[EPD];
-  bool live[EPD];
 #pragma unroll
   for (int k = 0; k < EPD; ++k) {
     const int e = tid + k * kFBS;
     const int r = e / N, c = e - r * N;
     const int row = r0 + r;
-    live[k] = row < N;
-    idx[k] = (long)row * N + c;
     uu[k] = make_float2(0.f, 0.f);
     zz[k] = make_float2(0.f, 0.f);
-    if (live[k]) {
-      const float2 u = a.Uj[idx[k]];
+    if (row < N) {
+      const long id = (long)row * N + c;
+      const float2 u = a.Uj[id];
       float2 tv = smem[r * M + c];
       tv.x *= a.inv_mm;
       tv.y *= a.inv_mm;
       if (a.mode == kModeAdjoint) {
-        const float ch = a.chi[idx[k]];
+        const float ch = a.chi[id];
         tv.x *= ch;
         tv.y *= ch;
       }
       const float2 z = make_float2(u.x - tv.x, u.y - tv.y);
-      a.Z[idx[k]] = z;
+      a.Z[id] = z;
       uu[k] = u;
       zz[k] = z;
     }
   }
-  // ---- inner products --------------------------------------------------------
+  __syncthreads();
+  float2* su = smem;                 // R*N values of u
+  float2* sz = smem + kFBS * EPD;    // R*N values of z
+#pragma unroll
+  for (int k = 0; k < EPD; ++k) {
+    su[tid + k * kFBS] = uu[k];
+    sz[tid + k * kFBS] = zz[k];
+  }
+  __syncthreads();
+  // ---- inner products ----------------------------------------------------------
+  // A group of 32 lanes owns one basis vector q < j (slot q == j carries
+  // alpha = u^H u and gamma = u^H z) and sums it over the block's elements
+  // e = g, g+32, ...; one 32-lane reduction then gives the block partial.
+  // Partial layout (per launch): s_q / alpha at q, t_q / gamma at j+1+q.
   const int G = a.G;
-  double2* part_s = a.part;
-  double2* part_t = a.part + (long)(a.ld + 1) * G;
-  double2* part_ag = a.part + (long)(2 * a.ld + 2) * G;
-  for (int c0 = 0; c0 < j; c0 += kCH) {
-    float2 acs[kCH], act[kCH];
+  const long base = (long)r0 * N;  // the block's rows are contiguous
+  const int RN = kFBS * EPD;
+  const int nrows_live = (N - r0) < R ? (N - r0) : R;
+  const int RNl = nrows_live * N;   // live elements of this block
+  const int grp = tid >> 5, g = tid & 31;
+  for (int c0 = 0; c0 <= j; c0 += kFBS / 32) {
+    const int q = c0 + grp;
+    float2 as = make_float2(0.f, 0.f), at = make_float2(0.f, 0.f);
+    if (q < j) {
+      const float2* Vq = a.V + (long)q * n + base;
+      for (int e0 = g; e0 < RNl; e0 += 32 * 8) {
+        float2 v[8];
 #pragma unroll
-    for (int q = 0; q < kCH; ++q) acs[q] = act[q] = make_float2(0.f, 0.f);
+        for (int k = 0; k < 8; ++k) {
+          const int e = e0 + 32 * k;
+          v[k] = e < RNl ? Vq[e] : make_float2(0.f, 0.f);
+        }
 #pragma unroll
-    for (int k = 0; k < EPD; ++k) {
-      if (!live[k]) continue;
-#pragma unroll
-      for (int q = 0; q < kCH; ++q) {
-        if (c0 + q < j) {
-          const float2 v = a.V[(long)(c0 + q) * n + idx[k]];
-          const float2 ps = cjmulf(v, uu[k]), pt = cjmulf(v, zz[k]);
-          acs[q].x += ps.x; acs[q].y += ps.y;
-          act[q].x += pt.x; act[q].y += pt.y;
+        for (int k = 0; k < 8; ++k) {
+          const int e = e0 + 32 * k;
+          if (e < RNl) {
+            const float2 ps = cjmulf(v[k], su[e]), pt = cjmulf(v[k], sz[e]);
+            as.x += ps.x; as.y += ps.y;
+            at.x += pt.x; at.y += pt.y;
+          }
         }
       }
+    } else if (q == j) {
+      for (int e = g; e < RNl; e += 32) {
+        const float2 u = su[e], z = sz[e];
+        as.x += u.x * u.x + u.y * u.y;
+        const float2 pg = cjmulf(u, z);
+        at.x += pg.x; at.y += pg.y;
+      }
     }
-    block_store<kFBS, kCH>(acs, part_s, G, c0, j);
-    block_store<kFBS, kCH>(act, part_t, G, c0, j);
-  }
-  {
-    float2 ag[2] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
-#pragma unroll
-    for (int k = 0; k < EPD; ++k) {
-      ag[0].x += uu[k].x * uu[k].x + uu[k].y * uu[k].y;
-      const float2 pg = cjmulf(uu[k], zz[k]);
-      ag[1].x += pg.x;
-      ag[1].y += pg.y;
+    (void)RN;
+    // 32-lane sums (rows 0+1 and rows 2+3 of the wave)
+    const float sx = half_wave_sum_f(as.x), sy = half_wave_sum_f(as.y);
+    const float tx = half_wave_sum_f(at.x), ty = half_wave_sum_f(at.y);
+    if (g == 0 && q <= j) {
+      put2(&a.part[(long)q * G + blockIdx.x], sx, sy);
+      put2(&a.part[(long)(j + 1 + q) * G + blockIdx.x], tx, ty);
     }
-    block_store<kFBS, 2>(ag, part_ag, G, 0, 2);
   }
   if (!arrive_last(&a.st->tickets[kTicketDots1])) return;
 
@@ -179,10 +200,19 @@ __global__ void __launch_bounds__(kFBS) k_arn_a(FusedArgs a) {
   __shared__ double2 lag[2];
   __shared__ double s_nu;
   __shared__ int s_stop;
+  __shared__ double2 lall[2 * LM];
   DevState* st = a.st;
-  finish_partials<kFBS>(part_s, G, j, ls);
-  finish_partials<kFBS>(part_t, G, j, lt);
-  finish_partials<kFBS>(part_ag, G, 2, lag);
+  finish_partials<kFBS>(a.part, G, 2 * (j + 1), lall);
+  __syncthreads();
+  for (int i = tid; i <= j; i += kFBS) {
+    if (i < j) {
+      ls[i] = lall[i];
+      lt[i] = lall[j + 1 + i];
+    } else {
+      lag[0] = lall[j];
+      lag[1] = lall[2 * j + 1];
+    }
+  }
   __syncthreads();
   if (tid == 0) {
     double ss = 0.0;

@@ -21,6 +21,7 @@ struct Plan {
 int plan_init(Plan& P, int N);
 int fft_threads(int M);
 int choose_embedding(int N);
+int pass_max_threads(int M, bool fp64);
 
 template <typename T>
 int launch_pass1(const Plan& P, hipStream_t st, const cplx<T>* x, cplx<T>* xw, const T* chi,
@@ -122,12 +123,14 @@ int launch_norm64(hipStream_t st, const double2* v, long n, double* partn, int G
 struct cgm_context {
   int N = 0, M = 0, device = 0;
   hipStream_t stream = nullptr;
-  cgm::Plan plan{};
+  cgm::Plan plan{};    // complex64 passes (autotuned tiles)
+  cgm::Plan plan64{};  // complex128 passes and the spectrum precompute (default tiles)
   float2* tw32 = nullptr;
   double2* tw64 = nullptr;
   float2* T32 = nullptr;   // N x M scratch of one complex64 matvec
   double2* T64 = nullptr;  // N x M scratch of one complex128 matvec (lazy)
   float mask_chi = 0.f;
+  float tuned_us[3] = {0.f, 0.f, 0.f};  // autotuned pass 1/2/3 times
   int path = 0;  // Arnoldi kernels: 0 = fused lagged CGS2 where supported, 1 = two-pass CGS2
   cgm::KrylovWs ws;
   cgm::DevState* st = nullptr;   // device

@@ -27,8 +27,16 @@ namespace cgm {
 // ---------------------------------------------------------------------------
 // pass kernels
 // ---------------------------------------------------------------------------
+// Work-group bound: float kernels accept tiles up to 1024 threads (autotuned
+// rows/columns per work-group); double kernels keep the tighter bound that
+// avoids register spills of the FP64 transforms.
+template <typename T, int M> struct PassWG {
+  static constexpr int value =
+      (sizeof(T) == 4 && FftShape<M>::pow2) ? (FftShape<M>::WG > 1024 ? FftShape<M>::WG : 1024) : FftShape<M>::WG;
+};
+
 template <typename T, int M>
-__global__ void __launch_bounds__(FftShape<M>::WG)
+__global__ void __launch_bounds__((PassWG<T, M>::value))
 k_pass1(const cplx<T>* __restrict__ x, cplx<T>* xw, const T* __restrict__ chi,
         const double* __restrict__ norm_ptr, int mode, cplx<T>* __restrict__ Tm, int N,
         int rows_per_wg, const cplx<T>* __restrict__ tw, const int* __restrict__ done) {
@@ -70,7 +78,7 @@ k_pass1(const cplx<T>* __restrict__ x, cplx<T>* xw, const T* __restrict__ chi,
 }
 
 template <typename T, typename G, int M>
-__global__ void __launch_bounds__(FftShape<M>::WG)
+__global__ void __launch_bounds__((PassWG<T, M>::value))
 k_pass2(cplx<T>* __restrict__ Tm, const G* __restrict__ gq, int N, int cols_per_wg,
         const cplx<T>* __restrict__ tw, const int* __restrict__ done) {
   if (done && *done) return;
@@ -114,7 +122,7 @@ k_pass2(cplx<T>* __restrict__ Tm, const G* __restrict__ gq, int N, int cols_per_
 }
 
 template <typename T, int M>
-__global__ void __launch_bounds__(FftShape<M>::WG)
+__global__ void __launch_bounds__((PassWG<T, M>::value))
 k_pass3(const cplx<T>* __restrict__ Tm, const cplx<T>* __restrict__ x, const T* __restrict__ chi,
         const cplx<T>* __restrict__ b, cplx<T>* __restrict__ y, int mode, int N, int rows_per_wg,
         T inv, const cplx<T>* __restrict__ tw, const int* __restrict__ done) {
@@ -279,6 +287,16 @@ int fft_threads(int M) {
   return 0;
 }
 
+// Largest work-group (threads) the pass kernels of size M accept.
+int pass_max_threads(int M, bool fp64) {
+  switch (M) {
+#define X(m) case m: return fp64 ? PassWG<double, m>::value : PassWG<float, m>::value;
+    CGM_FOR_EACH_M(X)
+#undef X
+  }
+  return 0;
+}
+
 int choose_embedding(int N) {
   static const int sizes[] = {16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512, 768,
                               1024, 1536, 2048, 3072, 4096};
@@ -301,6 +319,7 @@ int launch_pass1(const Plan& P, hipStream_t st, const cplx<T>* x, cplx<T>* xw, c
   const int grid = (N + rows - 1) / rows;
   const int threads = rows * fft_threads(M);
   const size_t lds = (size_t)rows * M * sizeof(cplx<T>);
+  if (threads > pass_max_threads(M, sizeof(T) == 8) || lds > 65536) return CGM_ERROR_UNSUPPORTED;
   switch (M) {
 #define X(m)                                                                                 \
   case m:                                                                                    \
@@ -322,6 +341,7 @@ int launch_pass2(const Plan& P, hipStream_t st, cplx<T>* Tm, const G* gq, const 
   const int grid = M / cols;
   const int threads = cols * fft_threads(M);
   const size_t lds = (size_t)cols * M * sizeof(cplx<T>);
+  if (threads > pass_max_threads(M, sizeof(T) == 8) || lds > 65536 || M % cols) return CGM_ERROR_UNSUPPORTED;
   switch (M) {
 #define X(m)                                                                                  \
   case m:                                                                                     \
@@ -343,6 +363,7 @@ int launch_pass3(const Plan& P, hipStream_t st, const cplx<T>* Tm, const cplx<T>
   const int grid = (N + rows - 1) / rows;
   const int threads = rows * fft_threads(M);
   const size_t lds = (size_t)rows * M * sizeof(cplx<T>);
+  if (threads > pass_max_threads(M, sizeof(T) == 8) || lds > 65536) return CGM_ERROR_UNSUPPORTED;
   const T inv = T(1.0 / ((double)M * (double)M));
   switch (M) {
 #define X(m)                                                                                   \

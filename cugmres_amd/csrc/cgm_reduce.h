@@ -39,6 +39,21 @@ __device__ inline float wave_sum_f(float v) {
   return (r0 + r1) + (r2 + r3);
 }
 
+// Sum over each half (32 lanes) of a wave: after the row-local DPP steps
+// every lane of a 16-lane row holds the row sum; rows 0+1 form the first
+// half, rows 2+3 the second.  Lanes of a half all get the half's sum.
+__device__ inline float half_wave_sum_f(float v) {
+  v += dpp_mov<0x128>(v);  // row_ror:8
+  v += dpp_mov<0x124>(v);  // row_ror:4
+  v += dpp_mov<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_mov<0xB1>(v);   // quad_perm [1,0,3,2]
+  const float r0 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 0));
+  const float r1 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 16));
+  const float r2 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 32));
+  const float r3 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 48));
+  return (threadIdx.x & 32) ? (r2 + r3) : (r0 + r1);
+}
+
 __device__ inline void put(double* p, double v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }

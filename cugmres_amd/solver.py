@@ -53,6 +53,17 @@ class Context:
     def handle(self):
         return self._ptr
 
+    def tiles(self):
+        """(rows per work-group of pass 1, columns of pass 2, rows of pass 3) and
+        their tuned times in us, as chosen by the autotuner."""
+        t = (_c.c_int * 3)()
+        us = (_c.c_float * 3)()
+        check(lib().cgm_get_tiles(self._ptr, t, us), "cgm_get_tiles")
+        return tuple(t), tuple(us)
+
+    def autotune(self):
+        check(lib().cgm_autotune(self._ptr), "cgm_autotune")
+
     def set_arnoldi_path(self, path):
         """0: fused lagged-CGS2 steps (default), 1: separate two-pass CGS2 kernels."""
         check(lib().cgm_set_arnoldi_path(self._ptr, int(path)), "cgm_set_arnoldi_path")

@@ -85,6 +85,15 @@ int cgm_destroy(cgm_ctx_t ctx);
  * as void*; NULL = the default stream). */
 int cgm_set_stream(cgm_ctx_t ctx, void* stream);
 int cgm_get_embedding(cgm_ctx_t ctx, int* M);
+/* Autotuner (run by cgm_create): times the work-group tiles of the three FFT
+ * passes (rows per work-group of passes 1 and 3, columns per work-group of
+ * pass 2, each in {1, 2, 4, 8} within 1024 threads and 64 KiB of LDS) with
+ * HIP events and keeps the fastest.  Call again after cgm_set_stream to tune
+ * on another stream. */
+int cgm_autotune(cgm_ctx_t ctx);
+/* tiles[0..2] = rows of pass 1, columns of pass 2, rows of pass 3; us[0..2]
+ * (optional) = their tuned times in microseconds. */
+int cgm_get_tiles(cgm_ctx_t ctx, int* tiles, float* us);
 /* Arnoldi kernels of cgm_gmres: 0 (default) = fused steps with lagged CGS2
  * (one global reduction and two basis passes per step; power-of-two N in
  * [64, 1024], restart <= 64, other cases fall back to 1); 1 = separate

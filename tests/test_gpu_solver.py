@@ -355,3 +355,20 @@ def test_fused_and_classic_arnoldi_agree(sol, orc, N, restart, cycles):
     k = min(len(i0["history"]), len(i1["history"]))
     h0 = i0["history"][0]
     assert np.allclose(i0["history"][:k], i1["history"][:k], rtol=1e-3, atol=1e-5 * h0)
+
+
+def test_autotuned_tiles_are_legal_and_keep_parity(sol, orc):
+    """The autotuner picks tiles within the kernels' limits, and a matvec with
+    every candidate tile gives the same result."""
+    from cugmres_amd import problems
+    N = 512
+    c = ctx_for(sol, N)
+    tiles, us = c.tiles()
+    assert all(t in (1, 2, 4, 8) for t in tiles) and all(u > 0 for u in us)
+    g = c.spectrum_helmholtz(problems.KH)
+    rng = np.random.default_rng(3)
+    chi = c.to_device(rng.uniform(0, 11.1, N * N), "f32")
+    x = c.to_device(rng.standard_normal(N * N) + 1j * rng.standard_normal(N * N))
+    y0 = np_(c.matvec(g, chi, x))
+    c.autotune()
+    assert rel(np_(c.matvec(g, chi, x)), y0) < 1e-6
