@@ -52,6 +52,17 @@ class RefineResult(ctypes.Structure):
                 ("converged", ctypes.c_int), ("rel_residual", ctypes.c_double)]
 
 
+class FgmresParams(ctypes.Structure):
+    _fields_ = [("restart", ctypes.c_int), ("max_cycles", ctypes.c_int), ("tol", ctypes.c_double),
+                ("x0_zero", ctypes.c_int), ("stall_ratio", ctypes.c_double)]
+
+
+class FgmresResult(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("cycles", ctypes.c_int), ("converged", ctypes.c_int),
+                ("rel_residual", ctypes.c_double), ("beta0", ctypes.c_double), ("bnorm", ctypes.c_double),
+                ("precond_cycles", ctypes.c_int), ("fell_back", ctypes.c_int)]
+
+
 # name -> (restype, argtypes); every symbol declared in include/cugmres.h
 SIGNATURES = {
     "cgm_status_string": (ctypes.c_char_p, [ctypes.c_int]),
@@ -76,6 +87,14 @@ SIGNATURES = {
                                  ctypes.POINTER(GmresParams), ctypes.POINTER(GmresResult), c_void_p]),
     "cgm_gmres_refine": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                         ctypes.POINTER(RefineParams), ctypes.POINTER(RefineResult)]),
+    "cgm_fno_num_params": (ctypes.c_int, [ctypes.POINTER(ctypes.c_size_t)]),
+    "cgm_fno_create": (ctypes.c_int, [c_void_p, ctypes.c_int, ctypes.c_float, ctypes.POINTER(c_void_p)]),
+    "cgm_fno_destroy": (ctypes.c_int, [c_void_p]),
+    "cgm_fno_set_params": (ctypes.c_int, [c_void_p, c_void_p]),
+    "cgm_fno_forward": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, ctypes.c_int]),
+    "cgm_fno_backward": (ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
+    "cgm_fgmres": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                  ctypes.POINTER(FgmresParams), ctypes.POINTER(FgmresResult), c_void_p]),
     "cgm_reference_plane_wave": (ctypes.c_int, [c_void_p, c_void_p]),
     "cgm_reference_chi": (ctypes.c_float, [ctypes.c_int]),
     "cgm_set_mask_contrast": (ctypes.c_int, [c_void_p, ctypes.c_float]),

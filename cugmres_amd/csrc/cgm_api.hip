@@ -94,7 +94,7 @@ extern "C" int cgm_destroy(cgm_ctx_t c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   krylov_free(c->ws);
-  void* ptrs[] = {c->tw32, c->tw64, c->T32, c->T64, c->st, c->chi_tmp, c->vec_tmp, c->v64a, c->chi32};
+  void* ptrs[] = {c->tw32, c->tw64, c->T32, c->T64, c->st, c->chi_tmp, c->vec_tmp, c->v64a, c->chi32, c->zb};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->hst) (void)hipHostFree(c->hst);
@@ -378,6 +378,49 @@ static int read_state(cgm_context* c) {
 
 // Core solver on float2 device vectors.  hist: optional device array,
 // hist[0] = beta0, hist[t] = estimate after t steps.
+// One restart cycle of plain GMRES from the residual in V_0 (its norm in
+// st->beta), up to the Givens-rotated least-squares problem; the caller
+// applies x += V y.  Fused lagged-CGS2 steps where supported.
+static int plain_cycle(cgm_context* c, const float2* gq, const float* chi, int fwd, int m, double* hist,
+                       int total) {
+  KrylovWs& ws = c->ws;
+  const long n = ws.n;
+  const bool fused = c->path != 1 && fused_supported(c->N, m);
+  int rc;
+  if (fused) {
+    // r = b - A x is in V_0; u_0 = r, first matvec's passes 1-2
+    float2* U0 = ws.U;
+    CGM_HIP(hipMemcpyAsync(U0, ws.V, sizeof(float2) * n, hipMemcpyDeviceToDevice, c->stream));
+    rc = launch_pass1<float>(c->plan, c->stream, U0, nullptr, chi, nullptr, fwd, c->T32, c->tw32, &c->st->done);
+    if (!rc) rc = launch_pass2<float, float2>(c->plan, c->stream, c->T32, gq, c->tw32, &c->st->done);
+    if (rc) return rc;
+    FusedLaunch L{};
+    L.N = c->N; L.m = m; L.ld = ws.m; L.mode = fwd; L.chi = chi; L.V = ws.V; L.Z = ws.Z;
+    L.T = c->T32; L.tw = c->tw32; L.part = ws.part; L.H = ws.H; L.Rm = ws.R; L.cs = ws.cs;
+    L.sn = ws.sn; L.g = ws.g; L.h1p = ws.h1p; L.a1 = ws.a1; L.a2 = ws.a2; L.scal = ws.scal;
+    L.st = c->st; L.hist = hist ? hist + total : nullptr;
+    for (int j = 0; j <= m; ++j) {
+      L.j = j;
+      L.Uj = ws.U + (long)(j & 1) * n;
+      L.Unext = ws.U + (long)((j + 1) & 1) * n;
+      if ((rc = launch_fused(c->stream, 0, L))) return rc;
+      if (j == m) break;
+      if ((rc = launch_fused(c->stream, 1, L))) return rc;
+      if ((rc = launch_pass2<float, float2>(c->plan, c->stream, c->T32, gq, c->tw32, &c->st->done))) return rc;
+    }
+  } else {
+    for (int i = 0; i < m; ++i) {
+      float2* vi = ws.V + (long)i * n;
+      const double* nrm = (i == 0) ? &c->st->beta : &c->st->hn;
+      rc = matvec32(c, gq, chi, vi, nullptr, vi + n, fwd, nrm, vi, &c->st->done);
+      if (rc) return rc;
+      rc = launch_cgs2_step(c->stream, ws, i, c->st, hist ? hist + 1 + total : nullptr);
+      if (rc) return rc;
+    }
+  }
+  return CGM_SUCCESS;
+}
+
 static int gmres_core(cgm_context* c, const float2* gq, const float* chi, const float2* b,
                       float2* x, const cgm_gmres_params* p, cgm_gmres_result* res, double* hist) {
   const long n = (long)c->N * c->N;
@@ -387,7 +430,6 @@ static int gmres_core(cgm_context* c, const float2* gq, const float* chi, const 
   KrylovWs& ws = c->ws;
   const int fwd = p->adjoint ? kModeAdjoint : kModeForward;
   const int resmode = p->adjoint ? kModeResidualAdj : kModeResidual;
-  const bool fused = c->path != 1 && fused_supported(c->N, m);
   // ||b|| and the absolute tolerance
   CGM_HIP(hipMemsetAsync(&c->st->status, 0, sizeof(int), c->stream));
   if ((rc = launch_norm_init(c->stream, ws, b, c->st, 1, p->tol))) return rc;
@@ -420,37 +462,7 @@ static int gmres_core(cgm_context* c, const float2* gq, const float* chi, const 
     if (h.bnorm == 0.0 || h.beta <= p->tol * h.bnorm) { res->converged = 1; break; }
     if (cycles >= p->max_cycles) break;
     // one restart cycle, enqueued without host synchronisation
-    if (fused) {
-      // r = b - A x is in V_0; u_0 = r, first matvec's passes 1-2
-      float2* U0 = ws.U;
-      CGM_HIP(hipMemcpyAsync(U0, ws.V, sizeof(float2) * n, hipMemcpyDeviceToDevice, c->stream));
-      rc = launch_pass1<float>(c->plan, c->stream, U0, nullptr, chi, nullptr, fwd, c->T32, c->tw32, &c->st->done);
-      if (!rc) rc = launch_pass2<float, float2>(c->plan, c->stream, c->T32, gq, c->tw32, &c->st->done);
-      if (rc) return rc;
-      FusedLaunch L{};
-      L.N = c->N; L.m = m; L.ld = ws.m; L.mode = fwd; L.chi = chi; L.V = ws.V; L.Z = ws.Z;
-      L.T = c->T32; L.tw = c->tw32; L.part = ws.part; L.H = ws.H; L.Rm = ws.R; L.cs = ws.cs;
-      L.sn = ws.sn; L.g = ws.g; L.h1p = ws.h1p; L.a1 = ws.a1; L.a2 = ws.a2; L.scal = ws.scal;
-      L.st = c->st; L.hist = hist ? hist + total : nullptr;
-      for (int j = 0; j <= m; ++j) {
-        L.j = j;
-        L.Uj = ws.U + (long)(j & 1) * n;
-        L.Unext = ws.U + (long)((j + 1) & 1) * n;
-        if ((rc = launch_fused(c->stream, 0, L))) return rc;
-        if (j == m) break;
-        if ((rc = launch_fused(c->stream, 1, L))) return rc;
-        if ((rc = launch_pass2<float, float2>(c->plan, c->stream, c->T32, gq, c->tw32, &c->st->done))) return rc;
-      }
-    } else {
-      for (int i = 0; i < m; ++i) {
-        float2* vi = ws.V + (long)i * n;
-        const double* nrm = (i == 0) ? &c->st->beta : &c->st->hn;
-        rc = matvec32(c, gq, chi, vi, nullptr, vi + n, fwd, nrm, vi, &c->st->done);
-        if (rc) return rc;
-        rc = launch_cgs2_step(c->stream, ws, i, c->st, hist ? hist + 1 + total : nullptr);
-        if (rc) return rc;
-      }
-    }
+    if ((rc = plain_cycle(c, gq, chi, fwd, m, hist, total))) return rc;
     if ((rc = launch_backsub_update(c->stream, ws, c->st, x))) return rc;
     if ((rc = read_state(c))) return rc;
     total += c->hst->k;
@@ -485,6 +497,138 @@ extern "C" int cgm_gmres(cgm_ctx_t c, const cgm_complex64* gamma, const float* c
   CGM_HIP(hipSetDevice(c->device));
   memset(res, 0, sizeof(*res));
   return gmres_core(c, (const float2*)gamma, chi, (const float2*)b, (float2*)x, p, res, hist);
+}
+
+// ---------------------------------------------------------------------------
+// Flexible GMRES with a right preconditioner
+// ---------------------------------------------------------------------------
+// Identity preconditioner: v_i /= *nrm in place and z_i = v_i.
+__global__ void k_norm_copy(float2* __restrict__ v, const double* __restrict__ nrm, float2* __restrict__ z, long n,
+                            const int* done) {
+  if (*done) return;
+  const double d = *nrm;
+  const float r = d > 0.0 ? (float)(1.0 / d) : 0.f;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    float2 a = v[e];
+    a.x *= r;
+    a.y *= r;
+    v[e] = a;
+    z[e] = a;
+  }
+}
+
+// One preconditioned cycle: z_i = P(v_i), w = A z_i, CGS2 step; the caller
+// applies x += Z y.
+static int precond_cycle(cgm_context* c, const float2* gq, const float* chi, cgm_fno* net, int m, double* hist,
+                         int total) {
+  KrylovWs& ws = c->ws;
+  const long n = ws.n;
+  int rc;
+  for (int i = 0; i < m; ++i) {
+    float2* vi = ws.V + (long)i * n;
+    float2* zi = c->zb + (long)i * n;
+    const double* nrm = (i == 0) ? &c->st->beta : &c->st->hn;
+    if (net) {
+      rc = fno_forward(net, c->stream, chi, vi, nrm, vi, zi, 1, false, &c->st->done);
+    } else {
+      hipLaunchKernelGGL(k_norm_copy, dim3(ws.G), dim3(kKB), 0, c->stream, vi, nrm, zi, n, &c->st->done);
+      rc = hipGetLastError() == hipSuccess ? CGM_SUCCESS : CGM_ERROR_DEVICE;
+    }
+    if (rc) return rc;
+    rc = matvec32(c, gq, chi, zi, nullptr, vi + n, kModeForward, nullptr, nullptr, &c->st->done);
+    if (rc) return rc;
+    rc = launch_cgs2_step(c->stream, ws, i, c->st, hist ? hist + 1 + total : nullptr);
+    if (rc) return rc;
+  }
+  return CGM_SUCCESS;
+}
+
+extern "C" int cgm_fgmres(cgm_ctx_t c, const cgm_complex64* gamma, const float* chi, const cgm_complex64* b,
+                          cgm_complex64* x, cgm_fno_t net, const cgm_fgmres_params* p, cgm_fgmres_result* res,
+                          double* hist) {
+  CGM_CHECK(c && gamma && chi && b && x && res && p, CGM_ERROR_INVALID_VALUE, "null argument");
+  CGM_CHECK(p->restart >= 1 && p->restart <= kMaxRestart, CGM_ERROR_INVALID_VALUE, "restart %d outside [1, %d]",
+            p->restart, kMaxRestart);
+  CGM_CHECK(p->max_cycles >= 0, CGM_ERROR_INVALID_VALUE, "max_cycles < 0");
+  CGM_CHECK(p->tol >= 0 && isfinite(p->tol), CGM_ERROR_INVALID_VALUE, "bad tolerance");
+  CGM_CHECK(!net || fno_context(net) == c, CGM_ERROR_INVALID_VALUE,
+            "the preconditioner was created on another context");
+  CGM_HIP(hipSetDevice(c->device));
+  memset(res, 0, sizeof(*res));
+  const long n = (long)c->N * c->N;
+  const int m = p->restart;
+  int rc = krylov_alloc(c->ws, n, m, c->stream);
+  if (rc) return rc;
+  if (!c->zb || c->zb_m < m) {
+    if (c->zb) CGM_HIP(hipFree(c->zb));
+    c->zb = nullptr;
+    c->zb_m = 0;
+    CGM_HIP(hipMalloc((void**)&c->zb, sizeof(float2) * n * m));
+    c->zb_m = m;
+  }
+  KrylovWs& ws = c->ws;
+  const float2* bb = (const float2*)b;
+  float2* xx = (float2*)x;
+  CGM_HIP(hipMemsetAsync(&c->st->status, 0, sizeof(int), c->stream));
+  if ((rc = launch_norm_init(c->stream, ws, bb, c->st, 1, p->tol))) return rc;
+  if (p->x0_zero) CGM_HIP(hipMemsetAsync(xx, 0, sizeof(float2) * n, c->stream));
+  int total = 0, cycles = 0;
+  bool precond = true;
+  bool last_precond = false;
+  double beta_prev = 0.0;
+  for (;;) {
+    if (p->x0_zero && cycles == 0) {
+      CGM_HIP(hipMemcpyAsync(ws.V, bb, sizeof(float2) * n, hipMemcpyDeviceToDevice, c->stream));
+    } else {
+      rc = matvec32(c, (const float2*)gamma, chi, xx, bb, ws.V, kModeResidual, nullptr, nullptr, nullptr);
+      if (rc) return rc;
+    }
+    if ((rc = launch_norm_init(c->stream, ws, ws.V, c->st, 0, 0.0))) return rc;
+    if ((rc = read_state(c))) return rc;
+    const DevState& h = *c->hst;
+    if (cycles == 0) {
+      res->beta0 = h.beta;
+      res->bnorm = h.bnorm;
+      if (hist) CGM_HIP(hipMemcpyAsync(hist, &h.beta, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    }
+    res->rel_residual = h.bnorm > 0 ? h.beta / h.bnorm : 0.0;
+    if (h.status) {
+      cgm_set_last_error("cgm_fgmres: non-finite residual");
+      res->iterations = total;
+      res->cycles = cycles;
+      return CGM_ERROR_NONFINITE;
+    }
+    if (h.bnorm == 0.0 || h.beta <= p->tol * h.bnorm) { res->converged = 1; break; }
+    if (cycles >= p->max_cycles) break;
+    // the true residual judges the preconditioner: a cycle that did not cut
+    // it below stall_ratio hands the rest of the solve to plain GMRES
+    if (precond && last_precond && p->stall_ratio > 0.0 && h.beta > p->stall_ratio * beta_prev) {
+      precond = false;
+      res->fell_back = 1;
+    }
+    beta_prev = h.beta;
+    if (precond) {
+      if ((rc = precond_cycle(c, (const float2*)gamma, chi, net, m, hist, total))) return rc;
+      if ((rc = launch_backsub_update(c->stream, ws, c->st, xx, c->zb))) return rc;
+      res->precond_cycles += 1;
+    } else {
+      if ((rc = plain_cycle(c, (const float2*)gamma, chi, kModeForward, m, hist, total))) return rc;
+      if ((rc = launch_backsub_update(c->stream, ws, c->st, xx))) return rc;
+    }
+    last_precond = precond;
+    if ((rc = read_state(c))) return rc;
+    total += c->hst->k;
+    ++cycles;
+    if (c->hst->status) {
+      cgm_set_last_error("cgm_fgmres: non-finite value in the Arnoldi process");
+      res->iterations = total;
+      res->cycles = cycles;
+      return CGM_ERROR_NONFINITE;
+    }
+  }
+  res->iterations = total;
+  res->cycles = cycles;
+  return res->converged ? CGM_SUCCESS : CGM_ERROR_NOT_CONVERGED;
 }
 
 // ---------------------------------------------------------------------------

@@ -1,0 +1,805 @@
+// Fourier-neural-operator (FNO) right preconditioner: BF16 MFMA inference and
+// training kernels for gfx950.
+//
+// The reference has no preconditioner (B/CUDA_GMRES/GMRES.cuh runs plain
+// GMRES); this network is the learned extension that flexible GMRES
+// (cgm_fgmres) applies to each Krylov vector.  For a unit-norm vector v on the
+// N x N grid and the contrast map chi:
+//
+//   a   = (chi * chi_scale, N Re v, N Im v)                 per cell
+//   h0  = a W_in + b_in                                      (3 -> 32 channels)
+//   h_{l+1} = act_l( h_l W_l + b_l + K_l(h_l) ),  l = 0..3,  act = GELU (tanh
+//             form) for l < 3, identity for l = 3
+//   out = h4 W_out + b_out                                   (32 -> 2)
+//   P(v) = s_id v + (out_0 + i out_1) / N                    (s_id learned)
+//
+// K_l is the spectral convolution of an FNO (Li et al., "Fourier neural
+// operator for parametric partial differential equations", ICLR 2021) with
+// retained row frequencies kx in {0..15, N-16..N-1} and column frequencies
+// ky in {0..15}:
+//   U[c,kx,ky]  = sum_{x,y} h[x,y,c] e^{-2 pi i (kx x + ky y)/N}
+//   V[o,kx,ky]  = sum_c U[c,kx,ky] Wspec[kx,ky,c,o]
+//   K(h)[x,y,o] = (1/N^2) sum_{kx,ky} w_ky Re(V[o,kx,ky] e^{2 pi i (kx x + ky y)/N}),
+//                 w_0 = 1, w_ky = 2 otherwise (the inverse real FFT of the
+//                 truncated spectrum).
+// Only 32 column frequencies (cos and -sin of 16 ky) survive, so the column
+// transforms are GEMMs against an N x 32 basis F[y,2ky] = cos(2 pi ky y/N),
+// F[y,2ky+1] = -sin(2 pi ky y/N), and every layer is
+//   k_rowproj : A_x = F^T H_x              per grid row x (K = N, MFMA)
+//   k_dftx    : DFT of A along x at the 32 kx (FP32)
+//   k_mix     : channel mixing per mode (FP32)
+//   k_invx    : inverse DFT along x (FP32) -> B_x
+//   k_pixel   : h' = act([h, F_y] [W; B_x] + b)  per cell (K = 64, MFMA)
+// The backward pass has the same four kernels with the roles transposed.
+//
+// Activations are BF16, channel-last ([sample][x][y][32], 64 bytes per cell);
+// MFMA accumulation, the spectral path and all weight gradients are FP32.
+#include <math.h>
+
+#include <vector>
+
+#include "cgm_internal.h"
+#include "cgm_reduce.h"
+
+namespace cgm {
+namespace fno {
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+constexpr int C = 32;   // channels
+constexpr int KY = 16;  // retained column frequencies
+constexpr int KX = 32;  // retained row frequencies
+constexpr int XC = 32;  // rows per partial of k_dftx
+constexpr int S = 36;   // LDS row stride (bf16) of k_rowproj: 16-bank offset between lane groups
+constexpr int kLayerParams = C * C + C + KX * KY * C * C * 2;
+constexpr long kOffLayer0 = 4 * C;
+constexpr long layer_off(int l) { return kOffLayer0 + (long)l * kLayerParams; }
+constexpr long kOffOut = kOffLayer0 + 4L * kLayerParams;
+constexpr long kOffSkip = kOffOut + 2 * C + 2;  // weight of the identity path
+constexpr long kNumParams = kOffSkip + 1;
+
+__device__ inline bf16x8 ld8(const bf16* p) { return __builtin_bit_cast(bf16x8, *(const u32x4*)p); }
+__device__ inline f32x4 mma(bf16x8 a, bf16x8 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+__device__ inline void st4(bf16* p, float a, float b, float c, float d) {
+  struct __align__(8) Q { bf16 v[4]; } q;
+  q.v[0] = (bf16)a;
+  q.v[1] = (bf16)b;
+  q.v[2] = (bf16)c;
+  q.v[3] = (bf16)d;
+  *(Q*)p = q;
+}
+
+constexpr float kG0 = 0.7978845608028654f;  // sqrt(2/pi)
+constexpr float kG1 = 0.044715f;
+__device__ inline float gelu(float x) {
+  const float t = tanhf(kG0 * (x + kG1 * x * x * x));
+  return 0.5f * x * (1.f + t);
+}
+__device__ inline float dgelu(float x) {
+  const float t = tanhf(kG0 * (x + kG1 * x * x * x));
+  return 0.5f * (1.f + t) + 0.5f * x * (1.f - t * t) * kG0 * (1.f + 3.f * kG1 * x * x);
+}
+
+__device__ inline bool skip(const int* done) { return done && *done; }
+
+// ---------------------------------------------------------------------------
+// lift: h0[p][c] = a0 W_in[0][c] + a1 W_in[1][c] + a2 W_in[2][c] + b_in[c].
+// With nrm the input is v / *nrm (written back to vw when given), which lets
+// FGMRES hand over its lazily normalised Krylov vector.
+__global__ void __launch_bounds__(256) k_lift(const float* __restrict__ chi, float chi_scale,
+                                              const float2* v, const double* nrm, float2* vw, float s,
+                                              const float* __restrict__ P, bf16* __restrict__ h0,
+                                              long total, const int* done) {
+  if (skip(done)) return;
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= total) return;
+  float2 vv = v[p];
+  if (nrm) {
+    const double d = *nrm;
+    const float r = d > 0.0 ? (float)(1.0 / d) : 0.f;
+    vv.x *= r;
+    vv.y *= r;
+    if (vw) vw[p] = vv;
+  }
+  const float a0 = chi[p] * chi_scale, a1 = vv.x * s, a2 = vv.y * s;
+  bf16* o = h0 + p * C;
+#pragma unroll
+  for (int c = 0; c < C; c += 4) {
+    float r[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      r[k] = a0 * P[c + k] + a1 * P[C + c + k] + a2 * P[2 * C + c + k] + P[3 * C + c + k];
+    st4(o + c, r[0], r[1], r[2], r[3]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Row projection (one grid row x of one sample per iteration):
+//   out[r][a][c] = sum_y Xt[a][y] Y[r][y][c]                  (Xt: 32 x N basis)
+// and with WZ also the weight-gradient partials of this block
+//   part[blk][a][c] += sum_y Z[r][y][a] Y[r][y][c],  part[blk][C*C + c] += sum_y Y[r][y][c].
+// Wave w owns output tile (a-tile w >> 1, c-tile w & 1); Y (and Z) of the row
+// are staged in LDS (row stride S) and read back as MFMA B (and A) operands.
+template <bool WZ>
+__global__ void __launch_bounds__(256) k_rowproj(const bf16* __restrict__ Xt, const bf16* __restrict__ Y,
+                                                 const bf16* __restrict__ Z, float* __restrict__ out,
+                                                 float* __restrict__ part, int N, int rows,
+                                                 const int* done) {
+  if (skip(done)) return;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  bf16* ly = reinterpret_cast<bf16*>(smem_raw);
+  bf16* lz = ly + (long)N * S;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int i = lane & 15, g = lane >> 4;
+  const int ta = w >> 1, tc = w & 1;
+  f32x4 accz = {0.f, 0.f, 0.f, 0.f};
+  float accb = 0.f;
+  for (int r = blockIdx.x; r < rows; r += gridDim.x) {
+    const bf16* Yr = Y + (long)r * N * C;
+    const bf16* Zr = WZ ? Z + (long)r * N * C : nullptr;
+    __syncthreads();
+    for (int q = tid; q < N * 4; q += 256) {
+      const int y = q >> 2, c8 = (q & 3) * 8;
+      const u32x4 val = *(const u32x4*)(Yr + (long)y * C + c8);
+      u32x2* d = reinterpret_cast<u32x2*>(ly + y * S + c8);
+      d[0] = u32x2{val[0], val[1]};
+      d[1] = u32x2{val[2], val[3]};
+      if (WZ) {
+        const u32x4 vz = *(const u32x4*)(Zr + (long)y * C + c8);
+        u32x2* dz = reinterpret_cast<u32x2*>(lz + y * S + c8);
+        dz[0] = u32x2{vz[0], vz[1]};
+        dz[1] = u32x2{vz[2], vz[3]};
+      }
+    }
+    __syncthreads();
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int y0 = 0; y0 < N; y0 += 32) {
+      const bf16x8 a = ld8(Xt + (long)(16 * ta + i) * N + y0 + 8 * g);
+      bf16x8 b;
+#pragma unroll
+      for (int v = 0; v < 8; ++v) b[v] = ly[(y0 + 8 * g + v) * S + 16 * tc + i];
+      acc = mma(a, b, acc);
+      if (WZ) {
+        bf16x8 az;
+#pragma unroll
+        for (int v = 0; v < 8; ++v) az[v] = lz[(y0 + 8 * g + v) * S + 16 * ta + i];
+        accz = mma(az, b, accz);
+      }
+    }
+    float* o = out + (long)r * C * C;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) o[(16 * ta + 4 * g + v) * C + 16 * tc + i] = acc[v];
+    if (WZ)
+      for (int y = tid >> 5; y < N; y += 8) accb += (float)ly[y * S + (tid & 31)];
+  }
+  if (WZ) {
+    float* pp = part + (long)blockIdx.x * (C * C + C);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) pp[(16 * ta + 4 * g + v) * C + 16 * tc + i] = accz[v];
+    __shared__ float sb[256];
+    sb[tid] = accb;
+    __syncthreads();
+    if (tid < C) {
+      float sacc = 0.f;
+      for (int k = 0; k < 8; ++k) sacc += sb[tid + 32 * k];
+      pp[C * C + tid] = sacc;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// DFT along x at the retained kx, one chunk of XC rows per block:
+//   out[b][chunk][kxi][ky][c] = s_ky sum_{x in chunk} (in[b][x][2ky][c] + i in[b][x][2ky+1][c]) e^{-2 pi i kx x/N}
+// with s_ky = 1 (forward) or w_ky / N^2 (backward of the inverse transform).
+__global__ void __launch_bounds__(1024) k_dftx(const float* __restrict__ in, float2* __restrict__ outp,
+                                               const float2* __restrict__ twN, int N, int scaled,
+                                               const int* done) {
+  if (skip(done)) return;
+  __shared__ float2 sz[XC][C + 1];
+  const int chunk = blockIdx.x, ky = blockIdx.y, b = blockIdx.z;
+  const int nch = gridDim.x;
+  const int tid = threadIdx.x;
+  const int x0 = chunk * XC;
+  {
+    const int xl = tid >> 5, c = tid & 31;
+    const float* row = in + ((long)(b * N + x0 + xl) * C + 2 * ky) * C;
+    sz[xl][c] = make_float2(row[c], row[C + c]);
+  }
+  __syncthreads();
+  const int kxi = tid >> 5, c = tid & 31;
+  const int kx = kxi < 16 ? kxi : N - KX + kxi;
+  float2 acc = make_float2(0.f, 0.f);
+#pragma unroll 4
+  for (int xl = 0; xl < XC; ++xl) {
+    const float2 e = twN[(kx * (x0 + xl)) % N];
+    const float2 z = sz[xl][c];
+    acc.x += z.x * e.x + z.y * e.y;  // z * conj(e)
+    acc.y += z.y * e.x - z.x * e.y;
+  }
+  float s = 1.f;
+  if (scaled) s = (ky == 0 ? 1.f : 2.f) / ((float)N * (float)N);
+  outp[(((long)(b * nch + chunk) * KX + kxi) * KY + ky) * C + c] = make_float2(acc.x * s, acc.y * s);
+}
+
+// ---------------------------------------------------------------------------
+// Channel mixing per mode, one (ky, sample) per block:
+//   U[kxi][ch] = sum of the k_dftx partials             (saved to usave)
+//   forward : V[kxi][o] = s_ky sum_c U[kxi][c] W[kxi][ky][c][o],  s_ky = w_ky / N^2
+//   backward: V[kxi][c] = sum_o conj(W[kxi][ky][c][o]) U[kxi][o]
+// V goes to vout[b][kxi][ky][ch].
+template <bool BWD>
+__global__ void __launch_bounds__(1024) k_mix(const float2* __restrict__ upart, int nch,
+                                              const float2* __restrict__ W, float2* __restrict__ usave,
+                                              float2* __restrict__ vout, int N, const int* done) {
+  if (skip(done)) return;
+  __shared__ float2 su[KX][C + 1];
+  const int ky = blockIdx.x, b = blockIdx.y;
+  const int tid = threadIdx.x;
+  const int kxi = tid >> 5, ch = tid & 31;
+  float2 acc = make_float2(0.f, 0.f);
+  for (int k = 0; k < nch; ++k) {
+    const float2 u = upart[(((long)(b * nch + k) * KX + kxi) * KY + ky) * C + ch];
+    acc.x += u.x;
+    acc.y += u.y;
+  }
+  su[kxi][ch] = acc;
+  const long slot = (((long)b * KX + kxi) * KY + ky) * C + ch;
+  if (usave) usave[slot] = acc;
+  __syncthreads();
+  const float2* Wm = W + (long)(kxi * KY + ky) * C * C;
+  float2 v = make_float2(0.f, 0.f);
+  if (!BWD) {
+    for (int c = 0; c < C; ++c) {
+      const float2 u = su[kxi][c], wv = Wm[c * C + ch];
+      v.x += u.x * wv.x - u.y * wv.y;
+      v.y += u.x * wv.y + u.y * wv.x;
+    }
+    const float sc = (ky == 0 ? 1.f : 2.f) / ((float)N * (float)N);
+    v.x *= sc;
+    v.y *= sc;
+  } else {
+    for (int o = 0; o < C; ++o) {
+      const float2 u = su[kxi][o], wv = Wm[ch * C + o];
+      v.x += wv.x * u.x + wv.y * u.y;  // conj(w) * u
+      v.y += wv.x * u.y - wv.y * u.x;
+    }
+  }
+  vout[slot] = v;
+}
+
+// Inverse DFT along x for XB rows per block (thread = (ch, ky), ky fastest,
+// so the 16 ky pairs of one channel are 64 contiguous bytes of the output):
+//   out[b][x][ch][2ky], [2ky+1] = Re, Im of sum_kxi V[kxi][ky][ch] e^{+2 pi i kx x/N}   (BF16)
+constexpr int XB = 8;
+__global__ void __launch_bounds__(512) k_invx(const float2* __restrict__ V, bf16* __restrict__ out,
+                                              const float2* __restrict__ twN, int N, const int* done) {
+  if (skip(done)) return;
+  __shared__ float2 se[XB][KX];
+  const int x0 = blockIdx.x * XB, b = blockIdx.y;
+  const int tid = threadIdx.x;
+  if (tid < XB * KX) {
+    const int xl = tid / KX, q = tid % KX;
+    const int kx = q < 16 ? q : N - KX + q;
+    se[xl][q] = twN[(kx * (x0 + xl)) % N];
+  }
+  const int ch = tid >> 4, ky = tid & 15;
+  float2 vq[KX];
+#pragma unroll
+  for (int q = 0; q < KX; ++q) vq[q] = V[(((long)b * KX + q) * KY + ky) * C + ch];
+  __syncthreads();
+  for (int xl = 0; xl < XB; ++xl) {
+    float2 a = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int q = 0; q < KX; ++q) {
+      const float2 e = se[xl][q];
+      a.x += vq[q].x * e.x - vq[q].y * e.y;
+      a.y += vq[q].x * e.y + vq[q].y * e.x;
+    }
+    struct __align__(4) P2 { bf16 v[2]; } pr;
+    pr.v[0] = (bf16)a.x;
+    pr.v[1] = (bf16)a.y;
+    *reinterpret_cast<P2*>(out + (((long)b * N + x0 + xl) * C + ch) * C + 2 * ky) = pr;
+  }
+}
+
+// Spectral weight gradient: G[kxi][ky][c][o] = sum_b conj(U[b][kxi][ky][c]) dV[b][kxi][ky][o].
+__global__ void __launch_bounds__(256) k_specgrad(const float2* __restrict__ U, const float2* __restrict__ dV,
+                                                  float2* __restrict__ G, int B) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)KX * KY * C * C) return;
+  const int o = (int)(e % C);
+  const int c = (int)((e / C) % C);
+  const long mode = e / (C * C);  // kxi * KY + ky
+  float2 acc = make_float2(0.f, 0.f);
+  for (int b = 0; b < B; ++b) {
+    const float2 u = U[((long)b * KX * KY + mode) * C + c];
+    const float2 d = dV[((long)b * KX * KY + mode) * C + o];
+    acc.x += u.x * d.x + u.y * d.y;
+    acc.y += u.x * d.y - u.y * d.x;
+  }
+  G[e] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// Per-cell GEMM with K = 64 and fused epilogue:
+//   out[p][o] = epi( sum_c P[p][c] A[o][c] + sum_k Q[y][k] R[b][x][o][k] + bias[o] )
+// EPI 0: identity, 1: GELU (pre-activation to pre_out when given),
+// 2: times GELU'(aux[p][o]).  The product is formed transposed (rows = output
+// channels, columns = 16 cells) so that lane l holds 4 consecutive channels of
+// cell l % 16 in both the B operand and the accumulator.  One wave covers 64
+// consecutive cells of a grid row.
+template <int EPI>
+__global__ void __launch_bounds__(256) k_pixel(const bf16* __restrict__ P, const bf16* __restrict__ Q,
+                                               const bf16* __restrict__ A, const bf16* __restrict__ R,
+                                               const float* __restrict__ bias, bf16* __restrict__ pre_out,
+                                               const bf16* __restrict__ aux, bf16* __restrict__ out, int N,
+                                               long waves, const int* done) {
+  if (skip(done)) return;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const long wv = (long)blockIdx.x * 4 + (tid >> 6);
+  if (wv >= waves) return;
+  const int segs = N / 64;
+  const long bx = wv / segs;  // sample * N + x
+  const int y0 = (int)(wv % segs) * 64;
+  const int i = lane & 15, g = lane >> 4;
+  bf16x8 aw[2], ar[2];
+  float bs[2][4];
+#pragma unroll
+  for (int to = 0; to < 2; ++to) {
+    aw[to] = ld8(A + (16 * to + i) * C + 8 * g);
+    ar[to] = ld8(R + (bx * C + 16 * to + i) * C + 8 * g);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) bs[to][v] = bias ? bias[16 * to + 4 * g + v] : 0.f;
+  }
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int y = y0 + 16 * t + i;
+    const long p = bx * N + y;
+    const bf16x8 bp = ld8(P + p * C + 8 * g);
+    const bf16x8 bq = ld8(Q + (long)y * C + 8 * g);
+#pragma unroll
+    for (int to = 0; to < 2; ++to) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+      acc = mma(aw[to], bp, acc);
+      acc = mma(ar[to], bq, acc);
+      const int o0 = 16 * to + 4 * g;
+      float r[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) r[v] = acc[v] + bs[to][v];
+      if (EPI == 1) {
+        if (pre_out) st4(pre_out + p * C + o0, r[0], r[1], r[2], r[3]);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) r[v] = gelu(r[v]);
+      } else if (EPI == 2) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) r[v] *= dgelu((float)aux[p * C + o0 + v]);
+      }
+      st4(out + p * C + o0, r[0], r[1], r[2], r[3]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// projection: z[p] = s_id vv[p] + (sum_c h[p][c] W_out[c][j] + b_out[j]) / s
+__global__ void __launch_bounds__(256) k_proj(const bf16* __restrict__ h, const float* __restrict__ P,
+                                              const float2* __restrict__ v, float s, float2* __restrict__ z,
+                                              long total, const int* done) {
+  if (skip(done)) return;
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= total) return;
+  const float* W = P + kOffOut;
+  float o0 = W[2 * C], o1 = W[2 * C + 1];
+#pragma unroll
+  for (int c8 = 0; c8 < C; c8 += 8) {
+    const bf16x8 hv = ld8(h + p * C + c8);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const float x = (float)hv[k];
+      o0 += x * W[2 * (c8 + k)];
+      o1 += x * W[2 * (c8 + k) + 1];
+    }
+  }
+  const float2 vv = v[p];
+  const float is = 1.f / s, sid = P[kOffSkip];
+  z[p] = make_float2(sid * vv.x + o0 * is, sid * vv.y + o1 * is);
+}
+
+// Block sums of NV per-thread values (every thread of the 256 passes vals);
+// thread k < NV writes the block's sum of value k to dst[k].
+template <int NV>
+__device__ inline void block_sums(const float* vals, float* dst) {
+  __shared__ float red[4][NV];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    const float t = wave_sum_f(vals[k]);
+    if (lane == 0) red[w][k] = t;
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < NV; k += 256) dst[k] = (red[0][k] + red[1][k]) + (red[2][k] + red[3][k]);
+}
+
+// projection backward: dz = dL/dz (re, im packed) -> d = dz / s;
+//   dh[p][c] = d0 W_out[c][0] + d1 W_out[c][1]   (BF16)
+//   part[blk] = (sum_p h[p][c] d_j  [c][j],  sum_p d_j,  sum_p Re(conj(dz) vv))
+__global__ void __launch_bounds__(256) k_proj_bwd(const float2* __restrict__ dz, const bf16* __restrict__ h,
+                                                  const float* __restrict__ P, const float2* __restrict__ v,
+                                                  float s, bf16* __restrict__ dh, float* __restrict__ part,
+                                                  long total) {
+  const float* W = P + kOffOut;
+  float acc[2 * C + 3];
+#pragma unroll
+  for (int k = 0; k < 2 * C + 3; ++k) acc[k] = 0.f;
+  const float is = 1.f / s;
+  for (long p = (long)blockIdx.x * 256 + threadIdx.x; p < total; p += (long)gridDim.x * 256) {
+    const float2 d = dz[p];
+    const float d0 = d.x * is, d1 = d.y * is;
+#pragma unroll
+    for (int c8 = 0; c8 < C; c8 += 8) {
+      const bf16x8 hv = ld8(h + p * C + c8);
+      float r[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int c = c8 + k;
+        const float x = (float)hv[k];
+        acc[2 * c] += x * d0;
+        acc[2 * c + 1] += x * d1;
+        r[k] = d0 * W[2 * c] + d1 * W[2 * c + 1];
+      }
+      st4(dh + p * C + c8, r[0], r[1], r[2], r[3]);
+      st4(dh + p * C + c8 + 4, r[4], r[5], r[6], r[7]);
+    }
+    acc[2 * C] += d0;
+    acc[2 * C + 1] += d1;
+    const float2 vv = v[p];
+    acc[2 * C + 2] += d.x * vv.x + d.y * vv.y;
+  }
+  block_sums<2 * C + 3>(acc, part + (long)blockIdx.x * (2 * C + 3));
+}
+
+// lift backward: part[blk] = (sum_p a_i dh0[p][c]  [i][c],  sum_p dh0[p][c])
+__global__ void __launch_bounds__(256) k_lift_bwd(const bf16* __restrict__ dh0, const float* __restrict__ chi,
+                                                  float chi_scale, const float2* __restrict__ v, float s,
+                                                  float* __restrict__ part, long total) {
+  float acc[4 * C];
+#pragma unroll
+  for (int k = 0; k < 4 * C; ++k) acc[k] = 0.f;
+  for (long p = (long)blockIdx.x * 256 + threadIdx.x; p < total; p += (long)gridDim.x * 256) {
+    const float2 vv = v[p];
+    const float a[3] = {chi[p] * chi_scale, vv.x * s, vv.y * s};
+#pragma unroll
+    for (int c8 = 0; c8 < C; c8 += 8) {
+      const bf16x8 hv = ld8(dh0 + p * C + c8);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const float d = (float)hv[k];
+        acc[c8 + k] += a[0] * d;
+        acc[C + c8 + k] += a[1] * d;
+        acc[2 * C + c8 + k] += a[2] * d;
+        acc[3 * C + c8 + k] += d;
+      }
+    }
+  }
+  block_sums<4 * C>(acc, part + (long)blockIdx.x * 4 * C);
+}
+
+// out[k] = sum_g part[g * width + k]: 64 outputs per block, 16 strided
+// slices of g per output combined in a fixed order.
+__global__ void __launch_bounds__(1024) k_reduce_part(const float* __restrict__ part, int G, int width,
+                                                      float* __restrict__ out) {
+  __shared__ float red[16][65];
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int k = blockIdx.x * 64 + tx;
+  float s = 0.f;
+  if (k < width)
+    for (int gi = ty; gi < G; gi += 16) s += part[(long)gi * width + k];
+  red[ty][tx] = s;
+  __syncthreads();
+  if (ty == 0 && k < width) {
+    float t = 0.f;
+    for (int q = 0; q < 16; ++q) t += red[q][tx];
+    out[k] = t;
+  }
+}
+
+// GEMM weights to BF16: Wt[l][o][c] = W_l[c][o] (forward A operand),
+// Wn[l][c][o] = W_l[c][o] (backward A operand).
+__global__ void __launch_bounds__(256) k_prep(const float* __restrict__ P, bf16* __restrict__ Wt,
+                                              bf16* __restrict__ Wn) {
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= 4 * C * C) return;
+  const int l = e / (C * C), r = e % (C * C), c = r / C, o = r % C;
+  const float w = P[layer_off(l) + r];
+  Wn[e] = (bf16)w;
+  Wt[l * C * C + o * C + c] = (bf16)w;
+}
+
+}  // namespace fno
+}  // namespace cgm
+
+using namespace cgm;
+using namespace cgm::fno;
+
+struct cgm_fno {
+  cgm_context* ctx = nullptr;
+  int N = 0, B = 0;
+  long n = 0;
+  float chi_scale = 1.f;
+  const float* params = nullptr;  // caller's FP32 parameters (device)
+  bf16* Wt = nullptr;             // 4 x 32 x 32
+  bf16* Wn = nullptr;
+  bf16* Ft = nullptr;             // 32 x N
+  bf16* Fy = nullptr;             // N x 32
+  float2* twN = nullptr;          // e^{2 pi i t/N}
+  bf16* h[5] = {};                // activations [B][n][32]
+  bf16* pre[3] = {};              // pre-activations (training)
+  bf16* dh[2] = {};               // backward ping-pong
+  float* A = nullptr;             // [B][N][32][32]
+  float2* upart = nullptr;        // [B][N/XC][32][16][32]
+  float2* U[4] = {};              // [B][32][16][32] (training)
+  float2* dV = nullptr;           // [B][32][16][32]
+  float2* Vm = nullptr;           // [B][32][16][32] mixed modes
+  bf16* Bt = nullptr;             // [B][N][32][32]
+  float* part = nullptr;          // reduction partials
+  int Gp = 0;                     // blocks of the reducing kernels
+  // saved by the last training forward
+  int fwd_batch = 0;
+  const float* chi_in = nullptr;
+  const float2* v_in = nullptr;
+};
+
+namespace {
+
+int fno_free(cgm_fno* f) {
+  void* ptrs[] = {f->Wt, f->Wn, f->Ft, f->Fy, f->twN, f->h[0], f->h[1], f->h[2], f->h[3], f->h[4],
+                  f->pre[0], f->pre[1], f->pre[2], f->dh[0], f->dh[1], f->A, f->upart, f->U[0],
+                  f->U[1], f->U[2], f->U[3], f->dV, f->Vm, f->Bt, f->part};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  return CGM_SUCCESS;
+}
+
+int rowproj_grid(int rows) { return rows < 256 ? rows : 256; }
+
+template <bool WZ>
+int launch_rowproj(cgm_fno* f, hipStream_t st, const bf16* Y, const bf16* Z, float* out, int B,
+                   const int* done) {
+  const int rows = B * f->N;
+  const int G = WZ ? f->Gp : rows;
+  const size_t lds = (size_t)(WZ ? 2 : 1) * f->N * S * sizeof(bf16);
+  hipLaunchKernelGGL((k_rowproj<WZ>), dim3(G), dim3(256), lds, st, f->Ft, Y, Z, out, f->part, f->N, rows,
+                     done);
+  CGM_HIP(hipGetLastError());
+  return CGM_SUCCESS;
+}
+
+template <int EPI>
+int launch_pixel(cgm_fno* f, hipStream_t st, const bf16* P, const bf16* A, const bf16* R, const float* bias,
+                 bf16* pre_out, const bf16* aux, bf16* out, int B, const int* done) {
+  const long waves = (long)B * f->n / 64;
+  const int blocks = (int)((waves + 3) / 4);
+  hipLaunchKernelGGL((k_pixel<EPI>), dim3(blocks), dim3(256), 0, st, P, f->Fy, A, R, bias, pre_out, aux, out,
+                     f->N, waves, done);
+  CGM_HIP(hipGetLastError());
+  return CGM_SUCCESS;
+}
+
+// spectral path of one layer: A (row projections) -> Bt (per-row mixing matrices)
+int spectral(cgm_fno* f, hipStream_t st, const float2* W, float2* usave, bool bwd, int B, const int* done) {
+  const int nch = f->N / XC;
+  hipLaunchKernelGGL(k_dftx, dim3(nch, KY, B), dim3(1024), 0, st, f->A, f->upart, f->twN, f->N, bwd ? 1 : 0,
+                     done);
+  if (!bwd)
+    hipLaunchKernelGGL((k_mix<false>), dim3(KY, B), dim3(1024), 0, st, f->upart, nch, W, usave, f->Vm, f->N,
+                       done);
+  else
+    hipLaunchKernelGGL((k_mix<true>), dim3(KY, B), dim3(1024), 0, st, f->upart, nch, W, usave, f->Vm, f->N, done);
+  hipLaunchKernelGGL(k_invx, dim3(f->N / XB, B), dim3(512), 0, st, f->Vm, f->Bt, f->twN, f->N, done);
+  CGM_HIP(hipGetLastError());
+  return CGM_SUCCESS;
+}
+
+}  // namespace
+
+namespace cgm {
+
+// Forward pass on B unit-norm vectors (or one lazily normalised vector with
+// nrm/vw).  train keeps activations for fno_backward.
+int fno_forward(cgm_fno* f, hipStream_t st, const float* chi, const float2* v, const double* nrm, float2* vw,
+                float2* z, int B, bool train, const int* done) {
+  const long total = (long)B * f->n;
+  const float s = (float)f->N;
+  int rc;
+  hipLaunchKernelGGL(k_lift, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, chi, f->chi_scale,
+                     v, nrm, vw, s, f->params, f->h[0], total, done);
+  CGM_HIP(hipGetLastError());
+  for (int l = 0; l < 4; ++l) {
+    const float* Lp = f->params + layer_off(l);
+    if ((rc = launch_rowproj<false>(f, st, f->h[l], nullptr, f->A, B, done))) return rc;
+    if ((rc = spectral(f, st, (const float2*)(Lp + C * C + C), train ? f->U[l] : nullptr, false, B, done)))
+      return rc;
+    const bf16* Wl = f->Wt + l * C * C;
+    if (l < 3)
+      rc = launch_pixel<1>(f, st, f->h[l], Wl, f->Bt, Lp + C * C, train ? f->pre[l] : nullptr, nullptr,
+                           f->h[l + 1], B, done);
+    else
+      rc = launch_pixel<0>(f, st, f->h[l], Wl, f->Bt, Lp + C * C, nullptr, nullptr, f->h[4], B, done);
+    if (rc) return rc;
+  }
+  const float2* vv = vw ? vw : v;
+  hipLaunchKernelGGL(k_proj, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, f->h[4], f->params, vv, s,
+                     z, total, done);
+  CGM_HIP(hipGetLastError());
+  if (train) {
+    f->fwd_batch = B;
+    f->chi_in = chi;
+    f->v_in = vv;
+  }
+  return CGM_SUCCESS;
+}
+
+cgm_context* fno_context(const cgm_fno* f) { return f->ctx; }
+
+}  // namespace cgm
+
+static int fno_backward(cgm_fno* f, const float2* dz, float* grad) {
+  hipStream_t st = f->ctx->stream;
+  const int B = f->fwd_batch;
+  const long total = (long)B * f->n;
+  const float s = (float)f->N;
+  int rc;
+  const int Gr = 256;
+  // projection
+  hipLaunchKernelGGL(k_proj_bwd, dim3(Gr), dim3(256), 0, st, dz, f->h[4], f->params, f->v_in, s, f->dh[0], f->part,
+                     total);
+  hipLaunchKernelGGL(k_reduce_part, dim3(2), dim3(1024), 0, st, f->part, Gr, 2 * C + 3, grad + kOffOut);
+  CGM_HIP(hipGetLastError());
+  // layer 3 has no activation: dPre_3 = dh4
+  int cur = 0;
+  for (int l = 3; l >= 0; --l) {
+    const float* Lp = f->params + layer_off(l);
+    float* Lg = grad + layer_off(l);
+    const bf16* dpre = f->dh[cur];
+    // g_B rows (basis F), dW_l and db_l partials
+    if ((rc = launch_rowproj<true>(f, st, dpre, f->h[l], f->A, B, nullptr))) return rc;
+    hipLaunchKernelGGL(k_reduce_part, dim3((C * C + C + 63) / 64), dim3(1024), 0, st, f->part, f->Gp, C * C + C,
+                       Lg);
+    // spectral adjoint: dV (saved), dA rows into Bt
+    if ((rc = spectral(f, st, (const float2*)(Lp + C * C + C), f->dV, true, B, nullptr))) return rc;
+    hipLaunchKernelGGL(k_specgrad, dim3(KX * KY * C * C / 256), dim3(256), 0, st, f->U[l], f->dV,
+                       (float2*)(Lg + C * C + C), B);
+    CGM_HIP(hipGetLastError());
+    const bf16* Wl = f->Wn + l * C * C;
+    if (l > 0)
+      rc = launch_pixel<2>(f, st, dpre, Wl, f->Bt, nullptr, nullptr, f->pre[l - 1], f->dh[cur ^ 1], B, nullptr);
+    else
+      rc = launch_pixel<0>(f, st, dpre, Wl, f->Bt, nullptr, nullptr, nullptr, f->dh[cur ^ 1], B, nullptr);
+    if (rc) return rc;
+    cur ^= 1;
+  }
+  hipLaunchKernelGGL(k_lift_bwd, dim3(Gr), dim3(256), 0, st, f->dh[cur], f->chi_in, f->chi_scale, f->v_in, s,
+                     f->part, total);
+  hipLaunchKernelGGL(k_reduce_part, dim3(2), dim3(1024), 0, st, f->part, Gr, 4 * C, grad);
+  CGM_HIP(hipGetLastError());
+  return CGM_SUCCESS;
+}
+
+extern "C" int cgm_fno_num_params(size_t* n) {
+  CGM_CHECK(n, CGM_ERROR_INVALID_VALUE, "null argument");
+  *n = (size_t)kNumParams;
+  return CGM_SUCCESS;
+}
+
+extern "C" int cgm_fno_create(cgm_ctx_t c, int batch, float chi_scale, cgm_fno_t* out) {
+  CGM_CHECK(c && out, CGM_ERROR_INVALID_VALUE, "null argument");
+  *out = nullptr;
+  CGM_CHECK(batch >= 1, CGM_ERROR_INVALID_VALUE, "batch %d < 1", batch);
+  const int N = c->N;
+  CGM_CHECK(N >= 64 && N <= 1024 && N % 64 == 0, CGM_ERROR_UNSUPPORTED,
+            "the FNO preconditioner needs 64 <= N <= 1024 with N %% 64 == 0 (N = %d)", N);
+  CGM_HIP(hipSetDevice(c->device));
+  cgm_fno* f = new cgm_fno();
+  f->ctx = c;
+  f->N = N;
+  f->B = batch;
+  f->n = (long)N * N;
+  f->chi_scale = chi_scale;
+  const size_t act = sizeof(bf16) * (size_t)batch * f->n * C;
+  int rc = CGM_SUCCESS;
+  auto alloc = [&](void** p, size_t bytes) {
+    if (rc) return;
+    if (hipMalloc(p, bytes) != hipSuccess) {
+      cgm_set_last_error("cgm_fno_create: hipMalloc of %zu bytes failed", bytes);
+      rc = CGM_ERROR_ALLOC;
+    }
+  };
+  alloc((void**)&f->Wt, sizeof(bf16) * 4 * C * C);
+  alloc((void**)&f->Wn, sizeof(bf16) * 4 * C * C);
+  alloc((void**)&f->Ft, sizeof(bf16) * C * N);
+  alloc((void**)&f->Fy, sizeof(bf16) * C * N);
+  alloc((void**)&f->twN, sizeof(float2) * N);
+  for (int l = 0; l < 5; ++l) alloc((void**)&f->h[l], act);
+  for (int l = 0; l < 3; ++l) alloc((void**)&f->pre[l], act);
+  for (int l = 0; l < 2; ++l) alloc((void**)&f->dh[l], act);
+  alloc((void**)&f->A, sizeof(float) * (size_t)batch * N * C * C);
+  alloc((void**)&f->upart, sizeof(float2) * (size_t)batch * (N / XC) * KX * KY * C);
+  for (int l = 0; l < 4; ++l) alloc((void**)&f->U[l], sizeof(float2) * (size_t)batch * KX * KY * C);
+  alloc((void**)&f->dV, sizeof(float2) * (size_t)batch * KX * KY * C);
+  alloc((void**)&f->Vm, sizeof(float2) * (size_t)batch * KX * KY * C);
+  alloc((void**)&f->Bt, sizeof(bf16) * (size_t)batch * N * C * C);
+  f->Gp = rowproj_grid(batch * N);
+  size_t pw = (size_t)f->Gp * (C * C + C);
+  if (pw < (size_t)256 * 4 * C) pw = (size_t)256 * 4 * C;
+  alloc((void**)&f->part, sizeof(float) * pw);
+  if (rc) {
+    fno_free(f);
+    delete f;
+    return rc;
+  }
+  // column basis and row twiddles (computed in double, stored rounded)
+  std::vector<bf16> ft((size_t)C * N), fy((size_t)C * N);
+  std::vector<float2> tw(N);
+  for (int y = 0; y < N; ++y) {
+    for (int ky = 0; ky < KY; ++ky) {
+      const double th = 2.0 * M_PI * (double)((long)ky * y % N) / N;
+      const float cs = (float)cos(th), sn = (float)-sin(th);
+      ft[(size_t)(2 * ky) * N + y] = (bf16)cs;
+      ft[(size_t)(2 * ky + 1) * N + y] = (bf16)sn;
+      fy[(size_t)y * C + 2 * ky] = (bf16)cs;
+      fy[(size_t)y * C + 2 * ky + 1] = (bf16)sn;
+    }
+    const double th = 2.0 * M_PI * y / N;
+    tw[y] = make_float2((float)cos(th), (float)sin(th));
+  }
+  CGM_HIP(hipMemcpy(f->Ft, ft.data(), sizeof(bf16) * ft.size(), hipMemcpyHostToDevice));
+  CGM_HIP(hipMemcpy(f->Fy, fy.data(), sizeof(bf16) * fy.size(), hipMemcpyHostToDevice));
+  CGM_HIP(hipMemcpy(f->twN, tw.data(), sizeof(float2) * N, hipMemcpyHostToDevice));
+  const size_t lds2 = (size_t)2 * N * S * sizeof(bf16);
+  CGM_HIP(hipFuncSetAttribute((const void*)k_rowproj<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2));
+  CGM_HIP(hipFuncSetAttribute((const void*)k_rowproj<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)(lds2 / 2)));
+  *out = f;
+  return CGM_SUCCESS;
+}
+
+extern "C" int cgm_fno_destroy(cgm_fno_t f) {
+  if (!f) return CGM_SUCCESS;
+  (void)hipSetDevice(f->ctx->device);
+  fno_free(f);
+  delete f;
+  return CGM_SUCCESS;
+}
+
+extern "C" int cgm_fno_set_params(cgm_fno_t f, const float* params) {
+  CGM_CHECK(f && params, CGM_ERROR_INVALID_VALUE, "null argument");
+  CGM_CHECK(((uintptr_t)params & 15) == 0, CGM_ERROR_INVALID_VALUE, "parameters must be 16-byte aligned");
+  CGM_HIP(hipSetDevice(f->ctx->device));
+  f->params = params;
+  hipLaunchKernelGGL(k_prep, dim3(4 * C * C / 256), dim3(256), 0, f->ctx->stream, params, f->Wt, f->Wn);
+  CGM_HIP(hipGetLastError());
+  return CGM_SUCCESS;
+}
+
+extern "C" int cgm_fno_forward(cgm_fno_t f, const float* chi, const cgm_complex64* v, cgm_complex64* z, int batch,
+                               int train) {
+  CGM_CHECK(f && chi && v && z, CGM_ERROR_INVALID_VALUE, "null argument");
+  CGM_CHECK(f->params, CGM_ERROR_INVALID_VALUE, "cgm_fno_set_params has not been called");
+  CGM_CHECK(batch >= 1 && batch <= f->B, CGM_ERROR_INVALID_VALUE, "batch %d outside [1, %d]", batch, f->B);
+  CGM_CHECK((const void*)v != (const void*)z, CGM_ERROR_INVALID_VALUE, "v and z must not alias");
+  CGM_HIP(hipSetDevice(f->ctx->device));
+  return fno_forward(f, f->ctx->stream, chi, (const float2*)v, nullptr, nullptr, (float2*)z, batch, train != 0,
+                     nullptr);
+}
+
+extern "C" int cgm_fno_backward(cgm_fno_t f, const cgm_complex64* dz, float* grad) {
+  CGM_CHECK(f && dz && grad, CGM_ERROR_INVALID_VALUE, "null argument");
+  CGM_CHECK(f->fwd_batch > 0, CGM_ERROR_INVALID_VALUE, "cgm_fno_backward needs a preceding training forward");
+  CGM_HIP(hipSetDevice(f->ctx->device));
+  return fno_backward(f, (const float2*)dz, grad);
+}

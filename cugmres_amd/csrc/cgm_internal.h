@@ -115,9 +115,21 @@ int launch_norm_init(hipStream_t st, const KrylovWs& ws, const float2* r, DevSta
                      double tol);
 int launch_cgs2_step(hipStream_t st, const KrylovWs& ws, int i, DevState* s, double* hist,
                      int part = -1);
-int launch_backsub_update(hipStream_t st, const KrylovWs& ws, DevState* s, float2* x);
+// x += basis y (basis = ws.V when null; FGMRES passes its Z basis)
+int launch_backsub_update(hipStream_t st, const KrylovWs& ws, DevState* s, float2* x,
+                          const float2* basis = nullptr);
 int launch_norm64(hipStream_t st, const double2* v, long n, double* partn, int G, DevState* s);
 
+}  // namespace cgm
+
+struct cgm_context;
+struct cgm_fno;
+namespace cgm {
+// FNO preconditioner (cgm_fno.hip): z = P(v) for B vectors; with nrm the
+// input is v / *nrm, written back to vw.  Every launch is skipped once *done.
+int fno_forward(cgm_fno* f, hipStream_t st, const float* chi, const float2* v, const double* nrm, float2* vw,
+                float2* z, int B, bool train, const int* done);
+cgm_context* fno_context(const cgm_fno* f);
 }  // namespace cgm
 
 struct cgm_context {
@@ -139,4 +151,6 @@ struct cgm_context {
   float2* vec_tmp = nullptr;     // right-hand sides built on the device (N*N)
   double2* v64a = nullptr;       // refinement residual (N*N, lazy)
   float* chi32 = nullptr;        // refinement contrast in float (N*N, lazy)
+  float2* zb = nullptr;          // FGMRES preconditioned basis (zb_m * N*N, lazy)
+  int zb_m = 0;
 };

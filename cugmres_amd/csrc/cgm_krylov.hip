@@ -411,9 +411,9 @@ int launch_cgs2_step(hipStream_t st, const KrylovWs& ws, int i, DevState* s, dou
   return CGM_SUCCESS;
 }
 
-int launch_backsub_update(hipStream_t st, const KrylovWs& ws, DevState* s, float2* x) {
+int launch_backsub_update(hipStream_t st, const KrylovWs& ws, DevState* s, float2* x, const float2* basis) {
   hipLaunchKernelGGL(k_backsub, dim3(1), dim3(kKB), 0, st, ws, s, ws.m);
-  hipLaunchKernelGGL(k_xupd, dim3(ws.G), dim3(kKB), 0, st, ws.V, ws.n, x, ws.y, s, ws.G);
+  hipLaunchKernelGGL(k_xupd, dim3(ws.G), dim3(kKB), 0, st, basis ? basis : ws.V, ws.n, x, ws.y, s, ws.G);
   CGM_HIP(hipGetLastError());
   return CGM_SUCCESS;
 }

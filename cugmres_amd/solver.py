@@ -150,6 +150,32 @@ class Context:
             info["history"] = hist[: r.iterations + 1].cpu().numpy()
         return x, info
 
+    def fgmres(self, gamma, chi, b, x=None, precond=None, restart=30, tol=1e-6, max_cycles=100,
+               stall_ratio=0.5, history=False):
+        """Right-preconditioned flexible GMRES(restart) with the learned
+        preconditioner `precond` (cugmres_amd.fno.Preconditioner; None = the
+        identity).  Plain GMRES takes over when a preconditioned cycle leaves
+        more than stall_ratio of the residual.  Returns (x, info dict)."""
+        t = self.torch
+        x0_zero = x is None
+        if x is None:
+            x = t.zeros_like(b)
+        hist = None
+        if history:
+            hist = t.zeros(restart * max(max_cycles, 1) + 1, dtype=t.float64, device=self.device)
+        p = _lib.FgmresParams(int(restart), int(max_cycles), float(tol), int(x0_zero), float(stall_ratio))
+        r = _lib.FgmresResult()
+        net = precond.handle if precond is not None else None
+        st = lib().cgm_fgmres(self._ptr, _ptr(gamma), _ptr(chi), _ptr(b), _ptr(x), net, _c.byref(p),
+                              _c.byref(r), _ptr(hist))
+        check(st, "cgm_fgmres", ok=(_lib.SUCCESS, _lib.ERROR_NOT_CONVERGED))
+        info = dict(iterations=r.iterations, cycles=r.cycles, converged=bool(r.converged),
+                    rel_residual=r.rel_residual, beta0=r.beta0, bnorm=r.bnorm,
+                    precond_cycles=r.precond_cycles, fell_back=bool(r.fell_back))
+        if history:
+            info["history"] = hist[: r.iterations + 1].cpu().numpy()
+        return x, info
+
     def gmres_refine(self, gamma32, gamma64, chi64, b, x=None, inner_restart=50, inner_tol=1e-4,
                      inner_max_cycles=40, outer_tol=1e-12, max_outer=30):
         t = self.torch

@@ -184,6 +184,70 @@ int cgm_gmres_refine(cgm_ctx_t ctx, const cgm_complex64* dev_gamma,
                      const cgm_complex128* dev_b, cgm_complex128* dev_x,
                      const cgm_refine_params* params, cgm_refine_result* result);
 
+/* ---- learned right preconditioner and flexible GMRES ----------------------- */
+/* No counterpart in the reference (B/CUDA_GMRES/GMRES.cuh:22-446 runs plain
+ * GMRES without a preconditioner).  A Fourier neural operator P with 4
+ * spectral layers, 32 channels and 16 x 16 retained modes (kx in
+ * {0..15, N-16..N-1}, ky in {0..15}) maps a unit-norm vector v and the
+ * contrast map chi to P(v) = s_id v + FNO(chi * chi_scale, N v) / N.  BF16
+ * activations and MFMA GEMMs, FP32 accumulation and spectral mixing.
+ * Parameters are one FP32 device array of cgm_fno_num_params() values:
+ *   W_in [3][32], b_in [32],
+ *   per layer l = 0..3: W_l [32 in][32 out], b_l [32],
+ *                       Wspec_l [32 kx][16 ky][32 in][32 out] complex (re, im),
+ *   W_out [32][2], b_out [2], s_id (learned weight of the identity path). */
+typedef struct cgm_fno* cgm_fno_t;
+
+int cgm_fno_num_params(size_t* count);
+/* Workspace for up to `batch` vectors at the context's N (64 <= N <= 1024,
+ * N % 64 == 0). */
+int cgm_fno_create(cgm_ctx_t ctx, int batch, float chi_scale, cgm_fno_t* net);
+int cgm_fno_destroy(cgm_fno_t net);
+/* Binds the parameter array (device, 16-byte aligned, kept by reference: it
+ * must outlive the calls that use it) and converts the GEMM weights to BF16.
+ * Call again after the parameters change. */
+int cgm_fno_set_params(cgm_fno_t net, const float* dev_params);
+/* z[b] = P(v[b]) for b < batch; v is [batch][N*N] unit-norm vectors, chi is
+ * [batch][N*N].  train = 1 keeps the activations for cgm_fno_backward (chi
+ * and v must then stay unchanged until it runs).  v and z must not alias. */
+int cgm_fno_forward(cgm_fno_t net, const float* dev_chi, const cgm_complex64* dev_v,
+                    cgm_complex64* dev_z, int batch, int train);
+/* Gradient of a real loss L with respect to every parameter, given
+ * dev_dz[b][p] = dL/dRe z + i dL/dIm z of the last training forward.
+ * dev_grad has the parameter layout and is overwritten. */
+int cgm_fno_backward(cgm_fno_t net, const cgm_complex64* dev_dz, float* dev_grad);
+
+typedef struct {
+  int restart;        /* Krylov dimension per cycle m (1..512)                     */
+  int max_cycles;     /* restart cap                                              */
+  double tol;         /* stop when the true ||b - A x|| <= tol ||b||              */
+  int x0_zero;        /* 1: start from x = 0                                      */
+  double stall_ratio; /* a preconditioned cycle that leaves more than this
+                         fraction of the residual switches the rest of the
+                         solve to plain GMRES (<= 0: never)                       */
+} cgm_fgmres_params;
+
+typedef struct {
+  int iterations;        /* Arnoldi steps in total (both phases)                 */
+  int cycles;            /* restart cycles in total                              */
+  int converged;
+  double rel_residual;   /* true relative residual at exit                       */
+  double beta0;
+  double bnorm;
+  int precond_cycles;    /* cycles run with the preconditioner                   */
+  int fell_back;         /* 1 when the stall rule switched to plain GMRES        */
+} cgm_fgmres_result;
+
+/* Right-preconditioned flexible GMRES(m) (Saad, SIAM J. Sci. Comput. 14
+ * (1993) 461-469): z_j = P(v_j), w = A z_j, CGS2 against V, x += Z y.  Plain
+ * GMRES (cgm_gmres's kernels) is the fallback and the judge: every cycle
+ * starts from the true residual b - A x.  net = NULL applies the identity
+ * (then the iterates equal cgm_gmres with the two-pass CGS2 kernels).
+ * dev_hist as in cgm_gmres. */
+int cgm_fgmres(cgm_ctx_t ctx, const cgm_complex64* dev_gamma, const float* dev_chi,
+               const cgm_complex64* dev_b, cgm_complex64* dev_x, cgm_fno_t net,
+               const cgm_fgmres_params* params, cgm_fgmres_result* result, double* dev_hist);
+
 /* ---- reference-compatible entry points ------------------------------------ */
 /* The reference's plane wave exp(-i k i) with k = 2*3.14f/(N/6.f)
  * (B/CUDA_GMRES/GMRES_kernels.cuh:12,557-572), on the device. */
