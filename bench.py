@@ -1,20 +1,25 @@
-"""Benchmark of the GMRES solve on one MI355X.
+"""Benchmark of the GMRES solve on one MI355X: learned-FGMRES vs plain GMRES.
 
-Workload (one "step" = one solve): the plain GMRES workload at N = 512
-(262,144 unknowns, 1024 x 1024 FFT embedding), complex64, restarted
-GMRES(30), tolerance 1e-6 on the true relative residual, x0 = 0.  The
-permittivity map is a seeded binary map of 8 x 8-cell blocks with values
-{1, 12.1} over the central 75 % of the area, h = lambda/20, and the plane
-wave comes in at a seeded angle.  Step k solves seed k.  All inputs are
-generated and copied to HBM before the timed region.
+Workload (one "step" = one solve with each method): the plain GMRES workload at
+N = 512 (262,144 unknowns, 1024 x 1024 FFT embedding), complex64, tolerance
+1e-6 on the true relative residual, x0 = 0.  The permittivity map is a seeded
+binary map of 8 x 8-cell blocks with values {1, 12.1} over the central 75 % of
+the area, h = lambda/20, and the plane wave comes in at a seeded angle.  Step
+k solves seed k with
+  - plain GMRES(30) (fused lagged-CGS2 kernels), and
+  - flexible GMRES(30) right-preconditioned by the FNO network, trained on the
+    GPU at N = 256 before the timed region (or loaded with --fno-params).
+All inputs are generated and copied to HBM before the timed region.
 
-A solve stops at the tolerance or after --max-cycles restart cycles; both
-the number of converged solves and the achieved residuals are printed, so
-a capped run is never reported as converged.
+A solve stops at the tolerance or after --max-cycles restart cycles; the
+number of converged solves, the achieved residuals and the wall time each
+method needs to reach a common residual level are printed, so a capped run
+is never reported as converged.
 
-Printed: one JSON line (rank 0) with the metric, roofline of the dominant
-kernel (measured live with HIP events on the solver's stream) and the CPU
-baseline (the oracle's complex64 GMRES(30) restatement timed on the host).
+Printed: one JSON line with the metric (value = learned-FGMRES solves/s), the
+plain GMRES figures beside it, the roofline of the learned path's dominant
+kernel (HIP events on the solver's stream) and the CPU baseline (the oracle's
+complex64 GMRES(30) restatement timed on the host).
 """
 import argparse
 import ctypes
@@ -43,6 +48,13 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=15.0,
                    help="budget of the CPU-baseline sample")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--fno-params", default=None, help="trained network (.npz from cugmres_amd.learn)")
+    p.add_argument("--train-samples", type=int, default=16000)
+    p.add_argument("--train-N", type=int, default=256)
+    p.add_argument("--stall-ratio", type=float, default=0.0,
+                   help="FGMRES falls back to plain GMRES when a cycle leaves more than this "
+                        "fraction of the residual (0: never)")
+    p.add_argument("--plain-only", action="store_true")
     p.add_argument("--mode", choices=["bench", "step", "copy"], default="bench",
                    help="bench: the benchmark; step: only the per-kernel profile of one Arnoldi "
                         "step (for rocprofv3 --pmc); copy: 3 copies of 256 MiB (counter calibration)")
@@ -78,38 +90,108 @@ def main():
                                a.restart)
         print(json.dumps(prof))
         return
-    stream = torch.cuda.current_stream()
     g = ctx.spectrum_helmholtz(problems.KH)
     seeds = list(range(a.warmup + a.steps))
     inputs = []
-    for s in seeds:
-        chi, b, theta = problems.config_plain(N, seed=s)
+    for s_ in seeds:
+        chi, b, theta = problems.config_plain(N, seed=s_)
         inputs.append((ctx.to_device(chi.reshape(-1), "f32"), ctx.to_device(b.reshape(-1)), theta))
     x = torch.zeros(N * N, dtype=torch.complex64, device=ctx.device)
+
+    # ---- the learned preconditioner (trained before the timed region) -------
+    net, training = None, None
+    if not a.plain_only:
+        from cugmres_amd import fno, learn
+        if a.fno_params:
+            params = fno.load(a.fno_params)
+            training = {"loaded": a.fno_params}
+        else:
+            tctx = solver.Context(a.train_N, 0)
+            t = time.perf_counter()
+            tnet, hist = learn.train(tctx, samples=a.train_samples, batch=8, lr=1e-3, seed=0)
+            torch.cuda.synchronize()
+            params = tnet.params.cpu().numpy()
+            training = {"N": a.train_N, "samples": a.train_samples, "batch": 8, "seconds": time.perf_counter() - t,
+                        "loss_first": hist[0][1], "loss_last_100_mean": float(np.mean([h[1] for h in hist[-100:]])),
+                        "kernels": "cgm_fno_forward/backward (BF16 MFMA)"}
+            tnet.close()
+            tctx.close()
+        net = fno.Preconditioner(ctx, params=params)
     torch.cuda.synchronize()
 
-    def solve(k):
+    def plain(k):
         dchi, db, _ = inputs[k]
         x.zero_()
-        return ctx.gmres(g, dchi, db, x, restart=a.restart, tol=a.tol, max_cycles=a.max_cycles)[1]
+        return ctx.gmres(g, dchi, db, x, restart=a.restart, tol=a.tol, max_cycles=a.max_cycles,
+                         history=True)[1]
 
-    for k in range(a.warmup):
-        solve(k)
-    torch.cuda.synchronize()
-    infos = []
-    t0 = time.perf_counter()
-    for k in range(a.warmup, a.warmup + a.steps):
-        infos.append(solve(k))
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    wall = t1 - t0
-    ms_per_step = wall / a.steps * 1e3
-    iters = [i["iterations"] for i in infos]
-    conv = [i["converged"] for i in infos]
-    resid = [i["rel_residual"] for i in infos]
+    def learned(k):
+        dchi, db, _ = inputs[k]
+        x.zero_()
+        return ctx.fgmres(g, dchi, db, x, precond=net, restart=a.restart, tol=a.tol,
+                          max_cycles=a.max_cycles, stall_ratio=a.stall_ratio, history=True)[1]
 
-    # ---- roofline of the dominant kernel ---------------------------------
+    def run(fn):
+        for k in range(a.warmup):
+            fn(k)
+        torch.cuda.synchronize()
+        infos = []
+        t0 = time.perf_counter()
+        for k in range(a.warmup, a.warmup + a.steps):
+            infos.append(fn(k))
+        torch.cuda.synchronize()
+        return infos, time.perf_counter() - t0
+
+    p_infos, p_wall = run(plain)
+    l_infos, l_wall = (run(learned) if net is not None else (None, None))
+
+    def summary(infos, wall):
+        its = [i["iterations"] for i in infos]
+        return {"solves_per_s": a.steps / wall, "ms_per_solve": wall / a.steps * 1e3,
+                "converged": int(sum(i["converged"] for i in infos)), "of": len(infos),
+                "iterations": its, "achieved_rel_residual": [i["rel_residual"] for i in infos],
+                "us_per_arnoldi_step": wall / max(1, sum(its)) * 1e6}
+
+    plain_s = summary(p_infos, p_wall)
+    out_learned = summary(l_infos, l_wall) if l_infos else None
+    if out_learned:
+        out_learned["precond_cycles"] = [i["precond_cycles"] for i in l_infos]
+        out_learned["fell_back"] = [i["fell_back"] for i in l_infos]
+        # wall time to a common residual: the worst residual plain GMRES ends at
+        level = max(plain_s["achieved_rel_residual"])
+        def t_to(infos, us_per_it):
+            res = []
+            for i in infos:
+                h = i["history"] / i["bnorm"]
+                hit = np.nonzero(h <= level)[0]
+                res.append(float(hit[0]) * us_per_it * 1e-3 if hit.size else None)
+            return res
+        tp = t_to(p_infos, plain_s["us_per_arnoldi_step"])
+        tl = t_to(l_infos, out_learned["us_per_arnoldi_step"])
+        ok = [(u, v) for u, v in zip(tp, tl) if u is not None and v is not None]
+        matched = {"level": level, "plain_ms": tp, "learned_ms": tl,
+                   "speedup": (sum(u for u, _ in ok) / sum(v for _, v in ok)) if ok else None}
+    else:
+        matched = None
+
+    # ---- roofline of the dominant kernel of the timed path -------------------
     prof = profile_kernels(ctx, g, inputs[0][0], inputs[0][1], a.restart)
+    fprof = profile_fno(ctx, net, inputs[0][0], inputs[0][1]) if net is not None else None
+    roof = prof["roofline"]
+    if fprof is not None:
+        # per FGMRES step: one forward pass (4 row projections, 4 cell GEMMs, ...),
+        # one matvec, one two-pass CGS2 step; the heaviest kernel class by time
+        per_step = {k: v["us"] * v["per_step"] for k, v in fprof["kernels"].items() if "per_step" in v}
+        for k in ("pass1_row_fft", "pass2_col_fft_mul", "pass3_row_ifft", "cgs2_dots",
+                  "cgs2_update_dots", "cgs2_update_norm_givens"):
+            per_step[k] = prof["kernels"][k]["us"]
+        dom = max(per_step, key=per_step.get)
+        src = fprof["kernels"] if dom in fprof["kernels"] else prof["kernels"]
+        ach = src[dom]["GBps"]
+        roof = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                "per_unit": f"{src[dom]['bytes']} B per launch at N={N}",
+                "us_per_launch": src[dom]["us"], "us_per_fgmres_step": per_step}
 
     # ---- the reference's own problem at the same N ------------------------
     refp = reference_problem(ctx, a.restart) if N == 512 else None
@@ -117,34 +199,73 @@ def main():
     # ---- CPU baseline ------------------------------------------------------
     cpu = None
     if not a.no_cpu:
-        cpu = cpu_baseline(N, a.restart, a.cpu_seconds, np.mean([i["cycles"] for i in infos]))
+        cpu = cpu_baseline(N, a.restart, a.cpu_seconds, np.mean([i["cycles"] for i in p_infos]))
 
+    head = out_learned if out_learned else plain_s
     out = {
         "metric": METRIC,
-        "value": a.steps / wall,
+        "value": head["solves_per_s"],
         "unit": "solves/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": head["ms_per_solve"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "c64",
         "data": "synthetic (seeded eps maps and plane waves)",
-        "config": {"workload": f"plain GMRES({a.restart}) N={N}, 8x8 blocks eps in {{1,12.1}} over "
-                               f"75% area, h=lambda/20, tol {a.tol:g}, cap {a.max_cycles} cycles",
+        "config": {"workload": f"{'learned FGMRES' if out_learned else 'plain GMRES'}({a.restart}) N={N}, "
+                               f"8x8 blocks eps in {{1,12.1}} over 75% area, h=lambda/20, tol {a.tol:g}, "
+                               f"cap {a.max_cycles} cycles",
                    "N": N, "unknowns": N * N, "embedding": ctx.M, "restart": a.restart,
                    "tol": a.tol, "max_cycles": a.max_cycles},
-        "solves": {"converged": int(sum(conv)), "of": len(conv), "iterations": iters,
-                   "achieved_rel_residual": resid,
-                   "us_per_arnoldi_step": wall / max(1, sum(iters)) * 1e6},
-        "roofline": prof["roofline"],
+        "learned": out_learned,
+        "plain": plain_s,
+        "speedup_solves_per_s": (out_learned["solves_per_s"] / plain_s["solves_per_s"]) if out_learned else None,
+        "residual_matched": matched,
+        "training": training,
+        "roofline": roof,
         "kernels": prof["kernels"],
+        "fno_kernels": fprof["kernels"] if fprof else None,
         "cpu_baseline": cpu,
         "reference_problem": refp,
     }
     print(json.dumps(out), flush=True)
+
+
+def profile_fno(ctx, net, dchi, db):
+    """Per-kernel durations of one forward pass of the network (HIP events on
+    the context stream, cgm_fno_profile) with their algorithmic bytes."""
+    import torch
+    from cugmres_amd import _lib
+    from cugmres_amd._lib import lib
+    N = ctx.N
+    n = N * N
+    v = db / torch.linalg.vector_norm(db)
+    z = torch.empty_like(v)
+    ms = (ctypes.c_float * 8)()
+    st = lib().cgm_fno_profile(net.handle, ctypes.c_void_p(dchi.data_ptr()), ctypes.c_void_p(v.data_ptr()),
+                               ctypes.c_void_p(z.data_ptr()), 50, ms)
+    _lib.check(st, "cgm_fno_profile")
+    names = ["fno_lift", "fno_rowproj", "fno_dftx", "fno_mix", "fno_invx", "fno_cell_gemm", "fno_proj",
+             "fno_forward_total"]
+    # algorithmic bytes per launch: BF16 activations 64 B per cell, complex64
+    # vectors 8 B, chi 4 B; per-row mixing matrices 2 KiB per grid row
+    bytes_ = {"fno_lift": n * (4 + 8 + 64), "fno_rowproj": n * 64 + N * 4096,
+              "fno_dftx": N * 4096 + (N // 32) * 16384 * 8, "fno_mix": (N // 32) * 16384 * 8 + 16384 * 16,
+              "fno_invx": 16384 * 8 + N * 2048, "fno_cell_gemm": n * 128 + N * 2048,
+              "fno_proj": n * (64 + 16)}
+    per_step = {"fno_lift": 1, "fno_rowproj": 4, "fno_dftx": 4, "fno_mix": 4, "fno_invx": 4,
+                "fno_cell_gemm": 4, "fno_proj": 1}
+    kernels = {}
+    for i, k in enumerate(names):
+        us = ms[i] * 1e3
+        e = {"us": us}
+        if k in bytes_:
+            e.update(bytes=bytes_[k], GBps=bytes_[k] / (us * 1e-6) / 1e9, per_step=per_step[k])
+        kernels[k] = e
+    return {"kernels": kernels}
 
 
 def profile_kernels(ctx, g, dchi, db, restart):

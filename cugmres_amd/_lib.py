@@ -85,6 +85,9 @@ SIGNATURES = {
     "cgm_matvec_c128": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int]),
     "cgm_gmres": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                  ctypes.POINTER(GmresParams), ctypes.POINTER(GmresResult), c_void_p]),
+    "cgm_gmres_batched": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_long, c_void_p, c_void_p,
+                                         ctypes.c_int, ctypes.POINTER(GmresParams), ctypes.POINTER(GmresResult),
+                                         c_void_p]),
     "cgm_gmres_refine": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                         ctypes.POINTER(RefineParams), ctypes.POINTER(RefineResult)]),
     "cgm_fno_num_params": (ctypes.c_int, [ctypes.POINTER(ctypes.c_size_t)]),
@@ -93,6 +96,7 @@ SIGNATURES = {
     "cgm_fno_set_params": (ctypes.c_int, [c_void_p, c_void_p]),
     "cgm_fno_forward": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, ctypes.c_int]),
     "cgm_fno_backward": (ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
+    "cgm_fno_profile": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, c_float_p]),
     "cgm_fgmres": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                   ctypes.POINTER(FgmresParams), ctypes.POINTER(FgmresResult), c_void_p]),
     "cgm_reference_plane_wave": (ctypes.c_int, [c_void_p, c_void_p]),

@@ -94,10 +94,12 @@ extern "C" int cgm_destroy(cgm_ctx_t c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   krylov_free(c->ws);
-  void* ptrs[] = {c->tw32, c->tw64, c->T32, c->T64, c->st, c->chi_tmp, c->vec_tmp, c->v64a, c->chi32, c->zb};
+  void* ptrs[] = {c->tw32, c->tw64, c->T32, c->T64, c->st, c->chi_tmp, c->vec_tmp, c->v64a, c->chi32, c->zb,
+                  c->stB, c->T32B};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->hst) (void)hipHostFree(c->hst);
+  if (c->hstB) (void)hipHostFree(c->hstB);
   delete c;
   return CGM_SUCCESS;
 }
@@ -497,6 +499,185 @@ extern "C" int cgm_gmres(cgm_ctx_t c, const cgm_complex64* gamma, const float* c
   CGM_HIP(hipSetDevice(c->device));
   memset(res, 0, sizeof(*res));
   return gmres_core(c, (const float2*)gamma, chi, (const float2*)b, (float2*)x, p, res, hist);
+}
+
+// ---------------------------------------------------------------------------
+// Batched GMRES: B systems of one grid size in lockstep
+// ---------------------------------------------------------------------------
+__global__ void k_copy_strided(const float2* __restrict__ src, long ss, float2* __restrict__ dst, long ds, long n) {
+  const long k = blockIdx.y;
+  src += k * ss;
+  dst += k * ds;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x)
+    dst[e] = src[e];
+}
+
+static int copy_strided(hipStream_t st, const float2* src, long ss, float2* dst, long ds, long n, int B) {
+  const long blocks = (n + 255) / 256;
+  hipLaunchKernelGGL(k_copy_strided, dim3((unsigned)(blocks < 1024 ? blocks : 1024), B), dim3(256), 0, st, src, ss,
+                     dst, ds, n);
+  CGM_HIP(hipGetLastError());
+  return CGM_SUCCESS;
+}
+
+static int read_states(cgm_context* c, int B) {
+  CGM_HIP(hipMemcpyAsync(c->hstB, c->stB, sizeof(DevState) * B, hipMemcpyDeviceToHost, c->stream));
+  CGM_HIP(hipStreamSynchronize(c->stream));
+  return CGM_SUCCESS;
+}
+
+extern "C" int cgm_gmres_batched(cgm_ctx_t c, const cgm_complex64* gamma, const float* chi, long chi_stride,
+                                 const cgm_complex64* b, cgm_complex64* x, int batch, const cgm_gmres_params* p,
+                                 cgm_gmres_result* results, double* hist) {
+  CGM_CHECK(c && gamma && chi && b && x && results, CGM_ERROR_INVALID_VALUE, "null argument");
+  CGM_CHECK(batch >= 1, CGM_ERROR_INVALID_VALUE, "batch %d < 1", batch);
+  CGM_CHECK(chi_stride == 0 || chi_stride >= (long)c->N * c->N, CGM_ERROR_INVALID_VALUE,
+            "chi_stride must be 0 (shared map) or >= N*N");
+  int rc = check_params(p);
+  if (rc) return rc;
+  CGM_CHECK(!p->adjoint, CGM_ERROR_UNSUPPORTED, "batched adjoint solves are not supported");
+  CGM_HIP(hipSetDevice(c->device));
+  memset(results, 0, sizeof(cgm_gmres_result) * batch);
+  const long n = (long)c->N * c->N;
+  const int m = p->restart;
+  const int B = batch;
+  const long hstride = (long)m * (p->max_cycles > 0 ? p->max_cycles : 1) + 1;
+  if (!fused_supported(c->N, m)) {
+    // grid sizes without the fused kernels: the systems one after another
+    for (int k = 0; k < B && !rc; ++k) {
+      const int r = gmres_core(c, (const float2*)gamma, chi + k * chi_stride, (const float2*)b + k * n,
+                               (float2*)x + k * n, p, &results[k], hist ? hist + k * hstride : nullptr);
+      if (r && r != CGM_ERROR_NOT_CONVERGED) rc = r;
+    }
+    if (rc) return rc;
+    for (int k = 0; k < B; ++k)
+      if (!results[k].converged) return CGM_ERROR_NOT_CONVERGED;
+    return CGM_SUCCESS;
+  }
+  if ((rc = krylov_alloc(c->ws, n, m, c->stream, B))) return rc;
+  KrylovWs& ws = c->ws;
+  if (c->stB_n < B) {
+    if (c->stB) CGM_HIP(hipFree(c->stB));
+    if (c->hstB) CGM_HIP(hipHostFree(c->hstB));
+    c->stB = nullptr;
+    c->hstB = nullptr;
+    c->stB_n = 0;
+    CGM_HIP(hipMalloc((void**)&c->stB, sizeof(DevState) * B));
+    CGM_HIP(hipHostMalloc((void**)&c->hstB, sizeof(DevState) * B));
+    c->stB_n = B;
+  }
+  const long NM = (long)c->N * c->M;
+  if (c->T32B_n < B) {
+    if (c->T32B) CGM_HIP(hipFree(c->T32B));
+    c->T32B = nullptr;
+    c->T32B_n = 0;
+    CGM_HIP(hipMalloc((void**)&c->T32B, sizeof(float2) * NM * B));
+    c->T32B_n = B;
+  }
+  const long sst = (long)sizeof(DevState);
+  const float2* gq = (const float2*)gamma;
+  const float2* bb = (const float2*)b;
+  float2* xx = (float2*)x;
+  DevState* stB = c->stB;
+  CGM_HIP(hipMemsetAsync(stB, 0, sizeof(DevState) * B, c->stream));
+  if (hist) CGM_HIP(hipMemsetAsync(hist, 0, sizeof(double) * hstride * B, c->stream));
+  if ((rc = launch_norm_init(c->stream, ws, bb, stB, 1, p->tol, n, sst))) return rc;
+  if (p->x0_zero) CGM_HIP(hipMemsetAsync(xx, 0, sizeof(float2) * n * B, c->stream));
+  // residual r = b - A x of every system into its V_0
+  BatchStrides rs;
+  rs.x = n;
+  rs.chi = chi_stride;
+  rs.b = n;
+  rs.y = ws.sBasis;
+  rs.T = NM;
+  rs.st = sst;
+  // u_0 = r and the first matvec's passes 1-2
+  BatchStrides us = rs;
+  us.x = 2 * n;
+  std::vector<int> done(B, 0);
+  int cycles = 0;
+  for (;;) {
+    if (p->x0_zero && cycles == 0) {
+      if ((rc = copy_strided(c->stream, bb, n, ws.V, ws.sBasis, n, B))) return rc;
+    } else {
+      rc = launch_pass1<float>(c->plan, c->stream, xx, nullptr, chi, nullptr, kModeResidual, c->T32B, c->tw32,
+                               nullptr, B, rs);
+      if (!rc) rc = launch_pass2<float, float2>(c->plan, c->stream, c->T32B, gq, c->tw32, nullptr, B, rs);
+      if (!rc)
+        rc = launch_pass3<float>(c->plan, c->stream, c->T32B, xx, chi, bb, ws.V, kModeResidual, c->tw32, nullptr,
+                                 B, rs);
+      if (rc) return rc;
+    }
+    if ((rc = launch_norm_init(c->stream, ws, ws.V, stB, 0, 0.0, ws.sBasis, sst))) return rc;
+    if ((rc = read_states(c, B))) return rc;
+    int active = 0;
+    for (int k = 0; k < B; ++k) {
+      const DevState& h = c->hstB[k];
+      cgm_gmres_result& r = results[k];
+      if (cycles == 0) {
+        r.beta0 = h.beta;
+        r.bnorm = h.bnorm;
+        if (hist) CGM_HIP(hipMemcpyAsync(hist + k * hstride, &r.beta0, sizeof(double), hipMemcpyHostToDevice,
+                                         c->stream));
+      }
+      if (done[k]) continue;
+      r.rel_residual = h.bnorm > 0 ? h.beta / h.bnorm : 0.0;
+      if (h.status) {
+        done[k] = 1;
+        continue;
+      }
+      if (h.bnorm == 0.0 || h.beta <= p->tol * h.bnorm) {
+        r.converged = 1;
+        done[k] = 1;
+        continue;
+      }
+      ++active;
+    }
+    if (!active || cycles >= p->max_cycles) break;
+    // one restart cycle of every live system (converged ones are skipped by
+    // their done flags), enqueued without host synchronisation
+    if ((rc = copy_strided(c->stream, ws.V, ws.sBasis, ws.U, 2 * n, n, B))) return rc;
+    rc = launch_pass1<float>(c->plan, c->stream, ws.U, nullptr, chi, nullptr, kModeForward, c->T32B, c->tw32,
+                             &stB->done, B, us);
+    if (!rc) rc = launch_pass2<float, float2>(c->plan, c->stream, c->T32B, gq, c->tw32, &stB->done, B, us);
+    if (rc) return rc;
+    FusedLaunch L{};
+    L.N = c->N; L.m = m; L.ld = ws.m; L.mode = kModeForward; L.chi = chi; L.V = ws.V; L.Z = ws.Z;
+    L.T = c->T32B; L.tw = c->tw32; L.part = ws.part; L.H = ws.H; L.Rm = ws.R; L.cs = ws.cs;
+    L.sn = ws.sn; L.g = ws.g; L.h1p = ws.h1p; L.a1 = ws.a1; L.a2 = ws.a2; L.scal = ws.scal;
+    L.st = stB; L.hist = hist ? hist + (long)cycles * m : nullptr;
+    L.B = B; L.sChi = chi_stride; L.sBasis = ws.sBasis; L.sU = 2 * n; L.sVec = n; L.sT = NM;
+    L.sPart = ws.sPart; L.sSmall = ws.sSmall; L.sSt = sst; L.sHist = hstride;
+    for (int j = 0; j <= m; ++j) {
+      L.j = j;
+      L.Uj = ws.U + (long)(j & 1) * n;
+      L.Unext = ws.U + (long)((j + 1) & 1) * n;
+      if ((rc = launch_fused(c->stream, 0, L))) return rc;
+      if (j == m) break;
+      if ((rc = launch_fused(c->stream, 1, L))) return rc;
+      if ((rc = launch_pass2<float, float2>(c->plan, c->stream, c->T32B, gq, c->tw32, &stB->done, B, us)))
+        return rc;
+    }
+    if ((rc = launch_backsub_update(c->stream, ws, stB, xx, nullptr, n, sst))) return rc;
+    if ((rc = read_states(c, B))) return rc;
+    for (int k = 0; k < B; ++k) {
+      if (done[k]) continue;
+      results[k].iterations += c->hstB[k].k;
+      results[k].cycles += 1;
+      if (c->hstB[k].status) done[k] = 1;
+    }
+    ++cycles;
+  }
+  int any_nonfinite = 0, all_conv = 1;
+  for (int k = 0; k < B; ++k) {
+    if (c->hstB[k].status) any_nonfinite = 1;
+    if (!results[k].converged) all_conv = 0;
+  }
+  if (any_nonfinite) {
+    cgm_set_last_error("cgm_gmres_batched: non-finite value in at least one system");
+    return CGM_ERROR_NONFINITE;
+  }
+  return all_conv ? CGM_SUCCESS : CGM_ERROR_NOT_CONVERGED;
 }
 
 // ---------------------------------------------------------------------------

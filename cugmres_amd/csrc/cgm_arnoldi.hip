@@ -63,7 +63,33 @@ struct FusedArgs {
   double2* scal;         // [0] = (1/nu, 0), [1] = d/nu
   DevState* st;
   double* hist;          // estimates, hist[j] after j steps of this cycle (may be null)
+  // per-system strides of a batched launch (system = blockIdx.y); elements,
+  // DevState in bytes
+  long sChi, sBasis, sU, sVec, sT, sPart, sSmall, sSt, sHist;
 };
+
+__device__ inline void batch_offset(FusedArgs& a) {
+  const long k = blockIdx.y;
+  if (k == 0) return;
+  a.chi += k * a.sChi;
+  a.V += k * a.sBasis;
+  a.Uj += k * a.sU;
+  a.Unext += k * a.sU;
+  a.Z += k * a.sVec;
+  a.T += k * a.sT;
+  a.part += k * a.sPart;
+  a.H += k * a.sSmall;
+  a.Rm += k * a.sSmall;
+  a.cs += k * a.sSmall;
+  a.sn += k * a.sSmall;
+  a.g += k * a.sSmall;
+  a.h1p += k * a.sSmall;
+  a.a1 += k * a.sSmall;
+  a.a2 += k * a.sSmall;
+  a.scal += k * a.sSmall;
+  a.st = boff(a.st, k * a.sSt);
+  if (a.hist) a.hist += k * a.sHist;
+}
 
 __device__ inline float2 cmulf(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -84,6 +110,7 @@ __device__ inline double2 dscale(double2 a, double s) { return make_double2(a.x 
 
 template <int M, int EPD>
 __global__ void __launch_bounds__(kFBS) k_arn_a(FusedArgs a) {
+  batch_offset(a);
   if (a.st->done) return;
   constexpr int NT = FftShape<M>::NT;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -343,6 +370,7 @@ __global__ void __launch_bounds__(kFBS) k_arn_a(FusedArgs a) {
 
 template <int M, int EPD>
 __global__ void __launch_bounds__(kFBS) k_arn_b(FusedArgs a) {
+  batch_offset(a);
   if (a.st->done) return;
   constexpr int NT = FftShape<M>::NT;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -466,14 +494,23 @@ int launch_fused(hipStream_t st, int which, const FusedLaunch& L) {
   a.scal = L.scal;
   a.st = L.st;
   a.hist = L.hist;
+  a.sChi = L.sChi;
+  a.sBasis = L.sBasis;
+  a.sU = L.sU;
+  a.sVec = L.sVec;
+  a.sT = L.sT;
+  a.sPart = L.sPart;
+  a.sSmall = L.sSmall;
+  a.sSt = L.sSt;
+  a.sHist = L.sHist;
   const size_t lds = (size_t)a.R * M * sizeof(float2);
   switch (L.N) {
 #define CGM_FUSED_CASE(NN, EPD)                                                                 \
   case NN:                                                                                      \
     if (which == 0)                                                                             \
-      hipLaunchKernelGGL((k_arn_a<2 * NN, EPD>), dim3(a.G), dim3(kFBS), lds, st, a);            \
+      hipLaunchKernelGGL((k_arn_a<2 * NN, EPD>), dim3(a.G, L.B), dim3(kFBS), lds, st, a);       \
     else                                                                                        \
-      hipLaunchKernelGGL((k_arn_b<2 * NN, EPD>), dim3(a.G), dim3(kFBS), lds, st, a);            \
+      hipLaunchKernelGGL((k_arn_b<2 * NN, EPD>), dim3(a.G, L.B), dim3(kFBS), lds, st, a);       \
     break;
     CGM_FUSED_CASE(64, 1)
     CGM_FUSED_CASE(128, 1)

@@ -803,3 +803,66 @@ extern "C" int cgm_fno_backward(cgm_fno_t f, const cgm_complex64* dz, float* gra
   CGM_HIP(hipSetDevice(f->ctx->device));
   return fno_backward(f, (const float2*)dz, grad);
 }
+
+// Average device time (ms, HIP events on the context stream, reps launches
+// after two warm-ups) of each forward kernel of the network on one vector of
+// the batch 1 layout: ms[0] lift, [1] row projection, [2] x-DFT, [3] mixing,
+// [4] inverse x-DFT, [5] cell GEMM (GELU epilogue), [6] projection, [7] one
+// whole forward pass.
+extern "C" int cgm_fno_profile(cgm_fno_t f, const float* chi, const cgm_complex64* v, cgm_complex64* z, int reps,
+                               float* ms) {
+  CGM_CHECK(f && chi && v && z && ms && reps > 0, CGM_ERROR_INVALID_VALUE, "null argument");
+  CGM_CHECK(f->params, CGM_ERROR_INVALID_VALUE, "cgm_fno_set_params has not been called");
+  CGM_HIP(hipSetDevice(f->ctx->device));
+  hipStream_t st = f->ctx->stream;
+  const long total = f->n;
+  const float s = (float)f->N;
+  const int nch = f->N / XC;
+  const float* Lp = f->params + layer_off(0);
+  const float2* W = (const float2*)(Lp + C * C + C);
+  hipEvent_t e0, e1;
+  CGM_HIP(hipEventCreate(&e0));
+  CGM_HIP(hipEventCreate(&e1));
+  int rc = CGM_SUCCESS;
+  for (int k = 0; k < 8 && !rc; ++k) {
+    for (int r = -2; r < reps && !rc; ++r) {
+      if (r == 0) CGM_HIP(hipEventRecord(e0, st));
+      switch (k) {
+        case 0:
+          hipLaunchKernelGGL(k_lift, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, chi, f->chi_scale,
+                             (const float2*)v, nullptr, nullptr, s, f->params, f->h[0], total, nullptr);
+          break;
+        case 1: rc = launch_rowproj<false>(f, st, f->h[0], nullptr, f->A, 1, nullptr); break;
+        case 2:
+          hipLaunchKernelGGL(k_dftx, dim3(nch, KY, 1), dim3(1024), 0, st, f->A, f->upart, f->twN, f->N, 0, nullptr);
+          break;
+        case 3:
+          hipLaunchKernelGGL((k_mix<false>), dim3(KY, 1), dim3(1024), 0, st, f->upart, nch, W, nullptr, f->Vm, f->N,
+                             nullptr);
+          break;
+        case 4:
+          hipLaunchKernelGGL(k_invx, dim3(f->N / XB, 1), dim3(512), 0, st, f->Vm, f->Bt, f->twN, f->N, nullptr);
+          break;
+        case 5:
+          rc = launch_pixel<1>(f, st, f->h[0], f->Wt, f->Bt, Lp + C * C, nullptr, nullptr, f->h[1], 1, nullptr);
+          break;
+        case 6:
+          hipLaunchKernelGGL(k_proj, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, f->h[4], f->params,
+                             (const float2*)v, s, (float2*)z, total, nullptr);
+          break;
+        default:
+          rc = fno_forward(f, st, chi, (const float2*)v, nullptr, nullptr, (float2*)z, 1, false, nullptr);
+          break;
+      }
+      if (hipGetLastError() != hipSuccess) rc = CGM_ERROR_DEVICE;
+    }
+    CGM_HIP(hipEventRecord(e1, st));
+    CGM_HIP(hipEventSynchronize(e1));
+    float t = 0.f;
+    CGM_HIP(hipEventElapsedTime(&t, e0, e1));
+    ms[k] = t / reps;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
+}

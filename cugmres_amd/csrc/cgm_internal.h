@@ -12,6 +12,22 @@ enum {
   kModeResidualAdj = 4   // y = b - (x - chi G(x))
 };
 
+// Per-system strides of a batched launch: system k = blockIdx.y reads and
+// writes at base + k * stride.  All zero (and gridDim.y = 1) for one system.
+struct BatchStrides {
+  long x = 0;    // input vector x (elements)
+  long chi = 0;  // contrast (elements)
+  long b = 0;    // right-hand side b (elements)
+  long y = 0;    // output vector y (elements)
+  long T = 0;    // FFT scratch (elements)
+  long st = 0;   // DevState (bytes): norm and done pointers
+};
+
+template <typename P>
+__host__ __device__ inline P* boff(P* p, long bytes) {
+  return p ? reinterpret_cast<P*>(reinterpret_cast<uintptr_t>(p) + bytes) : p;
+}
+
 struct Plan {
   int N, M;
   int rows1, rows3;  // rows per work-group in passes 1 and 3
@@ -25,13 +41,15 @@ int pass_max_threads(int M, bool fp64);
 
 template <typename T>
 int launch_pass1(const Plan& P, hipStream_t st, const cplx<T>* x, cplx<T>* xw, const T* chi,
-                 const double* norm_ptr, int mode, cplx<T>* Tm, const cplx<T>* tw, const int* done);
+                 const double* norm_ptr, int mode, cplx<T>* Tm, const cplx<T>* tw, const int* done,
+                 int B = 1, const BatchStrides& bs = BatchStrides());
 template <typename T, typename G>
 int launch_pass2(const Plan& P, hipStream_t st, cplx<T>* Tm, const G* gq, const cplx<T>* tw,
-                 const int* done);
+                 const int* done, int B = 1, const BatchStrides& bs = BatchStrides());
 template <typename T>
 int launch_pass3(const Plan& P, hipStream_t st, const cplx<T>* Tm, const cplx<T>* x, const T* chi,
-                 const cplx<T>* b, cplx<T>* y, int mode, const cplx<T>* tw, const int* done);
+                 const cplx<T>* b, cplx<T>* y, int mode, const cplx<T>* tw, const int* done,
+                 int B = 1, const BatchStrides& bs = BatchStrides());
 
 int spectrum_from_block(const Plan& P, hipStream_t st, const double2* blk, const double2* tw64,
                         float2* gq32, double2* gq64);
@@ -60,6 +78,12 @@ constexpr int kKB = 1024;  // threads per block of the streaming Krylov kernels
 
 struct KrylovWs {
   int m = 0;              // restart this workspace was sized for
+  int B = 1;              // systems (lockstep batch)
+  long sBasis = 0;        // per-system strides (elements): V
+  long sSmall = 0;        //   every small FP64 array (H, R, g, cs, sn, y, h1, h2, h1p, a1, a2, scal)
+  long sPart = 0;         //   part
+  long sPartn = 0;        //   partn
+  long sVec = 0;          //   Z (and half of U)
   int G = 0;              // blocks of the reduction kernels
   long n = 0;
   float2* V = nullptr;    // (m+1) * n basis
@@ -84,6 +108,8 @@ struct KrylovWs {
 
 struct FusedLaunch {
   int N, m, ld, j, mode;  // m: steps of this cycle; ld: leading dimension of H and R
+  int B = 1;              // systems; per-system strides below (elements, DevState in bytes)
+  long sChi = 0, sBasis = 0, sU = 0, sVec = 0, sT = 0, sPart = 0, sSmall = 0, sSt = 0, sHist = 0;
   const float* chi;
   float2* V;
   const float2* Uj;
@@ -109,15 +135,17 @@ int fused_blocks(int N);
 int launch_fused(hipStream_t st, int which, const FusedLaunch& L);
 
 // Krylov launches (cgm_krylov.hip)
-int krylov_alloc(KrylovWs& ws, long n, int m, hipStream_t st);
+int krylov_alloc(KrylovWs& ws, long n, int m, hipStream_t st, int B = 1);
 void krylov_free(KrylovWs& ws);
+// Batched launches take per-system strides of r (elements) and of the
+// DevState array (bytes); one system by default.
 int launch_norm_init(hipStream_t st, const KrylovWs& ws, const float2* r, DevState* s, int what,
-                     double tol);
+                     double tol, long sr = 0, long sst = 0);
 int launch_cgs2_step(hipStream_t st, const KrylovWs& ws, int i, DevState* s, double* hist,
                      int part = -1);
 // x += basis y (basis = ws.V when null; FGMRES passes its Z basis)
 int launch_backsub_update(hipStream_t st, const KrylovWs& ws, DevState* s, float2* x,
-                          const float2* basis = nullptr);
+                          const float2* basis = nullptr, long sx = 0, long sst = 0);
 int launch_norm64(hipStream_t st, const double2* v, long n, double* partn, int G, DevState* s);
 
 }  // namespace cgm
@@ -153,4 +181,9 @@ struct cgm_context {
   float* chi32 = nullptr;        // refinement contrast in float (N*N, lazy)
   float2* zb = nullptr;          // FGMRES preconditioned basis (zb_m * N*N, lazy)
   int zb_m = 0;
+  cgm::DevState* stB = nullptr;   // batched solves: one state per system (device)
+  cgm::DevState* hstB = nullptr;  // pinned host mirror
+  int stB_n = 0;
+  float2* T32B = nullptr;         // batched FFT scratch (T32B_n systems x N x M)
+  int T32B_n = 0;
 };

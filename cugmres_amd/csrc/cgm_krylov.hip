@@ -210,7 +210,14 @@ k_upd_norm_givens(const float2* __restrict__ V, long n, int i, float2* __restric
 // what = 2: result to norm_out only.
 __global__ void __launch_bounds__(kKB)
 k_norm_init(const float2* __restrict__ r, long n, double* partn, int G, DevState* st,
-            double2* g, int what, double tol) {
+            double2* g, int what, double tol, long sr, long spartn, long ssmall, long sst) {
+  {
+    const long k = blockIdx.y;
+    r += k * sr;
+    partn += k * spartn;
+    g += k * ssmall;
+    st = boff(st, k * sst);
+  }
   double ss = 0.0;
   for (long e = (long)blockIdx.x * kKB + threadIdx.x; e < n; e += (long)G * kKB) {
     const float2 v = r[e];
@@ -278,7 +285,14 @@ k_norm64(const double2* __restrict__ v, long n, double* partn, int G, DevState* 
 }
 
 // Solve R y = g[0:k] (upper triangular, row-major ld m) in FP64.
-__global__ void __launch_bounds__(kKB) k_backsub(KrylovWs ws, const DevState* st, int m) {
+__global__ void __launch_bounds__(kKB) k_backsub(KrylovWs ws, const DevState* st, int m, long sst) {
+  {
+    const long b = blockIdx.y;
+    ws.g += b * ws.sSmall;
+    ws.R += b * ws.sSmall;
+    ws.y += b * ws.sSmall;
+    st = boff(st, b * sst);
+  }
   const int k = st->k;
   __shared__ double2 gs[kMaxRestart];
   __shared__ double2 yi;
@@ -305,7 +319,14 @@ __global__ void __launch_bounds__(kKB) k_backsub(KrylovWs ws, const DevState* st
 // x += sum_{j < k} y_j V_j
 __global__ void __launch_bounds__(kKB)
 k_xupd(const float2* __restrict__ V, long n, float2* __restrict__ x, const double2* __restrict__ y,
-       const DevState* st, int G) {
+       const DevState* st, int G, long sV, long sx, long sy, long sst) {
+  {
+    const long b = blockIdx.y;
+    V += b * sV;
+    x += b * sx;
+    y += b * sy;
+    st = boff(st, b * sst);
+  }
   const int k = st->k;
   for (long e = (long)blockIdx.x * kKB + threadIdx.x; e < n; e += (long)G * kKB) {
     float2 xv = x[e];
@@ -321,37 +342,48 @@ k_xupd(const float2* __restrict__ V, long n, float2* __restrict__ x, const doubl
 }
 
 // ---------------------------------------------------------------------------
-int krylov_alloc(KrylovWs& ws, long n, int m, hipStream_t st) {
-  if (ws.m >= m && ws.n == n) return CGM_SUCCESS;
+int krylov_alloc(KrylovWs& ws, long n, int m, hipStream_t st, int B) {
+  if (ws.m >= m && ws.n == n && ws.B >= B) return CGM_SUCCESS;
+  if (ws.m > m) m = ws.m;
+  if (ws.B > B) B = ws.B;
   krylov_free(ws);
   ws.n = n;
   ws.m = m;
+  ws.B = B;
   long blocks = (n + kKB - 1) / kKB;
   ws.G = (int)(blocks < 256 ? blocks : 256);
   if (ws.G < 1) ws.G = 1;
   const size_t mm = (size_t)m;
-  CGM_HIP(hipMalloc((void**)&ws.V, sizeof(float2) * (size_t)n * (mm + 1)));
-  CGM_HIP(hipMalloc((void**)&ws.h1, sizeof(double2) * (mm + 1)));
-  CGM_HIP(hipMalloc((void**)&ws.h2, sizeof(double2) * (mm + 1)));
-  CGM_HIP(hipMalloc((void**)&ws.cs, sizeof(double) * mm));
-  CGM_HIP(hipMalloc((void**)&ws.sn, sizeof(double2) * mm));
-  CGM_HIP(hipMalloc((void**)&ws.g, sizeof(double2) * (mm + 1)));
-  CGM_HIP(hipMalloc((void**)&ws.R, sizeof(double2) * mm * mm));
-  CGM_HIP(hipMalloc((void**)&ws.y, sizeof(double2) * mm));
+  const size_t BB = (size_t)B;
   int gmax = ws.G;
   if (fused_supported((int)lrint(sqrt((double)n)), 1)) {
     const int gf = fused_blocks((int)lrint(sqrt((double)n)));
     if (gf > gmax) gmax = gf;
   }
-  CGM_HIP(hipMalloc((void**)&ws.part, sizeof(double2) * (2 * mm + 4) * gmax));
-  CGM_HIP(hipMalloc((void**)&ws.partn, sizeof(double) * gmax));
-  CGM_HIP(hipMalloc((void**)&ws.H, sizeof(double2) * (mm + 1) * mm));
-  CGM_HIP(hipMalloc((void**)&ws.h1p, sizeof(double2) * (mm + 1)));
-  CGM_HIP(hipMalloc((void**)&ws.a1, sizeof(double2) * (mm + 1)));
-  CGM_HIP(hipMalloc((void**)&ws.a2, sizeof(double2) * (mm + 1)));
-  CGM_HIP(hipMalloc((void**)&ws.scal, sizeof(double2) * 4));
-  CGM_HIP(hipMalloc((void**)&ws.U, sizeof(float2) * 2 * (size_t)n));
-  CGM_HIP(hipMalloc((void**)&ws.Z, sizeof(float2) * (size_t)n));
+  // per-system strides; every small FP64 array gets the largest small size
+  ws.sBasis = (long)n * (long)(mm + 1);
+  ws.sSmall = (long)((mm + 1) * mm > 4 ? (mm + 1) * mm : 4);
+  ws.sPart = (long)(2 * mm + 4) * gmax;
+  ws.sPartn = gmax;
+  ws.sVec = n;
+  const size_t sm = (size_t)ws.sSmall;
+  CGM_HIP(hipMalloc((void**)&ws.V, sizeof(float2) * (size_t)ws.sBasis * BB));
+  CGM_HIP(hipMalloc((void**)&ws.h1, sizeof(double2) * sm * BB));
+  CGM_HIP(hipMalloc((void**)&ws.h2, sizeof(double2) * sm * BB));
+  CGM_HIP(hipMalloc((void**)&ws.cs, sizeof(double) * sm * BB));
+  CGM_HIP(hipMalloc((void**)&ws.sn, sizeof(double2) * sm * BB));
+  CGM_HIP(hipMalloc((void**)&ws.g, sizeof(double2) * sm * BB));
+  CGM_HIP(hipMalloc((void**)&ws.R, sizeof(double2) * sm * BB));
+  CGM_HIP(hipMalloc((void**)&ws.y, sizeof(double2) * sm * BB));
+  CGM_HIP(hipMalloc((void**)&ws.part, sizeof(double2) * (size_t)ws.sPart * BB));
+  CGM_HIP(hipMalloc((void**)&ws.partn, sizeof(double) * (size_t)ws.sPartn * BB));
+  CGM_HIP(hipMalloc((void**)&ws.H, sizeof(double2) * sm * BB));
+  CGM_HIP(hipMalloc((void**)&ws.h1p, sizeof(double2) * sm * BB));
+  CGM_HIP(hipMalloc((void**)&ws.a1, sizeof(double2) * sm * BB));
+  CGM_HIP(hipMalloc((void**)&ws.a2, sizeof(double2) * sm * BB));
+  CGM_HIP(hipMalloc((void**)&ws.scal, sizeof(double2) * sm * BB));
+  CGM_HIP(hipMalloc((void**)&ws.U, sizeof(float2) * 2 * (size_t)n * BB));
+  CGM_HIP(hipMalloc((void**)&ws.Z, sizeof(float2) * (size_t)n * BB));
   (void)st;
   return CGM_SUCCESS;
 }
@@ -365,9 +397,10 @@ void krylov_free(KrylovWs& ws) {
 }
 
 int launch_norm_init(hipStream_t st, const KrylovWs& ws, const float2* r, DevState* s, int what,
-                     double tol) {
-  hipLaunchKernelGGL(k_norm_init, dim3(ws.G), dim3(kKB), 0, st, r, ws.n, ws.partn, ws.G, s, ws.g,
-                     what, tol);
+                     double tol, long sr, long sst) {
+  const int B = sst ? ws.B : 1;
+  hipLaunchKernelGGL(k_norm_init, dim3(ws.G, B), dim3(kKB), 0, st, r, ws.n, ws.partn, ws.G, s, ws.g,
+                     what, tol, sr, ws.sPartn, ws.sSmall, sst);
   CGM_HIP(hipGetLastError());
   return CGM_SUCCESS;
 }
@@ -411,9 +444,12 @@ int launch_cgs2_step(hipStream_t st, const KrylovWs& ws, int i, DevState* s, dou
   return CGM_SUCCESS;
 }
 
-int launch_backsub_update(hipStream_t st, const KrylovWs& ws, DevState* s, float2* x, const float2* basis) {
-  hipLaunchKernelGGL(k_backsub, dim3(1), dim3(kKB), 0, st, ws, s, ws.m);
-  hipLaunchKernelGGL(k_xupd, dim3(ws.G), dim3(kKB), 0, st, basis ? basis : ws.V, ws.n, x, ws.y, s, ws.G);
+int launch_backsub_update(hipStream_t st, const KrylovWs& ws, DevState* s, float2* x, const float2* basis,
+                          long sx, long sst) {
+  const int B = sst ? ws.B : 1;
+  hipLaunchKernelGGL(k_backsub, dim3(1, B), dim3(kKB), 0, st, ws, s, ws.m, sst);
+  hipLaunchKernelGGL(k_xupd, dim3(ws.G, B), dim3(kKB), 0, st, basis ? basis : ws.V, ws.n, x, ws.y, s, ws.G,
+                     ws.sBasis, sx, ws.sSmall, sst);
   CGM_HIP(hipGetLastError());
   return CGM_SUCCESS;
 }

@@ -39,7 +39,16 @@ template <typename T, int M>
 __global__ void __launch_bounds__((PassWG<T, M>::value))
 k_pass1(const cplx<T>* __restrict__ x, cplx<T>* xw, const T* __restrict__ chi,
         const double* __restrict__ norm_ptr, int mode, cplx<T>* __restrict__ Tm, int N,
-        int rows_per_wg, const cplx<T>* __restrict__ tw, const int* __restrict__ done) {
+        int rows_per_wg, const cplx<T>* __restrict__ tw, const int* __restrict__ done, BatchStrides bs) {
+  {
+    const long k = blockIdx.y;
+    x += k * bs.x;
+    if (xw) xw += k * bs.x;
+    if (chi) chi += k * bs.chi;
+    Tm += k * bs.T;
+    norm_ptr = boff(norm_ptr, k * bs.st);
+    done = boff(done, k * bs.st);
+  }
   if (done && *done) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T>* smem = reinterpret_cast<cplx<T>*>(smem_raw);
@@ -80,7 +89,9 @@ k_pass1(const cplx<T>* __restrict__ x, cplx<T>* xw, const T* __restrict__ chi,
 template <typename T, typename G, int M>
 __global__ void __launch_bounds__((PassWG<T, M>::value))
 k_pass2(cplx<T>* __restrict__ Tm, const G* __restrict__ gq, int N, int cols_per_wg,
-        const cplx<T>* __restrict__ tw, const int* __restrict__ done) {
+        const cplx<T>* __restrict__ tw, const int* __restrict__ done, BatchStrides bs) {
+  Tm += (long)blockIdx.y * bs.T;
+  done = boff(done, (long)blockIdx.y * bs.st);
   if (done && *done) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T>* smem = reinterpret_cast<cplx<T>*>(smem_raw);
@@ -125,7 +136,16 @@ template <typename T, int M>
 __global__ void __launch_bounds__((PassWG<T, M>::value))
 k_pass3(const cplx<T>* __restrict__ Tm, const cplx<T>* __restrict__ x, const T* __restrict__ chi,
         const cplx<T>* __restrict__ b, cplx<T>* __restrict__ y, int mode, int N, int rows_per_wg,
-        T inv, const cplx<T>* __restrict__ tw, const int* __restrict__ done) {
+        T inv, const cplx<T>* __restrict__ tw, const int* __restrict__ done, BatchStrides bs) {
+  {
+    const long k = blockIdx.y;
+    Tm += k * bs.T;
+    x += k * bs.x;
+    if (chi) chi += k * bs.chi;
+    if (b) b += k * bs.b;
+    y += k * bs.y;
+    done = boff(done, k * bs.st);
+  }
   if (done && *done) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T>* smem = reinterpret_cast<cplx<T>*>(smem_raw);
@@ -313,7 +333,8 @@ static int groups_for(int M, int target_threads) {
 
 template <typename T>
 int launch_pass1(const Plan& P, hipStream_t st, const cplx<T>* x, cplx<T>* xw, const T* chi,
-                 const double* norm_ptr, int mode, cplx<T>* Tm, const cplx<T>* tw, const int* done) {
+                 const double* norm_ptr, int mode, cplx<T>* Tm, const cplx<T>* tw, const int* done,
+                 int B, const BatchStrides& bs) {
   const int M = P.M, N = P.N;
   const int rows = P.rows1;
   const int grid = (N + rows - 1) / rows;
@@ -323,8 +344,8 @@ int launch_pass1(const Plan& P, hipStream_t st, const cplx<T>* x, cplx<T>* xw, c
   switch (M) {
 #define X(m)                                                                                 \
   case m:                                                                                    \
-    hipLaunchKernelGGL((k_pass1<T, m>), dim3(grid), dim3(threads), lds, st, x, xw, chi,      \
-                       norm_ptr, mode, Tm, N, rows, tw, done);                                \
+    hipLaunchKernelGGL((k_pass1<T, m>), dim3(grid, B), dim3(threads), lds, st, x, xw, chi,   \
+                       norm_ptr, mode, Tm, N, rows, tw, done, bs);                            \
     break;
     CGM_FOR_EACH_M(X)
 #undef X
@@ -335,7 +356,7 @@ int launch_pass1(const Plan& P, hipStream_t st, const cplx<T>* x, cplx<T>* xw, c
 
 template <typename T, typename G>
 int launch_pass2(const Plan& P, hipStream_t st, cplx<T>* Tm, const G* gq, const cplx<T>* tw,
-                 const int* done) {
+                 const int* done, int B, const BatchStrides& bs) {
   const int M = P.M, N = P.N;
   const int cols = P.cols2;
   const int grid = M / cols;
@@ -345,8 +366,8 @@ int launch_pass2(const Plan& P, hipStream_t st, cplx<T>* Tm, const G* gq, const 
   switch (M) {
 #define X(m)                                                                                  \
   case m:                                                                                     \
-    hipLaunchKernelGGL((k_pass2<T, G, m>), dim3(grid), dim3(threads), lds, st, Tm, gq, N, cols, \
-                       tw, done);                                                             \
+    hipLaunchKernelGGL((k_pass2<T, G, m>), dim3(grid, B), dim3(threads), lds, st, Tm, gq, N,    \
+                       cols, tw, done, bs);                                                   \
     break;
     CGM_FOR_EACH_M(X)
 #undef X
@@ -357,7 +378,8 @@ int launch_pass2(const Plan& P, hipStream_t st, cplx<T>* Tm, const G* gq, const 
 
 template <typename T>
 int launch_pass3(const Plan& P, hipStream_t st, const cplx<T>* Tm, const cplx<T>* x, const T* chi,
-                 const cplx<T>* b, cplx<T>* y, int mode, const cplx<T>* tw, const int* done) {
+                 const cplx<T>* b, cplx<T>* y, int mode, const cplx<T>* tw, const int* done,
+                 int B, const BatchStrides& bs) {
   const int M = P.M, N = P.N;
   const int rows = P.rows3;
   const int grid = (N + rows - 1) / rows;
@@ -368,8 +390,8 @@ int launch_pass3(const Plan& P, hipStream_t st, const cplx<T>* Tm, const cplx<T>
   switch (M) {
 #define X(m)                                                                                   \
   case m:                                                                                      \
-    hipLaunchKernelGGL((k_pass3<T, m>), dim3(grid), dim3(threads), lds, st, Tm, x, chi, b, y,   \
-                       mode, N, rows, inv, tw, done);                                          \
+    hipLaunchKernelGGL((k_pass3<T, m>), dim3(grid, B), dim3(threads), lds, st, Tm, x, chi, b,   \
+                       y, mode, N, rows, inv, tw, done, bs);                                   \
     break;
     CGM_FOR_EACH_M(X)
 #undef X
@@ -379,20 +401,21 @@ int launch_pass3(const Plan& P, hipStream_t st, const cplx<T>* Tm, const cplx<T>
 }
 
 template int launch_pass1<float>(const Plan&, hipStream_t, const float2*, float2*, const float*,
-                                 const double*, int, float2*, const float2*, const int*);
+                                 const double*, int, float2*, const float2*, const int*, int,
+                                 const BatchStrides&);
 template int launch_pass1<double>(const Plan&, hipStream_t, const double2*, double2*,
                                   const double*, const double*, int, double2*, const double2*,
-                                  const int*);
+                                  const int*, int, const BatchStrides&);
 template int launch_pass2<float, float2>(const Plan&, hipStream_t, float2*, const float2*,
-                                         const float2*, const int*);
+                                         const float2*, const int*, int, const BatchStrides&);
 template int launch_pass2<double, double2>(const Plan&, hipStream_t, double2*, const double2*,
-                                           const double2*, const int*);
+                                           const double2*, const int*, int, const BatchStrides&);
 template int launch_pass3<float>(const Plan&, hipStream_t, const float2*, const float2*,
                                  const float*, const float2*, float2*, int, const float2*,
-                                 const int*);
+                                 const int*, int, const BatchStrides&);
 template int launch_pass3<double>(const Plan&, hipStream_t, const double2*, const double2*,
                                   const double*, const double2*, double2*, int, const double2*,
-                                  const int*);
+                                  const int*, int, const BatchStrides&);
 
 int plan_init(Plan& P, int N) {
   P.N = N;

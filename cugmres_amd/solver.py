@@ -150,6 +150,35 @@ class Context:
             info["history"] = hist[: r.iterations + 1].cpu().numpy()
         return x, info
 
+    def gmres_batched(self, gamma, chi, b, x=None, restart=30, tol=1e-6, max_cycles=100, history=False):
+        """Lockstep GMRES(restart) of B systems: b (B, N*N) complex64, chi either
+        (B, N*N) (own maps) or (N*N,) (one map, multiple right-hand sides).
+        Returns (x, [info per system])."""
+        t = self.torch
+        B = b.shape[0]
+        n = self.N * self.N
+        chi_stride = 0 if chi.dim() == 1 else n
+        x0_zero = x is None
+        if x is None:
+            x = t.zeros_like(b)
+        hs = restart * max(max_cycles, 1) + 1
+        hist = t.zeros(B, hs, dtype=t.float64, device=self.device) if history else None
+        p = _lib.GmresParams(int(restart), int(max_cycles), float(tol), 0, int(x0_zero))
+        res = (_lib.GmresResult * B)()
+        st = lib().cgm_gmres_batched(self._ptr, _ptr(gamma), _ptr(chi), chi_stride, _ptr(b), _ptr(x), B,
+                                     _c.byref(p), res, _ptr(hist))
+        check(st, "cgm_gmres_batched", ok=(_lib.SUCCESS, _lib.ERROR_NOT_CONVERGED))
+        infos = []
+        h = hist.cpu().numpy() if history else None
+        for k in range(B):
+            r = res[k]
+            d = dict(iterations=r.iterations, cycles=r.cycles, converged=bool(r.converged),
+                     rel_residual=r.rel_residual, beta0=r.beta0, bnorm=r.bnorm)
+            if history:
+                d["history"] = h[k]
+            infos.append(d)
+        return x, infos
+
     def fgmres(self, gamma, chi, b, x=None, precond=None, restart=30, tol=1e-6, max_cycles=100,
                stall_ratio=0.5, history=False):
         """Right-preconditioned flexible GMRES(restart) with the learned

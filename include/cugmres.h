@@ -161,6 +161,21 @@ int cgm_gmres(cgm_ctx_t ctx, const cgm_complex64* dev_gamma, const float* dev_ch
               const cgm_complex64* dev_b, cgm_complex64* dev_x, const cgm_gmres_params* params,
               cgm_gmres_result* result, double* dev_hist);
 
+/* Batched GMRES(m): `batch` systems of the context's N in lockstep, every
+ * kernel of the solve launched once for all of them (system = grid y index).
+ * They share the kernel spectrum; chi is [batch][N*N] with chi_stride = N*N
+ * (own maps) or one map with chi_stride = 0 (multiple right-hand sides of
+ * one instance).  b and x are [batch][N*N]; results has batch entries.  A
+ * converged system stops at its own tolerance, the others go on.  dev_hist
+ * (optional) is [batch][restart*max_cycles + 1]: [0] = ||r0||, entry
+ * c*restart + j = estimate after j steps of cycle c.  Power-of-two N in
+ * [64, 1024] with restart <= 64 run batched; other sizes solve the systems
+ * one after another.  No counterpart in the reference, which solves one
+ * system per call (B/CUDA_GMRES/GMRES.cuh:22-35). */
+int cgm_gmres_batched(cgm_ctx_t ctx, const cgm_complex64* dev_gamma, const float* dev_chi, long chi_stride,
+                      const cgm_complex64* dev_b, cgm_complex64* dev_x, int batch,
+                      const cgm_gmres_params* params, cgm_gmres_result* results, double* dev_hist);
+
 /* FP64 iterative refinement: the true residual r = b - A x is formed with the
  * complex128 operator, the correction A d = r is solved by complex64
  * GMRES(inner_restart) to inner_tol, x += d, until ||r|| <= outer_tol ||b||. */
@@ -318,6 +333,14 @@ int cgm_cuSolve_LES(cgm_complex64* dev_A, cgm_complex64* dev_B, int m, cgm_ctx_t
  * dev_x as every basis vector; overwrites the context's solver workspace. */
 int cgm_profile_step(cgm_ctx_t ctx, const cgm_complex64* dev_gamma, const float* dev_chi,
                      const cgm_complex64* dev_x, int restart, int reps, float* ms);
+
+/* Average device time (ms, HIP events on the context stream, reps launches
+ * each) of the preconditioner's forward kernels on one vector: ms[0] lift,
+ * [1] row projection, [2] x-DFT, [3] mode mixing, [4] inverse x-DFT, [5] cell
+ * GEMM with GELU, [6] projection, [7] a whole forward pass.  Overwrites the
+ * network's activations. */
+int cgm_fno_profile(cgm_fno_t net, const float* dev_chi, const cgm_complex64* dev_v,
+                    cgm_complex64* dev_z, int reps, float* ms);
 
 /* ---- memory and host helpers ---------------------------------------------- */
 int cgm_malloc(void** ptr, size_t bytes);

@@ -1,0 +1,98 @@
+"""Batched lockstep GMRES (cgm_gmres_batched) against single solves.
+
+The batched kernels run the same per-system arithmetic as a single solve (same
+blocks, same reduction order), so every system must reproduce its single
+solve: equal iteration counts, solutions equal to float rounding (checked at
+relative L2 1e-6) and histories equal to 1e-6 of ||r0||.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import rel
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def sol():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from cugmres_amd import solver
+    return solver
+
+
+def _systems(N, B, eps_hi, seed0, shared=False):
+    from cugmres_amd import problems
+    chis, bs = [], []
+    for k in range(B):
+        eps = problems.block_eps_map(N, 8 if N >= 128 else 4, 0.5, 0.75, seed=seed0 + (0 if shared else k),
+                                     eps_hi=eps_hi)
+        chis.append(eps - 1.0)
+        bs.append(problems.plane_wave(N, problems.KH, 2 * math.pi * k / B))
+    return np.stack(chis), np.stack(bs)
+
+
+def _check(sol, N, B, eps_hi, shared, restart, max_cycles, tol=1e-6):
+    import torch
+    from cugmres_amd import problems
+    ctx = sol.Context(N)
+    g = ctx.spectrum_helmholtz(problems.KH)
+    chi, b = _systems(N, B, eps_hi, 7, shared)
+    chi_d = torch.from_numpy(chi.reshape(B, -1).astype(np.float32)).cuda()
+    b_d = torch.from_numpy(b.reshape(B, -1).astype(np.complex64)).cuda()
+    chi_arg = chi_d[0].contiguous() if shared else chi_d
+    xb, infos = ctx.gmres_batched(g, chi_arg, b_d, restart=restart, tol=tol, max_cycles=max_cycles,
+                                  history=True)
+    for k in range(B):
+        xs, i1 = ctx.gmres(g, chi_d[k].contiguous(), b_d[k].contiguous(), restart=restart, tol=tol,
+                           max_cycles=max_cycles, history=True)
+        ib = infos[k]
+        assert ib["iterations"] == i1["iterations"], (k, ib, i1)
+        assert ib["converged"] == i1["converged"]
+        assert rel(xb[k].cpu().numpy(), xs.cpu().numpy()) < 1e-6
+        h1 = i1["history"]
+        # batched history is laid out by cycle slot (c * restart + j)
+        hb = ib["history"][: ib["iterations"] + 1] if ib["cycles"] <= 1 else None
+        if hb is not None:
+            assert np.max(np.abs(hb - h1[: len(hb)])) <= 1e-6 * h1[0]
+    return infos
+
+
+def test_batched_own_maps_match_single(sol):
+    infos = _check(sol, 64, 5, 2.25, False, 20, 20)
+    assert all(i["converged"] for i in infos)
+    assert len({i["iterations"] for i in infos}) > 1   # systems stop at different steps
+
+
+def test_batched_multi_rhs_shared_map(sol):
+    # 8 plane waves on one map (chi_stride 0); whatever each reaches in 12
+    # cycles, it must be what its single solve reaches
+    infos = _check(sol, 128, 8, 2.25, True, 30, 12)
+    print([(i["iterations"], i["converged"], i["rel_residual"]) for i in infos])
+
+
+def test_batched_capped_hard_problem(sol):
+    # eps = 12.1 blocks: no system converges in 2 cycles; all must still match
+    infos = _check(sol, 64, 3, 12.1, False, 16, 2)
+    assert not any(i["converged"] for i in infos)
+
+
+def test_batched_zero_rhs_and_non_pow2(sol):
+    import torch
+    from cugmres_amd import problems
+    N = 64
+    ctx = sol.Context(N)
+    g = ctx.spectrum_helmholtz(problems.KH)
+    chi, b = _systems(N, 3, 2.25, 3)
+    b[1] = 0
+    chi_d = torch.from_numpy(chi.reshape(3, -1).astype(np.float32)).cuda()
+    b_d = torch.from_numpy(b.reshape(3, -1).astype(np.complex64)).cuda()
+    x, infos = ctx.gmres_batched(g, chi_d, b_d, restart=20, tol=1e-6, max_cycles=20)
+    assert infos[1]["converged"] and infos[1]["iterations"] == 0
+    assert float(torch.linalg.vector_norm(x[1])) == 0.0
+    assert infos[0]["converged"] and infos[2]["converged"]
+    # N = 96 has no fused kernels: the systems run one after another
+    _check(sol, 96, 2, 2.25, False, 20, 20)
