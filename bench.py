@@ -55,9 +55,10 @@ def parse():
                    help="FGMRES falls back to plain GMRES when a cycle leaves more than this "
                         "fraction of the residual (0: never)")
     p.add_argument("--plain-only", action="store_true")
-    p.add_argument("--mode", choices=["bench", "step", "copy"], default="bench",
+    p.add_argument("--mode", choices=["bench", "step", "copy", "batched"], default="bench",
                    help="bench: the benchmark; step: only the per-kernel profile of one Arnoldi "
-                        "step (for rocprofv3 --pmc); copy: 3 copies of 256 MiB (counter calibration)")
+                        "step (for rocprofv3 --pmc); copy: 3 copies of 256 MiB (counter calibration); "
+                        "batched: lockstep batches of 16 right-hand sides vs single solves")
     return p.parse_args()
 
 
@@ -80,6 +81,9 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes": src.numel() * 8, "copies": 3}))
+        return
+    if a.mode == "batched":
+        print(json.dumps(batched_throughput(a.restart)), flush=True)
         return
     N = a.N
     ctx = solver.Context(N, 0)
@@ -232,6 +236,43 @@ def main():
         "reference_problem": refp,
     }
     print(json.dumps(out), flush=True)
+
+
+def batched_throughput(restart, cycles=3, B=16):
+    """Batched workload shape: B plane waves at evenly spaced angles on one
+    seeded binary map (8 x 8 blocks, eps {1, 12.1}, 75 % area) per grid size,
+    `cycles` full GMRES(restart) cycles (tolerance 0, so every system runs
+    every step).  Times one cgm_gmres_batched call against B cgm_gmres calls
+    on the same systems."""
+    import numpy as np
+    import torch
+    from cugmres_amd import problems, solver
+    out = {"workload": f"{B} plane waves x one map, GMRES({restart}), {cycles} cycles", "sizes": {}}
+    for N in (64, 128, 256, 512):
+        ctx = solver.Context(N, 0)
+        g = ctx.spectrum_helmholtz(problems.KH)
+        eps = problems.block_eps_map(N, 8, 0.5, 0.75, seed=N)
+        chi = ctx.to_device((eps - 1.0).reshape(-1), "f32")
+        b = torch.stack([ctx.to_device(problems.plane_wave(N, problems.KH, 2 * math.pi * k / B).reshape(-1))
+                         for k in range(B)])
+        x = torch.zeros_like(b)
+        ctx.gmres_batched(g, chi, b, x, restart=restart, tol=0.0, max_cycles=1)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        _, infos = ctx.gmres_batched(g, chi, b, x.zero_(), restart=restart, tol=0.0, max_cycles=cycles)
+        torch.cuda.synchronize()
+        tb = time.perf_counter() - t
+        steps = sum(i["iterations"] for i in infos)
+        t = time.perf_counter()
+        for k in range(B):
+            ctx.gmres(g, chi, b[k], x[k].zero_(), restart=restart, tol=0.0, max_cycles=cycles)
+        torch.cuda.synchronize()
+        ts = time.perf_counter() - t
+        out["sizes"][N] = {"batched_ms": tb * 1e3, "single_ms": ts * 1e3, "speedup": ts / tb,
+                           "system_steps": steps, "us_per_system_step_batched": tb / steps * 1e6,
+                           "us_per_system_step_single": ts / steps * 1e6}
+        ctx.close()
+    return out
 
 
 def profile_fno(ctx, net, dchi, db):
