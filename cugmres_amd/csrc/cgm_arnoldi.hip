@@ -127,8 +127,7 @@ __global__ void __launch_bounds__(kFBS) k_arn_a(FusedArgs a) {
   float2* buf = smem + rl * M;
   if (fa) {
     const int row = r0 + rl;
-    const float2* in = a.T + (long)row * M;
-    for (int e = t; e < M; e += NT) buf[e] = row < N ? in[e] : make_float2(0.f, 0.f);
+    for (int e = t; e < M; e += NT) buf[e] = row < N ? a.T[tpos(row, e, N)] : make_float2(0.f, 0.f);
   }
   __syncthreads();
   lds_fft<float, M, +1>(buf, tws, t, fa);
@@ -442,8 +441,7 @@ __global__ void __launch_bounds__(kFBS) k_arn_b(FusedArgs a) {
   if (fa) {
     const int row = r0 + rl;
     if (row < N) {
-      float2* out = a.T + (long)row * M;
-      for (int e = t; e < M; e += NT) out[e] = buf[e];
+      for (int e = t; e < M; e += NT) a.T[tpos(row, e, N)] = buf[e];
     }
   }
 }

@@ -97,7 +97,10 @@ __global__ void __launch_bounds__(256) k_lift(const float* __restrict__ chi, flo
                                               const float* __restrict__ P, bf16* __restrict__ h0,
                                               long total, const int* done) {
   if (skip(done)) return;
-  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  // thread = (cell, quarter): 8 channels of one cell, one 16-byte store
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long p = e >> 2;
+  const int c8 = (int)(e & 3) * 8;
   if (p >= total) return;
   float2 vv = v[p];
   if (nrm) {
@@ -105,18 +108,16 @@ __global__ void __launch_bounds__(256) k_lift(const float* __restrict__ chi, flo
     const float r = d > 0.0 ? (float)(1.0 / d) : 0.f;
     vv.x *= r;
     vv.y *= r;
-    if (vw) vw[p] = vv;
+    if (vw && c8 == 0) vw[p] = vv;
   }
   const float a0 = chi[p] * chi_scale, a1 = vv.x * s, a2 = vv.y * s;
-  bf16* o = h0 + p * C;
+  struct __align__(16) Q8 { bf16 v[8]; } q;
 #pragma unroll
-  for (int c = 0; c < C; c += 4) {
-    float r[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-      r[k] = a0 * P[c + k] + a1 * P[C + c + k] + a2 * P[2 * C + c + k] + P[3 * C + c + k];
-    st4(o + c, r[0], r[1], r[2], r[3]);
+  for (int k = 0; k < 8; ++k) {
+    const int c = c8 + k;
+    q.v[k] = (bf16)(a0 * P[c] + a1 * P[C + c] + a2 * P[2 * C + c] + P[3 * C + c]);
   }
+  *reinterpret_cast<Q8*>(h0 + p * C + c8) = q;
 }
 
 // ---------------------------------------------------------------------------
@@ -198,33 +199,33 @@ __global__ void __launch_bounds__(256) k_rowproj(const bf16* __restrict__ Xt, co
 //   out[b][chunk][kxi][ky][c] = s_ky sum_{x in chunk} (in[b][x][2ky][c] + i in[b][x][2ky+1][c]) e^{-2 pi i kx x/N}
 // with s_ky = 1 (forward) or w_ky / N^2 (backward of the inverse transform).
 __global__ void __launch_bounds__(1024) k_dftx(const float* __restrict__ in, float2* __restrict__ outp,
-                                               const float2* __restrict__ twN, int N, int scaled,
+                                               const float2* __restrict__ Ex, int N, int scaled,
                                                const int* done) {
   if (skip(done)) return;
   __shared__ float2 sz[XC][C + 1];
+  __shared__ float2 se[KX][XC + 1];
   const int chunk = blockIdx.x, ky = blockIdx.y, b = blockIdx.z;
   const int nch = gridDim.x;
   const int tid = threadIdx.x;
+  const int hi = tid >> 5, lo = tid & 31;
   const int x0 = chunk * XC;
   {
-    const int xl = tid >> 5, c = tid & 31;
-    const float* row = in + ((long)(b * N + x0 + xl) * C + 2 * ky) * C;
-    sz[xl][c] = make_float2(row[c], row[C + c]);
+    const float* row = in + ((long)(b * N + x0 + hi) * C + 2 * ky) * C;
+    sz[hi][lo] = make_float2(row[lo], row[C + lo]);
+    se[hi][lo] = Ex[(long)hi * N + x0 + lo];   // frequency hi at row x0 + lo
   }
   __syncthreads();
-  const int kxi = tid >> 5, c = tid & 31;
-  const int kx = kxi < 16 ? kxi : N - KX + kxi;
   float2 acc = make_float2(0.f, 0.f);
-#pragma unroll 4
+#pragma unroll
   for (int xl = 0; xl < XC; ++xl) {
-    const float2 e = twN[(kx * (x0 + xl)) % N];
-    const float2 z = sz[xl][c];
+    const float2 e = se[hi][xl];
+    const float2 z = sz[xl][lo];
     acc.x += z.x * e.x + z.y * e.y;  // z * conj(e)
     acc.y += z.y * e.x - z.x * e.y;
   }
-  float s = 1.f;
-  if (scaled) s = (ky == 0 ? 1.f : 2.f) / ((float)N * (float)N);
-  outp[(((long)(b * nch + chunk) * KX + kxi) * KY + ky) * C + c] = make_float2(acc.x * s, acc.y * s);
+  float sc = 1.f;
+  if (scaled) sc = (ky == 0 ? 1.f : 2.f) / ((float)N * (float)N);
+  outp[(((long)(b * nch + chunk) * KX + hi) * KY + ky) * C + lo] = make_float2(acc.x * sc, acc.y * sc);
 }
 
 // ---------------------------------------------------------------------------
@@ -233,67 +234,78 @@ __global__ void __launch_bounds__(1024) k_dftx(const float* __restrict__ in, flo
 //   forward : V[kxi][o] = s_ky sum_c U[kxi][c] W[kxi][ky][c][o],  s_ky = w_ky / N^2
 //   backward: V[kxi][c] = sum_o conj(W[kxi][ky][c][o]) U[kxi][o]
 // V goes to vout[b][kxi][ky][ch].
+constexpr int KXG = 8;  // retained kx per block of k_mix
 template <bool BWD>
-__global__ void __launch_bounds__(1024) k_mix(const float2* __restrict__ upart, int nch,
-                                              const float2* __restrict__ W, float2* __restrict__ usave,
-                                              float2* __restrict__ vout, int N, const int* done) {
+__global__ void __launch_bounds__(256) k_mix(const float2* __restrict__ upart, int nch,
+                                             const float2* __restrict__ W, float2* __restrict__ usave,
+                                             float2* __restrict__ vout, int N, const int* done) {
   if (skip(done)) return;
-  __shared__ float2 su[KX][C + 1];
-  const int ky = blockIdx.x, b = blockIdx.y;
+  __shared__ float2 su[KXG][C + 1];
+  const int ky = blockIdx.x, b = blockIdx.z;
   const int tid = threadIdx.x;
-  const int kxi = tid >> 5, ch = tid & 31;
+  const int kl = tid >> 5, ch = tid & 31;
+  const int kxi = blockIdx.y * KXG + kl;
   float2 acc = make_float2(0.f, 0.f);
-  for (int k = 0; k < nch; ++k) {
-    const float2 u = upart[(((long)(b * nch + k) * KX + kxi) * KY + ky) * C + ch];
-    acc.x += u.x;
-    acc.y += u.y;
+  for (int k0 = 0; k0 < nch; k0 += 8) {
+    float2 u[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      u[j] = (k0 + j < nch) ? upart[(((long)(b * nch + k0 + j) * KX + kxi) * KY + ky) * C + ch]
+                            : make_float2(0.f, 0.f);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      acc.x += u[j].x;
+      acc.y += u[j].y;
+    }
   }
-  su[kxi][ch] = acc;
+  su[kl][ch] = acc;
   const long slot = (((long)b * KX + kxi) * KY + ky) * C + ch;
   if (usave) usave[slot] = acc;
   __syncthreads();
   const float2* Wm = W + (long)(kxi * KY + ky) * C * C;
   float2 v = make_float2(0.f, 0.f);
   if (!BWD) {
+#pragma unroll 8
     for (int c = 0; c < C; ++c) {
-      const float2 u = su[kxi][c], wv = Wm[c * C + ch];
-      v.x += u.x * wv.x - u.y * wv.y;
-      v.y += u.x * wv.y + u.y * wv.x;
+      const float2 uu = su[kl][c], wv = Wm[c * C + ch];
+      v.x += uu.x * wv.x - uu.y * wv.y;
+      v.y += uu.x * wv.y + uu.y * wv.x;
     }
     const float sc = (ky == 0 ? 1.f : 2.f) / ((float)N * (float)N);
     v.x *= sc;
     v.y *= sc;
   } else {
+#pragma unroll 8
     for (int o = 0; o < C; ++o) {
-      const float2 u = su[kxi][o], wv = Wm[ch * C + o];
-      v.x += wv.x * u.x + wv.y * u.y;  // conj(w) * u
-      v.y += wv.x * u.y - wv.y * u.x;
+      const float2 uu = su[kl][o], wv = Wm[ch * C + o];
+      v.x += wv.x * uu.x + wv.y * uu.y;  // conj(w) * u
+      v.y += wv.x * uu.y - wv.y * uu.x;
     }
   }
   vout[slot] = v;
 }
 
-// Inverse DFT along x for XB rows per block (thread = (ch, ky), ky fastest,
-// so the 16 ky pairs of one channel are 64 contiguous bytes of the output):
+// Inverse DFT along x, XB rows and KYG column frequencies per block
+// (thread = (ch, ky within the group, row phase)):
 //   out[b][x][ch][2ky], [2ky+1] = Re, Im of sum_kxi V[kxi][ky][ch] e^{+2 pi i kx x/N}   (BF16)
 constexpr int XB = 8;
+constexpr int KYG = 4;
 __global__ void __launch_bounds__(512) k_invx(const float2* __restrict__ V, bf16* __restrict__ out,
-                                              const float2* __restrict__ twN, int N, const int* done) {
+                                              const float2* __restrict__ Ex, int N, const int* done) {
   if (skip(done)) return;
   __shared__ float2 se[XB][KX];
-  const int x0 = blockIdx.x * XB, b = blockIdx.y;
+  const int x0 = blockIdx.x * XB, b = blockIdx.z;
   const int tid = threadIdx.x;
   if (tid < XB * KX) {
-    const int xl = tid / KX, q = tid % KX;
-    const int kx = q < 16 ? q : N - KX + q;
-    se[xl][q] = twN[(kx * (x0 + xl)) % N];
+    const int q = tid / XB, xl = tid % XB;
+    se[xl][q] = Ex[(long)q * N + x0 + xl];
   }
-  const int ch = tid >> 4, ky = tid & 15;
+  const int ch = tid & 31, ky = blockIdx.y * KYG + ((tid >> 5) & (KYG - 1)), xs = tid >> 7;
   float2 vq[KX];
 #pragma unroll
   for (int q = 0; q < KX; ++q) vq[q] = V[(((long)b * KX + q) * KY + ky) * C + ch];
   __syncthreads();
-  for (int xl = 0; xl < XB; ++xl) {
+  for (int xl = xs; xl < XB; xl += 512 / (32 * KYG)) {
     float2 a = make_float2(0.f, 0.f);
 #pragma unroll
     for (int q = 0; q < KX; ++q) {
@@ -339,15 +351,18 @@ __global__ void __launch_bounds__(256) k_pixel(const bf16* __restrict__ P, const
                                                const bf16* __restrict__ A, const bf16* __restrict__ R,
                                                const float* __restrict__ bias, bf16* __restrict__ pre_out,
                                                const bf16* __restrict__ aux, bf16* __restrict__ out, int N,
-                                               long waves, const int* done) {
+                                               const int* done) {
   if (skip(done)) return;
-  const int tid = threadIdx.x, lane = tid & 63;
-  const long wv = (long)blockIdx.x * 4 + (tid >> 6);
-  if (wv >= waves) return;
+  // each wave stages its 16-cell tile (16 x 32 BF16, rows padded to 80 B) so
+  // that every lane stores 16 contiguous bytes and a wave 1 KiB at once
+  __shared__ __align__(16) bf16 stage[2][4][16][40];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const long wv = (long)blockIdx.x * 4 + w;  // every block is full: waves % 4 == 0
   const int segs = N / 64;
   const long bx = wv / segs;  // sample * N + x
   const int y0 = (int)(wv % segs) * 64;
   const int i = lane & 15, g = lane >> 4;
+  const int rrow = lane >> 2, rcol = (lane & 3) * 8;
   bf16x8 aw[2], ar[2];
   float bs[2][4];
 #pragma unroll
@@ -357,12 +372,20 @@ __global__ void __launch_bounds__(256) k_pixel(const bf16* __restrict__ P, const
 #pragma unroll
     for (int v = 0; v < 4; ++v) bs[to][v] = bias ? bias[16 * to + 4 * g + v] : 0.f;
   }
+  // all four tiles' operands in flight before the first MFMA
+  bf16x8 bpt[4], bqt[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int y = y0 + 16 * t + i;
+    bpt[t] = ld8(P + (bx * N + y) * C + 8 * g);
+    bqt[t] = ld8(Q + (long)y * C + 8 * g);
+  }
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
     const int y = y0 + 16 * t + i;
     const long p = bx * N + y;
-    const bf16x8 bp = ld8(P + p * C + 8 * g);
-    const bf16x8 bq = ld8(Q + (long)y * C + 8 * g);
+    const bf16x8 bp = bpt[t];
+    const bf16x8 bq = bqt[t];
 #pragma unroll
     for (int to = 0; to < 2; ++to) {
       f32x4 acc = {0.f, 0.f, 0.f, 0.f};
@@ -373,15 +396,20 @@ __global__ void __launch_bounds__(256) k_pixel(const bf16* __restrict__ P, const
 #pragma unroll
       for (int v = 0; v < 4; ++v) r[v] = acc[v] + bs[to][v];
       if (EPI == 1) {
-        if (pre_out) st4(pre_out + p * C + o0, r[0], r[1], r[2], r[3]);
+        if (pre_out) st4(&stage[1][w][i][o0], r[0], r[1], r[2], r[3]);
 #pragma unroll
         for (int v = 0; v < 4; ++v) r[v] = gelu(r[v]);
       } else if (EPI == 2) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) r[v] *= dgelu((float)aux[p * C + o0 + v]);
       }
-      st4(out + p * C + o0, r[0], r[1], r[2], r[3]);
+      st4(&stage[0][w][i][o0], r[0], r[1], r[2], r[3]);
     }
+    __syncthreads();
+    const long po = (bx * N + y0 + 16 * t + rrow) * C + rcol;
+    *(u32x4*)(out + po) = *(const u32x4*)&stage[0][w][rrow][rcol];
+    if (EPI == 1 && pre_out) *(u32x4*)(pre_out + po) = *(const u32x4*)&stage[1][w][rrow][rcol];
+    __syncthreads();
   }
 }
 
@@ -536,7 +564,7 @@ struct cgm_fno {
   bf16* Wn = nullptr;
   bf16* Ft = nullptr;             // 32 x N
   bf16* Fy = nullptr;             // N x 32
-  float2* twN = nullptr;          // e^{2 pi i t/N}
+  float2* Ex = nullptr;           // [32 kxi][N]: e^{2 pi i kx x/N}
   bf16* h[5] = {};                // activations [B][n][32]
   bf16* pre[3] = {};              // pre-activations (training)
   bf16* dh[2] = {};               // backward ping-pong
@@ -557,7 +585,7 @@ struct cgm_fno {
 namespace {
 
 int fno_free(cgm_fno* f) {
-  void* ptrs[] = {f->Wt, f->Wn, f->Ft, f->Fy, f->twN, f->h[0], f->h[1], f->h[2], f->h[3], f->h[4],
+  void* ptrs[] = {f->Wt, f->Wn, f->Ft, f->Fy, f->Ex, f->h[0], f->h[1], f->h[2], f->h[3], f->h[4],
                   f->pre[0], f->pre[1], f->pre[2], f->dh[0], f->dh[1], f->A, f->upart, f->U[0],
                   f->U[1], f->U[2], f->U[3], f->dV, f->Vm, f->Bt, f->part};
   for (void* p : ptrs)
@@ -582,10 +610,10 @@ int launch_rowproj(cgm_fno* f, hipStream_t st, const bf16* Y, const bf16* Z, flo
 template <int EPI>
 int launch_pixel(cgm_fno* f, hipStream_t st, const bf16* P, const bf16* A, const bf16* R, const float* bias,
                  bf16* pre_out, const bf16* aux, bf16* out, int B, const int* done) {
-  const long waves = (long)B * f->n / 64;
-  const int blocks = (int)((waves + 3) / 4);
-  hipLaunchKernelGGL((k_pixel<EPI>), dim3(blocks), dim3(256), 0, st, P, f->Fy, A, R, bias, pre_out, aux, out,
-                     f->N, waves, done);
+  const long waves = (long)B * f->n / 64;  // N % 64 == 0, so waves % 4 == 0
+  CGM_CHECK(waves % 4 == 0, CGM_ERROR_UNSUPPORTED, "cell GEMM needs N %% 64 == 0");
+  hipLaunchKernelGGL((k_pixel<EPI>), dim3((unsigned)(waves / 4)), dim3(256), 0, st, P, f->Fy, A, R, bias, pre_out,
+                     aux, out, f->N, done);
   CGM_HIP(hipGetLastError());
   return CGM_SUCCESS;
 }
@@ -593,14 +621,14 @@ int launch_pixel(cgm_fno* f, hipStream_t st, const bf16* P, const bf16* A, const
 // spectral path of one layer: A (row projections) -> Bt (per-row mixing matrices)
 int spectral(cgm_fno* f, hipStream_t st, const float2* W, float2* usave, bool bwd, int B, const int* done) {
   const int nch = f->N / XC;
-  hipLaunchKernelGGL(k_dftx, dim3(nch, KY, B), dim3(1024), 0, st, f->A, f->upart, f->twN, f->N, bwd ? 1 : 0,
-                     done);
+  hipLaunchKernelGGL(k_dftx, dim3(nch, KY, B), dim3(1024), 0, st, f->A, f->upart, f->Ex, f->N, bwd ? 1 : 0, done);
   if (!bwd)
-    hipLaunchKernelGGL((k_mix<false>), dim3(KY, B), dim3(1024), 0, st, f->upart, nch, W, usave, f->Vm, f->N,
-                       done);
+    hipLaunchKernelGGL((k_mix<false>), dim3(KY, KX / KXG, B), dim3(256), 0, st, f->upart, nch, W, usave, f->Vm,
+                       f->N, done);
   else
-    hipLaunchKernelGGL((k_mix<true>), dim3(KY, B), dim3(1024), 0, st, f->upart, nch, W, usave, f->Vm, f->N, done);
-  hipLaunchKernelGGL(k_invx, dim3(f->N / XB, B), dim3(512), 0, st, f->Vm, f->Bt, f->twN, f->N, done);
+    hipLaunchKernelGGL((k_mix<true>), dim3(KY, KX / KXG, B), dim3(256), 0, st, f->upart, nch, W, usave, f->Vm,
+                       f->N, done);
+  hipLaunchKernelGGL(k_invx, dim3(f->N / XB, KY / KYG, B), dim3(512), 0, st, f->Vm, f->Bt, f->Ex, f->N, done);
   CGM_HIP(hipGetLastError());
   return CGM_SUCCESS;
 }
@@ -616,7 +644,7 @@ int fno_forward(cgm_fno* f, hipStream_t st, const float* chi, const float2* v, c
   const long total = (long)B * f->n;
   const float s = (float)f->N;
   int rc;
-  hipLaunchKernelGGL(k_lift, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, chi, f->chi_scale,
+  hipLaunchKernelGGL(k_lift, dim3((unsigned)((4 * total + 255) / 256)), dim3(256), 0, st, chi, f->chi_scale,
                      v, nrm, vw, s, f->params, f->h[0], total, done);
   CGM_HIP(hipGetLastError());
   for (int l = 0; l < 4; ++l) {
@@ -723,7 +751,7 @@ extern "C" int cgm_fno_create(cgm_ctx_t c, int batch, float chi_scale, cgm_fno_t
   alloc((void**)&f->Wn, sizeof(bf16) * 4 * C * C);
   alloc((void**)&f->Ft, sizeof(bf16) * C * N);
   alloc((void**)&f->Fy, sizeof(bf16) * C * N);
-  alloc((void**)&f->twN, sizeof(float2) * N);
+  alloc((void**)&f->Ex, sizeof(float2) * KX * N);
   for (int l = 0; l < 5; ++l) alloc((void**)&f->h[l], act);
   for (int l = 0; l < 3; ++l) alloc((void**)&f->pre[l], act);
   for (int l = 0; l < 2; ++l) alloc((void**)&f->dh[l], act);
@@ -744,7 +772,7 @@ extern "C" int cgm_fno_create(cgm_ctx_t c, int batch, float chi_scale, cgm_fno_t
   }
   // column basis and row twiddles (computed in double, stored rounded)
   std::vector<bf16> ft((size_t)C * N), fy((size_t)C * N);
-  std::vector<float2> tw(N);
+  std::vector<float2> ex((size_t)KX * N);
   for (int y = 0; y < N; ++y) {
     for (int ky = 0; ky < KY; ++ky) {
       const double th = 2.0 * M_PI * (double)((long)ky * y % N) / N;
@@ -754,12 +782,15 @@ extern "C" int cgm_fno_create(cgm_ctx_t c, int batch, float chi_scale, cgm_fno_t
       fy[(size_t)y * C + 2 * ky] = (bf16)cs;
       fy[(size_t)y * C + 2 * ky + 1] = (bf16)sn;
     }
-    const double th = 2.0 * M_PI * y / N;
-    tw[y] = make_float2((float)cos(th), (float)sin(th));
+    for (int q = 0; q < KX; ++q) {
+      const long kx = q < 16 ? q : N - KX + q;
+      const double ph = 2.0 * M_PI * (double)(kx * y % N) / N;
+      ex[(size_t)q * N + y] = make_float2((float)cos(ph), (float)sin(ph));
+    }
   }
   CGM_HIP(hipMemcpy(f->Ft, ft.data(), sizeof(bf16) * ft.size(), hipMemcpyHostToDevice));
   CGM_HIP(hipMemcpy(f->Fy, fy.data(), sizeof(bf16) * fy.size(), hipMemcpyHostToDevice));
-  CGM_HIP(hipMemcpy(f->twN, tw.data(), sizeof(float2) * N, hipMemcpyHostToDevice));
+  CGM_HIP(hipMemcpy(f->Ex, ex.data(), sizeof(float2) * ex.size(), hipMemcpyHostToDevice));
   const size_t lds2 = (size_t)2 * N * S * sizeof(bf16);
   CGM_HIP(hipFuncSetAttribute((const void*)k_rowproj<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2));
   CGM_HIP(hipFuncSetAttribute((const void*)k_rowproj<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -829,19 +860,20 @@ extern "C" int cgm_fno_profile(cgm_fno_t f, const float* chi, const cgm_complex6
       if (r == 0) CGM_HIP(hipEventRecord(e0, st));
       switch (k) {
         case 0:
-          hipLaunchKernelGGL(k_lift, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, chi, f->chi_scale,
-                             (const float2*)v, nullptr, nullptr, s, f->params, f->h[0], total, nullptr);
+          hipLaunchKernelGGL(k_lift, dim3((unsigned)((4 * total + 255) / 256)), dim3(256), 0, st, chi,
+                             f->chi_scale, (const float2*)v, nullptr, nullptr, s, f->params, f->h[0], total, nullptr);
           break;
         case 1: rc = launch_rowproj<false>(f, st, f->h[0], nullptr, f->A, 1, nullptr); break;
         case 2:
-          hipLaunchKernelGGL(k_dftx, dim3(nch, KY, 1), dim3(1024), 0, st, f->A, f->upart, f->twN, f->N, 0, nullptr);
+          hipLaunchKernelGGL(k_dftx, dim3(nch, KY, 1), dim3(1024), 0, st, f->A, f->upart, f->Ex, f->N, 0, nullptr);
           break;
         case 3:
-          hipLaunchKernelGGL((k_mix<false>), dim3(KY, 1), dim3(1024), 0, st, f->upart, nch, W, nullptr, f->Vm, f->N,
-                             nullptr);
+          hipLaunchKernelGGL((k_mix<false>), dim3(KY, KX / KXG, 1), dim3(256), 0, st, f->upart, nch, W, nullptr,
+                             f->Vm, f->N, nullptr);
           break;
         case 4:
-          hipLaunchKernelGGL(k_invx, dim3(f->N / XB, 1), dim3(512), 0, st, f->Vm, f->Bt, f->twN, f->N, nullptr);
+          hipLaunchKernelGGL(k_invx, dim3(f->N / XB, KY / KYG, 1), dim3(512), 0, st, f->Vm, f->Bt, f->Ex, f->N,
+                             nullptr);
           break;
         case 5:
           rc = launch_pixel<1>(f, st, f->h[0], f->Wt, f->Bt, Lp + C * C, nullptr, nullptr, f->h[1], 1, nullptr);

@@ -28,6 +28,15 @@ __host__ __device__ inline P* boff(P* p, long bytes) {
   return p ? reinterpret_cast<P*>(reinterpret_cast<uintptr_t>(p) + bytes) : p;
 }
 
+// The N x M scratch of a matvec is stored in column panels of kPW columns:
+// element (r, c) sits at ((c / kPW) * N + r) * kPW + c % kPW.  Row passes
+// (1 and 3) then move runs of kPW contiguous values per row, and a column
+// pass work-group with kPW columns reads and writes one contiguous block.
+constexpr int kPW = 8;
+__host__ __device__ inline long tpos(int r, int c, int N) {
+  return ((long)(c / kPW) * N + r) * kPW + (c % kPW);
+}
+
 struct Plan {
   int N, M;
   int rows1, rows3;  // rows per work-group in passes 1 and 3

@@ -81,8 +81,7 @@ k_pass1(const cplx<T>* __restrict__ x, cplx<T>* xw, const T* __restrict__ chi,
   __syncthreads();
   lds_fft<T, M, -1>(buf, tws, t);
   if (live) {
-    cplx<T>* out = Tm + (long)row * M;
-    for (int e = t; e < M; e += NT) out[e] = buf[e];
+    for (int e = t; e < M; e += NT) Tm[tpos(row, e, N)] = buf[e];
   }
 }
 
@@ -105,7 +104,7 @@ k_pass2(cplx<T>* __restrict__ Tm, const G* __restrict__ gq, int N, int cols_per_
   // gather the column tile: consecutive threads read consecutive columns
   for (int idx = threadIdx.x; idx < N * cols_per_wg; idx += nthr) {
     const int r = idx / cols_per_wg, c = idx - r * cols_per_wg;
-    smem[c * M + r] = Tm[(long)r * M + c0 + c];
+    smem[c * M + r] = Tm[tpos(r, c0 + c, N)];
   }
   for (int idx = threadIdx.x; idx < (M - N) * cols_per_wg; idx += nthr) {
     const int r = N + idx / cols_per_wg, c = idx % cols_per_wg;
@@ -128,7 +127,7 @@ k_pass2(cplx<T>* __restrict__ Tm, const G* __restrict__ gq, int N, int cols_per_
   lds_fft<T, M, +1>(buf, tws, t);
   for (int idx = threadIdx.x; idx < N * cols_per_wg; idx += nthr) {
     const int r = idx / cols_per_wg, c = idx - r * cols_per_wg;
-    Tm[(long)r * M + c0 + c] = smem[c * M + r];
+    Tm[tpos(r, c0 + c, N)] = smem[c * M + r];
   }
 }
 
@@ -157,8 +156,7 @@ k_pass3(const cplx<T>* __restrict__ Tm, const cplx<T>* __restrict__ x, const T* 
   const bool live = row < N;
   FftTwiddles<T, M> tws;
   tws.load(tw, t);
-  const cplx<T>* in = Tm + (long)row * M;
-  for (int e = t; e < M; e += NT) buf[e] = live ? in[e] : mk(T(0), T(0));
+  for (int e = t; e < M; e += NT) buf[e] = live ? Tm[tpos(row, e, N)] : mk(T(0), T(0));
   __syncthreads();
   lds_fft<T, M, +1>(buf, tws, t);
   if (!live) return;
