@@ -55,10 +55,12 @@ def parse():
                    help="FGMRES falls back to plain GMRES when a cycle leaves more than this "
                         "fraction of the residual (0: never)")
     p.add_argument("--plain-only", action="store_true")
-    p.add_argument("--mode", choices=["bench", "step", "copy", "batched"], default="bench",
+    p.add_argument("--mode", choices=["bench", "step", "copy", "batched", "fno-step", "copybw"], default="bench",
                    help="bench: the benchmark; step: only the per-kernel profile of one Arnoldi "
                         "step (for rocprofv3 --pmc); copy: 3 copies of 256 MiB (counter calibration); "
-                        "batched: lockstep batches of 16 right-hand sides vs single solves")
+                        "batched: lockstep batches of 16 right-hand sides vs single solves; fno-step: "
+                        "the per-kernel profile of one network forward pass (for rocprofv3 --pmc); copybw: "
+                        "device-copy bandwidth at 8-256 MB")
     return p.parse_args()
 
 
@@ -85,8 +87,31 @@ def main():
     if a.mode == "batched":
         print(json.dumps(batched_throughput(a.restart)), flush=True)
         return
+    if a.mode == "copybw":
+        out = {}
+        for mb in (8, 16, 32, 64, 256):
+            src = torch.ones(mb * 1024 * 1024 // 8, dtype=torch.complex64, device="cuda")
+            dst = torch.empty_like(src)
+            for _ in range(3):
+                dst.copy_(src)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(20):
+                dst.copy_(src)
+            e1.record()
+            torch.cuda.synchronize()
+            us = e0.elapsed_time(e1) / 20 * 1e3
+            out[f"{mb}MiB"] = {"us": us, "GBps_read_plus_write": 2 * src.numel() * 8 / (us * 1e-6) / 1e9}
+        print(json.dumps({"copy_bandwidth": out}), flush=True)
+        return
     N = a.N
     ctx = solver.Context(N, 0)
+    if a.mode == "fno-step":
+        from cugmres_amd import fno
+        chi, b, _ = problems.config_plain(N, seed=0)
+        net = fno.Preconditioner(ctx, params=fno.init_params(0, zero_out=False))
+        print(json.dumps(profile_fno(ctx, net, ctx.to_device(chi.reshape(-1), "f32"), ctx.to_device(b.reshape(-1)))))
+        return
     if a.mode == "step":
         chi, b, _ = problems.config_plain(N, seed=0)
         prof = profile_kernels(ctx, ctx.spectrum_helmholtz(problems.KH),
@@ -292,10 +317,14 @@ def profile_fno(ctx, net, dchi, db):
     names = ["fno_lift", "fno_rowproj", "fno_dftx", "fno_mix", "fno_invx", "fno_cell_gemm", "fno_proj",
              "fno_forward_total"]
     # algorithmic bytes per launch: BF16 activations 64 B per cell, complex64
-    # vectors 8 B, chi 4 B; per-row mixing matrices 2 KiB per grid row
+    # vectors 8 B, chi 4 B; FP32 row projections 4 KiB per grid row, x-DFT
+    # partials 128 KiB per 32 rows, spectral weights 4 MiB per layer, mixed
+    # modes 128 KiB, row twiddles 256 B per row, BF16 per-row mixing matrices
+    # 2 KiB per row
+    part = (N // 32) * 16384 * 8
     bytes_ = {"fno_lift": n * (4 + 8 + 64), "fno_rowproj": n * 64 + N * 4096,
-              "fno_dftx": N * 4096 + (N // 32) * 16384 * 8, "fno_mix": (N // 32) * 16384 * 8 + 16384 * 16,
-              "fno_invx": 16384 * 8 + N * 2048, "fno_cell_gemm": n * 128 + N * 2048,
+              "fno_dftx": N * 4096 + N * 256 + part, "fno_mix": part + 4 * 1024 * 1024 + 16384 * 8,
+              "fno_invx": 16384 * 8 + N * 256 + N * 2048, "fno_cell_gemm": n * 128 + N * 2048,
               "fno_proj": n * (64 + 16)}
     per_step = {"fno_lift": 1, "fno_rowproj": 4, "fno_dftx": 4, "fno_mix": 4, "fno_invx": 4,
                 "fno_cell_gemm": 4, "fno_proj": 1}

@@ -85,6 +85,7 @@ SIGNATURES = {
     "cgm_matvec_c128": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int]),
     "cgm_gmres": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                  ctypes.POINTER(GmresParams), ctypes.POINTER(GmresResult), c_void_p]),
+    "cgm_set_batched_graphs": (ctypes.c_int, [c_void_p, ctypes.c_int]),
     "cgm_gmres_batched": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_long, c_void_p, c_void_p,
                                          ctypes.c_int, ctypes.POINTER(GmresParams), ctypes.POINTER(GmresResult),
                                          c_void_p]),

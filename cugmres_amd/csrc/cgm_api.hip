@@ -100,6 +100,7 @@ extern "C" int cgm_destroy(cgm_ctx_t c) {
     if (p) (void)hipFree(p);
   if (c->hst) (void)hipHostFree(c->hst);
   if (c->hstB) (void)hipHostFree(c->hstB);
+  if (c->gstream) (void)hipStreamDestroy(c->gstream);
   delete c;
   return CGM_SUCCESS;
 }
@@ -520,9 +521,15 @@ static int copy_strided(hipStream_t st, const float2* src, long ss, float2* dst,
   return CGM_SUCCESS;
 }
 
-static int read_states(cgm_context* c, int B) {
-  CGM_HIP(hipMemcpyAsync(c->hstB, c->stB, sizeof(DevState) * B, hipMemcpyDeviceToHost, c->stream));
-  CGM_HIP(hipStreamSynchronize(c->stream));
+static int read_states(cgm_context* c, int B, hipStream_t st) {
+  CGM_HIP(hipMemcpyAsync(c->hstB, c->stB, sizeof(DevState) * B, hipMemcpyDeviceToHost, st));
+  CGM_HIP(hipStreamSynchronize(st));
+  return CGM_SUCCESS;
+}
+
+extern "C" int cgm_set_batched_graphs(cgm_ctx_t c, int enable) {
+  CGM_CHECK(c, CGM_ERROR_INVALID_VALUE, "null context");
+  c->use_graphs = enable ? 1 : 0;
   return CGM_SUCCESS;
 }
 
@@ -594,22 +601,64 @@ extern "C" int cgm_gmres_batched(cgm_ctx_t c, const cgm_complex64* gamma, const 
   // u_0 = r and the first matvec's passes 1-2
   BatchStrides us = rs;
   us.x = 2 * n;
+  // one restart cycle of every live system (converged ones are skipped by
+  // their done flags), then the next cycle's residual and its norm; the
+  // launch sequence and its pointers are the same every cycle
+  auto enqueue_cycle = [&](hipStream_t s, int cyc) -> int {
+    int r = copy_strided(s, ws.V, ws.sBasis, ws.U, 2 * n, n, B);
+    if (!r) r = launch_pass1<float>(c->plan, s, ws.U, nullptr, chi, nullptr, kModeForward, c->T32B, c->tw32,
+                                    &stB->done, B, us);
+    if (!r) r = launch_pass2<float, float2>(c->plan, s, c->T32B, gq, c->tw32, &stB->done, B, us);
+    if (r) return r;
+    FusedLaunch L{};
+    L.N = c->N; L.m = m; L.ld = ws.m; L.mode = kModeForward; L.chi = chi; L.V = ws.V; L.Z = ws.Z;
+    L.T = c->T32B; L.tw = c->tw32; L.part = ws.part; L.H = ws.H; L.Rm = ws.R; L.cs = ws.cs;
+    L.sn = ws.sn; L.g = ws.g; L.h1p = ws.h1p; L.a1 = ws.a1; L.a2 = ws.a2; L.scal = ws.scal;
+    L.st = stB; L.hist = hist ? hist + (long)cyc * m : nullptr;
+    L.B = B; L.sChi = chi_stride; L.sBasis = ws.sBasis; L.sU = 2 * n; L.sVec = n; L.sT = NM;
+    L.sPart = ws.sPart; L.sSmall = ws.sSmall; L.sSt = sst; L.sHist = hstride;
+    for (int j = 0; j <= m; ++j) {
+      L.j = j;
+      L.Uj = ws.U + (long)(j & 1) * n;
+      L.Unext = ws.U + (long)((j + 1) & 1) * n;
+      if ((r = launch_fused(s, 0, L))) return r;
+      if (j == m) break;
+      if ((r = launch_fused(s, 1, L))) return r;
+      if ((r = launch_pass2<float, float2>(c->plan, s, c->T32B, gq, c->tw32, &stB->done, B, us))) return r;
+    }
+    if ((r = launch_backsub_update(s, ws, stB, xx, nullptr, n, sst))) return r;
+    r = launch_pass1<float>(c->plan, s, xx, nullptr, chi, nullptr, kModeResidual, c->T32B, c->tw32, nullptr, B, rs);
+    if (!r) r = launch_pass2<float, float2>(c->plan, s, c->T32B, gq, c->tw32, nullptr, B, rs);
+    if (!r) r = launch_pass3<float>(c->plan, s, c->T32B, xx, chi, bb, ws.V, kModeResidual, c->tw32, nullptr, B, rs);
+    if (!r) r = launch_norm_init(s, ws, ws.V, stB, 0, 0.0, ws.sBasis, sst);
+    return r;
+  };
+  // first residual, eagerly on the context stream
+  if (p->x0_zero) {
+    if ((rc = copy_strided(c->stream, bb, n, ws.V, ws.sBasis, n, B))) return rc;
+  } else {
+    rc = launch_pass1<float>(c->plan, c->stream, xx, nullptr, chi, nullptr, kModeResidual, c->T32B, c->tw32,
+                             nullptr, B, rs);
+    if (!rc) rc = launch_pass2<float, float2>(c->plan, c->stream, c->T32B, gq, c->tw32, nullptr, B, rs);
+    if (!rc)
+      rc = launch_pass3<float>(c->plan, c->stream, c->T32B, xx, chi, bb, ws.V, kModeResidual, c->tw32, nullptr, B,
+                               rs);
+    if (rc) return rc;
+  }
+  if ((rc = launch_norm_init(c->stream, ws, ws.V, stB, 0, 0.0, ws.sBasis, sst))) return rc;
+  if ((rc = read_states(c, B, c->stream))) return rc;
+  // captured cycles run on the context's own non-blocking stream (a capture
+  // needs one; the caller's stream may be the null stream)
+  const bool graph = c->use_graphs && !hist;
+  hipStream_t cs = c->stream;
+  hipGraphExec_t exec = nullptr;
+  if (graph) {
+    if (!c->gstream) CGM_HIP(hipStreamCreateWithFlags(&c->gstream, hipStreamNonBlocking));
+    cs = c->gstream;
+  }
   std::vector<int> done(B, 0);
   int cycles = 0;
   for (;;) {
-    if (p->x0_zero && cycles == 0) {
-      if ((rc = copy_strided(c->stream, bb, n, ws.V, ws.sBasis, n, B))) return rc;
-    } else {
-      rc = launch_pass1<float>(c->plan, c->stream, xx, nullptr, chi, nullptr, kModeResidual, c->T32B, c->tw32,
-                               nullptr, B, rs);
-      if (!rc) rc = launch_pass2<float, float2>(c->plan, c->stream, c->T32B, gq, c->tw32, nullptr, B, rs);
-      if (!rc)
-        rc = launch_pass3<float>(c->plan, c->stream, c->T32B, xx, chi, bb, ws.V, kModeResidual, c->tw32, nullptr,
-                                 B, rs);
-      if (rc) return rc;
-    }
-    if ((rc = launch_norm_init(c->stream, ws, ws.V, stB, 0, 0.0, ws.sBasis, sst))) return rc;
-    if ((rc = read_states(c, B))) return rc;
     int active = 0;
     for (int k = 0; k < B; ++k) {
       const DevState& h = c->hstB[k];
@@ -619,6 +668,9 @@ extern "C" int cgm_gmres_batched(cgm_ctx_t c, const cgm_complex64* gamma, const 
         r.bnorm = h.bnorm;
         if (hist) CGM_HIP(hipMemcpyAsync(hist + k * hstride, &r.beta0, sizeof(double), hipMemcpyHostToDevice,
                                          c->stream));
+      } else if (!done[k]) {
+        r.iterations += h.k_last;
+        r.cycles += 1;
       }
       if (done[k]) continue;
       r.rel_residual = h.bnorm > 0 ? h.beta / h.bnorm : 0.0;
@@ -634,40 +686,37 @@ extern "C" int cgm_gmres_batched(cgm_ctx_t c, const cgm_complex64* gamma, const 
       ++active;
     }
     if (!active || cycles >= p->max_cycles) break;
-    // one restart cycle of every live system (converged ones are skipped by
-    // their done flags), enqueued without host synchronisation
-    if ((rc = copy_strided(c->stream, ws.V, ws.sBasis, ws.U, 2 * n, n, B))) return rc;
-    rc = launch_pass1<float>(c->plan, c->stream, ws.U, nullptr, chi, nullptr, kModeForward, c->T32B, c->tw32,
-                             &stB->done, B, us);
-    if (!rc) rc = launch_pass2<float, float2>(c->plan, c->stream, c->T32B, gq, c->tw32, &stB->done, B, us);
-    if (rc) return rc;
-    FusedLaunch L{};
-    L.N = c->N; L.m = m; L.ld = ws.m; L.mode = kModeForward; L.chi = chi; L.V = ws.V; L.Z = ws.Z;
-    L.T = c->T32B; L.tw = c->tw32; L.part = ws.part; L.H = ws.H; L.Rm = ws.R; L.cs = ws.cs;
-    L.sn = ws.sn; L.g = ws.g; L.h1p = ws.h1p; L.a1 = ws.a1; L.a2 = ws.a2; L.scal = ws.scal;
-    L.st = stB; L.hist = hist ? hist + (long)cycles * m : nullptr;
-    L.B = B; L.sChi = chi_stride; L.sBasis = ws.sBasis; L.sU = 2 * n; L.sVec = n; L.sT = NM;
-    L.sPart = ws.sPart; L.sSmall = ws.sSmall; L.sSt = sst; L.sHist = hstride;
-    for (int j = 0; j <= m; ++j) {
-      L.j = j;
-      L.Uj = ws.U + (long)(j & 1) * n;
-      L.Unext = ws.U + (long)((j + 1) & 1) * n;
-      if ((rc = launch_fused(c->stream, 0, L))) return rc;
-      if (j == m) break;
-      if ((rc = launch_fused(c->stream, 1, L))) return rc;
-      if ((rc = launch_pass2<float, float2>(c->plan, c->stream, c->T32B, gq, c->tw32, &stB->done, B, us)))
-        return rc;
+    if (graph) {
+      if (!exec) {
+        hipGraph_t gr = nullptr;
+        CGM_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+        rc = enqueue_cycle(cs, 0);
+        const hipError_t ec = hipStreamEndCapture(cs, &gr);
+        if (rc) {
+          if (gr) (void)hipGraphDestroy(gr);
+          return rc;
+        }
+        CGM_HIP(ec);
+        const hipError_t ei = hipGraphInstantiate(&exec, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        CGM_HIP(ei);
+      }
+      const hipError_t el = hipGraphLaunch(exec, cs);
+      if (el != hipSuccess) {
+        (void)hipGraphExecDestroy(exec);
+        CGM_HIP(el);
+      }
+    } else if ((rc = enqueue_cycle(cs, cycles))) {
+      return rc;
     }
-    if ((rc = launch_backsub_update(c->stream, ws, stB, xx, nullptr, n, sst))) return rc;
-    if ((rc = read_states(c, B))) return rc;
-    for (int k = 0; k < B; ++k) {
-      if (done[k]) continue;
-      results[k].iterations += c->hstB[k].k;
-      results[k].cycles += 1;
-      if (c->hstB[k].status) done[k] = 1;
+    rc = read_states(c, B, cs);
+    if (rc) {
+      if (exec) (void)hipGraphExecDestroy(exec);
+      return rc;
     }
     ++cycles;
   }
+  if (exec) CGM_HIP(hipGraphExecDestroy(exec));
   int any_nonfinite = 0, all_conv = 1;
   for (int k = 0; k < B; ++k) {
     if (c->hstB[k].status) any_nonfinite = 1;

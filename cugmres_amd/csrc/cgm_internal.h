@@ -76,7 +76,7 @@ struct DevState {
   int done;        // 1: remaining launches of this cycle are no-ops
   int k;           // valid Arnoldi columns of this cycle
   int status;      // 0 or CGM_ERROR_NONFINITE
-  int pad;
+  int k_last;      // k of the last back substitution (kept across the next cycle's reset)
   unsigned tickets[8];
 };
 
@@ -195,4 +195,6 @@ struct cgm_context {
   int stB_n = 0;
   float2* T32B = nullptr;         // batched FFT scratch (T32B_n systems x N x M)
   int T32B_n = 0;
+  hipStream_t gstream = nullptr;  // non-blocking stream of captured batched cycles (lazy)
+  int use_graphs = 1;             // capture and replay batched restart cycles
 };

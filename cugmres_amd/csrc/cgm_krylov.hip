@@ -285,7 +285,7 @@ k_norm64(const double2* __restrict__ v, long n, double* partn, int G, DevState* 
 }
 
 // Solve R y = g[0:k] (upper triangular, row-major ld m) in FP64.
-__global__ void __launch_bounds__(kKB) k_backsub(KrylovWs ws, const DevState* st, int m, long sst) {
+__global__ void __launch_bounds__(kKB) k_backsub(KrylovWs ws, DevState* st, int m, long sst) {
   {
     const long b = blockIdx.y;
     ws.g += b * ws.sSmall;
@@ -294,6 +294,7 @@ __global__ void __launch_bounds__(kKB) k_backsub(KrylovWs ws, const DevState* st
     st = boff(st, b * sst);
   }
   const int k = st->k;
+  if (threadIdx.x == 0) st->k_last = k;
   __shared__ double2 gs[kMaxRestart];
   __shared__ double2 yi;
   for (int j = threadIdx.x; j < k; j += kKB) gs[j] = ws.g[j];

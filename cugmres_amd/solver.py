@@ -150,6 +150,11 @@ class Context:
             info["history"] = hist[: r.iterations + 1].cpu().numpy()
         return x, info
 
+    def set_batched_graphs(self, enable):
+        """1 (default): batched restart cycles are captured once per call into a
+        HIP graph and replayed; 0: launched one by one."""
+        check(lib().cgm_set_batched_graphs(self._ptr, int(bool(enable))), "cgm_set_batched_graphs")
+
     def gmres_batched(self, gamma, chi, b, x=None, restart=30, tol=1e-6, max_cycles=100, history=False):
         """Lockstep GMRES(restart) of B systems: b (B, N*N) complex64, chi either
         (B, N*N) (own maps) or (N*N,) (one map, multiple right-hand sides).

@@ -170,8 +170,13 @@ int cgm_gmres(cgm_ctx_t ctx, const cgm_complex64* dev_gamma, const float* dev_ch
  * (optional) is [batch][restart*max_cycles + 1]: [0] = ||r0||, entry
  * c*restart + j = estimate after j steps of cycle c.  Power-of-two N in
  * [64, 1024] with restart <= 64 run batched; other sizes solve the systems
- * one after another.  No counterpart in the reference, which solves one
- * system per call (B/CUDA_GMRES/GMRES.cuh:22-35). */
+ * one after another.  Without dev_hist the launches of one restart cycle
+ * (and the next cycle's residual) are captured once per call into a HIP
+ * graph on the context's own non-blocking stream and replayed every cycle;
+ * cgm_set_batched_graphs(ctx, 0) launches them one by one instead.  No
+ * counterpart in the reference, which solves one system per call
+ * (B/CUDA_GMRES/GMRES.cuh:22-35). */
+int cgm_set_batched_graphs(cgm_ctx_t ctx, int enable);
 int cgm_gmres_batched(cgm_ctx_t ctx, const cgm_complex64* dev_gamma, const float* dev_chi, long chi_stride,
                       const cgm_complex64* dev_b, cgm_complex64* dev_x, int batch,
                       const cgm_gmres_params* params, cgm_gmres_result* results, double* dev_hist);

@@ -96,3 +96,36 @@ def test_batched_zero_rhs_and_non_pow2(sol):
     assert infos[0]["converged"] and infos[2]["converged"]
     # N = 96 has no fused kernels: the systems run one after another
     _check(sol, 96, 2, 2.25, False, 20, 20)
+
+
+def test_batched_graph_replays_match_eager_launches(sol):
+    # without history the cycles are captured once and replayed; every replay
+    # must do exactly what the one-by-one launches do (same kernels, same
+    # order: bitwise equal results), over several cycles with host reads and
+    # eager work between the replays
+    import torch
+    from cugmres_amd import problems
+    N, B = 64, 6
+    ctx = sol.Context(N)
+    g = ctx.spectrum_helmholtz(problems.KH)
+    chi, b = _systems(N, B, 2.25, 11)
+    chi_d = torch.from_numpy(chi.reshape(B, -1).astype(np.float32)).cuda()
+    b_d = torch.from_numpy(b.reshape(B, -1).astype(np.complex64)).cuda()
+    ctx.set_batched_graphs(1)
+    xg, ig = ctx.gmres_batched(g, chi_d, b_d, restart=10, tol=1e-7, max_cycles=12)
+    ctx.set_batched_graphs(0)
+    xe, ie = ctx.gmres_batched(g, chi_d, b_d, restart=10, tol=1e-7, max_cycles=12)
+    ctx.set_batched_graphs(1)
+    assert max(i["cycles"] for i in ig) > 2
+    for k in range(B):
+        assert ig[k]["iterations"] == ie[k]["iterations"]
+        assert ig[k]["converged"] == ie[k]["converged"]
+        assert ig[k]["rel_residual"] == ie[k]["rel_residual"]
+    assert torch.equal(xg, xe)
+    # and a second graph call on the same context (new capture) agrees too
+    xg2, ig2 = ctx.gmres_batched(g, chi_d, b_d, restart=10, tol=1e-7, max_cycles=12)
+    assert torch.equal(xg2, xe)
+    for k in range(B):
+        xs, i1 = ctx.gmres(g, chi_d[k].contiguous(), b_d[k].contiguous(), restart=10, tol=1e-7, max_cycles=12)
+        assert i1["iterations"] == ig[k]["iterations"]
+        assert rel(xg[k].cpu().numpy(), xs.cpu().numpy()) < 1e-6
