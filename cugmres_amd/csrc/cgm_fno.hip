@@ -75,15 +75,21 @@ __device__ inline void st4(bf16* p, float a, float b, float c, float d) {
   *(Q*)p = q;
 }
 
+// GELU in its tanh form, 0.5 x (1 + tanh(u)), u = sqrt(2/pi) (x + 0.044715 x^3),
+// evaluated as x * sigma(2u) with sigma(t) = 1 / (1 + e^-t) (the same function:
+// 0.5 (1 + tanh(u)) = sigma(2u)), so one hardware exp and one reciprocal
+// replace the tanh: the cell GEMM evaluates 32 of them per cell.
 constexpr float kG0 = 0.7978845608028654f;  // sqrt(2/pi)
 constexpr float kG1 = 0.044715f;
-__device__ inline float gelu(float x) {
-  const float t = tanhf(kG0 * (x + kG1 * x * x * x));
-  return 0.5f * x * (1.f + t);
+__device__ inline float sig2u(float x) {
+  const float u2 = 2.f * kG0 * (x + kG1 * x * x * x);
+  return __frcp_rn(1.f + __expf(-u2));
 }
+__device__ inline float gelu(float x) { return x * sig2u(x); }
+// d/dx [x sigma(2u)] = sigma + x sigma (1 - sigma) 2u'
 __device__ inline float dgelu(float x) {
-  const float t = tanhf(kG0 * (x + kG1 * x * x * x));
-  return 0.5f * (1.f + t) + 0.5f * x * (1.f - t * t) * kG0 * (1.f + 3.f * kG1 * x * x);
+  const float sg = sig2u(x);
+  return sg + x * sg * (1.f - sg) * 2.f * kG0 * (1.f + 3.f * kG1 * x * x);
 }
 
 __device__ inline bool skip(const int* done) { return done && *done; }
