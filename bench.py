@@ -207,6 +207,7 @@ def main():
     prof = profile_kernels(ctx, g, inputs[0][0], inputs[0][1], a.restart)
     fprof = profile_fno(ctx, net, inputs[0][0], inputs[0][1]) if net is not None else None
     roof = prof["roofline"]
+    roof["traffic"] = measured_traffic(roof["kernel"], N)
     if fprof is not None:
         # per FGMRES step: one forward pass (4 row projections, 4 cell GEMMs, ...),
         # one matvec, one two-pass CGS2 step; the heaviest kernel class by time
@@ -218,7 +219,7 @@ def main():
         src = fprof["kernels"] if dom in fprof["kernels"] else prof["kernels"]
         ach = src[dom]["GBps"]
         roof = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                "frac": ach / HBM_PEAK_GBS, "traffic": measured_traffic(dom, N),
                 "per_unit": f"{src[dom]['bytes']} B per launch at N={N}",
                 "us_per_launch": src[dom]["us"], "us_per_fgmres_step": per_step}
 
@@ -298,6 +299,21 @@ def batched_throughput(restart, cycles=3, B=16):
                            "us_per_system_step_single": ts / steps * 1e6}
         ctx.close()
     return out
+
+
+def measured_traffic(kernel, N):
+    """HBM bytes per launch of `kernel` at grid size N as measured by rocprofv3
+    (FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00, copy-calibrated) and committed in
+    profiles/hbm_traffic_n512.json; None when no measurement matches."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_n512.json")
+    try:
+        with open(path) as f:
+            t = json.load(f).get(kernel)
+    except (OSError, ValueError):
+        return None
+    if not t or t.get("N") != N:
+        return None
+    return t["bytes_per_launch"]
 
 
 def profile_fno(ctx, net, dchi, db):
