@@ -82,6 +82,8 @@ SIGNATURES = {
                                               ctypes.c_double, c_void_p, c_void_p]),
     "cgm_spectrum_helmholtz": (ctypes.c_int, [c_void_p, ctypes.c_double, c_void_p, c_void_p]),
     "cgm_matvec": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int]),
+    "cgm_matvec_batched": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_long, c_void_p, c_void_p,
+                                          ctypes.c_int, ctypes.c_int]),
     "cgm_matvec_c128": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int]),
     "cgm_gmres": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                  ctypes.POINTER(GmresParams), ctypes.POINTER(GmresResult), c_void_p]),

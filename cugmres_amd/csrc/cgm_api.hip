@@ -178,7 +178,7 @@ extern "C" int cgm_autotune(cgm_ctx_t c) {
   float best1 = 1e30f, best2 = 1e30f, best3 = 1e30f;
   int r1 = P.rows1, c2 = P.cols2, r3 = P.rows3;
   for (int k : cand) {
-    if (k * nt > pass_max_threads(M, false) || (size_t)k * M * sizeof(float2) > 65536) continue;
+    if (k * nt > pass_max_threads(M, false) || (size_t)k * lds_len(M, false) * sizeof(float2) > 65536) continue;
     Plan Q = P;
     Q.rows1 = Q.rows3 = k;
     const float t1 = time_it([&] { rc |= launch_pass1<float>(Q, c->stream, x, nullptr, chi, nullptr, kModeForward, c->T32, c->tw32, nullptr); });
@@ -359,6 +359,42 @@ extern "C" int cgm_matvec(cgm_ctx_t c, const cgm_complex64* gamma, const float* 
   CGM_HIP(hipSetDevice(c->device));
   return matvec32(c, (const float2*)gamma, chi, (const float2*)x, nullptr, (float2*)y, mode,
                   nullptr, nullptr, nullptr);
+}
+
+static int ensure_T32B(cgm_context* c, int B) {
+  if (c->T32B_n >= B) return CGM_SUCCESS;
+  if (c->T32B) CGM_HIP(hipFree(c->T32B));
+  c->T32B = nullptr;
+  c->T32B_n = 0;
+  CGM_HIP(hipMalloc((void**)&c->T32B, sizeof(float2) * (size_t)c->N * c->M * B));
+  c->T32B_n = B;
+  return CGM_SUCCESS;
+}
+
+extern "C" int cgm_matvec_batched(cgm_ctx_t c, const cgm_complex64* gamma, const float* chi, long chi_stride,
+                                  const cgm_complex64* x, cgm_complex64* y, int batch, int mode) {
+  CGM_CHECK(c && gamma && x && y, CGM_ERROR_INVALID_VALUE, "null argument");
+  CGM_CHECK(batch >= 1, CGM_ERROR_INVALID_VALUE, "batch %d < 1", batch);
+  CGM_CHECK(mode >= 0 && mode <= 2, CGM_ERROR_INVALID_VALUE, "mode %d", mode);
+  CGM_CHECK(mode == CGM_OP_GREEN || chi, CGM_ERROR_INVALID_VALUE, "null contrast");
+  CGM_CHECK(chi_stride == 0 || chi_stride >= (long)c->N * c->N, CGM_ERROR_INVALID_VALUE,
+            "chi_stride must be 0 (shared map) or >= N*N");
+  CGM_HIP(hipSetDevice(c->device));
+  int rc = ensure_T32B(c, batch);
+  if (rc) return rc;
+  const long n = (long)c->N * c->N;
+  BatchStrides bs;
+  bs.x = n;
+  bs.chi = chi_stride;
+  bs.y = n;
+  bs.T = (long)c->N * c->M;
+  rc = launch_pass1<float>(c->plan, c->stream, (const float2*)x, nullptr, chi, nullptr, mode, c->T32B, c->tw32,
+                           nullptr, batch, bs);
+  if (!rc) rc = launch_pass2<float, float2>(c->plan, c->stream, c->T32B, (const float2*)gamma, c->tw32, nullptr,
+                                            batch, bs);
+  if (!rc) rc = launch_pass3<float>(c->plan, c->stream, c->T32B, (const float2*)x, chi, nullptr, (float2*)y, mode,
+                                    c->tw32, nullptr, batch, bs);
+  return rc;
 }
 
 extern "C" int cgm_matvec_c128(cgm_ctx_t c, const cgm_complex128* gamma, const double* chi,
@@ -574,13 +610,7 @@ extern "C" int cgm_gmres_batched(cgm_ctx_t c, const cgm_complex64* gamma, const 
     c->stB_n = B;
   }
   const long NM = (long)c->N * c->M;
-  if (c->T32B_n < B) {
-    if (c->T32B) CGM_HIP(hipFree(c->T32B));
-    c->T32B = nullptr;
-    c->T32B_n = 0;
-    CGM_HIP(hipMalloc((void**)&c->T32B, sizeof(float2) * NM * B));
-    c->T32B_n = B;
-  }
+  if ((rc = ensure_T32B(c, B))) return rc;
   const long sst = (long)sizeof(DevState);
   const float2* gq = (const float2*)gamma;
   const float2* bb = (const float2*)b;

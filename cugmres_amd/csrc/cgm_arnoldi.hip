@@ -124,10 +124,11 @@ __global__ void __launch_bounds__(kFBS) k_arn_a(FusedArgs a) {
   const int t = tid % NT, rl = fa ? tid / NT : 0;
   FftTwiddles<float, M> tws;
   if (fa) tws.load(a.tw, t);
-  float2* buf = smem + rl * M;
+  constexpr int ML = LdsLen<float, M>::value;
+  float2* buf = smem + rl * ML;
   if (fa) {
     const int row = r0 + rl;
-    for (int e = t; e < M; e += NT) buf[e] = row < N ? a.T[tpos(row, e, N)] : make_float2(0.f, 0.f);
+    for (int e = t; e < M; e += NT) buf[lpad<float>(e)] = row < N ? a.T[tpos(row, e, N)] : make_float2(0.f, 0.f);
   }
   __syncthreads();
   lds_fft<float, M, +1>(buf, tws, t, fa);
@@ -144,7 +145,7 @@ __global__ void __launch_bounds__(kFBS) k_arn_a(FusedArgs a) {
     if (row < N) {
       const long id = (long)row * N + c;
       const float2 u = a.Uj[id];
-      float2 tv = smem[r * M + c];
+      float2 tv = smem[r * ML + lpad<float>(c)];
       tv.x *= a.inv_mm;
       tv.y *= a.inv_mm;
       if (a.mode == kModeAdjoint) {
@@ -429,19 +430,19 @@ __global__ void __launch_bounds__(kFBS) k_arn_b(FusedArgs a) {
         w.y *= ch;
       }
     }
-    smem[r * M + c] = w;
+    smem[r * LdsLen<float, M>::value + lpad<float>(c)] = w;
   }
   for (int e = tid; e < R * (M - N); e += kFBS) {
     const int r = e / (M - N), c = N + e % (M - N);
-    smem[r * M + c] = make_float2(0.f, 0.f);
+    smem[r * LdsLen<float, M>::value + lpad<float>(c)] = make_float2(0.f, 0.f);
   }
   __syncthreads();
-  float2* buf = smem + rl * M;
+  float2* buf = smem + rl * LdsLen<float, M>::value;
   lds_fft<float, M, -1>(buf, tws, t, fa);
   if (fa) {
     const int row = r0 + rl;
     if (row < N) {
-      for (int e = t; e < M; e += NT) a.T[tpos(row, e, N)] = buf[e];
+      for (int e = t; e < M; e += NT) a.T[tpos(row, e, N)] = buf[lpad<float>(e)];
     }
   }
 }
@@ -501,7 +502,7 @@ int launch_fused(hipStream_t st, int which, const FusedLaunch& L) {
   a.sSmall = L.sSmall;
   a.sSt = L.sSt;
   a.sHist = L.sHist;
-  const size_t lds = (size_t)a.R * M * sizeof(float2);
+  const size_t lds = (size_t)a.R * lds_len(M, false) * sizeof(float2);
   switch (L.N) {
 #define CGM_FUSED_CASE(NN, EPD)                                                                 \
   case NN:                                                                                      \

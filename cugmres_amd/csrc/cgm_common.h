@@ -44,6 +44,11 @@ template <typename V> __host__ __device__ inline V cmul(V a, V b) {
 }
 template <typename V, typename S> __host__ __device__ inline V cscale(V a, S s) { V r; r.x = a.x * s; r.y = a.y * s; return r; }
 
+// Complex elements one M-point transform occupies in LDS: single-precision
+// transforms keep one pad element after every 8 (see cgm_fft.h), double
+// transforms are unpadded.
+inline int lds_len(int M, bool fp64) { return fp64 ? M : M + M / 8; }
+
 // Work-group id remapped so that consecutive logical tiles land on the same
 // XCD: the dispatcher places work-group b on XCD (b % 8), so logical tile
 // (b % 8) * (G / 8) + b / 8 gives each XCD a contiguous range of tiles and

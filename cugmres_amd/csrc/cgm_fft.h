@@ -12,6 +12,19 @@
 
 namespace cgm {
 
+// ---- LDS layout ---------------------------------------------------------------
+// Single-precision transforms keep one pad element after every 8 in LDS:
+// element e sits at e + e/8.  The first radix-8 stage writes x[8j + r] (64-byte
+// lane stride for complex64), which without the pad lands every lane of a
+// wave in the same few banks; with it the stride is 72 bytes and the writes
+// spread over all banks, while stride-1 reads stay contiguous.  Double
+// transforms (the one-off spectrum and the FP64 refinement) are unpadded so
+// that a 4096-point row still fits 64 KiB.
+template <typename T> __host__ __device__ constexpr int lpad(int e) { return sizeof(T) == 4 ? e + (e >> 3) : e; }
+template <typename T, int M> struct LdsLen {
+  static constexpr int value = sizeof(T) == 4 ? M + M / 8 : M;
+};
+
 // ---- radix plans ------------------------------------------------------------
 // Elements per thread: 8 for powers of two (radices 8/4/2), 12 for 3*2^k
 // (radices 3/4/2).  Radices must divide EPT so that every thread owns exactly
@@ -171,7 +184,7 @@ __device__ inline void fft_stage(cplx<T>* buf, const cplx<T>* reg, int t, bool a
   for (int b = 0; b < NB; ++b) {
     const int j = t + b * NT;
 #pragma unroll
-    for (int r = 0; r < R; ++r) v[b][r] = buf[j + r * MR];
+    for (int r = 0; r < R; ++r) v[b][r] = buf[lpad<T>(j + r * MR)];
   }
   __syncthreads();
 #pragma unroll
@@ -189,7 +202,7 @@ __device__ inline void fft_stage(cplx<T>* buf, const cplx<T>* reg, int t, bool a
     dft<T, SIGN, R>(v[b]);
     const int base = (j / NS) * NS * R + k;
 #pragma unroll
-    for (int r = 0; r < R; ++r) buf[base + r * NS] = v[b][r];
+    for (int r = 0; r < R; ++r) buf[lpad<T>(base + r * NS)] = v[b][r];
   }
   __syncthreads();
 }
@@ -212,7 +225,8 @@ template <typename T, int M> struct FftTwiddles {
   __device__ inline void load(const cplx<T>* __restrict__ tw, int t) { fft_load_tw<T, M, 0>(reg, tw, t); }
 };
 
-// Full transform of buf[0..M) (LDS) by threads t in [0, NT).  The caller
+// Full transform of the M elements of buf (LDS, padded layout lpad<T>) by
+// threads t in [0, NT).  The caller
 // must have synchronised the work-group after filling buf; on return buf
 // holds the result and the group is synchronised.
 template <typename T, int M, int SIGN>

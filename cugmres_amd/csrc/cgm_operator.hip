@@ -56,7 +56,8 @@ k_pass1(const cplx<T>* __restrict__ x, cplx<T>* xw, const T* __restrict__ chi,
   const int t = threadIdx.x % NT;
   const int rl = threadIdx.x / NT;
   const int row = blockIdx.x * rows_per_wg + rl;
-  cplx<T>* buf = smem + rl * M;
+  constexpr int ML = LdsLen<T, M>::value;
+  cplx<T>* buf = smem + rl * ML;
   T s = T(1);
   if (norm_ptr) {
     const double nv = *norm_ptr;
@@ -76,12 +77,12 @@ k_pass1(const cplx<T>* __restrict__ x, cplx<T>* xw, const T* __restrict__ chi,
       }
       u = (mode == kModeForward || mode == kModeResidual) ? cscale(v, chi[rb + e]) : v;
     }
-    buf[e] = u;
+    buf[lpad<T>(e)] = u;
   }
   __syncthreads();
   lds_fft<T, M, -1>(buf, tws, t);
   if (live) {
-    for (int e = t; e < M; e += NT) Tm[tpos(row, e, N)] = buf[e];
+    for (int e = t; e < M; e += NT) Tm[tpos(row, e, N)] = buf[lpad<T>(e)];
   }
 }
 
@@ -96,6 +97,7 @@ k_pass2(cplx<T>* __restrict__ Tm, const G* __restrict__ gq, int N, int cols_per_
   cplx<T>* smem = reinterpret_cast<cplx<T>*>(smem_raw);
   constexpr int NT = FftShape<M>::NT;
   constexpr int H = M / 2 + 1;
+  constexpr int ML = LdsLen<T, M>::value;
   const int grp = xcd_tile(blockIdx.x, gridDim.x);
   const int c0 = grp * cols_per_wg;
   const int nthr = blockDim.x;
@@ -104,16 +106,16 @@ k_pass2(cplx<T>* __restrict__ Tm, const G* __restrict__ gq, int N, int cols_per_
   // gather the column tile: consecutive threads read consecutive columns
   for (int idx = threadIdx.x; idx < N * cols_per_wg; idx += nthr) {
     const int r = idx / cols_per_wg, c = idx - r * cols_per_wg;
-    smem[c * M + r] = Tm[tpos(r, c0 + c, N)];
+    smem[c * ML + lpad<T>(r)] = Tm[tpos(r, c0 + c, N)];
   }
   for (int idx = threadIdx.x; idx < (M - N) * cols_per_wg; idx += nthr) {
     const int r = N + idx / cols_per_wg, c = idx % cols_per_wg;
-    smem[c * M + r] = mk(T(0), T(0));
+    smem[c * ML + lpad<T>(r)] = mk(T(0), T(0));
   }
   __syncthreads();
   const int t = threadIdx.x % NT;
   const int cl = threadIdx.x / NT;
-  cplx<T>* buf = smem + cl * M;
+  cplx<T>* buf = smem + cl * ML;
   lds_fft<T, M, -1>(buf, tws, t);
   const int col = c0 + cl;
   const int cq = col <= M - col ? col : M - col;
@@ -121,13 +123,13 @@ k_pass2(cplx<T>* __restrict__ Tm, const G* __restrict__ gq, int N, int cols_per_
   for (int e = t; e < M; e += NT) {
     const int fr = e <= M - e ? e : M - e;
     const G g = grow[fr];
-    buf[e] = cmul(buf[e], mk(T(g.x), T(g.y)));
+    buf[lpad<T>(e)] = cmul(buf[lpad<T>(e)], mk(T(g.x), T(g.y)));
   }
   __syncthreads();
   lds_fft<T, M, +1>(buf, tws, t);
   for (int idx = threadIdx.x; idx < N * cols_per_wg; idx += nthr) {
     const int r = idx / cols_per_wg, c = idx - r * cols_per_wg;
-    Tm[tpos(r, c0 + c, N)] = smem[c * M + r];
+    Tm[tpos(r, c0 + c, N)] = smem[c * ML + lpad<T>(r)];
   }
 }
 
@@ -152,17 +154,18 @@ k_pass3(const cplx<T>* __restrict__ Tm, const cplx<T>* __restrict__ x, const T* 
   const int t = threadIdx.x % NT;
   const int rl = threadIdx.x / NT;
   const int row = blockIdx.x * rows_per_wg + rl;
-  cplx<T>* buf = smem + rl * M;
+  constexpr int ML = LdsLen<T, M>::value;
+  cplx<T>* buf = smem + rl * ML;
   const bool live = row < N;
   FftTwiddles<T, M> tws;
   tws.load(tw, t);
-  for (int e = t; e < M; e += NT) buf[e] = live ? Tm[tpos(row, e, N)] : mk(T(0), T(0));
+  for (int e = t; e < M; e += NT) buf[lpad<T>(e)] = live ? Tm[tpos(row, e, N)] : mk(T(0), T(0));
   __syncthreads();
   lds_fft<T, M, +1>(buf, tws, t);
   if (!live) return;
   const long rb = (long)row * N;
   for (int e = t; e < N; e += NT) {
-    const cplx<T> g = cscale(buf[e], inv);
+    const cplx<T> g = cscale(buf[lpad<T>(e)], inv);
     cplx<T> o;
     switch (mode) {
       case kModeForward: o = csub(x[rb + e], g); break;
@@ -337,7 +340,7 @@ int launch_pass1(const Plan& P, hipStream_t st, const cplx<T>* x, cplx<T>* xw, c
   const int rows = P.rows1;
   const int grid = (N + rows - 1) / rows;
   const int threads = rows * fft_threads(M);
-  const size_t lds = (size_t)rows * M * sizeof(cplx<T>);
+  const size_t lds = (size_t)rows * lds_len(M, sizeof(T) == 8) * sizeof(cplx<T>);
   if (threads > pass_max_threads(M, sizeof(T) == 8) || lds > 65536) return CGM_ERROR_UNSUPPORTED;
   switch (M) {
 #define X(m)                                                                                 \
@@ -359,7 +362,7 @@ int launch_pass2(const Plan& P, hipStream_t st, cplx<T>* Tm, const G* gq, const 
   const int cols = P.cols2;
   const int grid = M / cols;
   const int threads = cols * fft_threads(M);
-  const size_t lds = (size_t)cols * M * sizeof(cplx<T>);
+  const size_t lds = (size_t)cols * lds_len(M, sizeof(T) == 8) * sizeof(cplx<T>);
   if (threads > pass_max_threads(M, sizeof(T) == 8) || lds > 65536 || M % cols) return CGM_ERROR_UNSUPPORTED;
   switch (M) {
 #define X(m)                                                                                  \
@@ -382,7 +385,7 @@ int launch_pass3(const Plan& P, hipStream_t st, const cplx<T>* Tm, const cplx<T>
   const int rows = P.rows3;
   const int grid = (N + rows - 1) / rows;
   const int threads = rows * fft_threads(M);
-  const size_t lds = (size_t)rows * M * sizeof(cplx<T>);
+  const size_t lds = (size_t)rows * lds_len(M, sizeof(T) == 8) * sizeof(cplx<T>);
   if (threads > pass_max_threads(M, sizeof(T) == 8) || lds > 65536) return CGM_ERROR_UNSUPPORTED;
   const T inv = T(1.0 / ((double)M * (double)M));
   switch (M) {

@@ -118,6 +118,14 @@ class Context:
               "cgm_matvec")
         return y
 
+    def matvec_batched(self, gamma, chi, x, mode=_lib.OP_FORWARD, out=None):
+        """y[b] = A_b x[b] for x (B, N*N); chi (B, N*N) or (N*N,) shared."""
+        y = out if out is not None else self.torch.empty_like(x)
+        chi_stride = 0 if (chi is None or chi.dim() == 1) else self.N * self.N
+        check(lib().cgm_matvec_batched(self._ptr, _ptr(gamma), _ptr(chi), chi_stride, _ptr(x), _ptr(y),
+                                       int(x.shape[0]), int(mode)), "cgm_matvec_batched")
+        return y
+
     def matvec_c128(self, gamma64, chi64, x, mode=_lib.OP_FORWARD, out=None):
         y = out if out is not None else self.torch.empty_like(x)
         check(lib().cgm_matvec_c128(self._ptr, _ptr(gamma64), _ptr(chi64), _ptr(x), _ptr(y), int(mode)),

@@ -129,3 +129,22 @@ def test_batched_graph_replays_match_eager_launches(sol):
         xs, i1 = ctx.gmres(g, chi_d[k].contiguous(), b_d[k].contiguous(), restart=10, tol=1e-7, max_cycles=12)
         assert i1["iterations"] == ig[k]["iterations"]
         assert rel(xg[k].cpu().numpy(), xs.cpu().numpy()) < 1e-6
+
+
+def test_batched_matvec_equals_single(sol):
+    import torch
+    from cugmres_amd import problems
+    for N in (64, 96, 256):
+        ctx = sol.Context(N)
+        g = ctx.spectrum_helmholtz(problems.KH)
+        chi, b = _systems(N, 4, 12.1, 5)
+        chi_d = torch.from_numpy(chi.reshape(4, -1).astype(np.float32)).cuda()
+        x_d = torch.from_numpy(b.reshape(4, -1).astype(np.complex64)).cuda()
+        for mode in (0, 1, 2):
+            yb = ctx.matvec_batched(g, chi_d, x_d, mode)
+            ys = ctx.matvec_batched(g, chi_d[0].contiguous(), x_d, mode)   # one shared map
+            for k in range(4):
+                y1 = ctx.matvec(g, chi_d[k].contiguous(), x_d[k].contiguous(), mode)
+                assert torch.equal(yb[k], y1), (N, mode, k)
+                y0 = ctx.matvec(g, chi_d[0].contiguous(), x_d[k].contiguous(), mode)
+                assert torch.equal(ys[k], y0), (N, mode, k)
