@@ -55,12 +55,16 @@ def parse():
                    help="FGMRES falls back to plain GMRES when a cycle leaves more than this "
                         "fraction of the residual (0: never)")
     p.add_argument("--plain-only", action="store_true")
-    p.add_argument("--mode", choices=["bench", "step", "copy", "batched", "fno-step", "copybw"], default="bench",
+    p.add_argument("--mode", choices=["bench", "step", "copy", "batched", "fno-step", "copybw", "sweep"],
+                   default="bench",
                    help="bench: the benchmark; step: only the per-kernel profile of one Arnoldi "
                         "step (for rocprofv3 --pmc); copy: 3 copies of 256 MiB (counter calibration); "
                         "batched: lockstep batches of 16 right-hand sides vs single solves; fno-step: "
                         "the per-kernel profile of one network forward pass (for rocprofv3 --pmc); copybw: "
-                        "device-copy bandwidth at 8-256 MB")
+                        "device-copy bandwidth at 8-256 MB; sweep: the reference's own benchmark "
+                        "(its cylinder problem at N = 256..2048, Krylov dimension 3/30/49) through "
+                        "cgm_Fast_GMRES_with_CUDA beside its published V100 / GTX 1080 Ti times")
+    p.add_argument("--sweep-reps", type=int, default=20)
     return p.parse_args()
 
 
@@ -86,6 +90,11 @@ def main():
         return
     if a.mode == "batched":
         print(json.dumps(batched_throughput(a.restart)), flush=True)
+        return
+    if a.mode == "sweep":
+        from cugmres_amd import sweep
+        r = sweep.run(reps=a.sweep_reps, log=lambda s: print(s, flush=True))
+        print(json.dumps({"sweep": {f"{N}x{m}": v for (N, m), v in r.items()}}), flush=True)
         return
     if a.mode == "copybw":
         out = {}
@@ -294,9 +303,27 @@ def batched_throughput(restart, cycles=3, B=16):
             ctx.gmres(g, chi, b[k], x[k].zero_(), restart=restart, tol=0.0, max_cycles=cycles)
         torch.cuda.synchronize()
         ts = time.perf_counter() - t
+        # batched matvec: one launch per FFT pass for all B vectors
+        y = torch.empty_like(b)
+        for _ in range(3):
+            ctx.matvec_batched(g, chi, b, 0, y)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(20):
+            ctx.matvec_batched(g, chi, b, 0, y)
+        e1.record()
+        torch.cuda.synchronize()
+        us_mv = e0.elapsed_time(e1) / 20 * 1e3
+        M = ctx.M
+        # per vector: x twice (passes 1 and 3) and y once (8 B each), the N x M
+        # scratch written and read by passes 1-2-3 (8 B x 4); once per batch:
+        # the shared chi (4 B) and the quarter spectrum
+        mv_bytes = B * (24 * N * N + 32 * N * M) + 4 * N * N + 8 * (M // 2 + 1) ** 2
         out["sizes"][N] = {"batched_ms": tb * 1e3, "single_ms": ts * 1e3, "speedup": ts / tb,
                            "system_steps": steps, "us_per_system_step_batched": tb / steps * 1e6,
-                           "us_per_system_step_single": ts / steps * 1e6}
+                           "us_per_system_step_single": ts / steps * 1e6,
+                           "batched_matvec_us": us_mv, "batched_matvec_GBps": mv_bytes / (us_mv * 1e-6) / 1e9,
+                           "batched_matvec_hbm_frac": mv_bytes / (us_mv * 1e-6) / 1e9 / HBM_PEAK_GBS}
         ctx.close()
     return out
 

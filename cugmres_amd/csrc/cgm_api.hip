@@ -445,6 +445,7 @@ static int plain_cycle(cgm_context* c, const float2* gq, const float* chi, int f
       if ((rc = launch_fused(c->stream, 0, L))) return rc;
       if (j == m) break;
       if ((rc = launch_fused(c->stream, 1, L))) return rc;
+      if (j + 1 == m) continue;  // u_m needs no matvec
       if ((rc = launch_pass2<float, float2>(c->plan, c->stream, c->T32, gq, c->tw32, &c->st->done))) return rc;
     }
   } else {
@@ -516,6 +517,47 @@ static int gmres_core(cgm_context* c, const float2* gq, const float* chi, const 
   res->iterations = total;
   res->cycles = cycles;
   return res->converged ? CGM_SUCCESS : CGM_ERROR_NOT_CONVERGED;
+}
+
+// Exactly one cycle of m steps from x, entered only when ||b - A x|| >
+// tolerance (absolute) and never left early: the control of the reference's
+// unrestarted solver (B/CUDA_GMRES/GMRES.cuh:153,253).  m + 1 matvecs (the
+// initial residual and one per step); hist[0] = ||r0||, hist[t] = estimate
+// after t steps.  res->iterations = steps done (0 when not entered).
+int cgm::gmres_fixed_cycle(cgm_context* c, const float2* gq, const float* chi, const float2* b, float2* x,
+                           int m, int adjoint, double tolerance, double* hist, cgm_gmres_result* res) {
+  const long n = (long)c->N * c->N;
+  int rc = krylov_alloc(c->ws, n, m, c->stream);
+  if (rc) return rc;
+  KrylovWs& ws = c->ws;
+  memset(res, 0, sizeof(*res));
+  CGM_HIP(hipMemsetAsync(&c->st->status, 0, sizeof(int), c->stream));
+  // ||b||, and a zero absolute tolerance inside the cycle
+  if ((rc = launch_norm_init(c->stream, ws, b, c->st, 1, 0.0))) return rc;
+  rc = matvec32(c, gq, chi, x, b, ws.V, adjoint ? kModeResidualAdj : kModeResidual, nullptr, nullptr, nullptr);
+  if (rc) return rc;
+  if ((rc = launch_norm_init(c->stream, ws, ws.V, c->st, 0, 0.0))) return rc;
+  if ((rc = read_state(c))) return rc;
+  res->beta0 = c->hst->beta;
+  res->bnorm = c->hst->bnorm;
+  res->rel_residual = res->bnorm > 0 ? res->beta0 / res->bnorm : 0.0;
+  if (hist) CGM_HIP(hipMemcpyAsync(hist, &c->hst->beta, sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (c->hst->status) {
+    cgm_set_last_error("non-finite initial residual");
+    return CGM_ERROR_NONFINITE;
+  }
+  if (!(res->beta0 > tolerance)) return CGM_SUCCESS;
+  if ((rc = plain_cycle(c, gq, chi, adjoint ? kModeAdjoint : kModeForward, m, hist, 0))) return rc;
+  if ((rc = launch_backsub_update(c->stream, ws, c->st, x))) return rc;
+  if ((rc = read_state(c))) return rc;
+  res->iterations = c->hst->k;
+  res->cycles = 1;
+  res->rel_residual = res->bnorm > 0 ? c->hst->res_est / res->bnorm : 0.0;
+  if (c->hst->status) {
+    cgm_set_last_error("non-finite value in the Arnoldi process");
+    return CGM_ERROR_NONFINITE;
+  }
+  return CGM_SUCCESS;
 }
 
 static int check_params(const cgm_gmres_params* p) {
@@ -654,6 +696,7 @@ extern "C" int cgm_gmres_batched(cgm_ctx_t c, const cgm_complex64* gamma, const 
       if ((r = launch_fused(s, 0, L))) return r;
       if (j == m) break;
       if ((r = launch_fused(s, 1, L))) return r;
+      if (j + 1 == m) continue;  // u_m needs no matvec
       if ((r = launch_pass2<float, float2>(c->plan, s, c->T32B, gq, c->tw32, &stB->done, B, us))) return r;
     }
     if ((r = launch_backsub_update(s, ws, stB, xx, nullptr, n, sst))) return r;

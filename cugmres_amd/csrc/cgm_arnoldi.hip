@@ -1,5 +1,5 @@
 // Fused Arnoldi steps with lagged CGS2 (delayed reorthogonalisation) for
-// power-of-two grids 64 <= N <= 1024 and restart <= 64.
+// power-of-two grids 64 <= N <= 2048 and restart <= 64.
 //
 // CGS2 (two classical Gram-Schmidt passes per step) replaces the reference's
 // modified Gram-Schmidt (B/CUDA_GMRES/GMRES.cuh:289-303).  Here the second
@@ -27,6 +27,9 @@
 //   B(j): q_j and u_{j+1} in one pass over Q_j + pass 1 of A u_{j+1}
 //         (row FFTs of chi .* u_{j+1}).
 //   pass 2 (column FFTs, spectrum multiply), shared with the plain matvec.
+// The cycle's last step needs only s and alpha of u_m (column m-1 of H), so
+// B(m-1) stops after u_m, pass 2 is not launched and A(m) skips pass 3: a
+// cycle of m steps costs m matvecs.
 // A block owns R whole rows (R*N = 1024*EPD elements) so the row FFTs and the
 // element-wise Krylov work run in the same kernel.
 #include <math.h>
@@ -41,6 +44,7 @@ constexpr int kCH = 8;  // basis vectors per accumulation chunk
 
 struct FusedArgs {
   int N, R, m, ld, j, mode;  // m: steps of the cycle; ld: leading dimension of H, Rm
+  int tail;              // last launch of its kind in the cycle: no matvec work (A(m), B(m-1))
   int G;                 // blocks
   long n;
   float inv_mm;          // 1 / M^2
@@ -126,12 +130,15 @@ __global__ void __launch_bounds__(kFBS) k_arn_a(FusedArgs a) {
   if (fa) tws.load(a.tw, t);
   constexpr int ML = LdsLen<float, M>::value;
   float2* buf = smem + rl * ML;
-  if (fa) {
-    const int row = r0 + rl;
-    for (int e = t; e < M; e += NT) buf[lpad<float>(e)] = row < N ? a.T[tpos(row, e, N)] : make_float2(0.f, 0.f);
+  const bool tail = a.tail != 0;
+  if (!tail) {
+    if (fa) {
+      const int row = r0 + rl;
+      for (int e = t; e < M; e += NT) buf[lpad<float>(e)] = row < N ? a.T[tpos(row, e, N)] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    lds_fft<float, M, +1>(buf, tws, t, fa);
   }
-  __syncthreads();
-  lds_fft<float, M, +1>(buf, tws, t, fa);
   // ---- z = u - G(.), u and z of the block's R*N elements staged in LDS -------
   // (the FFT rows are no longer needed once every thread has read its values)
   float2 uu[EPD], zz[EPD];
@@ -145,18 +152,20 @@ __global__ void __launch_bounds__(kFBS) k_arn_a(FusedArgs a) {
     if (row < N) {
       const long id = (long)row * N + c;
       const float2 u = a.Uj[id];
-      float2 tv = smem[r * ML + lpad<float>(c)];
-      tv.x *= a.inv_mm;
-      tv.y *= a.inv_mm;
-      if (a.mode == kModeAdjoint) {
-        const float ch = a.chi[id];
-        tv.x *= ch;
-        tv.y *= ch;
-      }
-      const float2 z = make_float2(u.x - tv.x, u.y - tv.y);
-      a.Z[id] = z;
       uu[k] = u;
-      zz[k] = z;
+      if (!tail) {
+        float2 tv = smem[r * ML + lpad<float>(c)];
+        tv.x *= a.inv_mm;
+        tv.y *= a.inv_mm;
+        if (a.mode == kModeAdjoint) {
+          const float ch = a.chi[id];
+          tv.x *= ch;
+          tv.y *= ch;
+        }
+        const float2 z = make_float2(u.x - tv.x, u.y - tv.y);
+        a.Z[id] = z;
+        zz[k] = z;
+      }
     }
   }
   __syncthreads();
@@ -432,6 +441,7 @@ __global__ void __launch_bounds__(kFBS) k_arn_b(FusedArgs a) {
     }
     smem[r * LdsLen<float, M>::value + lpad<float>(c)] = w;
   }
+  if (a.tail) return;  // u_m needs no matvec (block-uniform)
   for (int e = tid; e < R * (M - N); e += kFBS) {
     const int r = e / (M - N), c = N + e % (M - N);
     smem[r * LdsLen<float, M>::value + lpad<float>(c)] = make_float2(0.f, 0.f);
@@ -449,7 +459,7 @@ __global__ void __launch_bounds__(kFBS) k_arn_b(FusedArgs a) {
 
 // ---------------------------------------------------------------------------
 bool fused_supported(int N, int restart) {
-  const bool pow2 = N >= 64 && N <= 1024 && (N & (N - 1)) == 0;
+  const bool pow2 = N >= 64 && N <= 2048 && (N & (N - 1)) == 0;
   return pow2 && restart >= 1 && restart <= 64;
 }
 
@@ -471,6 +481,7 @@ int launch_fused(hipStream_t st, int which, const FusedLaunch& L) {
   a.ld = L.ld;
   a.j = L.j;
   a.mode = L.mode;
+  a.tail = which == 0 ? (L.j == L.m) : (L.j == L.m - 1);
   a.G = fused_blocks(L.N);
   a.n = (long)L.N * L.N;
   a.inv_mm = (float)(1.0 / ((double)M * (double)M));
@@ -516,6 +527,7 @@ int launch_fused(hipStream_t st, int which, const FusedLaunch& L) {
     CGM_FUSED_CASE(256, 1)
     CGM_FUSED_CASE(512, 1)
     CGM_FUSED_CASE(1024, 2)
+    CGM_FUSED_CASE(2048, 2)
 #undef CGM_FUSED_CASE
     default:
       return CGM_ERROR_UNSUPPORTED;

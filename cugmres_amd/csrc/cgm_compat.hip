@@ -137,33 +137,18 @@ extern "C" int cgm_Fast_GMRES_with_CUDA(const cgm_complex64* gamma, const bool* 
   stamp(times, 1, nts);
   // The reference iterates only when ||r0|| > tolerance (GMRES.cuh:153) and
   // then runs exactly maxiter Arnoldi steps (its early exit is commented out,
-  // GMRES.cuh:253): one cycle, tolerance 0 inside the cycle.
+  // GMRES.cuh:253): one cycle with a zero tolerance inside it.
   cgm_gmres_result r;
-  memset(&r, 0, sizeof(r));
-  float2* x = (float2*)sol;
-  rc = krylov_alloc(c->ws, n, (int)maxiter, c->stream);
-  if (rc) return rc;
-  // initial residual check
-  {
-    cgm_gmres_params p0{(int)maxiter, 0, 0.0, for_gradient ? 1 : 0, 0};
-    rc = cgm_gmres(c, gamma, c->chi_tmp, (const cgm_complex64*)c->vec_tmp, sol, &p0, &r, nullptr);
-    if (rc != CGM_SUCCESS && rc != CGM_ERROR_NOT_CONVERGED) { (void)hipFree(resid); (void)hipFree(hist); return rc; }
+  rc = gmres_fixed_cycle(c, (const float2*)gamma, c->chi_tmp, c->vec_tmp, (float2*)sol, (int)maxiter,
+                         for_gradient ? 1 : 0, (double)tolerance, hist, &r);
+  if (rc) {
+    (void)hipFree(resid);
+    (void)hipFree(hist);
+    return rc;
   }
-  bool iterate = r.beta0 > tolerance;
-  double final_est = 0.0;
-  unsigned steps = 0;
-  if (iterate) {
-    cgm_gmres_params p{(int)maxiter, 1, 0.0, for_gradient ? 1 : 0, 0};
-    rc = cgm_gmres(c, gamma, c->chi_tmp, (const cgm_complex64*)c->vec_tmp, sol, &p, &r, hist);
-    if (rc != CGM_SUCCESS && rc != CGM_ERROR_NOT_CONVERGED) { (void)hipFree(resid); (void)hipFree(hist); return rc; }
-    steps = (unsigned)r.iterations;
-    double h[2] = {0, 0};
-    CGM_HIP(hipMemcpy(h, hist + steps, sizeof(double), hipMemcpyDeviceToHost));
-    final_est = h[0];
-  } else {
-    CGM_HIP(hipMemcpy(hist, &r.beta0, sizeof(double), hipMemcpyHostToDevice));
-  }
-  (void)x;
+  const unsigned steps = (unsigned)r.iterations;
+  double final_est = r.beta0;
+  if (steps > 0) CGM_HIP(hipMemcpy(&final_est, hist + steps, sizeof(double), hipMemcpyDeviceToHost));
   hipLaunchKernelGGL(k_hist_to_float, dim3((maxiter + 1 + 255) / 256), dim3(256), 0, c->stream,
                      hist, resid, (int)maxiter + 1);
   CGM_HIP(hipStreamSynchronize(c->stream));

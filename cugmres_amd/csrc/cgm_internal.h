@@ -139,6 +139,8 @@ struct FusedLaunch {
   DevState* st;
   double* hist;
 };
+int gmres_fixed_cycle(cgm_context* c, const float2* gq, const float* chi, const float2* b, float2* x, int m,
+                      int adjoint, double tolerance, double* hist, cgm_gmres_result* res);
 bool fused_supported(int N, int restart);
 int fused_blocks(int N);
 int launch_fused(hipStream_t st, int which, const FusedLaunch& L);

@@ -80,11 +80,12 @@ def disk_field(pi_, pj, k, eps, ci, cj, a, theta_inc=math.pi, phase_origin=None)
     return u
 
 
-def reference_problem_field(N, stride=1):
+def reference_problem_field(N, stride=1, k=None):
     """The reference's benchmark problem (eps = 2.25 disk of radius N/6 at
-    (N/3, N/2), k = 2*pi*6/N per cell, incident exp(-i k i)) on the N x N cell
-    centres, every `stride`-th row and column.  Returns (field, rows, cols)."""
-    k = 2.0 * math.pi / (N / 6.0)
+    (N/3, N/2), k = 2*pi*6/N per cell unless given, incident exp(-i k i)) on
+    the N x N cell centres, every `stride`-th row and column.  Returns
+    (field, rows, cols)."""
+    k = 2.0 * math.pi / (N / 6.0) if k is None else float(k)
     rows = np.arange(0, N, stride)
     cols = np.arange(0, N, stride)
     I, J = np.meshgrid(rows, cols, indexing="ij")

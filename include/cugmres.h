@@ -129,6 +129,12 @@ int cgm_spectrum_helmholtz(cgm_ctx_t ctx, double kh, cgm_complex64* dev_gamma,
  * contrast per cell (N*N float).  Asynchronous on the context stream. */
 int cgm_matvec(cgm_ctx_t ctx, const cgm_complex64* dev_gamma, const float* dev_chi,
                const cgm_complex64* dev_x, cgm_complex64* dev_y, int mode);
+/* The same for `batch` vectors of the context's N in one launch per FFT
+ * pass: x and y are [batch][N*N], chi is [batch][N*N] (chi_stride = N*N) or
+ * one map (chi_stride = 0).  No counterpart in the reference, whose matvec
+ * takes one vector (B/CUDA_GMRES/Fast_matvec.cuh:1-28). */
+int cgm_matvec_batched(cgm_ctx_t ctx, const cgm_complex64* dev_gamma, const float* dev_chi, long chi_stride,
+                       const cgm_complex64* dev_x, cgm_complex64* dev_y, int batch, int mode);
 /* FP64 variant (spectrum from dev_gamma64, chi as double). */
 int cgm_matvec_c128(cgm_ctx_t ctx, const cgm_complex128* dev_gamma64, const double* dev_chi,
                     const cgm_complex128* dev_x, cgm_complex128* dev_y, int mode);
