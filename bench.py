@@ -218,13 +218,15 @@ def main():
     roof = prof["roofline"]
     roof["traffic"] = measured_traffic(roof["kernel"], N)
     if fprof is not None:
-        # per FGMRES step: one forward pass (4 row projections, 4 cell GEMMs, ...),
-        # one matvec, one two-pass CGS2 step; the heaviest kernel class by time
+        # per FGMRES step: one forward pass (lift, 4 x x-DFT / mixing / inverse, 3 row cell GEMMs,
+        # the projecting cell GEMM), passes 1 and 2 of the matvec of z_j, and the fused flexible
+        # k_arn_a (pass 3 + the single reduction) and k_arn_b (basis update + norm); the heaviest
+        # kernel class by time (the fused pair is known only as the remainder of the step time)
         per_step = {k: v["us"] * v["per_step"] for k, v in fprof["kernels"].items() if "per_step" in v}
-        for k in ("pass1_row_fft", "pass2_col_fft_mul", "pass3_row_ifft", "cgs2_dots",
-                  "cgs2_update_dots", "cgs2_update_norm_givens"):
+        for k in ("pass1_row_fft", "pass2_col_fft_mul"):
             per_step[k] = prof["kernels"][k]["us"]
         dom = max(per_step, key=per_step.get)
+        per_step["fused_arnoldi_and_gaps"] = out_learned["us_per_arnoldi_step"] - sum(per_step.values())
         src = fprof["kernels"] if dom in fprof["kernels"] else prof["kernels"]
         ach = src[dom]["GBps"]
         roof = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -359,18 +361,22 @@ def profile_fno(ctx, net, dchi, db):
     _lib.check(st, "cgm_fno_profile")
     names = ["fno_lift", "fno_rowproj", "fno_dftx", "fno_mix", "fno_invx", "fno_cell_gemm", "fno_proj",
              "fno_forward_total"]
-    # algorithmic bytes per launch: BF16 activations 64 B per cell, complex64
-    # vectors 8 B, chi 4 B; FP32 row projections 4 KiB per grid row, x-DFT
-    # partials 128 KiB per 32 rows, spectral weights 4 MiB per layer, mixed
-    # modes 128 KiB, row twiddles 256 B per row, BF16 per-row mixing matrices
-    # 2 KiB per row
+    # launches per forward pass: fno_lift = k_lift_row (lift + layer-0 row
+    # projection), fno_cell_gemm = k_cell_row (cell GEMM + GELU + next row
+    # projection, layers 0-2), fno_proj = k_cell_proj (layer-3 cell GEMM + the
+    # 32 -> 2 projection); the stand-alone row projection runs only in the
+    # backward pass.  Algorithmic bytes per launch: BF16 activations 64 B per
+    # cell, complex64 vectors 8 B, chi 4 B; FP32 row projections 4 KiB per
+    # grid row, x-DFT partials 128 KiB per 32 rows, spectral weights 4 MiB per
+    # layer, mixed modes 128 KiB, row twiddles 256 B per row, BF16 per-row
+    # mixing matrices 2 KiB per row
     part = (N // 32) * 16384 * 8
-    bytes_ = {"fno_lift": n * (4 + 8 + 64), "fno_rowproj": n * 64 + N * 4096,
+    bytes_ = {"fno_lift": n * (4 + 8 + 64) + N * 4096, "fno_rowproj": n * 64 + N * 4096,
               "fno_dftx": N * 4096 + N * 256 + part, "fno_mix": part + 4 * 1024 * 1024 + 16384 * 8,
-              "fno_invx": 16384 * 8 + N * 256 + N * 2048, "fno_cell_gemm": n * 128 + N * 2048,
-              "fno_proj": n * (64 + 16)}
-    per_step = {"fno_lift": 1, "fno_rowproj": 4, "fno_dftx": 4, "fno_mix": 4, "fno_invx": 4,
-                "fno_cell_gemm": 4, "fno_proj": 1}
+              "fno_invx": 16384 * 8 + N * 256 + N * 2048, "fno_cell_gemm": n * 128 + N * 2048 + N * 4096,
+              "fno_proj": n * (64 + 16) + N * 2048}
+    per_step = {"fno_lift": 1, "fno_rowproj": 0, "fno_dftx": 4, "fno_mix": 4, "fno_invx": 4,
+                "fno_cell_gemm": 3, "fno_proj": 1}
     kernels = {}
     for i, k in enumerate(names):
         us = ms[i] * 1e3

@@ -820,6 +820,18 @@ __global__ void k_norm_copy(float2* __restrict__ v, const double* __restrict__ n
   }
 }
 
+// Identity preconditioner of the flexible fused steps: z = v / *nrm (v kept).
+__global__ void k_scale_copy(const float2* __restrict__ v, const double* __restrict__ nrm, float2* __restrict__ z,
+                             long n, const int* done) {
+  if (*done) return;
+  const double d = *nrm;
+  const float r = d > 0.0 ? (float)(1.0 / d) : 0.f;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    const float2 a = v[e];
+    z[e] = make_float2(a.x * r, a.y * r);
+  }
+}
+
 // One preconditioned cycle: z_i = P(v_i), w = A z_i, CGS2 step; the caller
 // applies x += Z y.
 static int precond_cycle(cgm_context* c, const float2* gq, const float* chi, cgm_fno* net, int m, double* hist,
@@ -827,6 +839,45 @@ static int precond_cycle(cgm_context* c, const float2* gq, const float* chi, cgm
   KrylovWs& ws = c->ws;
   const long n = ws.n;
   int rc;
+  if (c->path != 1 && fused_supported(c->N, m)) {
+    // flexible fused lagged-CGS2 steps: z_j = P(u_j / ||u_j||), w = A z_j,
+    // one reduction and one basis update per step; u_0 = r (in V_0, norm in
+    // st->beta), later norms from B(j-1) in st->unorm
+    CGM_HIP(hipMemcpyAsync(ws.U, ws.V, sizeof(float2) * n, hipMemcpyDeviceToDevice, c->stream));
+    FusedLaunch L{};
+    L.N = c->N; L.m = m; L.ld = ws.m; L.mode = kModeForward; L.chi = chi; L.V = ws.V; L.Z = ws.Z;
+    L.T = c->T32; L.tw = c->tw32; L.part = ws.part; L.H = ws.H; L.Rm = ws.R; L.cs = ws.cs;
+    L.sn = ws.sn; L.g = ws.g; L.h1p = ws.h1p; L.a1 = ws.a1; L.a2 = ws.a2; L.scal = ws.scal;
+    L.st = c->st; L.hist = hist ? hist + total : nullptr;
+    L.flex = 1;
+    L.partn = ws.partn;
+    for (int j = 0; j <= m; ++j) {
+      L.j = j;
+      L.Uj = ws.U + (long)(j & 1) * n;
+      L.Unext = ws.U + (long)((j + 1) & 1) * n;
+      L.Xin = nullptr;
+      if (j < m) {
+        float2* zj = c->zb + (long)j * n;
+        const double* nrm = j == 0 ? &c->st->beta : &c->st->unorm;
+        if (net) {
+          rc = fno_forward(net, c->stream, chi, L.Uj, nrm, nullptr, zj, 1, false, &c->st->done);
+        } else {
+          hipLaunchKernelGGL(k_scale_copy, dim3(ws.G), dim3(kKB), 0, c->stream, L.Uj, nrm, zj, n, &c->st->done);
+          rc = hipGetLastError() == hipSuccess ? CGM_SUCCESS : CGM_ERROR_DEVICE;
+        }
+        if (!rc)
+          rc = launch_pass1<float>(c->plan, c->stream, zj, nullptr, chi, nullptr, kModeForward, c->T32, c->tw32,
+                                   &c->st->done);
+        if (!rc) rc = launch_pass2<float, float2>(c->plan, c->stream, c->T32, gq, c->tw32, &c->st->done);
+        if (rc) return rc;
+        L.Xin = zj;
+      }
+      if ((rc = launch_fused(c->stream, 0, L))) return rc;
+      if (j == m) break;
+      if ((rc = launch_fused(c->stream, 1, L))) return rc;
+    }
+    return CGM_SUCCESS;
+  }
   for (int i = 0; i < m; ++i) {
     float2* vi = ws.V + (long)i * n;
     float2* zi = c->zb + (long)i * n;

@@ -27,6 +27,16 @@
 //   B(j): q_j and u_{j+1} in one pass over Q_j + pass 1 of A u_{j+1}
 //         (row FFTs of chi .* u_{j+1}).
 //   pass 2 (column FFTs, spectrum multiply), shared with the plain matvec.
+// Flexible steps (FGMRES with a right preconditioner P, possibly nonlinear):
+// the network is applied to the lagged vector, z_j = P(u_j / ||u_j||), and
+// w = A z_j takes the place of z = A u_j.  Any z_j keeps the flexible Arnoldi
+// relation A Z_m = Q_{m+1} H, and the same single reduction gives the first
+// pass of w against Q_{j+1} = [Q_j, q_j] directly:
+//   h[0:j] = t,  h[j] = q_j^H w = (gamma - s^H t)/nu,
+//   u_{j+1} = w - (h[j]/nu) u_j - Q_j (t - s h[j]/nu)
+// (the plain form with z/nu -> w, c -> t, d -> h[j]).  B(j) then also reduces
+// ||u_{j+1}||, the input scale of the next network application, and does no
+// row FFTs: the next matvec input is z_{j+1}, known only after the network.
 // The cycle's last step needs only s and alpha of u_m (column m-1 of H), so
 // B(m-1) stops after u_m, pass 2 is not launched and A(m) skips pass 3: a
 // cycle of m steps costs m matvecs.
@@ -45,12 +55,15 @@ constexpr int kCH = 8;  // basis vectors per accumulation chunk
 struct FusedArgs {
   int N, R, m, ld, j, mode;  // m: steps of the cycle; ld: leading dimension of H, Rm
   int tail;              // last launch of its kind in the cycle: no matvec work (A(m), B(m-1))
+  int flex;              // flexible steps (see below)
   int G;                 // blocks
   long n;
   float inv_mm;          // 1 / M^2
   const float* chi;
   float2* V;             // q_0 .. q_m
   const float2* Uj;      // u_j
+  const float2* Xin;     // matvec input: u_j, or z_j = P(u_j / ||u_j||) with flex
+  double* partn;         // ||u_{j+1}||^2 partials (flex)
   float2* Unext;         // u_{j+1}
   float2* Z;             // z = A u_j
   float2* T;             // N x M FFT scratch
@@ -69,7 +82,7 @@ struct FusedArgs {
   double* hist;          // estimates, hist[j] after j steps of this cycle (may be null)
   // per-system strides of a batched launch (system = blockIdx.y); elements,
   // DevState in bytes
-  long sChi, sBasis, sU, sVec, sT, sPart, sSmall, sSt, sHist;
+  long sChi, sBasis, sU, sVec, sT, sPart, sSmall, sSt, sHist, sPartn;
 };
 
 __device__ inline void batch_offset(FusedArgs& a) {
@@ -78,7 +91,9 @@ __device__ inline void batch_offset(FusedArgs& a) {
   a.chi += k * a.sChi;
   a.V += k * a.sBasis;
   a.Uj += k * a.sU;
+  if (a.Xin) a.Xin += k * (a.flex ? a.sVec : a.sU);
   a.Unext += k * a.sU;
+  if (a.partn) a.partn += k * a.sPartn;
   a.Z += k * a.sVec;
   a.T += k * a.sT;
   a.part += k * a.sPart;
@@ -154,6 +169,7 @@ __global__ void __launch_bounds__(kFBS) k_arn_a(FusedArgs a) {
       const float2 u = a.Uj[id];
       uu[k] = u;
       if (!tail) {
+        const float2 xin = a.flex ? a.Xin[id] : u;
         float2 tv = smem[r * ML + lpad<float>(c)];
         tv.x *= a.inv_mm;
         tv.y *= a.inv_mm;
@@ -162,7 +178,7 @@ __global__ void __launch_bounds__(kFBS) k_arn_a(FusedArgs a) {
           tv.x *= ch;
           tv.y *= ch;
         }
-        const float2 z = make_float2(u.x - tv.x, u.y - tv.y);
+        const float2 z = make_float2(xin.x - tv.x, xin.y - tv.y);
         a.Z[id] = z;
         zz[k] = z;
       }
@@ -337,6 +353,29 @@ __global__ void __launch_bounds__(kFBS) k_arn_a(FusedArgs a) {
     }
     return;
   }
+  if (a.flex) {
+    // first pass of w against Q_{j+1} and the update coefficients
+    __shared__ double2 s_hj;
+    if (tid == 0) {
+      double2 st_ = make_double2(0.0, 0.0);
+      for (int i = 0; i < j; ++i) st_ = dadd(st_, dcjmul(ls[i], lt[i]));
+      const double2 hj = dscale(dsub(lag[1], st_), 1.0 / nu);
+      a.h1p[j] = hj;
+      s_hj = hj;
+      a.scal[0] = make_double2(1.0 / nu, 0.0);
+      a.scal[1] = dscale(hj, 1.0 / nu);
+      a.scal[2] = make_double2(1.0, 0.0);
+    }
+    __syncthreads();
+    const double2 dnu = dscale(s_hj, 1.0 / nu);
+    for (int i = tid; i < j; i += kFBS) {
+      a.h1p[i] = lt[i];
+      a.a1[i] = ls[i];
+      a.a2[i] = dsub(lt[i], dmul(ls[i], dnu));
+    }
+    if (tid == 0) reset_ticket(&st->tickets[kTicketDots1]);
+    return;
+  }
   // p = H[0:j+1, 0:j] s: one wave per row, lane c multiplies column c
   {
     const int wave = tid >> 6, lane = tid & 63;
@@ -364,6 +403,7 @@ __global__ void __launch_bounds__(kFBS) k_arn_a(FusedArgs a) {
     s_dd = dd;
     a.scal[0] = make_double2(1.0 / nu, 0.0);
     a.scal[1] = dscale(dd, 1.0 / nu);
+    a.scal[2] = make_double2(1.0 / nu, 0.0);
   }
   __syncthreads();
   const double2 ddnu = dscale(s_dd, 1.0 / nu);
@@ -394,7 +434,9 @@ __global__ void __launch_bounds__(kFBS) k_arn_b(FusedArgs a) {
   if (fa) tws.load(a.tw, t);
   const double2 sc0 = a.scal[0], sc1 = a.scal[1];
   const float invnu = (float)sc0.x;
+  const float wsc = (float)a.scal[2].x;  // coefficient of z (plain: 1/nu) or w (flex: 1)
   const float2 ddnu = make_float2((float)sc1.x, (float)sc1.y);
+  float uu2 = 0.f;
 #pragma unroll
   for (int k = 0; k < EPD; ++k) {
     const int e = tid + k * kFBS;
@@ -430,8 +472,9 @@ __global__ void __launch_bounds__(kFBS) k_arn_b(FusedArgs a) {
       const float2 q = make_float2((u.x - qa.x) * invnu, (u.y - qa.y) * invnu);
       a.V[(long)j * n + id] = q;
       const float2 du = cmulf(ddnu, u);
-      const float2 un = make_float2(z.x * invnu - du.x - qb.x, z.y * invnu - du.y - qb.y);
+      const float2 un = make_float2(z.x * wsc - du.x - qb.x, z.y * wsc - du.y - qb.y);
       a.Unext[id] = un;
+      uu2 += un.x * un.x + un.y * un.y;
       w = un;
       if (a.mode == kModeForward) {
         const float ch = a.chi[id];
@@ -440,6 +483,26 @@ __global__ void __launch_bounds__(kFBS) k_arn_b(FusedArgs a) {
       }
     }
     smem[r * LdsLen<float, M>::value + lpad<float>(c)] = w;
+  }
+  if (a.flex) {
+    // ||u_{j+1}||: block partial, the last block sums them in a fixed order
+    if (a.tail) return;  // no network application follows u_m
+    __shared__ double red[kFBS / 64];
+    const double ws_ = wave_sum((double)uu2);
+    if ((tid & 63) == 0) red[tid >> 6] = ws_;
+    __syncthreads();
+    if (tid == 0) {
+      double sb = 0.0;
+      for (int k = 0; k < kFBS / 64; ++k) sb += red[k];
+      put(&a.partn[blockIdx.x], sb);
+    }
+    if (!arrive_last(&a.st->tickets[kTicketNorm])) return;
+    const double sq = sum_partials(a.partn, a.G);
+    if (tid == 0) {
+      a.st->unorm = sqrt(sq);
+      reset_ticket(&a.st->tickets[kTicketNorm]);
+    }
+    return;
   }
   if (a.tail) return;  // u_m needs no matvec (block-uniform)
   for (int e = tid; e < R * (M - N); e += kFBS) {
@@ -481,6 +544,10 @@ int launch_fused(hipStream_t st, int which, const FusedLaunch& L) {
   a.ld = L.ld;
   a.j = L.j;
   a.mode = L.mode;
+  a.flex = L.flex;
+  a.Xin = L.Xin;
+  a.partn = L.partn;
+  a.sPartn = L.sPartn;
   a.tail = which == 0 ? (L.j == L.m) : (L.j == L.m - 1);
   a.G = fused_blocks(L.N);
   a.n = (long)L.N * L.N;

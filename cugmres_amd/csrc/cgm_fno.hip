@@ -25,12 +25,16 @@
 // Only 32 column frequencies (cos and -sin of 16 ky) survive, so the column
 // transforms are GEMMs against an N x 32 basis F[y,2ky] = cos(2 pi ky y/N),
 // F[y,2ky+1] = -sin(2 pi ky y/N), and every layer is
-//   k_rowproj : A_x = F^T H_x              per grid row x (K = N, MFMA)
-//   k_dftx    : DFT of A along x at the 32 kx (FP32)
-//   k_mix     : channel mixing per mode (FP32)
-//   k_invx    : inverse DFT along x (FP32) -> B_x
-//   k_pixel   : h' = act([h, F_y] [W; B_x] + b)  per cell (K = 64, MFMA)
-// The backward pass has the same four kernels with the roles transposed.
+//   row projection : A_x = F^T H_x          per grid row x (K = N, MFMA)
+//   k_dftx         : DFT of A along x at the 32 kx (FP32)
+//   k_mix          : channel mixing per mode (FP32)
+//   k_invx         : inverse DFT along x (FP32) -> B_x
+//   cell GEMM      : h' = act([h, F_y] [W; B_x] + b)  per cell (K = 64, MFMA)
+// In the forward pass the row projection of each layer's input runs inside
+// the kernel that produces it (k_lift_row, k_cell_row) and the last cell GEMM
+// carries the 32 -> 2 projection (k_cell_proj): 1 + 4 x 4 launches.  The
+// backward pass has the separate kernels (k_rowproj, k_pixel) with the roles
+// transposed.
 //
 // Activations are BF16, channel-last ([sample][x][y][32], 64 bytes per cell);
 // MFMA accumulation, the spectral path and all weight gradients are FP32.
@@ -93,38 +97,6 @@ __device__ inline float dgelu(float x) {
 }
 
 __device__ inline bool skip(const int* done) { return done && *done; }
-
-// ---------------------------------------------------------------------------
-// lift: h0[p][c] = a0 W_in[0][c] + a1 W_in[1][c] + a2 W_in[2][c] + b_in[c].
-// With nrm the input is v / *nrm (written back to vw when given), which lets
-// FGMRES hand over its lazily normalised Krylov vector.
-__global__ void __launch_bounds__(256) k_lift(const float* __restrict__ chi, float chi_scale,
-                                              const float2* v, const double* nrm, float2* vw, float s,
-                                              const float* __restrict__ P, bf16* __restrict__ h0,
-                                              long total, const int* done) {
-  if (skip(done)) return;
-  // thread = (cell, quarter): 8 channels of one cell, one 16-byte store
-  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long p = e >> 2;
-  const int c8 = (int)(e & 3) * 8;
-  if (p >= total) return;
-  float2 vv = v[p];
-  if (nrm) {
-    const double d = *nrm;
-    const float r = d > 0.0 ? (float)(1.0 / d) : 0.f;
-    vv.x *= r;
-    vv.y *= r;
-    if (vw && c8 == 0) vw[p] = vv;
-  }
-  const float a0 = chi[p] * chi_scale, a1 = vv.x * s, a2 = vv.y * s;
-  struct __align__(16) Q8 { bf16 v[8]; } q;
-#pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    const int c = c8 + k;
-    q.v[k] = (bf16)(a0 * P[c] + a1 * P[C + c] + a2 * P[2 * C + c] + P[3 * C + c]);
-  }
-  *reinterpret_cast<Q8*>(h0 + p * C + c8) = q;
-}
 
 // ---------------------------------------------------------------------------
 // Row projection (one grid row x of one sample per iteration):
@@ -420,28 +392,254 @@ __global__ void __launch_bounds__(256) k_pixel(const bf16* __restrict__ P, const
 }
 
 // ---------------------------------------------------------------------------
-// projection: z[p] = s_id vv[p] + (sum_c h[p][c] W_out[c][j] + b_out[j]) / s
-__global__ void __launch_bounds__(256) k_proj(const bf16* __restrict__ h, const float* __restrict__ P,
-                                              const float2* __restrict__ v, float s, float2* __restrict__ z,
-                                              long total, const int* done) {
-  if (skip(done)) return;
-  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= total) return;
-  const float* W = P + kOffOut;
-  float o0 = W[2 * C], o1 = W[2 * C + 1];
+// Forward-pass kernels that own whole grid rows, so that the row projection of
+// the activations they produce (the first step of the next layer's spectral
+// convolution) runs on the row while it is still in LDS:
+//   k_lift_row  : lift of one row, then A_x = F^T H0_x
+//   k_cell_row  : cell GEMM + GELU of one row (layers 0-2), then A_x = F^T H'_x
+//   k_cell_proj : cell GEMM of layer 3 with the 32 -> 2 projection and the
+//                 identity path, z = s_id v + (h4 W_out + b_out) / s
+// The row is staged as [y][S] BF16 (the layout of k_rowproj).
+
+// A_x = F^T H_x from the row staged in LDS.  NW waves: wave w owns output
+// tile (a-tile (w >> 1) & 1, c-tile w & 1); with NW = 8 waves w and w + 4
+// split K in halves and the upper half adds into the lower through red.
+template <int NW>
+__device__ inline void row_project(const bf16* __restrict__ Xt, const bf16* ly, int N, float* __restrict__ o,
+                                   float* red) {
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int i = lane & 15, g = lane >> 4;
+  const int ta = (w >> 1) & 1, tc = w & 1;
+  const int kh = NW == 8 ? (w >> 2) : 0;
+  const int ylo = NW == 8 ? kh * (N / 2) : 0;
+  const int yhi = NW == 8 ? ylo + N / 2 : N;
+  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+  // K steps of 32 in groups of 4 with the basis loads of a group in flight
+  // together (the basis rows come from L2; one load per step would expose
+  // its latency N/32 times)
+  const int steps = (yhi - ylo) / 32;
+  const bf16* xa = Xt + (long)(16 * ta + i) * N + ylo + 8 * g;
+  for (int s0 = 0; s0 < steps; s0 += 4) {
+    bf16x8 a[4], b[4];
 #pragma unroll
-  for (int c8 = 0; c8 < C; c8 += 8) {
-    const bf16x8 hv = ld8(h + p * C + c8);
+    for (int u = 0; u < 4; ++u)
+      if (s0 + u < steps) a[u] = ld8(xa + 32 * (s0 + u));
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (s0 + u < steps) {
+        const int yb = ylo + 32 * (s0 + u) + 8 * g;
+#pragma unroll
+        for (int v = 0; v < 8; ++v) b[u][v] = ly[(yb + v) * S + 16 * tc + i];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (s0 + u < steps) acc = mma(a[u], b[u], acc);
+  }
+  if (NW == 8) {
+    if (kh == 1) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) red[((w & 3) * 64 + lane) * 4 + v] = acc[v];
+    }
+    __syncthreads();
+    if (kh == 1) return;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) acc[v] += red[(w * 64 + lane) * 4 + v];
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v) o[(16 * ta + 4 * g + v) * C + 16 * tc + i] = acc[v];
+}
+
+// One block (512 threads) per grid row x of one sample: the lift
+//   h0[p][c] = a0 W_in[0][c] + a1 W_in[1][c] + a2 W_in[2][c] + b_in[c]
+// of the row to global memory and LDS, then its row projection into Aout[x].
+// With nrm the input is v / *nrm (written back to vw when given), which lets
+// FGMRES hand over its lazily normalised Krylov vector.
+__global__ void __launch_bounds__(512) k_lift_row(const float* __restrict__ chi, float chi_scale, const float2* v,
+                                                  const double* nrm, float2* vw, float s,
+                                                  const float* __restrict__ P, bf16* __restrict__ h0,
+                                                  const bf16* __restrict__ Xt, float* __restrict__ Aout, int N,
+                                                  const int* done) {
+  if (skip(done)) return;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  bf16* lrow = reinterpret_cast<bf16*>(smem_raw);
+  __shared__ float red[4 * 64 * 4];
+  const long bx = blockIdx.x;
+  float r = 1.f;
+  if (nrm) {
+    const double d = *nrm;
+    r = d > 0.0 ? (float)(1.0 / d) : 0.f;
+  }
+  // item q = (cell y, quarter): 8 channels, one 16-byte store; the four items
+  // of a cell are adjacent lanes, so an in-place vw == v is read before it is
+  // written
+  for (int q = threadIdx.x; q < N * 4; q += 512) {
+    const int y = q >> 2, c8 = (q & 3) * 8;
+    const long p = bx * N + y;
+    float2 vv = v[p];
+    if (nrm) {
+      vv.x *= r;
+      vv.y *= r;
+      if (vw && c8 == 0) vw[p] = vv;
+    }
+    const float a0 = chi[p] * chi_scale, a1 = vv.x * s, a2 = vv.y * s;
+    struct __align__(16) Q8 { bf16 v[8]; } hq;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      const float x = (float)hv[k];
-      o0 += x * W[2 * (c8 + k)];
-      o1 += x * W[2 * (c8 + k) + 1];
+      const int c = c8 + k;
+      hq.v[k] = (bf16)(a0 * P[c] + a1 * P[C + c] + a2 * P[2 * C + c] + P[3 * C + c]);
+    }
+    *reinterpret_cast<Q8*>(h0 + p * C + c8) = hq;
+    const u32x4 bits = __builtin_bit_cast(u32x4, hq);
+    u32x2* d = reinterpret_cast<u32x2*>(lrow + y * S + c8);
+    d[0] = u32x2{bits[0], bits[1]};
+    d[1] = u32x2{bits[2], bits[3]};
+  }
+  __syncthreads();
+  row_project<8>(Xt, lrow, N, Aout + bx * C * C, red);
+}
+
+// One block (8 waves) per grid row x: h'[y] = GELU([h[y], F_y] [W; B_x] + b)
+// for the row's N cells (16-cell tiles t = wave + 8k, as in k_pixel) into
+// LDS, then one coalesced copy to out and the row projection into Anext[x].
+// With pre_out the pre-activations go to global memory as well (training).
+__global__ void __launch_bounds__(512) k_cell_row(const bf16* __restrict__ P, const bf16* __restrict__ Q,
+                                                  const bf16* __restrict__ A, const bf16* __restrict__ R,
+                                                  const float* __restrict__ bias, bf16* __restrict__ pre_out,
+                                                  bf16* __restrict__ out, const bf16* __restrict__ Xt,
+                                                  float* __restrict__ Anext, int N, const int* done) {
+  if (skip(done)) return;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  bf16* lrow = reinterpret_cast<bf16*>(smem_raw);
+  __shared__ float red[4 * 64 * 4];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const long bx = blockIdx.x;  // sample * N + x
+  const int i = lane & 15, g = lane >> 4;
+  bf16x8 aw[2], ar[2];
+  float bs[2][4];
+#pragma unroll
+  for (int to = 0; to < 2; ++to) {
+    aw[to] = ld8(A + (16 * to + i) * C + 8 * g);
+    ar[to] = ld8(R + (bx * C + 16 * to + i) * C + 8 * g);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) bs[to][v] = bias[16 * to + 4 * g + v];
+  }
+  const int ntile = N / 16;
+  for (int k0 = 0; w + 8 * k0 < ntile; k0 += 4) {
+    bf16x8 bpt[4], bqt[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int t = w + 8 * (k0 + u);
+      if (t < ntile) {
+        const int y = 16 * t + i;
+        bpt[u] = ld8(P + (bx * N + y) * C + 8 * g);
+        bqt[u] = ld8(Q + (long)y * C + 8 * g);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int t = w + 8 * (k0 + u);
+      if (t >= ntile) break;
+      const int y = 16 * t + i;
+#pragma unroll
+      for (int to = 0; to < 2; ++to) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        acc = mma(aw[to], bpt[u], acc);
+        acc = mma(ar[to], bqt[u], acc);
+        const int o0 = 16 * to + 4 * g;
+        float r[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) r[v] = acc[v] + bs[to][v];
+        if (pre_out) st4(pre_out + (bx * N + y) * C + o0, r[0], r[1], r[2], r[3]);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) r[v] = gelu(r[v]);
+        st4(lrow + y * S + o0, r[0], r[1], r[2], r[3]);
+      }
     }
   }
-  const float2 vv = v[p];
-  const float is = 1.f / s, sid = P[kOffSkip];
-  z[p] = make_float2(sid * vv.x + o0 * is, sid * vv.y + o1 * is);
+  __syncthreads();
+  for (int q = tid; q < N * 4; q += 512) {
+    const int y = q >> 2, c8 = (q & 3) * 8;
+    const u32x2* sp = reinterpret_cast<const u32x2*>(lrow + y * S + c8);
+    const u32x2 lo = sp[0], hi = sp[1];
+    *(u32x4*)(out + (bx * N + y) * C + c8) = u32x4{lo[0], lo[1], hi[0], hi[1]};
+  }
+  row_project<8>(Xt, lrow, N, Anext + bx * C * C, red);
+}
+
+// Layer 3 (no activation) fused with the projection: per cell
+//   h4 = [h3, F_y] [W; B_x] + b,  z = s_id vv / *vnrm + (h4 W_out + b_out) / s
+// in FP32 (vnrm null: no scaling); h4 goes to hout as BF16 when given
+// (training).  Grid as k_pixel.
+__global__ void __launch_bounds__(256) k_cell_proj(const bf16* __restrict__ P, const bf16* __restrict__ Q,
+                                                   const bf16* __restrict__ A, const bf16* __restrict__ R,
+                                                   const float* __restrict__ bias, bf16* __restrict__ hout,
+                                                   const float* __restrict__ Pp, const float2* __restrict__ vv,
+                                                   const double* vnrm, float s, float2* __restrict__ z, int N,
+                                                   const int* done) {
+  if (skip(done)) return;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const long wv = (long)blockIdx.x * 4 + w;  // every block is full: waves % 4 == 0
+  const int segs = N / 64;
+  const long bx = wv / segs;
+  const int y0 = (int)(wv % segs) * 64;
+  const int i = lane & 15, g = lane >> 4;
+  bf16x8 aw[2], ar[2];
+  float bs[2][4], wo[2][4][2];
+  const float* Wout = Pp + kOffOut;
+#pragma unroll
+  for (int to = 0; to < 2; ++to) {
+    aw[to] = ld8(A + (16 * to + i) * C + 8 * g);
+    ar[to] = ld8(R + (bx * C + 16 * to + i) * C + 8 * g);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int o = 16 * to + 4 * g + v;
+      bs[to][v] = bias[o];
+      wo[to][v][0] = Wout[2 * o];
+      wo[to][v][1] = Wout[2 * o + 1];
+    }
+  }
+  float sid = Pp[kOffSkip];
+  if (vnrm) {
+    const double d = *vnrm;
+    sid *= d > 0.0 ? (float)(1.0 / d) : 0.f;
+  }
+  const float b0 = Wout[2 * C], b1 = Wout[2 * C + 1], is = 1.f / s;
+  bf16x8 bpt[4], bqt[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int y = y0 + 16 * t + i;
+    bpt[t] = ld8(P + (bx * N + y) * C + 8 * g);
+    bqt[t] = ld8(Q + (long)y * C + 8 * g);
+  }
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const long p = bx * N + y0 + 16 * t + i;
+    float o0 = 0.f, o1 = 0.f;
+#pragma unroll
+    for (int to = 0; to < 2; ++to) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+      acc = mma(aw[to], bpt[t], acc);
+      acc = mma(ar[to], bqt[t], acc);
+      float r[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        r[v] = acc[v] + bs[to][v];
+        o0 += r[v] * wo[to][v][0];
+        o1 += r[v] * wo[to][v][1];
+      }
+      if (hout) st4(hout + p * C + 16 * to + 4 * g, r[0], r[1], r[2], r[3]);
+    }
+    // the four lane groups g hold channels 4g..4g+3 (+16) of cell i
+    o0 += __shfl_xor(o0, 16, 64);
+    o1 += __shfl_xor(o1, 16, 64);
+    o0 += __shfl_xor(o0, 32, 64);
+    o1 += __shfl_xor(o1, 32, 64);
+    if (g == 0) {
+      const float2 x = vv[p];
+      z[p] = make_float2(sid * x.x + (o0 + b0) * is, sid * x.y + (o1 + b1) * is);
+    }
+  }
 }
 
 // Block sums of NV per-thread values (every thread of the 256 passes vals);
@@ -649,27 +847,30 @@ int fno_forward(cgm_fno* f, hipStream_t st, const float* chi, const float2* v, c
                 float2* z, int B, bool train, const int* done) {
   const long total = (long)B * f->n;
   const float s = (float)f->N;
+  const int rows = B * f->N;
+  const size_t lds = (size_t)f->N * S * sizeof(bf16);
   int rc;
-  hipLaunchKernelGGL(k_lift, dim3((unsigned)((4 * total + 255) / 256)), dim3(256), 0, st, chi, f->chi_scale,
-                     v, nrm, vw, s, f->params, f->h[0], total, done);
+  hipLaunchKernelGGL(k_lift_row, dim3(rows), dim3(512), lds, st, chi, f->chi_scale, v, nrm, vw, s, f->params,
+                     f->h[0], f->Ft, f->A, f->N, done);
   CGM_HIP(hipGetLastError());
+  const float2* vv = vw ? vw : v;
   for (int l = 0; l < 4; ++l) {
     const float* Lp = f->params + layer_off(l);
-    if ((rc = launch_rowproj<false>(f, st, f->h[l], nullptr, f->A, B, done))) return rc;
     if ((rc = spectral(f, st, (const float2*)(Lp + C * C + C), train ? f->U[l] : nullptr, false, B, done)))
       return rc;
     const bf16* Wl = f->Wt + l * C * C;
-    if (l < 3)
-      rc = launch_pixel<1>(f, st, f->h[l], Wl, f->Bt, Lp + C * C, train ? f->pre[l] : nullptr, nullptr,
-                           f->h[l + 1], B, done);
-    else
-      rc = launch_pixel<0>(f, st, f->h[l], Wl, f->Bt, Lp + C * C, nullptr, nullptr, f->h[4], B, done);
-    if (rc) return rc;
+    if (l < 3) {
+      hipLaunchKernelGGL(k_cell_row, dim3(rows), dim3(512), lds, st, f->h[l], f->Fy, Wl, f->Bt, Lp + C * C,
+                         train ? f->pre[l] : nullptr, f->h[l + 1], f->Ft, f->A, f->N, done);
+    } else {
+      const long waves = total / 64;
+      CGM_CHECK(waves % 4 == 0, CGM_ERROR_UNSUPPORTED, "cell GEMM needs N %% 64 == 0");
+      hipLaunchKernelGGL(k_cell_proj, dim3((unsigned)(waves / 4)), dim3(256), 0, st, f->h[3], f->Fy, Wl, f->Bt,
+                         Lp + C * C, train ? f->h[4] : nullptr, f->params, vv, vw ? nullptr : nrm, s, z, f->N,
+                         done);
+    }
+    CGM_HIP(hipGetLastError());
   }
-  const float2* vv = vw ? vw : v;
-  hipLaunchKernelGGL(k_proj, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, f->h[4], f->params, vv, s,
-                     z, total, done);
-  CGM_HIP(hipGetLastError());
   if (train) {
     f->fwd_batch = B;
     f->chi_in = chi;
@@ -801,6 +1002,8 @@ extern "C" int cgm_fno_create(cgm_ctx_t c, int batch, float chi_scale, cgm_fno_t
   CGM_HIP(hipFuncSetAttribute((const void*)k_rowproj<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2));
   CGM_HIP(hipFuncSetAttribute((const void*)k_rowproj<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)(lds2 / 2)));
+  CGM_HIP(hipFuncSetAttribute((const void*)k_lift_row, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(lds2 / 2)));
+  CGM_HIP(hipFuncSetAttribute((const void*)k_cell_row, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(lds2 / 2)));
   *out = f;
   return CGM_SUCCESS;
 }
@@ -842,10 +1045,12 @@ extern "C" int cgm_fno_backward(cgm_fno_t f, const cgm_complex64* dz, float* gra
 }
 
 // Average device time (ms, HIP events on the context stream, reps launches
-// after two warm-ups) of each forward kernel of the network on one vector of
-// the batch 1 layout: ms[0] lift, [1] row projection, [2] x-DFT, [3] mixing,
-// [4] inverse x-DFT, [5] cell GEMM (GELU epilogue), [6] projection, [7] one
-// whole forward pass.
+// after two warm-ups) of each kernel of the network on one vector of the
+// batch 1 layout: ms[0] lift with the first row projection (k_lift_row),
+// [1] row projection alone (k_rowproj, used by the backward pass), [2] x-DFT,
+// [3] mixing, [4] inverse x-DFT, [5] cell GEMM with GELU and the next row
+// projection (k_cell_row), [6] last cell GEMM with the projection
+// (k_cell_proj), [7] one whole forward pass.
 extern "C" int cgm_fno_profile(cgm_fno_t f, const float* chi, const cgm_complex64* v, cgm_complex64* z, int reps,
                                float* ms) {
   CGM_CHECK(f && chi && v && z && ms && reps > 0, CGM_ERROR_INVALID_VALUE, "null argument");
@@ -857,6 +1062,7 @@ extern "C" int cgm_fno_profile(cgm_fno_t f, const float* chi, const cgm_complex6
   const int nch = f->N / XC;
   const float* Lp = f->params + layer_off(0);
   const float2* W = (const float2*)(Lp + C * C + C);
+  const size_t lds = (size_t)f->N * S * sizeof(bf16);
   hipEvent_t e0, e1;
   CGM_HIP(hipEventCreate(&e0));
   CGM_HIP(hipEventCreate(&e1));
@@ -866,8 +1072,8 @@ extern "C" int cgm_fno_profile(cgm_fno_t f, const float* chi, const cgm_complex6
       if (r == 0) CGM_HIP(hipEventRecord(e0, st));
       switch (k) {
         case 0:
-          hipLaunchKernelGGL(k_lift, dim3((unsigned)((4 * total + 255) / 256)), dim3(256), 0, st, chi,
-                             f->chi_scale, (const float2*)v, nullptr, nullptr, s, f->params, f->h[0], total, nullptr);
+          hipLaunchKernelGGL(k_lift_row, dim3(f->N), dim3(512), lds, st, chi, f->chi_scale, (const float2*)v,
+                             nullptr, nullptr, s, f->params, f->h[0], f->Ft, f->A, f->N, nullptr);
           break;
         case 1: rc = launch_rowproj<false>(f, st, f->h[0], nullptr, f->A, 1, nullptr); break;
         case 2:
@@ -882,11 +1088,13 @@ extern "C" int cgm_fno_profile(cgm_fno_t f, const float* chi, const cgm_complex6
                              nullptr);
           break;
         case 5:
-          rc = launch_pixel<1>(f, st, f->h[0], f->Wt, f->Bt, Lp + C * C, nullptr, nullptr, f->h[1], 1, nullptr);
+          hipLaunchKernelGGL(k_cell_row, dim3(f->N), dim3(512), lds, st, f->h[0], f->Fy, f->Wt, f->Bt, Lp + C * C,
+                             nullptr, f->h[1], f->Ft, f->A, f->N, nullptr);
           break;
         case 6:
-          hipLaunchKernelGGL(k_proj, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, f->h[4], f->params,
-                             (const float2*)v, s, (float2*)z, total, nullptr);
+          hipLaunchKernelGGL(k_cell_proj, dim3((unsigned)(total / 256)), dim3(256), 0, st, f->h[3], f->Fy, f->Wt,
+                             f->Bt, Lp + C * C, nullptr, f->params, (const float2*)v, nullptr, s, (float2*)z, f->N,
+                             nullptr);
           break;
         default:
           rc = fno_forward(f, st, chi, (const float2*)v, nullptr, nullptr, (float2*)z, 1, false, nullptr);

@@ -73,6 +73,7 @@ struct DevState {
   double tol_abs;  // tol * ||b||
   double res_est;  // latest |g_{i+1}|
   double norm_out; // result slot of the stand-alone norm kernel
+  double unorm;    // ||u_{j+1}|| from the flexible fused step (the network's input scale)
   int done;        // 1: remaining launches of this cycle are no-ops
   int k;           // valid Arnoldi columns of this cycle
   int status;      // 0 or CGM_ERROR_NONFINITE
@@ -117,8 +118,12 @@ struct KrylovWs {
 
 struct FusedLaunch {
   int N, m, ld, j, mode;  // m: steps of this cycle; ld: leading dimension of H and R
+  int flex = 0;           // flexible (right-preconditioned) steps: the matvec input is Xin = z_j
+  const float2* Xin = nullptr;
+  double* partn = nullptr;  // norm partials (flex)
   int B = 1;              // systems; per-system strides below (elements, DevState in bytes)
   long sChi = 0, sBasis = 0, sU = 0, sVec = 0, sT = 0, sPart = 0, sSmall = 0, sSt = 0, sHist = 0;
+  long sPartn = 0;
   const float* chi;
   float2* V;
   const float2* Uj;
@@ -165,7 +170,8 @@ struct cgm_context;
 struct cgm_fno;
 namespace cgm {
 // FNO preconditioner (cgm_fno.hip): z = P(v) for B vectors; with nrm the
-// input is v / *nrm, written back to vw.  Every launch is skipped once *done.
+// input is v / *nrm, written back to vw when given (v is left unchanged
+// otherwise).  Every launch is skipped once *done.
 int fno_forward(cgm_fno* f, hipStream_t st, const float* chi, const float2* v, const double* nrm, float2* vw,
                 float2* z, int B, bool train, const int* done);
 cgm_context* fno_context(const cgm_fno* f);
