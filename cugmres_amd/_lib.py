@@ -102,6 +102,9 @@ SIGNATURES = {
     "cgm_fno_profile": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, c_float_p]),
     "cgm_fgmres": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                   ctypes.POINTER(FgmresParams), ctypes.POINTER(FgmresResult), c_void_p]),
+    "cgm_fgmres_batched": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_long, c_void_p, c_void_p,
+                                          c_void_p, ctypes.c_int, ctypes.POINTER(FgmresParams),
+                                          ctypes.POINTER(FgmresResult), c_void_p]),
     "cgm_reference_plane_wave": (ctypes.c_int, [c_void_p, c_void_p]),
     "cgm_reference_chi": (ctypes.c_float, [ctypes.c_int]),
     "cgm_set_mask_contrast": (ctypes.c_int, [c_void_p, ctypes.c_float]),

@@ -832,6 +832,29 @@ __global__ void k_scale_copy(const float2* __restrict__ v, const double* __restr
   }
 }
 
+// The preconditioned basis Z, at least len elements.
+static int ensure_zb(cgm_context* c, long len) {
+  if (c->zb && c->zb_len >= len) return CGM_SUCCESS;
+  if (c->zb) CGM_HIP(hipFree(c->zb));
+  c->zb = nullptr;
+  c->zb_len = 0;
+  CGM_HIP(hipMalloc((void**)&c->zb, sizeof(float2) * (size_t)len));
+  c->zb_len = len;
+  return CGM_SUCCESS;
+}
+
+// z /= *nrm for system blockIdx.y (z + y * sz, nrm + y * sn bytes): the
+// identity preconditioner of batched flexible steps.
+__global__ void k_scale_inplace(float2* __restrict__ z, long sz, const double* nrm, long sn, long n) {
+  z += (long)blockIdx.y * sz;
+  const double d = *boff(nrm, (long)blockIdx.y * sn);
+  const float r = d > 0.0 ? (float)(1.0 / d) : 0.f;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    const float2 a = z[e];
+    z[e] = make_float2(a.x * r, a.y * r);
+  }
+}
+
 // One preconditioned cycle: z_i = P(v_i), w = A z_i, CGS2 step; the caller
 // applies x += Z y.
 static int precond_cycle(cgm_context* c, const float2* gq, const float* chi, cgm_fno* net, int m, double* hist,
@@ -913,13 +936,7 @@ extern "C" int cgm_fgmres(cgm_ctx_t c, const cgm_complex64* gamma, const float* 
   const int m = p->restart;
   int rc = krylov_alloc(c->ws, n, m, c->stream);
   if (rc) return rc;
-  if (!c->zb || c->zb_m < m) {
-    if (c->zb) CGM_HIP(hipFree(c->zb));
-    c->zb = nullptr;
-    c->zb_m = 0;
-    CGM_HIP(hipMalloc((void**)&c->zb, sizeof(float2) * n * m));
-    c->zb_m = m;
-  }
+  if ((rc = ensure_zb(c, n * m))) return rc;
   KrylovWs& ws = c->ws;
   const float2* bb = (const float2*)b;
   float2* xx = (float2*)x;
@@ -983,6 +1000,222 @@ extern "C" int cgm_fgmres(cgm_ctx_t c, const cgm_complex64* gamma, const float* 
   res->iterations = total;
   res->cycles = cycles;
   return res->converged ? CGM_SUCCESS : CGM_ERROR_NOT_CONVERGED;
+}
+
+// ---------------------------------------------------------------------------
+// Batched flexible GMRES: B systems of one grid size in lockstep, one network
+// launch per step for all of them
+// ---------------------------------------------------------------------------
+extern "C" int cgm_fgmres_batched(cgm_ctx_t c, const cgm_complex64* gamma, const float* chi, long chi_stride,
+                                  const cgm_complex64* b, cgm_complex64* x, cgm_fno_t net, int batch,
+                                  const cgm_fgmres_params* p, cgm_fgmres_result* results, double* hist) {
+  CGM_CHECK(c && gamma && chi && b && x && results && p, CGM_ERROR_INVALID_VALUE, "null argument");
+  CGM_CHECK(batch >= 1, CGM_ERROR_INVALID_VALUE, "batch %d < 1", batch);
+  CGM_CHECK(chi_stride == 0 || chi_stride >= (long)c->N * c->N, CGM_ERROR_INVALID_VALUE,
+            "chi_stride must be 0 (shared map) or >= N*N");
+  CGM_CHECK(p->restart >= 1 && p->restart <= kMaxRestart, CGM_ERROR_INVALID_VALUE, "restart %d outside [1, %d]",
+            p->restart, kMaxRestart);
+  CGM_CHECK(p->max_cycles >= 0, CGM_ERROR_INVALID_VALUE, "max_cycles < 0");
+  CGM_CHECK(p->tol >= 0 && isfinite(p->tol), CGM_ERROR_INVALID_VALUE, "bad tolerance");
+  CGM_CHECK(p->stall_ratio == 0.0, CGM_ERROR_UNSUPPORTED,
+            "cgm_fgmres_batched has no plain-GMRES fallback: stall_ratio must be 0");
+  CGM_CHECK(fused_supported(c->N, p->restart), CGM_ERROR_UNSUPPORTED,
+            "cgm_fgmres_batched needs a power-of-two N in [64, 2048] and restart <= 64");
+  CGM_CHECK(!net || fno_context(net) == c, CGM_ERROR_INVALID_VALUE,
+            "the preconditioner was created on another context");
+  CGM_CHECK(!net || fno_batch(net) >= batch, CGM_ERROR_INVALID_VALUE,
+            "the preconditioner was created for %d vectors per launch, fewer than batch %d",
+            net ? fno_batch(net) : 0, batch);
+  CGM_HIP(hipSetDevice(c->device));
+  memset(results, 0, sizeof(cgm_fgmres_result) * batch);
+  const long n = (long)c->N * c->N;
+  const int m = p->restart;
+  const int B = batch;
+  const long hstride = (long)m * (p->max_cycles > 0 ? p->max_cycles : 1) + 1;
+  int rc = krylov_alloc(c->ws, n, m, c->stream, B);
+  if (rc) return rc;
+  KrylovWs& ws = c->ws;
+  if (c->stB_n < B) {
+    if (c->stB) CGM_HIP(hipFree(c->stB));
+    if (c->hstB) CGM_HIP(hipHostFree(c->hstB));
+    c->stB = nullptr;
+    c->hstB = nullptr;
+    c->stB_n = 0;
+    CGM_HIP(hipMalloc((void**)&c->stB, sizeof(DevState) * B));
+    CGM_HIP(hipHostMalloc((void**)&c->hstB, sizeof(DevState) * B));
+    c->stB_n = B;
+  }
+  if ((rc = ensure_T32B(c, B))) return rc;
+  if ((rc = ensure_zb(c, ws.sBasis * B))) return rc;  // Z of system k at zb + k * sBasis
+  const long NM = (long)c->N * c->M;
+  const long sst = (long)sizeof(DevState);
+  const float2* gq = (const float2*)gamma;
+  const float2* bb = (const float2*)b;
+  float2* xx = (float2*)x;
+  DevState* stB = c->stB;
+  CGM_HIP(hipMemsetAsync(stB, 0, sizeof(DevState) * B, c->stream));
+  if (hist) CGM_HIP(hipMemsetAsync(hist, 0, sizeof(double) * hstride * B, c->stream));
+  if ((rc = launch_norm_init(c->stream, ws, bb, stB, 1, p->tol, n, sst))) return rc;
+  if (p->x0_zero) CGM_HIP(hipMemsetAsync(xx, 0, sizeof(float2) * n * B, c->stream));
+  BatchStrides rs;  // residual r = b - A x of every system into its V_0
+  rs.x = n;
+  rs.chi = chi_stride;
+  rs.b = n;
+  rs.y = ws.sBasis;
+  rs.T = NM;
+  rs.st = sst;
+  BatchStrides zs = rs;  // the matvec of z_j
+  zs.x = ws.sBasis;
+  auto residual = [&](hipStream_t s) -> int {
+    int r = launch_pass1<float>(c->plan, s, xx, nullptr, chi, nullptr, kModeResidual, c->T32B, c->tw32, nullptr, B, rs);
+    if (!r) r = launch_pass2<float, float2>(c->plan, s, c->T32B, gq, c->tw32, nullptr, B, rs);
+    if (!r) r = launch_pass3<float>(c->plan, s, c->T32B, xx, chi, bb, ws.V, kModeResidual, c->tw32, nullptr, B, rs);
+    if (!r) r = launch_norm_init(s, ws, ws.V, stB, 0, 0.0, ws.sBasis, sst);
+    return r;
+  };
+  // one flexible restart cycle of every live system, x += Z y, the next
+  // residual and its norm
+  auto enqueue_cycle = [&](hipStream_t s, int cyc) -> int {
+    int r = copy_strided(s, ws.V, ws.sBasis, ws.U, 2 * n, n, B);
+    if (r) return r;
+    FusedLaunch L{};
+    L.N = c->N; L.m = m; L.ld = ws.m; L.mode = kModeForward; L.chi = chi; L.V = ws.V; L.Z = ws.Z;
+    L.T = c->T32B; L.tw = c->tw32; L.part = ws.part; L.H = ws.H; L.Rm = ws.R; L.cs = ws.cs;
+    L.sn = ws.sn; L.g = ws.g; L.h1p = ws.h1p; L.a1 = ws.a1; L.a2 = ws.a2; L.scal = ws.scal;
+    L.st = stB; L.hist = hist ? hist + (long)cyc * m : nullptr;
+    L.B = B; L.sChi = chi_stride; L.sBasis = ws.sBasis; L.sU = 2 * n; L.sVec = n; L.sT = NM;
+    L.sPart = ws.sPart; L.sSmall = ws.sSmall; L.sSt = sst; L.sHist = hstride;
+    L.flex = 1;
+    L.partn = ws.partn;
+    L.sPartn = ws.sPartn;
+    L.sXin = ws.sBasis;
+    for (int j = 0; j <= m; ++j) {
+      L.j = j;
+      L.Uj = ws.U + (long)(j & 1) * n;
+      L.Unext = ws.U + (long)((j + 1) & 1) * n;
+      L.Xin = nullptr;
+      if (j < m) {
+        float2* zj = c->zb + (long)j * n;
+        const double* nrm = j == 0 ? &stB->beta : &stB->unorm;
+        if (net) {
+          FnoIO io;
+          io.chi = chi;
+          io.sChi = chi_stride;
+          io.v = L.Uj;
+          io.sV = 2 * n;
+          io.nrm = nrm;
+          io.sNrm = sst;
+          io.z = zj;
+          io.sZ = ws.sBasis;
+          r = fno_forward_io(net, s, io, B, false, nullptr);
+        } else {
+          r = copy_strided(s, L.Uj, 2 * n, zj, ws.sBasis, n, B);  // z = u (scaled below)
+          if (!r) {
+            hipLaunchKernelGGL(k_scale_inplace, dim3(ws.G, B), dim3(kKB), 0, s, zj, ws.sBasis, nrm, sst, n);
+            r = hipGetLastError() == hipSuccess ? CGM_SUCCESS : CGM_ERROR_DEVICE;
+          }
+        }
+        if (!r)
+          r = launch_pass1<float>(c->plan, s, zj, nullptr, chi, nullptr, kModeForward, c->T32B, c->tw32, &stB->done,
+                                  B, zs);
+        if (!r) r = launch_pass2<float, float2>(c->plan, s, c->T32B, gq, c->tw32, &stB->done, B, zs);
+        if (r) return r;
+        L.Xin = zj;
+      }
+      if ((r = launch_fused(s, 0, L))) return r;
+      if (j == m) break;
+      if ((r = launch_fused(s, 1, L))) return r;
+    }
+    if ((r = launch_backsub_update(s, ws, stB, xx, c->zb, n, sst))) return r;
+    return residual(s);
+  };
+  // first residual, eagerly on the context stream
+  if (p->x0_zero) {
+    if ((rc = copy_strided(c->stream, bb, n, ws.V, ws.sBasis, n, B))) return rc;
+    if ((rc = launch_norm_init(c->stream, ws, ws.V, stB, 0, 0.0, ws.sBasis, sst))) return rc;
+  } else if ((rc = residual(c->stream))) {
+    return rc;
+  }
+  if ((rc = read_states(c, B, c->stream))) return rc;
+  const bool graph = c->use_graphs && !hist;
+  hipStream_t cs = c->stream;
+  hipGraphExec_t exec = nullptr;
+  if (graph) {
+    if (!c->gstream) CGM_HIP(hipStreamCreateWithFlags(&c->gstream, hipStreamNonBlocking));
+    cs = c->gstream;
+  }
+  std::vector<int> done(B, 0);
+  int cycles = 0;
+  for (;;) {
+    int active = 0;
+    for (int k = 0; k < B; ++k) {
+      const DevState& h = c->hstB[k];
+      cgm_fgmres_result& r = results[k];
+      if (cycles == 0) {
+        r.beta0 = h.beta;
+        r.bnorm = h.bnorm;
+        if (hist) CGM_HIP(hipMemcpyAsync(hist + k * hstride, &c->hstB[k].beta, sizeof(double),
+                                         hipMemcpyHostToDevice, c->stream));
+      } else if (!done[k]) {
+        r.iterations += h.k_last;
+        r.cycles += 1;
+        r.precond_cycles += net ? 1 : 0;
+      }
+      if (done[k]) continue;
+      r.rel_residual = h.bnorm > 0 ? h.beta / h.bnorm : 0.0;
+      if (h.status) {
+        done[k] = 1;
+        continue;
+      }
+      if (h.bnorm == 0.0 || h.beta <= p->tol * h.bnorm) {
+        r.converged = 1;
+        done[k] = 1;
+        continue;
+      }
+      ++active;
+    }
+    if (!active || cycles >= p->max_cycles) break;
+    if (graph) {
+      if (!exec) {
+        hipGraph_t gr = nullptr;
+        CGM_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+        rc = enqueue_cycle(cs, 0);
+        const hipError_t ec = hipStreamEndCapture(cs, &gr);
+        if (rc) {
+          if (gr) (void)hipGraphDestroy(gr);
+          return rc;
+        }
+        CGM_HIP(ec);
+        const hipError_t ei = hipGraphInstantiate(&exec, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        CGM_HIP(ei);
+      }
+      const hipError_t el = hipGraphLaunch(exec, cs);
+      if (el != hipSuccess) {
+        (void)hipGraphExecDestroy(exec);
+        CGM_HIP(el);
+      }
+    } else if ((rc = enqueue_cycle(cs, cycles))) {
+      return rc;
+    }
+    rc = read_states(c, B, cs);
+    if (rc) {
+      if (exec) (void)hipGraphExecDestroy(exec);
+      return rc;
+    }
+    ++cycles;
+  }
+  if (exec) CGM_HIP(hipGraphExecDestroy(exec));
+  int any_nonfinite = 0, all_conv = 1;
+  for (int k = 0; k < B; ++k) {
+    if (c->hstB[k].status) any_nonfinite = 1;
+    if (!results[k].converged) all_conv = 0;
+  }
+  if (any_nonfinite) {
+    cgm_set_last_error("cgm_fgmres_batched: non-finite value in at least one system");
+    return CGM_ERROR_NONFINITE;
+  }
+  return all_conv ? CGM_SUCCESS : CGM_ERROR_NOT_CONVERGED;
 }
 
 // ---------------------------------------------------------------------------

@@ -82,7 +82,7 @@ struct FusedArgs {
   double* hist;          // estimates, hist[j] after j steps of this cycle (may be null)
   // per-system strides of a batched launch (system = blockIdx.y); elements,
   // DevState in bytes
-  long sChi, sBasis, sU, sVec, sT, sPart, sSmall, sSt, sHist, sPartn;
+  long sChi, sBasis, sU, sVec, sT, sPart, sSmall, sSt, sHist, sPartn, sXin;
 };
 
 __device__ inline void batch_offset(FusedArgs& a) {
@@ -91,7 +91,7 @@ __device__ inline void batch_offset(FusedArgs& a) {
   a.chi += k * a.sChi;
   a.V += k * a.sBasis;
   a.Uj += k * a.sU;
-  if (a.Xin) a.Xin += k * (a.flex ? a.sVec : a.sU);
+  if (a.Xin) a.Xin += k * a.sXin;
   a.Unext += k * a.sU;
   if (a.partn) a.partn += k * a.sPartn;
   a.Z += k * a.sVec;
@@ -548,6 +548,7 @@ int launch_fused(hipStream_t st, int which, const FusedLaunch& L) {
   a.Xin = L.Xin;
   a.partn = L.partn;
   a.sPartn = L.sPartn;
+  a.sXin = L.sXin;
   a.tail = which == 0 ? (L.j == L.m) : (L.j == L.m - 1);
   a.G = fused_blocks(L.N);
   a.n = (long)L.N * L.N;

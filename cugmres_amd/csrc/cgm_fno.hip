@@ -194,7 +194,9 @@ __global__ void __launch_bounds__(1024) k_dftx(const float* __restrict__ in, flo
   }
   __syncthreads();
   float2 acc = make_float2(0.f, 0.f);
-#pragma unroll
+  // partial unrolling: a full unroll hoists all 64 LDS operands and spills
+  // past the 128 VGPRs of a 1024-thread block
+#pragma unroll 8
   for (int xl = 0; xl < XC; ++xl) {
     const float2 e = se[hi][xl];
     const float2 z = sz[xl][lo];
@@ -454,10 +456,11 @@ __device__ inline void row_project(const bf16* __restrict__ Xt, const bf16* ly, 
 //   h0[p][c] = a0 W_in[0][c] + a1 W_in[1][c] + a2 W_in[2][c] + b_in[c]
 // of the row to global memory and LDS, then its row projection into Aout[x].
 // With nrm the input is v / *nrm (written back to vw when given), which lets
-// FGMRES hand over its lazily normalised Krylov vector.
-__global__ void __launch_bounds__(512) k_lift_row(const float* __restrict__ chi, float chi_scale, const float2* v,
-                                                  const double* nrm, float2* vw, float s,
-                                                  const float* __restrict__ P, bf16* __restrict__ h0,
+// FGMRES hand over its lazily normalised Krylov vector.  Sample k reads chi,
+// v (and vw) at k * sChi, k * sV and its norm at nrm + k * sNrm bytes.
+__global__ void __launch_bounds__(512) k_lift_row(const float* __restrict__ chi, long sChi, float chi_scale,
+                                                  const float2* v, long sV, const double* nrm, long sNrm, float2* vw,
+                                                  float s, const float* __restrict__ P, bf16* __restrict__ h0,
                                                   const bf16* __restrict__ Xt, float* __restrict__ Aout, int N,
                                                   const int* done) {
   if (skip(done)) return;
@@ -465,9 +468,14 @@ __global__ void __launch_bounds__(512) k_lift_row(const float* __restrict__ chi,
   bf16* lrow = reinterpret_cast<bf16*>(smem_raw);
   __shared__ float red[4 * 64 * 4];
   const long bx = blockIdx.x;
+  const long ks = bx / N;                     // sample
+  const long rin = (bx - ks * N) * N;         // the row's first cell in the sample
+  chi += ks * sChi;
+  v += ks * sV;
+  if (vw) vw += ks * sV;
   float r = 1.f;
   if (nrm) {
-    const double d = *nrm;
+    const double d = *boff(nrm, ks * sNrm);
     r = d > 0.0 ? (float)(1.0 / d) : 0.f;
   }
   // item q = (cell y, quarter): 8 channels, one 16-byte store; the four items
@@ -475,14 +483,14 @@ __global__ void __launch_bounds__(512) k_lift_row(const float* __restrict__ chi,
   // written
   for (int q = threadIdx.x; q < N * 4; q += 512) {
     const int y = q >> 2, c8 = (q & 3) * 8;
-    const long p = bx * N + y;
-    float2 vv = v[p];
+    const long p = bx * N + y, pin = rin + y;
+    float2 vv = v[pin];
     if (nrm) {
       vv.x *= r;
       vv.y *= r;
-      if (vw && c8 == 0) vw[p] = vv;
+      if (vw && c8 == 0) vw[pin] = vv;
     }
-    const float a0 = chi[p] * chi_scale, a1 = vv.x * s, a2 = vv.y * s;
+    const float a0 = chi[pin] * chi_scale, a1 = vv.x * s, a2 = vv.y * s;
     struct __align__(16) Q8 { bf16 v[8]; } hq;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
@@ -570,13 +578,14 @@ __global__ void __launch_bounds__(512) k_cell_row(const bf16* __restrict__ P, co
 // Layer 3 (no activation) fused with the projection: per cell
 //   h4 = [h3, F_y] [W; B_x] + b,  z = s_id vv / *vnrm + (h4 W_out + b_out) / s
 // in FP32 (vnrm null: no scaling); h4 goes to hout as BF16 when given
-// (training).  Grid as k_pixel.
+// (training).  Sample k reads vv and writes z at k * sV, k * sZ, its norm at
+// vnrm + k * sNrm bytes.  Grid as k_pixel.
 __global__ void __launch_bounds__(256) k_cell_proj(const bf16* __restrict__ P, const bf16* __restrict__ Q,
                                                    const bf16* __restrict__ A, const bf16* __restrict__ R,
                                                    const float* __restrict__ bias, bf16* __restrict__ hout,
                                                    const float* __restrict__ Pp, const float2* __restrict__ vv,
-                                                   const double* vnrm, float s, float2* __restrict__ z, int N,
-                                                   const int* done) {
+                                                   long sV, const double* vnrm, long sNrm, float s,
+                                                   float2* __restrict__ z, long sZ, int N, const int* done) {
   if (skip(done)) return;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const long wv = (long)blockIdx.x * 4 + w;  // every block is full: waves % 4 == 0
@@ -584,6 +593,11 @@ __global__ void __launch_bounds__(256) k_cell_proj(const bf16* __restrict__ P, c
   const long bx = wv / segs;
   const int y0 = (int)(wv % segs) * 64;
   const int i = lane & 15, g = lane >> 4;
+  const long ks = bx / N;                 // sample
+  const long rin = (bx - ks * N) * N;     // the row's first cell in the sample
+  vv += ks * sV;
+  z += ks * sZ;
+  if (vnrm) vnrm = boff(vnrm, ks * sNrm);
   bf16x8 aw[2], ar[2];
   float bs[2][4], wo[2][4][2];
   const float* Wout = Pp + kOffOut;
@@ -636,8 +650,9 @@ __global__ void __launch_bounds__(256) k_cell_proj(const bf16* __restrict__ P, c
     o0 += __shfl_xor(o0, 32, 64);
     o1 += __shfl_xor(o1, 32, 64);
     if (g == 0) {
-      const float2 x = vv[p];
-      z[p] = make_float2(sid * x.x + (o0 + b0) * is, sid * x.y + (o1 + b1) * is);
+      const long pin = rin + y0 + 16 * t + i;
+      const float2 x = vv[pin];
+      z[pin] = make_float2(sid * x.x + (o0 + b0) * is, sid * x.y + (o1 + b1) * is);
     }
   }
 }
@@ -843,17 +858,16 @@ namespace cgm {
 
 // Forward pass on B unit-norm vectors (or one lazily normalised vector with
 // nrm/vw).  train keeps activations for fno_backward.
-int fno_forward(cgm_fno* f, hipStream_t st, const float* chi, const float2* v, const double* nrm, float2* vw,
-                float2* z, int B, bool train, const int* done) {
-  const long total = (long)B * f->n;
+int fno_forward_io(cgm_fno* f, hipStream_t st, const FnoIO& io, int B, bool train, const int* done) {
   const float s = (float)f->N;
   const int rows = B * f->N;
+  const long total = (long)B * f->n;
   const size_t lds = (size_t)f->N * S * sizeof(bf16);
   int rc;
-  hipLaunchKernelGGL(k_lift_row, dim3(rows), dim3(512), lds, st, chi, f->chi_scale, v, nrm, vw, s, f->params,
-                     f->h[0], f->Ft, f->A, f->N, done);
+  hipLaunchKernelGGL(k_lift_row, dim3(rows), dim3(512), lds, st, io.chi, io.sChi, f->chi_scale, io.v, io.sV, io.nrm,
+                     io.sNrm, io.vw, s, f->params, f->h[0], f->Ft, f->A, f->N, done);
   CGM_HIP(hipGetLastError());
-  const float2* vv = vw ? vw : v;
+  const float2* vv = io.vw ? io.vw : io.v;
   for (int l = 0; l < 4; ++l) {
     const float* Lp = f->params + layer_off(l);
     if ((rc = spectral(f, st, (const float2*)(Lp + C * C + C), train ? f->U[l] : nullptr, false, B, done)))
@@ -866,18 +880,35 @@ int fno_forward(cgm_fno* f, hipStream_t st, const float* chi, const float2* v, c
       const long waves = total / 64;
       CGM_CHECK(waves % 4 == 0, CGM_ERROR_UNSUPPORTED, "cell GEMM needs N %% 64 == 0");
       hipLaunchKernelGGL(k_cell_proj, dim3((unsigned)(waves / 4)), dim3(256), 0, st, f->h[3], f->Fy, Wl, f->Bt,
-                         Lp + C * C, train ? f->h[4] : nullptr, f->params, vv, vw ? nullptr : nrm, s, z, f->N,
-                         done);
+                         Lp + C * C, train ? f->h[4] : nullptr, f->params, vv, io.sV, io.vw ? nullptr : io.nrm,
+                         io.sNrm, s, io.z, io.sZ, f->N, done);
     }
     CGM_HIP(hipGetLastError());
   }
   if (train) {
     f->fwd_batch = B;
-    f->chi_in = chi;
+    f->chi_in = io.chi;
     f->v_in = vv;
   }
   return CGM_SUCCESS;
 }
+
+int fno_forward(cgm_fno* f, hipStream_t st, const float* chi, const float2* v, const double* nrm, float2* vw,
+                float2* z, int B, bool train, const int* done) {
+  FnoIO io;
+  io.chi = chi;
+  io.sChi = f->n;
+  io.v = v;
+  io.sV = f->n;
+  io.nrm = nrm;
+  io.sNrm = 0;
+  io.vw = vw;
+  io.z = z;
+  io.sZ = f->n;
+  return fno_forward_io(f, st, io, B, train, done);
+}
+
+int fno_batch(const cgm_fno* f) { return f->B; }
 
 cgm_context* fno_context(const cgm_fno* f) { return f->ctx; }
 
@@ -1072,8 +1103,8 @@ extern "C" int cgm_fno_profile(cgm_fno_t f, const float* chi, const cgm_complex6
       if (r == 0) CGM_HIP(hipEventRecord(e0, st));
       switch (k) {
         case 0:
-          hipLaunchKernelGGL(k_lift_row, dim3(f->N), dim3(512), lds, st, chi, f->chi_scale, (const float2*)v,
-                             nullptr, nullptr, s, f->params, f->h[0], f->Ft, f->A, f->N, nullptr);
+          hipLaunchKernelGGL(k_lift_row, dim3(f->N), dim3(512), lds, st, chi, f->n, f->chi_scale, (const float2*)v,
+                             f->n, nullptr, 0L, nullptr, s, f->params, f->h[0], f->Ft, f->A, f->N, nullptr);
           break;
         case 1: rc = launch_rowproj<false>(f, st, f->h[0], nullptr, f->A, 1, nullptr); break;
         case 2:
@@ -1093,8 +1124,8 @@ extern "C" int cgm_fno_profile(cgm_fno_t f, const float* chi, const cgm_complex6
           break;
         case 6:
           hipLaunchKernelGGL(k_cell_proj, dim3((unsigned)(total / 256)), dim3(256), 0, st, f->h[3], f->Fy, f->Wt,
-                             f->Bt, Lp + C * C, nullptr, f->params, (const float2*)v, nullptr, s, (float2*)z, f->N,
-                             nullptr);
+                             f->Bt, Lp + C * C, nullptr, f->params, (const float2*)v, f->n, nullptr, 0L, s,
+                             (float2*)z, f->n, f->N, nullptr);
           break;
         default:
           rc = fno_forward(f, st, chi, (const float2*)v, nullptr, nullptr, (float2*)z, 1, false, nullptr);

@@ -120,6 +120,7 @@ struct FusedLaunch {
   int N, m, ld, j, mode;  // m: steps of this cycle; ld: leading dimension of H and R
   int flex = 0;           // flexible (right-preconditioned) steps: the matvec input is Xin = z_j
   const float2* Xin = nullptr;
+  long sXin = 0;            // per-system stride of Xin (elements)
   double* partn = nullptr;  // norm partials (flex)
   int B = 1;              // systems; per-system strides below (elements, DevState in bytes)
   long sChi = 0, sBasis = 0, sU = 0, sVec = 0, sT = 0, sPart = 0, sSmall = 0, sSt = 0, sHist = 0;
@@ -174,6 +175,22 @@ namespace cgm {
 // otherwise).  Every launch is skipped once *done.
 int fno_forward(cgm_fno* f, hipStream_t st, const float* chi, const float2* v, const double* nrm, float2* vw,
                 float2* z, int B, bool train, const int* done);
+// The same with per-sample strides (elements; the norm's in bytes): sample k
+// reads chi + k sChi (sChi = 0: one shared map), v + k sV, *(nrm + k sNrm)
+// and writes z + k sZ (and vw + k sV).
+struct FnoIO {
+  const float* chi = nullptr;
+  long sChi = 0;
+  const float2* v = nullptr;
+  long sV = 0;
+  const double* nrm = nullptr;
+  long sNrm = 0;
+  float2* vw = nullptr;
+  float2* z = nullptr;
+  long sZ = 0;
+};
+int fno_forward_io(cgm_fno* f, hipStream_t st, const FnoIO& io, int B, bool train, const int* done);
+int fno_batch(const cgm_fno* f);
 cgm_context* fno_context(const cgm_fno* f);
 }  // namespace cgm
 
@@ -196,8 +213,8 @@ struct cgm_context {
   float2* vec_tmp = nullptr;     // right-hand sides built on the device (N*N)
   double2* v64a = nullptr;       // refinement residual (N*N, lazy)
   float* chi32 = nullptr;        // refinement contrast in float (N*N, lazy)
-  float2* zb = nullptr;          // FGMRES preconditioned basis (zb_m * N*N, lazy)
-  int zb_m = 0;
+  float2* zb = nullptr;          // FGMRES preconditioned basis (zb_len elements, lazy)
+  long zb_len = 0;
   cgm::DevState* stB = nullptr;   // batched solves: one state per system (device)
   cgm::DevState* hstB = nullptr;  // pinned host mirror
   int stB_n = 0;

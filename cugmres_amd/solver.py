@@ -218,6 +218,40 @@ class Context:
             info["history"] = hist[: r.iterations + 1].cpu().numpy()
         return x, info
 
+    def fgmres_batched(self, gamma, chi, b, x=None, precond=None, restart=30, tol=1e-6, max_cycles=100,
+                       history=False):
+        """Lockstep flexible GMRES(restart) of B systems, one network forward
+        pass per step for all of them: b (B, N*N) complex64, chi (B, N*N) or
+        (N*N,) shared; precond a Preconditioner created with batch >= B (None
+        = the identity).  No plain-GMRES fallback.  Returns (x, [info per
+        system])."""
+        t = self.torch
+        B = b.shape[0]
+        n = self.N * self.N
+        chi_stride = 0 if chi.dim() == 1 else n
+        x0_zero = x is None
+        if x is None:
+            x = t.zeros_like(b)
+        hs = restart * max(max_cycles, 1) + 1
+        hist = t.zeros(B, hs, dtype=t.float64, device=self.device) if history else None
+        p = _lib.FgmresParams(int(restart), int(max_cycles), float(tol), int(x0_zero), 0.0)
+        res = (_lib.FgmresResult * B)()
+        net = precond.handle if precond is not None else None
+        st = lib().cgm_fgmres_batched(self._ptr, _ptr(gamma), _ptr(chi), chi_stride, _ptr(b), _ptr(x), net, B,
+                                      _c.byref(p), res, _ptr(hist))
+        check(st, "cgm_fgmres_batched", ok=(_lib.SUCCESS, _lib.ERROR_NOT_CONVERGED))
+        infos = []
+        h = hist.cpu().numpy() if history else None
+        for k in range(B):
+            r = res[k]
+            d = dict(iterations=r.iterations, cycles=r.cycles, converged=bool(r.converged),
+                     rel_residual=r.rel_residual, beta0=r.beta0, bnorm=r.bnorm,
+                     precond_cycles=r.precond_cycles, fell_back=bool(r.fell_back))
+            if history:
+                d["history"] = h[k]
+            infos.append(d)
+        return x, infos
+
     def gmres_refine(self, gamma32, gamma64, chi64, b, x=None, inner_restart=50, inner_tol=1e-4,
                      inner_max_cycles=40, outer_tol=1e-12, max_outer=30):
         t = self.torch

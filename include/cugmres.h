@@ -265,14 +265,28 @@ typedef struct {
 } cgm_fgmres_result;
 
 /* Right-preconditioned flexible GMRES(m) (Saad, SIAM J. Sci. Comput. 14
- * (1993) 461-469): z_j = P(v_j), w = A z_j, CGS2 against V, x += Z y.  Plain
- * GMRES (cgm_gmres's kernels) is the fallback and the judge: every cycle
- * starts from the true residual b - A x.  net = NULL applies the identity
- * (then the iterates equal cgm_gmres with the two-pass CGS2 kernels).
- * dev_hist as in cgm_gmres. */
+ * (1993) 461-469): z_j = P(v_j / ||v_j||), w = A z_j, orthogonalised against
+ * V, x += Z y.  For power-of-two N in [64, 2048] and m <= 64 the network takes
+ * the lagged once-orthogonalised vector and each step has one reduction
+ * (fused lagged CGS2); otherwise two-pass CGS2.  Plain GMRES (cgm_gmres's
+ * kernels) is the fallback and the judge: every cycle starts from the true
+ * residual b - A x.  net = NULL applies the identity (then the iterates equal
+ * cgm_gmres's).  dev_hist as in cgm_gmres.  No counterpart in the reference,
+ * which has no preconditioner (B/CUDA_GMRES/GMRES.cuh:22-446). */
 int cgm_fgmres(cgm_ctx_t ctx, const cgm_complex64* dev_gamma, const float* dev_chi,
                const cgm_complex64* dev_b, cgm_complex64* dev_x, cgm_fno_t net,
                const cgm_fgmres_params* params, cgm_fgmres_result* result, double* dev_hist);
+/* `batch` systems of the context's N in lockstep: b and x are [batch][N*N],
+ * chi is [batch][N*N] (chi_stride = N*N) or one map (chi_stride = 0); each
+ * step applies the network to all live systems in one forward pass (net
+ * created with at least `batch` vectors, or NULL for the identity).  Needs
+ * the fused kernels (power-of-two N in [64, 2048], restart <= 64) and
+ * stall_ratio = 0 (no fallback).  results[batch]; dev_hist, when given,
+ * [batch][restart * max_cycles + 1].  Each system's iterates equal its own
+ * cgm_fgmres solve.  No counterpart in the reference. */
+int cgm_fgmres_batched(cgm_ctx_t ctx, const cgm_complex64* dev_gamma, const float* dev_chi, long chi_stride,
+                       const cgm_complex64* dev_b, cgm_complex64* dev_x, cgm_fno_t net, int batch,
+                       const cgm_fgmres_params* params, cgm_fgmres_result* results, double* dev_hist);
 
 /* ---- reference-compatible entry points ------------------------------------ */
 /* The reference's plane wave exp(-i k i) with k = 2*3.14f/(N/6.f)

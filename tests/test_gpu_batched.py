@@ -148,3 +148,33 @@ def test_batched_matvec_equals_single(sol):
                 assert torch.equal(yb[k], y1), (N, mode, k)
                 y0 = ctx.matvec(g, chi_d[0].contiguous(), x_d[k].contiguous(), mode)
                 assert torch.equal(ys[k], y0), (N, mode, k)
+
+
+@pytest.mark.parametrize("N,B,shared,with_net", [(64, 3, False, True), (128, 4, True, True), (64, 2, False, False)])
+def test_fgmres_batched_equals_single_solves(sol, N, B, shared, with_net):
+    """Lockstep flexible GMRES: every system reproduces its own cgm_fgmres
+    solve (same network on each vector, same per-system arithmetic)."""
+    import torch
+    from cugmres_amd import fno, problems
+    chis, bs = _systems(N, B, 2.25, 40, shared=shared)
+    ctx = sol.Context(N)
+    g = ctx.spectrum_helmholtz(problems.KH)
+    chi_d = ctx.to_device(chis[0], "f32") if shared else ctx.to_device(chis, "f32").reshape(B, N * N)
+    b_d = ctx.to_device(bs, "c64").reshape(B, N * N)
+    net = None
+    if with_net:
+        p = fno.init_params(5, zero_out=False)
+        fno.unpack(p)["W_out"][...] *= 0.05
+        net = fno.Preconditioner(ctx, batch=B, params=p)
+    xb, ib = ctx.fgmres_batched(g, chi_d, b_d, precond=net, restart=12, tol=1e-7, max_cycles=4, history=True)
+    for k in range(B):
+        chi_k = chi_d if shared else chi_d[k].contiguous()
+        xs, isg = ctx.fgmres(g, chi_k, b_d[k].contiguous(), precond=net, restart=12, tol=1e-7, max_cycles=4,
+                             stall_ratio=0.0, history=True)
+        assert ib[k]["iterations"] == isg["iterations"]
+        assert ib[k]["converged"] == isg["converged"]
+        assert rel(xb[k].cpu().numpy(), xs.cpu().numpy()) < 1e-5
+        h1, h2 = ib[k]["history"][: isg["iterations"] + 1], isg["history"]
+        assert np.max(np.abs(h1 - h2)) <= 1e-5 * h2[0]
+        assert ib[k]["rel_residual"] == pytest.approx(isg["rel_residual"], rel=1e-4)
+    torch.cuda.synchronize()
