@@ -1,15 +1,21 @@
 """Benchmark of the GMRES solve on one MI355X: learned-FGMRES vs plain GMRES.
 
-Workload (one "step" = one solve with each method): the plain GMRES workload at
-N = 512 (262,144 unknowns, 1024 x 1024 FFT embedding), complex64, tolerance
-1e-6 on the true relative residual, x0 = 0.  The permittivity map is a seeded
-binary map of 8 x 8-cell blocks with values {1, 12.1} over the central 75 % of
-the area, h = lambda/20, and the plane wave comes in at a seeded angle.  Step
-k solves seed k with
-  - plain GMRES(30) (fused lagged-CGS2 kernels), and
-  - flexible GMRES(30) right-preconditioned by the FNO network, trained on the
-    GPU at N = 256 before the timed region (or loaded with --fno-params).
-All inputs are generated and copied to HBM before the timed region.
+Workload (one "step" = one batch of --batch systems solved in lockstep with
+each method): the plain GMRES workload at N = 512 (262,144 unknowns,
+1024 x 1024 FFT embedding), complex64, tolerance 1e-6 on the true relative
+residual, x0 = 0.  The permittivity map is a seeded binary map of 8 x 8-cell
+blocks with values {1, 12.1} over the central 75 % of the area, h = lambda/20,
+and the plane wave comes in at a seeded angle; every system has its own seed.
+Step k solves seeds k*B .. k*B+B-1 with
+  - plain GMRES(30) (cgm_gmres_batched: fused lagged-CGS2 kernels, every
+    launch covering the B systems), and
+  - flexible GMRES(30) right-preconditioned by the FNO network
+    (cgm_fgmres_batched: one network forward pass per step for the B
+    systems), trained on the GPU at N = 256 before the timed region (or
+    loaded with --fno-params).
+Each system's iterates equal its own single solve (tested), and the same B
+systems are also solved one at a time ("single").  All inputs are generated
+and copied to HBM before the timed region.
 
 A solve stops at the tolerance or after --max-cycles restart cycles; the
 number of converged solves, the achieved residuals and the wall time each
@@ -55,6 +61,7 @@ def parse():
                    help="FGMRES falls back to plain GMRES when a cycle leaves more than this "
                         "fraction of the residual (0: never)")
     p.add_argument("--plain-only", action="store_true")
+    p.add_argument("--batch", type=int, default=4, help="systems per lockstep batch (one step)")
     p.add_argument("--mode", choices=["bench", "step", "copy", "batched", "fno-step", "copybw", "sweep"],
                    default="bench",
                    help="bench: the benchmark; step: only the per-kernel profile of one Arnoldi "
@@ -129,11 +136,18 @@ def main():
         print(json.dumps(prof))
         return
     g = ctx.spectrum_helmholtz(problems.KH)
-    seeds = list(range(a.warmup + a.steps))
-    inputs = []
-    for s_ in seeds:
+    B = a.batch
+    if B < 1 or (B > 1 and a.stall_ratio > 0):
+        print(json.dumps({"error": "--batch must be >= 1, and batched learned solves have no stall fallback "
+                                   "(use --stall-ratio 0 or --batch 1)"}))
+        sys.exit(2)
+    inputs = []  # per system: (chi, b, theta) on the device
+    for s_ in range((a.warmup + a.steps) * B):
         chi, b, theta = problems.config_plain(N, seed=s_)
         inputs.append((ctx.to_device(chi.reshape(-1), "f32"), ctx.to_device(b.reshape(-1)), theta))
+    batches = [(torch.stack([inputs[k * B + j][0] for j in range(B)]),
+                torch.stack([inputs[k * B + j][1] for j in range(B)])) for k in range(a.warmup + a.steps)]
+    xb = torch.zeros(B, N * N, dtype=torch.complex64, device=ctx.device)
     x = torch.zeros(N * N, dtype=torch.complex64, device=ctx.device)
 
     # ---- the learned preconditioner (trained before the timed region) -------
@@ -154,20 +168,21 @@ def main():
                         "kernels": "cgm_fno_forward/backward (BF16 MFMA)"}
             tnet.close()
             tctx.close()
-        net = fno.Preconditioner(ctx, params=params)
+        net = fno.Preconditioner(ctx, batch=B, params=params)
     torch.cuda.synchronize()
 
     def plain(k):
-        dchi, db, _ = inputs[k]
-        x.zero_()
-        return ctx.gmres(g, dchi, db, x, restart=a.restart, tol=a.tol, max_cycles=a.max_cycles,
-                         history=True)[1]
+        chib, bb = batches[k]
+        return ctx.gmres_batched(g, chib, bb, xb.zero_(), restart=a.restart, tol=a.tol, max_cycles=a.max_cycles,
+                                 history=True)[1]
 
     def learned(k):
-        dchi, db, _ = inputs[k]
-        x.zero_()
-        return ctx.fgmres(g, dchi, db, x, precond=net, restart=a.restart, tol=a.tol,
-                          max_cycles=a.max_cycles, stall_ratio=a.stall_ratio, history=True)[1]
+        chib, bb = batches[k]
+        if B == 1:
+            return [ctx.fgmres(g, chib[0], bb[0], xb[0].zero_(), precond=net, restart=a.restart, tol=a.tol,
+                               max_cycles=a.max_cycles, stall_ratio=a.stall_ratio, history=True)[1]]
+        return ctx.fgmres_batched(g, chib, bb, xb.zero_(), precond=net, restart=a.restart, tol=a.tol,
+                                  max_cycles=a.max_cycles, history=True)[1]
 
     def run(fn):
         for k in range(a.warmup):
@@ -176,23 +191,25 @@ def main():
         infos = []
         t0 = time.perf_counter()
         for k in range(a.warmup, a.warmup + a.steps):
-            infos.append(fn(k))
+            infos.extend(fn(k))
         torch.cuda.synchronize()
         return infos, time.perf_counter() - t0
 
     p_infos, p_wall = run(plain)
     l_infos, l_wall = (run(learned) if net is not None else (None, None))
 
-    def summary(infos, wall):
+    def summary(infos, wall, nsolve):
         its = [i["iterations"] for i in infos]
-        return {"solves_per_s": a.steps / wall, "ms_per_solve": wall / a.steps * 1e3,
+        return {"solves_per_s": nsolve / wall, "ms_per_solve": wall / nsolve * 1e3,
                 "converged": int(sum(i["converged"] for i in infos)), "of": len(infos),
                 "iterations": its, "achieved_rel_residual": [i["rel_residual"] for i in infos],
                 "us_per_arnoldi_step": wall / max(1, sum(its)) * 1e6}
 
-    plain_s = summary(p_infos, p_wall)
-    out_learned = summary(l_infos, l_wall) if l_infos else None
+    plain_s = summary(p_infos, p_wall, a.steps * B)
+    plain_s["us_per_batch_step"] = plain_s["us_per_arnoldi_step"] * B
+    out_learned = summary(l_infos, l_wall, a.steps * B) if l_infos else None
     if out_learned:
+        out_learned["us_per_batch_step"] = out_learned["us_per_arnoldi_step"] * B
         out_learned["precond_cycles"] = [i["precond_cycles"] for i in l_infos]
         out_learned["fell_back"] = [i["fell_back"] for i in l_infos]
         # wall time to a common residual: the worst residual plain GMRES ends at
@@ -212,27 +229,47 @@ def main():
     else:
         matched = None
 
+    # ---- the same systems one at a time (first timed batch) -----------------
+    def single(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        infos = [fn(j) for j in range(B)]
+        torch.cuda.synchronize()
+        w = time.perf_counter() - t0
+        its = sum(i["iterations"] for i in infos)
+        return {"solves_per_s": B / w, "ms_per_solve": w / B * 1e3, "us_per_arnoldi_step": w / max(1, its) * 1e6}
+
+    def plain1(j):
+        dchi, db, _ = inputs[a.warmup * B + j]
+        return ctx.gmres(g, dchi, db, x.zero_(), restart=a.restart, tol=a.tol, max_cycles=a.max_cycles)[1]
+
+    def learned1(j):
+        dchi, db, _ = inputs[a.warmup * B + j]
+        return ctx.fgmres(g, dchi, db, x.zero_(), precond=net, restart=a.restart, tol=a.tol,
+                          max_cycles=a.max_cycles, stall_ratio=a.stall_ratio)[1]
+
+    single_out = {"systems": B, "plain": single(plain1), "learned": single(learned1) if net is not None else None}
+
     # ---- roofline of the dominant kernel of the timed path -------------------
     prof = profile_kernels(ctx, g, inputs[0][0], inputs[0][1], a.restart)
-    fprof = profile_fno(ctx, net, inputs[0][0], inputs[0][1]) if net is not None else None
+    fprof = profile_fno(ctx, net, batches[0][0], batches[0][1], B) if net is not None else None
     roof = prof["roofline"]
     roof["traffic"] = measured_traffic(roof["kernel"], N)
     if fprof is not None:
-        # per FGMRES step: one forward pass (lift, 4 x x-DFT / mixing / inverse, 3 row cell GEMMs,
-        # the projecting cell GEMM), passes 1 and 2 of the matvec of z_j, and the fused flexible
-        # k_arn_a (pass 3 + the single reduction) and k_arn_b (basis update + norm); the heaviest
-        # kernel class by time (the fused pair is known only as the remainder of the step time)
+        # per batch step of learned FGMRES: one forward pass for the B vectors (lift, 4 x x-DFT /
+        # mixing / inverse, 3 row cell GEMMs, the projecting cell GEMM), then passes 1-2 of the B
+        # matvecs of z_j and the fused flexible k_arn_a / k_arn_b; the heaviest kernel class by
+        # time (the matvec and the fused pair are known only as the remainder of the step time)
         per_step = {k: v["us"] * v["per_step"] for k, v in fprof["kernels"].items() if "per_step" in v}
-        for k in ("pass1_row_fft", "pass2_col_fft_mul"):
-            per_step[k] = prof["kernels"][k]["us"]
         dom = max(per_step, key=per_step.get)
-        per_step["fused_arnoldi_and_gaps"] = out_learned["us_per_arnoldi_step"] - sum(per_step.values())
-        src = fprof["kernels"] if dom in fprof["kernels"] else prof["kernels"]
+        per_step["matvec_arnoldi_and_gaps"] = out_learned["us_per_batch_step"] - sum(per_step.values())
+        src = fprof["kernels"]
         ach = src[dom]["GBps"]
+        traffic = measured_traffic(dom, N) if B == 1 else None
         roof = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": ach / HBM_PEAK_GBS, "traffic": measured_traffic(dom, N),
-                "per_unit": f"{src[dom]['bytes']} B per launch at N={N}",
-                "us_per_launch": src[dom]["us"], "us_per_fgmres_step": per_step}
+                "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
+                "per_unit": f"{src[dom]['bytes']} B per launch ({B} vectors at N={N})",
+                "us_per_launch": src[dom]["us"], "us_per_batch_step": per_step}
 
     # ---- the reference's own problem at the same N ------------------------
     refp = reference_problem(ctx, a.restart) if N == 512 else None
@@ -250,7 +287,7 @@ def main():
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": head["ms_per_solve"],
+        "ms_per_step": head["ms_per_solve"] * B,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -258,13 +295,14 @@ def main():
         "data": "synthetic (seeded eps maps and plane waves)",
         "config": {"workload": f"{'learned FGMRES' if out_learned else 'plain GMRES'}({a.restart}) N={N}, "
                                f"8x8 blocks eps in {{1,12.1}} over 75% area, h=lambda/20, tol {a.tol:g}, "
-                               f"cap {a.max_cycles} cycles",
+                               f"cap {a.max_cycles} cycles, {B} seeded systems per step in lockstep",
                    "N": N, "unknowns": N * N, "embedding": ctx.M, "restart": a.restart,
-                   "tol": a.tol, "max_cycles": a.max_cycles},
+                   "tol": a.tol, "max_cycles": a.max_cycles, "batch": B},
         "learned": out_learned,
         "plain": plain_s,
         "speedup_solves_per_s": (out_learned["solves_per_s"] / plain_s["solves_per_s"]) if out_learned else None,
         "residual_matched": matched,
+        "single": single_out,
         "training": training,
         "roofline": roof,
         "kernels": prof["kernels"],
@@ -345,19 +383,22 @@ def measured_traffic(kernel, N):
     return t["bytes_per_launch"]
 
 
-def profile_fno(ctx, net, dchi, db):
-    """Per-kernel durations of one forward pass of the network (HIP events on
-    the context stream, cgm_fno_profile) with their algorithmic bytes."""
+def profile_fno(ctx, net, dchi, db, batch=1):
+    """Per-kernel durations of one forward pass of the network on `batch`
+    vectors (HIP events on the context stream, cgm_fno_profile) with their
+    algorithmic bytes."""
     import torch
     from cugmres_amd import _lib
     from cugmres_amd._lib import lib
     N = ctx.N
     n = N * N
-    v = db / torch.linalg.vector_norm(db)
+    dchi = dchi.reshape(batch, n).contiguous()
+    db = db.reshape(batch, n)
+    v = (db / torch.linalg.vector_norm(db, dim=1, keepdim=True)).contiguous()
     z = torch.empty_like(v)
     ms = (ctypes.c_float * 8)()
     st = lib().cgm_fno_profile(net.handle, ctypes.c_void_p(dchi.data_ptr()), ctypes.c_void_p(v.data_ptr()),
-                               ctypes.c_void_p(z.data_ptr()), 50, ms)
+                               ctypes.c_void_p(z.data_ptr()), batch, 50, ms)
     _lib.check(st, "cgm_fno_profile")
     names = ["fno_lift", "fno_rowproj", "fno_dftx", "fno_mix", "fno_invx", "fno_cell_gemm", "fno_proj",
              "fno_forward_total"]
@@ -370,11 +411,13 @@ def profile_fno(ctx, net, dchi, db):
     # grid row, x-DFT partials 128 KiB per 32 rows, spectral weights 4 MiB per
     # layer, mixed modes 128 KiB, row twiddles 256 B per row, BF16 per-row
     # mixing matrices 2 KiB per row
+    # (per vector; the spectral weights once per launch)
     part = (N // 32) * 16384 * 8
-    bytes_ = {"fno_lift": n * (4 + 8 + 64) + N * 4096, "fno_rowproj": n * 64 + N * 4096,
-              "fno_dftx": N * 4096 + N * 256 + part, "fno_mix": part + 4 * 1024 * 1024 + 16384 * 8,
-              "fno_invx": 16384 * 8 + N * 256 + N * 2048, "fno_cell_gemm": n * 128 + N * 2048 + N * 4096,
-              "fno_proj": n * (64 + 16) + N * 2048}
+    Bv = batch
+    bytes_ = {"fno_lift": Bv * (n * (4 + 8 + 64) + N * 4096), "fno_rowproj": Bv * (n * 64 + N * 4096),
+              "fno_dftx": Bv * (N * 4096 + part) + N * 256, "fno_mix": Bv * (part + 16384 * 8) + 4 * 1024 * 1024,
+              "fno_invx": Bv * (16384 * 8 + N * 2048) + N * 256,
+              "fno_cell_gemm": Bv * (n * 128 + N * 2048 + N * 4096), "fno_proj": Bv * (n * (64 + 16) + N * 2048)}
     per_step = {"fno_lift": 1, "fno_rowproj": 0, "fno_dftx": 4, "fno_mix": 4, "fno_invx": 4,
                 "fno_cell_gemm": 3, "fno_proj": 1}
     kernels = {}

@@ -99,7 +99,8 @@ SIGNATURES = {
     "cgm_fno_set_params": (ctypes.c_int, [c_void_p, c_void_p]),
     "cgm_fno_forward": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, ctypes.c_int]),
     "cgm_fno_backward": (ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
-    "cgm_fno_profile": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, c_float_p]),
+    "cgm_fno_profile": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, ctypes.c_int,
+                                       c_float_p]),
     "cgm_fgmres": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                   ctypes.POINTER(FgmresParams), ctypes.POINTER(FgmresResult), c_void_p]),
     "cgm_fgmres_batched": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_long, c_void_p, c_void_p,

@@ -1076,19 +1076,22 @@ extern "C" int cgm_fno_backward(cgm_fno_t f, const cgm_complex64* dz, float* gra
 }
 
 // Average device time (ms, HIP events on the context stream, reps launches
-// after two warm-ups) of each kernel of the network on one vector of the
-// batch 1 layout: ms[0] lift with the first row projection (k_lift_row),
+// after two warm-ups) of each kernel of the network on `batch` vectors
+// (contiguous [batch][N*N] chi, v, z): ms[0] lift with the first row projection (k_lift_row),
 // [1] row projection alone (k_rowproj, used by the backward pass), [2] x-DFT,
 // [3] mixing, [4] inverse x-DFT, [5] cell GEMM with GELU and the next row
 // projection (k_cell_row), [6] last cell GEMM with the projection
 // (k_cell_proj), [7] one whole forward pass.
-extern "C" int cgm_fno_profile(cgm_fno_t f, const float* chi, const cgm_complex64* v, cgm_complex64* z, int reps,
-                               float* ms) {
+extern "C" int cgm_fno_profile(cgm_fno_t f, const float* chi, const cgm_complex64* v, cgm_complex64* z, int batch,
+                               int reps, float* ms) {
   CGM_CHECK(f && chi && v && z && ms && reps > 0, CGM_ERROR_INVALID_VALUE, "null argument");
+  CGM_CHECK(batch >= 1 && batch <= f->B, CGM_ERROR_INVALID_VALUE, "batch %d outside [1, %d]", batch, f->B);
   CGM_CHECK(f->params, CGM_ERROR_INVALID_VALUE, "cgm_fno_set_params has not been called");
   CGM_HIP(hipSetDevice(f->ctx->device));
   hipStream_t st = f->ctx->stream;
-  const long total = f->n;
+  const int B = batch;
+  const long total = (long)B * f->n;
+  const unsigned rows = (unsigned)(B * f->N);
   const float s = (float)f->N;
   const int nch = f->N / XC;
   const float* Lp = f->params + layer_off(0);
@@ -1103,23 +1106,23 @@ extern "C" int cgm_fno_profile(cgm_fno_t f, const float* chi, const cgm_complex6
       if (r == 0) CGM_HIP(hipEventRecord(e0, st));
       switch (k) {
         case 0:
-          hipLaunchKernelGGL(k_lift_row, dim3(f->N), dim3(512), lds, st, chi, f->n, f->chi_scale, (const float2*)v,
+          hipLaunchKernelGGL(k_lift_row, dim3(rows), dim3(512), lds, st, chi, f->n, f->chi_scale, (const float2*)v,
                              f->n, nullptr, 0L, nullptr, s, f->params, f->h[0], f->Ft, f->A, f->N, nullptr);
           break;
-        case 1: rc = launch_rowproj<false>(f, st, f->h[0], nullptr, f->A, 1, nullptr); break;
+        case 1: rc = launch_rowproj<false>(f, st, f->h[0], nullptr, f->A, B, nullptr); break;
         case 2:
-          hipLaunchKernelGGL(k_dftx, dim3(nch, KY, 1), dim3(1024), 0, st, f->A, f->upart, f->Ex, f->N, 0, nullptr);
+          hipLaunchKernelGGL(k_dftx, dim3(nch, KY, B), dim3(1024), 0, st, f->A, f->upart, f->Ex, f->N, 0, nullptr);
           break;
         case 3:
-          hipLaunchKernelGGL((k_mix<false>), dim3(KY, KX / KXG, 1), dim3(256), 0, st, f->upart, nch, W, nullptr,
+          hipLaunchKernelGGL((k_mix<false>), dim3(KY, KX / KXG, B), dim3(256), 0, st, f->upart, nch, W, nullptr,
                              f->Vm, f->N, nullptr);
           break;
         case 4:
-          hipLaunchKernelGGL(k_invx, dim3(f->N / XB, KY / KYG, 1), dim3(512), 0, st, f->Vm, f->Bt, f->Ex, f->N,
+          hipLaunchKernelGGL(k_invx, dim3(f->N / XB, KY / KYG, B), dim3(512), 0, st, f->Vm, f->Bt, f->Ex, f->N,
                              nullptr);
           break;
         case 5:
-          hipLaunchKernelGGL(k_cell_row, dim3(f->N), dim3(512), lds, st, f->h[0], f->Fy, f->Wt, f->Bt, Lp + C * C,
+          hipLaunchKernelGGL(k_cell_row, dim3(rows), dim3(512), lds, st, f->h[0], f->Fy, f->Wt, f->Bt, Lp + C * C,
                              nullptr, f->h[1], f->Ft, f->A, f->N, nullptr);
           break;
         case 6:
@@ -1128,7 +1131,7 @@ extern "C" int cgm_fno_profile(cgm_fno_t f, const float* chi, const cgm_complex6
                              (float2*)z, f->n, f->N, nullptr);
           break;
         default:
-          rc = fno_forward(f, st, chi, (const float2*)v, nullptr, nullptr, (float2*)z, 1, false, nullptr);
+          rc = fno_forward(f, st, chi, (const float2*)v, nullptr, nullptr, (float2*)z, B, false, nullptr);
           break;
       }
       if (hipGetLastError() != hipSuccess) rc = CGM_ERROR_DEVICE;

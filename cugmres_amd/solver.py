@@ -188,7 +188,7 @@ class Context:
             d = dict(iterations=r.iterations, cycles=r.cycles, converged=bool(r.converged),
                      rel_residual=r.rel_residual, beta0=r.beta0, bnorm=r.bnorm)
             if history:
-                d["history"] = h[k]
+                d["history"] = h[k][: r.iterations + 1]
             infos.append(d)
         return x, infos
 
@@ -248,7 +248,7 @@ class Context:
                      rel_residual=r.rel_residual, beta0=r.beta0, bnorm=r.bnorm,
                      precond_cycles=r.precond_cycles, fell_back=bool(r.fell_back))
             if history:
-                d["history"] = h[k]
+                d["history"] = h[k][: r.iterations + 1]
             infos.append(d)
         return x, infos
 

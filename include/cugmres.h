@@ -360,12 +360,14 @@ int cgm_profile_step(cgm_ctx_t ctx, const cgm_complex64* dev_gamma, const float*
                      const cgm_complex64* dev_x, int restart, int reps, float* ms);
 
 /* Average device time (ms, HIP events on the context stream, reps launches
- * each) of the preconditioner's forward kernels on one vector: ms[0] lift,
- * [1] row projection, [2] x-DFT, [3] mode mixing, [4] inverse x-DFT, [5] cell
- * GEMM with GELU, [6] projection, [7] a whole forward pass.  Overwrites the
- * network's activations. */
+ * each) of the preconditioner's kernels on `batch` vectors ([batch][N*N] chi,
+ * v, z): ms[0] lift with the first row projection, [1] stand-alone row
+ * projection (backward pass), [2] x-DFT, [3] mode mixing, [4] inverse x-DFT,
+ * [5] cell GEMM with GELU and the next row projection, [6] last cell GEMM
+ * with the projection, [7] a whole forward pass.  Overwrites the network's
+ * activations. */
 int cgm_fno_profile(cgm_fno_t net, const float* dev_chi, const cgm_complex64* dev_v,
-                    cgm_complex64* dev_z, int reps, float* ms);
+                    cgm_complex64* dev_z, int batch, int reps, float* ms);
 
 /* ---- memory and host helpers ---------------------------------------------- */
 int cgm_malloc(void** ptr, size_t bytes);
