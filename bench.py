@@ -403,22 +403,26 @@ def profile_fno(ctx, net, dchi, db, batch=1):
     names = ["fno_lift", "fno_rowproj", "fno_dftx", "fno_mix", "fno_invx", "fno_cell_gemm", "fno_proj",
              "fno_forward_total"]
     # launches per forward pass: fno_lift = k_lift_row (lift + layer-0 row
-    # projection), fno_cell_gemm = k_cell_row (cell GEMM + GELU + next row
-    # projection, layers 0-2), fno_proj = k_cell_proj (layer-3 cell GEMM + the
-    # 32 -> 2 projection); the stand-alone row projection runs only in the
-    # backward pass.  Algorithmic bytes per launch: BF16 activations 64 B per
-    # cell, complex64 vectors 8 B, chi 4 B; FP32 row projections 4 KiB per
-    # grid row, x-DFT partials 128 KiB per 32 rows, spectral weights 4 MiB per
-    # layer, mixed modes 128 KiB, row twiddles 256 B per row, BF16 per-row
-    # mixing matrices 2 KiB per row
+    # projection); fno_cell_gemm = k_cell_row (B_x of the row, cell GEMM +
+    # GELU + next row projection, layers 0-2); fno_proj = k_cell_proj (layer-3
+    # cell GEMM + the 32 -> 2 projection).  For one vector the row kernels of
+    # layers 0-2 form B_x themselves (inverse x-DFT from the mixed modes) and
+    # k_invx runs for layer 3 only; for more, k_invx forms every B_x.  The
+    # stand-alone row projection runs only in the backward pass.  Algorithmic
+    # bytes per launch: BF16 activations 64 B per cell, complex64 vectors 8 B,
+    # chi 4 B; FP32 row projections 4 KiB per grid row, x-DFT partials
+    # 128 KiB per 32 rows, spectral weights 4 MiB per layer, mixed modes
+    # 128 KiB per vector, row twiddles 256 B per row, BF16 per-row mixing
+    # matrices 2 KiB per row
     # (per vector; the spectral weights once per launch)
     part = (N // 32) * 16384 * 8
     Bv = batch
     bytes_ = {"fno_lift": Bv * (n * (4 + 8 + 64) + N * 4096), "fno_rowproj": Bv * (n * 64 + N * 4096),
               "fno_dftx": Bv * (N * 4096 + part) + N * 256, "fno_mix": Bv * (part + 16384 * 8) + 4 * 1024 * 1024,
               "fno_invx": Bv * (16384 * 8 + N * 2048) + N * 256,
-              "fno_cell_gemm": Bv * (n * 128 + N * 2048 + N * 4096), "fno_proj": Bv * (n * (64 + 16) + N * 2048)}
-    per_step = {"fno_lift": 1, "fno_rowproj": 0, "fno_dftx": 4, "fno_mix": 4, "fno_invx": 4,
+              "fno_cell_gemm": Bv * (n * 128 + N * 4096 + (128 * 1024 if batch == 1 else N * 2048)) + N * 256,
+              "fno_proj": Bv * (n * (64 + 16) + N * 2048)}
+    per_step = {"fno_lift": 1, "fno_rowproj": 0, "fno_dftx": 4, "fno_mix": 4, "fno_invx": 1 if batch == 1 else 4,
                 "fno_cell_gemm": 3, "fno_proj": 1}
     kernels = {}
     for i, k in enumerate(names):

@@ -31,8 +31,10 @@
 //   k_invx         : inverse DFT along x (FP32) -> B_x
 //   cell GEMM      : h' = act([h, F_y] [W; B_x] + b)  per cell (K = 64, MFMA)
 // In the forward pass the row projection of each layer's input runs inside
-// the kernel that produces it (k_lift_row, k_cell_row) and the last cell GEMM
-// carries the 32 -> 2 projection (k_cell_proj): 1 + 4 x 4 launches.  The
+// the kernel that produces it (k_lift_row, k_cell_row), for one sample the
+// inverse x-DFT of each row of layers 0-2 runs in the cell GEMM of that row,
+// and the last cell GEMM carries the 32 -> 2 projection (k_cell_proj):
+// 1 + 3 x 3 + 4 launches for one sample, 1 + 4 x 4 for more.  The
 // backward pass has the separate kernels (k_rowproj, k_pixel) with the roles
 // transposed.
 //
@@ -398,7 +400,8 @@ __global__ void __launch_bounds__(256) k_pixel(const bf16* __restrict__ P, const
 // the activations they produce (the first step of the next layer's spectral
 // convolution) runs on the row while it is still in LDS:
 //   k_lift_row  : lift of one row, then A_x = F^T H0_x
-//   k_cell_row  : cell GEMM + GELU of one row (layers 0-2), then A_x = F^T H'_x
+//   k_cell_row  : B_x of the row, cell GEMM + GELU of the row (layers 0-2),
+//                 then A_x = F^T H'_x
 //   k_cell_proj : cell GEMM of layer 3 with the 32 -> 2 projection and the
 //                 identity path, z = s_id v + (h4 W_out + b_out) / s
 // The row is staged as [y][S] BF16 (the layout of k_rowproj).
@@ -507,12 +510,21 @@ __global__ void __launch_bounds__(512) k_lift_row(const float* __restrict__ chi,
   row_project<8>(Xt, lrow, N, Aout + bx * C * C, red);
 }
 
-// One block (8 waves) per grid row x: h'[y] = GELU([h[y], F_y] [W; B_x] + b)
-// for the row's N cells (16-cell tiles t = wave + 8k, as in k_pixel) into
-// LDS, then one coalesced copy to out and the row projection into Anext[x].
-// With pre_out the pre-activations go to global memory as well (training).
+// One block (8 waves) per grid row x of sample k (layers 0-2):
+//   1. B_x to LDS (BF16), the MFMA A operand of the spectral half of the
+//      GEMM: with R, copied from R[k][x] (k_invx's output); without, formed
+//      here as the inverse x-DFT of the mixed modes at this row (thread
+//      (ky, o): B_x[o][2ky] + i B_x[o][2ky+1] = sum_kxi Vm[k][kxi][ky][o]
+//      e^{+2 pi i kx x/N}), which saves k_invx's launch but reads all of
+//      Vm[k] (128 KiB, from L2) per row: the better choice for one sample;
+//   2. h'[y] = GELU([h[y], F_y] [W; B_x] + b) for the row's N cells (16-cell
+//      tiles t = wave + 8u, as in k_pixel) into LDS (pre-activations to
+//      pre_out when given, for training);
+//   3. one coalesced copy of the row to out and its row projection into
+//      Anext[x].
 __global__ void __launch_bounds__(512) k_cell_row(const bf16* __restrict__ P, const bf16* __restrict__ Q,
                                                   const bf16* __restrict__ A, const bf16* __restrict__ R,
+                                                  const float2* __restrict__ Vm, const float2* __restrict__ Ex,
                                                   const float* __restrict__ bias, bf16* __restrict__ pre_out,
                                                   bf16* __restrict__ out, const bf16* __restrict__ Xt,
                                                   float* __restrict__ Anext, int N, const int* done) {
@@ -520,28 +532,67 @@ __global__ void __launch_bounds__(512) k_cell_row(const bf16* __restrict__ P, co
   extern __shared__ __align__(16) unsigned char smem_raw[];
   bf16* lrow = reinterpret_cast<bf16*>(smem_raw);
   __shared__ float red[4 * 64 * 4];
+  __shared__ __align__(16) bf16 lb[C][C + 8];  // B_x as [o][k], rows padded to 80 B
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const long bx = blockIdx.x;  // sample * N + x
+  const long ks = bx / N;
+  const int xr = (int)(bx - ks * N);
   const int i = lane & 15, g = lane >> 4;
-  bf16x8 aw[2], ar[2];
+  const int ntile = N / 16;
+  // the first tiles' cell operands are in flight while B_x is formed
+  bf16x8 bpt[4], bqt[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int t = w + 8 * u;
+    if (t < ntile) {
+      const int y = 16 * t + i;
+      bpt[u] = ld8(P + (bx * N + y) * C + 8 * g);
+      bqt[u] = ld8(Q + (long)y * C + 8 * g);
+    }
+  }
+  if (R) {
+    if (tid < C * C / 8) {  // 128 threads x 8 BF16
+      const int o = tid >> 2, k8 = (tid & 3) * 8;
+      *reinterpret_cast<u32x4*>(&lb[o][k8]) = *reinterpret_cast<const u32x4*>(R + (bx * C + o) * C + k8);
+    }
+  } else {
+    const int ky = tid >> 5, o = tid & 31;
+    const float2* Vs = Vm + (ks * KX * KY + ky) * C + o;
+    float2 vq[KX];
+#pragma unroll
+    for (int q = 0; q < KX; ++q) vq[q] = Vs[(long)q * KY * C];
+    float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int q = 0; q < KX; ++q) {
+      const float2 e = Ex[(long)q * N + xr];
+      acc.x += vq[q].x * e.x - vq[q].y * e.y;
+      acc.y += vq[q].x * e.y + vq[q].y * e.x;
+    }
+    lb[o][2 * ky] = (bf16)acc.x;
+    lb[o][2 * ky + 1] = (bf16)acc.y;
+  }
+  bf16x8 aw[2];
   float bs[2][4];
 #pragma unroll
   for (int to = 0; to < 2; ++to) {
     aw[to] = ld8(A + (16 * to + i) * C + 8 * g);
-    ar[to] = ld8(R + (bx * C + 16 * to + i) * C + 8 * g);
 #pragma unroll
     for (int v = 0; v < 4; ++v) bs[to][v] = bias[16 * to + 4 * g + v];
   }
-  const int ntile = N / 16;
-  for (int k0 = 0; w + 8 * k0 < ntile; k0 += 4) {
-    bf16x8 bpt[4], bqt[4];
+  __syncthreads();
+  bf16x8 ar[2];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int t = w + 8 * (k0 + u);
-      if (t < ntile) {
-        const int y = 16 * t + i;
-        bpt[u] = ld8(P + (bx * N + y) * C + 8 * g);
-        bqt[u] = ld8(Q + (long)y * C + 8 * g);
+  for (int to = 0; to < 2; ++to) ar[to] = *reinterpret_cast<const bf16x8*>(&lb[16 * to + i][8 * g]);
+  for (int k0 = 0; w + 8 * k0 < ntile; k0 += 4) {
+    if (k0 > 0) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int t = w + 8 * (k0 + u);
+        if (t < ntile) {
+          const int y = 16 * t + i;
+          bpt[u] = ld8(P + (bx * N + y) * C + 8 * g);
+          bqt[u] = ld8(Q + (long)y * C + 8 * g);
+        }
       }
     }
 #pragma unroll
@@ -554,14 +605,14 @@ __global__ void __launch_bounds__(512) k_cell_row(const bf16* __restrict__ P, co
         f32x4 acc = {0.f, 0.f, 0.f, 0.f};
         acc = mma(aw[to], bpt[u], acc);
         acc = mma(ar[to], bqt[u], acc);
-        const int o0 = 16 * to + 4 * g;
+        const int oc0 = 16 * to + 4 * g;
         float r[4];
 #pragma unroll
         for (int v = 0; v < 4; ++v) r[v] = acc[v] + bs[to][v];
-        if (pre_out) st4(pre_out + (bx * N + y) * C + o0, r[0], r[1], r[2], r[3]);
+        if (pre_out) st4(pre_out + (bx * N + y) * C + oc0, r[0], r[1], r[2], r[3]);
 #pragma unroll
         for (int v = 0; v < 4; ++v) r[v] = gelu(r[v]);
-        st4(lrow + y * S + o0, r[0], r[1], r[2], r[3]);
+        st4(lrow + y * S + oc0, r[0], r[1], r[2], r[3]);
       }
     }
   }
@@ -656,6 +707,7 @@ __global__ void __launch_bounds__(256) k_cell_proj(const bf16* __restrict__ P, c
     }
   }
 }
+
 
 // Block sums of NV per-thread values (every thread of the 256 passes vals);
 // thread k < NV writes the block's sum of value k to dst[k].
@@ -838,7 +890,8 @@ int launch_pixel(cgm_fno* f, hipStream_t st, const bf16* P, const bf16* A, const
 }
 
 // spectral path of one layer: A (row projections) -> Bt (per-row mixing matrices)
-int spectral(cgm_fno* f, hipStream_t st, const float2* W, float2* usave, bool bwd, int B, const int* done) {
+int spectral(cgm_fno* f, hipStream_t st, const float2* W, float2* usave, bool bwd, int B, const int* done,
+             bool invx = true) {
   const int nch = f->N / XC;
   hipLaunchKernelGGL(k_dftx, dim3(nch, KY, B), dim3(1024), 0, st, f->A, f->upart, f->Ex, f->N, bwd ? 1 : 0, done);
   if (!bwd)
@@ -847,7 +900,8 @@ int spectral(cgm_fno* f, hipStream_t st, const float2* W, float2* usave, bool bw
   else
     hipLaunchKernelGGL((k_mix<true>), dim3(KY, KX / KXG, B), dim3(256), 0, st, f->upart, nch, W, usave, f->Vm,
                        f->N, done);
-  hipLaunchKernelGGL(k_invx, dim3(f->N / XB, KY / KYG, B), dim3(512), 0, st, f->Vm, f->Bt, f->Ex, f->N, done);
+  if (invx)
+    hipLaunchKernelGGL(k_invx, dim3(f->N / XB, KY / KYG, B), dim3(512), 0, st, f->Vm, f->Bt, f->Ex, f->N, done);
   CGM_HIP(hipGetLastError());
   return CGM_SUCCESS;
 }
@@ -861,8 +915,8 @@ namespace cgm {
 int fno_forward_io(cgm_fno* f, hipStream_t st, const FnoIO& io, int B, bool train, const int* done) {
   const float s = (float)f->N;
   const int rows = B * f->N;
-  const long total = (long)B * f->n;
   const size_t lds = (size_t)f->N * S * sizeof(bf16);
+  const bool fold = B == 1;
   int rc;
   hipLaunchKernelGGL(k_lift_row, dim3(rows), dim3(512), lds, st, io.chi, io.sChi, f->chi_scale, io.v, io.sV, io.nrm,
                      io.sNrm, io.vw, s, f->params, f->h[0], f->Ft, f->A, f->N, done);
@@ -870,14 +924,19 @@ int fno_forward_io(cgm_fno* f, hipStream_t st, const FnoIO& io, int B, bool trai
   const float2* vv = io.vw ? io.vw : io.v;
   for (int l = 0; l < 4; ++l) {
     const float* Lp = f->params + layer_off(l);
-    if ((rc = spectral(f, st, (const float2*)(Lp + C * C + C), train ? f->U[l] : nullptr, false, B, done)))
+    // x-DFT and mixing; for one sample the inverse x-DFT of each row of
+    // layers 0-2 runs in its cell GEMM, otherwise k_invx forms every B_x once
+    const bool fold_l = fold && l < 3;
+    if ((rc = spectral(f, st, (const float2*)(Lp + C * C + C), train ? f->U[l] : nullptr, false, B, done,
+                       !fold_l)))
       return rc;
     const bf16* Wl = f->Wt + l * C * C;
     if (l < 3) {
-      hipLaunchKernelGGL(k_cell_row, dim3(rows), dim3(512), lds, st, f->h[l], f->Fy, Wl, f->Bt, Lp + C * C,
-                         train ? f->pre[l] : nullptr, f->h[l + 1], f->Ft, f->A, f->N, done);
+      hipLaunchKernelGGL(k_cell_row, dim3(rows), dim3(512), lds, st, f->h[l], f->Fy, Wl, fold ? nullptr : f->Bt,
+                         f->Vm, f->Ex, Lp + C * C, train ? f->pre[l] : nullptr, f->h[l + 1], f->Ft, f->A, f->N,
+                         done);
     } else {
-      const long waves = total / 64;
+      const long waves = (long)B * f->n / 64;
       CGM_CHECK(waves % 4 == 0, CGM_ERROR_UNSUPPORTED, "cell GEMM needs N %% 64 == 0");
       hipLaunchKernelGGL(k_cell_proj, dim3((unsigned)(waves / 4)), dim3(256), 0, st, f->h[3], f->Fy, Wl, f->Bt,
                          Lp + C * C, train ? f->h[4] : nullptr, f->params, vv, io.sV, io.vw ? nullptr : io.nrm,
@@ -1080,8 +1139,8 @@ extern "C" int cgm_fno_backward(cgm_fno_t f, const cgm_complex64* dz, float* gra
 // (contiguous [batch][N*N] chi, v, z): ms[0] lift with the first row projection (k_lift_row),
 // [1] row projection alone (k_rowproj, used by the backward pass), [2] x-DFT,
 // [3] mixing, [4] inverse x-DFT, [5] cell GEMM with GELU and the next row
-// projection (k_cell_row), [6] last cell GEMM with the projection
-// (k_cell_proj), [7] one whole forward pass.
+// projection (k_cell_row, forming B_x itself for batch 1), [6] last cell
+// GEMM with the projection (k_cell_proj), [7] one whole forward pass.
 extern "C" int cgm_fno_profile(cgm_fno_t f, const float* chi, const cgm_complex64* v, cgm_complex64* z, int batch,
                                int reps, float* ms) {
   CGM_CHECK(f && chi && v && z && ms && reps > 0, CGM_ERROR_INVALID_VALUE, "null argument");
@@ -1092,6 +1151,7 @@ extern "C" int cgm_fno_profile(cgm_fno_t f, const float* chi, const cgm_complex6
   const int B = batch;
   const long total = (long)B * f->n;
   const unsigned rows = (unsigned)(B * f->N);
+  const bf16* Rb = B == 1 ? nullptr : f->Bt;  // as the forward pass
   const float s = (float)f->N;
   const int nch = f->N / XC;
   const float* Lp = f->params + layer_off(0);
@@ -1122,8 +1182,8 @@ extern "C" int cgm_fno_profile(cgm_fno_t f, const float* chi, const cgm_complex6
                              nullptr);
           break;
         case 5:
-          hipLaunchKernelGGL(k_cell_row, dim3(rows), dim3(512), lds, st, f->h[0], f->Fy, f->Wt, f->Bt, Lp + C * C,
-                             nullptr, f->h[1], f->Ft, f->A, f->N, nullptr);
+          hipLaunchKernelGGL(k_cell_row, dim3(rows), dim3(512), lds, st, f->h[0], f->Fy, f->Wt, Rb, f->Vm, f->Ex,
+                             Lp + C * C, nullptr, f->h[1], f->Ft, f->A, f->N, nullptr);
           break;
         case 6:
           hipLaunchKernelGGL(k_cell_proj, dim3((unsigned)(total / 256)), dim3(256), 0, st, f->h[3], f->Fy, f->Wt,
