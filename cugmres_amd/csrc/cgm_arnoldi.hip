@@ -194,54 +194,76 @@ __global__ void __launch_bounds__(kFBS) k_arn_a(FusedArgs a) {
   }
   __syncthreads();
   // ---- inner products ----------------------------------------------------------
-  // A group of 32 lanes owns one basis vector q < j (slot q == j carries
-  // alpha = u^H u and gamma = u^H z) and sums it over the block's elements
-  // e = g, g+32, ...; one 32-lane reduction then gives the block partial.
+  // Work units (q, slice): basis vector q < j (slot q == j carries
+  // alpha = u^H u and gamma = u^H z) over 256 of the block's elements, one
+  // batch of 8 loads per lane.  The 32 lane groups take the (j+1) * S units
+  // in turn, so every group streams basis data whatever j is; each unit's
+  // 32-lane sums go to LDS and the S slices of a vector are added in a fixed
+  // order (FP64) into the block partial.
   // Partial layout (per launch): s_q / alpha at q, t_q / gamma at j+1+q.
   const int G = a.G;
   const long base = (long)r0 * N;  // the block's rows are contiguous
-  const int RN = kFBS * EPD;
   const int nrows_live = (N - r0) < R ? (N - r0) : R;
   const int RNl = nrows_live * N;   // live elements of this block
   const int grp = tid >> 5, g = tid & 31;
-  for (int c0 = 0; c0 <= j; c0 += kFBS / 32) {
-    const int q = c0 + grp;
+  constexpr int kSlice = 256;
+  __shared__ float4 lunit[65 * (kFBS * EPD / kSlice)];
+  const int S = (RNl + kSlice - 1) / kSlice;
+  const int units = (j + 1) * S;
+  for (int u0 = 0; u0 < units; u0 += kFBS / 32) {
+    const int u = u0 + grp;
     float2 as = make_float2(0.f, 0.f), at = make_float2(0.f, 0.f);
-    if (q < j) {
-      const float2* Vq = a.V + (long)q * n + base;
-      for (int e0 = g; e0 < RNl; e0 += 32 * 8) {
+    if (u < units) {
+      const int q = u / S;
+      const int e_lo = (u - q * S) * kSlice;
+      const int e_hi = (RNl < e_lo + kSlice) ? RNl : e_lo + kSlice;
+      if (q < j) {
+        const float2* Vq = a.V + (long)q * n + base;
         float2 v[8];
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-          const int e = e0 + 32 * k;
-          v[k] = e < RNl ? Vq[e] : make_float2(0.f, 0.f);
+          const int e = e_lo + g + 32 * k;
+          v[k] = e < e_hi ? Vq[e] : make_float2(0.f, 0.f);
         }
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-          const int e = e0 + 32 * k;
-          if (e < RNl) {
+          const int e = e_lo + g + 32 * k;
+          if (e < e_hi) {
             const float2 ps = cjmulf(v[k], su[e]), pt = cjmulf(v[k], sz[e]);
             as.x += ps.x; as.y += ps.y;
             at.x += pt.x; at.y += pt.y;
           }
         }
-      }
-    } else if (q == j) {
-      for (int e = g; e < RNl; e += 32) {
-        const float2 u = su[e], z = sz[e];
-        as.x += u.x * u.x + u.y * u.y;
-        const float2 pg = cjmulf(u, z);
-        at.x += pg.x; at.y += pg.y;
+      } else {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          const int e = e_lo + g + 32 * k;
+          if (e < e_hi) {
+            const float2 uv = su[e], zv = sz[e];
+            as.x += uv.x * uv.x + uv.y * uv.y;
+            const float2 pg = cjmulf(uv, zv);
+            at.x += pg.x; at.y += pg.y;
+          }
+        }
       }
     }
-    (void)RN;
     // 32-lane sums (rows 0+1 and rows 2+3 of the wave)
     const float sx = half_wave_sum_f(as.x), sy = half_wave_sum_f(as.y);
     const float tx = half_wave_sum_f(at.x), ty = half_wave_sum_f(at.y);
-    if (g == 0 && q <= j) {
-      put2(&a.part[(long)q * G + blockIdx.x], sx, sy);
-      put2(&a.part[(long)(j + 1 + q) * G + blockIdx.x], tx, ty);
+    if (g == 0 && u < units) lunit[u] = make_float4(sx, sy, tx, ty);
+  }
+  __syncthreads();
+  for (int q = tid; q <= j; q += kFBS) {
+    double sx = 0.0, sy = 0.0, tx = 0.0, ty = 0.0;
+    for (int sl = 0; sl < S; ++sl) {
+      const float4 pu = lunit[q * S + sl];
+      sx += pu.x;
+      sy += pu.y;
+      tx += pu.z;
+      ty += pu.w;
     }
+    put2(&a.part[(long)q * G + blockIdx.x], sx, sy);
+    put2(&a.part[(long)(j + 1 + q) * G + blockIdx.x], tx, ty);
   }
   if (!arrive_last(&a.st->tickets[kTicketDots1])) return;
 
