@@ -1,5 +1,6 @@
 // Fused Arnoldi steps with lagged CGS2 (delayed reorthogonalisation) for
-// power-of-two grids 64 <= N <= 2048 and restart <= 64.
+// grids N = 2^k (64 <= N <= 2048) and N = 3 * 2^k (96 <= N <= 768) with
+// restart <= 64.
 //
 // CGS2 (two classical Gram-Schmidt passes per step) replaces the reference's
 // modified Gram-Schmidt (B/CUDA_GMRES/GMRES.cuh:289-303).  Here the second
@@ -40,8 +41,9 @@
 // The cycle's last step needs only s and alpha of u_m (column m-1 of H), so
 // B(m-1) stops after u_m, pass 2 is not launched and A(m) skips pass 3: a
 // cycle of m steps costs m matvecs.
-// A block owns R whole rows (R*N = 1024*EPD elements) so the row FFTs and the
-// element-wise Krylov work run in the same kernel.
+// A block owns R whole rows (R = floor(1024*EPD / N); R*N = 1024*EPD for
+// powers of two) so the row FFTs and the element-wise Krylov work run in the
+// same kernel.
 #include <math.h>
 
 #include "cgm_fft.h"
@@ -164,7 +166,7 @@ __global__ void __launch_bounds__(kFBS) k_arn_a(FusedArgs a) {
     const int row = r0 + r;
     uu[k] = make_float2(0.f, 0.f);
     zz[k] = make_float2(0.f, 0.f);
-    if (row < N) {
+    if (r < R && row < N) {
       const long id = (long)row * N + c;
       const float2 u = a.Uj[id];
       uu[k] = u;
@@ -465,7 +467,7 @@ __global__ void __launch_bounds__(kFBS) k_arn_b(FusedArgs a) {
     const int r = e / N, c = e - r * N;
     const int row = r0 + r;
     float2 w = make_float2(0.f, 0.f);
-    if (row < N) {
+    if (r < R && row < N) {
       const long id = (long)row * N + c;
       const float2 u = a.Uj[id], z = a.Z[id];
       float2 qa = make_float2(0.f, 0.f), qb = make_float2(0.f, 0.f);
@@ -504,7 +506,7 @@ __global__ void __launch_bounds__(kFBS) k_arn_b(FusedArgs a) {
         w.y *= ch;
       }
     }
-    smem[r * LdsLen<float, M>::value + lpad<float>(c)] = w;
+    if (r < R) smem[r * LdsLen<float, M>::value + lpad<float>(c)] = w;
   }
   if (a.flex) {
     // ||u_{j+1}||: block partial, the last block sums them in a fixed order
@@ -545,10 +547,13 @@ __global__ void __launch_bounds__(kFBS) k_arn_b(FusedArgs a) {
 // ---------------------------------------------------------------------------
 bool fused_supported(int N, int restart) {
   const bool pow2 = N >= 64 && N <= 2048 && (N & (N - 1)) == 0;
-  return pow2 && restart >= 1 && restart <= 64;
+  const bool three = N == 96 || N == 192 || N == 384 || N == 768;
+  return (pow2 || three) && restart >= 1 && restart <= 64;
 }
 
-static int fused_epd(int N) { return N >= 1024 ? 2 : 1; }
+// elements per thread: two from 1024 (powers of two) or 384 (3 * 2^k) up,
+// so that a block holds at least two rows where it can
+static int fused_epd(int N) { return N >= 1024 || N == 384 || N == 768 ? 2 : 1; }
 
 int fused_blocks(int N) {
   const int R = kFBS * fused_epd(N) / N;
@@ -603,7 +608,10 @@ int launch_fused(hipStream_t st, int which, const FusedLaunch& L) {
   a.sSmall = L.sSmall;
   a.sSt = L.sSt;
   a.sHist = L.sHist;
-  const size_t lds = (size_t)a.R * lds_len(M, false) * sizeof(float2);
+  // the row FFTs, then u and z of the block staged in the same space
+  size_t lds = (size_t)a.R * lds_len(M, false) * sizeof(float2);
+  const size_t stage = (size_t)2 * kFBS * epd * sizeof(float2);
+  if (lds < stage) lds = stage;
   switch (L.N) {
 #define CGM_FUSED_CASE(NN, EPD)                                                                 \
   case NN:                                                                                      \
@@ -618,6 +626,10 @@ int launch_fused(hipStream_t st, int which, const FusedLaunch& L) {
     CGM_FUSED_CASE(512, 1)
     CGM_FUSED_CASE(1024, 2)
     CGM_FUSED_CASE(2048, 2)
+    CGM_FUSED_CASE(96, 1)
+    CGM_FUSED_CASE(192, 1)
+    CGM_FUSED_CASE(384, 2)
+    CGM_FUSED_CASE(768, 2)
 #undef CGM_FUSED_CASE
     default:
       return CGM_ERROR_UNSUPPORTED;

@@ -174,35 +174,38 @@ __device__ inline void fft_stage(cplx<T>* buf, const cplx<T>* reg, int t, bool a
   constexpr int NT = FftShape<M>::NT;
   constexpr int NB = EPT / R;        // butterflies per thread
   constexpr int MR = M / R;
-  if (!active) {  // threads outside the transform still take part in the barriers
-    __syncthreads();
-    __syncthreads();
-    return;
-  }
+  // Threads outside the transform (active = false) skip the data work but
+  // reach the same two barriers: the barriers stay outside any branch, so a
+  // wave whose lanes are partly inside the transform (rows * NT not a multiple
+  // of 64) still executes exactly two of them.
   cplx<T> v[NB][R];
+  if (active) {
 #pragma unroll
-  for (int b = 0; b < NB; ++b) {
-    const int j = t + b * NT;
+    for (int b = 0; b < NB; ++b) {
+      const int j = t + b * NT;
 #pragma unroll
-    for (int r = 0; r < R; ++r) v[b][r] = buf[lpad<T>(j + r * MR)];
+      for (int r = 0; r < R; ++r) v[b][r] = buf[lpad<T>(j + r * MR)];
+    }
   }
   __syncthreads();
+  if (active) {
 #pragma unroll
-  for (int b = 0; b < NB; ++b) {
-    const int j = t + b * NT;
-    const int k = j % NS;
-    if constexpr (NS > 1) {
+    for (int b = 0; b < NB; ++b) {
+      const int j = t + b * NT;
+      const int k = j % NS;
+      if constexpr (NS > 1) {
 #pragma unroll
-      for (int r = 1; r < R; ++r) {
-        cplx<T> w = reg[OFF + b * (R - 1) + r - 1];
-        if (SIGN > 0) w.y = -w.y;
-        v[b][r] = cmul(v[b][r], w);
+        for (int r = 1; r < R; ++r) {
+          cplx<T> w = reg[OFF + b * (R - 1) + r - 1];
+          if (SIGN > 0) w.y = -w.y;
+          v[b][r] = cmul(v[b][r], w);
+        }
       }
-    }
-    dft<T, SIGN, R>(v[b]);
-    const int base = (j / NS) * NS * R + k;
+      dft<T, SIGN, R>(v[b]);
+      const int base = (j / NS) * NS * R + k;
 #pragma unroll
-    for (int r = 0; r < R; ++r) buf[lpad<T>(base + r * NS)] = v[b][r];
+      for (int r = 0; r < R; ++r) buf[lpad<T>(base + r * NS)] = v[b][r];
+    }
   }
   __syncthreads();
 }

@@ -512,7 +512,7 @@ __global__ void __launch_bounds__(512) k_lift_row(const float* __restrict__ chi,
 
 // One block (8 waves) per grid row x of sample k (layers 0-2):
 //   1. B_x to LDS (BF16), the MFMA A operand of the spectral half of the
-//      GEMM: with R, copied from R[k][x] (k_invx's output); without, formed
+//      GEMM: copied from R[k][x] (k_invx's output), or with FOLD formed
 //      here as the inverse x-DFT of the mixed modes at this row (thread
 //      (ky, o): B_x[o][2ky] + i B_x[o][2ky+1] = sum_kxi Vm[k][kxi][ky][o]
 //      e^{+2 pi i kx x/N}), which saves k_invx's launch but reads all of
@@ -522,6 +522,7 @@ __global__ void __launch_bounds__(512) k_lift_row(const float* __restrict__ chi,
 //      pre_out when given, for training);
 //   3. one coalesced copy of the row to out and its row projection into
 //      Anext[x].
+template <bool FOLD>
 __global__ void __launch_bounds__(512) k_cell_row(const bf16* __restrict__ P, const bf16* __restrict__ Q,
                                                   const bf16* __restrict__ A, const bf16* __restrict__ R,
                                                   const float2* __restrict__ Vm, const float2* __restrict__ Ex,
@@ -550,7 +551,7 @@ __global__ void __launch_bounds__(512) k_cell_row(const bf16* __restrict__ P, co
       bqt[u] = ld8(Q + (long)y * C + 8 * g);
     }
   }
-  if (R) {
+  if (!FOLD) {
     if (tid < C * C / 8) {  // 128 threads x 8 BF16
       const int o = tid >> 2, k8 = (tid & 3) * 8;
       *reinterpret_cast<u32x4*>(&lb[o][k8]) = *reinterpret_cast<const u32x4*>(R + (bx * C + o) * C + k8);
@@ -932,9 +933,9 @@ int fno_forward_io(cgm_fno* f, hipStream_t st, const FnoIO& io, int B, bool trai
       return rc;
     const bf16* Wl = f->Wt + l * C * C;
     if (l < 3) {
-      hipLaunchKernelGGL(k_cell_row, dim3(rows), dim3(512), lds, st, f->h[l], f->Fy, Wl, fold ? nullptr : f->Bt,
-                         f->Vm, f->Ex, Lp + C * C, train ? f->pre[l] : nullptr, f->h[l + 1], f->Ft, f->A, f->N,
-                         done);
+      hipLaunchKernelGGL(fold ? k_cell_row<true> : k_cell_row<false>, dim3(rows), dim3(512), lds, st, f->h[l],
+                         f->Fy, Wl, f->Bt, f->Vm, f->Ex, Lp + C * C, train ? f->pre[l] : nullptr, f->h[l + 1], f->Ft,
+                         f->A, f->N, done);
     } else {
       const long waves = (long)B * f->n / 64;
       CGM_CHECK(waves % 4 == 0, CGM_ERROR_UNSUPPORTED, "cell GEMM needs N %% 64 == 0");
@@ -1093,7 +1094,10 @@ extern "C" int cgm_fno_create(cgm_ctx_t c, int batch, float chi_scale, cgm_fno_t
   CGM_HIP(hipFuncSetAttribute((const void*)k_rowproj<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)(lds2 / 2)));
   CGM_HIP(hipFuncSetAttribute((const void*)k_lift_row, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(lds2 / 2)));
-  CGM_HIP(hipFuncSetAttribute((const void*)k_cell_row, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(lds2 / 2)));
+  CGM_HIP(hipFuncSetAttribute((const void*)k_cell_row<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)(lds2 / 2)));
+  CGM_HIP(hipFuncSetAttribute((const void*)k_cell_row<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)(lds2 / 2)));
   *out = f;
   return CGM_SUCCESS;
 }
@@ -1151,7 +1155,6 @@ extern "C" int cgm_fno_profile(cgm_fno_t f, const float* chi, const cgm_complex6
   const int B = batch;
   const long total = (long)B * f->n;
   const unsigned rows = (unsigned)(B * f->N);
-  const bf16* Rb = B == 1 ? nullptr : f->Bt;  // as the forward pass
   const float s = (float)f->N;
   const int nch = f->N / XC;
   const float* Lp = f->params + layer_off(0);
@@ -1182,8 +1185,9 @@ extern "C" int cgm_fno_profile(cgm_fno_t f, const float* chi, const cgm_complex6
                              nullptr);
           break;
         case 5:
-          hipLaunchKernelGGL(k_cell_row, dim3(rows), dim3(512), lds, st, f->h[0], f->Fy, f->Wt, Rb, f->Vm, f->Ex,
-                             Lp + C * C, nullptr, f->h[1], f->Ft, f->A, f->N, nullptr);
+          hipLaunchKernelGGL(B == 1 ? k_cell_row<true> : k_cell_row<false>, dim3(rows), dim3(512), lds, st, f->h[0],
+                             f->Fy, f->Wt, f->Bt, f->Vm, f->Ex, Lp + C * C, nullptr, f->h[1], f->Ft, f->A, f->N,
+                             nullptr);
           break;
         case 6:
           hipLaunchKernelGGL(k_cell_proj, dim3((unsigned)(total / 256)), dim3(256), 0, st, f->h[3], f->Fy, f->Wt,

@@ -94,8 +94,10 @@ def test_batched_zero_rhs_and_non_pow2(sol):
     assert infos[1]["converged"] and infos[1]["iterations"] == 0
     assert float(torch.linalg.vector_norm(x[1])) == 0.0
     assert infos[0]["converged"] and infos[2]["converged"]
-    # N = 96 has no fused kernels: the systems run one after another
+    # N = 96 (3 * 2^5) runs the fused kernels; N = 80 has none, and its
+    # systems run one after another
     _check(sol, 96, 2, 2.25, False, 20, 20)
+    _check(sol, 80, 2, 2.25, False, 20, 20)
 
 
 def test_batched_graph_replays_match_eager_launches(sol):

@@ -324,7 +324,8 @@ def test_plain_config_full_size_invariants(sol, orc):
 
 
 @pytest.mark.parametrize("N,restart,cycles", [(64, 30, 20), (128, 20, 20), (256, 30, 20), (512, 30, 3),
-                                              (1024, 30, 3), (2048, 30, 2)])
+                                              (1024, 30, 3), (2048, 30, 2), (96, 30, 20), (192, 30, 20),
+                                              (384, 30, 3), (768, 30, 2)])
 def test_fused_and_classic_arnoldi_agree(sol, orc, N, restart, cycles):
     """Fused lagged-CGS2 steps (default) and the two-pass CGS2 kernels give
     the same iterates, iteration counts and Givens estimates (converged
