@@ -323,7 +323,7 @@ def batched_throughput(restart, cycles=3, B=16):
     import torch
     from cugmres_amd import problems, solver
     out = {"workload": f"{B} plane waves x one map, GMRES({restart}), {cycles} cycles", "sizes": {}}
-    for N in (64, 128, 256, 512):
+    for N in (64, 96, 128, 192, 256, 384, 512):
         ctx = solver.Context(N, 0)
         g = ctx.spectrum_helmholtz(problems.KH)
         eps = problems.block_eps_map(N, 8, 0.5, 0.75, seed=N)

@@ -197,11 +197,13 @@ __global__ void __launch_bounds__(kFBS) k_arn_a(FusedArgs a) {
   __syncthreads();
   // ---- inner products ----------------------------------------------------------
   // Work units (q, slice): basis vector q < j (slot q == j carries
-  // alpha = u^H u and gamma = u^H z) over 256 of the block's elements, one
-  // batch of 8 loads per lane.  The 32 lane groups take the (j+1) * S units
-  // in turn, so every group streams basis data whatever j is; each unit's
-  // 32-lane sums go to LDS and the S slices of a vector are added in a fixed
-  // order (FP64) into the block partial.
+  // alpha = u^H u and gamma = u^H z) over one slice of the block's elements
+  // (a multiple of 256: batches of 8 loads per lane).  A vector is cut into
+  // S = 32 / (j+1) slices (at least 1, at most one per 256 elements), so the
+  // 32 lane groups all stream basis data at small j and each group reduces
+  // once per vector at large j; each unit's 32-lane sums go to LDS and the S
+  // slices of a vector are added in a fixed order (FP64) into the block
+  // partial.
   // Partial layout (per launch): s_q / alpha at q, t_q / gamma at j+1+q.
   const int G = a.G;
   const long base = (long)r0 * N;  // the block's rows are contiguous
@@ -210,42 +212,43 @@ __global__ void __launch_bounds__(kFBS) k_arn_a(FusedArgs a) {
   const int grp = tid >> 5, g = tid & 31;
   constexpr int kSlice = 256;
   __shared__ float4 lunit[65 * (kFBS * EPD / kSlice)];
-  const int S = (RNl + kSlice - 1) / kSlice;
+  const int nb = (RNl + kSlice - 1) / kSlice;  // 256-element batches of the block
+  int S = (kFBS / 32) / (j + 1);
+  S = S < 1 ? 1 : (S > nb ? nb : S);
+  const int slice = ((nb + S - 1) / S) * kSlice;
   const int units = (j + 1) * S;
   for (int u0 = 0; u0 < units; u0 += kFBS / 32) {
     const int u = u0 + grp;
     float2 as = make_float2(0.f, 0.f), at = make_float2(0.f, 0.f);
     if (u < units) {
       const int q = u / S;
-      const int e_lo = (u - q * S) * kSlice;
-      const int e_hi = (RNl < e_lo + kSlice) ? RNl : e_lo + kSlice;
+      const int e_lo = (u - q * S) * slice;
+      const int e_hi = (RNl < e_lo + slice) ? RNl : e_lo + slice;
       if (q < j) {
         const float2* Vq = a.V + (long)q * n + base;
-        float2 v[8];
+        for (int e0 = e_lo + g; e0 < e_hi; e0 += kSlice) {
+          float2 v[8];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-          const int e = e_lo + g + 32 * k;
-          v[k] = e < e_hi ? Vq[e] : make_float2(0.f, 0.f);
-        }
+          for (int k = 0; k < 8; ++k) {
+            const int e = e0 + 32 * k;
+            v[k] = e < e_hi ? Vq[e] : make_float2(0.f, 0.f);
+          }
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-          const int e = e_lo + g + 32 * k;
-          if (e < e_hi) {
-            const float2 ps = cjmulf(v[k], su[e]), pt = cjmulf(v[k], sz[e]);
-            as.x += ps.x; as.y += ps.y;
-            at.x += pt.x; at.y += pt.y;
+          for (int k = 0; k < 8; ++k) {
+            const int e = e0 + 32 * k;
+            if (e < e_hi) {
+              const float2 ps = cjmulf(v[k], su[e]), pt = cjmulf(v[k], sz[e]);
+              as.x += ps.x; as.y += ps.y;
+              at.x += pt.x; at.y += pt.y;
+            }
           }
         }
       } else {
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-          const int e = e_lo + g + 32 * k;
-          if (e < e_hi) {
-            const float2 uv = su[e], zv = sz[e];
-            as.x += uv.x * uv.x + uv.y * uv.y;
-            const float2 pg = cjmulf(uv, zv);
-            at.x += pg.x; at.y += pg.y;
-          }
+        for (int e = e_lo + g; e < e_hi; e += 32) {
+          const float2 uv = su[e], zv = sz[e];
+          as.x += uv.x * uv.x + uv.y * uv.y;
+          const float2 pg = cjmulf(uv, zv);
+          at.x += pg.x; at.y += pg.y;
         }
       }
     }
