@@ -67,7 +67,8 @@ def parse():
                    help="bench: the benchmark; step: only the per-kernel profile of one Arnoldi "
                         "step (for rocprofv3 --pmc); copy: 3 copies of 256 MiB (counter calibration); "
                         "batched: lockstep batches of 16 right-hand sides vs single solves; fno-step: "
-                        "the per-kernel profile of one network forward pass (for rocprofv3 --pmc); copybw: "
+                        "the per-kernel profile of one network forward pass on --batch vectors (for rocprofv3 "
+                        "--pmc); copybw: "
                         "device-copy bandwidth at 8-256 MB; sweep: the reference's own benchmark "
                         "(its cylinder problem at N = 256..2048, Krylov dimension 3/30/49) through "
                         "cgm_Fast_GMRES_with_CUDA beside its published V100 / GTX 1080 Ti times")
@@ -124,9 +125,12 @@ def main():
     ctx = solver.Context(N, 0)
     if a.mode == "fno-step":
         from cugmres_amd import fno
-        chi, b, _ = problems.config_plain(N, seed=0)
-        net = fno.Preconditioner(ctx, params=fno.init_params(0, zero_out=False))
-        print(json.dumps(profile_fno(ctx, net, ctx.to_device(chi.reshape(-1), "f32"), ctx.to_device(b.reshape(-1)))))
+        B = a.batch
+        cb = [problems.config_plain(N, seed=s_)[:2] for s_ in range(B)]
+        net = fno.Preconditioner(ctx, batch=B, params=fno.init_params(0, zero_out=False))
+        dchi = ctx.to_device(np.stack([c.reshape(-1) for c, _ in cb]), "f32")
+        db = ctx.to_device(np.stack([b.reshape(-1) for _, b in cb]))
+        print(json.dumps(profile_fno(ctx, net, dchi, db, B)))
         return
     if a.mode == "step":
         chi, b, _ = problems.config_plain(N, seed=0)
@@ -265,7 +269,7 @@ def main():
         per_step["matvec_arnoldi_and_gaps"] = out_learned["us_per_batch_step"] - sum(per_step.values())
         src = fprof["kernels"]
         ach = src[dom]["GBps"]
-        traffic = measured_traffic(dom, N) if B == 1 else None
+        traffic = measured_traffic(dom, N, B)
         roof = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
                 "per_unit": f"{src[dom]['bytes']} B per launch ({B} vectors at N={N})",
@@ -368,14 +372,16 @@ def batched_throughput(restart, cycles=3, B=16):
     return out
 
 
-def measured_traffic(kernel, N):
-    """HBM bytes per launch of `kernel` at grid size N as measured by rocprofv3
-    (FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00, copy-calibrated) and committed in
-    profiles/hbm_traffic_n512.json; None when no measurement matches."""
+def measured_traffic(kernel, N, batch=1):
+    """HBM bytes per launch of `kernel` at grid size N and `batch` vectors per
+    launch as measured by rocprofv3 (FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00,
+    copy-calibrated) and committed in profiles/hbm_traffic_n512.json (key
+    `kernel`, or `kernel`_b<batch> for batches); None when none matches."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic_n512.json")
+    key = kernel if batch == 1 else f"{kernel}_b{batch}"
     try:
         with open(path) as f:
-            t = json.load(f).get(kernel)
+            t = json.load(f).get(key)
     except (OSError, ValueError):
         return None
     if not t or t.get("N") != N:
