@@ -132,7 +132,7 @@ __device__ inline double2 dscale(double2 a, double s) { return make_double2(a.x 
 template <int M, int EPD>
 __global__ void __launch_bounds__(kFBS) k_arn_a(FusedArgs a) {
   batch_offset(a);
-  if (a.st->done) return;
+  const int dn = a.st->done;  // tested once this block's loads are in flight
   constexpr int NT = FftShape<M>::NT;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   float2* smem = reinterpret_cast<float2*>(smem_raw);
@@ -148,10 +148,28 @@ __global__ void __launch_bounds__(kFBS) k_arn_a(FusedArgs a) {
   constexpr int ML = LdsLen<float, M>::value;
   float2* buf = smem + rl * ML;
   const bool tail = a.tail != 0;
+  // the row values of pass 3 and this thread's u values, loaded before the
+  // done test so that their latency overlaps it
+  constexpr int EPT = FftShape<M>::EPT;
+  float2 trow[EPT];
+  if (!tail && fa) {
+    const int row = r0 + rl;
+#pragma unroll
+    for (int k = 0; k < EPT; ++k)
+      trow[k] = row < N ? a.T[tpos(row, t + k * NT, N)] : make_float2(0.f, 0.f);
+  }
+  float2 upre[EPD];
+#pragma unroll
+  for (int k = 0; k < EPD; ++k) {
+    const int e = tid + k * kFBS;
+    const int r = e / N, c = e - r * N;
+    upre[k] = (r < R && r0 + r < N) ? a.Uj[(long)(r0 + r) * N + c] : make_float2(0.f, 0.f);
+  }
+  if (dn) return;
   if (!tail) {
     if (fa) {
-      const int row = r0 + rl;
-      for (int e = t; e < M; e += NT) buf[lpad<float>(e)] = row < N ? a.T[tpos(row, e, N)] : make_float2(0.f, 0.f);
+#pragma unroll
+      for (int k = 0; k < EPT; ++k) buf[lpad<float>(t + k * NT)] = trow[k];
     }
     __syncthreads();
     lds_fft<float, M, +1>(buf, tws, t, fa);
@@ -168,7 +186,7 @@ __global__ void __launch_bounds__(kFBS) k_arn_a(FusedArgs a) {
     zz[k] = make_float2(0.f, 0.f);
     if (r < R && row < N) {
       const long id = (long)row * N + c;
-      const float2 u = a.Uj[id];
+      const float2 u = upre[k];
       uu[k] = u;
       if (!tail) {
         const float2 xin = a.flex ? a.Xin[id] : u;
@@ -280,7 +298,18 @@ __global__ void __launch_bounds__(kFBS) k_arn_a(FusedArgs a) {
   __shared__ double s_nu;
   __shared__ int s_stop;
   __shared__ double2 lall[2 * LM];
+  __shared__ double2 lh1p[LM];
+  __shared__ double2 s_g;
   DevState* st = a.st;
+  // the small math's inputs, fetched together with the partial sums
+  for (int i = tid; i < j; i += kFBS) {
+    lh1p[i] = a.h1p[i];
+    if (i < j - 1) {
+      lcs[i] = a.cs[i];
+      lsn[i] = a.sn[i];
+    }
+  }
+  if (tid == 0 && j > 0) s_g = a.g[j - 1];
   finish_partials<kFBS>(a.part, G, 2 * (j + 1), lall);
   __syncthreads();
   for (int i = tid; i <= j; i += kFBS) {
@@ -319,14 +348,10 @@ __global__ void __launch_bounds__(kFBS) k_arn_a(FusedArgs a) {
     // complete column j-1 of H and rotate it
     for (int i = tid; i <= j; i += kFBS) {
       double2 hv;
-      if (i < j) hv = dadd(a.h1p[i], ls[i]);
+      if (i < j) hv = dadd(lh1p[i], ls[i]);
       else hv = make_double2(nu, 0.0);
       lcol[i] = hv;
       a.H[(long)i * a.ld + (j - 1)] = hv;
-      if (i < j - 1) {
-        lcs[i] = a.cs[i];
-        lsn[i] = a.sn[i];
-      }
     }
     __syncthreads();
     if (tid == 0) {
@@ -357,7 +382,7 @@ __global__ void __launch_bounds__(kFBS) k_arn_a(FusedArgs a) {
       lp[i] = r;
       a.cs[i] = c;
       a.sn[i] = s;
-      const double2 gi = a.g[i];
+      const double2 gi = s_g;  // g[j-1]
       const double2 gn = make_double2(-(s.x * gi.x + s.y * gi.y), -(s.x * gi.y - s.y * gi.x));
       a.g[i + 1] = gn;
       a.g[i] = make_double2(c * gi.x, c * gi.y);
@@ -447,7 +472,7 @@ __global__ void __launch_bounds__(kFBS) k_arn_a(FusedArgs a) {
 template <int M, int EPD>
 __global__ void __launch_bounds__(kFBS) k_arn_b(FusedArgs a) {
   batch_offset(a);
-  if (a.st->done) return;
+  const int dn = a.st->done;  // tested once this block's u and z loads are in flight
   constexpr int NT = FftShape<M>::NT;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   float2* smem = reinterpret_cast<float2*>(smem_raw);
@@ -464,6 +489,17 @@ __global__ void __launch_bounds__(kFBS) k_arn_b(FusedArgs a) {
   const float wsc = (float)a.scal[2].x;  // coefficient of z (plain: 1/nu) or w (flex: 1)
   const float2 ddnu = make_float2((float)sc1.x, (float)sc1.y);
   float uu2 = 0.f;
+  float2 up[EPD], zp[EPD];
+#pragma unroll
+  for (int k = 0; k < EPD; ++k) {
+    const int e = tid + k * kFBS;
+    const int r = e / N, c = e - r * N;
+    const bool live = r < R && r0 + r < N;
+    const long id = (long)(r0 + r) * N + c;
+    up[k] = live ? a.Uj[id] : make_float2(0.f, 0.f);
+    zp[k] = live ? a.Z[id] : make_float2(0.f, 0.f);
+  }
+  if (dn) return;
 #pragma unroll
   for (int k = 0; k < EPD; ++k) {
     const int e = tid + k * kFBS;
@@ -472,7 +508,7 @@ __global__ void __launch_bounds__(kFBS) k_arn_b(FusedArgs a) {
     float2 w = make_float2(0.f, 0.f);
     if (r < R && row < N) {
       const long id = (long)row * N + c;
-      const float2 u = a.Uj[id], z = a.Z[id];
+      const float2 u = up[k], z = zp[k];
       float2 qa = make_float2(0.f, 0.f), qb = make_float2(0.f, 0.f);
       int i = 0;
       for (; i + 4 <= j; i += 4) {
@@ -554,9 +590,9 @@ bool fused_supported(int N, int restart) {
   return (pow2 || three) && restart >= 1 && restart <= 64;
 }
 
-// elements per thread: two from 1024 (powers of two) or 384 (3 * 2^k) up,
-// so that a block holds at least two rows where it can
-static int fused_epd(int N) { return N >= 1024 || N == 384 || N == 768 ? 2 : 1; }
+// elements per thread: two from 1024 (powers of two) and at 384, so that a
+// block holds at least two rows where registers allow (one row at 768)
+static int fused_epd(int N) { return N >= 1024 || N == 384 ? 2 : 1; }
 
 int fused_blocks(int N) {
   const int R = kFBS * fused_epd(N) / N;
@@ -632,7 +668,7 @@ int launch_fused(hipStream_t st, int which, const FusedLaunch& L) {
     CGM_FUSED_CASE(96, 1)
     CGM_FUSED_CASE(192, 1)
     CGM_FUSED_CASE(384, 2)
-    CGM_FUSED_CASE(768, 2)
+    CGM_FUSED_CASE(768, 1)
 #undef CGM_FUSED_CASE
     default:
       return CGM_ERROR_UNSUPPORTED;
