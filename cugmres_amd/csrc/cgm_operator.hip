@@ -49,35 +49,44 @@ k_pass1(const cplx<T>* __restrict__ x, cplx<T>* xw, const T* __restrict__ chi,
     norm_ptr = boff(norm_ptr, k * bs.st);
     done = boff(done, k * bs.st);
   }
-  if (done && *done) return;
+  const int dn = done ? *done : 0;  // tested once the row loads are in flight
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T>* smem = reinterpret_cast<cplx<T>*>(smem_raw);
   constexpr int NT = FftShape<M>::NT;
+  constexpr int EPT = FftShape<M>::EPT;
   const int t = threadIdx.x % NT;
   const int rl = threadIdx.x / NT;
   const int row = blockIdx.x * rows_per_wg + rl;
   constexpr int ML = LdsLen<T, M>::value;
   cplx<T>* buf = smem + rl * ML;
+  const bool live = row < N;
+  const long rb = (long)row * N;
+  const bool mul = mode == kModeForward || mode == kModeResidual;
+  FftTwiddles<T, M> tws;
+  tws.load(tw, t);
+  cplx<T> xv[EPT];
+  T cv[EPT];
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) {
+    const int e = t + k * NT;
+    xv[k] = (live && e < N) ? x[rb + e] : mk(T(0), T(0));
+    cv[k] = (live && e < N && mul) ? chi[rb + e] : T(1);
+  }
+  if (dn) return;
   T s = T(1);
   if (norm_ptr) {
     const double nv = *norm_ptr;
     s = nv > 0.0 ? T(1.0 / nv) : T(0);
   }
-  const bool live = row < N;
-  const long rb = (long)row * N;
-  FftTwiddles<T, M> tws;
-  tws.load(tw, t);
-  for (int e = t; e < M; e += NT) {
-    cplx<T> u = mk(T(0), T(0));
-    if (live && e < N) {
-      cplx<T> v = x[rb + e];
-      if (norm_ptr) {
-        v = cscale(v, s);
-        if (xw) xw[rb + e] = v;
-      }
-      u = (mode == kModeForward || mode == kModeResidual) ? cscale(v, chi[rb + e]) : v;
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) {
+    const int e = t + k * NT;
+    cplx<T> v = xv[k];
+    if (norm_ptr) {
+      v = cscale(v, s);
+      if (xw && live && e < N) xw[rb + e] = v;
     }
-    buf[lpad<T>(e)] = u;
+    buf[lpad<T>(e)] = mul ? cscale(v, cv[k]) : v;
   }
   __syncthreads();
   lds_fft<T, M, -1>(buf, tws, t);
@@ -92,7 +101,7 @@ k_pass2(cplx<T>* __restrict__ Tm, const G* __restrict__ gq, int N, int cols_per_
         const cplx<T>* __restrict__ tw, const int* __restrict__ done, BatchStrides bs) {
   Tm += (long)blockIdx.y * bs.T;
   done = boff(done, (long)blockIdx.y * bs.st);
-  if (done && *done) return;
+  const int dn = done ? *done : 0;  // tested once the column loads are in flight
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T>* smem = reinterpret_cast<cplx<T>*>(smem_raw);
   constexpr int NT = FftShape<M>::NT;
@@ -100,13 +109,39 @@ k_pass2(cplx<T>* __restrict__ Tm, const G* __restrict__ gq, int N, int cols_per_
   constexpr int ML = LdsLen<T, M>::value;
   const int grp = xcd_tile(blockIdx.x, gridDim.x);
   const int c0 = grp * cols_per_wg;
-  const int nthr = blockDim.x;
+  const int nthr = blockDim.x;  // cols_per_wg * NT
   FftTwiddles<T, M> tws;
   tws.load(tw, threadIdx.x % NT);
-  // gather the column tile: consecutive threads read consecutive columns
-  for (int idx = threadIdx.x; idx < N * cols_per_wg; idx += nthr) {
-    const int r = idx / cols_per_wg, c = idx - r * cols_per_wg;
-    smem[c * ML + lpad<T>(r)] = Tm[tpos(r, c0 + c, N)];
+  // gather the column tile: consecutive threads read consecutive columns.
+  // Single precision: N <= M/2 + 1, so each thread gathers at most EPT/2 + 1
+  // values into registers before the done test; double precision (whose
+  // registers are spoken for) tests first and gathers straight to LDS.
+  if constexpr (sizeof(T) == 4) {
+    constexpr int GR = FftShape<M>::EPT / 2 + 1;
+    cplx<T> gv[GR];
+#pragma unroll
+    for (int k = 0; k < GR; ++k) {
+      const int idx = threadIdx.x + k * nthr;
+      if (idx < N * cols_per_wg) {
+        const int r = idx / cols_per_wg, c = idx - r * cols_per_wg;
+        gv[k] = Tm[tpos(r, c0 + c, N)];
+      }
+    }
+    if (dn) return;
+#pragma unroll
+    for (int k = 0; k < GR; ++k) {
+      const int idx = threadIdx.x + k * nthr;
+      if (idx < N * cols_per_wg) {
+        const int r = idx / cols_per_wg, c = idx - r * cols_per_wg;
+        smem[c * ML + lpad<T>(r)] = gv[k];
+      }
+    }
+  } else {
+    if (dn) return;
+    for (int idx = threadIdx.x; idx < N * cols_per_wg; idx += nthr) {
+      const int r = idx / cols_per_wg, c = idx - r * cols_per_wg;
+      smem[c * ML + lpad<T>(r)] = Tm[tpos(r, c0 + c, N)];
+    }
   }
   for (int idx = threadIdx.x; idx < (M - N) * cols_per_wg; idx += nthr) {
     const int r = N + idx / cols_per_wg, c = idx % cols_per_wg;
