@@ -267,27 +267,31 @@ __global__ void __launch_bounds__(256) k_mix(const float2* __restrict__ upart, i
   vout[slot] = v;
 }
 
-// Inverse DFT along x, XB rows and KYG column frequencies per block
-// (thread = (ch, ky within the group, row phase)):
+// Inverse DFT along x, XB rows of one sample per block (thread = (ky, ch)):
 //   out[b][x][ch][2ky], [2ky+1] = Re, Im of sum_kxi V[kxi][ky][ch] e^{+2 pi i kx x/N}   (BF16)
-constexpr int XB = 8;
-constexpr int KYG = 4;
+// The block's XB x 32 x 32 outputs are assembled in LDS and leave as 16-byte
+// stores of whole 2 KiB rows.  Every block reads its sample's 128 KiB of
+// modes: XB = 8 for batches (fewer reads), XB = 2 for one vector (enough
+// blocks).
+template <int XB>
 __global__ void __launch_bounds__(512) k_invx(const float2* __restrict__ V, bf16* __restrict__ out,
                                               const float2* __restrict__ Ex, int N, const int* done) {
   if (skip(done)) return;
   __shared__ float2 se[XB][KX];
+  __shared__ __align__(16) bf16 lt[XB][C][C];
   const int x0 = blockIdx.x * XB, b = blockIdx.z;
   const int tid = threadIdx.x;
   if (tid < XB * KX) {
     const int q = tid / XB, xl = tid % XB;
     se[xl][q] = Ex[(long)q * N + x0 + xl];
   }
-  const int ch = tid & 31, ky = blockIdx.y * KYG + ((tid >> 5) & (KYG - 1)), xs = tid >> 7;
+  const int ch = tid & 31, ky = tid >> 5;
   float2 vq[KX];
 #pragma unroll
   for (int q = 0; q < KX; ++q) vq[q] = V[(((long)b * KX + q) * KY + ky) * C + ch];
   __syncthreads();
-  for (int xl = xs; xl < XB; xl += 512 / (32 * KYG)) {
+#pragma unroll 2
+  for (int xl = 0; xl < XB; ++xl) {
     float2 a = make_float2(0.f, 0.f);
 #pragma unroll
     for (int q = 0; q < KX; ++q) {
@@ -295,11 +299,13 @@ __global__ void __launch_bounds__(512) k_invx(const float2* __restrict__ V, bf16
       a.x += vq[q].x * e.x - vq[q].y * e.y;
       a.y += vq[q].x * e.y + vq[q].y * e.x;
     }
-    struct __align__(4) P2 { bf16 v[2]; } pr;
-    pr.v[0] = (bf16)a.x;
-    pr.v[1] = (bf16)a.y;
-    *reinterpret_cast<P2*>(out + (((long)b * N + x0 + xl) * C + ch) * C + 2 * ky) = pr;
+    lt[xl][ch][2 * ky] = (bf16)a.x;
+    lt[xl][ch][2 * ky + 1] = (bf16)a.y;
   }
+  __syncthreads();
+  u32x4* dst = reinterpret_cast<u32x4*>(out + ((long)b * N + x0) * C * C);
+  const u32x4* src = reinterpret_cast<const u32x4*>(&lt[0][0][0]);
+  for (int q = tid; q < XB * C * C / 8; q += 512) dst[q] = src[q];
 }
 
 // Spectral weight gradient: G[kxi][ky][c][o] = sum_b conj(U[b][kxi][ky][c]) dV[b][kxi][ky][o].
@@ -890,6 +896,13 @@ int launch_pixel(cgm_fno* f, hipStream_t st, const bf16* P, const bf16* A, const
   return CGM_SUCCESS;
 }
 
+void launch_invx(cgm_fno* f, hipStream_t st, int B, const int* done) {
+  if (B >= 4)
+    hipLaunchKernelGGL(k_invx<8>, dim3(f->N / 8, 1, B), dim3(512), 0, st, f->Vm, f->Bt, f->Ex, f->N, done);
+  else
+    hipLaunchKernelGGL(k_invx<2>, dim3(f->N / 2, 1, B), dim3(512), 0, st, f->Vm, f->Bt, f->Ex, f->N, done);
+}
+
 // spectral path of one layer: A (row projections) -> Bt (per-row mixing matrices)
 int spectral(cgm_fno* f, hipStream_t st, const float2* W, float2* usave, bool bwd, int B, const int* done,
              bool invx = true) {
@@ -902,7 +915,7 @@ int spectral(cgm_fno* f, hipStream_t st, const float2* W, float2* usave, bool bw
     hipLaunchKernelGGL((k_mix<true>), dim3(KY, KX / KXG, B), dim3(256), 0, st, f->upart, nch, W, usave, f->Vm,
                        f->N, done);
   if (invx)
-    hipLaunchKernelGGL(k_invx, dim3(f->N / XB, KY / KYG, B), dim3(512), 0, st, f->Vm, f->Bt, f->Ex, f->N, done);
+    launch_invx(f, st, B, done);
   CGM_HIP(hipGetLastError());
   return CGM_SUCCESS;
 }
@@ -1181,8 +1194,7 @@ extern "C" int cgm_fno_profile(cgm_fno_t f, const float* chi, const cgm_complex6
                              f->Vm, f->N, nullptr);
           break;
         case 4:
-          hipLaunchKernelGGL(k_invx, dim3(f->N / XB, KY / KYG, B), dim3(512), 0, st, f->Vm, f->Bt, f->Ex, f->N,
-                             nullptr);
+          launch_invx(f, st, B, nullptr);
           break;
         case 5:
           hipLaunchKernelGGL(B == 1 ? k_cell_row<true> : k_cell_row<false>, dim3(rows), dim3(512), lds, st, f->h[0],
