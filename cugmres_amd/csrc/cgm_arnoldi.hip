@@ -129,8 +129,8 @@ __device__ inline double2 dsub(double2 a, double2 b) { return make_double2(a.x -
 __device__ inline double2 dadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ inline double2 dscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
 
-template <int M, int EPD>
-__global__ void __launch_bounds__(kFBS) k_arn_a(FusedArgs a) {
+template <int M, int EPD, int BS>
+__global__ void __launch_bounds__(BS) k_arn_a(FusedArgs a) {
   batch_offset(a);
   const int dn = a.st->done;  // tested once this block's loads are in flight
   constexpr int NT = FftShape<M>::NT;
@@ -161,7 +161,7 @@ __global__ void __launch_bounds__(kFBS) k_arn_a(FusedArgs a) {
   float2 upre[EPD];
 #pragma unroll
   for (int k = 0; k < EPD; ++k) {
-    const int e = tid + k * kFBS;
+    const int e = tid + k * BS;
     const int r = e / N, c = e - r * N;
     upre[k] = (r < R && r0 + r < N) ? a.Uj[(long)(r0 + r) * N + c] : make_float2(0.f, 0.f);
   }
@@ -179,7 +179,7 @@ __global__ void __launch_bounds__(kFBS) k_arn_a(FusedArgs a) {
   float2 uu[EPD], zz[EPD];
 #pragma unroll
   for (int k = 0; k < EPD; ++k) {
-    const int e = tid + k * kFBS;
+    const int e = tid + k * BS;
     const int r = e / N, c = e - r * N;
     const int row = r0 + r;
     uu[k] = make_float2(0.f, 0.f);
@@ -206,11 +206,11 @@ __global__ void __launch_bounds__(kFBS) k_arn_a(FusedArgs a) {
   }
   __syncthreads();
   float2* su = smem;                 // R*N values of u
-  float2* sz = smem + kFBS * EPD;    // R*N values of z
+  float2* sz = smem + BS * EPD;    // R*N values of z
 #pragma unroll
   for (int k = 0; k < EPD; ++k) {
-    su[tid + k * kFBS] = uu[k];
-    sz[tid + k * kFBS] = zz[k];
+    su[tid + k * BS] = uu[k];
+    sz[tid + k * BS] = zz[k];
   }
   __syncthreads();
   // ---- inner products ----------------------------------------------------------
@@ -229,13 +229,13 @@ __global__ void __launch_bounds__(kFBS) k_arn_a(FusedArgs a) {
   const int RNl = nrows_live * N;   // live elements of this block
   const int grp = tid >> 5, g = tid & 31;
   constexpr int kSlice = 256;
-  __shared__ float4 lunit[65 * (kFBS * EPD / kSlice)];
+  __shared__ float4 lunit[65 * (BS * EPD / kSlice)];
   const int nb = (RNl + kSlice - 1) / kSlice;  // 256-element batches of the block
-  int S = (kFBS / 32) / (j + 1);
+  int S = (BS / 32) / (j + 1);
   S = S < 1 ? 1 : (S > nb ? nb : S);
   const int slice = ((nb + S - 1) / S) * kSlice;
   const int units = (j + 1) * S;
-  for (int u0 = 0; u0 < units; u0 += kFBS / 32) {
+  for (int u0 = 0; u0 < units; u0 += BS / 32) {
     const int u = u0 + grp;
     float2 as = make_float2(0.f, 0.f), at = make_float2(0.f, 0.f);
     if (u < units) {
@@ -244,15 +244,15 @@ __global__ void __launch_bounds__(kFBS) k_arn_a(FusedArgs a) {
       const int e_hi = (RNl < e_lo + slice) ? RNl : e_lo + slice;
       if (q < j) {
         const float2* Vq = a.V + (long)q * n + base;
-        for (int e0 = e_lo + g; e0 < e_hi; e0 += kSlice) {
-          float2 v[8];
+        for (int e0 = e_lo + g; e0 < e_hi; e0 += 2 * kSlice) {
+          float2 v[16];
 #pragma unroll
-          for (int k = 0; k < 8; ++k) {
+          for (int k = 0; k < 16; ++k) {
             const int e = e0 + 32 * k;
             v[k] = e < e_hi ? Vq[e] : make_float2(0.f, 0.f);
           }
 #pragma unroll
-          for (int k = 0; k < 8; ++k) {
+          for (int k = 0; k < 16; ++k) {
             const int e = e0 + 32 * k;
             if (e < e_hi) {
               const float2 ps = cjmulf(v[k], su[e]), pt = cjmulf(v[k], sz[e]);
@@ -276,7 +276,7 @@ __global__ void __launch_bounds__(kFBS) k_arn_a(FusedArgs a) {
     if (g == 0 && u < units) lunit[u] = make_float4(sx, sy, tx, ty);
   }
   __syncthreads();
-  for (int q = tid; q <= j; q += kFBS) {
+  for (int q = tid; q <= j; q += BS) {
     double sx = 0.0, sy = 0.0, tx = 0.0, ty = 0.0;
     for (int sl = 0; sl < S; ++sl) {
       const float4 pu = lunit[q * S + sl];
@@ -302,7 +302,7 @@ __global__ void __launch_bounds__(kFBS) k_arn_a(FusedArgs a) {
   __shared__ double2 s_g;
   DevState* st = a.st;
   // the small math's inputs, fetched together with the partial sums
-  for (int i = tid; i < j; i += kFBS) {
+  for (int i = tid; i < j; i += BS) {
     lh1p[i] = a.h1p[i];
     if (i < j - 1) {
       lcs[i] = a.cs[i];
@@ -310,9 +310,9 @@ __global__ void __launch_bounds__(kFBS) k_arn_a(FusedArgs a) {
     }
   }
   if (tid == 0 && j > 0) s_g = a.g[j - 1];
-  finish_partials<kFBS>(a.part, G, 2 * (j + 1), lall);
+  finish_partials<BS>(a.part, G, 2 * (j + 1), lall);
   __syncthreads();
-  for (int i = tid; i <= j; i += kFBS) {
+  for (int i = tid; i <= j; i += BS) {
     if (i < j) {
       ls[i] = lall[i];
       lt[i] = lall[j + 1 + i];
@@ -346,7 +346,7 @@ __global__ void __launch_bounds__(kFBS) k_arn_a(FusedArgs a) {
   const double nu = s_nu;
   if (j > 0 && !s_stop) {
     // complete column j-1 of H and rotate it
-    for (int i = tid; i <= j; i += kFBS) {
+    for (int i = tid; i <= j; i += BS) {
       double2 hv;
       if (i < j) hv = dadd(lh1p[i], ls[i]);
       else hv = make_double2(nu, 0.0);
@@ -395,7 +395,7 @@ __global__ void __launch_bounds__(kFBS) k_arn_a(FusedArgs a) {
       if (!isfinite(est) || est <= st->tol_abs || !(nu > 0.0) || j == a.m) s_stop = 1;
     }
     __syncthreads();
-    for (int i = tid; i < j; i += kFBS) a.Rm[(long)i * a.ld + (j - 1)] = lp[i];
+    for (int i = tid; i < j; i += BS) a.Rm[(long)i * a.ld + (j - 1)] = lp[i];
   }
   __syncthreads();
   if (s_stop || j == a.m) {
@@ -420,7 +420,7 @@ __global__ void __launch_bounds__(kFBS) k_arn_a(FusedArgs a) {
     }
     __syncthreads();
     const double2 dnu = dscale(s_hj, 1.0 / nu);
-    for (int i = tid; i < j; i += kFBS) {
+    for (int i = tid; i < j; i += BS) {
       a.h1p[i] = lt[i];
       a.a1[i] = ls[i];
       a.a2[i] = dsub(lt[i], dmul(ls[i], dnu));
@@ -431,7 +431,7 @@ __global__ void __launch_bounds__(kFBS) k_arn_a(FusedArgs a) {
   // p = H[0:j+1, 0:j] s: one wave per row, lane c multiplies column c
   {
     const int wave = tid >> 6, lane = tid & 63;
-    for (int r = wave; r <= j; r += kFBS / 64) {
+    for (int r = wave; r <= j; r += BS / 64) {
       double2 v = make_double2(0.0, 0.0);
       if (lane < j && lane >= r - 1) {
         const double2 h = (lane == j - 1) ? lcol[r] : a.H[(long)r * a.ld + lane];
@@ -459,7 +459,7 @@ __global__ void __launch_bounds__(kFBS) k_arn_a(FusedArgs a) {
   }
   __syncthreads();
   const double2 ddnu = dscale(s_dd, 1.0 / nu);
-  for (int i = tid; i < j; i += kFBS) {
+  for (int i = tid; i < j; i += BS) {
     const double2 hi = dscale(dsub(lt[i], lp[i]), 1.0 / nu);
     a.h1p[i] = hi;
     const double2 ci = dadd(dscale(lp[i], 1.0 / nu), hi);
@@ -469,8 +469,8 @@ __global__ void __launch_bounds__(kFBS) k_arn_a(FusedArgs a) {
   if (tid == 0) reset_ticket(&st->tickets[kTicketDots1]);
 }
 
-template <int M, int EPD>
-__global__ void __launch_bounds__(kFBS) k_arn_b(FusedArgs a) {
+template <int M, int EPD, int BS>
+__global__ void __launch_bounds__(BS) k_arn_b(FusedArgs a) {
   batch_offset(a);
   const int dn = a.st->done;  // tested once this block's u and z loads are in flight
   constexpr int NT = FftShape<M>::NT;
@@ -492,7 +492,7 @@ __global__ void __launch_bounds__(kFBS) k_arn_b(FusedArgs a) {
   float2 up[EPD], zp[EPD];
 #pragma unroll
   for (int k = 0; k < EPD; ++k) {
-    const int e = tid + k * kFBS;
+    const int e = tid + k * BS;
     const int r = e / N, c = e - r * N;
     const bool live = r < R && r0 + r < N;
     const long id = (long)(r0 + r) * N + c;
@@ -500,42 +500,65 @@ __global__ void __launch_bounds__(kFBS) k_arn_b(FusedArgs a) {
     zp[k] = live ? a.Z[id] : make_float2(0.f, 0.f);
   }
   if (dn) return;
+  // Q_j a1 and Q_j a2 for the thread's EPD elements, QC basis vectors at a
+  // time for all elements (QC * EPD loads in flight)
+  constexpr int QC = EPD == 1 ? 4 : 2;
+  float2 qa[EPD], qb[EPD];
+  long ids[EPD];
+  bool lv[EPD];
 #pragma unroll
   for (int k = 0; k < EPD; ++k) {
-    const int e = tid + k * kFBS;
+    const int e = tid + k * BS;
     const int r = e / N, c = e - r * N;
-    const int row = r0 + r;
-    float2 w = make_float2(0.f, 0.f);
-    if (r < R && row < N) {
-      const long id = (long)row * N + c;
-      const float2 u = up[k], z = zp[k];
-      float2 qa = make_float2(0.f, 0.f), qb = make_float2(0.f, 0.f);
-      int i = 0;
-      for (; i + 4 <= j; i += 4) {
-        float2 v[4];
+    lv[k] = r < R && r0 + r < N;
+    ids[k] = lv[k] ? (long)(r0 + r) * N + c : 0;
+    qa[k] = make_float2(0.f, 0.f);
+    qb[k] = make_float2(0.f, 0.f);
+  }
+  {
+    int i = 0;
+    for (; i + QC <= j; i += QC) {
+      float2 v[QC][EPD];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) v[q] = a.V[(long)(i + q) * n + id];
+      for (int q = 0; q < QC; ++q)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const double2 c1 = a.a1[i + q], c2 = a.a2[i + q];
-          const float2 f1 = make_float2((float)c1.x, (float)c1.y), f2 = make_float2((float)c2.x, (float)c2.y);
-          const float2 p1 = cmulf(f1, v[q]), p2 = cmulf(f2, v[q]);
-          qa.x += p1.x; qa.y += p1.y;
-          qb.x += p2.x; qb.y += p2.y;
+        for (int k = 0; k < EPD; ++k) v[q][k] = lv[k] ? a.V[(long)(i + q) * n + ids[k]] : make_float2(0.f, 0.f);
+#pragma unroll
+      for (int q = 0; q < QC; ++q) {
+        const double2 c1 = a.a1[i + q], c2 = a.a2[i + q];
+        const float2 f1 = make_float2((float)c1.x, (float)c1.y), f2 = make_float2((float)c2.x, (float)c2.y);
+#pragma unroll
+        for (int k = 0; k < EPD; ++k) {
+          const float2 p1 = cmulf(f1, v[q][k]), p2 = cmulf(f2, v[q][k]);
+          qa[k].x += p1.x; qa[k].y += p1.y;
+          qb[k].x += p2.x; qb[k].y += p2.y;
         }
       }
-      for (; i < j; ++i) {
-        const float2 v = a.V[(long)i * n + id];
-        const double2 c1 = a.a1[i], c2 = a.a2[i];
-        const float2 p1 = cmulf(make_float2((float)c1.x, (float)c1.y), v);
-        const float2 p2 = cmulf(make_float2((float)c2.x, (float)c2.y), v);
-        qa.x += p1.x; qa.y += p1.y;
-        qb.x += p2.x; qb.y += p2.y;
+    }
+    for (; i < j; ++i) {
+      const double2 c1 = a.a1[i], c2 = a.a2[i];
+      const float2 f1 = make_float2((float)c1.x, (float)c1.y), f2 = make_float2((float)c2.x, (float)c2.y);
+#pragma unroll
+      for (int k = 0; k < EPD; ++k) {
+        const float2 v = lv[k] ? a.V[(long)i * n + ids[k]] : make_float2(0.f, 0.f);
+        const float2 p1 = cmulf(f1, v), p2 = cmulf(f2, v);
+        qa[k].x += p1.x; qa[k].y += p1.y;
+        qb[k].x += p2.x; qb[k].y += p2.y;
       }
-      const float2 q = make_float2((u.x - qa.x) * invnu, (u.y - qa.y) * invnu);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < EPD; ++k) {
+    const int e = tid + k * BS;
+    const int r = e / N, c = e - r * N;
+    float2 w = make_float2(0.f, 0.f);
+    if (lv[k]) {
+      const long id = ids[k];
+      const float2 u = up[k], z = zp[k];
+      const float2 q = make_float2((u.x - qa[k].x) * invnu, (u.y - qa[k].y) * invnu);
       a.V[(long)j * n + id] = q;
       const float2 du = cmulf(ddnu, u);
-      const float2 un = make_float2(z.x * wsc - du.x - qb.x, z.y * wsc - du.y - qb.y);
+      const float2 un = make_float2(z.x * wsc - du.x - qb[k].x, z.y * wsc - du.y - qb[k].y);
       a.Unext[id] = un;
       uu2 += un.x * un.x + un.y * un.y;
       w = un;
@@ -550,13 +573,13 @@ __global__ void __launch_bounds__(kFBS) k_arn_b(FusedArgs a) {
   if (a.flex) {
     // ||u_{j+1}||: block partial, the last block sums them in a fixed order
     if (a.tail) return;  // no network application follows u_m
-    __shared__ double red[kFBS / 64];
+    __shared__ double red[BS / 64];
     const double ws_ = wave_sum((double)uu2);
     if ((tid & 63) == 0) red[tid >> 6] = ws_;
     __syncthreads();
     if (tid == 0) {
       double sb = 0.0;
-      for (int k = 0; k < kFBS / 64; ++k) sb += red[k];
+      for (int k = 0; k < BS / 64; ++k) sb += red[k];
       put(&a.partn[blockIdx.x], sb);
     }
     if (!arrive_last(&a.st->tickets[kTicketNorm])) return;
@@ -568,7 +591,7 @@ __global__ void __launch_bounds__(kFBS) k_arn_b(FusedArgs a) {
     return;
   }
   if (a.tail) return;  // u_m needs no matvec (block-uniform)
-  for (int e = tid; e < R * (M - N); e += kFBS) {
+  for (int e = tid; e < R * (M - N); e += BS) {
     const int r = e / (M - N), c = N + e % (M - N);
     smem[r * LdsLen<float, M>::value + lpad<float>(c)] = make_float2(0.f, 0.f);
   }
@@ -590,12 +613,25 @@ bool fused_supported(int N, int restart) {
   return (pow2 || three) && restart >= 1 && restart <= 64;
 }
 
-// elements per thread: two from 1024 (powers of two) and at 384, so that a
-// block holds at least two rows where registers allow (one row at 768)
-static int fused_epd(int N) { return N >= 1024 || N == 384 ? 2 : 1; }
+// Work-group shapes.  Large: 1,024 threads, one element per thread (two from
+// N = 1024 and at N = 384), so a block holds at least two rows where
+// registers allow.  Small (N in {64, 96, 128, 192, 256, 512}): 256 threads
+// with four elements each; the same rows per block, so every wave of the
+// block takes part in the row transforms and four blocks share a CU.
+struct FusedShape {
+  int epd, bs;
+};
+static bool fused_small_ok(int N) {
+  return N == 64 || N == 96 || N == 128 || N == 192 || N == 256 || N == 512;
+}
+static FusedShape fused_shape(int N, bool small) {
+  if (small && fused_small_ok(N)) return {4, 256};
+  return {N >= 1024 || N == 384 ? 2 : 1, kFBS};
+}
 
 int fused_blocks(int N) {
-  const int R = kFBS * fused_epd(N) / N;
+  const FusedShape f = fused_shape(N, false);  // both shapes give the same rows per block
+  const int R = f.bs * f.epd / N;
   return (N + R - 1) / R;
 }
 
@@ -604,8 +640,9 @@ int launch_fused(hipStream_t st, int which, const FusedLaunch& L) {
   FusedArgs a;
   a.N = L.N;
   const int M = 2 * L.N;
-  const int epd = fused_epd(L.N);
-  a.R = kFBS * epd / L.N;
+  const FusedShape fs = fused_shape(L.N, L.small != 0);
+  const int epd = fs.epd;
+  a.R = fs.bs * epd / L.N;
   a.m = L.m;
   a.ld = L.ld;
   a.j = L.j;
@@ -649,27 +686,38 @@ int launch_fused(hipStream_t st, int which, const FusedLaunch& L) {
   a.sHist = L.sHist;
   // the row FFTs, then u and z of the block staged in the same space
   size_t lds = (size_t)a.R * lds_len(M, false) * sizeof(float2);
-  const size_t stage = (size_t)2 * kFBS * epd * sizeof(float2);
+  const size_t stage = (size_t)2 * fs.bs * epd * sizeof(float2);
   if (lds < stage) lds = stage;
-  switch (L.N) {
+#define CGM_FUSED_LAUNCH(NN, EPD, BSZ)                                                          \
+  {                                                                                           \
+    if (which == 0)                                                                           \
+      hipLaunchKernelGGL((k_arn_a<2 * NN, EPD, BSZ>), dim3(a.G, L.B), dim3(BSZ), lds, st, a); \
+    else                                                                                      \
+      hipLaunchKernelGGL((k_arn_b<2 * NN, EPD, BSZ>), dim3(a.G, L.B), dim3(BSZ), lds, st, a); \
+  }
 #define CGM_FUSED_CASE(NN, EPD)                                                                 \
-  case NN:                                                                                      \
-    if (which == 0)                                                                             \
-      hipLaunchKernelGGL((k_arn_a<2 * NN, EPD>), dim3(a.G, L.B), dim3(kFBS), lds, st, a);       \
-    else                                                                                        \
-      hipLaunchKernelGGL((k_arn_b<2 * NN, EPD>), dim3(a.G, L.B), dim3(kFBS), lds, st, a);       \
+  case NN:                                                                                    \
+    CGM_FUSED_LAUNCH(NN, EPD, kFBS)                                                           \
     break;
-    CGM_FUSED_CASE(64, 1)
-    CGM_FUSED_CASE(128, 1)
-    CGM_FUSED_CASE(256, 1)
-    CGM_FUSED_CASE(512, 1)
+#define CGM_FUSED_CASE2(NN, EPD)                                                                \
+  case NN:                                                                                    \
+    if (fs.bs == 256) CGM_FUSED_LAUNCH(NN, 4, 256)                                            \
+    else CGM_FUSED_LAUNCH(NN, EPD, kFBS)                                                      \
+    break;
+  switch (L.N) {
+    CGM_FUSED_CASE2(64, 1)
+    CGM_FUSED_CASE2(128, 1)
+    CGM_FUSED_CASE2(256, 1)
+    CGM_FUSED_CASE2(512, 1)
     CGM_FUSED_CASE(1024, 2)
     CGM_FUSED_CASE(2048, 2)
-    CGM_FUSED_CASE(96, 1)
-    CGM_FUSED_CASE(192, 1)
+    CGM_FUSED_CASE2(96, 1)
+    CGM_FUSED_CASE2(192, 1)
     CGM_FUSED_CASE(384, 2)
     CGM_FUSED_CASE(768, 1)
 #undef CGM_FUSED_CASE
+#undef CGM_FUSED_CASE2
+#undef CGM_FUSED_LAUNCH
     default:
       return CGM_ERROR_UNSUPPORTED;
   }

@@ -119,6 +119,7 @@ struct KrylovWs {
 struct FusedLaunch {
   int N, m, ld, j, mode;  // m: steps of this cycle; ld: leading dimension of H and R
   int flex = 0;           // flexible (right-preconditioned) steps: the matvec input is Xin = z_j
+  int small = 0;          // 256-thread work-groups where available (see fused_shape)
   const float2* Xin = nullptr;
   long sXin = 0;            // per-system stride of Xin (elements)
   double* partn = nullptr;  // norm partials (flex)

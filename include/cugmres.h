@@ -94,10 +94,16 @@ int cgm_autotune(cgm_ctx_t ctx);
 /* tiles[0..2] = rows of pass 1, columns of pass 2, rows of pass 3; us[0..2]
  * (optional) = their tuned times in microseconds. */
 int cgm_get_tiles(cgm_ctx_t ctx, int* tiles, float* us);
-/* Arnoldi kernels of cgm_gmres: 0 (default) = fused steps with lagged CGS2
- * (one global reduction and two basis passes per step; power-of-two N in
- * [64, 1024], restart <= 64, other cases fall back to 1); 1 = separate
- * kernels with two-pass CGS2 (three basis passes per step). */
+/* Arnoldi kernels of the solvers: 0 (default) = fused steps with lagged CGS2
+ * (one global reduction and two basis passes per step; N = 2^k in [64, 2048]
+ * or 3 * 2^k in [96, 768], restart <= 64, other cases fall back to 1), with
+ * 256-thread work-groups once a launch covers batch * N >= 2048 grid rows and
+ * 1,024-thread ones below (the two shapes sum in different orders, so a
+ * batched system and its single solve agree to rounding, bitwise under path
+ * 2 or 3);
+ * 1 = separate kernels with two-pass CGS2 (three basis passes per step);
+ * 2 / 3 = fused steps with 256- / 1,024-thread work-groups always (the
+ * 256-thread shape exists for N in {64, 96, 128, 192, 256, 512}). */
 int cgm_set_arnoldi_path(cgm_ctx_t ctx, int path);
 /* Number of complex entries of a kernel spectrum: (M/2+1)^2.  The spectrum of
  * an even circulant embedding is even in both frequencies, so only this

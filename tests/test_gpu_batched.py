@@ -1,9 +1,10 @@
 """Batched lockstep GMRES (cgm_gmres_batched) against single solves.
 
-The batched kernels run the same per-system arithmetic as a single solve (same
-blocks, same reduction order), so every system must reproduce its single
-solve: equal iteration counts, solutions equal to float rounding (checked at
-relative L2 1e-6) and histories equal to 1e-6 of ||r0||.
+With one work-group shape for both (cgm_set_arnoldi_path 3: 1,024 threads,
+or 2: 256 threads) the batched kernels run the same per-system arithmetic as
+a single solve (same blocks, same reduction order), so every system must
+reproduce its single solve: equal iteration counts, solutions equal to float
+rounding (checked at relative L2 1e-6) and histories equal to 1e-6 of ||r0||.
 """
 import math
 
@@ -35,10 +36,11 @@ def _systems(N, B, eps_hi, seed0, shared=False):
     return np.stack(chis), np.stack(bs)
 
 
-def _check(sol, N, B, eps_hi, shared, restart, max_cycles, tol=1e-6):
+def _check(sol, N, B, eps_hi, shared, restart, max_cycles, tol=1e-6, path=3):
     import torch
     from cugmres_amd import problems
     ctx = sol.Context(N)
+    ctx.set_arnoldi_path(path)
     g = ctx.spectrum_helmholtz(problems.KH)
     chi, b = _systems(N, B, eps_hi, 7, shared)
     chi_d = torch.from_numpy(chi.reshape(B, -1).astype(np.float32)).cuda()
@@ -61,8 +63,9 @@ def _check(sol, N, B, eps_hi, shared, restart, max_cycles, tol=1e-6):
     return infos
 
 
-def test_batched_own_maps_match_single(sol):
-    infos = _check(sol, 64, 5, 2.25, False, 20, 20)
+@pytest.mark.parametrize("path", [3, 2])
+def test_batched_own_maps_match_single(sol, path):
+    infos = _check(sol, 64, 5, 2.25, False, 20, 20, path=path)
     assert all(i["converged"] for i in infos)
     assert len({i["iterations"] for i in infos}) > 1   # systems stop at different steps
 
@@ -152,14 +155,16 @@ def test_batched_matvec_equals_single(sol):
                 assert torch.equal(ys[k], y0), (N, mode, k)
 
 
-@pytest.mark.parametrize("N,B,shared,with_net", [(64, 3, False, True), (128, 4, True, True), (64, 2, False, False)])
-def test_fgmres_batched_equals_single_solves(sol, N, B, shared, with_net):
+@pytest.mark.parametrize("N,B,shared,with_net,path", [(64, 3, False, True, 3), (128, 4, True, True, 2),
+                                                     (64, 2, False, False, 3), (512, 4, False, True, 2)])
+def test_fgmres_batched_equals_single_solves(sol, N, B, shared, with_net, path):
     """Lockstep flexible GMRES: every system reproduces its own cgm_fgmres
     solve (same network on each vector, same per-system arithmetic)."""
     import torch
     from cugmres_amd import fno, problems
     chis, bs = _systems(N, B, 2.25, 40, shared=shared)
     ctx = sol.Context(N)
+    ctx.set_arnoldi_path(path)
     g = ctx.spectrum_helmholtz(problems.KH)
     chi_d = ctx.to_device(chis[0], "f32") if shared else ctx.to_device(chis, "f32").reshape(B, N * N)
     b_d = ctx.to_device(bs, "c64").reshape(B, N * N)
