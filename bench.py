@@ -402,16 +402,18 @@ def profile_fno(ctx, net, dchi, db, batch=1):
     db = db.reshape(batch, n)
     v = (db / torch.linalg.vector_norm(db, dim=1, keepdim=True)).contiguous()
     z = torch.empty_like(v)
-    ms = (ctypes.c_float * 8)()
+    ms = (ctypes.c_float * 9)()
     st = lib().cgm_fno_profile(net.handle, ctypes.c_void_p(dchi.data_ptr()), ctypes.c_void_p(v.data_ptr()),
                                ctypes.c_void_p(z.data_ptr()), batch, 50, ms)
     _lib.check(st, "cgm_fno_profile")
     names = ["fno_lift", "fno_rowproj", "fno_dftx", "fno_mix", "fno_invx", "fno_cell_gemm", "fno_proj",
-             "fno_forward_total"]
+             "fno_forward_total", "fno_cell_gemm_lift"]
     # launches per forward pass: fno_lift = k_lift_row (lift + layer-0 row
-    # projection); fno_cell_gemm = k_cell_row (B_x of the row, cell GEMM +
-    # GELU + next row projection, layers 0-2); fno_proj = k_cell_proj (layer-3
-    # cell GEMM + the 32 -> 2 projection).  For one vector the row kernels of
+    # projection; h0 stays in LDS); fno_cell_gemm = k_cell_row (B_x of the
+    # row, cell GEMM + GELU + next row projection, layers 1-2);
+    # fno_cell_gemm_lift = the same for layer 0, lifting its row from chi and
+    # v; fno_proj = k_cell_proj (layer-3 cell GEMM + the 32 -> 2 projection).
+    # For one vector the row kernels of
     # layers 0-2 form B_x themselves (inverse x-DFT from the mixed modes) and
     # k_invx runs for layer 3 only; for more, k_invx forms every B_x.  The
     # stand-alone row projection runs only in the backward pass.  Algorithmic
@@ -423,13 +425,15 @@ def profile_fno(ctx, net, dchi, db, batch=1):
     # (per vector; the spectral weights once per launch)
     part = (N // 32) * 16384 * 8
     Bv = batch
-    bytes_ = {"fno_lift": Bv * (n * (4 + 8 + 64) + N * 4096), "fno_rowproj": Bv * (n * 64 + N * 4096),
+    bx = 128 * 1024 if batch == 1 else N * 2048   # B_x source per vector: modes (folded) or k_invx's rows
+    bytes_ = {"fno_lift": Bv * (n * (4 + 8) + N * 4096), "fno_rowproj": Bv * (n * 64 + N * 4096),
               "fno_dftx": Bv * (N * 4096 + part) + N * 256, "fno_mix": Bv * (part + 16384 * 8) + 4 * 1024 * 1024,
               "fno_invx": Bv * (16384 * 8 + N * 2048) + N * 256,
-              "fno_cell_gemm": Bv * (n * 128 + N * 4096 + (128 * 1024 if batch == 1 else N * 2048)) + N * 256,
+              "fno_cell_gemm": Bv * (n * 128 + N * 4096 + bx) + N * 256,
+              "fno_cell_gemm_lift": Bv * (n * (4 + 8 + 64) + N * 4096 + bx) + N * 256,
               "fno_proj": Bv * (n * (64 + 16) + N * 2048)}
     per_step = {"fno_lift": 1, "fno_rowproj": 0, "fno_dftx": 4, "fno_mix": 4, "fno_invx": 1 if batch == 1 else 4,
-                "fno_cell_gemm": 3, "fno_proj": 1}
+                "fno_cell_gemm": 2, "fno_cell_gemm_lift": 1, "fno_proj": 1}
     kernels = {}
     for i, k in enumerate(names):
         us = ms[i] * 1e3

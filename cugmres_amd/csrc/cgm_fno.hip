@@ -34,7 +34,9 @@
 // the kernel that produces it (k_lift_row, k_cell_row), for one sample the
 // inverse x-DFT of each row of layers 0-2 runs in the cell GEMM of that row,
 // and the last cell GEMM carries the 32 -> 2 projection (k_cell_proj):
-// 1 + 3 x 3 + 4 launches for one sample, 1 + 4 x 4 for more.  The
+// 1 + 3 x 3 + 4 launches for one sample, 1 + 4 x 4 for more.  h0 is a
+// pointwise function of (chi, v): outside training it never reaches global
+// memory, and layer 0's cell GEMM lifts its row again.  The
 // backward pass has the separate kernels (k_rowproj, k_pixel) with the roles
 // transposed.
 //
@@ -59,7 +61,7 @@ typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 constexpr int C = 32;   // channels
 constexpr int KY = 16;  // retained column frequencies
 constexpr int KX = 32;  // retained row frequencies
-constexpr int XC = 32;  // rows per partial of k_dftx
+constexpr int XC = 32;  // fewest rows per partial of k_dftx (sizes the partials)
 constexpr int S = 36;   // LDS row stride (bf16) of k_rowproj: 16-bank offset between lane groups
 constexpr int kLayerParams = C * C + C + KX * KY * C * C * 2;
 constexpr long kOffLayer0 = 4 * C;
@@ -83,13 +85,19 @@ __device__ inline void st4(bf16* p, float a, float b, float c, float d) {
 
 // GELU in its tanh form, 0.5 x (1 + tanh(u)), u = sqrt(2/pi) (x + 0.044715 x^3),
 // evaluated as x * sigma(2u) with sigma(t) = 1 / (1 + e^-t) (the same function:
-// 0.5 (1 + tanh(u)) = sigma(2u)), so one hardware exp and one reciprocal
-// replace the tanh: the cell GEMM evaluates 32 of them per cell.
+// 0.5 (1 + tanh(u)) = sigma(2u)): e^-2u = 2^(x (kE1 + kE2 x^2)) with the
+// constants folded, then the hardware exp2 and reciprocal (v_exp_f32,
+// v_rcp_f32, 1 ulp; the activations are stored as BF16).  Seven VALU
+// instructions per activation, where an IEEE division alone takes ten; the
+// cell GEMM evaluates 32 activations per cell.  Limits: x -> +inf gives
+// 2^-inf = 0 and x; x -> -inf gives rcp(inf) = 0 and -0.
 constexpr float kG0 = 0.7978845608028654f;  // sqrt(2/pi)
 constexpr float kG1 = 0.044715f;
+constexpr float kE1 = -2.0 * 0.7978845608028654 * 1.4426950408889634;             // -2 sqrt(2/pi) log2(e)
+constexpr float kE2 = -2.0 * 0.7978845608028654 * 1.4426950408889634 * 0.044715;  // kE1 * 0.044715
 __device__ inline float sig2u(float x) {
-  const float u2 = 2.f * kG0 * (x + kG1 * x * x * x);
-  return __frcp_rn(1.f + __expf(-u2));
+  const float t = x * __builtin_fmaf(x * x, kE2, kE1);
+  return __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(t));
 }
 __device__ inline float gelu(float x) { return x * sig2u(x); }
 // d/dx [x sigma(2u)] = sigma + x sigma (1 - sigma) 2u'
@@ -175,39 +183,58 @@ __global__ void __launch_bounds__(256) k_rowproj(const bf16* __restrict__ Xt, co
 }
 
 // ---------------------------------------------------------------------------
-// DFT along x at the retained kx, one chunk of XC rows per block:
+// DFT along x at the retained kx, one chunk of XR rows per block:
 //   out[b][chunk][kxi][ky][c] = s_ky sum_{x in chunk} (in[b][x][2ky][c] + i in[b][x][2ky+1][c]) e^{-2 pi i kx x/N}
 // with s_ky = 1 (forward) or w_ky / N^2 (backward of the inverse transform).
-__global__ void __launch_bounds__(1024) k_dftx(const float* __restrict__ in, float2* __restrict__ outp,
-                                               const float2* __restrict__ Ex, int N, int scaled,
-                                               const int* done) {
+// Thread (kp, cp) of 256 accumulates the 2 x 2 outputs kxi in {2kp, 2kp+1},
+// c in {cp, cp + 16}: four LDS reads per four complex products.  XR = 64
+// halves the partials (and k_mix's reads of them) where batches give enough
+// blocks; XR = 32 keeps 256 blocks for one vector.
+template <int XR>
+__global__ void __launch_bounds__(256) k_dftx(const float* __restrict__ in, float2* __restrict__ outp,
+                                              const float2* __restrict__ Ex, int N, int scaled,
+                                              const int* done) {
   if (skip(done)) return;
-  __shared__ float2 sz[XC][C + 1];
-  __shared__ float2 se[KX][XC + 1];
+  __shared__ float2 sz[XR][C + 1];
+  __shared__ float2 se[KX][XR + 1];
   const int chunk = blockIdx.x, ky = blockIdx.y, b = blockIdx.z;
   const int nch = gridDim.x;
   const int tid = threadIdx.x;
-  const int hi = tid >> 5, lo = tid & 31;
-  const int x0 = chunk * XC;
-  {
-    const float* row = in + ((long)(b * N + x0 + hi) * C + 2 * ky) * C;
-    sz[hi][lo] = make_float2(row[lo], row[C + lo]);
-    se[hi][lo] = Ex[(long)hi * N + x0 + lo];   // frequency hi at row x0 + lo
+  const int x0 = chunk * XR;
+#pragma unroll
+  for (int e = tid; e < XR * C; e += 256) {
+    const int xr = e >> 5, c = e & 31;
+    const float* row = in + ((long)(b * N + x0 + xr) * C + 2 * ky) * C;
+    sz[xr][c] = make_float2(row[c], row[C + c]);
+  }
+#pragma unroll
+  for (int e = tid; e < KX * XR; e += 256) {
+    const int q = e / XR, xl = e % XR;
+    se[q][xl] = Ex[(long)q * N + x0 + xl];  // frequency q at row x0 + xl
   }
   __syncthreads();
-  float2 acc = make_float2(0.f, 0.f);
-  // partial unrolling: a full unroll hoists all 64 LDS operands and spills
-  // past the 128 VGPRs of a 1024-thread block
+  const int kp = tid >> 4, cp = tid & 15;
+  float2 acc[2][2] = {};
 #pragma unroll 8
-  for (int xl = 0; xl < XC; ++xl) {
-    const float2 e = se[hi][xl];
-    const float2 z = sz[xl][lo];
-    acc.x += z.x * e.x + z.y * e.y;  // z * conj(e)
-    acc.y += z.y * e.x - z.x * e.y;
+  for (int xl = 0; xl < XR; ++xl) {
+    const float2 e[2] = {se[2 * kp][xl], se[2 * kp + 1][xl]};
+    const float2 z[2] = {sz[xl][cp], sz[xl][cp + 16]};
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        acc[a][c].x += z[c].x * e[a].x + z[c].y * e[a].y;  // z * conj(e)
+        acc[a][c].y += z[c].y * e[a].x - z[c].x * e[a].y;
+      }
   }
   float sc = 1.f;
   if (scaled) sc = (ky == 0 ? 1.f : 2.f) / ((float)N * (float)N);
-  outp[(((long)(b * nch + chunk) * KX + hi) * KY + ky) * C + lo] = make_float2(acc.x * sc, acc.y * sc);
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+      outp[(((long)(b * nch + chunk) * KX + 2 * kp + a) * KY + ky) * C + cp + 16 * c] =
+          make_float2(acc[a][c].x * sc, acc[a][c].y * sc);
 }
 
 // ---------------------------------------------------------------------------
@@ -412,12 +439,31 @@ __global__ void __launch_bounds__(256) k_pixel(const bf16* __restrict__ P, const
 //                 identity path, z = s_id v + (h4 W_out + b_out) / s
 // The row is staged as [y][S] BF16 (the layout of k_rowproj).
 
-// A_x = F^T H_x from the row staged in LDS.  NW waves: wave w owns output
-// tile (a-tile (w >> 1) & 1, c-tile w & 1); with NW = 8 waves w and w + 4
-// split K in halves and the upper half adds into the lower through red.
+// Row stride (BF16) of the projection basis F^T staged in LDS by the row
+// kernels: 16 bytes of padding move row a + 1 four banks on, so the 16 rows
+// an A operand read touches fall in different banks.
+__host__ __device__ constexpr int basis_ld(int N) { return N + 8; }
+// LDS bytes of the forward row kernels: the row ([y][S]) and the basis.
+inline size_t row_kernel_lds(int N) { return (size_t)N * S * sizeof(bf16) + (size_t)C * basis_ld(N) * sizeof(bf16); }
+
+// F^T (32 x N, global) to LDS with row stride basis_ld(N), 16 bytes per item.
+template <int NT>
+__device__ inline void basis_to_lds(const bf16* __restrict__ Xt, bf16* lb, int N) {
+  const int per_row = N / 8;
+  for (int q = threadIdx.x; q < C * per_row; q += NT) {
+    const int a = q / per_row, y8 = (q - a * per_row) * 8;
+    *reinterpret_cast<u32x4*>(lb + a * basis_ld(N) + y8) = *reinterpret_cast<const u32x4*>(Xt + (long)a * N + y8);
+  }
+}
+
+// A_x = F^T H_x from the row staged in LDS, F^T read with row stride ldx
+// (from global memory or, in the forward row kernels, from LDS).  NW waves:
+// wave w owns output tile (a-tile (w >> 1) & 1, c-tile w & 1); with NW = 8
+// waves w and w + 4 split K in halves and the upper half adds into the lower
+// through red.
 template <int NW>
-__device__ inline void row_project(const bf16* __restrict__ Xt, const bf16* ly, int N, float* __restrict__ o,
-                                   float* red) {
+__device__ inline void row_project(const bf16* __restrict__ Xt, int ldx, const bf16* ly, int N,
+                                   float* __restrict__ o, float* red) {
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int i = lane & 15, g = lane >> 4;
   const int ta = (w >> 1) & 1, tc = w & 1;
@@ -429,7 +475,7 @@ __device__ inline void row_project(const bf16* __restrict__ Xt, const bf16* ly, 
   // together (the basis rows come from L2; one load per step would expose
   // its latency N/32 times)
   const int steps = (yhi - ylo) / 32;
-  const bf16* xa = Xt + (long)(16 * ta + i) * N + ylo + 8 * g;
+  const bf16* xa = Xt + (long)(16 * ta + i) * ldx + ylo + 8 * g;
   for (int s0 = 0; s0 < steps; s0 += 4) {
     bf16x8 a[4], b[4];
 #pragma unroll
@@ -461,59 +507,93 @@ __device__ inline void row_project(const bf16* __restrict__ Xt, const bf16* ly, 
   for (int v = 0; v < 4; ++v) o[(16 * ta + 4 * g + v) * C + 16 * tc + i] = acc[v];
 }
 
-// One block (512 threads) per grid row x of one sample: the lift
-//   h0[p][c] = a0 W_in[0][c] + a1 W_in[1][c] + a2 W_in[2][c] + b_in[c]
-// of the row to global memory and LDS, then its row projection into Aout[x].
-// With nrm the input is v / *nrm (written back to vw when given), which lets
-// FGMRES hand over its lazily normalised Krylov vector.  Sample k reads chi,
-// v (and vw) at k * sChi, k * sV and its norm at nrm + k * sNrm bytes.
-__global__ void __launch_bounds__(512) k_lift_row(const float* __restrict__ chi, long sChi, float chi_scale,
-                                                  const float2* v, long sV, const double* nrm, long sNrm, float2* vw,
-                                                  float s, const float* __restrict__ P, bf16* __restrict__ h0,
-                                                  const bf16* __restrict__ Xt, float* __restrict__ Aout, int N,
-                                                  const int* done) {
-  if (skip(done)) return;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  bf16* lrow = reinterpret_cast<bf16*>(smem_raw);
-  __shared__ float red[4 * 64 * 4];
-  const long bx = blockIdx.x;
-  const long ks = bx / N;                     // sample
-  const long rin = (bx - ks * N) * N;         // the row's first cell in the sample
-  chi += ks * sChi;
-  v += ks * sV;
-  if (vw) vw += ks * sV;
+// Input of the lift: sample k reads chi at k * sChi, v at k * sV and its
+// norm at nrm + k * sNrm bytes (nrm null: v is used as it is).
+struct LiftIn {
+  const float* chi;
+  long sChi;
+  float chi_scale;
+  const float2* v;
+  long sV;
+  const double* nrm;
+  long sNrm;
+  float s;          // v's scale (N: unit vectors become O(1) per cell)
+  const float* P;   // W_in (3 x 32) followed by b_in (32)
+};
+
+// The lift of grid row xr of sample ks into the LDS row lrow ([y][S] BF16):
+//   h0[y][c] = a0 W_in[0][c] + a1 W_in[1][c] + a2 W_in[2][c] + b_in[c],
+//   (a0, a1, a2) = (chi chi_scale, s Re v', s Im v'),  v' = v / *nrm,
+// one item q = (cell y, quarter) of 8 channels per thread and step (the four
+// items of a cell are adjacent lanes, so an in-place vw == v is read before
+// it is written).  With h0 the row also goes to global memory (16-byte
+// stores), with vw v' goes back to vw.  Both row kernels that lift call
+// this, so they round the same sums.
+template <int NT>
+__device__ inline void lift_row_to_lds(const LiftIn& L, long ks, int xr, int N, bf16* lrow, bf16* h0,
+                                       float2* vw) {
+  const long rin = (long)xr * N;  // the row's first cell in the sample
+  const float* chi = L.chi + ks * L.sChi;
+  const float2* v = L.v + ks * L.sV;
+  if (vw) vw += ks * L.sV;
   float r = 1.f;
-  if (nrm) {
-    const double d = *boff(nrm, ks * sNrm);
+  if (L.nrm) {
+    const double d = *boff(L.nrm, ks * L.sNrm);
     r = d > 0.0 ? (float)(1.0 / d) : 0.f;
   }
-  // item q = (cell y, quarter): 8 channels, one 16-byte store; the four items
-  // of a cell are adjacent lanes, so an in-place vw == v is read before it is
-  // written
-  for (int q = threadIdx.x; q < N * 4; q += 512) {
-    const int y = q >> 2, c8 = (q & 3) * 8;
-    const long p = bx * N + y, pin = rin + y;
+  // NT % 4 == 0: every item of this thread has the same 8 channels, whose
+  // 4 x 8 lift parameters stay in registers
+  static_assert(NT % 4 == 0, "items of a thread must share their channels");
+  const int c8 = (threadIdx.x & 3) * 8;
+  float w0[8], w1[8], w2[8], wb[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    w0[k] = L.P[c8 + k];
+    w1[k] = L.P[C + c8 + k];
+    w2[k] = L.P[2 * C + c8 + k];
+    wb[k] = L.P[3 * C + c8 + k];
+  }
+  for (int q = threadIdx.x; q < N * 4; q += NT) {
+    const int y = q >> 2;
+    const long pin = rin + y;
     float2 vv = v[pin];
-    if (nrm) {
+    if (L.nrm) {
       vv.x *= r;
       vv.y *= r;
       if (vw && c8 == 0) vw[pin] = vv;
     }
-    const float a0 = chi[pin] * chi_scale, a1 = vv.x * s, a2 = vv.y * s;
+    const float a0 = chi[pin] * L.chi_scale, a1 = vv.x * L.s, a2 = vv.y * L.s;
     struct __align__(16) Q8 { bf16 v[8]; } hq;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const int c = c8 + k;
-      hq.v[k] = (bf16)(a0 * P[c] + a1 * P[C + c] + a2 * P[2 * C + c] + P[3 * C + c]);
-    }
-    *reinterpret_cast<Q8*>(h0 + p * C + c8) = hq;
+    for (int k = 0; k < 8; ++k)
+      hq.v[k] = (bf16)__builtin_fmaf(a2, w2[k], __builtin_fmaf(a1, w1[k], __builtin_fmaf(a0, w0[k], wb[k])));
+    if (h0) *reinterpret_cast<Q8*>(h0 + ((ks * N + xr) * N + y) * C + c8) = hq;
     const u32x4 bits = __builtin_bit_cast(u32x4, hq);
     u32x2* d = reinterpret_cast<u32x2*>(lrow + y * S + c8);
     d[0] = u32x2{bits[0], bits[1]};
     d[1] = u32x2{bits[2], bits[3]};
   }
+}
+
+// One block (512 threads) per grid row x of one sample: the lift of the row
+// to LDS (and to h0 when given: training keeps it for the backward pass),
+// then its row projection into Aout[x].  With nrm the input is v / *nrm
+// (written back to vw when given), which lets FGMRES hand over its lazily
+// normalised Krylov vector.
+__global__ void __launch_bounds__(512) k_lift_row(LiftIn L, float2* vw, bf16* __restrict__ h0,
+                                                  const bf16* __restrict__ Xt, float* __restrict__ Aout, int N,
+                                                  const int* done) {
+  if (skip(done)) return;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  bf16* lrow = reinterpret_cast<bf16*>(smem_raw);
+  bf16* lbas = lrow + N * S;
+  __shared__ float red[4 * 64 * 4];
+  const long bx = blockIdx.x;
+  const long ks = bx / N;  // sample
+  basis_to_lds<512>(Xt, lbas, N);
+  lift_row_to_lds<512>(L, ks, (int)(bx - ks * N), N, lrow, h0, vw);
   __syncthreads();
-  row_project<8>(Xt, lrow, N, Aout + bx * C * C, red);
+  row_project<8>(lbas, basis_ld(N), lrow, N, Aout + bx * C * C, red);
 }
 
 // One block (8 waves) per grid row x of sample k (layers 0-2):
@@ -528,8 +608,13 @@ __global__ void __launch_bounds__(512) k_lift_row(const float* __restrict__ chi,
 //      pre_out when given, for training);
 //   3. one coalesced copy of the row to out and its row projection into
 //      Anext[x].
-template <bool FOLD>
-__global__ void __launch_bounds__(512) k_cell_row(const bf16* __restrict__ P, const bf16* __restrict__ Q,
+// With LIFT (layer 0) h[y] is not read from P: the row is lifted from (chi,
+// v) into the LDS row first (lift_row_to_lds, the same sums as k_lift_row),
+// 12 bytes per cell instead of 64, and the tiles read their cells from there
+// (each wave reads only its own tiles' cells before it overwrites them with
+// h').
+template <bool FOLD, bool LIFT>
+__global__ void __launch_bounds__(512) k_cell_row(const bf16* __restrict__ P, LiftIn L, const bf16* __restrict__ Q,
                                                   const bf16* __restrict__ A, const bf16* __restrict__ R,
                                                   const float2* __restrict__ Vm, const float2* __restrict__ Ex,
                                                   const float* __restrict__ bias, bf16* __restrict__ pre_out,
@@ -538,6 +623,7 @@ __global__ void __launch_bounds__(512) k_cell_row(const bf16* __restrict__ P, co
   if (skip(done)) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   bf16* lrow = reinterpret_cast<bf16*>(smem_raw);
+  bf16* lbas = lrow + N * S;  // projection basis F^T, row stride basis_ld(N)
   __shared__ float red[4 * 64 * 4];
   __shared__ __align__(16) bf16 lb[C][C + 8];  // B_x as [o][k], rows padded to 80 B
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -553,10 +639,12 @@ __global__ void __launch_bounds__(512) k_cell_row(const bf16* __restrict__ P, co
     const int t = w + 8 * u;
     if (t < ntile) {
       const int y = 16 * t + i;
-      bpt[u] = ld8(P + (bx * N + y) * C + 8 * g);
+      if (!LIFT) bpt[u] = ld8(P + (bx * N + y) * C + 8 * g);
       bqt[u] = ld8(Q + (long)y * C + 8 * g);
     }
   }
+  basis_to_lds<512>(Xt, lbas, N);
+  if (LIFT) lift_row_to_lds<512>(L, ks, xr, N, lrow, nullptr, nullptr);
   if (!FOLD) {
     if (tid < C * C / 8) {  // 128 threads x 8 BF16
       const int o = tid >> 2, k8 = (tid & 3) * 8;
@@ -597,8 +685,19 @@ __global__ void __launch_bounds__(512) k_cell_row(const bf16* __restrict__ P, co
         const int t = w + 8 * (k0 + u);
         if (t < ntile) {
           const int y = 16 * t + i;
-          bpt[u] = ld8(P + (bx * N + y) * C + 8 * g);
+          if (!LIFT) bpt[u] = ld8(P + (bx * N + y) * C + 8 * g);
           bqt[u] = ld8(Q + (long)y * C + 8 * g);
+        }
+      }
+    }
+    if (LIFT) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int t = w + 8 * (k0 + u);
+        if (t < ntile) {
+          const u32x2* sp = reinterpret_cast<const u32x2*>(lrow + (16 * t + i) * S + 8 * g);
+          const u32x2 lo = sp[0], hi = sp[1];
+          bpt[u] = __builtin_bit_cast(bf16x8, u32x4{lo[0], lo[1], hi[0], hi[1]});
         }
       }
     }
@@ -630,7 +729,7 @@ __global__ void __launch_bounds__(512) k_cell_row(const bf16* __restrict__ P, co
     const u32x2 lo = sp[0], hi = sp[1];
     *(u32x4*)(out + (bx * N + y) * C + c8) = u32x4{lo[0], lo[1], hi[0], hi[1]};
   }
-  row_project<8>(Xt, lrow, N, Anext + bx * C * C, red);
+  row_project<8>(lbas, basis_ld(N), lrow, N, Anext + bx * C * C, red);
 }
 
 // Layer 3 (no activation) fused with the projection: per cell
@@ -903,11 +1002,21 @@ void launch_invx(cgm_fno* f, hipStream_t st, int B, const int* done) {
     hipLaunchKernelGGL(k_invx<2>, dim3(f->N / 2, 1, B), dim3(512), 0, st, f->Vm, f->Bt, f->Ex, f->N, done);
 }
 
+// x-DFT of A (B vectors) into partials over chunks of 64 (B >= 4) or 32
+// rows; returns the number of chunks
+int launch_dftx(cgm_fno* f, hipStream_t st, int scaled, int B, const int* done) {
+  if (B >= 4 && f->N % 64 == 0) {
+    hipLaunchKernelGGL(k_dftx<64>, dim3(f->N / 64, KY, B), dim3(256), 0, st, f->A, f->upart, f->Ex, f->N, scaled, done);
+    return f->N / 64;
+  }
+  hipLaunchKernelGGL(k_dftx<32>, dim3(f->N / 32, KY, B), dim3(256), 0, st, f->A, f->upart, f->Ex, f->N, scaled, done);
+  return f->N / 32;
+}
+
 // spectral path of one layer: A (row projections) -> Bt (per-row mixing matrices)
 int spectral(cgm_fno* f, hipStream_t st, const float2* W, float2* usave, bool bwd, int B, const int* done,
              bool invx = true) {
-  const int nch = f->N / XC;
-  hipLaunchKernelGGL(k_dftx, dim3(nch, KY, B), dim3(1024), 0, st, f->A, f->upart, f->Ex, f->N, bwd ? 1 : 0, done);
+  const int nch = launch_dftx(f, st, bwd ? 1 : 0, B, done);
   if (!bwd)
     hipLaunchKernelGGL((k_mix<false>), dim3(KY, KX / KXG, B), dim3(256), 0, st, f->upart, nch, W, usave, f->Vm,
                        f->N, done);
@@ -929,13 +1038,21 @@ namespace cgm {
 int fno_forward_io(cgm_fno* f, hipStream_t st, const FnoIO& io, int B, bool train, const int* done) {
   const float s = (float)f->N;
   const int rows = B * f->N;
-  const size_t lds = (size_t)f->N * S * sizeof(bf16);
+  const size_t lds = row_kernel_lds(f->N);
   const bool fold = B == 1;
   int rc;
-  hipLaunchKernelGGL(k_lift_row, dim3(rows), dim3(512), lds, st, io.chi, io.sChi, f->chi_scale, io.v, io.sV, io.nrm,
-                     io.sNrm, io.vw, s, f->params, f->h[0], f->Ft, f->A, f->N, done);
+  LiftIn L{io.chi, io.sChi, f->chi_scale, io.v, io.sV, io.nrm, io.sNrm, s, f->params};
+  hipLaunchKernelGGL(k_lift_row, dim3(rows), dim3(512), lds, st, L, io.vw, train ? f->h[0] : nullptr, f->Ft, f->A,
+                     f->N, done);
   CGM_HIP(hipGetLastError());
   const float2* vv = io.vw ? io.vw : io.v;
+  // layer 0 lifts its rows again from v' (normalised by k_lift_row into vw
+  // when given) instead of reading h0
+  LiftIn L0 = L;
+  if (io.vw) {
+    L0.v = io.vw;
+    L0.nrm = nullptr;
+  }
   for (int l = 0; l < 4; ++l) {
     const float* Lp = f->params + layer_off(l);
     // x-DFT and mixing; for one sample the inverse x-DFT of each row of
@@ -946,9 +1063,10 @@ int fno_forward_io(cgm_fno* f, hipStream_t st, const FnoIO& io, int B, bool trai
       return rc;
     const bf16* Wl = f->Wt + l * C * C;
     if (l < 3) {
-      hipLaunchKernelGGL(fold ? k_cell_row<true> : k_cell_row<false>, dim3(rows), dim3(512), lds, st, f->h[l],
-                         f->Fy, Wl, f->Bt, f->Vm, f->Ex, Lp + C * C, train ? f->pre[l] : nullptr, f->h[l + 1], f->Ft,
-                         f->A, f->N, done);
+      auto kern = l == 0 ? (fold ? k_cell_row<true, true> : k_cell_row<false, true>)
+                         : (fold ? k_cell_row<true, false> : k_cell_row<false, false>);
+      hipLaunchKernelGGL(kern, dim3(rows), dim3(512), lds, st, f->h[l], L0, f->Fy, Wl, f->Bt, f->Vm, f->Ex,
+                         Lp + C * C, train ? f->pre[l] : nullptr, f->h[l + 1], f->Ft, f->A, f->N, done);
     } else {
       const long waves = (long)B * f->n / 64;
       CGM_CHECK(waves % 4 == 0, CGM_ERROR_UNSUPPORTED, "cell GEMM needs N %% 64 == 0");
@@ -1106,11 +1224,12 @@ extern "C" int cgm_fno_create(cgm_ctx_t c, int batch, float chi_scale, cgm_fno_t
   CGM_HIP(hipFuncSetAttribute((const void*)k_rowproj<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2));
   CGM_HIP(hipFuncSetAttribute((const void*)k_rowproj<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)(lds2 / 2)));
-  CGM_HIP(hipFuncSetAttribute((const void*)k_lift_row, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(lds2 / 2)));
-  CGM_HIP(hipFuncSetAttribute((const void*)k_cell_row<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)(lds2 / 2)));
-  CGM_HIP(hipFuncSetAttribute((const void*)k_cell_row<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)(lds2 / 2)));
+  CGM_HIP(hipFuncSetAttribute((const void*)k_lift_row, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)row_kernel_lds(N)));
+  const void* rowk[] = {(const void*)k_cell_row<true, false>, (const void*)k_cell_row<false, false>,
+                        (const void*)k_cell_row<true, true>, (const void*)k_cell_row<false, true>};
+  for (const void* k : rowk)
+    CGM_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)row_kernel_lds(N)));
   *out = f;
   return CGM_SUCCESS;
 }
@@ -1156,8 +1275,10 @@ extern "C" int cgm_fno_backward(cgm_fno_t f, const cgm_complex64* dz, float* gra
 // (contiguous [batch][N*N] chi, v, z): ms[0] lift with the first row projection (k_lift_row),
 // [1] row projection alone (k_rowproj, used by the backward pass), [2] x-DFT,
 // [3] mixing, [4] inverse x-DFT, [5] cell GEMM with GELU and the next row
-// projection (k_cell_row, forming B_x itself for batch 1), [6] last cell
-// GEMM with the projection (k_cell_proj), [7] one whole forward pass.
+// projection (k_cell_row of layers 1-2, forming B_x itself for batch 1), [6]
+// last cell GEMM with the projection (k_cell_proj), [7] one whole forward
+// pass, [8] the cell GEMM of layer 0 (k_cell_row lifting its row from chi
+// and v).  The lift of [0] keeps h0 in LDS only, as in inference.
 extern "C" int cgm_fno_profile(cgm_fno_t f, const float* chi, const cgm_complex64* v, cgm_complex64* z, int batch,
                                int reps, float* ms) {
   CGM_CHECK(f && chi && v && z && ms && reps > 0, CGM_ERROR_INVALID_VALUE, "null argument");
@@ -1169,26 +1290,25 @@ extern "C" int cgm_fno_profile(cgm_fno_t f, const float* chi, const cgm_complex6
   const long total = (long)B * f->n;
   const unsigned rows = (unsigned)(B * f->N);
   const float s = (float)f->N;
-  const int nch = f->N / XC;
+  int nch = 0;
   const float* Lp = f->params + layer_off(0);
   const float2* W = (const float2*)(Lp + C * C + C);
-  const size_t lds = (size_t)f->N * S * sizeof(bf16);
+  const size_t lds = row_kernel_lds(f->N);
   hipEvent_t e0, e1;
   CGM_HIP(hipEventCreate(&e0));
   CGM_HIP(hipEventCreate(&e1));
+  const LiftIn L{chi, f->n, f->chi_scale, (const float2*)v, f->n, nullptr, 0L, s, f->params};
   int rc = CGM_SUCCESS;
-  for (int k = 0; k < 8 && !rc; ++k) {
+  for (int k = 0; k < 9 && !rc; ++k) {
     for (int r = -2; r < reps && !rc; ++r) {
       if (r == 0) CGM_HIP(hipEventRecord(e0, st));
       switch (k) {
-        case 0:
-          hipLaunchKernelGGL(k_lift_row, dim3(rows), dim3(512), lds, st, chi, f->n, f->chi_scale, (const float2*)v,
-                             f->n, nullptr, 0L, nullptr, s, f->params, f->h[0], f->Ft, f->A, f->N, nullptr);
+        case 0:  // the first warm-up also stores h0, which cases 1 and 5 read
+          hipLaunchKernelGGL(k_lift_row, dim3(rows), dim3(512), lds, st, L, nullptr, r == -2 ? f->h[0] : nullptr,
+                             f->Ft, f->A, f->N, nullptr);
           break;
         case 1: rc = launch_rowproj<false>(f, st, f->h[0], nullptr, f->A, B, nullptr); break;
-        case 2:
-          hipLaunchKernelGGL(k_dftx, dim3(nch, KY, B), dim3(1024), 0, st, f->A, f->upart, f->Ex, f->N, 0, nullptr);
-          break;
+        case 2: nch = launch_dftx(f, st, 0, B, nullptr); break;
         case 3:
           hipLaunchKernelGGL((k_mix<false>), dim3(KY, KX / KXG, B), dim3(256), 0, st, f->upart, nch, W, nullptr,
                              f->Vm, f->N, nullptr);
@@ -1197,9 +1317,14 @@ extern "C" int cgm_fno_profile(cgm_fno_t f, const float* chi, const cgm_complex6
           launch_invx(f, st, B, nullptr);
           break;
         case 5:
-          hipLaunchKernelGGL(B == 1 ? k_cell_row<true> : k_cell_row<false>, dim3(rows), dim3(512), lds, st, f->h[0],
-                             f->Fy, f->Wt, f->Bt, f->Vm, f->Ex, Lp + C * C, nullptr, f->h[1], f->Ft, f->A, f->N,
-                             nullptr);
+          hipLaunchKernelGGL((B == 1 ? k_cell_row<true, false> : k_cell_row<false, false>), dim3(rows), dim3(512), lds,
+                             st, f->h[0], L, f->Fy, f->Wt, f->Bt, f->Vm, f->Ex, Lp + C * C, nullptr, f->h[1], f->Ft,
+                             f->A, f->N, nullptr);
+          break;
+        case 8:
+          hipLaunchKernelGGL((B == 1 ? k_cell_row<true, true> : k_cell_row<false, true>), dim3(rows), dim3(512), lds,
+                             st, nullptr, L, f->Fy, f->Wt, f->Bt, f->Vm, f->Ex, Lp + C * C, nullptr, f->h[1], f->Ft,
+                             f->A, f->N, nullptr);
           break;
         case 6:
           hipLaunchKernelGGL(k_cell_proj, dim3((unsigned)(total / 256)), dim3(256), 0, st, f->h[3], f->Fy, f->Wt,

@@ -369,9 +369,10 @@ int cgm_profile_step(cgm_ctx_t ctx, const cgm_complex64* dev_gamma, const float*
  * each) of the preconditioner's kernels on `batch` vectors ([batch][N*N] chi,
  * v, z): ms[0] lift with the first row projection, [1] stand-alone row
  * projection (backward pass), [2] x-DFT, [3] mode mixing, [4] inverse x-DFT,
- * [5] cell GEMM with GELU and the next row projection, [6] last cell GEMM
- * with the projection, [7] a whole forward pass.  Overwrites the network's
- * activations. */
+ * [5] cell GEMM with GELU and the next row projection (layers 1-2), [6] last
+ * cell GEMM with the projection, [7] a whole forward pass, [8] the cell GEMM
+ * of layer 0, which lifts its row from chi and v itself (ms holds 9
+ * entries).  Overwrites the network's activations. */
 int cgm_fno_profile(cgm_fno_t net, const float* dev_chi, const cgm_complex64* dev_v,
                     cgm_complex64* dev_z, int batch, int reps, float* ms);
 

@@ -137,8 +137,11 @@ def gpu():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("N,B", [(64, 2), (128, 1), (256, 2)])
+@pytest.mark.parametrize("N,B", [(64, 2), (128, 1), (256, 2), (128, 4), (1024, 1)])
 def test_fno_forward_matches_oracle(gpu, N, B):
+    # B = 1 forms B_x inside the row kernels, B = 2 with k_invx and x-DFT
+    # chunks of 32 rows, B = 4 with chunks of 64 rows; N = 1024 is the
+    # largest grid, whose row kernels hold 146 KiB of LDS
     solver, fno = gpu
     chi, v, p = _problem(N, B, 11)
     ctx = solver.Context(N)
@@ -153,7 +156,7 @@ def test_fno_forward_matches_oracle(gpu, N, B):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("N,B", [(64, 2), (128, 1)])
+@pytest.mark.parametrize("N,B", [(64, 2), (128, 1), (128, 4)])
 def test_fno_backward_matches_oracle(gpu, N, B):
     solver, fno = gpu
     chi, v, p = _problem(N, B, 21)
