@@ -437,14 +437,21 @@ __global__ void __launch_bounds__(256) k_pixel(const bf16* __restrict__ P, const
 //                 then A_x = F^T H'_x
 //   k_cell_proj : cell GEMM of layer 3 with the 32 -> 2 projection and the
 //                 identity path, z = s_id v + (h4 W_out + b_out) / s
-// The row is staged as [y][S] BF16 (the layout of k_rowproj).
+// The row kernels stage the row they project channel-major, [c][y] BF16 with
+// row stride basis_ld(N) (the K = y operand of the projection is then 16
+// contiguous bytes per lane), and the projection basis F^T likewise.
 
-// Row stride (BF16) of the projection basis F^T staged in LDS by the row
-// kernels: 16 bytes of padding move row a + 1 four banks on, so the 16 rows
-// an A operand read touches fall in different banks.
+// Row stride (BF16) of the channel-major row and of the projection basis F^T
+// staged in LDS by the row kernels: 16 bytes of padding move row a + 1 four
+// banks on, so the 16 rows an MFMA operand read touches fall in different
+// banks.
 __host__ __device__ constexpr int basis_ld(int N) { return N + 8; }
-// LDS bytes of the forward row kernels: the row ([y][S]) and the basis.
-inline size_t row_kernel_lds(int N) { return (size_t)N * S * sizeof(bf16) + (size_t)C * basis_ld(N) * sizeof(bf16); }
+// LDS bytes of the forward row kernels: the basis and the channel-major row
+// or (layer 0's cell GEMM) the row as [y][S].
+inline size_t row_kernel_lds(int N) {
+  const size_t t = (size_t)C * basis_ld(N) * sizeof(bf16), r = (size_t)N * S * sizeof(bf16);
+  return t + (t > r ? t : r);
+}
 
 // F^T (32 x N, global) to LDS with row stride basis_ld(N), 16 bytes per item.
 template <int NT>
@@ -456,13 +463,14 @@ __device__ inline void basis_to_lds(const bf16* __restrict__ Xt, bf16* lb, int N
   }
 }
 
-// A_x = F^T H_x from the row staged in LDS, F^T read with row stride ldx
-// (from global memory or, in the forward row kernels, from LDS).  NW waves:
-// wave w owns output tile (a-tile (w >> 1) & 1, c-tile w & 1); with NW = 8
-// waves w and w + 4 split K in halves and the upper half adds into the lower
-// through red.
-template <int NW>
-__device__ inline void row_project(const bf16* __restrict__ Xt, int ldx, const bf16* ly, int N,
+// A_x = F^T H_x from the row staged in LDS, channel-major (TR: lt[c][y], row
+// stride basis_ld(N); the B operand is one 16-byte read) or as [y][S] (eight
+// 2-byte reads), F^T read with row stride ldx (from global memory or LDS).
+// NW waves: wave w owns output tile (a-tile (w >> 1) & 1, c-tile w & 1);
+// with NW = 8 waves w and w + 4 split K in halves and the upper half adds
+// into the lower through red.
+template <int NW, bool TR>
+__device__ inline void row_project(const bf16* __restrict__ Xt, int ldx, const bf16* lt, int N,
                                    float* __restrict__ o, float* red) {
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int i = lane & 15, g = lane >> 4;
@@ -476,6 +484,7 @@ __device__ inline void row_project(const bf16* __restrict__ Xt, int ldx, const b
   // its latency N/32 times)
   const int steps = (yhi - ylo) / 32;
   const bf16* xa = Xt + (long)(16 * ta + i) * ldx + ylo + 8 * g;
+  const bf16* xb = lt + (16 * tc + i) * basis_ld(N) + ylo + 8 * g;
   for (int s0 = 0; s0 < steps; s0 += 4) {
     bf16x8 a[4], b[4];
 #pragma unroll
@@ -484,9 +493,13 @@ __device__ inline void row_project(const bf16* __restrict__ Xt, int ldx, const b
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       if (s0 + u < steps) {
-        const int yb = ylo + 32 * (s0 + u) + 8 * g;
+        if (TR) {
+          b[u] = ld8(xb + 32 * (s0 + u));
+        } else {
+          const int yb = ylo + 32 * (s0 + u) + 8 * g;
 #pragma unroll
-        for (int v = 0; v < 8; ++v) b[u][v] = ly[(yb + v) * S + 16 * tc + i];
+          for (int v = 0; v < 8; ++v) b[u][v] = lt[(yb + v) * S + 16 * tc + i];
+        }
       }
     }
 #pragma unroll
@@ -521,16 +534,28 @@ struct LiftIn {
   const float* P;   // W_in (3 x 32) followed by b_in (32)
 };
 
-// The lift of grid row xr of sample ks into the LDS row lrow ([y][S] BF16):
-//   h0[y][c] = a0 W_in[0][c] + a1 W_in[1][c] + a2 W_in[2][c] + b_in[c],
-//   (a0, a1, a2) = (chi chi_scale, s Re v', s Im v'),  v' = v / *nrm,
-// one item q = (cell y, quarter) of 8 channels per thread and step (the four
-// items of a cell are adjacent lanes, so an in-place vw == v is read before
-// it is written).  With h0 the row also goes to global memory (16-byte
-// stores), with vw v' goes back to vw.  Both row kernels that lift call
-// this, so they round the same sums.
-template <int NT>
-__device__ inline void lift_row_to_lds(const LiftIn& L, long ks, int xr, int N, bf16* lrow, bf16* h0,
+// h0 of one cell at 8 channels c0..c0+7 from a = (chi chi_scale, s Re v',
+// s Im v') and the lift parameters w (W_in rows 0-2 and b_in, channels
+// c0..c0+7 at w[0..7], w[8..15], w[16..23], w[24..31]):
+//   h0[c] = a0 W_in[0][c] + a1 W_in[1][c] + a2 W_in[2][c] + b_in[c]
+// The one rounding of the lift: both row kernels that lift call this.
+__device__ inline bf16x8 lift8(float a0, float a1, float a2, const float* w) {
+  bf16x8 h;
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+    h[k] = (bf16)__builtin_fmaf(a2, w[16 + k], __builtin_fmaf(a1, w[8 + k], __builtin_fmaf(a0, w[k], w[24 + k])));
+  return h;
+}
+
+// The lift of grid row xr of sample ks into the LDS row lt, channel-major
+// [c][y] with row stride basis_ld(N) (TR) or [y][S], with (a0, a1, a2) =
+// (chi chi_scale, s Re v', s Im v'), v' = v / *nrm: one item q = (cell y,
+// quarter) of 8 channels per thread and step (the four items of a cell are
+// adjacent lanes, so an in-place vw == v is read before it is written).
+// With h0 the row also goes to global memory (16-byte stores), with vw v'
+// goes back to vw.
+template <int NT, bool TR>
+__device__ inline void lift_row_to_lds(const LiftIn& L, long ks, int xr, int N, bf16* lt, bf16* h0,
                                        float2* vw) {
   const long rin = (long)xr * N;  // the row's first cell in the sample
   const float* chi = L.chi + ks * L.sChi;
@@ -545,13 +570,11 @@ __device__ inline void lift_row_to_lds(const LiftIn& L, long ks, int xr, int N, 
   // 4 x 8 lift parameters stay in registers
   static_assert(NT % 4 == 0, "items of a thread must share their channels");
   const int c8 = (threadIdx.x & 3) * 8;
-  float w0[8], w1[8], w2[8], wb[8];
+  float w[32];
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
-    w0[k] = L.P[c8 + k];
-    w1[k] = L.P[C + c8 + k];
-    w2[k] = L.P[2 * C + c8 + k];
-    wb[k] = L.P[3 * C + c8 + k];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) w[8 * j + k] = L.P[j * C + c8 + k];
   }
   for (int q = threadIdx.x; q < N * 4; q += NT) {
     const int y = q >> 2;
@@ -562,16 +585,17 @@ __device__ inline void lift_row_to_lds(const LiftIn& L, long ks, int xr, int N, 
       vv.y *= r;
       if (vw && c8 == 0) vw[pin] = vv;
     }
-    const float a0 = chi[pin] * L.chi_scale, a1 = vv.x * L.s, a2 = vv.y * L.s;
-    struct __align__(16) Q8 { bf16 v[8]; } hq;
+    const bf16x8 hq = lift8(chi[pin] * L.chi_scale, vv.x * L.s, vv.y * L.s, w);
+    if (h0) *reinterpret_cast<u32x4*>(h0 + ((ks * N + xr) * N + y) * C + c8) = __builtin_bit_cast(u32x4, hq);
+    if (TR) {
 #pragma unroll
-    for (int k = 0; k < 8; ++k)
-      hq.v[k] = (bf16)__builtin_fmaf(a2, w2[k], __builtin_fmaf(a1, w1[k], __builtin_fmaf(a0, w0[k], wb[k])));
-    if (h0) *reinterpret_cast<Q8*>(h0 + ((ks * N + xr) * N + y) * C + c8) = hq;
-    const u32x4 bits = __builtin_bit_cast(u32x4, hq);
-    u32x2* d = reinterpret_cast<u32x2*>(lrow + y * S + c8);
-    d[0] = u32x2{bits[0], bits[1]};
-    d[1] = u32x2{bits[2], bits[3]};
+      for (int k = 0; k < 8; ++k) lt[(c8 + k) * basis_ld(N) + y] = hq[k];
+    } else {
+      const u32x4 bits = __builtin_bit_cast(u32x4, hq);
+      u32x2* d = reinterpret_cast<u32x2*>(lt + y * S + c8);
+      d[0] = u32x2{bits[0], bits[1]};
+      d[1] = u32x2{bits[2], bits[3]};
+    }
   }
 }
 
@@ -585,15 +609,15 @@ __global__ void __launch_bounds__(512) k_lift_row(LiftIn L, float2* vw, bf16* __
                                                   const int* done) {
   if (skip(done)) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  bf16* lrow = reinterpret_cast<bf16*>(smem_raw);
-  bf16* lbas = lrow + N * S;
+  bf16* lt = reinterpret_cast<bf16*>(smem_raw);  // the row, channel-major
+  bf16* lbas = lt + C * basis_ld(N);
   __shared__ float red[4 * 64 * 4];
   const long bx = blockIdx.x;
   const long ks = bx / N;  // sample
   basis_to_lds<512>(Xt, lbas, N);
-  lift_row_to_lds<512>(L, ks, (int)(bx - ks * N), N, lrow, h0, vw);
+  lift_row_to_lds<512, true>(L, ks, (int)(bx - ks * N), N, lt, h0, vw);
   __syncthreads();
-  row_project<8>(lbas, basis_ld(N), lrow, N, Aout + bx * C * C, red);
+  row_project<8, true>(lbas, basis_ld(N), lt, N, Aout + bx * C * C, red);
 }
 
 // One block (8 waves) per grid row x of sample k (layers 0-2):
@@ -604,15 +628,17 @@ __global__ void __launch_bounds__(512) k_lift_row(LiftIn L, float2* vw, bf16* __
 //      e^{+2 pi i kx x/N}), which saves k_invx's launch but reads all of
 //      Vm[k] (128 KiB, from L2) per row: the better choice for one sample;
 //   2. h'[y] = GELU([h[y], F_y] [W; B_x] + b) for the row's N cells (16-cell
-//      tiles t = wave + 8u, as in k_pixel) into LDS (pre-activations to
-//      pre_out when given, for training);
-//   3. one coalesced copy of the row to out and its row projection into
-//      Anext[x].
+//      tiles t = wave + 8u, as in k_pixel) to out (each lane stores the 4
+//      channels it holds of one cell, 8 bytes) and channel-major into LDS
+//      (pre-activations to pre_out when given, for training);
+//   3. the row projection of h' into Anext[x], with the basis staged in LDS.
 // With LIFT (layer 0) h[y] is not read from P: the row is lifted from (chi,
-// v) into the LDS row first (lift_row_to_lds, the same sums as k_lift_row),
-// 12 bytes per cell instead of 64, and the tiles read their cells from there
-// (each wave reads only its own tiles' cells before it overwrites them with
-// h').
+// v) into LDS as [y][S] first (lift_row_to_lds, the rounding of k_lift_row),
+// 12 bytes per cell instead of 64; the tiles read their cells from there and
+// write h' back in place (each wave reads its own tiles' cells before it
+// overwrites them), and the row leaves in one coalesced copy.  (Lifting in
+// registers per tile instead holds the 32 lift weights of a lane through the
+// tile loop: 170 VGPRs, one work-group per CU.)
 template <bool FOLD, bool LIFT>
 __global__ void __launch_bounds__(512) k_cell_row(const bf16* __restrict__ P, LiftIn L, const bf16* __restrict__ Q,
                                                   const bf16* __restrict__ A, const bf16* __restrict__ R,
@@ -622,8 +648,8 @@ __global__ void __launch_bounds__(512) k_cell_row(const bf16* __restrict__ P, Li
                                                   float* __restrict__ Anext, int N, const int* done) {
   if (skip(done)) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  bf16* lrow = reinterpret_cast<bf16*>(smem_raw);
-  bf16* lbas = lrow + N * S;  // projection basis F^T, row stride basis_ld(N)
+  bf16* lbas = reinterpret_cast<bf16*>(smem_raw);  // the basis F^T, row stride basis_ld(N)
+  bf16* lt = lbas + C * basis_ld(N);               // h' channel-major (LIFT: the row as [y][S])
   __shared__ float red[4 * 64 * 4];
   __shared__ __align__(16) bf16 lb[C][C + 8];  // B_x as [o][k], rows padded to 80 B
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -644,7 +670,7 @@ __global__ void __launch_bounds__(512) k_cell_row(const bf16* __restrict__ P, Li
     }
   }
   basis_to_lds<512>(Xt, lbas, N);
-  if (LIFT) lift_row_to_lds<512>(L, ks, xr, N, lrow, nullptr, nullptr);
+  if (LIFT) lift_row_to_lds<512, false>(L, ks, xr, N, lt, nullptr, nullptr);
   if (!FOLD) {
     if (tid < C * C / 8) {  // 128 threads x 8 BF16
       const int o = tid >> 2, k8 = (tid & 3) * 8;
@@ -695,7 +721,7 @@ __global__ void __launch_bounds__(512) k_cell_row(const bf16* __restrict__ P, Li
       for (int u = 0; u < 4; ++u) {
         const int t = w + 8 * (k0 + u);
         if (t < ntile) {
-          const u32x2* sp = reinterpret_cast<const u32x2*>(lrow + (16 * t + i) * S + 8 * g);
+          const u32x2* sp = reinterpret_cast<const u32x2*>(lt + (16 * t + i) * S + 8 * g);
           const u32x2 lo = sp[0], hi = sp[1];
           bpt[u] = __builtin_bit_cast(bf16x8, u32x4{lo[0], lo[1], hi[0], hi[1]});
         }
@@ -718,18 +744,26 @@ __global__ void __launch_bounds__(512) k_cell_row(const bf16* __restrict__ P, Li
         if (pre_out) st4(pre_out + (bx * N + y) * C + oc0, r[0], r[1], r[2], r[3]);
 #pragma unroll
         for (int v = 0; v < 4; ++v) r[v] = gelu(r[v]);
-        st4(lrow + y * S + oc0, r[0], r[1], r[2], r[3]);
+        if (LIFT) {
+          st4(lt + y * S + oc0, r[0], r[1], r[2], r[3]);
+        } else {
+          st4(out + (bx * N + y) * C + oc0, r[0], r[1], r[2], r[3]);
+#pragma unroll
+          for (int v = 0; v < 4; ++v) lt[(oc0 + v) * basis_ld(N) + y] = (bf16)r[v];
+        }
       }
     }
   }
   __syncthreads();
-  for (int q = tid; q < N * 4; q += 512) {
-    const int y = q >> 2, c8 = (q & 3) * 8;
-    const u32x2* sp = reinterpret_cast<const u32x2*>(lrow + y * S + c8);
-    const u32x2 lo = sp[0], hi = sp[1];
-    *(u32x4*)(out + (bx * N + y) * C + c8) = u32x4{lo[0], lo[1], hi[0], hi[1]};
+  if (LIFT) {
+    for (int q = tid; q < N * 4; q += 512) {
+      const int y = q >> 2, c8 = (q & 3) * 8;
+      const u32x2* sp = reinterpret_cast<const u32x2*>(lt + y * S + c8);
+      const u32x2 lo = sp[0], hi = sp[1];
+      *(u32x4*)(out + (bx * N + y) * C + c8) = u32x4{lo[0], lo[1], hi[0], hi[1]};
+    }
   }
-  row_project<8>(lbas, basis_ld(N), lrow, N, Anext + bx * C * C, red);
+  row_project<8, !LIFT>(lbas, basis_ld(N), lt, N, Anext + bx * C * C, red);
 }
 
 // Layer 3 (no activation) fused with the projection: per cell
