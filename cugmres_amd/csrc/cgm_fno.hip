@@ -294,23 +294,29 @@ __global__ void __launch_bounds__(256) k_mix(const float2* __restrict__ upart, i
   vout[slot] = v;
 }
 
-// Inverse DFT along x, XB rows of one sample per block (thread = (ky, ch)):
+// Inverse DFT along x, XP row pairs of one sample per block (thread = (ky, ch)):
 //   out[b][x][ch][2ky], [2ky+1] = Re, Im of sum_kxi V[kxi][ky][ch] e^{+2 pi i kx x/N}   (BF16)
-// The block's XB x 32 x 32 outputs are assembled in LDS and leave as 16-byte
+// Rows x and N - x have conjugate twiddles (kx integer), so with V = a + ib,
+// e = c + is and the sums S1 = sum a c, S2 = sum b s, S3 = sum a s,
+// S4 = sum b c (four products per mode for two rows):
+//   B_x = (S1 - S2) + i (S3 + S4),  B_{N-x} = (S1 + S2) + i (S4 - S3).
+// Pair p >= 1 is rows (p, N - p); pair 0 is rows 0 and N/2, whose twiddles
+// are 1 and (-1)^kx = (-1)^kxi (N even, kx = kxi or N - 32 + kxi).  The
+// block's 2 XP x 32 x 32 outputs are assembled in LDS and leave as 16-byte
 // stores of whole 2 KiB rows.  Every block reads its sample's 128 KiB of
-// modes: XB = 8 for batches (fewer reads), XB = 2 for one vector (enough
+// modes: XP = 4 for batches (fewer reads), XP = 1 for one vector (enough
 // blocks).
-template <int XB>
+template <int XP>
 __global__ void __launch_bounds__(512) k_invx(const float2* __restrict__ V, bf16* __restrict__ out,
                                               const float2* __restrict__ Ex, int N, const int* done) {
   if (skip(done)) return;
-  __shared__ float2 se[XB][KX];
-  __shared__ __align__(16) bf16 lt[XB][C][C];
-  const int x0 = blockIdx.x * XB, b = blockIdx.z;
+  __shared__ float2 se[XP][KX];
+  __shared__ __align__(16) bf16 lt[2 * XP][C][C];
+  const int p0 = blockIdx.x * XP, b = blockIdx.z;
   const int tid = threadIdx.x;
-  if (tid < XB * KX) {
-    const int q = tid / XB, xl = tid % XB;
-    se[xl][q] = Ex[(long)q * N + x0 + xl];
+  if (tid < XP * KX) {
+    const int q = tid / XP, pl = tid % XP;
+    se[pl][q] = Ex[(long)q * N + p0 + pl];
   }
   const int ch = tid & 31, ky = tid >> 5;
   float2 vq[KX];
@@ -318,21 +324,40 @@ __global__ void __launch_bounds__(512) k_invx(const float2* __restrict__ V, bf16
   for (int q = 0; q < KX; ++q) vq[q] = V[(((long)b * KX + q) * KY + ky) * C + ch];
   __syncthreads();
 #pragma unroll 2
-  for (int xl = 0; xl < XB; ++xl) {
-    float2 a = make_float2(0.f, 0.f);
+  for (int pl = 0; pl < XP; ++pl) {
+    float s1 = 0.f, s2 = 0.f, s3 = 0.f, s4 = 0.f;
 #pragma unroll
     for (int q = 0; q < KX; ++q) {
-      const float2 e = se[xl][q];
-      a.x += vq[q].x * e.x - vq[q].y * e.y;
-      a.y += vq[q].x * e.y + vq[q].y * e.x;
+      const float2 e = se[pl][q];
+      s1 += vq[q].x * e.x;
+      s2 += vq[q].y * e.y;
+      s3 += vq[q].x * e.y;
+      s4 += vq[q].y * e.x;
     }
-    lt[xl][ch][2 * ky] = (bf16)a.x;
-    lt[xl][ch][2 * ky + 1] = (bf16)a.y;
+    lt[pl][ch][2 * ky] = (bf16)(s1 - s2);
+    lt[pl][ch][2 * ky + 1] = (bf16)(s3 + s4);
+    if (p0 + pl > 0) {
+      lt[XP + pl][ch][2 * ky] = (bf16)(s1 + s2);
+      lt[XP + pl][ch][2 * ky + 1] = (bf16)(s4 - s3);
+    } else {  // row N/2
+      float t1 = 0.f, t2 = 0.f;
+#pragma unroll
+      for (int q = 0; q < KX; q += 2) {
+        t1 += vq[q].x - vq[q + 1].x;
+        t2 += vq[q].y - vq[q + 1].y;
+      }
+      lt[XP][ch][2 * ky] = (bf16)t1;
+      lt[XP][ch][2 * ky + 1] = (bf16)t2;
+    }
   }
   __syncthreads();
-  u32x4* dst = reinterpret_cast<u32x4*>(out + ((long)b * N + x0) * C * C);
+  // staged row r: x = p0 + r (r < XP), else N - (p0 + r - XP), pair 0's N/2
   const u32x4* src = reinterpret_cast<const u32x4*>(&lt[0][0][0]);
-  for (int q = tid; q < XB * C * C / 8; q += 512) dst[q] = src[q];
+  for (int q = tid; q < 2 * XP * C * C / 8; q += 512) {
+    const int r = q / (C * C / 8), pl = r < XP ? r : r - XP;
+    const int x = r < XP ? p0 + pl : (p0 + pl == 0 ? N / 2 : N - p0 - pl);
+    reinterpret_cast<u32x4*>(out + ((long)b * N + x) * C * C)[q - r * (C * C / 8)] = src[q];
+  }
 }
 
 // Spectral weight gradient: G[kxi][ky][c][o] = sum_b conj(U[b][kxi][ky][c]) dV[b][kxi][ky][o].
@@ -1031,9 +1056,9 @@ int launch_pixel(cgm_fno* f, hipStream_t st, const bf16* P, const bf16* A, const
 
 void launch_invx(cgm_fno* f, hipStream_t st, int B, const int* done) {
   if (B >= 4)
-    hipLaunchKernelGGL(k_invx<8>, dim3(f->N / 8, 1, B), dim3(512), 0, st, f->Vm, f->Bt, f->Ex, f->N, done);
+    hipLaunchKernelGGL(k_invx<4>, dim3(f->N / 8, 1, B), dim3(512), 0, st, f->Vm, f->Bt, f->Ex, f->N, done);
   else
-    hipLaunchKernelGGL(k_invx<2>, dim3(f->N / 2, 1, B), dim3(512), 0, st, f->Vm, f->Bt, f->Ex, f->N, done);
+    hipLaunchKernelGGL(k_invx<1>, dim3(f->N / 2, 1, B), dim3(512), 0, st, f->Vm, f->Bt, f->Ex, f->N, done);
 }
 
 // x-DFT of A (B vectors) into partials over chunks of 64 (B >= 4) or 32
