@@ -186,45 +186,64 @@ __global__ void __launch_bounds__(256) k_rowproj(const bf16* __restrict__ Xt, co
 // DFT along x at the retained kx, one chunk of XR rows per block:
 //   out[b][chunk][kxi][ky][c] = s_ky sum_{x in chunk} (in[b][x][2ky][c] + i in[b][x][2ky+1][c]) e^{-2 pi i kx x/N}
 // with s_ky = 1 (forward) or w_ky / N^2 (backward of the inverse transform).
-// Thread (kp, cp) of 256 accumulates the 2 x 2 outputs kxi in {2kp, 2kp+1},
-// c in {cp, cp + 16}: four LDS reads per four complex products.  XR = 64
-// halves the partials (and k_mix's reads of them) where batches give enough
-// blocks; XR = 32 keeps 256 blocks for one vector.
+// A chunk is XR/2 row pairs: pair p >= 1 is rows (p, N - p), whose twiddles
+// are conjugate (kx integer), so with P = A_x + A_{N-x}, M = A_x - A_{N-x}
+// and e = e^{2 pi i kx x/N} = c + is the pair adds c P - i s M (half the
+// products of two rows); pair 0 is rows 0 and N/2 (twiddles 1 and
+// (-1)^kx = (-1)^kxi: P at even kxi, M at odd).  The partials of all chunks
+// sum to the whole transform.  Thread (kp, cp) of 256 accumulates the 2 x 2
+// outputs kxi in {2kp, 2kp+1}, c in {cp, cp + 16}.
 template <int XR>
 __global__ void __launch_bounds__(256) k_dftx(const float* __restrict__ in, float2* __restrict__ outp,
                                               const float2* __restrict__ Ex, int N, int scaled,
                                               const int* done) {
   if (skip(done)) return;
-  __shared__ float2 sz[XR][C + 1];
-  __shared__ float2 se[KX][XR + 1];
+  constexpr int XH = XR / 2;  // pairs per chunk
+  __shared__ float2 sp[XH][C + 1];
+  __shared__ float2 sm[XH][C + 1];
+  __shared__ float2 se[KX][XH + 1];
   const int chunk = blockIdx.x, ky = blockIdx.y, b = blockIdx.z;
   const int nch = gridDim.x;
   const int tid = threadIdx.x;
-  const int x0 = chunk * XR;
+  const int p0 = chunk * XH;
 #pragma unroll
-  for (int e = tid; e < XR * C; e += 256) {
-    const int xr = e >> 5, c = e & 31;
-    const float* row = in + ((long)(b * N + x0 + xr) * C + 2 * ky) * C;
-    sz[xr][c] = make_float2(row[c], row[C + c]);
+  for (int e = tid; e < XH * C; e += 256) {
+    const int pl = e >> 5, c = e & 31;
+    const int x = p0 + pl, x2 = x == 0 ? N / 2 : N - x;
+    const float* r1 = in + ((long)(b * N + x) * C + 2 * ky) * C;
+    const float* r2 = in + ((long)(b * N + x2) * C + 2 * ky) * C;
+    const float2 u1 = make_float2(r1[c], r1[C + c]), u2 = make_float2(r2[c], r2[C + c]);
+    sp[pl][c] = make_float2(u1.x + u2.x, u1.y + u2.y);
+    sm[pl][c] = make_float2(u1.x - u2.x, u1.y - u2.y);
   }
 #pragma unroll
-  for (int e = tid; e < KX * XR; e += 256) {
-    const int q = e / XR, xl = e % XR;
-    se[q][xl] = Ex[(long)q * N + x0 + xl];  // frequency q at row x0 + xl
+  for (int e = tid; e < KX * XH; e += 256) {
+    const int q = e / XH, pl = e % XH;
+    se[q][pl] = Ex[(long)q * N + p0 + pl];  // frequency q at row p0 + pl
   }
   __syncthreads();
   const int kp = tid >> 4, cp = tid & 15;
   float2 acc[2][2] = {};
-#pragma unroll 8
-  for (int xl = 0; xl < XR; ++xl) {
-    const float2 e[2] = {se[2 * kp][xl], se[2 * kp + 1][xl]};
-    const float2 z[2] = {sz[xl][cp], sz[xl][cp + 16]};
+  int pl0 = 0;
+  if (p0 == 0) {  // rows 0 and N/2: P at kxi = 2kp, M at 2kp + 1
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      acc[0][c] = sp[0][cp + 16 * c];
+      acc[1][c] = sm[0][cp + 16 * c];
+    }
+    pl0 = 1;
+  }
+#pragma unroll 4
+  for (int pl = pl0; pl < XH; ++pl) {
+    const float2 e[2] = {se[2 * kp][pl], se[2 * kp + 1][pl]};
+    const float2 P[2] = {sp[pl][cp], sp[pl][cp + 16]};
+    const float2 M[2] = {sm[pl][cp], sm[pl][cp + 16]};
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
-        acc[a][c].x += z[c].x * e[a].x + z[c].y * e[a].y;  // z * conj(e)
-        acc[a][c].y += z[c].y * e[a].x - z[c].x * e[a].y;
+        acc[a][c].x += e[a].x * P[c].x + e[a].y * M[c].y;  // c P - i s M
+        acc[a][c].y += e[a].x * P[c].y - e[a].y * M[c].x;
       }
   }
   float sc = 1.f;
@@ -702,20 +721,40 @@ __global__ void __launch_bounds__(512) k_cell_row(const bf16* __restrict__ P, Li
       *reinterpret_cast<u32x4*>(&lb[o][k8]) = *reinterpret_cast<const u32x4*>(R + (bx * C + o) * C + k8);
     }
   } else {
+    // the sums and their order of k_invx (rows x and N - x from the
+    // twiddles of the pair's first row p), so one sample gets the B_x that
+    // k_invx forms for a batch, bit for bit
     const int ky = tid >> 5, o = tid & 31;
     const float2* Vs = Vm + (ks * KX * KY + ky) * C + o;
     float2 vq[KX];
 #pragma unroll
     for (int q = 0; q < KX; ++q) vq[q] = Vs[(long)q * KY * C];
-    float2 acc = make_float2(0.f, 0.f);
+    float bre, bim;
+    if (xr == N / 2) {
+      float t1 = 0.f, t2 = 0.f;
 #pragma unroll
-    for (int q = 0; q < KX; ++q) {
-      const float2 e = Ex[(long)q * N + xr];
-      acc.x += vq[q].x * e.x - vq[q].y * e.y;
-      acc.y += vq[q].x * e.y + vq[q].y * e.x;
+      for (int q = 0; q < KX; q += 2) {
+        t1 += vq[q].x - vq[q + 1].x;
+        t2 += vq[q].y - vq[q + 1].y;
+      }
+      bre = t1;
+      bim = t2;
+    } else {
+      const int pr = xr < N / 2 ? xr : N - xr;
+      float s1 = 0.f, s2 = 0.f, s3 = 0.f, s4 = 0.f;
+#pragma unroll
+      for (int q = 0; q < KX; ++q) {
+        const float2 e = Ex[(long)q * N + pr];
+        s1 += vq[q].x * e.x;
+        s2 += vq[q].y * e.y;
+        s3 += vq[q].x * e.y;
+        s4 += vq[q].y * e.x;
+      }
+      bre = xr == pr ? s1 - s2 : s1 + s2;
+      bim = xr == pr ? s3 + s4 : s4 - s3;
     }
-    lb[o][2 * ky] = (bf16)acc.x;
-    lb[o][2 * ky + 1] = (bf16)acc.y;
+    lb[o][2 * ky] = (bf16)bre;
+    lb[o][2 * ky + 1] = (bf16)bim;
   }
   bf16x8 aw[2];
   float bs[2][4];
@@ -1061,15 +1100,12 @@ void launch_invx(cgm_fno* f, hipStream_t st, int B, const int* done) {
     hipLaunchKernelGGL(k_invx<1>, dim3(f->N / 2, 1, B), dim3(512), 0, st, f->Vm, f->Bt, f->Ex, f->N, done);
 }
 
-// x-DFT of A (B vectors) into partials over chunks of 64 (B >= 4) or 32
-// rows; returns the number of chunks
+// x-DFT of A (B vectors) into partials over chunks of 64 rows, whatever B
+// (the same partial sums for one vector and for a batch); returns the number
+// of chunks
 int launch_dftx(cgm_fno* f, hipStream_t st, int scaled, int B, const int* done) {
-  if (B >= 4 && f->N % 64 == 0) {
-    hipLaunchKernelGGL(k_dftx<64>, dim3(f->N / 64, KY, B), dim3(256), 0, st, f->A, f->upart, f->Ex, f->N, scaled, done);
-    return f->N / 64;
-  }
-  hipLaunchKernelGGL(k_dftx<32>, dim3(f->N / 32, KY, B), dim3(256), 0, st, f->A, f->upart, f->Ex, f->N, scaled, done);
-  return f->N / 32;
+  hipLaunchKernelGGL(k_dftx<64>, dim3(f->N / 64, KY, B), dim3(256), 0, st, f->A, f->upart, f->Ex, f->N, scaled, done);
+  return f->N / 64;
 }
 
 // spectral path of one layer: A (row projections) -> Bt (per-row mixing matrices)

@@ -139,9 +139,8 @@ def gpu():
 @pytest.mark.gpu
 @pytest.mark.parametrize("N,B", [(64, 2), (128, 1), (256, 2), (128, 4), (1024, 1)])
 def test_fno_forward_matches_oracle(gpu, N, B):
-    # B = 1 forms B_x inside the row kernels, B = 2 with k_invx and x-DFT
-    # chunks of 32 rows, B = 4 with chunks of 64 rows; N = 1024 is the
-    # largest grid, whose row kernels hold 146 KiB of LDS
+    # B = 1 forms B_x inside the row kernels, B >= 2 with k_invx; N = 1024
+    # is the largest grid, whose row kernels hold 146 KiB of LDS
     solver, fno = gpu
     chi, v, p = _problem(N, B, 11)
     ctx = solver.Context(N)
@@ -153,6 +152,24 @@ def test_fno_forward_matches_oracle(gpu, N, B):
     err = rel(z - v, zr - v)
     print(f"N={N} B={B} forward rel err of the correction {err:.3e}")
     assert err < 1.5e-2
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("N", [64, 256])
+def test_fno_forward_batch_equals_single_vectors(gpu, N):
+    # one vector (inverse x-DFT inside the row kernels) and a batch (k_invx)
+    # form the same sums in the same order: equal bit for bit
+    solver, fno = gpu
+    B = 3
+    chi, v, p = _problem(N, B, 5)
+    ctx = solver.Context(N)
+    chi_d = torch.from_numpy(chi.reshape(B, -1).astype(np.float32)).cuda()
+    v_d = torch.from_numpy(v.reshape(B, -1).astype(np.complex64)).cuda()
+    zb = fno.Preconditioner(ctx, batch=B, params=p).forward(chi_d, v_d)
+    net1 = fno.Preconditioner(ctx, batch=1, params=p)
+    for k in range(B):
+        z1 = net1.forward(chi_d[k:k + 1].contiguous(), v_d[k:k + 1].contiguous())
+        assert torch.equal(zb[k:k + 1], z1), k
 
 
 @pytest.mark.gpu
