@@ -62,7 +62,7 @@ def parse():
                         "fraction of the residual (0: never)")
     p.add_argument("--plain-only", action="store_true")
     p.add_argument("--batch", type=int, default=4, help="systems per lockstep batch (one step)")
-    p.add_argument("--mode", choices=["bench", "step", "copy", "batched", "fno-step", "copybw", "sweep"],
+    p.add_argument("--mode", choices=["bench", "step", "copy", "batched", "fno-step", "copybw", "sweep", "cycles"],
                    default="bench",
                    help="bench: the benchmark; step: only the per-kernel profile of one Arnoldi "
                         "step (for rocprofv3 --pmc); copy: 3 copies of 256 MiB (counter calibration); "
@@ -71,7 +71,10 @@ def parse():
                         "--pmc); copybw: "
                         "device-copy bandwidth at 8-256 MB; sweep: the reference's own benchmark "
                         "(its cylinder problem at N = 256..2048, Krylov dimension 3/30/49) through "
-                        "cgm_Fast_GMRES_with_CUDA beside its published V100 / GTX 1080 Ti times")
+                        "cgm_Fast_GMRES_with_CUDA beside its published V100 / GTX 1080 Ti times; cycles: "
+                        "3 x (2 restart cycles of cgm_fgmres_batched with a seeded untrained network, then "
+                        "2 of cgm_gmres_batched) on --batch systems (for rocprofv3 kernel traces and "
+                        "counters of whole solver steps)")
     p.add_argument("--sweep-reps", type=int, default=20)
     return p.parse_args()
 
@@ -131,6 +134,28 @@ def main():
         dchi = ctx.to_device(np.stack([c.reshape(-1) for c, _ in cb]), "f32")
         db = ctx.to_device(np.stack([b.reshape(-1) for _, b in cb]))
         print(json.dumps(profile_fno(ctx, net, dchi, db, B)))
+        return
+    if a.mode == "cycles":
+        from cugmres_amd import fno
+        B = a.batch
+        g = ctx.spectrum_helmholtz(problems.KH)
+        cb = [problems.config_plain(N, seed=s_)[:2] for s_ in range(B)]
+        dchi = ctx.to_device(np.stack([c.reshape(-1) for c, _ in cb]), "f32").reshape(B, N * N)
+        db = ctx.to_device(np.stack([b.reshape(-1) for _, b in cb])).reshape(B, N * N)
+        p = fno.init_params(3, zero_out=False)
+        fno.unpack(p)["W_out"][...] *= 0.05
+        net = fno.Preconditioner(ctx, batch=B, params=p)
+        out = {"learned_ms": [], "plain_ms": []}
+        for _ in range(3):
+            t0 = time.perf_counter()
+            ctx.fgmres_batched(g, dchi, db, precond=net, restart=a.restart, tol=1e-9, max_cycles=2)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            ctx.gmres_batched(g, dchi, db, restart=a.restart, tol=1e-9, max_cycles=2)
+            torch.cuda.synchronize()
+            out["learned_ms"].append((t1 - t0) * 1e3)
+            out["plain_ms"].append((time.perf_counter() - t1) * 1e3)
+        print(json.dumps({"cycles": out, "batch": B, "N": N, "restart": a.restart}), flush=True)
         return
     if a.mode == "step":
         chi, b, _ = problems.config_plain(N, seed=0)
@@ -419,11 +444,11 @@ def profile_fno(ctx, net, dchi, db, batch=1):
     # stand-alone row projection runs only in the backward pass.  Algorithmic
     # bytes per launch: BF16 activations 64 B per cell, complex64 vectors 8 B,
     # chi 4 B; FP32 row projections 4 KiB per grid row, x-DFT partials
-    # 128 KiB per 32 rows, spectral weights 4 MiB per layer, mixed modes
+    # 128 KiB per 64 rows, spectral weights 4 MiB per layer, mixed modes
     # 128 KiB per vector, row twiddles 256 B per row, BF16 per-row mixing
     # matrices 2 KiB per row
     # (per vector; the spectral weights once per launch)
-    part = (N // 32) * 16384 * 8
+    part = (N // 64) * 16384 * 8   # x-DFT partials: one per 64 rows
     Bv = batch
     bx = 128 * 1024 if batch == 1 else N * 2048   # B_x source per vector: modes (folded) or k_invx's rows
     bytes_ = {"fno_lift": Bv * (n * (4 + 8) + N * 4096), "fno_rowproj": Bv * (n * 64 + N * 4096),
