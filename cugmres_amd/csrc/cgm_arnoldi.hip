@@ -158,12 +158,14 @@ __global__ void __launch_bounds__(BS) k_arn_a(FusedArgs a) {
     for (int k = 0; k < EPT; ++k)
       trow[k] = row < N ? a.T[tpos(row, t + k * NT, N)] : make_float2(0.f, 0.f);
   }
-  float2 upre[EPD];
+  float2 upre[EPD], xpre[EPD];  // u_j and (flexible steps) the matvec input z_j
 #pragma unroll
   for (int k = 0; k < EPD; ++k) {
     const int e = tid + k * BS;
     const int r = e / N, c = e - r * N;
-    upre[k] = (r < R && r0 + r < N) ? a.Uj[(long)(r0 + r) * N + c] : make_float2(0.f, 0.f);
+    const bool in = r < R && r0 + r < N;
+    upre[k] = in ? a.Uj[(long)(r0 + r) * N + c] : make_float2(0.f, 0.f);
+    xpre[k] = in && a.flex && !tail ? a.Xin[(long)(r0 + r) * N + c] : upre[k];
   }
   if (dn) return;
   if (!tail) {
@@ -189,7 +191,7 @@ __global__ void __launch_bounds__(BS) k_arn_a(FusedArgs a) {
       const float2 u = upre[k];
       uu[k] = u;
       if (!tail) {
-        const float2 xin = a.flex ? a.Xin[id] : u;
+        const float2 xin = xpre[k];
         float2 tv = smem[r * ML + lpad<float>(c)];
         tv.x *= a.inv_mm;
         tv.y *= a.inv_mm;
@@ -562,13 +564,13 @@ __global__ void __launch_bounds__(BS) k_arn_b(FusedArgs a) {
       a.Unext[id] = un;
       uu2 += un.x * un.x + un.y * un.y;
       w = un;
-      if (a.mode == kModeForward) {
+      if (a.mode == kModeForward && !a.flex) {  // flexible steps transform z_j, not u_{j+1}
         const float ch = a.chi[id];
         w.x *= ch;
         w.y *= ch;
       }
     }
-    if (r < R) smem[r * LdsLen<float, M>::value + lpad<float>(c)] = w;
+    if (r < R && !a.flex) smem[r * LdsLen<float, M>::value + lpad<float>(c)] = w;
   }
   if (a.flex) {
     // ||u_{j+1}||: block partial, the last block sums them in a fixed order
