@@ -97,8 +97,9 @@ int cgm_get_tiles(cgm_ctx_t ctx, int* tiles, float* us);
 /* Arnoldi kernels of the solvers: 0 (default) = fused steps with lagged CGS2
  * (one global reduction and two basis passes per step; N = 2^k in [64, 2048]
  * or 3 * 2^k in [96, 768], restart <= 64, other cases fall back to 1), with
- * 256-thread work-groups once a launch covers batch * N >= 2048 grid rows and
- * 1,024-thread ones below (the two shapes sum in different orders, so a
+ * 256-thread work-groups once a launch has at least 512 work-groups of 1024 / N
+ * rows (batch * N * N >= 2^19) and 1,024-thread ones below (the two shapes
+ * sum in different orders, so a
  * batched system and its single solve agree to rounding, bitwise under path
  * 2 or 3);
  * 1 = separate kernels with two-pass CGS2 (three basis passes per step);
