@@ -234,8 +234,8 @@ extern "C" int cgm_set_arnoldi_path(cgm_ctx_t c, int path) {
 // a work-group 1024 / N rows, so a launch has B N^2 / 1024 work-groups;
 // path 0 takes the 256-thread shape once that is at least 512 (two per CU,
 // B N^2 >= 2^19), where its fuller waves pay (per system-step: N = 512,
-// 4 systems 28.7 against 36.2 us; 16 systems at N = 192 / 256 / 384 / 512:
-// 5.6 / 6.7 / 20.3 / 25.4 against 7.3 / 8.9 / 23.5 / 34.8), and the
+// 4 systems 28.7 against 36.2 us; 16 systems at N = 192 / 256 / 512:
+// 5.7 / 6.8 / 25.4 against 7.3 / 8.9 / 34.8), and the
 // 1,024-thread shape below, whose 16 waves per work-group hide more latency
 // when there are fewer work-groups (one system at N = 512: 54.6 against
 // 70.4 us; 16 systems at N = 128: 3.05 against 3.46 us).
