@@ -678,11 +678,11 @@ __global__ void __launch_bounds__(512) k_lift_row(LiftIn L, float2* vw, bf16* __
 //   3. the row projection of h' into Anext[x], with the basis staged in LDS.
 // With LIFT (layer 0) h[y] is not read from P: the row is lifted from (chi,
 // v) into LDS as [y][S] first (lift_row_to_lds, the rounding of k_lift_row),
-// 12 bytes per cell instead of 64; the tiles read their cells from there and
-// write h' back in place (each wave reads its own tiles' cells before it
-// overwrites them), and the row leaves in one coalesced copy.  (Lifting in
-// registers per tile instead holds the 32 lift weights of a lane through the
-// tile loop: 170 VGPRs, one work-group per CU.)
+// 12 bytes per cell instead of 64, where the basis goes otherwise; the tiles
+// read their cells from there, and the basis replaces the row after the
+// tile loop.  (Lifting in registers per tile instead holds the 32 lift
+// weights of a lane through the tile loop: 170 VGPRs, one work-group per CU;
+// so does holding the basis in registers through the loop: 142 VGPRs.)
 template <bool FOLD, bool LIFT>
 __global__ void __launch_bounds__(512) k_cell_row(const bf16* __restrict__ P, LiftIn L, const bf16* __restrict__ Q,
                                                   const bf16* __restrict__ A, const bf16* __restrict__ R,
@@ -692,8 +692,11 @@ __global__ void __launch_bounds__(512) k_cell_row(const bf16* __restrict__ P, Li
                                                   float* __restrict__ Anext, int N, const int* done) {
   if (skip(done)) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  bf16* lbas = reinterpret_cast<bf16*>(smem_raw);  // the basis F^T, row stride basis_ld(N)
-  bf16* lt = lbas + C * basis_ld(N);               // h' channel-major (LIFT: the row as [y][S])
+  // region 1: the basis F^T (row stride basis_ld(N)); with LIFT first the
+  // lifted row h0 as [y][S], the basis after the tiles have read it
+  bf16* lbas = reinterpret_cast<bf16*>(smem_raw);
+  bf16* lrow = lbas;
+  bf16* lt = lbas + (N * S > C * basis_ld(N) ? N * S : C * basis_ld(N));  // region 2: h' channel-major
   __shared__ float red[4 * 64 * 4];
   __shared__ __align__(16) bf16 lb[C][C + 8];  // B_x as [o][k], rows padded to 80 B
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -713,8 +716,10 @@ __global__ void __launch_bounds__(512) k_cell_row(const bf16* __restrict__ P, Li
       bqt[u] = ld8(Q + (long)y * C + 8 * g);
     }
   }
-  basis_to_lds<512>(Xt, lbas, N);
-  if (LIFT) lift_row_to_lds<512, false>(L, ks, xr, N, lt, nullptr, nullptr);
+  if (LIFT)
+    lift_row_to_lds<512, false>(L, ks, xr, N, lrow, nullptr, nullptr);
+  else
+    basis_to_lds<512>(Xt, lbas, N);
   if (!FOLD) {
     if (tid < C * C / 8) {  // 128 threads x 8 BF16
       const int o = tid >> 2, k8 = (tid & 3) * 8;
@@ -785,7 +790,7 @@ __global__ void __launch_bounds__(512) k_cell_row(const bf16* __restrict__ P, Li
       for (int u = 0; u < 4; ++u) {
         const int t = w + 8 * (k0 + u);
         if (t < ntile) {
-          const u32x2* sp = reinterpret_cast<const u32x2*>(lt + (16 * t + i) * S + 8 * g);
+          const u32x2* sp = reinterpret_cast<const u32x2*>(lrow + (16 * t + i) * S + 8 * g);
           const u32x2 lo = sp[0], hi = sp[1];
           bpt[u] = __builtin_bit_cast(bf16x8, u32x4{lo[0], lo[1], hi[0], hi[1]});
         }
@@ -808,26 +813,18 @@ __global__ void __launch_bounds__(512) k_cell_row(const bf16* __restrict__ P, Li
         if (pre_out) st4(pre_out + (bx * N + y) * C + oc0, r[0], r[1], r[2], r[3]);
 #pragma unroll
         for (int v = 0; v < 4; ++v) r[v] = gelu(r[v]);
-        if (LIFT) {
-          st4(lt + y * S + oc0, r[0], r[1], r[2], r[3]);
-        } else {
-          st4(out + (bx * N + y) * C + oc0, r[0], r[1], r[2], r[3]);
+        st4(out + (bx * N + y) * C + oc0, r[0], r[1], r[2], r[3]);
 #pragma unroll
-          for (int v = 0; v < 4; ++v) lt[(oc0 + v) * basis_ld(N) + y] = (bf16)r[v];
-        }
+        for (int v = 0; v < 4; ++v) lt[(oc0 + v) * basis_ld(N) + y] = (bf16)r[v];
       }
     }
   }
   __syncthreads();
-  if (LIFT) {
-    for (int q = tid; q < N * 4; q += 512) {
-      const int y = q >> 2, c8 = (q & 3) * 8;
-      const u32x2* sp = reinterpret_cast<const u32x2*>(lt + y * S + c8);
-      const u32x2 lo = sp[0], hi = sp[1];
-      *(u32x4*)(out + (bx * N + y) * C + c8) = u32x4{lo[0], lo[1], hi[0], hi[1]};
-    }
+  if (LIFT) {  // the lifted row is dead: the basis takes its place
+    basis_to_lds<512>(Xt, lbas, N);
+    __syncthreads();
   }
-  row_project<8, !LIFT>(lbas, basis_ld(N), lt, N, Anext + bx * C * C, red);
+  row_project<8, true>(lbas, basis_ld(N), lt, N, Anext + bx * C * C, red);
 }
 
 // Layer 3 (no activation) fused with the projection: per cell
