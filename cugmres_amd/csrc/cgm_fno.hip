@@ -497,11 +497,30 @@ inline size_t row_kernel_lds(int N) {
   return t + (t > r ? t : r);
 }
 
-// F^T (32 x N, global) to LDS with row stride basis_ld(N), 16 bytes per item.
+// F^T (32 x N, global) to LDS with row stride basis_ld(N), 16 bytes per item,
+// four items per thread in flight together (the loads are unconditional, on
+// a clamped item, so that no branch waits for one load before the next is
+// issued).
 template <int NT>
 __device__ inline void basis_to_lds(const bf16* __restrict__ Xt, bf16* lb, int N) {
-  const int per_row = N / 8;
-  for (int q = threadIdx.x; q < C * per_row; q += NT) {
+  const int per_row = N / 8, items = C * per_row;
+  u32x4 v[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    int q = threadIdx.x + k * NT;
+    q = q < items ? q : items - 1;
+    const int a = q / per_row, y8 = (q - a * per_row) * 8;
+    v[k] = *reinterpret_cast<const u32x4*>(Xt + (long)a * N + y8);
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int q = threadIdx.x + k * NT;
+    if (q < items) {
+      const int a = q / per_row, y8 = (q - a * per_row) * 8;
+      *reinterpret_cast<u32x4*>(lb + a * basis_ld(N) + y8) = v[k];
+    }
+  }
+  for (int q = threadIdx.x + 4 * NT; q < items; q += NT) {  // N > 16 NT / C only
     const int a = q / per_row, y8 = (q - a * per_row) * 8;
     *reinterpret_cast<u32x4*>(lb + a * basis_ld(N) + y8) = *reinterpret_cast<const u32x4*>(Xt + (long)a * N + y8);
   }
@@ -705,26 +724,40 @@ __global__ void __launch_bounds__(512) k_cell_row(const bf16* __restrict__ P, Li
   const int xr = (int)(bx - ks * N);
   const int i = lane & 15, g = lane >> 4;
   const int ntile = N / 16;
-  // the first tiles' cell operands are in flight while B_x is formed
+  // the first tiles' cell operands and B_x are in flight while the basis
+  // (or the lifted row) is staged; the loads are unconditional (tiles past
+  // the row's end re-read its last tile, unused), so that no branch waits for
+  // one load before the next is issued
+  // the layer's weights and bias: loaded first, except with FOLD, whose
+  // inverse x-DFT holds 64 registers of modes until B_x is formed
+  bf16x8 aw[2];
+  float bs[2][4];
+  auto load_weights = [&]() {
+#pragma unroll
+    for (int to = 0; to < 2; ++to) {
+      aw[to] = ld8(A + (16 * to + i) * C + 8 * g);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) bs[to][v] = bias[16 * to + 4 * g + v];
+    }
+  };
+  if (!FOLD) load_weights();
   bf16x8 bpt[4], bqt[4];
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
-    const int t = w + 8 * u;
-    if (t < ntile) {
-      const int y = 16 * t + i;
-      if (!LIFT) bpt[u] = ld8(P + (bx * N + y) * C + 8 * g);
-      bqt[u] = ld8(Q + (long)y * C + 8 * g);
-    }
+    int t = w + 8 * u;
+    t = t < ntile ? t : ntile - 1;
+    const int y = 16 * t + i;
+    if (!LIFT) bpt[u] = ld8(P + (bx * N + y) * C + 8 * g);
+    bqt[u] = ld8(Q + (long)y * C + 8 * g);
   }
+  u32x4 bxv;  // 128 threads x 8 BF16 of B_x (the other threads read a copy)
+  if (!FOLD) bxv = *reinterpret_cast<const u32x4*>(R + (bx * C + ((tid >> 2) & 31)) * C + (tid & 3) * 8);
   if (LIFT)
     lift_row_to_lds<512, false>(L, ks, xr, N, lrow, nullptr, nullptr);
   else
     basis_to_lds<512>(Xt, lbas, N);
   if (!FOLD) {
-    if (tid < C * C / 8) {  // 128 threads x 8 BF16
-      const int o = tid >> 2, k8 = (tid & 3) * 8;
-      *reinterpret_cast<u32x4*>(&lb[o][k8]) = *reinterpret_cast<const u32x4*>(R + (bx * C + o) * C + k8);
-    }
+    if (tid < C * C / 8) *reinterpret_cast<u32x4*>(&lb[tid >> 2][(tid & 3) * 8]) = bxv;
   } else {
     // the sums and their order of k_invx (rows x and N - x from the
     // twiddles of the pair's first row p), so one sample gets the B_x that
@@ -760,14 +793,7 @@ __global__ void __launch_bounds__(512) k_cell_row(const bf16* __restrict__ P, Li
     }
     lb[o][2 * ky] = (bf16)bre;
     lb[o][2 * ky + 1] = (bf16)bim;
-  }
-  bf16x8 aw[2];
-  float bs[2][4];
-#pragma unroll
-  for (int to = 0; to < 2; ++to) {
-    aw[to] = ld8(A + (16 * to + i) * C + 8 * g);
-#pragma unroll
-    for (int v = 0; v < 4; ++v) bs[to][v] = bias[16 * to + 4 * g + v];
+    load_weights();
   }
   __syncthreads();
   bf16x8 ar[2];
@@ -777,12 +803,11 @@ __global__ void __launch_bounds__(512) k_cell_row(const bf16* __restrict__ P, Li
     if (k0 > 0) {
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        const int t = w + 8 * (k0 + u);
-        if (t < ntile) {
-          const int y = 16 * t + i;
-          if (!LIFT) bpt[u] = ld8(P + (bx * N + y) * C + 8 * g);
-          bqt[u] = ld8(Q + (long)y * C + 8 * g);
-        }
+        int t = w + 8 * (k0 + u);
+        t = t < ntile ? t : ntile - 1;
+        const int y = 16 * t + i;
+        if (!LIFT) bpt[u] = ld8(P + (bx * N + y) * C + 8 * g);
+        bqt[u] = ld8(Q + (long)y * C + 8 * g);
       }
     }
     if (LIFT) {
