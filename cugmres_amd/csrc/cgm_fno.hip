@@ -108,6 +108,8 @@ __device__ inline float dgelu(float x) {
 
 __device__ inline bool skip(const int* done) { return done && *done; }
 
+__device__ const double kOne = 1.0;  // the norm of a vector that needs no scaling
+
 // ---------------------------------------------------------------------------
 // Row projection (one grid row x of one sample per iteration):
 //   out[r][a][c] = sum_y Xt[a][y] Y[r][y][c]                  (Xt: 32 x N basis)
@@ -874,7 +876,19 @@ __global__ void __launch_bounds__(256) k_cell_proj(const bf16* __restrict__ P, c
   const long rin = (bx - ks * N) * N;     // the row's first cell in the sample
   vv += ks * sV;
   z += ks * sZ;
-  if (vnrm) vnrm = boff(vnrm, ks * sNrm);
+  // every load of the wave is issued before the first use of any: the cell
+  // operands, the vector's values, the weights, then the vector's norm (read
+  // through a pointer that is never null, so that no branch waits for the
+  // loads before it)
+  bf16x8 bpt[4], bqt[4];
+  float2 vx[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int y = y0 + 16 * t + i;
+    bpt[t] = ld8(P + (bx * N + y) * C + 8 * g);
+    bqt[t] = ld8(Q + (long)y * C + 8 * g);
+    vx[t] = vv[rin + y];
+  }
   bf16x8 aw[2], ar[2];
   float bs[2][4], wo[2][4][2];
   const float* Wout = Pp + kOffOut;
@@ -890,19 +904,9 @@ __global__ void __launch_bounds__(256) k_cell_proj(const bf16* __restrict__ P, c
       wo[to][v][1] = Wout[2 * o + 1];
     }
   }
-  float sid = Pp[kOffSkip];
-  if (vnrm) {
-    const double d = *vnrm;
-    sid *= d > 0.0 ? (float)(1.0 / d) : 0.f;
-  }
+  const double d = *(vnrm ? boff(vnrm, ks * sNrm) : &kOne);  // no norm: 1
+  const float sid = Pp[kOffSkip] * (d > 0.0 ? (float)(1.0 / d) : 0.f);
   const float b0 = Wout[2 * C], b1 = Wout[2 * C + 1], is = 1.f / s;
-  bf16x8 bpt[4], bqt[4];
-#pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    const int y = y0 + 16 * t + i;
-    bpt[t] = ld8(P + (bx * N + y) * C + 8 * g);
-    bqt[t] = ld8(Q + (long)y * C + 8 * g);
-  }
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
     const long p = bx * N + y0 + 16 * t + i;
@@ -927,9 +931,8 @@ __global__ void __launch_bounds__(256) k_cell_proj(const bf16* __restrict__ P, c
     o0 += __shfl_xor(o0, 32, 64);
     o1 += __shfl_xor(o1, 32, 64);
     if (g == 0) {
-      const long pin = rin + y0 + 16 * t + i;
-      const float2 x = vv[pin];
-      z[pin] = make_float2(sid * x.x + (o0 + b0) * is, sid * x.y + (o1 + b1) * is);
+      const float2 x = vx[t];
+      z[rin + y0 + 16 * t + i] = make_float2(sid * x.x + (o0 + b0) * is, sid * x.y + (o1 + b1) * is);
     }
   }
 }
