@@ -616,7 +616,8 @@ __device__ inline bf16x8 lift8(float a0, float a1, float a2, const float* w) {
 // [c][y] with row stride basis_ld(N) (TR) or [y][S], with (a0, a1, a2) =
 // (chi chi_scale, s Re v', s Im v'), v' = v / *nrm: one item q = (cell y,
 // quarter) of 8 channels per thread and step (the four items of a cell are
-// adjacent lanes, so an in-place vw == v is read before it is written).
+// adjacent lanes of one load, and a thread's loads all precede its stores,
+// so an in-place vw == v is read before it is written).
 // With h0 the row also goes to global memory (16-byte stores), with vw v'
 // goes back to vw.
 template <int NT, bool TR>
@@ -641,25 +642,40 @@ __device__ inline void lift_row_to_lds(const LiftIn& L, long ks, int xr, int N, 
 #pragma unroll
     for (int j = 0; j < 4; ++j) w[8 * j + k] = L.P[j * C + c8 + k];
   }
-  for (int q = threadIdx.x; q < N * 4; q += NT) {
-    const int y = q >> 2;
-    const long pin = rin + y;
-    float2 vv = v[pin];
-    if (L.nrm) {
-      vv.x *= r;
-      vv.y *= r;
-      if (vw && c8 == 0) vw[pin] = vv;
-    }
-    const bf16x8 hq = lift8(chi[pin] * L.chi_scale, vv.x * L.s, vv.y * L.s, w);
-    if (h0) *reinterpret_cast<u32x4*>(h0 + ((ks * N + xr) * N + y) * C + c8) = __builtin_bit_cast(u32x4, hq);
-    if (TR) {
+  // four items per thread at a time: their loads first (unconditional, on
+  // clamped items), then the lift and the stores
+  for (int q0 = threadIdx.x; q0 < N * 4; q0 += 4 * NT) {
+    float cv[4];
+    float2 xv[4];
 #pragma unroll
-      for (int k = 0; k < 8; ++k) lt[(c8 + k) * basis_ld(N) + y] = hq[k];
-    } else {
-      const u32x4 bits = __builtin_bit_cast(u32x4, hq);
-      u32x2* d = reinterpret_cast<u32x2*>(lt + y * S + c8);
-      d[0] = u32x2{bits[0], bits[1]};
-      d[1] = u32x2{bits[2], bits[3]};
+    for (int u = 0; u < 4; ++u) {
+      const int q = q0 + u * NT < N * 4 ? q0 + u * NT : N * 4 - 1;
+      cv[u] = chi[rin + (q >> 2)];
+      xv[u] = v[rin + (q >> 2)];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int q = q0 + u * NT;
+      if (q >= N * 4) break;
+      const int y = q >> 2;
+      const long pin = rin + y;
+      float2 vv = xv[u];
+      if (L.nrm) {
+        vv.x *= r;
+        vv.y *= r;
+        if (vw && c8 == 0) vw[pin] = vv;
+      }
+      const bf16x8 hq = lift8(cv[u] * L.chi_scale, vv.x * L.s, vv.y * L.s, w);
+      if (h0) *reinterpret_cast<u32x4*>(h0 + ((ks * N + xr) * N + y) * C + c8) = __builtin_bit_cast(u32x4, hq);
+      if (TR) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) lt[(c8 + k) * basis_ld(N) + y] = hq[k];
+      } else {
+        const u32x4 bits = __builtin_bit_cast(u32x4, hq);
+        u32x2* d = reinterpret_cast<u32x2*>(lt + y * S + c8);
+        d[0] = u32x2{bits[0], bits[1]};
+        d[1] = u32x2{bits[2], bits[3]};
+      }
     }
   }
 }
