@@ -158,14 +158,18 @@ __global__ void __launch_bounds__(BS) k_arn_a(FusedArgs a) {
     for (int k = 0; k < EPT; ++k)
       trow[k] = row < N ? a.T[tpos(row, t + k * NT, N)] : make_float2(0.f, 0.f);
   }
-  float2 upre[EPD], xpre[EPD];  // u_j and (flexible steps) the matvec input z_j
+  // u_j and the matvec input (z_j for flexible steps, else u_j again: a
+  // second independent load rather than a select that waits for the first)
+  const float2* xsrc = a.flex && !tail ? a.Xin : a.Uj;
+  float2 upre[EPD], xpre[EPD];
 #pragma unroll
   for (int k = 0; k < EPD; ++k) {
     const int e = tid + k * BS;
     const int r = e / N, c = e - r * N;
     const bool in = r < R && r0 + r < N;
-    upre[k] = in ? a.Uj[(long)(r0 + r) * N + c] : make_float2(0.f, 0.f);
-    xpre[k] = in && a.flex && !tail ? a.Xin[(long)(r0 + r) * N + c] : upre[k];
+    const long id = in ? (long)(r0 + r) * N + c : 0;
+    upre[k] = in ? a.Uj[id] : make_float2(0.f, 0.f);
+    xpre[k] = in ? xsrc[id] : make_float2(0.f, 0.f);
   }
   if (dn) return;
   if (!tail) {
