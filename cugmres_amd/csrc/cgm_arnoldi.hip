@@ -478,7 +478,7 @@ __global__ void __launch_bounds__(BS) k_arn_a(FusedArgs a) {
 template <int M, int EPD, int BS>
 __global__ void __launch_bounds__(BS) k_arn_b(FusedArgs a) {
   batch_offset(a);
-  const int dn = a.st->done;  // tested once this block's u and z loads are in flight
+  const int dn = a.st->done;  // tested once this block's u, z and basis loads are issued
   constexpr int NT = FftShape<M>::NT;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   float2* smem = reinterpret_cast<float2*>(smem_raw);
@@ -505,7 +505,6 @@ __global__ void __launch_bounds__(BS) k_arn_b(FusedArgs a) {
     up[k] = live ? a.Uj[id] : make_float2(0.f, 0.f);
     zp[k] = live ? a.Z[id] : make_float2(0.f, 0.f);
   }
-  if (dn) return;
   // Q_j a1 and Q_j a2 for the thread's EPD elements, QC basis vectors at a
   // time for all elements (QC * EPD loads in flight)
   constexpr int QC = EPD == 1 ? 4 : 2;
@@ -553,6 +552,10 @@ __global__ void __launch_bounds__(BS) k_arn_b(FusedArgs a) {
       }
     }
   }
+  // the done test comes after the basis loads (a finished system's block
+  // reads in-bounds stale columns and writes nothing), so that waiting for
+  // it does not also wait for u and z before the first basis load is issued
+  if (dn) return;
 #pragma unroll
   for (int k = 0; k < EPD; ++k) {
     const int e = tid + k * BS;
