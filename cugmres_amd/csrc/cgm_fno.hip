@@ -325,8 +325,8 @@ __global__ void __launch_bounds__(256) k_mix(const float2* __restrict__ upart, i
 // are 1 and (-1)^kx = (-1)^kxi (N even, kx = kxi or N - 32 + kxi).  The
 // block's 2 XP x 32 x 32 outputs are assembled in LDS and leave as 16-byte
 // stores of whole 2 KiB rows.  Every block reads its sample's 128 KiB of
-// modes: XP = 4 for batches (fewer reads), XP = 1 for one vector (enough
-// blocks).
+// modes: XP = 8 from 8 vectors and XP = 4 from 4 (fewer reads), XP = 1 for
+// fewer (enough blocks).
 template <int XP>
 __global__ void __launch_bounds__(512) k_invx(const float2* __restrict__ V, bf16* __restrict__ out,
                                               const float2* __restrict__ Ex, int N, const int* done) {
@@ -1135,7 +1135,9 @@ int launch_pixel(cgm_fno* f, hipStream_t st, const bf16* P, const bf16* A, const
 }
 
 void launch_invx(cgm_fno* f, hipStream_t st, int B, const int* done) {
-  if (B >= 4)
+  if (B >= 8)  // every work-group reads its vector's 128 KiB of modes: fewer, larger ones
+    hipLaunchKernelGGL(k_invx<8>, dim3(f->N / 16, 1, B), dim3(512), 0, st, f->Vm, f->Bt, f->Ex, f->N, done);
+  else if (B >= 4)
     hipLaunchKernelGGL(k_invx<4>, dim3(f->N / 8, 1, B), dim3(512), 0, st, f->Vm, f->Bt, f->Ex, f->N, done);
   else
     hipLaunchKernelGGL(k_invx<1>, dim3(f->N / 2, 1, B), dim3(512), 0, st, f->Vm, f->Bt, f->Ex, f->N, done);
