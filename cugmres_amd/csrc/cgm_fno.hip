@@ -528,13 +528,12 @@ __device__ inline void basis_to_lds(const bf16* __restrict__ Xt, bf16* lb, int N
   }
 }
 
-// A_x = F^T H_x from the row staged in LDS, channel-major (TR: lt[c][y], row
-// stride basis_ld(N); the B operand is one 16-byte read) or as [y][S] (eight
-// 2-byte reads), F^T read with row stride ldx (from global memory or LDS).
-// NW waves: wave w owns output tile (a-tile (w >> 1) & 1, c-tile w & 1);
-// with NW = 8 waves w and w + 4 split K in halves and the upper half adds
-// into the lower through red.
-template <int NW, bool TR>
+// A_x = F^T H_x from the row staged channel-major in LDS (lt[c][y], row
+// stride basis_ld(N): the B operand is one 16-byte read), F^T read with row
+// stride ldx (from global memory or LDS).  NW waves: wave w owns output tile
+// (a-tile (w >> 1) & 1, c-tile w & 1); with NW = 8 waves w and w + 4 split K
+// in halves and the upper half adds into the lower through red.
+template <int NW>
 __device__ inline void row_project(const bf16* __restrict__ Xt, int ldx, const bf16* lt, int N,
                                    float* __restrict__ o, float* red) {
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -556,17 +555,8 @@ __device__ inline void row_project(const bf16* __restrict__ Xt, int ldx, const b
     for (int u = 0; u < 4; ++u)
       if (s0 + u < steps) a[u] = ld8(xa + 32 * (s0 + u));
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      if (s0 + u < steps) {
-        if (TR) {
-          b[u] = ld8(xb + 32 * (s0 + u));
-        } else {
-          const int yb = ylo + 32 * (s0 + u) + 8 * g;
-#pragma unroll
-          for (int v = 0; v < 8; ++v) b[u][v] = lt[(yb + v) * S + 16 * tc + i];
-        }
-      }
-    }
+    for (int u = 0; u < 4; ++u)
+      if (s0 + u < steps) b[u] = ld8(xb + 32 * (s0 + u));
 #pragma unroll
     for (int u = 0; u < 4; ++u)
       if (s0 + u < steps) acc = mma(a[u], b[u], acc);
@@ -698,7 +688,7 @@ __global__ void __launch_bounds__(512) k_lift_row(LiftIn L, float2* vw, bf16* __
   basis_to_lds<512>(Xt, lbas, N);
   lift_row_to_lds<512, true>(L, ks, (int)(bx - ks * N), N, lt, h0, vw);
   __syncthreads();
-  row_project<8, true>(lbas, basis_ld(N), lt, N, Aout + bx * C * C, red);
+  row_project<8>(lbas, basis_ld(N), lt, N, Aout + bx * C * C, red);
 }
 
 // One block (8 waves) per grid row x of sample k (layers 0-2):
@@ -867,7 +857,7 @@ __global__ void __launch_bounds__(512) k_cell_row(const bf16* __restrict__ P, Li
     basis_to_lds<512>(Xt, lbas, N);
     __syncthreads();
   }
-  row_project<8, true>(lbas, basis_ld(N), lt, N, Anext + bx * C * C, red);
+  row_project<8>(lbas, basis_ld(N), lt, N, Anext + bx * C * C, red);
 }
 
 // Layer 3 (no activation) fused with the projection: per cell
