@@ -990,7 +990,7 @@ extern "C" int cgm_fgmres(cgm_ctx_t c, const cgm_complex64* gamma, const float* 
     }
     if (h.bnorm == 0.0 || h.beta <= p->tol * h.bnorm) { res->converged = 1; break; }
     if (cycles >= p->max_cycles) break;
-    // the true residual judges the preconditioner: a cycle that did not cut
+    // the true residual decides on the preconditioner: a cycle that did not cut
     // it below stall_ratio hands the rest of the solve to plain GMRES
     if (precond && last_precond && p->stall_ratio > 0.0 && h.beta > p->stall_ratio * beta_prev) {
       precond = false;

@@ -259,7 +259,7 @@ k_fft_cols(cplx<T>* __restrict__ data, int cols_per_wg, const cplx<T>* __restric
 // Green's-function generators (FP64)
 // ---------------------------------------------------------------------------
 // Block g[i, j] (i, j < N) of the BTTB generator.
-//  kind 0 (north-star operator, k0^2 G): off-diagonal kh^2 (i/4) H0(kh rho),
+//  kind 0 (the benchmark operator, k0^2 G): off-diagonal kh^2 (i/4) H0(kh rho),
 //         self term (i pi ka/2) H1(ka) - 1, ka = kh / sqrt(pi).
 //  kind 1 (reference operator, B/MKL_GMRES_test_09122019/source/loop_functions.h:139-142):
 //         (i/4) H0(k sqrt((i - sr)^2 + (j - sc)^2)) for every (i, j), the

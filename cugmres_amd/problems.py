@@ -1,4 +1,4 @@
-"""Seeded problem generators for the north-star workloads (host side, numpy).
+"""Seeded problem generators for the benchmark workloads (host side, numpy).
 
 The reference builds one fixed problem (an eps = 2.25 disk, x0 = b, plane wave
 along the row index; B/CUDA_GMRES/GMRES_kernels.cuh:12-19,557-572).  The

@@ -125,7 +125,7 @@ int cgm_spectrum_from_block(cgm_ctx_t ctx, const cgm_complex128* dev_block,
  * col_shift = 0 is the C++ orientation, the reverse the Python one. */
 int cgm_spectrum_reference(cgm_ctx_t ctx, double wave_number, double row_shift, double col_shift,
                            cgm_complex64* dev_gamma, cgm_complex128* dev_gamma64);
-/* The north-star operator k0^2 G for cells of size h, kh = k0 h:
+/* The benchmark operator k0^2 G for cells of size h, kh = k0 h:
  * off-diagonal kh^2 (i/4) H0(kh |r_p - r_q|/h), self term (i pi ka/2) H1(ka) - 1,
  * ka = kh / sqrt(pi).  Either output may be NULL. */
 int cgm_spectrum_helmholtz(cgm_ctx_t ctx, double kh, cgm_complex64* dev_gamma,
@@ -276,7 +276,7 @@ typedef struct {
  * V, x += Z y.  For power-of-two N in [64, 2048] and m <= 64 the network takes
  * the lagged once-orthogonalised vector and each step has one reduction
  * (fused lagged CGS2); otherwise two-pass CGS2.  Plain GMRES (cgm_gmres's
- * kernels) is the fallback and the judge: every cycle starts from the true
+ * kernels) is the fallback and the referee: every cycle starts from the true
  * residual b - A x.  net = NULL applies the identity (then the iterates equal
  * cgm_gmres's).  dev_hist as in cgm_gmres.  No counterpart in the reference,
  * which has no preconditioner (B/CUDA_GMRES/GMRES.cuh:22-446). */

@@ -18,7 +18,7 @@
  *   - standard restarted GMRES(m) with CGS2, unitary Givens and back
  *     substitution, and FP64 iterative refinement: textbook algorithms (Saad,
  *     "Iterative Methods for Sparse Linear Systems", 2nd ed., Alg. 6.9/6.11;
- *     Giraud et al. 2005 for CGS2) that the north-star operator requires.
+ *     Giraud et al. 2005 for CGS2) that the volume-integral operator requires.
  *
  * Parity pins for every function are listed in oracle/README.md.
  */

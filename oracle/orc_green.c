@@ -57,7 +57,7 @@ void orc_ref_gprep_f32(int N, float *gprep) {
   free(blk);
 }
 
-/* k0^2 G for the north-star operator, FP64.  rho in cells, kh = k0 h. */
+/* k0^2 G for the h^2-weighted Hankel operator of the benchmark, FP64.  rho in cells, kh = k0 h. */
 void orc_baseline_green_block(int N, double kh, double *blk) {
   const double ka = kh / sqrt(ORC_PI);
   for (int i = 0; i < N; ++i)

@@ -1,7 +1,7 @@
 """GPU parity tests: the HIP path through the C ABI against the CPU oracle.
 
 Tolerances: complex64 operator and Krylov vectors (relative L2 2e-6 for one
-matvec, 1e-5 for converged solutions), complex128 paths
+matvec, 1e-5 for converged solutions against the reference solve), complex128 paths
 1e-12 / 1e-10.
 """
 import ctypes
@@ -302,7 +302,7 @@ def test_refinement_fp64(sol, orc):
 
 
 def test_plain_config_full_size_invariants(sol, orc):
-    """N = 512 north-star workload, GMRES(30): true residual is non-increasing
+    """N = 512 benchmark workload, GMRES(30): true residual is non-increasing
     over restarts and equals the oracle's residual of the returned iterate."""
     from cugmres_amd import problems
     N = 512
