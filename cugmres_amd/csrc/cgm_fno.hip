@@ -208,20 +208,35 @@ __global__ void __launch_bounds__(256) k_dftx(const float* __restrict__ in, floa
   const int nch = gridDim.x;
   const int tid = threadIdx.x;
   const int p0 = chunk * XH;
+  // every load of the staging first, then the LDS writes (a load-wait-write
+  // chain per item would expose the load latency once per item)
+  constexpr int NA = XH * C / 256, NE = KX * XH / 256;
+  static_assert(NA * 256 == XH * C && NE * 256 == KX * XH, "staging items must fill the work-group");
+  float2 u1[NA], u2[NA], ev[NE];
 #pragma unroll
-  for (int e = tid; e < XH * C; e += 256) {
-    const int pl = e >> 5, c = e & 31;
+  for (int k = 0; k < NA; ++k) {
+    const int e = tid + 256 * k, pl = e >> 5, c = e & 31;
     const int x = p0 + pl, x2 = x == 0 ? N / 2 : N - x;
     const float* r1 = in + ((long)(b * N + x) * C + 2 * ky) * C;
     const float* r2 = in + ((long)(b * N + x2) * C + 2 * ky) * C;
-    const float2 u1 = make_float2(r1[c], r1[C + c]), u2 = make_float2(r2[c], r2[C + c]);
-    sp[pl][c] = make_float2(u1.x + u2.x, u1.y + u2.y);
-    sm[pl][c] = make_float2(u1.x - u2.x, u1.y - u2.y);
+    u1[k] = make_float2(r1[c], r1[C + c]);
+    u2[k] = make_float2(r2[c], r2[C + c]);
   }
 #pragma unroll
-  for (int e = tid; e < KX * XH; e += 256) {
-    const int q = e / XH, pl = e % XH;
-    se[q][pl] = Ex[(long)q * N + p0 + pl];  // frequency q at row p0 + pl
+  for (int k = 0; k < NE; ++k) {
+    const int e = tid + 256 * k, q = e / XH, pl = e % XH;
+    ev[k] = Ex[(long)q * N + p0 + pl];  // frequency q at row p0 + pl
+  }
+#pragma unroll
+  for (int k = 0; k < NA; ++k) {
+    const int e = tid + 256 * k, pl = e >> 5, c = e & 31;
+    sp[pl][c] = make_float2(u1[k].x + u2[k].x, u1[k].y + u2[k].y);
+    sm[pl][c] = make_float2(u1[k].x - u2[k].x, u1[k].y - u2[k].y);
+  }
+#pragma unroll
+  for (int k = 0; k < NE; ++k) {
+    const int e = tid + 256 * k;
+    se[e / XH][e % XH] = ev[k];
   }
   __syncthreads();
   const int kp = tid >> 4, cp = tid & 15;
@@ -295,11 +310,16 @@ __global__ void __launch_bounds__(256) k_mix(const float2* __restrict__ upart, i
   const float2* Wm = W + (long)(kxi * KY + ky) * C * C;
   float2 v = make_float2(0.f, 0.f);
   if (!BWD) {
-#pragma unroll 8
+    // all 32 weights of this output in flight at once (8 at a time would
+    // expose the load latency four times)
+    float2 wv[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) wv[c] = Wm[c * C + ch];
+#pragma unroll
     for (int c = 0; c < C; ++c) {
-      const float2 uu = su[kl][c], wv = Wm[c * C + ch];
-      v.x += uu.x * wv.x - uu.y * wv.y;
-      v.y += uu.x * wv.y + uu.y * wv.x;
+      const float2 uu = su[kl][c];
+      v.x += uu.x * wv[c].x - uu.y * wv[c].y;
+      v.y += uu.x * wv[c].y + uu.y * wv[c].x;
     }
     const float sc = (ky == 0 ? 1.f : 2.f) / ((float)N * (float)N);
     v.x *= sc;
