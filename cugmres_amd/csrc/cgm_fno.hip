@@ -827,6 +827,23 @@ __global__ void __launch_bounds__(512) k_cell_row(const bf16* __restrict__ P, Li
   bf16x8 ar[2];
 #pragma unroll
   for (int to = 0; to < 2; ++to) ar[to] = *reinterpret_cast<const bf16x8*>(&lb[16 * to + i][8 * g]);
+  // LIFT with at most four tiles per wave (N <= 512): every wave takes its
+  // cells from the lifted row first, then the basis replaces the row before
+  // the tile loop, its load latency hidden by the other waves' GEMMs
+  // (block-uniform branch)
+  const bool early = LIFT && ntile <= 32;
+  if (early) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      int t = w + 8 * u;
+      t = t < ntile ? t : ntile - 1;
+      const u32x2* sp = reinterpret_cast<const u32x2*>(lrow + (16 * t + i) * S + 8 * g);
+      const u32x2 lo = sp[0], hi = sp[1];
+      bpt[u] = __builtin_bit_cast(bf16x8, u32x4{lo[0], lo[1], hi[0], hi[1]});
+    }
+    __syncthreads();
+    basis_to_lds<512>(Xt, lbas, N);
+  }
   for (int k0 = 0; w + 8 * k0 < ntile; k0 += 4) {
     if (k0 > 0) {
 #pragma unroll
@@ -838,7 +855,7 @@ __global__ void __launch_bounds__(512) k_cell_row(const bf16* __restrict__ P, Li
         bqt[u] = ld8(Q + (long)y * C + 8 * g);
       }
     }
-    if (LIFT) {
+    if (LIFT && !early) {
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const int t = w + 8 * (k0 + u);
@@ -873,7 +890,7 @@ __global__ void __launch_bounds__(512) k_cell_row(const bf16* __restrict__ P, Li
     }
   }
   __syncthreads();
-  if (LIFT) {  // the lifted row is dead: the basis takes its place
+  if (LIFT && !early) {  // the lifted row is dead: the basis takes its place
     basis_to_lds<512>(Xt, lbas, N);
     __syncthreads();
   }
