@@ -178,7 +178,7 @@ __global__ void __launch_bounds__(BS) k_arn_a(FusedArgs a) {
       for (int k = 0; k < EPT; ++k) buf[lpad<float>(t + k * NT)] = trow[k];
     }
     __syncthreads();
-    lds_fft<float, M, +1>(buf, tws, t, fa);
+    lds_fft<float, M, +1, false, true>(buf, tws, t, fa);  // columns < N are used
   }
   // ---- z = u - G(.), u and z of the block's R*N elements staged in LDS -------
   // (the FFT rows are no longer needed once every thread has read its values)
@@ -606,7 +606,7 @@ __global__ void __launch_bounds__(BS) k_arn_b(FusedArgs a) {
   }
   __syncthreads();
   float2* buf = smem + rl * LdsLen<float, M>::value;
-  lds_fft<float, M, -1>(buf, tws, t, fa);
+  lds_fft<float, M, -1, true>(buf, tws, t, fa);  // the rows are zero from N <= M/2 on
   if (fa) {
     const int row = r0 + rl;
     if (row < N) {

@@ -130,6 +130,44 @@ __device__ inline void dft(cplx<T>* v) {
   else if constexpr (R == 8) dft8<T, SIGN>(v);
 }
 
+// The same DFTs when the upper half of the inputs, v[R/2 .. R-1], is zero
+// (the first stage of a transform of zero-padded data): v[0 .. R/2-1] are
+// read, all R outputs written.
+template <typename T, int SIGN>
+__device__ inline void dft4_lo(cplx<T>* v) {  // v[2] = v[3] = 0
+  const cplx<T> a = v[0], d = mul_si<T, SIGN>(v[1]);
+  v[2] = csub(a, v[1]);
+  v[0] = cadd(a, v[1]);
+  v[1] = cadd(a, d);
+  v[3] = csub(a, d);
+}
+template <typename T, int SIGN, int R>
+__device__ inline void dft_lo(cplx<T>* v) {
+  if constexpr (R == 2) {
+    v[1] = v[0];
+  } else if constexpr (R == 4) {
+    dft4_lo<T, SIGN>(v);
+  } else {
+    static_assert(R == 8, "half-zero inputs need an even radix");
+    cplx<T> e[4] = {v[0], v[2], v[0], v[0]};  // v[4], v[6] are zero
+    cplx<T> o[4] = {v[1], v[3], v[1], v[1]};  // v[5], v[7] are zero
+    dft4_lo<T, SIGN>(e);
+    dft4_lo<T, SIGN>(o);
+    const T h = T(0.70710678118654752440);
+    cplx<T> o1 = cmul(o[1], mk(h, T(SIGN) * h));
+    cplx<T> o2 = mul_si<T, SIGN>(o[2]);
+    cplx<T> o3 = cmul(o[3], mk(-h, T(SIGN) * h));
+    v[0] = cadd(e[0], o[0]);
+    v[4] = csub(e[0], o[0]);
+    v[1] = cadd(e[1], o1);
+    v[5] = csub(e[1], o1);
+    v[2] = cadd(e[2], o2);
+    v[6] = csub(e[2], o2);
+    v[3] = cadd(e[3], o3);
+    v[7] = csub(e[3], o3);
+  }
+}
+
 // Twiddles of one thread for every stage with span NS > 1, flattened:
 // stage S, butterfly b, r = 1..R-1 at tw_offset<M>(S) + b*(R-1) + r-1.
 template <int M> __host__ __device__ constexpr int fft_stage_span(int S) {
@@ -167,8 +205,11 @@ __device__ inline void fft_load_tw(cplx<T>* reg, const cplx<T>* __restrict__ tw,
   }
 }
 
-// One Stockham stage for the butterflies owned by thread t.
-template <typename T, int M, int SIGN, int R, int NS, int OFF>
+// One Stockham stage for the butterflies owned by thread t.  LO: the inputs
+// j + r M/R with r >= R/2 (the upper half of the data, first stage only)
+// are zero and not read.  HI: only the outputs below M/2 (r < R/2 of the
+// last stage) are wanted and written.
+template <typename T, int M, int SIGN, int R, int NS, int OFF, bool LO = false, bool HI = false>
 __device__ inline void fft_stage(cplx<T>* buf, const cplx<T>* reg, int t, bool active = true) {
   constexpr int EPT = FftShape<M>::EPT;
   constexpr int NT = FftShape<M>::NT;
@@ -178,13 +219,15 @@ __device__ inline void fft_stage(cplx<T>* buf, const cplx<T>* reg, int t, bool a
   // reach the same two barriers: the barriers stay outside any branch, so a
   // wave whose lanes are partly inside the transform (rows * NT not a multiple
   // of 64) still executes exactly two of them.
+  static_assert(!LO || (NS == 1 && R % 2 == 0), "half-zero inputs: first stage, even radix");
+  static_assert(!HI || (NS * R == M && R % 2 == 0), "half outputs: last stage, even radix");
   cplx<T> v[NB][R];
   if (active) {
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       const int j = t + b * NT;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[b][r] = buf[lpad<T>(j + r * MR)];
+      for (int r = 0; r < (LO ? R / 2 : R); ++r) v[b][r] = buf[lpad<T>(j + r * MR)];
     }
   }
   __syncthreads();
@@ -201,21 +244,26 @@ __device__ inline void fft_stage(cplx<T>* buf, const cplx<T>* reg, int t, bool a
           v[b][r] = cmul(v[b][r], w);
         }
       }
-      dft<T, SIGN, R>(v[b]);
+      if constexpr (LO)
+        dft_lo<T, SIGN, R>(v[b]);
+      else
+        dft<T, SIGN, R>(v[b]);
       const int base = (j / NS) * NS * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf[lpad<T>(base + r * NS)] = v[b][r];
+      for (int r = 0; r < (HI ? R / 2 : R); ++r) buf[lpad<T>(base + r * NS)] = v[b][r];
     }
   }
   __syncthreads();
 }
 
-template <typename T, int M, int SIGN, int S, int NS>
+template <typename T, int M, int SIGN, int S, int NS, bool LO, bool HI>
 __device__ inline void fft_stages(cplx<T>* buf, const cplx<T>* reg, int t, bool active) {
   if constexpr (S < fft_num_stages<M>()) {
     constexpr int R = fft_radix<M>(S);
-    fft_stage<T, M, SIGN, R, NS, fft_tw_offset<M>(S)>(buf, reg, t, active);
-    fft_stages<T, M, SIGN, S + 1, NS * R>(buf, reg, t, active);
+    constexpr bool lo = LO && S == 0 && R % 2 == 0;
+    constexpr bool hi = HI && S == fft_num_stages<M>() - 1 && R % 2 == 0;
+    fft_stage<T, M, SIGN, R, NS, fft_tw_offset<M>(S), lo, hi>(buf, reg, t, active);
+    fft_stages<T, M, SIGN, S + 1, NS * R, LO, HI>(buf, reg, t, active);
   }
 }
 
@@ -231,10 +279,14 @@ template <typename T, int M> struct FftTwiddles {
 // Full transform of the M elements of buf (LDS, padded layout lpad<T>) by
 // threads t in [0, NT).  The caller
 // must have synchronised the work-group after filling buf; on return buf
-// holds the result and the group is synchronised.
-template <typename T, int M, int SIGN>
+// holds the result and the group is synchronised.  LO: buf[M/2 ..] is zero
+// (zero-padded data; used when the first radix is even, the upper half is
+// then never read).  HI: only the outputs below M/2 are used (when the last
+// radix is even the others are not written and buf[M/2 ..] is left as it
+// was).
+template <typename T, int M, int SIGN, bool LO = false, bool HI = false>
 __device__ inline void lds_fft(cplx<T>* buf, const FftTwiddles<T, M>& tws, int t, bool active = true) {
-  fft_stages<T, M, SIGN, 0, 1>(buf, tws.reg, t, active);
+  fft_stages<T, M, SIGN, 0, 1, LO, HI>(buf, tws.reg, t, active);
 }
 
 }  // namespace cgm

@@ -89,7 +89,7 @@ k_pass1(const cplx<T>* __restrict__ x, cplx<T>* xw, const T* __restrict__ chi,
     buf[lpad<T>(e)] = mul ? cscale(v, cv[k]) : v;
   }
   __syncthreads();
-  lds_fft<T, M, -1>(buf, tws, t);
+  lds_fft<T, M, -1, true>(buf, tws, t);  // the row is zero from N <= M/2 on
   if (live) {
     for (int e = t; e < M; e += NT) Tm[tpos(row, e, N)] = buf[lpad<T>(e)];
   }
@@ -151,7 +151,7 @@ k_pass2(cplx<T>* __restrict__ Tm, const G* __restrict__ gq, int N, int cols_per_
   const int t = threadIdx.x % NT;
   const int cl = threadIdx.x / NT;
   cplx<T>* buf = smem + cl * ML;
-  lds_fft<T, M, -1>(buf, tws, t);
+  lds_fft<T, M, -1, true>(buf, tws, t);  // rows N.. are zero
   const int col = c0 + cl;
   const int cq = col <= M - col ? col : M - col;
   const G* grow = gq + (long)cq * H;
@@ -161,7 +161,7 @@ k_pass2(cplx<T>* __restrict__ Tm, const G* __restrict__ gq, int N, int cols_per_
     buf[lpad<T>(e)] = cmul(buf[lpad<T>(e)], mk(T(g.x), T(g.y)));
   }
   __syncthreads();
-  lds_fft<T, M, +1>(buf, tws, t);
+  lds_fft<T, M, +1, false, true>(buf, tws, t);  // rows < N are kept
   for (int idx = threadIdx.x; idx < N * cols_per_wg; idx += nthr) {
     const int r = idx / cols_per_wg, c = idx - r * cols_per_wg;
     Tm[tpos(r, c0 + c, N)] = smem[c * ML + lpad<T>(r)];
@@ -196,7 +196,7 @@ k_pass3(const cplx<T>* __restrict__ Tm, const cplx<T>* __restrict__ x, const T* 
   tws.load(tw, t);
   for (int e = t; e < M; e += NT) buf[lpad<T>(e)] = live ? Tm[tpos(row, e, N)] : mk(T(0), T(0));
   __syncthreads();
-  lds_fft<T, M, +1>(buf, tws, t);
+  lds_fft<T, M, +1, false, true>(buf, tws, t);  // columns < N are used
   if (!live) return;
   const long rb = (long)row * N;
   for (int e = t; e < N; e += NT) {
