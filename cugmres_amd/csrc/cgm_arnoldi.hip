@@ -600,8 +600,9 @@ __global__ void __launch_bounds__(BS) k_arn_b(FusedArgs a) {
     return;
   }
   if (a.tail) return;  // u_m needs no matvec (block-uniform)
-  for (int e = tid; e < R * (M - N); e += BS) {
-    const int r = e / (M - N), c = N + e % (M - N);
+  constexpr int ZE = fft_lo_read_end<M>();  // zeros the transform reads
+  for (int e = tid; e < R * (ZE - N); e += BS) {
+    const int r = e / (ZE - N), c = N + e % (ZE - N);
     smem[r * LdsLen<float, M>::value + lpad<float>(c)] = make_float2(0.f, 0.f);
   }
   __syncthreads();

@@ -276,6 +276,11 @@ template <typename T, int M> struct FftTwiddles {
   __device__ inline void load(const cplx<T>* __restrict__ tw, int t) { fft_load_tw<T, M, 0>(reg, tw, t); }
 };
 
+// Length of the zero padding a transform with LO still reads: the first
+// stage of an even first radix never reads buf[M/2 ..], so zero-padded data
+// need zeros only up to M/2 there (up to M otherwise).
+template <int M> __host__ __device__ constexpr int fft_lo_read_end() { return fft_radix<M>(0) % 2 == 0 ? M / 2 : M; }
+
 // Full transform of the M elements of buf (LDS, padded layout lpad<T>) by
 // threads t in [0, NT).  The caller
 // must have synchronised the work-group after filling buf; on return buf

@@ -86,7 +86,7 @@ k_pass1(const cplx<T>* __restrict__ x, cplx<T>* xw, const T* __restrict__ chi,
       v = cscale(v, s);
       if (xw && live && e < N) xw[rb + e] = v;
     }
-    buf[lpad<T>(e)] = mul ? cscale(v, cv[k]) : v;
+    if (e < fft_lo_read_end<M>()) buf[lpad<T>(e)] = mul ? cscale(v, cv[k]) : v;
   }
   __syncthreads();
   lds_fft<T, M, -1, true>(buf, tws, t);  // the row is zero from N <= M/2 on
@@ -143,7 +143,7 @@ k_pass2(cplx<T>* __restrict__ Tm, const G* __restrict__ gq, int N, int cols_per_
       smem[c * ML + lpad<T>(r)] = Tm[tpos(r, c0 + c, N)];
     }
   }
-  for (int idx = threadIdx.x; idx < (M - N) * cols_per_wg; idx += nthr) {
+  for (int idx = threadIdx.x; idx < (fft_lo_read_end<M>() - N) * cols_per_wg; idx += nthr) {
     const int r = N + idx / cols_per_wg, c = idx % cols_per_wg;
     smem[c * ML + lpad<T>(r)] = mk(T(0), T(0));
   }
