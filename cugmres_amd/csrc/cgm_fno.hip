@@ -897,6 +897,99 @@ __global__ void __launch_bounds__(512) k_cell_row(const bf16* __restrict__ P, Li
   row_project<8>(lbas, basis_ld(N), lt, N, Anext + bx * C * C, red);
 }
 
+// k_cell_row<false, false> (B_x from R, the cells from P) with RPW
+// consecutive rows per work-group, for N <= 512 (a wave owns at most four
+// 16-cell tiles, the same ones in every row).  The basis F^T, the layer's
+// weights and the F_y operands are staged once for the RPW rows, and the
+// next row's cells and B_x are loaded before the current row's projection,
+// so that the HBM reads of one row overlap the LDS work of the other.  The
+// arithmetic of a row is k_cell_row's, operation for operation.
+template <int RPW>
+__global__ void __launch_bounds__(512) k_cell_rows(const bf16* __restrict__ P, const bf16* __restrict__ Q,
+                                                   const bf16* __restrict__ A, const bf16* __restrict__ R,
+                                                   const float* __restrict__ bias, bf16* __restrict__ pre_out,
+                                                   bf16* __restrict__ out, const bf16* __restrict__ Xt,
+                                                   float* __restrict__ Anext, int N, const int* done) {
+  if (skip(done)) return;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  bf16* lbas = reinterpret_cast<bf16*>(smem_raw);  // the basis F^T
+  bf16* lt = lbas + C * basis_ld(N);               // h' of the row, channel-major
+  __shared__ float red[4 * 64 * 4];
+  __shared__ __align__(16) bf16 lb[C][C + 8];  // B_x as [o][k]
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int i = lane & 15, g = lane >> 4;
+  const int ntile = N / 16;
+  const long bx0 = (long)blockIdx.x * RPW;  // sample * N + x of the first row
+  bf16x8 aw[2];
+  float bs[2][4];
+#pragma unroll
+  for (int to = 0; to < 2; ++to) {
+    aw[to] = ld8(A + (16 * to + i) * C + 8 * g);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) bs[to][v] = bias[16 * to + 4 * g + v];
+  }
+  bf16x8 bpt[4], bqt[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    int t = w + 8 * u;
+    t = t < ntile ? t : ntile - 1;
+    const int y = 16 * t + i;
+    bpt[u] = ld8(P + (bx0 * N + y) * C + 8 * g);
+    bqt[u] = ld8(Q + (long)y * C + 8 * g);
+  }
+  const int bxo = (((tid >> 2) & 31)) * C + (tid & 3) * 8;  // this thread's 8 BF16 of B_x
+  u32x4 bxv = *reinterpret_cast<const u32x4*>(R + bx0 * C * C + bxo);
+  basis_to_lds<512>(Xt, lbas, N);
+  // unrolled: as a loop the row-invariant addresses stay live across the
+  // iterations (about 155 VGPRs at RPW 2, one work-group per CU); unrolled
+  // at RPW 2 it needs 109, two work-groups per CU
+#pragma unroll
+  for (int rr = 0; rr < RPW; ++rr) {
+    const long bx = bx0 + rr;
+    if (tid < C * C / 8) *reinterpret_cast<u32x4*>(&lb[tid >> 2][(tid & 3) * 8]) = bxv;
+    __syncthreads();
+    bf16x8 ar[2];
+#pragma unroll
+    for (int to = 0; to < 2; ++to) ar[to] = *reinterpret_cast<const bf16x8*>(&lb[16 * to + i][8 * g]);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int t = w + 8 * u;
+      if (t >= ntile) break;
+      const int y = 16 * t + i;
+#pragma unroll
+      for (int to = 0; to < 2; ++to) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        acc = mma(aw[to], bpt[u], acc);
+        acc = mma(ar[to], bqt[u], acc);
+        const int oc0 = 16 * to + 4 * g;
+        float r[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) r[v] = acc[v] + bs[to][v];
+        if (pre_out) st4(pre_out + (bx * N + y) * C + oc0, r[0], r[1], r[2], r[3]);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) r[v] = gelu(r[v]);
+        st4(out + (bx * N + y) * C + oc0, r[0], r[1], r[2], r[3]);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) lt[(oc0 + v) * basis_ld(N) + y] = (bf16)r[v];
+      }
+    }
+    __syncthreads();
+    if (rr + 1 < RPW) {  // the next row's operands (block-uniform branch)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        int t = w + 8 * u;
+        t = t < ntile ? t : ntile - 1;
+        bpt[u] = ld8(P + ((bx + 1) * N + 16 * t + i) * C + 8 * g);
+      }
+      bxv = *reinterpret_cast<const u32x4*>(R + (bx + 1) * C * C + bxo);
+    }
+    // waves of the upper K half leave row_project after its barrier; every
+    // wave has read lt and red of this row before the next row's first
+    // barrier, after which they are written again
+    row_project<8>(lbas, basis_ld(N), lt, N, Anext + bx * C * C, red);
+  }
+}
+
 // Layer 3 (no activation) fused with the projection: per cell
 //   h4 = [h3, F_y] [W; B_x] + b,  z = s_id vv / *vnrm + (h4 W_out + b_out) / s
 // in FP32 (vnrm null: no scaling); h4 goes to hout as BF16 when given
@@ -1198,6 +1291,19 @@ int spectral(cgm_fno* f, hipStream_t st, const float2* W, float2* usave, bool bw
 
 namespace cgm {
 
+// Cell GEMM of layers 1-2 on B > 1 samples (B_x from R): two rows per
+// work-group where the grid still holds at least 1024 work-groups.
+void launch_cell_rows(const cgm_fno* f, hipStream_t st, int rows, const bf16* P, const bf16* W, const float* bias,
+                      bf16* pre, bf16* out, const int* done) {
+  const size_t lds = row_kernel_lds(f->N);
+  if (f->N <= 512 && rows % 2 == 0 && rows / 2 >= 1024)
+    hipLaunchKernelGGL(k_cell_rows<2>, dim3(rows / 2), dim3(512), lds, st, P, f->Fy, W, f->Bt, bias, pre, out, f->Ft,
+                       f->A, f->N, done);
+  else
+    hipLaunchKernelGGL((k_cell_row<false, false>), dim3(rows), dim3(512), lds, st, P, LiftIn{}, f->Fy, W, f->Bt,
+                       nullptr, nullptr, bias, pre, out, f->Ft, f->A, f->N, done);
+}
+
 // Forward pass on B unit-norm vectors (or one lazily normalised vector with
 // nrm/vw).  train keeps activations for fno_backward.
 int fno_forward_io(cgm_fno* f, hipStream_t st, const FnoIO& io, int B, bool train, const int* done) {
@@ -1227,9 +1333,10 @@ int fno_forward_io(cgm_fno* f, hipStream_t st, const FnoIO& io, int B, bool trai
                        !fold_l)))
       return rc;
     const bf16* Wl = f->Wt + l * C * C;
-    if (l < 3) {
-      auto kern = l == 0 ? (fold ? k_cell_row<true, true> : k_cell_row<false, true>)
-                         : (fold ? k_cell_row<true, false> : k_cell_row<false, false>);
+    if (l > 0 && l < 3 && !fold) {
+      launch_cell_rows(f, st, rows, f->h[l], Wl, Lp + C * C, train ? f->pre[l] : nullptr, f->h[l + 1], done);
+    } else if (l < 3) {
+      auto kern = l == 0 ? (fold ? k_cell_row<true, true> : k_cell_row<false, true>) : k_cell_row<true, false>;
       hipLaunchKernelGGL(kern, dim3(rows), dim3(512), lds, st, f->h[l], L0, f->Fy, Wl, f->Bt, f->Vm, f->Ex,
                          Lp + C * C, train ? f->pre[l] : nullptr, f->h[l + 1], f->Ft, f->A, f->N, done);
     } else {
@@ -1392,7 +1499,8 @@ extern "C" int cgm_fno_create(cgm_ctx_t c, int batch, float chi_scale, cgm_fno_t
   CGM_HIP(hipFuncSetAttribute((const void*)k_lift_row, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)row_kernel_lds(N)));
   const void* rowk[] = {(const void*)k_cell_row<true, false>, (const void*)k_cell_row<false, false>,
-                        (const void*)k_cell_row<true, true>, (const void*)k_cell_row<false, true>};
+                        (const void*)k_cell_row<true, true>,  (const void*)k_cell_row<false, true>,
+                        (const void*)k_cell_rows<2>};
   for (const void* k : rowk)
     CGM_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)row_kernel_lds(N)));
   *out = f;
@@ -1482,9 +1590,11 @@ extern "C" int cgm_fno_profile(cgm_fno_t f, const float* chi, const cgm_complex6
           launch_invx(f, st, B, nullptr);
           break;
         case 5:
-          hipLaunchKernelGGL((B == 1 ? k_cell_row<true, false> : k_cell_row<false, false>), dim3(rows), dim3(512), lds,
-                             st, f->h[0], L, f->Fy, f->Wt, f->Bt, f->Vm, f->Ex, Lp + C * C, nullptr, f->h[1], f->Ft,
-                             f->A, f->N, nullptr);
+          if (B == 1)
+            hipLaunchKernelGGL((k_cell_row<true, false>), dim3(rows), dim3(512), lds, st, f->h[0], L, f->Fy, f->Wt,
+                               f->Bt, f->Vm, f->Ex, Lp + C * C, nullptr, f->h[1], f->Ft, f->A, f->N, nullptr);
+          else
+            launch_cell_rows(f, st, rows, f->h[0], f->Wt, Lp + C * C, nullptr, f->h[1], nullptr);
           break;
         case 8:
           hipLaunchKernelGGL((B == 1 ? k_cell_row<true, true> : k_cell_row<false, true>), dim3(rows), dim3(512), lds,

@@ -137,10 +137,11 @@ def gpu():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("N,B", [(64, 2), (128, 1), (256, 2), (128, 4), (1024, 1)])
+@pytest.mark.parametrize("N,B", [(64, 2), (128, 1), (256, 2), (128, 4), (256, 8), (1024, 1)])
 def test_fno_forward_matches_oracle(gpu, N, B):
     # B = 1 forms B_x inside the row kernels, B >= 2 with k_invx; N = 1024
-    # is the largest grid, whose row kernels hold 146 KiB of LDS
+    # is the largest grid, whose row kernels hold 146 KiB of LDS; 2048 rows
+    # (256, 8) take the cell GEMM of layers 1-2 two rows per work-group
     solver, fno = gpu
     chi, v, p = _problem(N, B, 11)
     ctx = solver.Context(N)
@@ -155,12 +156,12 @@ def test_fno_forward_matches_oracle(gpu, N, B):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("N", [64, 256])
-def test_fno_forward_batch_equals_single_vectors(gpu, N):
-    # one vector (inverse x-DFT inside the row kernels) and a batch (k_invx)
-    # form the same sums in the same order: equal bit for bit
+@pytest.mark.parametrize("N,B", [(64, 3), (256, 3), (256, 8)])
+def test_fno_forward_batch_equals_single_vectors(gpu, N, B):
+    # one vector (inverse x-DFT inside the row kernels) and a batch (k_invx;
+    # at 2048 rows two rows per work-group in the cell GEMM) form the same
+    # sums in the same order: equal bit for bit
     solver, fno = gpu
-    B = 3
     chi, v, p = _problem(N, B, 5)
     ctx = solver.Context(N)
     chi_d = torch.from_numpy(chi.reshape(B, -1).astype(np.float32)).cuda()
@@ -173,7 +174,7 @@ def test_fno_forward_batch_equals_single_vectors(gpu, N):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("N,B", [(64, 2), (128, 1), (128, 4)])
+@pytest.mark.parametrize("N,B", [(64, 2), (128, 1), (128, 4), (256, 8)])
 def test_fno_backward_matches_oracle(gpu, N, B):
     solver, fno = gpu
     chi, v, p = _problem(N, B, 21)
