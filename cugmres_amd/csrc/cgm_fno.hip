@@ -629,64 +629,81 @@ __device__ inline bf16x8 lift8(float a0, float a1, float a2, const float* w) {
 // adjacent lanes of one load, and a thread's loads all precede its stores,
 // so an in-place vw == v is read before it is written).
 // With h0 the row also goes to global memory (16-byte stores), with vw v'
-// goes back to vw.
-template <int NT, bool TR>
-__device__ inline void lift_row_to_lds(const LiftIn& L, long ks, int xr, int N, bf16* lt, bf16* h0,
-                                       float2* vw) {
+// goes back to vw.  In two halves: lift_load issues the loads of one group
+// of four items per thread (items q0 + u NT, clamped to the row, and the
+// sample's norm, all unconditional), lift_store forms and stores them.
+struct LiftRegs {
+  float cv[4];
+  float2 xv[4];
+  double d;  // the sample's norm (1 without nrm)
+};
+template <int NT>
+__device__ inline void lift_load(const LiftIn& L, long ks, int xr, int N, int q0, LiftRegs& g) {
   const long rin = (long)xr * N;  // the row's first cell in the sample
   const float* chi = L.chi + ks * L.sChi;
   const float2* v = L.v + ks * L.sV;
-  if (vw) vw += ks * L.sV;
-  float r = 1.f;
-  if (L.nrm) {
-    const double d = *boff(L.nrm, ks * L.sNrm);
-    r = d > 0.0 ? (float)(1.0 / d) : 0.f;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int q = q0 + u * NT < N * 4 ? q0 + u * NT : N * 4 - 1;
+    g.cv[u] = chi[rin + (q >> 2)];
+    g.xv[u] = v[rin + (q >> 2)];
   }
-  // NT % 4 == 0: every item of this thread has the same 8 channels, whose
-  // 4 x 8 lift parameters stay in registers
-  static_assert(NT % 4 == 0, "items of a thread must share their channels");
+  // a global (not flat) load, which LDS waits do not wait for
+  typedef __attribute__((address_space(1))) const double gdouble;
+  g.d = *(gdouble*)(L.nrm ? boff(L.nrm, ks * L.sNrm) : &kOne);
+}
+// The 4 x 8 lift parameters of this thread's channels (NT % 4 == 0: every
+// item of a thread has the same 8 channels).
+__device__ inline void lift_weights(const LiftIn& L, float* w) {
   const int c8 = (threadIdx.x & 3) * 8;
-  float w[32];
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) w[8 * j + k] = L.P[j * C + c8 + k];
   }
-  // four items per thread at a time: their loads first (unconditional, on
-  // clamped items), then the lift and the stores
+}
+template <int NT, bool TR>
+__device__ inline void lift_store(const LiftIn& L, const LiftRegs& g, const float* w, long ks, int xr, int N,
+                                  int q0, bf16* lt, bf16* h0, float2* vw) {
+  static_assert(NT % 4 == 0, "items of a thread must share their channels");
+  const long rin = (long)xr * N;
+  const int c8 = (threadIdx.x & 3) * 8;
+  const float r = L.nrm ? (g.d > 0.0 ? (float)(1.0 / g.d) : 0.f) : 1.f;
+  if (vw) vw += ks * L.sV;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int q = q0 + u * NT;
+    if (q >= N * 4) break;
+    const int y = q >> 2;
+    const long pin = rin + y;
+    float2 vv = g.xv[u];
+    if (L.nrm) {
+      vv.x *= r;
+      vv.y *= r;
+      if (vw && c8 == 0) vw[pin] = vv;
+    }
+    const bf16x8 hq = lift8(g.cv[u] * L.chi_scale, vv.x * L.s, vv.y * L.s, w);
+    if (h0) *reinterpret_cast<u32x4*>(h0 + ((ks * N + xr) * N + y) * C + c8) = __builtin_bit_cast(u32x4, hq);
+    if (TR) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) lt[(c8 + k) * basis_ld(N) + y] = hq[k];
+    } else {
+      const u32x4 bits = __builtin_bit_cast(u32x4, hq);
+      u32x2* d = reinterpret_cast<u32x2*>(lt + y * S + c8);
+      d[0] = u32x2{bits[0], bits[1]};
+      d[1] = u32x2{bits[2], bits[3]};
+    }
+  }
+}
+template <int NT, bool TR>
+__device__ inline void lift_row_to_lds(const LiftIn& L, long ks, int xr, int N, bf16* lt, bf16* h0,
+                                       float2* vw) {
+  float w[32];
+  lift_weights(L, w);
   for (int q0 = threadIdx.x; q0 < N * 4; q0 += 4 * NT) {
-    float cv[4];
-    float2 xv[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int q = q0 + u * NT < N * 4 ? q0 + u * NT : N * 4 - 1;
-      cv[u] = chi[rin + (q >> 2)];
-      xv[u] = v[rin + (q >> 2)];
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int q = q0 + u * NT;
-      if (q >= N * 4) break;
-      const int y = q >> 2;
-      const long pin = rin + y;
-      float2 vv = xv[u];
-      if (L.nrm) {
-        vv.x *= r;
-        vv.y *= r;
-        if (vw && c8 == 0) vw[pin] = vv;
-      }
-      const bf16x8 hq = lift8(cv[u] * L.chi_scale, vv.x * L.s, vv.y * L.s, w);
-      if (h0) *reinterpret_cast<u32x4*>(h0 + ((ks * N + xr) * N + y) * C + c8) = __builtin_bit_cast(u32x4, hq);
-      if (TR) {
-#pragma unroll
-        for (int k = 0; k < 8; ++k) lt[(c8 + k) * basis_ld(N) + y] = hq[k];
-      } else {
-        const u32x4 bits = __builtin_bit_cast(u32x4, hq);
-        u32x2* d = reinterpret_cast<u32x2*>(lt + y * S + c8);
-        d[0] = u32x2{bits[0], bits[1]};
-        d[1] = u32x2{bits[2], bits[3]};
-      }
-    }
+    LiftRegs g;
+    lift_load<NT>(L, ks, xr, N, q0, g);
+    lift_store<NT, TR>(L, g, w, ks, xr, N, q0, lt, h0, vw);
   }
 }
 
@@ -709,6 +726,38 @@ __global__ void __launch_bounds__(512) k_lift_row(LiftIn L, float2* vw, bf16* __
   lift_row_to_lds<512, true>(L, ks, (int)(bx - ks * N), N, lt, h0, vw);
   __syncthreads();
   row_project<8>(lbas, basis_ld(N), lt, N, Aout + bx * C * C, red);
+}
+
+// k_lift_row with two grid rows of one sample per block (N even and
+// N <= 512, one group of four items per thread): the basis is staged once,
+// and the second row's chi and v are loaded before the first row's
+// projection.
+__global__ void __launch_bounds__(512) k_lift_rows2(LiftIn L, float2* vw, bf16* __restrict__ h0,
+                                                    const bf16* __restrict__ Xt, float* __restrict__ Aout, int N,
+                                                    const int* done) {
+  if (skip(done)) return;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  bf16* lt = reinterpret_cast<bf16*>(smem_raw);  // the row, channel-major
+  bf16* lbas = lt + C * basis_ld(N);
+  __shared__ float red[4 * 64 * 4];
+  const long bx0 = (long)blockIdx.x * 2;
+  const long ks = bx0 / N;  // sample of both rows
+  const int xr0 = (int)(bx0 - ks * N);
+  LiftRegs g;
+  lift_load<512>(L, ks, xr0, N, threadIdx.x, g);
+  basis_to_lds<512>(Xt, lbas, N);
+#pragma unroll
+  for (int rr = 0; rr < 2; ++rr) {
+    if (rr) __syncthreads();  // the first row's projection has read lt
+    {
+      float w[32];
+      lift_weights(L, w);
+      lift_store<512, true>(L, g, w, ks, xr0 + rr, N, threadIdx.x, lt, h0, vw);
+    }
+    __syncthreads();
+    if (rr == 0) lift_load<512>(L, ks, xr0 + 1, N, threadIdx.x, g);
+    row_project<8>(lbas, basis_ld(N), lt, N, Aout + (bx0 + rr) * C * C, red);
+  }
 }
 
 // One block (8 waves) per grid row x of sample k (layers 0-2):
@@ -987,6 +1036,104 @@ __global__ void __launch_bounds__(512) k_cell_rows(const bf16* __restrict__ P, c
     // wave has read lt and red of this row before the next row's first
     // barrier, after which they are written again
     row_project<8>(lbas, basis_ld(N), lt, N, Anext + bx * C * C, red);
+  }
+}
+
+// k_cell_row<false, true> (layer 0 of a batch) with two grid rows of one
+// sample per block, N even and N <= 512.  One LDS region holds the basis
+// for both rows; the other holds a row lifted from (chi, v) as [y][S] until
+// every wave has taken its cells, then h' channel-major for the row
+// projection.  The weights, the F_y operands and the basis are loaded once;
+// the second row's chi, v and B_x are loaded before the first row's
+// projection.  The arithmetic of a row is k_cell_row's.
+__global__ void __launch_bounds__(512) k_cell_rows_lift2(LiftIn L, const bf16* __restrict__ Q,
+                                                         const bf16* __restrict__ A, const bf16* __restrict__ R,
+                                                         const float* __restrict__ bias, bf16* __restrict__ pre_out,
+                                                         bf16* __restrict__ out, const bf16* __restrict__ Xt,
+                                                         float* __restrict__ Anext, int N, const int* done) {
+  if (skip(done)) return;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  bf16* lbas = reinterpret_cast<bf16*>(smem_raw);  // the basis F^T
+  bf16* lx = lbas + C * basis_ld(N);               // the lifted row, then h'
+  __shared__ float red[4 * 64 * 4];
+  __shared__ __align__(16) bf16 lb[C][C + 8];  // B_x as [o][k]
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int i = lane & 15, g = lane >> 4;
+  const int ntile = N / 16;
+  const long bx0 = (long)blockIdx.x * 2;
+  const long ks = bx0 / N;  // sample of both rows
+  const int xr0 = (int)(bx0 - ks * N);
+  bf16x8 aw[2];
+  float bs[2][4];
+#pragma unroll
+  for (int to = 0; to < 2; ++to) {
+    aw[to] = ld8(A + (16 * to + i) * C + 8 * g);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) bs[to][v] = bias[16 * to + 4 * g + v];
+  }
+  bf16x8 bqt[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    int t = w + 8 * u;
+    t = t < ntile ? t : ntile - 1;
+    bqt[u] = ld8(Q + (long)(16 * t + i) * C + 8 * g);
+  }
+  const int bxo = ((tid >> 2) & 31) * C + (tid & 3) * 8;  // this thread's 8 BF16 of B_x
+  u32x4 bxv = *reinterpret_cast<const u32x4*>(R + bx0 * C * C + bxo);
+  LiftRegs lg;
+  lift_load<512>(L, ks, xr0, N, tid, lg);
+  basis_to_lds<512>(Xt, lbas, N);
+#pragma unroll
+  for (int rr = 0; rr < 2; ++rr) {
+    const long bx = bx0 + rr;
+    if (rr) __syncthreads();  // the first row's projection has read lx
+    {
+      float wl[32];
+      lift_weights(L, wl);
+      lift_store<512, false>(L, lg, wl, ks, xr0 + rr, N, tid, lx, nullptr, nullptr);
+    }
+    if (tid < C * C / 8) *reinterpret_cast<u32x4*>(&lb[tid >> 2][(tid & 3) * 8]) = bxv;
+    __syncthreads();
+    bf16x8 ar[2], bpt[4];
+#pragma unroll
+    for (int to = 0; to < 2; ++to) ar[to] = *reinterpret_cast<const bf16x8*>(&lb[16 * to + i][8 * g]);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      int t = w + 8 * u;
+      t = t < ntile ? t : ntile - 1;
+      const u32x2* sp = reinterpret_cast<const u32x2*>(lx + (16 * t + i) * S + 8 * g);
+      const u32x2 lo = sp[0], hi = sp[1];
+      bpt[u] = __builtin_bit_cast(bf16x8, u32x4{lo[0], lo[1], hi[0], hi[1]});
+    }
+    __syncthreads();  // every wave has its cells: lx takes h'
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int t = w + 8 * u;
+      if (t >= ntile) break;
+      const int y = 16 * t + i;
+#pragma unroll
+      for (int to = 0; to < 2; ++to) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        acc = mma(aw[to], bpt[u], acc);
+        acc = mma(ar[to], bqt[u], acc);
+        const int oc0 = 16 * to + 4 * g;
+        float r[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) r[v] = acc[v] + bs[to][v];
+        if (pre_out) st4(pre_out + (bx * N + y) * C + oc0, r[0], r[1], r[2], r[3]);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) r[v] = gelu(r[v]);
+        st4(out + (bx * N + y) * C + oc0, r[0], r[1], r[2], r[3]);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) lx[(oc0 + v) * basis_ld(N) + y] = (bf16)r[v];
+      }
+    }
+    __syncthreads();
+    if (rr == 0) {  // the second row's operands
+      lift_load<512>(L, ks, xr0 + 1, N, tid, lg);
+      bxv = *reinterpret_cast<const u32x4*>(R + (bx0 + 1) * C * C + bxo);
+    }
+    row_project<8>(lbas, basis_ld(N), lx, N, Anext + bx * C * C, red);
   }
 }
 
@@ -1291,12 +1438,38 @@ int spectral(cgm_fno* f, hipStream_t st, const float2* W, float2* usave, bool bw
 
 namespace cgm {
 
+// Two grid rows per block where N allows it (N even, N <= 512) and the
+// grid still holds at least 1024 blocks.
+inline bool two_rows(int N, int rows) { return N % 2 == 0 && N <= 512 && rows / 2 >= 1024; }
+
+// The lift with layer 0's first row projection (k_lift_row / k_lift_rows2).
+void launch_lift(const cgm_fno* f, hipStream_t st, int rows, const LiftIn& L, float2* vw, bf16* h0,
+                 const int* done) {
+  const size_t lds = row_kernel_lds(f->N);
+  if (two_rows(f->N, rows))
+    hipLaunchKernelGGL(k_lift_rows2, dim3(rows / 2), dim3(512), lds, st, L, vw, h0, f->Ft, f->A, f->N, done);
+  else
+    hipLaunchKernelGGL(k_lift_row, dim3(rows), dim3(512), lds, st, L, vw, h0, f->Ft, f->A, f->N, done);
+}
+
+// Layer 0's cell GEMM on B > 1 samples, lifting its rows from (chi, v).
+void launch_cell_rows_lift(const cgm_fno* f, hipStream_t st, int rows, const LiftIn& L, const bf16* W,
+                           const float* bias, bf16* pre, bf16* out, const int* done) {
+  const size_t lds = row_kernel_lds(f->N);
+  if (two_rows(f->N, rows))
+    hipLaunchKernelGGL(k_cell_rows_lift2, dim3(rows / 2), dim3(512), lds, st, L, f->Fy, W, f->Bt, bias, pre, out,
+                       f->Ft, f->A, f->N, done);
+  else
+    hipLaunchKernelGGL((k_cell_row<false, true>), dim3(rows), dim3(512), lds, st, nullptr, L, f->Fy, W, f->Bt,
+                       nullptr, nullptr, bias, pre, out, f->Ft, f->A, f->N, done);
+}
+
 // Cell GEMM of layers 1-2 on B > 1 samples (B_x from R): two rows per
 // work-group where the grid still holds at least 1024 work-groups.
 void launch_cell_rows(const cgm_fno* f, hipStream_t st, int rows, const bf16* P, const bf16* W, const float* bias,
                       bf16* pre, bf16* out, const int* done) {
   const size_t lds = row_kernel_lds(f->N);
-  if (f->N <= 512 && rows % 2 == 0 && rows / 2 >= 1024)
+  if (two_rows(f->N, rows))
     hipLaunchKernelGGL(k_cell_rows<2>, dim3(rows / 2), dim3(512), lds, st, P, f->Fy, W, f->Bt, bias, pre, out, f->Ft,
                        f->A, f->N, done);
   else
@@ -1313,8 +1486,7 @@ int fno_forward_io(cgm_fno* f, hipStream_t st, const FnoIO& io, int B, bool trai
   const bool fold = B == 1;
   int rc;
   LiftIn L{io.chi, io.sChi, f->chi_scale, io.v, io.sV, io.nrm, io.sNrm, s, f->params};
-  hipLaunchKernelGGL(k_lift_row, dim3(rows), dim3(512), lds, st, L, io.vw, train ? f->h[0] : nullptr, f->Ft, f->A,
-                     f->N, done);
+  launch_lift(f, st, rows, L, io.vw, train ? f->h[0] : nullptr, done);
   CGM_HIP(hipGetLastError());
   const float2* vv = io.vw ? io.vw : io.v;
   // layer 0 lifts its rows again from v' (normalised by k_lift_row into vw
@@ -1335,8 +1507,10 @@ int fno_forward_io(cgm_fno* f, hipStream_t st, const FnoIO& io, int B, bool trai
     const bf16* Wl = f->Wt + l * C * C;
     if (l > 0 && l < 3 && !fold) {
       launch_cell_rows(f, st, rows, f->h[l], Wl, Lp + C * C, train ? f->pre[l] : nullptr, f->h[l + 1], done);
+    } else if (l == 0 && !fold) {
+      launch_cell_rows_lift(f, st, rows, L0, Wl, Lp + C * C, train ? f->pre[0] : nullptr, f->h[1], done);
     } else if (l < 3) {
-      auto kern = l == 0 ? (fold ? k_cell_row<true, true> : k_cell_row<false, true>) : k_cell_row<true, false>;
+      auto kern = l == 0 ? k_cell_row<true, true> : k_cell_row<true, false>;
       hipLaunchKernelGGL(kern, dim3(rows), dim3(512), lds, st, f->h[l], L0, f->Fy, Wl, f->Bt, f->Vm, f->Ex,
                          Lp + C * C, train ? f->pre[l] : nullptr, f->h[l + 1], f->Ft, f->A, f->N, done);
     } else {
@@ -1500,7 +1674,8 @@ extern "C" int cgm_fno_create(cgm_ctx_t c, int batch, float chi_scale, cgm_fno_t
                               (int)row_kernel_lds(N)));
   const void* rowk[] = {(const void*)k_cell_row<true, false>, (const void*)k_cell_row<false, false>,
                         (const void*)k_cell_row<true, true>,  (const void*)k_cell_row<false, true>,
-                        (const void*)k_cell_rows<2>};
+                        (const void*)k_cell_rows<2>,          (const void*)k_cell_rows_lift2,
+                        (const void*)k_lift_rows2};
   for (const void* k : rowk)
     CGM_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)row_kernel_lds(N)));
   *out = f;
@@ -1577,8 +1752,7 @@ extern "C" int cgm_fno_profile(cgm_fno_t f, const float* chi, const cgm_complex6
       if (r == 0) CGM_HIP(hipEventRecord(e0, st));
       switch (k) {
         case 0:  // the first warm-up also stores h0, which cases 1 and 5 read
-          hipLaunchKernelGGL(k_lift_row, dim3(rows), dim3(512), lds, st, L, nullptr, r == -2 ? f->h[0] : nullptr,
-                             f->Ft, f->A, f->N, nullptr);
+          launch_lift(f, st, rows, L, nullptr, r == -2 ? f->h[0] : nullptr, nullptr);
           break;
         case 1: rc = launch_rowproj<false>(f, st, f->h[0], nullptr, f->A, B, nullptr); break;
         case 2: nch = launch_dftx(f, st, 0, B, nullptr); break;
@@ -1597,9 +1771,11 @@ extern "C" int cgm_fno_profile(cgm_fno_t f, const float* chi, const cgm_complex6
             launch_cell_rows(f, st, rows, f->h[0], f->Wt, Lp + C * C, nullptr, f->h[1], nullptr);
           break;
         case 8:
-          hipLaunchKernelGGL((B == 1 ? k_cell_row<true, true> : k_cell_row<false, true>), dim3(rows), dim3(512), lds,
-                             st, nullptr, L, f->Fy, f->Wt, f->Bt, f->Vm, f->Ex, Lp + C * C, nullptr, f->h[1], f->Ft,
-                             f->A, f->N, nullptr);
+          if (B == 1)
+            hipLaunchKernelGGL((k_cell_row<true, true>), dim3(rows), dim3(512), lds, st, nullptr, L, f->Fy, f->Wt,
+                               f->Bt, f->Vm, f->Ex, Lp + C * C, nullptr, f->h[1], f->Ft, f->A, f->N, nullptr);
+          else
+            launch_cell_rows_lift(f, st, rows, L, f->Wt, Lp + C * C, nullptr, f->h[1], nullptr);
           break;
         case 6:
           hipLaunchKernelGGL(k_cell_proj, dim3((unsigned)(total / 256)), dim3(256), 0, st, f->h[3], f->Fy, f->Wt,
