@@ -728,34 +728,35 @@ __global__ void __launch_bounds__(512) k_lift_row(LiftIn L, float2* vw, bf16* __
   row_project<8>(lbas, basis_ld(N), lt, N, Aout + bx * C * C, red);
 }
 
-// k_lift_row with two grid rows of one sample per block (N even and
+// k_lift_row with RPW grid rows of one sample per block (N % RPW == 0 and
 // N <= 512, one group of four items per thread): the basis is staged once,
-// and the second row's chi and v are loaded before the first row's
+// and each next row's chi and v are loaded before the current row's
 // projection.
-__global__ void __launch_bounds__(512) k_lift_rows2(LiftIn L, float2* vw, bf16* __restrict__ h0,
-                                                    const bf16* __restrict__ Xt, float* __restrict__ Aout, int N,
-                                                    const int* done) {
+template <int RPW>
+__global__ void __launch_bounds__(512) k_lift_rows(LiftIn L, float2* vw, bf16* __restrict__ h0,
+                                                   const bf16* __restrict__ Xt, float* __restrict__ Aout, int N,
+                                                   const int* done) {
   if (skip(done)) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   bf16* lt = reinterpret_cast<bf16*>(smem_raw);  // the row, channel-major
   bf16* lbas = lt + C * basis_ld(N);
   __shared__ float red[4 * 64 * 4];
-  const long bx0 = (long)blockIdx.x * 2;
-  const long ks = bx0 / N;  // sample of both rows
+  const long bx0 = (long)blockIdx.x * RPW;
+  const long ks = bx0 / N;  // sample of all RPW rows
   const int xr0 = (int)(bx0 - ks * N);
   LiftRegs g;
   lift_load<512>(L, ks, xr0, N, threadIdx.x, g);
   basis_to_lds<512>(Xt, lbas, N);
 #pragma unroll
-  for (int rr = 0; rr < 2; ++rr) {
-    if (rr) __syncthreads();  // the first row's projection has read lt
+  for (int rr = 0; rr < RPW; ++rr) {
+    if (rr) __syncthreads();  // the previous row's projection has read lt
     {
       float w[32];
       lift_weights(L, w);
       lift_store<512, true>(L, g, w, ks, xr0 + rr, N, threadIdx.x, lt, h0, vw);
     }
     __syncthreads();
-    if (rr == 0) lift_load<512>(L, ks, xr0 + 1, N, threadIdx.x, g);
+    if (rr + 1 < RPW) lift_load<512>(L, ks, xr0 + rr + 1, N, threadIdx.x, g);
     row_project<8>(lbas, basis_ld(N), lt, N, Aout + (bx0 + rr) * C * C, red);
   }
 }
@@ -1442,12 +1443,15 @@ namespace cgm {
 // grid still holds at least 1024 blocks.
 inline bool two_rows(int N, int rows) { return N % 2 == 0 && N <= 512 && rows / 2 >= 1024; }
 
-// The lift with layer 0's first row projection (k_lift_row / k_lift_rows2).
+// The lift with layer 0's first row projection (k_lift_row / k_lift_rows):
+// four rows per block where the grid still holds 1024 blocks, else two.
 void launch_lift(const cgm_fno* f, hipStream_t st, int rows, const LiftIn& L, float2* vw, bf16* h0,
                  const int* done) {
   const size_t lds = row_kernel_lds(f->N);
-  if (two_rows(f->N, rows))
-    hipLaunchKernelGGL(k_lift_rows2, dim3(rows / 2), dim3(512), lds, st, L, vw, h0, f->Ft, f->A, f->N, done);
+  if (f->N % 4 == 0 && f->N <= 512 && rows / 4 >= 1024)
+    hipLaunchKernelGGL(k_lift_rows<4>, dim3(rows / 4), dim3(512), lds, st, L, vw, h0, f->Ft, f->A, f->N, done);
+  else if (two_rows(f->N, rows))
+    hipLaunchKernelGGL(k_lift_rows<2>, dim3(rows / 2), dim3(512), lds, st, L, vw, h0, f->Ft, f->A, f->N, done);
   else
     hipLaunchKernelGGL(k_lift_row, dim3(rows), dim3(512), lds, st, L, vw, h0, f->Ft, f->A, f->N, done);
 }
@@ -1675,7 +1679,7 @@ extern "C" int cgm_fno_create(cgm_ctx_t c, int batch, float chi_scale, cgm_fno_t
   const void* rowk[] = {(const void*)k_cell_row<true, false>, (const void*)k_cell_row<false, false>,
                         (const void*)k_cell_row<true, true>,  (const void*)k_cell_row<false, true>,
                         (const void*)k_cell_rows<2>,          (const void*)k_cell_rows_lift2,
-                        (const void*)k_lift_rows2};
+                        (const void*)k_lift_rows<2>,          (const void*)k_lift_rows<4>};
   for (const void* k : rowk)
     CGM_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)row_kernel_lds(N)));
   *out = f;

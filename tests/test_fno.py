@@ -137,11 +137,12 @@ def gpu():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("N,B", [(64, 2), (128, 1), (256, 2), (128, 4), (256, 8), (1024, 1)])
+@pytest.mark.parametrize("N,B", [(64, 2), (128, 1), (256, 2), (128, 4), (256, 8), (256, 16), (1024, 1)])
 def test_fno_forward_matches_oracle(gpu, N, B):
     # B = 1 forms B_x inside the row kernels, B >= 2 with k_invx; N = 1024
-    # is the largest grid, whose row kernels hold 146 KiB of LDS; 2048 rows
-    # (256, 8) take the cell GEMM of layers 1-2 two rows per work-group
+    # is the largest grid, whose row kernels hold 146 KiB of LDS; from 2048
+    # rows (256, 8) the row kernels take two rows per work-group, from 4096
+    # (256, 16) the lift four
     solver, fno = gpu
     chi, v, p = _problem(N, B, 11)
     ctx = solver.Context(N)
@@ -156,7 +157,7 @@ def test_fno_forward_matches_oracle(gpu, N, B):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("N,B", [(64, 3), (256, 3), (256, 8)])
+@pytest.mark.parametrize("N,B", [(64, 3), (256, 3), (256, 8), (256, 16)])
 def test_fno_forward_batch_equals_single_vectors(gpu, N, B):
     # one vector (inverse x-DFT inside the row kernels) and a batch (k_invx;
     # at 2048 rows two rows per work-group in the cell GEMM) form the same
@@ -174,7 +175,7 @@ def test_fno_forward_batch_equals_single_vectors(gpu, N, B):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("N,B", [(64, 2), (128, 1), (128, 4), (256, 8)])
+@pytest.mark.parametrize("N,B", [(64, 2), (128, 1), (128, 4), (256, 8), (256, 16)])
 def test_fno_backward_matches_oracle(gpu, N, B):
     solver, fno = gpu
     chi, v, p = _problem(N, B, 21)
