@@ -991,8 +991,8 @@ __global__ void __launch_bounds__(512) k_cell_rows(const bf16* __restrict__ P, c
   u32x4 bxv = *reinterpret_cast<const u32x4*>(R + bx0 * C * C + bxo);
   basis_to_lds<512>(Xt, lbas, N);
   // unrolled: as a loop the row-invariant addresses stay live across the
-  // iterations (about 155 VGPRs at RPW 2, one work-group per CU); unrolled
-  // at RPW 2 it needs 109, two work-groups per CU
+  // iterations (about 155 VGPRs, one work-group per CU); unrolled it needs
+  // 109 at RPW 2 and 121 at RPW 4, two work-groups per CU
 #pragma unroll
   for (int rr = 0; rr < RPW; ++rr) {
     const long bx = bx0 + rr;
@@ -1468,12 +1468,16 @@ void launch_cell_rows_lift(const cgm_fno* f, hipStream_t st, int rows, const Lif
                        nullptr, nullptr, bias, pre, out, f->Ft, f->A, f->N, done);
 }
 
-// Cell GEMM of layers 1-2 on B > 1 samples (B_x from R): two rows per
-// work-group where the grid still holds at least 1024 work-groups.
+// Cell GEMM of layers 1-2 on B > 1 samples (B_x from R): four rows per
+// work-group where the grid still holds at least 1024 work-groups, else two
+// where it does.
 void launch_cell_rows(const cgm_fno* f, hipStream_t st, int rows, const bf16* P, const bf16* W, const float* bias,
                       bf16* pre, bf16* out, const int* done) {
   const size_t lds = row_kernel_lds(f->N);
-  if (two_rows(f->N, rows))
+  if (f->N % 4 == 0 && f->N <= 512 && rows / 4 >= 1024)
+    hipLaunchKernelGGL(k_cell_rows<4>, dim3(rows / 4), dim3(512), lds, st, P, f->Fy, W, f->Bt, bias, pre, out, f->Ft,
+                       f->A, f->N, done);
+  else if (two_rows(f->N, rows))
     hipLaunchKernelGGL(k_cell_rows<2>, dim3(rows / 2), dim3(512), lds, st, P, f->Fy, W, f->Bt, bias, pre, out, f->Ft,
                        f->A, f->N, done);
   else
@@ -1678,7 +1682,8 @@ extern "C" int cgm_fno_create(cgm_ctx_t c, int batch, float chi_scale, cgm_fno_t
                               (int)row_kernel_lds(N)));
   const void* rowk[] = {(const void*)k_cell_row<true, false>, (const void*)k_cell_row<false, false>,
                         (const void*)k_cell_row<true, true>,  (const void*)k_cell_row<false, true>,
-                        (const void*)k_cell_rows<2>,          (const void*)k_cell_rows_lift2,
+                        (const void*)k_cell_rows<2>,          (const void*)k_cell_rows<4>,
+                        (const void*)k_cell_rows_lift2,
                         (const void*)k_lift_rows<2>,          (const void*)k_lift_rows<4>};
   for (const void* k : rowk)
     CGM_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)row_kernel_lds(N)));
