@@ -1040,14 +1040,15 @@ __global__ void __launch_bounds__(512) k_cell_rows(const bf16* __restrict__ P, c
   }
 }
 
-// k_cell_row<false, true> (layer 0 of a batch) with two grid rows of one
-// sample per block, N even and N <= 512.  One LDS region holds the basis
-// for both rows; the other holds a row lifted from (chi, v) as [y][S] until
-// every wave has taken its cells, then h' channel-major for the row
+// k_cell_row<false, true> (layer 0 of a batch) with RPW grid rows of one
+// sample per block, N % RPW == 0 and N <= 512.  One LDS region holds the
+// basis for all rows; the other holds a row lifted from (chi, v) as [y][S]
+// until every wave has taken its cells, then h' channel-major for the row
 // projection.  The weights, the F_y operands and the basis are loaded once;
-// the second row's chi, v and B_x are loaded before the first row's
+// each next row's chi, v and B_x are loaded before the current row's
 // projection.  The arithmetic of a row is k_cell_row's.
-__global__ void __launch_bounds__(512) k_cell_rows_lift2(LiftIn L, const bf16* __restrict__ Q,
+template <int RPW>
+__global__ void __launch_bounds__(512) k_cell_rows_lift(LiftIn L, const bf16* __restrict__ Q,
                                                          const bf16* __restrict__ A, const bf16* __restrict__ R,
                                                          const float* __restrict__ bias, bf16* __restrict__ pre_out,
                                                          bf16* __restrict__ out, const bf16* __restrict__ Xt,
@@ -1061,8 +1062,8 @@ __global__ void __launch_bounds__(512) k_cell_rows_lift2(LiftIn L, const bf16* _
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int i = lane & 15, g = lane >> 4;
   const int ntile = N / 16;
-  const long bx0 = (long)blockIdx.x * 2;
-  const long ks = bx0 / N;  // sample of both rows
+  const long bx0 = (long)blockIdx.x * RPW;
+  const long ks = bx0 / N;  // sample of all RPW rows
   const int xr0 = (int)(bx0 - ks * N);
   bf16x8 aw[2];
   float bs[2][4];
@@ -1085,9 +1086,9 @@ __global__ void __launch_bounds__(512) k_cell_rows_lift2(LiftIn L, const bf16* _
   lift_load<512>(L, ks, xr0, N, tid, lg);
   basis_to_lds<512>(Xt, lbas, N);
 #pragma unroll
-  for (int rr = 0; rr < 2; ++rr) {
+  for (int rr = 0; rr < RPW; ++rr) {
     const long bx = bx0 + rr;
-    if (rr) __syncthreads();  // the first row's projection has read lx
+    if (rr) __syncthreads();  // the previous row's projection has read lx
     {
       float wl[32];
       lift_weights(L, wl);
@@ -1130,9 +1131,9 @@ __global__ void __launch_bounds__(512) k_cell_rows_lift2(LiftIn L, const bf16* _
       }
     }
     __syncthreads();
-    if (rr == 0) {  // the second row's operands
-      lift_load<512>(L, ks, xr0 + 1, N, tid, lg);
-      bxv = *reinterpret_cast<const u32x4*>(R + (bx0 + 1) * C * C + bxo);
+    if (rr + 1 < RPW) {  // the next row's operands
+      lift_load<512>(L, ks, xr0 + rr + 1, N, tid, lg);
+      bxv = *reinterpret_cast<const u32x4*>(R + (bx + 1) * C * C + bxo);
     }
     row_project<8>(lbas, basis_ld(N), lx, N, Anext + bx * C * C, red);
   }
@@ -1460,8 +1461,11 @@ void launch_lift(const cgm_fno* f, hipStream_t st, int rows, const LiftIn& L, fl
 void launch_cell_rows_lift(const cgm_fno* f, hipStream_t st, int rows, const LiftIn& L, const bf16* W,
                            const float* bias, bf16* pre, bf16* out, const int* done) {
   const size_t lds = row_kernel_lds(f->N);
-  if (two_rows(f->N, rows))
-    hipLaunchKernelGGL(k_cell_rows_lift2, dim3(rows / 2), dim3(512), lds, st, L, f->Fy, W, f->Bt, bias, pre, out,
+  if (f->N % 4 == 0 && f->N <= 512 && rows / 4 >= 1024)
+    hipLaunchKernelGGL(k_cell_rows_lift<4>, dim3(rows / 4), dim3(512), lds, st, L, f->Fy, W, f->Bt, bias, pre, out,
+                       f->Ft, f->A, f->N, done);
+  else if (two_rows(f->N, rows))
+    hipLaunchKernelGGL(k_cell_rows_lift<2>, dim3(rows / 2), dim3(512), lds, st, L, f->Fy, W, f->Bt, bias, pre, out,
                        f->Ft, f->A, f->N, done);
   else
     hipLaunchKernelGGL((k_cell_row<false, true>), dim3(rows), dim3(512), lds, st, nullptr, L, f->Fy, W, f->Bt,
@@ -1683,7 +1687,7 @@ extern "C" int cgm_fno_create(cgm_ctx_t c, int batch, float chi_scale, cgm_fno_t
   const void* rowk[] = {(const void*)k_cell_row<true, false>, (const void*)k_cell_row<false, false>,
                         (const void*)k_cell_row<true, true>,  (const void*)k_cell_row<false, true>,
                         (const void*)k_cell_rows<2>,          (const void*)k_cell_rows<4>,
-                        (const void*)k_cell_rows_lift2,
+                        (const void*)k_cell_rows_lift<2>,     (const void*)k_cell_rows_lift<4>,
                         (const void*)k_lift_rows<2>,          (const void*)k_lift_rows<4>};
   for (const void* k : rowk)
     CGM_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)row_kernel_lds(N)));
