@@ -992,7 +992,7 @@ __global__ void __launch_bounds__(512) k_cell_rows(const bf16* __restrict__ P, c
   basis_to_lds<512>(Xt, lbas, N);
   // unrolled: as a loop the row-invariant addresses stay live across the
   // iterations (about 155 VGPRs, one work-group per CU); unrolled it needs
-  // 109 at RPW 2 and 121 at RPW 4, two work-groups per CU
+  // 109 at RPW 2 and 121 at RPW 4 and 8, two work-groups per CU
 #pragma unroll
   for (int rr = 0; rr < RPW; ++rr) {
     const long bx = bx0 + rr;
@@ -1472,13 +1472,15 @@ void launch_cell_rows_lift(const cgm_fno* f, hipStream_t st, int rows, const Lif
                        nullptr, nullptr, bias, pre, out, f->Ft, f->A, f->N, done);
 }
 
-// Cell GEMM of layers 1-2 on B > 1 samples (B_x from R): four rows per
-// work-group where the grid still holds at least 1024 work-groups, else two
-// where it does.
+// Cell GEMM of layers 1-2 on B > 1 samples (B_x from R): the most rows per
+// work-group (8, 4 or 2) that leave at least 1024 work-groups.
 void launch_cell_rows(const cgm_fno* f, hipStream_t st, int rows, const bf16* P, const bf16* W, const float* bias,
                       bf16* pre, bf16* out, const int* done) {
   const size_t lds = row_kernel_lds(f->N);
-  if (f->N % 4 == 0 && f->N <= 512 && rows / 4 >= 1024)
+  if (f->N % 8 == 0 && f->N <= 512 && rows / 8 >= 1024)
+    hipLaunchKernelGGL(k_cell_rows<8>, dim3(rows / 8), dim3(512), lds, st, P, f->Fy, W, f->Bt, bias, pre, out, f->Ft,
+                       f->A, f->N, done);
+  else if (f->N % 4 == 0 && f->N <= 512 && rows / 4 >= 1024)
     hipLaunchKernelGGL(k_cell_rows<4>, dim3(rows / 4), dim3(512), lds, st, P, f->Fy, W, f->Bt, bias, pre, out, f->Ft,
                        f->A, f->N, done);
   else if (two_rows(f->N, rows))
@@ -1687,6 +1689,7 @@ extern "C" int cgm_fno_create(cgm_ctx_t c, int batch, float chi_scale, cgm_fno_t
   const void* rowk[] = {(const void*)k_cell_row<true, false>, (const void*)k_cell_row<false, false>,
                         (const void*)k_cell_row<true, true>,  (const void*)k_cell_row<false, true>,
                         (const void*)k_cell_rows<2>,          (const void*)k_cell_rows<4>,
+                        (const void*)k_cell_rows<8>,
                         (const void*)k_cell_rows_lift<2>,     (const void*)k_cell_rows_lift<4>,
                         (const void*)k_lift_rows<2>,          (const void*)k_lift_rows<4>};
   for (const void* k : rowk)

@@ -137,12 +137,13 @@ def gpu():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("N,B", [(64, 2), (128, 1), (256, 2), (128, 4), (256, 8), (256, 16), (1024, 1)])
+@pytest.mark.parametrize("N,B", [(64, 2), (128, 1), (256, 2), (128, 4), (256, 8), (256, 16), (512, 16),
+                                 (1024, 1)])
 def test_fno_forward_matches_oracle(gpu, N, B):
     # B = 1 forms B_x inside the row kernels, B >= 2 with k_invx; N = 1024
     # is the largest grid, whose row kernels hold 146 KiB of LDS; from 2048
     # rows (256, 8) the row kernels take two rows per work-group, from 4096
-    # (256, 16) the lift four
+    # (256, 16) four, from 8192 (512, 16) the cell GEMM of layers 1-2 eight
     solver, fno = gpu
     chi, v, p = _problem(N, B, 11)
     ctx = solver.Context(N)
@@ -157,7 +158,7 @@ def test_fno_forward_matches_oracle(gpu, N, B):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("N,B", [(64, 3), (256, 3), (256, 8), (256, 16)])
+@pytest.mark.parametrize("N,B", [(64, 3), (256, 3), (256, 8), (256, 16), (512, 16)])
 def test_fno_forward_batch_equals_single_vectors(gpu, N, B):
     # one vector (inverse x-DFT inside the row kernels) and a batch (k_invx;
     # at 2048 rows two rows per work-group in the cell GEMM) form the same
