@@ -444,11 +444,14 @@ def profile_fno(ctx, net, dchi, db, batch=1):
     _lib.check(st, "cgm_fno_profile")
     names = ["fno_lift", "fno_rowproj", "fno_dftx", "fno_mix", "fno_invx", "fno_cell_gemm", "fno_proj",
              "fno_forward_total", "fno_cell_gemm_lift"]
-    # launches per forward pass: fno_lift = k_lift_row (lift + layer-0 row
-    # projection; h0 stays in LDS); fno_cell_gemm = k_cell_row (B_x of the
-    # row, cell GEMM + GELU + next row projection, layers 1-2);
-    # fno_cell_gemm_lift = the same for layer 0, lifting its row from chi and
-    # v; fno_proj = k_cell_proj (layer-3 cell GEMM + the 32 -> 2 projection).
+    # launches per forward pass: fno_lift = k_lift_row / k_lift_rows (lift +
+    # layer-0 row projection; h0 stays in LDS); fno_cell_gemm = k_cell_row /
+    # k_cell_rows (B_x of the row, cell GEMM + GELU + next row projection,
+    # layers 1-2); fno_cell_gemm_lift = the same for layer 0 (k_cell_row /
+    # k_cell_rows_lift), lifting its row from chi and v; fno_proj =
+    # k_cell_proj (layer-3 cell GEMM + the 32 -> 2 projection).  Batches of
+    # at least 2048 grid rows take the *_rows kernels, several grid rows per
+    # work-group; the algorithmic bytes are the same.
     # For one vector the row kernels of
     # layers 0-2 form B_x themselves (inverse x-DFT from the mixed modes) and
     # k_invx runs for layer 3 only; for more, k_invx forms every B_x.  The
