@@ -95,7 +95,7 @@ extern "C" int cgm_destroy(cgm_ctx_t c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   krylov_free(c->ws);
   void* ptrs[] = {c->tw32, c->tw64, c->T32, c->T64, c->st, c->chi_tmp, c->vec_tmp, c->v64a, c->chi32, c->zb,
-                  c->stB, c->T32B};
+                  c->stB, c->T32B, c->les_ws};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->hst) (void)hipHostFree(c->hst);

@@ -122,44 +122,56 @@ extern "C" int cgm_Fast_GMRES_with_CUDA(const cgm_complex64* gamma, const bool* 
   const long n = (long)N * N;
   int rc = ensure_tmp(c);
   if (rc) return rc;
+  // Every exit after the first allocation goes through `fail`, which frees
+  // both device arrays; only a successful call hands `resid` to the caller.
   float* resid = nullptr;
-  CGM_HIP(hipMalloc((void**)&resid, sizeof(float) * (maxiter + 1)));
   double* hist = nullptr;
-  CGM_HIP(hipMalloc((void**)&hist, sizeof(double) * (maxiter + 2)));
-  CGM_HIP(hipMemsetAsync(hist, 0, sizeof(double) * (maxiter + 2), c->stream));
-  if ((rc = mask_to_chi(c, mask))) return rc;
+  hipError_t he = hipSuccess;
+  cgm_gmres_result r;
+  unsigned steps = 0;
+  double final_est = 0.0;
+  if ((he = hipMalloc((void**)&resid, sizeof(float) * (maxiter + 1))) != hipSuccess) goto hip_fail;
+  if ((he = hipMalloc((void**)&hist, sizeof(double) * (maxiter + 2))) != hipSuccess) goto hip_fail;
+  if ((he = hipMemsetAsync(hist, 0, sizeof(double) * (maxiter + 2), c->stream)) != hipSuccess) goto hip_fail;
+  if ((rc = mask_to_chi(c, mask))) goto fail;
   if (for_gradient)
     hipLaunchKernelGGL(k_unit, dim3(1024), dim3(256), 0, c->stream, (float2*)c->vec_tmp, n, (long)h_index_of_max);
   else
     hipLaunchKernelGGL(k_ref_plane_wave, dim3(1024), dim3(256), 0, c->stream, c->vec_tmp, (int)N,
                        ref_wave_number((int)N));
-  CGM_HIP(hipGetLastError());
+  if ((he = hipGetLastError()) != hipSuccess) goto hip_fail;
   stamp(times, 1, nts);
   // The reference iterates only when ||r0|| > tolerance (GMRES.cuh:153) and
   // then runs exactly maxiter Arnoldi steps (its early exit is commented out,
   // GMRES.cuh:253): one cycle with a zero tolerance inside it.
-  cgm_gmres_result r;
   rc = gmres_fixed_cycle(c, (const float2*)gamma, c->chi_tmp, c->vec_tmp, (float2*)sol, (int)maxiter,
                          for_gradient ? 1 : 0, (double)tolerance, hist, &r);
-  if (rc) {
-    (void)hipFree(resid);
-    (void)hipFree(hist);
-    return rc;
-  }
-  const unsigned steps = (unsigned)r.iterations;
-  double final_est = r.beta0;
-  if (steps > 0) CGM_HIP(hipMemcpy(&final_est, hist + steps, sizeof(double), hipMemcpyDeviceToHost));
+  if (rc) goto fail;
+  steps = (unsigned)r.iterations;
+  final_est = r.beta0;
+  if (steps > 0 &&
+      (he = hipMemcpy(&final_est, hist + steps, sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess)
+    goto hip_fail;
   hipLaunchKernelGGL(k_hist_to_float, dim3((maxiter + 1 + 255) / 256), dim3(256), 0, c->stream,
                      hist, resid, (int)maxiter + 1);
-  CGM_HIP(hipStreamSynchronize(c->stream));
+  if ((he = hipGetLastError()) != hipSuccess) goto hip_fail;
+  if ((he = hipStreamSynchronize(c->stream)) != hipSuccess) goto hip_fail;
   for (unsigned t = 2; t + 2 < nts; ++t) stamp(times, t, nts);
   *dev_actual_residual = resid;
   *GMRES_n = steps;
   *h_res_vs_tol_p = final_est > tolerance;
   stamp(times, nts - 2, nts);
-  CGM_HIP(hipFree(hist));
+  (void)hipFree(hist);
   stamp(times, nts - 1, nts);
   return CGM_SUCCESS;
+hip_fail:
+  cgm_set_last_error("HIP error %s in cgm_Fast_GMRES_with_CUDA: %s", hipGetErrorName(he), hipGetErrorString(he));
+  rc = CGM_ERROR_DEVICE;
+fail:
+  (void)hipStreamSynchronize(c->stream);
+  if (resid) (void)hipFree(resid);
+  if (hist) (void)hipFree(hist);
+  return rc;
 }
 
 static int gx_common(const cgm_complex64* gamma, const bool* mask, const cgm_complex64* x,
@@ -294,12 +306,13 @@ extern "C" int cgm_cuSolve_LES(cgm_complex64* A, cgm_complex64* B, int m, cgm_ct
   CGM_CHECK(c && A && B && ipiv && info, CGM_ERROR_INVALID_VALUE, "null argument");
   CGM_CHECK(m >= 1 && m <= kMaxRestart, CGM_ERROR_INVALID_VALUE, "m=%d outside [1, %d]", m, kMaxRestart);
   CGM_HIP(hipSetDevice(c->device));
-  double2* W = nullptr;
-  CGM_HIP(hipMalloc((void**)&W, sizeof(double2) * (size_t)m * m));
-  hipLaunchKernelGGL(k_lu_solve, dim3(1), dim3(256), 0, c->stream, (float2*)A, (float2*)B, m, W, ipiv, info);
+  // the FP64 copy of A lives in the context, allocated once at its largest
+  // size (kMaxRestart^2), so repeated calls allocate nothing
+  if (!c->les_ws) CGM_HIP(hipMalloc((void**)&c->les_ws, sizeof(double2) * (size_t)kMaxRestart * kMaxRestart));
+  hipLaunchKernelGGL(k_lu_solve, dim3(1), dim3(256), 0, c->stream, (float2*)A, (float2*)B, m, c->les_ws, ipiv,
+                     info);
   hipError_t e = hipGetLastError();
-  (void)hipStreamSynchronize(c->stream);
-  (void)hipFree(W);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   CGM_HIP(e);
   return CGM_SUCCESS;
 }

@@ -1341,6 +1341,7 @@ using namespace cgm::fno;
 
 struct cgm_fno {
   cgm_context* ctx = nullptr;
+  int device = 0;  // kept here so the network can be destroyed after its context
   int N = 0, B = 0;
   long n = 0;
   float chi_scale = 1.f;
@@ -1623,6 +1624,7 @@ extern "C" int cgm_fno_create(cgm_ctx_t c, int batch, float chi_scale, cgm_fno_t
   CGM_HIP(hipSetDevice(c->device));
   cgm_fno* f = new cgm_fno();
   f->ctx = c;
+  f->device = c->device;
   f->N = N;
   f->B = batch;
   f->n = (long)N * N;
@@ -1700,7 +1702,7 @@ extern "C" int cgm_fno_create(cgm_ctx_t c, int batch, float chi_scale, cgm_fno_t
 
 extern "C" int cgm_fno_destroy(cgm_fno_t f) {
   if (!f) return CGM_SUCCESS;
-  (void)hipSetDevice(f->ctx->device);
+  (void)hipSetDevice(f->device);
   fno_free(f);
   delete f;
   return CGM_SUCCESS;

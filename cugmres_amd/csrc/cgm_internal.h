@@ -212,6 +212,7 @@ struct cgm_context {
   cgm::DevState* hst = nullptr;  // pinned host mirror
   float* chi_tmp = nullptr;      // mask -> contrast conversions (N*N)
   float2* vec_tmp = nullptr;     // right-hand sides built on the device (N*N)
+  double2* les_ws = nullptr;     // cgm_cuSolve_LES: FP64 copy of A (kMaxRestart^2, lazy)
   double2* v64a = nullptr;       // refinement residual (N*N, lazy)
   float* chi32 = nullptr;        // refinement contrast in float (N*N, lazy)
   float2* zb = nullptr;          // FGMRES preconditioned basis (zb_len elements, lazy)

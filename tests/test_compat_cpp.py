@@ -1,0 +1,96 @@
+"""The reference-named C++ shim (include/cugmres_compat.hpp) and a compiled
+reference-style caller (tests/compat/caller.cpp, built by
+__graft_entry__.build()).
+
+The caller runs launch_GMRES's sequence (B/CUDA_GMRES/launch_GMRES.cuh:93-233)
+through Fast_GMRES_with_CUDA, both G_x_fft_matvec overloads and cuSolve_LES
+under their reference names (B/CUDA_GMRES/GMRES.cuh:22-35,450-458,
+B/CUDA_GMRES/Fast_matvec.cuh:1-6,31-35); its outputs are checked against the
+oracle.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, golden, rel
+
+CALLER = os.path.join(ROOT, "tests", "compat", "caller")
+
+
+def test_compat_header_compiles_standalone(tmp_path):
+    """The shim compiles as C++ on its own (no HIP headers needed) and
+    declares the reference's names with their parameter lists."""
+    src = tmp_path / "probe.cpp"
+    src.write_text(
+        '#include "cugmres_compat.hpp"\n'
+        "void (*f1)(const cuComplex*, const bool*, cuComplex*, float**, unsigned int*, cufftHandle,\n"
+        "           cublasHandle_t*, const float, const bool, const unsigned int, bool*, unsigned int,\n"
+        "           cusolverDnHandle_t, timespec*, const unsigned int) = &Fast_GMRES_with_CUDA;\n"
+        "void (*f2)(cuComplex*, bool*, cuComplex*, cuComplex*, cufftHandle, const unsigned int) =\n"
+        "    &G_x_fft_matvec;\n"
+        "void (*f3)(cuComplex*, cuComplex*, cuComplex*, cufftHandle, const unsigned int) = &G_x_fft_matvec;\n"
+        "void (*f4)(cuComplex*, cuComplex*, const int, cusolverDnHandle_t, cuComplex*, int*, int*,\n"
+        "           timespec*, unsigned int*) = &cuSolve_LES;\n"
+        "static_assert(sizeof(cuComplex) == 8, \"layout of cuComplex\");\n")
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", "-I",
+                        os.path.join(ROOT, "include"), str(src)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+def test_caller_is_built_and_links_the_library():
+    assert os.path.exists(CALLER), "tests/compat/caller missing: run __graft_entry__.build()"
+    out = subprocess.run(["ldd", CALLER], capture_output=True, text=True).stdout
+    assert "libcugmres.so" in out and "not found" not in out.split("libcugmres.so")[1].split("\n")[0]
+
+
+@pytest.mark.gpu
+def test_reference_style_caller_matches_oracle(orc, tmp_path):
+    N, m = 64, 30
+    gprep = golden("gprep.npz")["g64"].astype(np.complex64)      # IN/G_prep_64.txt
+    mask = orc.ref_cylinder_mask(N).reshape(-1).astype(np.uint8)
+    gp = tmp_path / "gprep.bin"
+    mk = tmp_path / "mask.bin"
+    gprep.tofile(gp)
+    mask.tofile(mk)
+    prefix = str(tmp_path) + "/"
+    print(f"compat caller N={N} maxiter={m}", flush=True)
+    r = subprocess.run([CALLER, str(N), str(m), str(gp), str(mk), prefix], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    fields = dict(kv.split("=") for kv in r.stdout.split() if "=" in kv)
+    assert int(fields["GMRES_n"]) == m and int(fields["grad_n"]) == m
+    assert int(fields["clock_i"]) == 2
+    assert int(fields["stamped"]) == int(fields["nts"])   # every timestamp of the schedule is written
+
+    def load(name):
+        return np.fromfile(prefix + name, np.complex64)
+
+    blk, _ = orc.gprep_to_block(N, gprep)
+    op = orc.Op(N, blk)
+    chi = orc.ref_chi(N) * mask.astype(np.float64)
+    b = orc.ref_plane_wave(N).reshape(-1).astype(np.complex128)
+    # Fast_GMRES_with_CUDA: one cycle of m steps from x0 = b
+    xr, info = op.gmres(chi, b, x0=b, restart=m, tol=0.0, max_cycles=1, history=True)
+    assert float(fields["residual0"]) == pytest.approx(info["beta0"], rel=1e-5)
+    assert float(fields["residual"]) == pytest.approx(info["history"][m], rel=1e-3)
+    assert int(fields["res_vs_tol"]) == int(float(fields["residual"]) > 0.001)
+    assert rel(load("solution.bin"), xr) < 1e-5
+    # for_gradient: A^T x = e_idx from x0 = b (GMRES.cuh:97-111, GMRES_kernels.cuh:212-244)
+    idx = N * N // 2 + N // 3
+    e = np.zeros(N * N, np.complex128)
+    e[idx] = 1.0
+    xg, ig = op.gmres(chi, e, x0=b, restart=m, tol=0.0, max_cycles=1, adjoint=True, history=True)
+    assert rel(load("gradient.bin"), xg) < 1e-5
+    assert float(fields["grad_residual"]) == pytest.approx(ig["history"][m], rel=1e-3)
+    # the two G_x_fft_matvec overloads on the plane wave
+    assert rel(load("matvec.bin"), op.matvec(chi, b, 0)) < 2e-6
+    assert rel(load("matvec_green.bin"), op.matvec(chi, b, 2)) < 2e-6
+    # cuSolve_LES on the caller's fixed 4 x 4 system
+    A = np.zeros((4, 4), np.complex128)
+    for rr in range(4):
+        for c in range(4):
+            A[rr, c] = (4.0 if rr == c else 0.0) + 0.25 * ((rr + 2 * c) % 3) + 0.125j * (rr - c)
+    B = np.array([1.0 + rr - 0.5j * rr for rr in range(4)])
+    assert rel(load("les.bin"), np.linalg.solve(A, B)) < 1e-6
