@@ -63,6 +63,11 @@ class FgmresResult(ctypes.Structure):
                 ("precond_cycles", ctypes.c_int), ("fell_back", ctypes.c_int)]
 
 
+class Precond(ctypes.Structure):
+    """cgm_precond: the tile inverses and/or the network (NULL = absent)."""
+    _fields_ = [("tiles", c_void_p), ("net", c_void_p)]
+
+
 # name -> (restype, argtypes); every symbol declared in include/cugmres.h
 SIGNATURES = {
     "cgm_status_string": (ctypes.c_char_p, [ctypes.c_int]),
@@ -93,6 +98,9 @@ SIGNATURES = {
                                          c_void_p]),
     "cgm_gmres_refine": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                         ctypes.POINTER(RefineParams), ctypes.POINTER(RefineResult)]),
+    "cgm_gmres_refine_pc": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                           ctypes.POINTER(Precond), ctypes.POINTER(RefineParams),
+                                           ctypes.POINTER(RefineResult)]),
     "cgm_fno_num_params": (ctypes.c_int, [ctypes.POINTER(ctypes.c_size_t)]),
     "cgm_fno_create": (ctypes.c_int, [c_void_p, ctypes.c_int, ctypes.c_float, ctypes.POINTER(c_void_p)]),
     "cgm_fno_destroy": (ctypes.c_int, [c_void_p]),
@@ -106,6 +114,20 @@ SIGNATURES = {
     "cgm_fgmres_batched": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_long, c_void_p, c_void_p,
                                           c_void_p, ctypes.c_int, ctypes.POINTER(FgmresParams),
                                           ctypes.POINTER(FgmresResult), c_void_p]),
+    "cgm_block_helmholtz": (ctypes.c_int, [c_void_p, ctypes.c_double, c_void_p]),
+    "cgm_block_reference": (ctypes.c_int, [c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                           c_void_p]),
+    "cgm_tiles_create": (ctypes.c_int, [c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(c_void_p)]),
+    "cgm_tiles_destroy": (ctypes.c_int, [c_void_p]),
+    "cgm_tiles_memory": (ctypes.c_int, [c_void_p, ctypes.POINTER(ctypes.c_size_t)]),
+    "cgm_tiles_factor": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_long, ctypes.c_int]),
+    "cgm_tiles_apply": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_int]),
+    "cgm_set_flexible_fp64": (ctypes.c_int, [c_void_p, c_void_p]),
+    "cgm_fgmres_pc": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.POINTER(Precond),
+                                     ctypes.POINTER(FgmresParams), ctypes.POINTER(FgmresResult), c_void_p]),
+    "cgm_fgmres_pc_batched": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_long, c_void_p, c_void_p,
+                                             ctypes.POINTER(Precond), ctypes.c_int, ctypes.POINTER(FgmresParams),
+                                             ctypes.POINTER(FgmresResult), c_void_p]),
     "cgm_reference_plane_wave": (ctypes.c_int, [c_void_p, c_void_p]),
     "cgm_reference_chi": (ctypes.c_float, [ctypes.c_int]),
     "cgm_set_mask_contrast": (ctypes.c_int, [c_void_p, ctypes.c_float]),
@@ -125,6 +147,7 @@ SIGNATURES = {
     "cgm_synchronize": (ctypes.c_int, [c_void_p]),
     "cgm_bessel01": (None, [ctypes.c_double] + [c_double_p] * 4),
     "cgm_green_values": (ctypes.c_int, [c_void_p, ctypes.c_size_t, ctypes.c_double, c_void_p]),
+    "cgm_profile_longrec": (ctypes.c_int, [c_void_p, ctypes.c_int, ctypes.c_int, c_float_p]),
     "cgm_profile_step": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, ctypes.c_int,
                                         c_float_p]),
 }

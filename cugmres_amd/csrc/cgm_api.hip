@@ -94,8 +94,9 @@ extern "C" int cgm_destroy(cgm_ctx_t c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   krylov_free(c->ws);
+  lr_free(c->lr);
   void* ptrs[] = {c->tw32, c->tw64, c->T32, c->T64, c->st, c->chi_tmp, c->vec_tmp, c->v64a, c->chi32, c->zb,
-                  c->stB, c->T32B, c->les_ws};
+                  c->stB, c->T32B, c->les_ws, c->pc_v, c->pc_r, c->pc_z, c->chi64, c->z64, c->w64, c->T64B};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->hst) (void)hipHostFree(c->hst);
@@ -337,6 +338,21 @@ extern "C" int cgm_spectrum_reference(cgm_ctx_t c, double k, double sr, double s
 extern "C" int cgm_spectrum_helmholtz(cgm_ctx_t c, double kh, cgm_complex64* g32,
                                       cgm_complex128* g64) {
   return spectrum_generated(c, 0, kh, 0.0, 0.0, g32, g64);
+}
+
+static int block_generated(cgm_context* c, int kind, double k, double sr, double sc, cgm_complex128* blk) {
+  CGM_CHECK(c && blk, CGM_ERROR_INVALID_VALUE, "null argument");
+  CGM_CHECK(k > 0 && isfinite(k), CGM_ERROR_INVALID_VALUE, "wave number must be positive");
+  CGM_HIP(hipSetDevice(c->device));
+  return green_block(c->stream, (double2*)blk, c->N, kind, k, sr, sc);
+}
+
+extern "C" int cgm_block_helmholtz(cgm_ctx_t c, double kh, cgm_complex128* blk) {
+  return block_generated(c, 0, kh, 0.0, 0.0, blk);
+}
+
+extern "C" int cgm_block_reference(cgm_ctx_t c, double k, double sr, double sc, cgm_complex128* blk) {
+  return block_generated(c, 1, k, sr, sc, blk);
 }
 
 // ---------------------------------------------------------------------------
@@ -677,7 +693,7 @@ extern "C" int cgm_gmres_batched(cgm_ctx_t c, const cgm_complex64* gamma, const 
   DevState* stB = c->stB;
   CGM_HIP(hipMemsetAsync(stB, 0, sizeof(DevState) * B, c->stream));
   if (hist) CGM_HIP(hipMemsetAsync(hist, 0, sizeof(double) * hstride * B, c->stream));
-  if ((rc = launch_norm_init(c->stream, ws, bb, stB, 1, p->tol, n, sst))) return rc;
+  if ((rc = launch_norm_init(c->stream, ws, bb, stB, 1, p->tol, n, sst, B))) return rc;
   if (p->x0_zero) CGM_HIP(hipMemsetAsync(xx, 0, sizeof(float2) * n * B, c->stream));
   // residual r = b - A x of every system into its V_0
   BatchStrides rs;
@@ -717,11 +733,11 @@ extern "C" int cgm_gmres_batched(cgm_ctx_t c, const cgm_complex64* gamma, const 
       if (j + 1 == m) continue;  // u_m needs no matvec
       if ((r = launch_pass2<float, float2>(c->plan, s, c->T32B, gq, c->tw32, &stB->done, B, us))) return r;
     }
-    if ((r = launch_backsub_update(s, ws, stB, xx, nullptr, n, sst))) return r;
+    if ((r = launch_backsub_update(s, ws, stB, xx, nullptr, n, sst, B))) return r;
     r = launch_pass1<float>(c->plan, s, xx, nullptr, chi, nullptr, kModeResidual, c->T32B, c->tw32, nullptr, B, rs);
     if (!r) r = launch_pass2<float, float2>(c->plan, s, c->T32B, gq, c->tw32, nullptr, B, rs);
     if (!r) r = launch_pass3<float>(c->plan, s, c->T32B, xx, chi, bb, ws.V, kModeResidual, c->tw32, nullptr, B, rs);
-    if (!r) r = launch_norm_init(s, ws, ws.V, stB, 0, 0.0, ws.sBasis, sst);
+    if (!r) r = launch_norm_init(s, ws, ws.V, stB, 0, 0.0, ws.sBasis, sst, B);
     return r;
   };
   // first residual, eagerly on the context stream
@@ -736,7 +752,7 @@ extern "C" int cgm_gmres_batched(cgm_ctx_t c, const cgm_complex64* gamma, const 
                                rs);
     if (rc) return rc;
   }
-  if ((rc = launch_norm_init(c->stream, ws, ws.V, stB, 0, 0.0, ws.sBasis, sst))) return rc;
+  if ((rc = launch_norm_init(c->stream, ws, ws.V, stB, 0, 0.0, ws.sBasis, sst, B))) return rc;
   if ((rc = read_states(c, B, c->stream))) return rc;
   // captured cycles run on the context's own non-blocking stream (a capture
   // needs one; the caller's stream may be the null stream)
@@ -873,10 +889,118 @@ __global__ void k_scale_inplace(float2* __restrict__ z, long sz, const double* n
   }
 }
 
+// z += rho z2 per system, rho = st->norm_out of that system (the two-level
+// preconditioner's correction).
+__global__ void k_add_scaled(float2* __restrict__ z, long sz, const float2* __restrict__ z2, long n,
+                             const DevState* st, long sst) {
+  z += (long)blockIdx.y * sz;
+  z2 += (long)blockIdx.y * n;
+  st = boff(st, (long)blockIdx.y * sst);
+  if (st->done) return;
+  const float rho = (float)st->norm_out;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    const float2 a = z[e], b = z2[e];
+    z[e] = make_float2(a.x + rho * b.x, a.y + rho * b.y);
+  }
+}
+
+// Scratch of the two-level preconditioner, `systems` vectors each.
+static int ensure_pc(cgm_context* c, long systems) {
+  const long len = systems * (long)c->N * c->N;
+  if (c->pc_v && c->pc_len >= len) return CGM_SUCCESS;
+  void* ptrs[] = {c->pc_v, c->pc_r, c->pc_z};
+  for (void* q : ptrs)
+    if (q) CGM_HIP(hipFree(q));
+  c->pc_v = c->pc_r = c->pc_z = nullptr;
+  c->pc_len = 0;
+  CGM_HIP(hipMalloc((void**)&c->pc_v, sizeof(float2) * 2 * (size_t)len));
+  CGM_HIP(hipMalloc((void**)&c->pc_r, sizeof(float2) * (size_t)len));
+  CGM_HIP(hipMalloc((void**)&c->pc_z, sizeof(float2) * (size_t)len));
+  c->pc_len = len;
+  return CGM_SUCCESS;
+}
+
+// z = P(v / nrm) for B systems with the strides of io; system k's state is
+// dst + k sst bytes (its done flag skips the work once set).
+//   tiles only:   z = T v'                              (v' = v / nrm)
+//   network only: z = F(v')
+//   both (two-level, multiplicative): z1 = T v', r = v' - A z1,
+//                 z = z1 + ||r|| F(r / ||r||)
+// The network is thus a global correction on what the tile inverses leave:
+// it is trained on the residuals r of the local solves.  The two-level form
+// needs ensure_pc(c, B) and uses the context's B-system FFT scratch `Tm`.
+static int precond_apply(cgm_context* c, const float2* gq, const Precond& P, hipStream_t st, const FnoIO& io, int B,
+                         DevState* dst, long sst, float2* Tm) {
+  const int* done = &dst->done;
+  if (!P.tiles) return fno_forward_io(P.net, st, io, B, false, B == 1 ? done : nullptr);
+  if (!P.net) return tiles_apply_io(P.tiles, st, io, B, done, sst);
+  const long n = (long)c->N * c->N;
+  // v' (normalised input): the caller's write-back target, or the scratch
+  // (the write-back uses the input's stride io.sV; the scratch holds
+  // 2 n per system, the largest input stride of the flexible solvers)
+  FnoIO t = io;
+  const float2* vh = io.v;
+  const long svh = io.sV;
+  if (io.nrm) {
+    if (!io.vw) {
+      if ((long)(B - 1) * io.sV + n > 2 * c->pc_len) {
+        cgm_set_last_error("two-level preconditioner: input stride %ld exceeds the scratch", io.sV);
+        return CGM_ERROR_UNSUPPORTED;
+      }
+      t.vw = c->pc_v;
+    }
+    vh = t.vw;
+  }
+  int rc = tiles_apply_io(P.tiles, st, t, B, done, sst);
+  if (rc) return rc;
+  // r = v' - A z1 into pc_r
+  BatchStrides bs;
+  bs.x = io.sZ;
+  bs.chi = io.sChi;
+  bs.b = svh;
+  bs.y = n;
+  bs.T = (long)c->N * c->M;
+  bs.st = sst;
+  rc = launch_pass1<float>(c->plan, st, io.z, nullptr, io.chi, nullptr, kModeResidual, Tm, c->tw32, done, B, bs);
+  if (!rc) rc = launch_pass2<float, float2>(c->plan, st, Tm, gq, c->tw32, done, B, bs);
+  if (!rc)
+    rc = launch_pass3<float>(c->plan, st, Tm, io.z, io.chi, vh, c->pc_r, kModeResidual, c->tw32, done, B, bs);
+  // ||r|| into norm_out of each system
+  if (!rc) rc = launch_norm_init(st, c->ws, c->pc_r, dst, 2, 0.0, n, sst, B);
+  if (rc) return rc;
+  FnoIO f;
+  f.chi = io.chi;
+  f.sChi = io.sChi;
+  f.v = c->pc_r;
+  f.sV = n;
+  f.nrm = &dst->norm_out;
+  f.sNrm = sst;
+  f.z = c->pc_z;
+  f.sZ = n;
+  if ((rc = fno_forward_io(P.net, st, f, B, false, B == 1 ? done : nullptr))) return rc;
+  hipLaunchKernelGGL(k_add_scaled, dim3(c->ws.G, B), dim3(kKB), 0, st, io.z, io.sZ, c->pc_z, n, dst, sst);
+  CGM_HIP(hipGetLastError());
+  return CGM_SUCCESS;
+}
+
+static FnoIO single_io(long n, const float* chi, const float2* v, const double* nrm, float2* vw, float2* z) {
+  FnoIO io;
+  io.chi = chi;
+  io.sChi = n;
+  io.v = v;
+  io.sV = n;
+  io.nrm = nrm;
+  io.vw = vw;
+  io.z = z;
+  io.sZ = n;
+  return io;
+}
+
 // One preconditioned cycle: z_i = P(v_i), w = A z_i, CGS2 step; the caller
 // applies x += Z y.
-static int precond_cycle(cgm_context* c, const float2* gq, const float* chi, cgm_fno* net, int m, double* hist,
+static int precond_cycle(cgm_context* c, const float2* gq, const float* chi, const Precond& P, int m, double* hist,
                          int total) {
+  const bool has_pc = P.net || P.tiles;
   KrylovWs& ws = c->ws;
   const long n = ws.n;
   int rc;
@@ -901,8 +1025,8 @@ static int precond_cycle(cgm_context* c, const float2* gq, const float* chi, cgm
       if (j < m) {
         float2* zj = c->zb + (long)j * n;
         const double* nrm = j == 0 ? &c->st->beta : &c->st->unorm;
-        if (net) {
-          rc = fno_forward(net, c->stream, chi, L.Uj, nrm, nullptr, zj, 1, false, &c->st->done);
+        if (has_pc) {
+          rc = precond_apply(c, gq, P, c->stream, single_io(n, chi, L.Uj, nrm, nullptr, zj), 1, c->st, 0, c->T32);
         } else {
           hipLaunchKernelGGL(k_scale_copy, dim3(ws.G), dim3(kKB), 0, c->stream, L.Uj, nrm, zj, n, &c->st->done);
           rc = hipGetLastError() == hipSuccess ? CGM_SUCCESS : CGM_ERROR_DEVICE;
@@ -924,8 +1048,8 @@ static int precond_cycle(cgm_context* c, const float2* gq, const float* chi, cgm
     float2* vi = ws.V + (long)i * n;
     float2* zi = c->zb + (long)i * n;
     const double* nrm = (i == 0) ? &c->st->beta : &c->st->hn;
-    if (net) {
-      rc = fno_forward(net, c->stream, chi, vi, nrm, vi, zi, 1, false, &c->st->done);
+    if (has_pc) {
+      rc = precond_apply(c, gq, P, c->stream, single_io(n, chi, vi, nrm, vi, zi), 1, c->st, 0, c->T32);
     } else {
       hipLaunchKernelGGL(k_norm_copy, dim3(ws.G), dim3(kKB), 0, c->stream, vi, nrm, zi, n, &c->st->done);
       rc = hipGetLastError() == hipSuccess ? CGM_SUCCESS : CGM_ERROR_DEVICE;
@@ -939,23 +1063,355 @@ static int precond_cycle(cgm_context* c, const float2* gq, const float* chi, cgm
   return CGM_SUCCESS;
 }
 
+// The preconditioner must belong to the context and, for the tiles, hold
+// maps for the systems (one shared map, or at least `systems`).
+static int check_precond(cgm_context* c, const Precond& P, int systems, long chi_stride) {
+  CGM_CHECK(!P.net || fno_context(P.net) == c, CGM_ERROR_INVALID_VALUE,
+            "the preconditioner was created on another context");
+  CGM_CHECK(!P.net || fno_batch(P.net) >= systems, CGM_ERROR_INVALID_VALUE,
+            "the preconditioner was created for %d vectors per launch, fewer than batch %d",
+            P.net ? fno_batch(P.net) : 0, systems);
+  if (P.tiles) {
+    CGM_CHECK(tiles_context(P.tiles) == c, CGM_ERROR_INVALID_VALUE, "the tiles were created on another context");
+    const int maps = tiles_maps(P.tiles);
+    CGM_CHECK(maps >= 1, CGM_ERROR_INVALID_VALUE, "the tile preconditioner has not been factored");
+    CGM_CHECK(maps == 1 ? (systems == 1 || chi_stride == 0) : (maps >= systems && chi_stride != 0),
+              CGM_ERROR_INVALID_VALUE, "%d factored maps do not match %d systems (chi_stride %ld)", maps, systems,
+              chi_stride);
+  }
+  return CGM_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------
+// Long-recurrence flexible GMRES (restart > 64): B systems (B = 1 uses the
+// context's own state), cycles of up to kMaxRestartLR steps with the whole
+// basis in HBM (cgm_longrec.hip).
+// ---------------------------------------------------------------------------
+struct LrCall {
+  cgm_context* c;
+  const float2* gq;
+  const float* chi;
+  long chi_stride;
+  const Precond* P;
+  bool use_pc;
+  DevState* dst;
+  long sst;
+  int B;
+  float2* Tm;
+};
+
+static int lr_precond_step(void* vp, int, const float2* u, long su, float2* z, long sz) {
+  const LrCall& L = *static_cast<const LrCall*>(vp);
+  const long n = (long)L.c->N * L.c->N;
+  if (!L.use_pc) return lr_scale(L.c->stream, u, su, z, sz, n, L.dst, L.sst, L.B);
+  FnoIO io;
+  io.chi = L.chi;
+  io.sChi = L.chi_stride;
+  io.v = u;
+  io.sV = su;
+  io.nrm = &L.dst->unorm;
+  io.sNrm = L.sst;
+  io.z = z;
+  io.sZ = sz;
+  return precond_apply(L.c, L.gq, *L.P, L.c->stream, io, L.B, L.dst, L.sst, L.Tm);
+}
+
+// Strided precision conversions of B vectors of n elements (grid y = vector).
+__global__ void k_c64_to_c128(const float2* __restrict__ a, long sa, double2* __restrict__ o, long so, long n) {
+  a += (long)blockIdx.y * sa;
+  o += (long)blockIdx.y * so;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x)
+    o[e] = make_double2(a[e].x, a[e].y);
+}
+__global__ void k_c128_to_c64(const double2* __restrict__ a, long sa, float2* __restrict__ o, long so, long n) {
+  a += (long)blockIdx.y * sa;
+  o += (long)blockIdx.y * so;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x)
+    o[e] = make_float2((float)a[e].x, (float)a[e].y);
+}
+// y = b - w (b complex64, w complex128 = A x), rounded to complex64.
+__global__ void k_resid_mixed(const float2* __restrict__ b, long sb, const double2* __restrict__ w, long sw,
+                              float2* __restrict__ y, long sy, long n) {
+  b += (long)blockIdx.y * sb;
+  w += (long)blockIdx.y * sw;
+  y += (long)blockIdx.y * sy;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x)
+    y[e] = make_float2((float)((double)b[e].x - w[e].x), (float)((double)b[e].y - w[e].y));
+}
+__global__ void k_f32_to_f64(const float* __restrict__ a, long sa, double* __restrict__ o, long so, long n) {
+  a += (long)blockIdx.y * sa;
+  o += (long)blockIdx.y * so;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) o[e] = a[e];
+}
+
+extern "C" int cgm_set_flexible_fp64(cgm_ctx_t c, const cgm_complex128* g64) {
+  CGM_CHECK(c, CGM_ERROR_INVALID_VALUE, "null context");
+  c->flex_g64 = (const double2*)g64;
+  return CGM_SUCCESS;
+}
+
+// FP64 buffers of the long-recurrence matvecs for B systems and `maps` maps.
+static int ensure_flex64(cgm_context* c, int B, int maps) {
+  const long n = (long)c->N * c->N;
+  if (c->chi64_len < (long)maps * n) {
+    if (c->chi64) CGM_HIP(hipFree(c->chi64));
+    c->chi64 = nullptr;
+    c->chi64_len = 0;
+    CGM_HIP(hipMalloc((void**)&c->chi64, sizeof(double) * (size_t)maps * n));
+    c->chi64_len = (long)maps * n;
+  }
+  if (c->v64_len < (long)B * n) {
+    if (c->z64) CGM_HIP(hipFree(c->z64));
+    if (c->w64) CGM_HIP(hipFree(c->w64));
+    c->z64 = c->w64 = nullptr;
+    c->v64_len = 0;
+    CGM_HIP(hipMalloc((void**)&c->z64, sizeof(double2) * (size_t)B * n));
+    CGM_HIP(hipMalloc((void**)&c->w64, sizeof(double2) * (size_t)B * n));
+    c->v64_len = (long)B * n;
+  }
+  if (c->T64B_n < B) {
+    if (c->T64B) CGM_HIP(hipFree(c->T64B));
+    c->T64B = nullptr;
+    c->T64B_n = 0;
+    CGM_HIP(hipMalloc((void**)&c->T64B, sizeof(double2) * (size_t)B * c->N * c->M));
+    c->T64B_n = B;
+  }
+  return CGM_SUCCESS;
+}
+
+// y = b - A x (resid) or A x in FP64 for B systems: x (complex64, stride sx)
+// is widened into z64, the FP64 passes write w64, which is rounded into y
+// (stride sy).  Exact for the complex64 input up to FP64 rounding.
+static int matvec64_from32(cgm_context* c, const float2* x, long sx, const float2* b, long sb, float2* y, long sy,
+                           bool resid, long chi_stride, int B, const int* done, long sst) {
+  const long n = (long)c->N * c->N;
+  const dim3 grid((unsigned)((n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024), B);
+  hipLaunchKernelGGL(k_c64_to_c128, grid, dim3(256), 0, c->stream, x, sx, c->z64, n, n);
+  CGM_HIP(hipGetLastError());
+  BatchStrides bs;
+  bs.x = n;
+  bs.chi = chi_stride ? n : 0;
+  bs.y = n;
+  bs.T = (long)c->N * c->M;
+  bs.st = sst;
+  int rc = launch_pass1<double>(c->plan64, c->stream, c->z64, nullptr, c->chi64, nullptr, kModeForward, c->T64B,
+                                c->tw64, done, B, bs);
+  if (!rc) rc = launch_pass2<double, double2>(c->plan64, c->stream, c->T64B, c->flex_g64, c->tw64, done, B, bs);
+  if (!rc)
+    rc = launch_pass3<double>(c->plan64, c->stream, c->T64B, c->z64, c->chi64, nullptr, c->w64, kModeForward,
+                              c->tw64, done, B, bs);
+  if (rc) return rc;
+  if (resid)
+    hipLaunchKernelGGL(k_resid_mixed, grid, dim3(256), 0, c->stream, b, sb, c->w64, n, y, sy, n);
+  else
+    hipLaunchKernelGGL(k_c128_to_c64, grid, dim3(256), 0, c->stream, c->w64, n, y, sy, n);
+  CGM_HIP(hipGetLastError());
+  return CGM_SUCCESS;
+}
+
+static int lr_matvec_step(void* vp, int, const float2* z, long sz, float2* w, long sw) {
+  const LrCall& L = *static_cast<const LrCall*>(vp);
+  cgm_context* c = L.c;
+  if (c->flex_g64) return matvec64_from32(c, z, sz, nullptr, 0, w, sw, false, L.chi_stride, L.B, &L.dst->done, L.sst);
+  BatchStrides bs;
+  bs.x = sz;
+  bs.chi = L.chi_stride;
+  bs.y = sw;
+  bs.T = (long)c->N * c->M;
+  bs.st = L.sst;
+  const int* done = &L.dst->done;
+  int rc = launch_pass1<float>(c->plan, c->stream, z, nullptr, L.chi, nullptr, kModeForward, L.Tm, c->tw32, done, L.B,
+                               bs);
+  if (!rc) rc = launch_pass2<float, float2>(c->plan, c->stream, L.Tm, L.gq, c->tw32, done, L.B, bs);
+  if (!rc)
+    rc = launch_pass3<float>(c->plan, c->stream, L.Tm, z, L.chi, nullptr, w, kModeForward, c->tw32, done, L.B, bs);
+  return rc;
+}
+
+static int ensure_states(cgm_context* c, int B) {
+  if (c->stB_n >= B) return CGM_SUCCESS;
+  if (c->stB) CGM_HIP(hipFree(c->stB));
+  if (c->hstB) CGM_HIP(hipHostFree(c->hstB));
+  c->stB = nullptr;
+  c->hstB = nullptr;
+  c->stB_n = 0;
+  CGM_HIP(hipMalloc((void**)&c->stB, sizeof(DevState) * B));
+  CGM_HIP(hipHostMalloc((void**)&c->hstB, sizeof(DevState) * B));
+  c->stB_n = B;
+  return CGM_SUCCESS;
+}
+
+// results[k] for k < B; `batched` selects the per-system state array (B may
+// be 1 there too).  stall_ratio applies to B = 1 only (the caller checks).
+static int lr_solve(cgm_context* c, const float2* gq, const float* chi, long chi_stride, const float2* b, float2* x,
+                    const Precond& P, int B, bool batched, const cgm_fgmres_params* p, cgm_fgmres_result* res,
+                    double* hist) {
+  const long n = (long)c->N * c->N;
+  const int m = p->restart;
+  int rc = krylov_alloc(c->ws, n, 1, c->stream, B);
+  if (!rc) rc = lr_alloc(c->lr, n, m, B);
+  if (rc) {
+    lr_free(c->lr);
+    cgm_set_last_error("long-recurrence workspace of %zu MiB for %d systems, restart %d: allocation failed",
+                       lr_bytes(n, m, B) >> 20, B, m);
+    return CGM_ERROR_ALLOC;
+  }
+  DevState* dst = c->st;
+  DevState* hst = c->hst;
+  long sst = 0;
+  float2* Tm = c->T32;
+  if (batched) {
+    if ((rc = ensure_states(c, B))) return rc;
+    if ((rc = ensure_T32B(c, B))) return rc;
+    dst = c->stB;
+    hst = c->hstB;
+    sst = (long)sizeof(DevState);
+    Tm = c->T32B;
+  }
+  if (P.net && P.tiles && (rc = ensure_pc(c, B))) return rc;
+  if (c->flex_g64) {
+    const int maps = chi_stride ? B : 1;
+    if ((rc = ensure_flex64(c, B, maps))) return rc;
+    const dim3 grid((unsigned)((n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024), maps);
+    hipLaunchKernelGGL(k_f32_to_f64, grid, dim3(256), 0, c->stream, chi, chi_stride, c->chi64, n, n);
+    CGM_HIP(hipGetLastError());
+  }
+  const long hstride = (long)m * (p->max_cycles > 0 ? p->max_cycles : 1) + 1;
+  CGM_HIP(hipMemsetAsync(dst, 0, sizeof(DevState) * (batched ? B : 1), c->stream));
+  if (hist) CGM_HIP(hipMemsetAsync(hist, 0, sizeof(double) * hstride * B, c->stream));
+  if ((rc = launch_norm_init(c->stream, c->ws, b, dst, 1, p->tol, n, sst, B))) return rc;
+  if (p->x0_zero) CGM_HIP(hipMemsetAsync(x, 0, sizeof(float2) * n * B, c->stream));
+  LrCall L{c, gq, chi, chi_stride, &P, P.net || P.tiles, dst, sst, B, Tm};
+  const LrStep pre{lr_precond_step, &L};
+  const LrStep mv{lr_matvec_step, &L};
+  BatchStrides rs;  // r = b - A x into u_0 of every system
+  rs.x = n;
+  rs.chi = chi_stride;
+  rs.b = n;
+  rs.y = 2 * n;
+  rs.T = (long)c->N * c->M;
+  rs.st = sst;
+  std::vector<int> fin(B, 0);
+  std::vector<double> beta_prev(B, 0.0);
+  int cycles = 0;
+  bool last_pc = false;
+  for (;;) {
+    if (p->x0_zero && cycles == 0) {
+      rc = copy_strided(c->stream, b, n, c->lr.U, 2 * n, n, B);
+    } else if (c->flex_g64) {
+      rc = matvec64_from32(c, x, n, b, n, c->lr.U, 2 * n, true, chi_stride, B, nullptr, sst);
+    } else {
+      rc = launch_pass1<float>(c->plan, c->stream, x, nullptr, chi, nullptr, kModeResidual, Tm, c->tw32, nullptr, B, rs);
+      if (!rc) rc = launch_pass2<float, float2>(c->plan, c->stream, Tm, gq, c->tw32, nullptr, B, rs);
+      if (!rc)
+        rc = launch_pass3<float>(c->plan, c->stream, Tm, x, chi, b, c->lr.U, kModeResidual, c->tw32, nullptr, B, rs);
+    }
+    if (!rc) rc = launch_norm_init(c->stream, c->ws, c->lr.U, dst, 0, 0.0, 2 * n, sst, B);
+    if (rc) return rc;
+    CGM_HIP(hipMemcpyAsync(hst, dst, sizeof(DevState) * (batched ? B : 1), hipMemcpyDeviceToHost, c->stream));
+    CGM_HIP(hipStreamSynchronize(c->stream));
+    int active = 0;
+    for (int k = 0; k < B; ++k) {
+      const DevState& h = *boff(hst, (long)k * sst);
+      cgm_fgmres_result& r = res[k];
+      if (cycles == 0) {
+        r.beta0 = h.beta;
+        r.bnorm = h.bnorm;
+        if (hist)
+          CGM_HIP(hipMemcpyAsync(hist + k * hstride, &boff(hst, (long)k * sst)->beta, sizeof(double),
+                                 hipMemcpyHostToDevice, c->stream));
+      }
+      if (fin[k]) continue;
+      r.rel_residual = h.bnorm > 0 ? h.beta / h.bnorm : 0.0;
+      if (h.status) {
+        fin[k] = 1;
+        continue;
+      }
+      if (h.bnorm == 0.0 || h.beta <= p->tol * h.bnorm) {
+        r.converged = 1;
+        fin[k] = 1;
+        continue;
+      }
+      ++active;
+    }
+    if (!active || cycles >= p->max_cycles) break;
+    // the stall rule of cgm_fgmres (one system): a preconditioned cycle that
+    // left more than stall_ratio of the residual hands over to the identity
+    if (B == 1 && L.use_pc && last_pc && p->stall_ratio > 0.0 && hst->beta > p->stall_ratio * beta_prev[0]) {
+      L.use_pc = false;
+      res[0].fell_back = 1;
+    }
+    beta_prev[0] = hst->beta;
+    last_pc = L.use_pc;
+    if ((rc = lr_cycle(c->lr, c->stream, dst, hst, sst, B, m, hist ? hist + (long)cycles * m : nullptr, hstride, x, n,
+                       pre, mv)))
+      return rc;
+    CGM_HIP(hipMemcpyAsync(hst, dst, sizeof(DevState) * (batched ? B : 1), hipMemcpyDeviceToHost, c->stream));
+    CGM_HIP(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < B; ++k) {
+      if (fin[k]) continue;
+      const DevState& h = *boff(hst, (long)k * sst);
+      res[k].iterations += h.k_last;
+      res[k].cycles += 1;
+      if (L.use_pc) res[k].precond_cycles += 1;
+    }
+    ++cycles;
+  }
+  int any_nonfinite = 0, all_conv = 1;
+  for (int k = 0; k < B; ++k) {
+    if (boff(hst, (long)k * sst)->status) any_nonfinite = 1;
+    if (!res[k].converged) all_conv = 0;
+  }
+  if (any_nonfinite) {
+    cgm_set_last_error("flexible GMRES: non-finite value in at least one system");
+    return CGM_ERROR_NONFINITE;
+  }
+  return all_conv ? CGM_SUCCESS : CGM_ERROR_NOT_CONVERGED;
+}
+
+static int fgmres_core(cgm_context* c, const cgm_complex64* gamma, const float* chi, const cgm_complex64* b,
+                       cgm_complex64* x, const Precond& P, const cgm_fgmres_params* p, cgm_fgmres_result* res,
+                       double* hist);
+
 extern "C" int cgm_fgmres(cgm_ctx_t c, const cgm_complex64* gamma, const float* chi, const cgm_complex64* b,
                           cgm_complex64* x, cgm_fno_t net, const cgm_fgmres_params* p, cgm_fgmres_result* res,
                           double* hist) {
+  Precond P;
+  P.net = net;
+  return fgmres_core(c, gamma, chi, b, x, P, p, res, hist);
+}
+
+extern "C" int cgm_fgmres_pc(cgm_ctx_t c, const cgm_complex64* gamma, const float* chi, const cgm_complex64* b,
+                             cgm_complex64* x, const cgm_precond* pc, const cgm_fgmres_params* p,
+                             cgm_fgmres_result* res, double* hist) {
+  Precond P;
+  if (pc) {
+    P.net = pc->net;
+    P.tiles = pc->tiles;
+  }
+  return fgmres_core(c, gamma, chi, b, x, P, p, res, hist);
+}
+
+static int fgmres_core(cgm_context* c, const cgm_complex64* gamma, const float* chi, const cgm_complex64* b,
+                       cgm_complex64* x, const Precond& P, const cgm_fgmres_params* p, cgm_fgmres_result* res,
+                       double* hist) {
   CGM_CHECK(c && gamma && chi && b && x && res && p, CGM_ERROR_INVALID_VALUE, "null argument");
-  CGM_CHECK(p->restart >= 1 && p->restart <= kMaxRestart, CGM_ERROR_INVALID_VALUE, "restart %d outside [1, %d]",
-            p->restart, kMaxRestart);
+  CGM_CHECK(p->restart >= 1 && p->restart <= kMaxRestartLR, CGM_ERROR_INVALID_VALUE, "restart %d outside [1, %d]",
+            p->restart, kMaxRestartLR);
   CGM_CHECK(p->max_cycles >= 0, CGM_ERROR_INVALID_VALUE, "max_cycles < 0");
   CGM_CHECK(p->tol >= 0 && isfinite(p->tol), CGM_ERROR_INVALID_VALUE, "bad tolerance");
-  CGM_CHECK(!net || fno_context(net) == c, CGM_ERROR_INVALID_VALUE,
-            "the preconditioner was created on another context");
+  int rc = check_precond(c, P, 1, 0);
+  if (rc) return rc;
   CGM_HIP(hipSetDevice(c->device));
   memset(res, 0, sizeof(*res));
+  if (p->restart > kFusedMaxRestart)
+    return lr_solve(c, (const float2*)gamma, chi, 0, (const float2*)b, (float2*)x, P, 1, false, p, res, hist);
   const long n = (long)c->N * c->N;
   const int m = p->restart;
-  int rc = krylov_alloc(c->ws, n, m, c->stream);
+  rc = krylov_alloc(c->ws, n, m, c->stream);
   if (rc) return rc;
   if ((rc = ensure_zb(c, n * m))) return rc;
+  if (P.net && P.tiles && (rc = ensure_pc(c, 1))) return rc;
   KrylovWs& ws = c->ws;
   const float2* bb = (const float2*)b;
   float2* xx = (float2*)x;
@@ -998,7 +1454,7 @@ extern "C" int cgm_fgmres(cgm_ctx_t c, const cgm_complex64* gamma, const float* 
     }
     beta_prev = h.beta;
     if (precond) {
-      if ((rc = precond_cycle(c, (const float2*)gamma, chi, net, m, hist, total))) return rc;
+      if ((rc = precond_cycle(c, (const float2*)gamma, chi, P, m, hist, total))) return rc;
       if ((rc = launch_backsub_update(c->stream, ws, c->st, xx, c->zb))) return rc;
       res->precond_cycles += 1;
     } else {
@@ -1025,28 +1481,54 @@ extern "C" int cgm_fgmres(cgm_ctx_t c, const cgm_complex64* gamma, const float* 
 // Batched flexible GMRES: B systems of one grid size in lockstep, one network
 // launch per step for all of them
 // ---------------------------------------------------------------------------
+static int fgmres_batched_core(cgm_context* c, const cgm_complex64* gamma, const float* chi, long chi_stride,
+                               const cgm_complex64* b, cgm_complex64* x, const Precond& P, int batch,
+                               const cgm_fgmres_params* p, cgm_fgmres_result* results, double* hist);
+
 extern "C" int cgm_fgmres_batched(cgm_ctx_t c, const cgm_complex64* gamma, const float* chi, long chi_stride,
                                   const cgm_complex64* b, cgm_complex64* x, cgm_fno_t net, int batch,
                                   const cgm_fgmres_params* p, cgm_fgmres_result* results, double* hist) {
+  Precond P;
+  P.net = net;
+  return fgmres_batched_core(c, gamma, chi, chi_stride, b, x, P, batch, p, results, hist);
+}
+
+extern "C" int cgm_fgmres_pc_batched(cgm_ctx_t c, const cgm_complex64* gamma, const float* chi, long chi_stride,
+                                     const cgm_complex64* b, cgm_complex64* x, const cgm_precond* pc, int batch,
+                                     const cgm_fgmres_params* p, cgm_fgmres_result* results, double* hist) {
+  Precond P;
+  if (pc) {
+    P.net = pc->net;
+    P.tiles = pc->tiles;
+  }
+  return fgmres_batched_core(c, gamma, chi, chi_stride, b, x, P, batch, p, results, hist);
+}
+
+static int fgmres_batched_core(cgm_context* c, const cgm_complex64* gamma, const float* chi, long chi_stride,
+                               const cgm_complex64* b, cgm_complex64* x, const Precond& P, int batch,
+                               const cgm_fgmres_params* p, cgm_fgmres_result* results, double* hist) {
   CGM_CHECK(c && gamma && chi && b && x && results && p, CGM_ERROR_INVALID_VALUE, "null argument");
   CGM_CHECK(batch >= 1, CGM_ERROR_INVALID_VALUE, "batch %d < 1", batch);
   CGM_CHECK(chi_stride == 0 || chi_stride >= (long)c->N * c->N, CGM_ERROR_INVALID_VALUE,
             "chi_stride must be 0 (shared map) or >= N*N");
-  CGM_CHECK(p->restart >= 1 && p->restart <= kMaxRestart, CGM_ERROR_INVALID_VALUE, "restart %d outside [1, %d]",
-            p->restart, kMaxRestart);
+  CGM_CHECK(p->restart >= 1 && p->restart <= kMaxRestartLR, CGM_ERROR_INVALID_VALUE, "restart %d outside [1, %d]",
+            p->restart, kMaxRestartLR);
   CGM_CHECK(p->max_cycles >= 0, CGM_ERROR_INVALID_VALUE, "max_cycles < 0");
   CGM_CHECK(p->tol >= 0 && isfinite(p->tol), CGM_ERROR_INVALID_VALUE, "bad tolerance");
   CGM_CHECK(p->stall_ratio == 0.0, CGM_ERROR_UNSUPPORTED,
             "cgm_fgmres_batched has no plain-GMRES fallback: stall_ratio must be 0");
-  CGM_CHECK(fused_supported(c->N, p->restart), CGM_ERROR_UNSUPPORTED,
-            "cgm_fgmres_batched needs a power-of-two N in [64, 2048] and restart <= 64");
-  CGM_CHECK(!net || fno_context(net) == c, CGM_ERROR_INVALID_VALUE,
-            "the preconditioner was created on another context");
-  CGM_CHECK(!net || fno_batch(net) >= batch, CGM_ERROR_INVALID_VALUE,
-            "the preconditioner was created for %d vectors per launch, fewer than batch %d",
-            net ? fno_batch(net) : 0, batch);
+  CGM_CHECK(p->restart > kFusedMaxRestart || fused_supported(c->N, p->restart), CGM_ERROR_UNSUPPORTED,
+            "cgm_fgmres_batched with restart <= 64 needs a power-of-two N in [64, 2048]");
+  {
+    const int rcp = check_precond(c, P, batch, chi_stride);
+    if (rcp) return rcp;
+  }
+  const bool has_pc = P.net || P.tiles;
   CGM_HIP(hipSetDevice(c->device));
   memset(results, 0, sizeof(cgm_fgmres_result) * batch);
+  if (p->restart > kFusedMaxRestart)
+    return lr_solve(c, (const float2*)gamma, chi, chi_stride, (const float2*)b, (float2*)x, P, batch, true, p,
+                    results, hist);
   const long n = (long)c->N * c->N;
   const int m = p->restart;
   const int B = batch;
@@ -1066,6 +1548,7 @@ extern "C" int cgm_fgmres_batched(cgm_ctx_t c, const cgm_complex64* gamma, const
   }
   if ((rc = ensure_T32B(c, B))) return rc;
   if ((rc = ensure_zb(c, ws.sBasis * B))) return rc;  // Z of system k at zb + k * sBasis
+  if (P.net && P.tiles && (rc = ensure_pc(c, B))) return rc;
   const long NM = (long)c->N * c->M;
   const long sst = (long)sizeof(DevState);
   const float2* gq = (const float2*)gamma;
@@ -1074,7 +1557,7 @@ extern "C" int cgm_fgmres_batched(cgm_ctx_t c, const cgm_complex64* gamma, const
   DevState* stB = c->stB;
   CGM_HIP(hipMemsetAsync(stB, 0, sizeof(DevState) * B, c->stream));
   if (hist) CGM_HIP(hipMemsetAsync(hist, 0, sizeof(double) * hstride * B, c->stream));
-  if ((rc = launch_norm_init(c->stream, ws, bb, stB, 1, p->tol, n, sst))) return rc;
+  if ((rc = launch_norm_init(c->stream, ws, bb, stB, 1, p->tol, n, sst, B))) return rc;
   if (p->x0_zero) CGM_HIP(hipMemsetAsync(xx, 0, sizeof(float2) * n * B, c->stream));
   BatchStrides rs;  // residual r = b - A x of every system into its V_0
   rs.x = n;
@@ -1089,7 +1572,7 @@ extern "C" int cgm_fgmres_batched(cgm_ctx_t c, const cgm_complex64* gamma, const
     int r = launch_pass1<float>(c->plan, s, xx, nullptr, chi, nullptr, kModeResidual, c->T32B, c->tw32, nullptr, B, rs);
     if (!r) r = launch_pass2<float, float2>(c->plan, s, c->T32B, gq, c->tw32, nullptr, B, rs);
     if (!r) r = launch_pass3<float>(c->plan, s, c->T32B, xx, chi, bb, ws.V, kModeResidual, c->tw32, nullptr, B, rs);
-    if (!r) r = launch_norm_init(s, ws, ws.V, stB, 0, 0.0, ws.sBasis, sst);
+    if (!r) r = launch_norm_init(s, ws, ws.V, stB, 0, 0.0, ws.sBasis, sst, B);
     return r;
   };
   // one flexible restart cycle of every live system, x += Z y, the next
@@ -1117,7 +1600,7 @@ extern "C" int cgm_fgmres_batched(cgm_ctx_t c, const cgm_complex64* gamma, const
       if (j < m) {
         float2* zj = c->zb + (long)j * n;
         const double* nrm = j == 0 ? &stB->beta : &stB->unorm;
-        if (net) {
+        if (has_pc) {
           FnoIO io;
           io.chi = chi;
           io.sChi = chi_stride;
@@ -1127,7 +1610,7 @@ extern "C" int cgm_fgmres_batched(cgm_ctx_t c, const cgm_complex64* gamma, const
           io.sNrm = sst;
           io.z = zj;
           io.sZ = ws.sBasis;
-          r = fno_forward_io(net, s, io, B, false, nullptr);
+          r = precond_apply(c, gq, P, s, io, B, stB, sst, c->T32B);
         } else {
           r = copy_strided(s, L.Uj, 2 * n, zj, ws.sBasis, n, B);  // z = u (scaled below)
           if (!r) {
@@ -1146,13 +1629,13 @@ extern "C" int cgm_fgmres_batched(cgm_ctx_t c, const cgm_complex64* gamma, const
       if (j == m) break;
       if ((r = launch_fused(s, 1, L))) return r;
     }
-    if ((r = launch_backsub_update(s, ws, stB, xx, c->zb, n, sst))) return r;
+    if ((r = launch_backsub_update(s, ws, stB, xx, c->zb, n, sst, B))) return r;
     return residual(s);
   };
   // first residual, eagerly on the context stream
   if (p->x0_zero) {
     if ((rc = copy_strided(c->stream, bb, n, ws.V, ws.sBasis, n, B))) return rc;
-    if ((rc = launch_norm_init(c->stream, ws, ws.V, stB, 0, 0.0, ws.sBasis, sst))) return rc;
+    if ((rc = launch_norm_init(c->stream, ws, ws.V, stB, 0, 0.0, ws.sBasis, sst, B))) return rc;
   } else if ((rc = residual(c->stream))) {
     return rc;
   }
@@ -1179,7 +1662,7 @@ extern "C" int cgm_fgmres_batched(cgm_ctx_t c, const cgm_complex64* gamma, const
       } else if (!done[k]) {
         r.iterations += h.k_last;
         r.cycles += 1;
-        r.precond_cycles += net ? 1 : 0;
+        r.precond_cycles += has_pc ? 1 : 0;
       }
       if (done[k]) continue;
       r.rel_residual = h.bnorm > 0 ? h.beta / h.bnorm : 0.0;
@@ -1262,12 +1745,20 @@ __global__ void k_d2f(const double* __restrict__ a, float* __restrict__ o, long 
     o[e] = (float)a[e];
 }
 
-extern "C" int cgm_gmres_refine(cgm_ctx_t c, const cgm_complex64* g32, const cgm_complex128* g64,
-                                const double* chi, const cgm_complex128* b, cgm_complex128* x,
-                                const cgm_refine_params* p, cgm_refine_result* res) {
+// FP64 refinement; the correction is solved by complex64 GMRES(inner_restart)
+// (P null) or by flexible GMRES with the preconditioner P (restart up to
+// kMaxRestartLR: the long-recurrence path above 64).
+static int refine_core(cgm_context* c, const cgm_complex64* g32, const cgm_complex128* g64, const double* chi,
+                       const cgm_complex128* b, cgm_complex128* x, const Precond* P, const cgm_refine_params* p,
+                       cgm_refine_result* res) {
   CGM_CHECK(c && g32 && g64 && chi && b && x && p && res, CGM_ERROR_INVALID_VALUE, "null argument");
-  CGM_CHECK(p->inner_restart >= 1 && p->inner_restart <= kMaxRestart && p->max_outer >= 0,
-            CGM_ERROR_INVALID_VALUE, "bad refinement parameters");
+  const int maxr = P ? kMaxRestartLR : kMaxRestart;
+  CGM_CHECK(p->inner_restart >= 1 && p->inner_restart <= maxr && p->max_outer >= 0, CGM_ERROR_INVALID_VALUE,
+            "bad refinement parameters (inner restart %d outside [1, %d])", p->inner_restart, maxr);
+  if (P) {
+    const int rcp = check_precond(c, *P, 1, 0);
+    if (rcp) return rcp;
+  }
   CGM_HIP(hipSetDevice(c->device));
   memset(res, 0, sizeof(*res));
   const long n = (long)c->N * c->N;
@@ -1276,7 +1767,7 @@ extern "C" int cgm_gmres_refine(cgm_ctx_t c, const cgm_complex64* g32, const cgm
   if (!c->vec_tmp) CGM_HIP(hipMalloc((void**)&c->vec_tmp, sizeof(float2) * n));
   float2* d32 = nullptr;
   CGM_HIP(hipMalloc((void**)&d32, sizeof(float2) * n));
-  int rc = krylov_alloc(c->ws, n, p->inner_restart, c->stream);
+  int rc = krylov_alloc(c->ws, n, P ? 1 : p->inner_restart, c->stream);
   if (rc) { (void)hipFree(d32); return rc; }
   const int blocks = 1024;
   hipLaunchKernelGGL(k_d2f, dim3(blocks), dim3(256), 0, c->stream, chi, c->chi32, n);
@@ -1285,6 +1776,7 @@ extern "C" int cgm_gmres_refine(cgm_ctx_t c, const cgm_complex64* g32, const cgm
   if (!rc) rc = read_state(c);
   const double bn = c->hst->norm_out;
   cgm_gmres_params ip{p->inner_restart, p->inner_max_cycles, p->inner_tol, 0, 1};
+  cgm_fgmres_params fp{p->inner_restart, p->inner_max_cycles, p->inner_tol, 1, 0.0};
   for (int it = 0; !rc && it <= p->max_outer; ++it) {
     rc = matvec64(c, (const double2*)g64, chi, (const double2*)x, (const double2*)b, c->v64a,
                   kModeResidual);
@@ -1298,11 +1790,21 @@ extern "C" int cgm_gmres_refine(cgm_ctx_t c, const cgm_complex64* g32, const cgm
     if (it == p->max_outer) break;
     hipLaunchKernelGGL(k_to_float_scaled, dim3(blocks), dim3(256), 0, c->stream, c->v64a,
                        c->vec_tmp, n, c->st);
-    cgm_gmres_result ir;
-    memset(&ir, 0, sizeof(ir));
-    int irc = gmres_core(c, (const float2*)g32, c->chi32, c->vec_tmp, d32, &ip, &ir, nullptr);
+    int irc, iters;
+    if (P) {
+      cgm_fgmres_result ir;
+      memset(&ir, 0, sizeof(ir));
+      irc = fgmres_core(c, g32, c->chi32, (const cgm_complex64*)c->vec_tmp, (cgm_complex64*)d32, *P, &fp, &ir,
+                        nullptr);
+      iters = ir.iterations;
+    } else {
+      cgm_gmres_result ir;
+      memset(&ir, 0, sizeof(ir));
+      irc = gmres_core(c, (const float2*)g32, c->chi32, c->vec_tmp, d32, &ip, &ir, nullptr);
+      iters = ir.iterations;
+    }
     if (irc && irc != CGM_ERROR_NOT_CONVERGED) { rc = irc; break; }
-    res->inner_iterations += ir.iterations;
+    res->inner_iterations += iters;
     res->outer += 1;
     // the inner solve overwrote norm_out; recompute ||r|| for the scale
     if ((rc = launch_norm64(c->stream, c->v64a, n, c->ws.partn, c->ws.G, c->st))) break;
@@ -1313,6 +1815,22 @@ extern "C" int cgm_gmres_refine(cgm_ctx_t c, const cgm_complex64* g32, const cgm
   (void)hipFree(d32);
   if (rc) return rc;
   return res->converged ? CGM_SUCCESS : CGM_ERROR_NOT_CONVERGED;
+}
+
+extern "C" int cgm_gmres_refine(cgm_ctx_t c, const cgm_complex64* g32, const cgm_complex128* g64,
+                                const double* chi, const cgm_complex128* b, cgm_complex128* x,
+                                const cgm_refine_params* p, cgm_refine_result* res) {
+  return refine_core(c, g32, g64, chi, b, x, nullptr, p, res);
+}
+
+extern "C" int cgm_gmres_refine_pc(cgm_ctx_t c, const cgm_complex64* g32, const cgm_complex128* g64,
+                                   const double* chi, const cgm_complex128* b, cgm_complex128* x,
+                                   const cgm_precond* pc, const cgm_refine_params* p, cgm_refine_result* res) {
+  CGM_CHECK(pc, CGM_ERROR_INVALID_VALUE, "null preconditioner");
+  Precond P;
+  P.net = pc->net;
+  P.tiles = pc->tiles;
+  return refine_core(c, g32, g64, chi, b, x, &P, p, res);
 }
 
 // ---------------------------------------------------------------------------

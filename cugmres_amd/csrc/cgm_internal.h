@@ -84,6 +84,7 @@ struct DevState {
 enum { kTicketDots1 = 0, kTicketDots2 = 1, kTicketNorm = 2, kTicketInit = 3, kTicketMisc = 4 };
 
 constexpr int kMaxRestart = 512;
+constexpr int kFusedMaxRestart = 64;  // longest cycle of the fused lagged-CGS2 kernels
 constexpr int kKB = 1024;  // threads per block of the streaming Krylov kernels
 
 struct KrylovWs {
@@ -156,20 +157,60 @@ int launch_fused(hipStream_t st, int which, const FusedLaunch& L);
 int krylov_alloc(KrylovWs& ws, long n, int m, hipStream_t st, int B = 1);
 void krylov_free(KrylovWs& ws);
 // Batched launches take per-system strides of r (elements) and of the
-// DevState array (bytes); one system by default.
+// DevState array (bytes) and the number of systems of the call (at most
+// ws.B); one system by default.
 int launch_norm_init(hipStream_t st, const KrylovWs& ws, const float2* r, DevState* s, int what,
-                     double tol, long sr = 0, long sst = 0);
+                     double tol, long sr = 0, long sst = 0, int nsys = 1);
 int launch_cgs2_step(hipStream_t st, const KrylovWs& ws, int i, DevState* s, double* hist,
                      int part = -1);
 // x += basis y (basis = ws.V when null; FGMRES passes its Z basis)
 int launch_backsub_update(hipStream_t st, const KrylovWs& ws, DevState* s, float2* x,
-                          const float2* basis = nullptr, long sx = 0, long sst = 0);
+                          const float2* basis = nullptr, long sx = 0, long sst = 0, int nsys = 1);
 int launch_norm64(hipStream_t st, const double2* v, long n, double* partn, int G, DevState* s);
 
+// Long-recurrence flexible GMRES (cgm_longrec.hip): restart up to
+// kMaxRestartLR, the whole basis in HBM, lagged CGS2 with two basis passes
+// per step.
+constexpr int kMaxRestartLR = 4096;
+struct LrWs {
+  int m = 0, B = 0;
+  long n = 0;
+  int nchunks = 0, nuchunks = 0;
+  long sPart = 0, sSmall = 0;
+  float2* Q = nullptr;     // [B][m+1][n]
+  float2* Z = nullptr;     // [B][m][n]
+  float2* U = nullptr;     // [B][2][n]
+  float2* W = nullptr;     // [B][n]
+  double2* part = nullptr;
+  double2 *s = nullptr, *t = nullptr, *h1p = nullptr, *g = nullptr, *y = nullptr, *sn = nullptr;
+  double* cs = nullptr;
+  float2 *a1 = nullptr, *a2 = nullptr;
+  double* scal = nullptr;
+  double2* R = nullptr;    // [B][m][m] column-major
+  double* partn = nullptr;
+};
+int lr_alloc(LrWs& w, long n, int m, int B);
+void lr_free(LrWs& w);
+// Device bytes of a long-recurrence workspace.
+size_t lr_bytes(long n, int m, int B);
+// step j: (j, u_j, its per-system stride, z_j or w output, its stride)
+using LrStepFn = int (*)(void* ctx, int j, const float2* in, long sin, float2* out, long sout);
+struct LrStep {
+  LrStepFn fn;
+  void* ctx;
+  int operator()(int j, const float2* in, long sin, float2* out, long sout) const {
+    return fn(ctx, j, in, sin, out, sout);
+  }
+};
+int lr_cycle(LrWs& w, hipStream_t st, DevState* dst, DevState* hst, long sst, int B, int m, double* hist,
+             long shist, float2* x, long sx, const LrStep& precond, const LrStep& matvec);
+int lr_scale(hipStream_t st, const float2* u, long su, float2* z, long sz, long n, const DevState* dst, long sst,
+             int B);
 }  // namespace cgm
 
 struct cgm_context;
 struct cgm_fno;
+struct cgm_tiles;
 namespace cgm {
 // FNO preconditioner (cgm_fno.hip): z = P(v) for B vectors; with nrm the
 // input is v / *nrm, written back to vw when given (v is left unchanged
@@ -193,6 +234,20 @@ struct FnoIO {
 int fno_forward_io(cgm_fno* f, hipStream_t st, const FnoIO& io, int B, bool train, const int* done);
 int fno_batch(const cgm_fno* f);
 cgm_context* fno_context(const cgm_fno* f);
+
+// Tile preconditioner (cgm_tiles.hip): z = P_tiles(v / nrm) for nsys systems
+// with the strides of io (chi unused: the tiles were factored for their maps);
+// system s is skipped once *(done + s sd bytes).
+int tiles_apply_io(const cgm_tiles* t, hipStream_t st, const FnoIO& io, int nsys, const int* done, long sd);
+cgm_context* tiles_context(const cgm_tiles* t);
+int tiles_maps(const cgm_tiles* t);
+
+// The right preconditioner of the flexible solvers: the tile inverses, the
+// network, or the identity (both null).
+struct Precond {
+  cgm_fno* net = nullptr;
+  cgm_tiles* tiles = nullptr;
+};
 }  // namespace cgm
 
 struct cgm_context {
@@ -208,6 +263,15 @@ struct cgm_context {
   float tuned_us[3] = {0.f, 0.f, 0.f};  // autotuned pass 1/2/3 times
   int path = 0;  // Arnoldi kernels: 0 = fused lagged CGS2 where supported, 1 = two-pass CGS2
   cgm::KrylovWs ws;
+  cgm::LrWs lr;                  // long-recurrence FGMRES workspace (lazy)
+  // FP64 matvecs of the long-recurrence steps (cgm_set_flexible_fp64)
+  const double2* flex_g64 = nullptr;  // caller's FP64 spectrum (not owned)
+  double* chi64 = nullptr;            // contrast maps in double (chi64_len values, lazy)
+  double2* z64 = nullptr;             // z_j, x or b in complex128 (v64_len, lazy)
+  double2* w64 = nullptr;             // w = A z_j or the residual in complex128 (v64_len, lazy)
+  double2* T64B = nullptr;            // FP64 FFT scratch of the systems (T64B_n x N x M, lazy)
+  long chi64_len = 0, v64_len = 0;
+  int T64B_n = 0;
   cgm::DevState* st = nullptr;   // device
   cgm::DevState* hst = nullptr;  // pinned host mirror
   float* chi_tmp = nullptr;      // mask -> contrast conversions (N*N)
@@ -216,6 +280,10 @@ struct cgm_context {
   double2* v64a = nullptr;       // refinement residual (N*N, lazy)
   float* chi32 = nullptr;        // refinement contrast in float (N*N, lazy)
   float2* zb = nullptr;          // FGMRES preconditioned basis (zb_len elements, lazy)
+  float2* pc_v = nullptr;        // two-level preconditioner: normalised inputs (2 pc_len, lazy)
+  float2* pc_r = nullptr;        //   residuals of the tile solves (pc_len)
+  float2* pc_z = nullptr;        //   network corrections (pc_len)
+  long pc_len = 0;
   long zb_len = 0;
   cgm::DevState* stB = nullptr;   // batched solves: one state per system (device)
   cgm::DevState* hstB = nullptr;  // pinned host mirror

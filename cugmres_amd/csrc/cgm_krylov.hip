@@ -398,8 +398,8 @@ void krylov_free(KrylovWs& ws) {
 }
 
 int launch_norm_init(hipStream_t st, const KrylovWs& ws, const float2* r, DevState* s, int what,
-                     double tol, long sr, long sst) {
-  const int B = sst ? ws.B : 1;
+                     double tol, long sr, long sst, int nsys) {
+  const int B = nsys;
   hipLaunchKernelGGL(k_norm_init, dim3(ws.G, B), dim3(kKB), 0, st, r, ws.n, ws.partn, ws.G, s, ws.g,
                      what, tol, sr, ws.sPartn, ws.sSmall, sst);
   CGM_HIP(hipGetLastError());
@@ -446,8 +446,8 @@ int launch_cgs2_step(hipStream_t st, const KrylovWs& ws, int i, DevState* s, dou
 }
 
 int launch_backsub_update(hipStream_t st, const KrylovWs& ws, DevState* s, float2* x, const float2* basis,
-                          long sx, long sst) {
-  const int B = sst ? ws.B : 1;
+                          long sx, long sst, int nsys) {
+  const int B = nsys;
   hipLaunchKernelGGL(k_backsub, dim3(1, B), dim3(kKB), 0, st, ws, s, ws.m, sst);
   hipLaunchKernelGGL(k_xupd, dim3(ws.G, B), dim3(kKB), 0, st, basis ? basis : ws.V, ws.n, x, ws.y, s, ws.G,
                      ws.sBasis, sx, ws.sSmall, sst);

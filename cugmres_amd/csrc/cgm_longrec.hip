@@ -1,0 +1,753 @@
+// Long-recurrence flexible GMRES: restart lengths up to kMaxRestartLR with
+// the whole Krylov basis resident in HBM.
+//
+// On the high-contrast benchmark maps a restart of 30-256 steps discards the
+// directions that took hundreds of steps to find, and restarted GMRES stalls
+// (FGMRES(256) with the tile preconditioner ends at 1e-2 at N = 256); a
+// 288 GB GPU holds thousands of basis vectors per system (2 MiB each at
+// N = 512), so the cycle can be as long as the solve.  At that length the
+// step cost is the basis traffic, and this path is built for it:
+//   - lagged CGS2 (the DCGS2 scheme of cgm_arnoldi.hip, flexible form): one
+//     projection pass and one update pass over the basis per step, so the
+//     basis is read twice per step (the two-pass CGS2 kernels read it four
+//     times once it has more than 32 vectors);
+//   - the projection kernel tiles the basis 16 vectors x 1024 elements per
+//     work-group (u_j and w in registers, re-read once per 16 vectors), the
+//     update kernel streams one element per thread with 16 basis loads in
+//     flight, so both have tens of thousands of loads outstanding at large j;
+//   - partial sums are reduced in a fixed order by a separate kernel (no
+//     tickets), the FP64 Givens update of the completed column runs in one
+//     work-group per system with the rotations staged through LDS.
+// Step j (Q_j = [q_0..q_{j-1}] orthonormal, u_j the once-orthogonalised
+// vector, ||u_j|| known):
+//   z_j = P(u_j / ||u_j||),  w = A z_j                  (preconditioner, matvec)
+//   s = Q_j^H u_j, t = Q_j^H w, alpha = u_j^H u_j, gamma = u_j^H w   (PROJ, RED)
+//   nu = sqrt(alpha - |s|^2), column j-1 of H = [h1p + s; nu], Givens (FIN)
+//   h1p = [t; (gamma - s^H t)/nu], c = h1p[j]/nu
+//   q_j = (u_j - Q_j s)/nu,  u_{j+1} = w - c u_j - Q_j (t - s c)      (UPD)
+// (the flexible form of the lagged scheme: A Z_m = Q_{m+1} H holds for any
+// z_j).  Step m runs PROJ/RED/FIN on u_m only, completing column m-1.
+// Systems of a batch (grid y) are independent; each has its own basis and
+// state and stops at its own tolerance.
+#include <math.h>
+
+#include <vector>
+
+#include "cgm_reduce.h"
+
+namespace cgm {
+
+namespace {
+
+constexpr int kBS = 256;     // threads per streaming work-group
+constexpr int kPE = 4;       // elements per thread in the projection (chunk = 1024)
+constexpr int kJT = 16;      // basis vectors per projection work-group
+constexpr int kNV = 2 * kJT + 2;  // values per projection work-group: s, t, (alpha, gamma)
+constexpr int kUU = 16;      // basis loads in flight per thread in the update
+constexpr int kStage = 1024; // rotations / coefficients staged per LDS round
+
+// Per-system state of the long-recurrence path (device), at a stride of
+// `small` elements per system for the FP64 vectors and `ld*ld` for R.
+struct LrArgs {
+  int j, m, ld, nchunks, ngroups, nuchunks;
+  long n;
+  float2* Q;         // [B][ld+1][n] basis
+  float2* Z;         // [B][ld][n] preconditioned vectors
+  const float2* U;   // u_j (stride sU)
+  float2* Unext;     // u_{j+1}
+  long sU;
+  const float2* W;   // w = A z_j (stride sW)
+  long sW;
+  double2* part;     // [B][ngroups_max][kNV][nchunks]
+  long sPart;
+  double2* s;        // [B][ld+1]
+  double2* t;
+  double2* h1p;
+  double2* g;
+  double2* y;
+  double* cs;
+  double2* sn;
+  float2* a1;        // [B][ld+1] update coefficients (float)
+  float2* a2;
+  double* scal;      // [B][8]: alpha, gamma, 1/nu, c.re, c.im
+  long sSmall;
+  double2* R;        // [B][ld][ld] column-major
+  double* partn;     // [B][nuchunks] norm partials of u_{j+1}
+  long sPartn;
+  DevState* st;
+  long sst;
+  double* hist;      // [B][hstride] (may be null): entry j = estimate after j steps
+  long shist;
+};
+
+__device__ inline bool sys_done(const LrArgs& a, const DevState*& st) {
+  st = boff(a.st, (long)blockIdx.y * a.sst);
+  return st->done != 0;
+}
+
+__device__ inline float2 cj_mul(float2 a, float2 b) {  // conj(a) b
+  return make_float2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+
+// PROJ: partial sums of s_v = q_v^H u, t_v = q_v^H w for the kJT vectors of
+// group g over chunk c, and (group 0) alpha = u^H u, gamma = u^H w.  Stored
+// value-major, part[sys][g][value][chunk], so the reduction reads contiguous
+// runs.  with_w = 0 at the cycle's last step (u_m only).
+__global__ void __launch_bounds__(kBS) k_lr_proj(LrArgs a, int with_w) {
+  const DevState* st;
+  if (sys_done(a, st)) return;
+  const int y = blockIdx.y;
+  const int g = blockIdx.x / a.nchunks, c = blockIdx.x % a.nchunks;
+  const int v0 = g * kJT;
+  const int nv = min(kJT, a.j - v0);
+  const long base = (long)c * (kBS * kPE) + threadIdx.x;
+  const float2* u = a.U + (long)y * a.sU;
+  const float2* w = a.W + (long)y * a.sW;
+  const float2* Q = a.Q + (long)y * (a.ld + 1) * a.n;
+  float2 ue[kPE], we[kPE];
+#pragma unroll
+  for (int k = 0; k < kPE; ++k) {
+    const long e = base + k * kBS;
+    const bool in = e < a.n;
+    ue[k] = in ? u[e] : make_float2(0.f, 0.f);
+    we[k] = (in && with_w) ? w[e] : make_float2(0.f, 0.f);
+  }
+  float2 as[kJT], at[kJT];
+#pragma unroll
+  for (int v = 0; v < kJT; ++v) {
+    as[v] = make_float2(0.f, 0.f);
+    at[v] = make_float2(0.f, 0.f);
+    if (v < nv) {
+      const float2* q = Q + (long)(v0 + v) * a.n;
+      float2 qe[kPE];
+#pragma unroll
+      for (int k = 0; k < kPE; ++k) {
+        const long e = base + k * kBS;
+        qe[k] = e < a.n ? q[e] : make_float2(0.f, 0.f);
+      }
+#pragma unroll
+      for (int k = 0; k < kPE; ++k) {
+        const float2 p = cj_mul(qe[k], ue[k]), r = cj_mul(qe[k], we[k]);
+        as[v].x += p.x;
+        as[v].y += p.y;
+        at[v].x += r.x;
+        at[v].y += r.y;
+      }
+    }
+  }
+  float2 al = make_float2(0.f, 0.f), ga = make_float2(0.f, 0.f);
+  if (g == 0) {
+#pragma unroll
+    for (int k = 0; k < kPE; ++k) {
+      al.x += ue[k].x * ue[k].x + ue[k].y * ue[k].y;
+      const float2 r = cj_mul(ue[k], we[k]);
+      ga.x += r.x;
+      ga.y += r.y;
+    }
+  }
+  // block reduction: wave sums (DPP), then FP64 over the four waves
+  __shared__ double red[kBS / 64][kNV][2];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int nvals = 2 * nv + (g == 0 ? 2 : 0);
+#pragma unroll
+  for (int v = 0; v < kJT; ++v) {
+    if (v < nv) {
+      const float sr = wave_sum_f(as[v].x), si = wave_sum_f(as[v].y);
+      const float tr = wave_sum_f(at[v].x), ti = wave_sum_f(at[v].y);
+      if (lane == 0) {
+        red[wave][v][0] = sr;
+        red[wave][v][1] = si;
+        red[wave][kJT + v][0] = tr;
+        red[wave][kJT + v][1] = ti;
+      }
+    }
+  }
+  if (g == 0) {
+    const float ar = wave_sum_f(al.x), gr = wave_sum_f(ga.x), gi = wave_sum_f(ga.y);
+    if (lane == 0) {
+      red[wave][2 * kJT][0] = ar;
+      red[wave][2 * kJT][1] = 0.0;
+      red[wave][2 * kJT + 1][0] = gr;
+      red[wave][2 * kJT + 1][1] = gi;
+    }
+  }
+  __syncthreads();
+  (void)nvals;
+  const int idx = threadIdx.x;
+  const bool valid = idx < kNV && ((idx < kJT && idx < nv) || (idx >= kJT && idx < 2 * kJT && idx - kJT < nv) ||
+                                   (idx >= 2 * kJT && g == 0));
+  if (valid) {
+    double re = 0.0, im = 0.0;
+#pragma unroll
+    for (int wv = 0; wv < kBS / 64; ++wv) {
+      re += red[wv][idx][0];
+      im += red[wv][idx][1];
+    }
+    a.part[(long)y * a.sPart + ((long)g * kNV + idx) * a.nchunks + c] = make_double2(re, im);
+  }
+}
+
+// RED: sums of the chunk partials in a fixed order; group g writes s, t of
+// its vectors and (g = 0) alpha, gamma.
+__global__ void __launch_bounds__(kBS) k_lr_red(LrArgs a) {
+  const DevState* st;
+  if (sys_done(a, st)) return;
+  const int y = blockIdx.y;
+  const int g = blockIdx.x;
+  const int v0 = g * kJT;
+  const int nv = min(kJT, a.j - v0);
+  __shared__ double2 out[kNV];
+  // values of this group: s (nv), t (nv), alpha/gamma (group 0)
+  const double2* P = a.part + (long)y * a.sPart + (long)g * kNV * a.nchunks;
+  for (int idx = threadIdx.x / 4; idx < kNV; idx += kBS / 4) {
+    const bool valid = (idx < kJT && idx < nv) || (idx >= kJT && idx < 2 * kJT && idx - kJT < nv) ||
+                       (idx >= 2 * kJT && g == 0);
+    double re = 0.0, im = 0.0;
+    if (valid) {
+      for (int c = threadIdx.x % 4; c < a.nchunks; c += 4) {
+        const double2 p = P[(long)idx * a.nchunks + c];
+        re += p.x;
+        im += p.y;
+      }
+    }
+    re += __shfl_xor(re, 1, 64);
+    im += __shfl_xor(im, 1, 64);
+    re += __shfl_xor(re, 2, 64);
+    im += __shfl_xor(im, 2, 64);
+    if (threadIdx.x % 4 == 0) out[idx] = make_double2(re, im);
+  }
+  __syncthreads();
+  double2* s = a.s + (long)y * a.sSmall;
+  double2* t = a.t + (long)y * a.sSmall;
+  if ((int)threadIdx.x < nv) {
+    s[v0 + threadIdx.x] = out[threadIdx.x];
+    t[v0 + threadIdx.x] = out[kJT + threadIdx.x];
+  }
+  if (g == 0 && threadIdx.x == 0) {
+    double* sc = a.scal + (long)y * 8;
+    sc[0] = out[2 * kJT].x;
+    sc[1] = out[2 * kJT + 1].x;
+    sc[2] = out[2 * kJT + 1].y;
+  }
+}
+
+__device__ inline double2 dmul2(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ inline double2 dcj2(double2 a, double2 b) { return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x); }
+
+// FIN: nu, column j-1 of H with its Givens rotations, the estimate, and the
+// coefficients of UPD.  One work-group per system.
+__global__ void __launch_bounds__(kBS) k_lr_fin(LrArgs a) {
+  DevState* st = boff(a.st, (long)blockIdx.y * a.sst);
+  if (st->done) return;
+  const int y = blockIdx.y;
+  const int j = a.j;
+  const double2* s = a.s + (long)y * a.sSmall;
+  const double2* t = a.t + (long)y * a.sSmall;
+  double2* h1p = a.h1p + (long)y * a.sSmall;
+  double2* gv = a.g + (long)y * a.sSmall;
+  double* cs = a.cs + (long)y * a.sSmall;
+  double2* sn = a.sn + (long)y * a.sSmall;
+  float2* a1 = a.a1 + (long)y * a.sSmall;
+  float2* a2 = a.a2 + (long)y * a.sSmall;
+  double* sc = a.scal + (long)y * 8;
+  double2* Rc = a.R + (long)y * a.ld * a.ld + (long)(j > 0 ? j - 1 : 0) * a.ld;  // column j-1
+  // |s|^2 and s^H t
+  double ss = 0.0, str = 0.0, sti = 0.0;
+  for (int i = threadIdx.x; i < j; i += kBS) {
+    const double2 si = s[i], ti = t[i];
+    ss += si.x * si.x + si.y * si.y;
+    const double2 p = dcj2(si, ti);
+    str += p.x;
+    sti += p.y;
+  }
+  __shared__ double red[3][kBS / 64];
+  ss = wave_sum(ss);
+  str = wave_sum(str);
+  sti = wave_sum(sti);
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = ss;
+    red[1][threadIdx.x >> 6] = str;
+    red[2][threadIdx.x >> 6] = sti;
+  }
+  __syncthreads();
+  ss = str = sti = 0.0;
+  for (int w = 0; w < kBS / 64; ++w) {
+    ss += red[0][w];
+    str += red[1][w];
+    sti += red[2][w];
+  }
+  const double alpha = sc[0];
+  const double nu2 = alpha - ss;
+  const double nu = nu2 > 0.0 ? sqrt(nu2) : 0.0;
+  __shared__ int s_done;
+  if (threadIdx.x == 0) s_done = 0;
+  if (j > 0) {
+    // column j-1: rows 0..j-1 = h1p + s, row j = nu, then rotations 0..j-2
+    for (int i = threadIdx.x; i < j; i += kBS) Rc[i] = make_double2(h1p[i].x + s[i].x, h1p[i].y + s[i].y);
+    __syncthreads();
+    __shared__ double2 lcol[kStage + 1];
+    __shared__ double2 lsn[kStage];
+    __shared__ double lcs[kStage];
+    __shared__ double2 carry;
+    if (threadIdx.x == 0) carry = Rc[0];
+    for (int i0 = 0; i0 < j - 1; i0 += kStage) {
+      const int cnt = min(kStage, j - 1 - i0);
+      for (int i = threadIdx.x; i < cnt; i += kBS) {
+        lcol[i + 1] = Rc[i0 + i + 1];
+        lsn[i] = sn[i0 + i];
+        lcs[i] = cs[i0 + i];
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        double2 av = carry;
+        for (int i = 0; i < cnt; ++i) {
+          const double c = lcs[i];
+          const double2 sv = lsn[i], bv = lcol[i + 1];
+          // [a; b] <- [c a + s b; -conj(s) a + c b]
+          const double2 na = make_double2(c * av.x + (sv.x * bv.x - sv.y * bv.y), c * av.y + (sv.x * bv.y + sv.y * bv.x));
+          const double2 nb = make_double2(-(sv.x * av.x + sv.y * av.y) + c * bv.x, -(sv.x * av.y - sv.y * av.x) + c * bv.y);
+          lcol[i] = na;
+          av = nb;
+        }
+        carry = av;
+      }
+      __syncthreads();
+      for (int i = threadIdx.x; i < cnt; i += kBS) Rc[i0 + i] = lcol[i];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      const double2 av = carry;
+      const double aa = hypot(av.x, av.y);
+      const double d = hypot(aa, nu);
+      double c;
+      double2 sv, r;
+      if (aa == 0.0) {
+        c = 0.0;
+        sv = make_double2(1.0, 0.0);
+        r = make_double2(nu, 0.0);
+      } else {
+        c = aa / d;
+        sv = make_double2(av.x / aa * nu / d, av.y / aa * nu / d);
+        r = make_double2(av.x / aa * d, av.y / aa * d);
+      }
+      Rc[j - 1] = r;
+      cs[j - 1] = c;
+      sn[j - 1] = sv;
+      const double2 gi = gv[j - 1];
+      gv[j] = make_double2(-(sv.x * gi.x + sv.y * gi.y), -(sv.x * gi.y - sv.y * gi.x));
+      gv[j - 1] = make_double2(c * gi.x, c * gi.y);
+      const double est = hypot(gv[j].x, gv[j].y);
+      st->res_est = est;
+      st->k = j;
+      if (a.hist) a.hist[(long)y * a.shist + j] = est;
+      const bool bad = !isfinite(est) || !isfinite(nu);
+      if (bad) st->status = CGM_ERROR_NONFINITE;
+      // the cycle stops at a fifth of the tolerance: in complex64 the true
+      // residual of the iterate stays a few 1e-7 above the estimate on the
+      // benchmark physics (1.03-1.08e-6 at an estimate of 1e-6, N = 64), so
+      // stopping at the tolerance itself would end just above it
+      if (bad || est <= 0.2 * st->tol_abs || !(nu > 0.0) || j == a.m) {
+        st->done = 1;
+        s_done = 1;
+      }
+    }
+    __syncthreads();
+  } else if (threadIdx.x == 0 && !(nu > 0.0)) {
+    st->done = 1;  // u_0 = 0: nothing to do
+    s_done = 1;
+  }
+  __syncthreads();
+  if (s_done || j >= a.m) return;
+  // first pass of w against [Q_j, q_j] and the update coefficients
+  const double2 hjj = make_double2((sc[1] - str) / nu, (sc[2] - sti) / nu);
+  const double2 cc = make_double2(hjj.x / nu, hjj.y / nu);
+  for (int i = threadIdx.x; i < j; i += kBS) {
+    const double2 si = s[i], ti = t[i];
+    h1p[i] = ti;
+    a1[i] = make_float2((float)si.x, (float)si.y);
+    const double2 p = dmul2(si, cc);
+    a2[i] = make_float2((float)(ti.x - p.x), (float)(ti.y - p.y));
+  }
+  if (threadIdx.x == 0) {
+    h1p[j] = hjj;
+    sc[3] = 1.0 / nu;
+    sc[4] = cc.x;
+    sc[5] = cc.y;
+  }
+}
+
+// UPD: q_j = (u_j - Q_j a1) / nu and u_{j+1} = w - c u_j - Q_j a2, one
+// element per thread, kUU basis loads in flight; ||u_{j+1}||^2 partials,
+// the last block forms ||u_{j+1}|| (st->unorm).
+__global__ void __launch_bounds__(kBS) k_lr_upd(LrArgs a) {
+  DevState* st = boff(a.st, (long)blockIdx.y * a.sst);
+  if (st->done) return;
+  const int y = blockIdx.y;
+  const int j = a.j;
+  const long e = (long)blockIdx.x * kBS + threadIdx.x;
+  const bool in = e < a.n;
+  const float2* Q = a.Q + (long)y * (a.ld + 1) * a.n;
+  const float2* a1 = a.a1 + (long)y * a.sSmall;
+  const float2* a2 = a.a2 + (long)y * a.sSmall;
+  __shared__ float2 c1[kStage], c2[kStage];
+  float2 acc1 = make_float2(0.f, 0.f), acc2 = make_float2(0.f, 0.f);
+  for (int i0 = 0; i0 < j; i0 += kStage) {
+    const int cnt = min(kStage, j - i0);
+    __syncthreads();
+    for (int i = threadIdx.x; i < cnt; i += kBS) {
+      c1[i] = a1[i0 + i];
+      c2[i] = a2[i0 + i];
+    }
+    __syncthreads();
+    if (in) {
+      int i = 0;
+      for (; i + kUU <= cnt; i += kUU) {
+        float2 q[kUU];
+#pragma unroll
+        for (int u = 0; u < kUU; ++u) q[u] = Q[(long)(i0 + i + u) * a.n + e];
+#pragma unroll
+        for (int u = 0; u < kUU; ++u) {
+          const float2 k1 = c1[i + u], k2 = c2[i + u];
+          acc1.x += k1.x * q[u].x - k1.y * q[u].y;
+          acc1.y += k1.x * q[u].y + k1.y * q[u].x;
+          acc2.x += k2.x * q[u].x - k2.y * q[u].y;
+          acc2.y += k2.x * q[u].y + k2.y * q[u].x;
+        }
+      }
+      for (; i < cnt; ++i) {
+        const float2 qv = Q[(long)(i0 + i) * a.n + e];
+        const float2 k1 = c1[i], k2 = c2[i];
+        acc1.x += k1.x * qv.x - k1.y * qv.y;
+        acc1.y += k1.x * qv.y + k1.y * qv.x;
+        acc2.x += k2.x * qv.x - k2.y * qv.y;
+        acc2.y += k2.x * qv.y + k2.y * qv.x;
+      }
+    }
+  }
+  const double* sc = a.scal + (long)y * 8;
+  double ssq = 0.0;
+  if (in) {
+    const float inu = (float)sc[3];
+    const float2 cc = make_float2((float)sc[4], (float)sc[5]);
+    const float2 uv = a.U[(long)y * a.sU + e];
+    const float2 wv = a.W[(long)y * a.sW + e];
+    float2* qj = a.Q + (long)y * (a.ld + 1) * a.n + (long)j * a.n;
+    qj[e] = make_float2((uv.x - acc1.x) * inu, (uv.y - acc1.y) * inu);
+    const float2 cu = make_float2(cc.x * uv.x - cc.y * uv.y, cc.x * uv.y + cc.y * uv.x);
+    const float2 un = make_float2(wv.x - cu.x - acc2.x, wv.y - cu.y - acc2.y);
+    a.Unext[(long)y * a.sU + e] = un;
+    ssq = (double)un.x * un.x + (double)un.y * un.y;
+  }
+  __shared__ double redn[kBS / 64];
+  ssq = wave_sum(ssq);
+  if ((threadIdx.x & 63) == 0) redn[threadIdx.x >> 6] = ssq;
+  __syncthreads();
+  double* partn = a.partn + (long)y * a.sPartn;
+  if (threadIdx.x == 0) {
+    double sum = 0.0;
+    for (int w = 0; w < kBS / 64; ++w) sum += redn[w];
+    put(&partn[blockIdx.x], sum);
+  }
+  // arrive_last counts against gridDim.x: every block of this system
+  if (!arrive_last(&st->tickets[kTicketMisc])) return;
+  const double tot = sum_partials(partn, a.nuchunks);
+  if (threadIdx.x == 0) {
+    st->unorm = sqrt(tot);
+    reset_ticket(&st->tickets[kTicketMisc]);
+  }
+}
+
+// First vector of a cycle: u_0 = r (already in U), ||u_0|| = beta; g = beta e1.
+__global__ void k_lr_init(LrArgs a) {
+  DevState* st = boff(a.st, (long)blockIdx.y * a.sst);
+  if (threadIdx.x == 0) {
+    st->unorm = st->beta;
+    a.g[(long)blockIdx.y * a.sSmall] = make_double2(st->beta, 0.0);
+  }
+}
+
+// R y = g[0:k] (R upper triangular, column-major ld), FP64, then the state's
+// k is kept in k_last.
+__global__ void __launch_bounds__(1024) k_lr_backsub(LrArgs a) {
+  DevState* st = boff(a.st, (long)blockIdx.y * a.sst);
+  const int y = blockIdx.y;
+  const int k = st->k;
+  double2* gv = a.g + (long)y * a.sSmall;
+  double2* yv = a.y + (long)y * a.sSmall;
+  const double2* R = a.R + (long)y * a.ld * a.ld;
+  if (threadIdx.x == 0) st->k_last = k;
+  __shared__ double2 yi;
+  for (int i = k - 1; i >= 0; --i) {
+    if (threadIdx.x == 0) {
+      const double2 r = R[(long)i * a.ld + i], gi = gv[i];
+      const double den = r.x * r.x + r.y * r.y;
+      yi = make_double2((gi.x * r.x + gi.y * r.y) / den, (gi.y * r.x - gi.x * r.y) / den);
+      yv[i] = yi;
+    }
+    __syncthreads();
+    const double2 yy = yi;
+    for (int r = threadIdx.x; r < i; r += blockDim.x) {
+      const double2 rv = R[(long)i * a.ld + r];
+      double2 gg = gv[r];
+      gg.x -= rv.x * yy.x - rv.y * yy.y;
+      gg.y -= rv.x * yy.y + rv.y * yy.x;
+      gv[r] = gg;
+    }
+    __syncthreads();
+  }
+}
+
+// x += Z y over the k_last columns, one element per thread, kUU loads in
+// flight.  The sum is accumulated in FP64 with y in FP64: with a strong
+// preconditioner the terms y_j z_j can be far larger than their sum, and an
+// FP32 sum would leave a rounding error of that larger size in x.
+__global__ void __launch_bounds__(kBS) k_lr_xupd(LrArgs a, float2* __restrict__ x, long sx) {
+  const DevState* st = boff(a.st, (long)blockIdx.y * a.sst);
+  const int y = blockIdx.y;
+  const int k = st->k_last;
+  const long e = (long)blockIdx.x * kBS + threadIdx.x;
+  const float2* Z = a.Z + (long)y * a.ld * a.n;
+  const double2* yv = a.y + (long)y * a.sSmall;
+  __shared__ double2 cy[kStage / 2];
+  double ax = 0.0, ay = 0.0;
+  for (int i0 = 0; i0 < k; i0 += kStage / 2) {
+    const int cnt = min(kStage / 2, k - i0);
+    __syncthreads();
+    for (int i = threadIdx.x; i < cnt; i += kBS) cy[i] = yv[i0 + i];
+    __syncthreads();
+    if (e < a.n) {
+      int i = 0;
+      for (; i + kUU <= cnt; i += kUU) {
+        float2 z[kUU];
+#pragma unroll
+        for (int u = 0; u < kUU; ++u) z[u] = Z[(long)(i0 + i + u) * a.n + e];
+#pragma unroll
+        for (int u = 0; u < kUU; ++u) {
+          const double2 c = cy[i + u];
+          ax += c.x * z[u].x - c.y * z[u].y;
+          ay += c.x * z[u].y + c.y * z[u].x;
+        }
+      }
+      for (; i < cnt; ++i) {
+        const float2 zv = Z[(long)(i0 + i) * a.n + e];
+        const double2 c = cy[i];
+        ax += c.x * zv.x - c.y * zv.y;
+        ay += c.x * zv.y + c.y * zv.x;
+      }
+    }
+  }
+  if (e < a.n && k > 0) {
+    const float2 xv = x[(long)y * sx + e];
+    x[(long)y * sx + e] = make_float2((float)(xv.x + ax), (float)(xv.y + ay));
+  }
+}
+
+// z = u / ||u|| per system (identity preconditioner).
+__global__ void __launch_bounds__(kBS) k_lr_scale(const float2* __restrict__ u, long su, float2* __restrict__ z,
+                                                  long sz, long n, const DevState* st, long sst) {
+  st = boff(st, (long)blockIdx.y * sst);
+  if (st->done) return;
+  const double d = st->unorm;
+  const float r = d > 0.0 ? (float)(1.0 / d) : 0.f;
+  const long e = (long)blockIdx.x * kBS + threadIdx.x;
+  if (e < n) {
+    const float2 v = u[(long)blockIdx.y * su + e];
+    z[(long)blockIdx.y * sz + e] = make_float2(v.x * r, v.y * r);
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+int lr_alloc(LrWs& w, long n, int m, int B) {
+  if (w.m >= m && w.n == n && w.B >= B) return CGM_SUCCESS;
+  if (w.m > m && w.n == n) m = w.m;
+  if (w.B > B && w.n == n) B = w.B;
+  lr_free(w);
+  w.n = n;
+  w.m = m;
+  w.B = B;
+  w.nchunks = (int)((n + kBS * kPE - 1) / (kBS * kPE));
+  w.nuchunks = (int)((n + kBS - 1) / kBS);
+  const int ngroups = (m + kJT - 1) / kJT;
+  w.sPart = (long)ngroups * kNV * w.nchunks;
+  w.sSmall = m + 1;
+  const size_t BB = (size_t)B;
+  CGM_HIP(hipMalloc((void**)&w.Q, sizeof(float2) * (size_t)(m + 1) * n * BB));
+  CGM_HIP(hipMalloc((void**)&w.Z, sizeof(float2) * (size_t)m * n * BB));
+  CGM_HIP(hipMalloc((void**)&w.U, sizeof(float2) * 2 * (size_t)n * BB));
+  CGM_HIP(hipMalloc((void**)&w.W, sizeof(float2) * (size_t)n * BB));
+  CGM_HIP(hipMalloc((void**)&w.part, sizeof(double2) * (size_t)w.sPart * BB));
+  const size_t sm = (size_t)w.sSmall * BB;
+  CGM_HIP(hipMalloc((void**)&w.s, sizeof(double2) * sm));
+  CGM_HIP(hipMalloc((void**)&w.t, sizeof(double2) * sm));
+  CGM_HIP(hipMalloc((void**)&w.h1p, sizeof(double2) * sm));
+  CGM_HIP(hipMalloc((void**)&w.g, sizeof(double2) * sm));
+  CGM_HIP(hipMalloc((void**)&w.y, sizeof(double2) * sm));
+  CGM_HIP(hipMalloc((void**)&w.cs, sizeof(double) * sm));
+  CGM_HIP(hipMalloc((void**)&w.sn, sizeof(double2) * sm));
+  CGM_HIP(hipMalloc((void**)&w.a1, sizeof(float2) * sm));
+  CGM_HIP(hipMalloc((void**)&w.a2, sizeof(float2) * sm));
+  CGM_HIP(hipMalloc((void**)&w.scal, sizeof(double) * 8 * BB));
+  CGM_HIP(hipMalloc((void**)&w.R, sizeof(double2) * (size_t)m * m * BB));
+  CGM_HIP(hipMalloc((void**)&w.partn, sizeof(double) * (size_t)w.nuchunks * BB));
+  return CGM_SUCCESS;
+}
+
+void lr_free(LrWs& w) {
+  void* ptrs[] = {w.Q, w.Z, w.U, w.W, w.part, w.s, w.t, w.h1p, w.g, w.y, w.cs, w.sn, w.a1, w.a2, w.scal, w.R,
+                  w.partn};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  w = LrWs();
+}
+
+size_t lr_bytes(long n, int m, int B) {
+  const long nchunks = (n + kBS * kPE - 1) / (kBS * kPE);
+  const long ngroups = (m + kJT - 1) / kJT;
+  return (size_t)B * (sizeof(float2) * ((size_t)(2 * m + 4) * n) + sizeof(double2) * (size_t)m * m +
+                      sizeof(double2) * (size_t)ngroups * kNV * nchunks + 80 * (size_t)(m + 1));
+}
+
+static LrArgs lr_args(const LrWs& w, DevState* st, long sst, double* hist, long shist, int m) {
+  LrArgs a{};
+  a.m = m;
+  a.ld = w.m;
+  a.nchunks = w.nchunks;
+  a.nuchunks = w.nuchunks;
+  a.n = w.n;
+  a.Q = w.Q;
+  a.Z = w.Z;
+  a.sU = 2 * w.n;
+  a.sW = w.n;
+  a.W = w.W;
+  a.part = w.part;
+  a.sPart = w.sPart;
+  a.s = w.s;
+  a.t = w.t;
+  a.h1p = w.h1p;
+  a.g = w.g;
+  a.y = w.y;
+  a.cs = w.cs;
+  a.sn = w.sn;
+  a.a1 = w.a1;
+  a.a2 = w.a2;
+  a.scal = w.scal;
+  a.sSmall = w.sSmall;
+  a.R = w.R;
+  a.partn = w.partn;
+  a.sPartn = w.nuchunks;
+  a.st = st;
+  a.sst = sst;
+  a.hist = hist;
+  a.shist = shist;
+  return a;
+}
+
+// One long-recurrence cycle of m steps for B systems, from u_0 = r in U[0]
+// (its norm in st->beta), then x += Z y.  `precond` computes z_j from
+// (u_j, its norm) into Z[j] for all systems; `matvec` computes w = A z_j.
+// Every `poll` steps the host reads the states and stops enqueueing once
+// every system is done.
+int lr_cycle(LrWs& w, hipStream_t st, DevState* dst, DevState* hst, long sst, int B, int m, double* hist,
+             long shist, float2* x, long sx, const LrStep& precond, const LrStep& matvec) {
+  const long n = w.n;
+  LrArgs a = lr_args(w, dst, sst, hist, shist, m);
+  hipLaunchKernelGGL(k_lr_init, dim3(1, B), dim3(64), 0, st, a);
+  CGM_HIP(hipGetLastError());
+  const int poll = 32;
+  for (int j = 0; j <= m; ++j) {
+    a.j = j;
+    a.U = w.U + (long)(j & 1) * n;
+    a.Unext = w.U + (long)((j + 1) & 1) * n;
+    a.ngroups = (j + kJT - 1) / kJT;
+    const int with_w = j < m;
+    if (with_w) {
+      int rc = precond(j, a.U, 2 * n, w.Z + (long)j * n, (long)w.m * n);
+      if (!rc) rc = matvec(j, w.Z + (long)j * n, (long)w.m * n, w.W, n);
+      if (rc) return rc;
+    }
+    const int gx = (a.ngroups > 0 ? a.ngroups : 1) * w.nchunks;
+    hipLaunchKernelGGL(k_lr_proj, dim3(gx, B), dim3(kBS), 0, st, a, with_w);
+    hipLaunchKernelGGL(k_lr_red, dim3(a.ngroups > 0 ? a.ngroups : 1, B), dim3(kBS), 0, st, a);
+    hipLaunchKernelGGL(k_lr_fin, dim3(1, B), dim3(kBS), 0, st, a);
+    if (with_w) hipLaunchKernelGGL(k_lr_upd, dim3(w.nuchunks, B), dim3(kBS), 0, st, a);
+    CGM_HIP(hipGetLastError());
+    if (j >= 2 * poll && j % poll == 0 && j < m) {
+      CGM_HIP(hipMemcpyAsync(hst, dst, sizeof(DevState) * (sst ? B : 1), hipMemcpyDeviceToHost, st));
+      CGM_HIP(hipStreamSynchronize(st));
+      bool all = true;
+      for (int k = 0; k < B && all; ++k) all = boff(hst, (long)k * sst)->done != 0;
+      if (all) break;
+    }
+  }
+  hipLaunchKernelGGL(k_lr_backsub, dim3(1, B), dim3(1024), 0, st, a);
+  hipLaunchKernelGGL(k_lr_xupd, dim3(w.nuchunks, B), dim3(kBS), 0, st, a, x, sx);
+  CGM_HIP(hipGetLastError());
+  return CGM_SUCCESS;
+}
+
+int lr_scale(hipStream_t st, const float2* u, long su, float2* z, long sz, long n, const DevState* dst, long sst,
+             int B) {
+  hipLaunchKernelGGL(k_lr_scale, dim3((n + kBS - 1) / kBS, B), dim3(kBS), 0, st, u, su, z, sz, n, dst, sst);
+  CGM_HIP(hipGetLastError());
+  return CGM_SUCCESS;
+}
+
+}  // namespace cgm
+
+// ---------------------------------------------------------------------------
+// diagnostics: average device time of the four step kernels at basis size j
+// ---------------------------------------------------------------------------
+extern "C" int cgm_profile_longrec(cgm_ctx_t c, int j, int reps, float* ms) {
+  using namespace cgm;
+  CGM_CHECK(c && ms, CGM_ERROR_INVALID_VALUE, "null argument");
+  CGM_CHECK(j >= 1 && j < kMaxRestartLR, CGM_ERROR_INVALID_VALUE, "j=%d outside [1, %d)", j, kMaxRestartLR);
+  CGM_CHECK(reps >= 1, CGM_ERROR_INVALID_VALUE, "reps < 1");
+  CGM_HIP(hipSetDevice(c->device));
+  const long n = (long)c->N * c->N;
+  int rc = lr_alloc(c->lr, n, j + 1, 1);
+  if (rc) return rc;
+  LrWs& w = c->lr;
+  // zero data: the kernels' cost does not depend on the values; the state
+  // keeps done = 0 so every launch does its full work (k_lr_fin, which
+  // would end the cycle on a zero vector, is timed last)
+  CGM_HIP(hipMemsetAsync(w.Q, 0, sizeof(float2) * (size_t)(j + 1) * n, c->stream));
+  CGM_HIP(hipMemsetAsync(w.U, 0, sizeof(float2) * 2 * (size_t)n, c->stream));
+  CGM_HIP(hipMemsetAsync(w.W, 0, sizeof(float2) * (size_t)n, c->stream));
+  CGM_HIP(hipMemsetAsync(w.scal, 0, sizeof(double) * 8, c->stream));
+  CGM_HIP(hipMemsetAsync(w.a1, 0, sizeof(float2) * (size_t)w.sSmall, c->stream));
+  CGM_HIP(hipMemsetAsync(w.a2, 0, sizeof(float2) * (size_t)w.sSmall, c->stream));
+  CGM_HIP(hipMemsetAsync(c->st, 0, sizeof(DevState), c->stream));
+  LrArgs a = lr_args(w, c->st, 0, nullptr, 0, j + 1);
+  a.j = j;
+  a.U = w.U;
+  a.Unext = w.U + n;
+  a.ngroups = (j + kJT - 1) / kJT;
+  hipEvent_t e0, e1;
+  CGM_HIP(hipEventCreate(&e0));
+  CGM_HIP(hipEventCreate(&e1));
+  auto time_it = [&](auto&& launch) -> float {
+    launch();
+    (void)hipEventRecord(e0, c->stream);
+    for (int r = 0; r < reps; ++r) launch();
+    (void)hipEventRecord(e1, c->stream);
+    (void)hipEventSynchronize(e1);
+    float t = 0.f;
+    (void)hipEventElapsedTime(&t, e0, e1);
+    return t / reps;
+  };
+  ms[0] = time_it([&] { hipLaunchKernelGGL(k_lr_proj, dim3(a.ngroups * w.nchunks, 1), dim3(kBS), 0, c->stream, a, 1); });
+  ms[1] = time_it([&] { hipLaunchKernelGGL(k_lr_red, dim3(a.ngroups, 1), dim3(kBS), 0, c->stream, a); });
+  ms[3] = time_it([&] { hipLaunchKernelGGL(k_lr_upd, dim3(w.nuchunks, 1), dim3(kBS), 0, c->stream, a); });
+  ms[2] = time_it([&] {
+    (void)hipMemsetAsync(&c->st->done, 0, sizeof(int), c->stream);
+    hipLaunchKernelGGL(k_lr_fin, dim3(1, 1), dim3(kBS), 0, c->stream, a);
+  });
+  const hipError_t err = hipGetLastError();
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  CGM_HIP(err);
+  CGM_HIP(hipStreamSynchronize(c->stream));
+  return CGM_SUCCESS;
+}

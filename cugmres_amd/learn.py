@@ -86,14 +86,17 @@ class Sampler:
         return out
 
 
-def residual_chain(ctx, gamma, net, chi_d, v, depth):
+def residual_chain(ctx, gamma, net, chi_d, v, depth, tiles=None):
     """depth steps of v <- (v - A P(v)) / ||v - A P(v)|| with the current
-    network: the residuals of preconditioned Richardson iteration, which span
+    preconditioner (the network, or the tile inverses followed by the
+    network): the residuals of preconditioned Richardson iteration, which span
     the Krylov space flexible GMRES builds with P, so training on them aims
     the loss at the directions P still fails on."""
     t = ctx.torch
+    from .tiles import TwoLevel
+    P = TwoLevel(tiles, net) if tiles is not None else None
     for _ in range(depth):
-        z = net.forward(chi_d, v)
+        z = P.apply(ctx, gamma, chi_d, v) if P else net.forward(chi_d, v)
         for b in range(v.shape[0]):
             r = v[b] - ctx.matvec(gamma, chi_d[b], z[b], OP_FORWARD)
             v[b] = r / t.linalg.vector_norm(r)
@@ -135,12 +138,15 @@ class Adam:
 
 
 def train(ctx, samples=20000, batch=8, lr=1e-3, seed=0, params=None, log=None, log_every=100,
-          clip=1.0, identity_weight=0.0, chain=3):
+          clip=1.0, identity_weight=0.0, chain=3, tiles_T=0):
     """Trains a Preconditioner on `samples` seeded (map, vector) pairs at the
     context's N, starting from `params` or from init_params(seed) with the
     identity path weighted by identity_weight.  Step k trains on vectors
-    taken k % (chain + 1) steps down the residual chain.  Returns
-    (Preconditioner, history of (step, loss))."""
+    taken k % (chain + 1) steps down the residual chain.  With tiles_T > 0
+    the network is the second level of tiles.TwoLevel: every step factors the
+    T x T tile inverses of its maps and trains on the normalised residuals
+    r = v - A T v they leave.  Returns (Preconditioner, history of (step,
+    loss))."""
     t = ctx.torch
     gamma = ctx.spectrum_helmholtz(problems.KH)
     if params is None:
@@ -152,6 +158,11 @@ def train(ctx, samples=20000, batch=8, lr=1e-3, seed=0, params=None, log=None, l
     p = net.params.clone()
     opt = Adam(p, lr)
     sampler = Sampler(ctx, gamma)
+    tl = None
+    if tiles_T:
+        from .tiles import Tiles, tile_residual
+        blk = ctx.block_helmholtz(problems.KH)
+        tl = Tiles(ctx, tiles_T, batch)
     steps = max(1, samples // batch)
     hist = []
     t0 = time.time()
@@ -159,8 +170,12 @@ def train(ctx, samples=20000, batch=8, lr=1e-3, seed=0, params=None, log=None, l
         seeds = [seed * 1_000_003 + k * batch + b for b in range(batch)]
         chi_d = t.from_numpy(sample_maps(ctx.N, seeds).reshape(batch, -1)).to(ctx.device)
         v = sampler.batch(chi_d, seeds[0])
+        if tl is not None:
+            tl.factor(blk, chi_d)
         if chain:
-            v = residual_chain(ctx, gamma, net, chi_d, v, k % (chain + 1))
+            v = residual_chain(ctx, gamma, net, chi_d, v, k % (chain + 1), tl)
+        if tl is not None:
+            v = tile_residual(ctx, gamma, tl, chi_d, v)[1].contiguous()
         z = net.forward(chi_d, v, train=True)
         L, dz = residual_loss_grad(ctx, gamma, chi_d, v, z)
         g = net.backward(dz)
@@ -175,6 +190,8 @@ def train(ctx, samples=20000, batch=8, lr=1e-3, seed=0, params=None, log=None, l
         hist.append((k, L))
         if log and (k % log_every == 0 or k == steps - 1):
             log(f"step {k:6d}/{steps} loss {L:.4e} |g| {gn:.3e} lr {cur:.2e} ({time.time() - t0:.1f} s)")
+    if tl is not None:
+        tl.close()
     return net, hist
 
 
@@ -189,12 +206,14 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--identity-weight", type=float, default=0.0)
     ap.add_argument("--chain", type=int, default=3)
+    ap.add_argument("--tiles", type=int, default=0,
+                    help="train the network as the second level after T x T tile inverses (0: alone)")
     ap.add_argument("--out", default="fno_params.npz")
     a = ap.parse_args()
     from .solver import Context
     ctx = Context(a.N)
     net, hist = train(ctx, a.samples, a.batch, a.lr, a.seed, log=lambda s: print(s, flush=True),
-                      identity_weight=a.identity_weight, chain=a.chain)
+                      identity_weight=a.identity_weight, chain=a.chain, tiles_T=a.tiles)
     fno.save(a.out, net.params.cpu().numpy(), N=a.N, samples=a.samples, seed=a.seed,
              loss=np.array([h[1] for h in hist], np.float32))
     print(f"saved {a.out}")

@@ -64,6 +64,13 @@ class Context:
     def autotune(self):
         check(lib().cgm_autotune(self._ptr), "cgm_autotune")
 
+    def set_flexible_fp64(self, gamma64):
+        """Long-recurrence flexible solves form A z_j and their residuals with
+        the FP64 passes and this complex128 spectrum (None: complex64 passes).
+        The tensor is kept alive by the context object."""
+        self._flex_g64 = gamma64
+        check(lib().cgm_set_flexible_fp64(self._ptr, _ptr(gamma64)), "cgm_set_flexible_fp64")
+
     def set_arnoldi_path(self, path):
         """0: fused lagged-CGS2 steps (default), 1: separate two-pass CGS2 kernels."""
         check(lib().cgm_set_arnoldi_path(self._ptr, int(path)), "cgm_set_arnoldi_path")
@@ -110,6 +117,21 @@ class Context:
         check(lib().cgm_spectrum_from_block(self._ptr, _ptr(b), _ptr(g32), _ptr(g64)),
               "cgm_spectrum_from_block")
         return (g32, g64) if fp64 else g32
+
+    def block_helmholtz(self, kh):
+        """The N x N generator block g[|di|, |dj|] of the benchmark operator
+        (complex128, device)."""
+        t = self.torch
+        blk = t.empty(self.N * self.N, dtype=t.complex128, device=self.device)
+        check(lib().cgm_block_helmholtz(self._ptr, float(kh), _ptr(blk)), "cgm_block_helmholtz")
+        return blk
+
+    def block_reference(self, k, row_shift=0.5, col_shift=0.0):
+        t = self.torch
+        blk = t.empty(self.N * self.N, dtype=t.complex128, device=self.device)
+        check(lib().cgm_block_reference(self._ptr, float(k), float(row_shift), float(col_shift), _ptr(blk)),
+              "cgm_block_reference")
+        return blk
 
     # ---- operator ---------------------------------------------------------
     def matvec(self, gamma, chi, x, mode=_lib.OP_FORWARD, out=None):
@@ -194,9 +216,11 @@ class Context:
 
     def fgmres(self, gamma, chi, b, x=None, precond=None, restart=30, tol=1e-6, max_cycles=100,
                stall_ratio=0.5, history=False):
-        """Right-preconditioned flexible GMRES(restart) with the learned
-        preconditioner `precond` (cugmres_amd.fno.Preconditioner; None = the
-        identity).  Plain GMRES takes over when a preconditioned cycle leaves
+        """Right-preconditioned flexible GMRES(restart) with the preconditioner
+        `precond`: the learned network (cugmres_amd.fno.Preconditioner), the
+        tile inverses (cugmres_amd.tiles.Tiles), both (cugmres_amd.tiles.TwoLevel)
+        or None (the identity).  A restart above 64 runs the long-recurrence
+        path (the whole basis in device memory, up to 4096 steps per cycle).  Plain GMRES takes over when a preconditioned cycle leaves
         more than stall_ratio of the residual.  Returns (x, info dict)."""
         t = self.torch
         x0_zero = x is None
@@ -207,10 +231,10 @@ class Context:
             hist = t.zeros(restart * max(max_cycles, 1) + 1, dtype=t.float64, device=self.device)
         p = _lib.FgmresParams(int(restart), int(max_cycles), float(tol), int(x0_zero), float(stall_ratio))
         r = _lib.FgmresResult()
-        net = precond.handle if precond is not None else None
-        st = lib().cgm_fgmres(self._ptr, _ptr(gamma), _ptr(chi), _ptr(b), _ptr(x), net, _c.byref(p),
-                              _c.byref(r), _ptr(hist))
-        check(st, "cgm_fgmres", ok=(_lib.SUCCESS, _lib.ERROR_NOT_CONVERGED))
+        pc = _precond(precond)
+        st = lib().cgm_fgmres_pc(self._ptr, _ptr(gamma), _ptr(chi), _ptr(b), _ptr(x), _c.byref(pc), _c.byref(p),
+                                 _c.byref(r), _ptr(hist))
+        check(st, "cgm_fgmres_pc", ok=(_lib.SUCCESS, _lib.ERROR_NOT_CONVERGED))
         info = dict(iterations=r.iterations, cycles=r.cycles, converged=bool(r.converged),
                     rel_residual=r.rel_residual, beta0=r.beta0, bnorm=r.bnorm,
                     precond_cycles=r.precond_cycles, fell_back=bool(r.fell_back))
@@ -236,10 +260,10 @@ class Context:
         hist = t.zeros(B, hs, dtype=t.float64, device=self.device) if history else None
         p = _lib.FgmresParams(int(restart), int(max_cycles), float(tol), int(x0_zero), 0.0)
         res = (_lib.FgmresResult * B)()
-        net = precond.handle if precond is not None else None
-        st = lib().cgm_fgmres_batched(self._ptr, _ptr(gamma), _ptr(chi), chi_stride, _ptr(b), _ptr(x), net, B,
-                                      _c.byref(p), res, _ptr(hist))
-        check(st, "cgm_fgmres_batched", ok=(_lib.SUCCESS, _lib.ERROR_NOT_CONVERGED))
+        pc = _precond(precond)
+        st = lib().cgm_fgmres_pc_batched(self._ptr, _ptr(gamma), _ptr(chi), chi_stride, _ptr(b), _ptr(x),
+                                         _c.byref(pc), B, _c.byref(p), res, _ptr(hist))
+        check(st, "cgm_fgmres_pc_batched", ok=(_lib.SUCCESS, _lib.ERROR_NOT_CONVERGED))
         infos = []
         h = hist.cpu().numpy() if history else None
         for k in range(B):
@@ -263,8 +287,46 @@ class Context:
         st = lib().cgm_gmres_refine(self._ptr, _ptr(gamma32), _ptr(gamma64), _ptr(chi64), _ptr(b),
                                     _ptr(x), _c.byref(p), _c.byref(r))
         check(st, "cgm_gmres_refine", ok=(_lib.SUCCESS, _lib.ERROR_NOT_CONVERGED))
-        return x, dict(outer=r.outer, inner_iterations=r.inner_iterations,
-                       converged=bool(r.converged), rel_residual=r.rel_residual)
+        return x, _refine_out(r)
+
+    def gmres_refine_pc(self, gamma32, gamma64, chi64, b, precond, x=None, inner_restart=1000, inner_tol=1e-4,
+                        inner_max_cycles=2, outer_tol=1e-12, max_outer=30):
+        """FP64 refinement whose corrections are solved by flexible GMRES with
+        `precond` (tiles.Tiles, tiles.TwoLevel or fno.Preconditioner; the tiles
+        factored for chi64 rounded to float)."""
+        t = self.torch
+        if x is None:
+            x = t.zeros_like(b)
+        p = _lib.RefineParams(int(inner_restart), int(inner_max_cycles), float(inner_tol),
+                              float(outer_tol), int(max_outer))
+        r = _lib.RefineResult()
+        pc = _precond(precond)
+        st = lib().cgm_gmres_refine_pc(self._ptr, _ptr(gamma32), _ptr(gamma64), _ptr(chi64), _ptr(b), _ptr(x),
+                                       _c.byref(pc), _c.byref(p), _c.byref(r))
+        check(st, "cgm_gmres_refine_pc", ok=(_lib.SUCCESS, _lib.ERROR_NOT_CONVERGED))
+        return x, _refine_out(r)
+
+
+def _refine_out(r):
+    return dict(outer=r.outer, inner_iterations=r.inner_iterations, converged=bool(r.converged),
+                rel_residual=r.rel_residual)
+
+
+def _precond(precond):
+    """cgm_precond from a preconditioner object: the tile inverses
+    (tiles.Tiles), the network (fno.Preconditioner) or None (the identity)."""
+    pc = _lib.Precond(None, None)
+    if precond is None:
+        return pc
+    from . import tiles as _tiles
+    if isinstance(precond, _tiles.TwoLevel):
+        pc.tiles = precond.tiles.handle
+        pc.net = precond.net.handle
+    elif isinstance(precond, _tiles.Tiles):
+        pc.tiles = precond.handle
+    else:
+        pc.net = precond.handle
+    return pc
 
 
 def reference_chi(N):

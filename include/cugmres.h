@@ -295,6 +295,79 @@ int cgm_fgmres_batched(cgm_ctx_t ctx, const cgm_complex64* dev_gamma, const floa
                        const cgm_complex64* dev_b, cgm_complex64* dev_x, cgm_fno_t net, int batch,
                        const cgm_fgmres_params* params, cgm_fgmres_result* results, double* dev_hist);
 
+/* ---- tile preconditioner ------------------------------------------------------ */
+/* No counterpart in the reference, which has no preconditioner
+ * (B/CUDA_GMRES/GMRES.cuh:22-446).  The grid is cut into T x T tiles
+ * (T = 8, 16 or 32, N % T == 0) and P applies the exact inverse of the operator
+ * restricted to each tile,
+ *     (P v)_t = (I - G_t diag(chi_t))^{-1} v_t,  G_t[p, q] = g[|p_r-q_r|, |p_c-q_c|],
+ * with g the N x N generator block of the operator.  The inverses are formed
+ * once per map in FP32 complex (batched LU with partial pivoting) and take
+ * T^4 * 8 bytes per tile: (N/T)^2 T^4 * 8 = 512 MiB per map at N = 512, T = 16. */
+typedef struct cgm_tiles* cgm_tiles_t;
+
+/* The generator block g[|di|, |dj|] (N*N complex128, device memory) of the
+ * benchmark operator (as cgm_spectrum_helmholtz) or of the reference's
+ * operator (as cgm_spectrum_reference). */
+int cgm_block_helmholtz(cgm_ctx_t ctx, double kh, cgm_complex128* dev_block);
+int cgm_block_reference(cgm_ctx_t ctx, double wave_number, double row_shift, double col_shift,
+                        cgm_complex128* dev_block);
+/* Workspace for up to `maps` contrast maps at the context's N. */
+int cgm_tiles_create(cgm_ctx_t ctx, int T, int maps, cgm_tiles_t* tiles);
+int cgm_tiles_destroy(cgm_tiles_t tiles);
+/* Device bytes the workspace holds. */
+int cgm_tiles_memory(cgm_tiles_t tiles, size_t* bytes);
+/* Forms and inverts every tile's operator for `maps` maps (chi is
+ * [maps][N*N] with chi_stride >= N*N, or one map with maps = 1).  One map
+ * serves any number of right-hand sides.  Synchronous.  Returns
+ * CGM_ERROR_NONFINITE if a tile's operator is singular. */
+int cgm_tiles_factor(cgm_tiles_t tiles, const cgm_complex128* dev_block, const float* dev_chi, long chi_stride,
+                     int maps);
+/* y[b] = P x[b] for b < batch ([batch][N*N]; map b, or the one map).
+ * Asynchronous on the context stream. */
+int cgm_tiles_apply(cgm_tiles_t tiles, const cgm_complex64* dev_x, cgm_complex64* dev_y, int batch);
+
+/* The right preconditioner of cgm_fgmres_pc / cgm_fgmres_pc_batched: the tile
+ * inverses (tiles), the network (net), both, or the identity (both NULL).
+ * Both set is the two-level preconditioner: z1 = T v, r = v - A z1,
+ * P(v) = z1 + ||r|| F(r / ||r||), the network F a global correction on what
+ * the tile inverses leave (trained on those residuals). */
+typedef struct {
+  cgm_tiles_t tiles;
+  cgm_fno_t net;
+} cgm_precond;
+
+/* Precision of the long-recurrence flexible steps (restart > 64): with an FP64
+ * spectrum set here (from cgm_spectrum_*(..., dev_gamma64)), w = A z_j and
+ * each cycle's residual are formed by the complex128 FFT passes from the
+ * complex64 vectors and rounded once.  A strong preconditioner returns z_j
+ * of large norm, and the complex64 product's error, about 1e-7 ||A|| ||z_j||,
+ * then stalls the true residual near 1e-5 while the estimates keep falling;
+ * the FP64 product removes that floor.  NULL restores the complex64 passes.
+ * The array must outlive the solves that use it.  No counterpart in the
+ * reference. */
+int cgm_set_flexible_fp64(cgm_ctx_t ctx, const cgm_complex128* dev_gamma64);
+
+/* cgm_fgmres / cgm_fgmres_batched with the preconditioner given as a
+ * cgm_precond.  cgm_fgmres(..., net, ...) is cgm_fgmres_pc with {NULL, net}.
+ * The batched form needs the factored maps to match chi: one map (chi_stride
+ * = 0) or one per system. */
+int cgm_fgmres_pc(cgm_ctx_t ctx, const cgm_complex64* dev_gamma, const float* dev_chi,
+                  const cgm_complex64* dev_b, cgm_complex64* dev_x, const cgm_precond* pc,
+                  const cgm_fgmres_params* params, cgm_fgmres_result* result, double* dev_hist);
+int cgm_fgmres_pc_batched(cgm_ctx_t ctx, const cgm_complex64* dev_gamma, const float* dev_chi, long chi_stride,
+                          const cgm_complex64* dev_b, cgm_complex64* dev_x, const cgm_precond* pc, int batch,
+                          const cgm_fgmres_params* params, cgm_fgmres_result* results, double* dev_hist);
+
+/* cgm_gmres_refine with the correction solved by flexible GMRES with the
+ * preconditioner pc (inner_restart up to 4096; above 64 the whole basis stays
+ * in device memory).  The tiles of pc must have been factored for the map
+ * chi rounded to float.  No counterpart in the reference. */
+int cgm_gmres_refine_pc(cgm_ctx_t ctx, const cgm_complex64* dev_gamma,
+                        const cgm_complex128* dev_gamma64, const double* dev_chi,
+                        const cgm_complex128* dev_b, cgm_complex128* dev_x, const cgm_precond* pc,
+                        const cgm_refine_params* params, cgm_refine_result* result);
+
 /* ---- reference-compatible entry points ------------------------------------ */
 /* The reference's plane wave exp(-i k i) with k = 2*3.14f/(N/6.f)
  * (B/CUDA_GMRES/GMRES_kernels.cuh:12,557-572), on the device. */
@@ -376,6 +449,14 @@ int cgm_profile_step(cgm_ctx_t ctx, const cgm_complex64* dev_gamma, const float*
  * entries).  Overwrites the network's activations. */
 int cgm_fno_profile(cgm_fno_t net, const float* dev_chi, const cgm_complex64* dev_v,
                     cgm_complex64* dev_z, int batch, int reps, float* ms);
+
+/* Average device time (ms, HIP events on the context stream, reps launches
+ * each) of the long-recurrence step kernels (restart > 64) at basis size j:
+ * ms[0] projection of u and w on the j basis vectors, [1] reduction of its
+ * partial sums, [2] Givens update of the completed column (includes a 4-byte
+ * memset per launch), [3] basis update forming q_j and u_{j+1} (ms holds 4
+ * entries).  Overwrites the context's long-recurrence workspace. */
+int cgm_profile_longrec(cgm_ctx_t ctx, int j, int reps, float* ms);
 
 /* ---- memory and host helpers ---------------------------------------------- */
 int cgm_malloc(void** ptr, size_t bytes);

@@ -185,3 +185,26 @@ def test_fgmres_batched_equals_single_solves(sol, N, B, shared, with_net, path):
         assert np.max(np.abs(h1 - h2)) <= 1e-5 * h2[0]
         assert ib[k]["rel_residual"] == pytest.approx(isg["rel_residual"], rel=1e-4)
     torch.cuda.synchronize()
+
+
+def test_smaller_batch_after_larger_leaves_memory_past_x_alone(sol):
+    """A batched call after a larger one on the same context touches only its
+    own systems: rows of the buffer past the caller's x keep their sentinel
+    values (the launches are sized by the call's batch, not the workspace's)."""
+    import torch
+    N = 64
+    ctx = sol.Context(N, 0)
+    g = ctx.spectrum_helmholtz(__import__("cugmres_amd.problems", fromlist=["KH"]).KH)
+    chis, bs = _systems(N, 6, 2.25, 300)
+    dchi = torch.as_tensor(chis.reshape(6, -1).astype(np.float32)).to(ctx.device)
+    db = torch.as_tensor(bs.reshape(6, -1).astype(np.complex64)).to(ctx.device)
+    for fn in ("gmres_batched", "fgmres_batched"):
+        getattr(ctx, fn)(g, dchi, db, restart=20, tol=1e-6, max_cycles=3)   # workspace for 6 systems
+        buf = torch.full((6, N * N), 7.0 + 3.0j, dtype=torch.complex64, device=ctx.device)
+        x = buf[:2]
+        getattr(ctx, fn)(g, dchi[:2], db[:2].contiguous(), x.zero_(), restart=20, tol=1e-6, max_cycles=3)
+        torch.cuda.synchronize()
+        tail = buf[2:].cpu().numpy()
+        assert np.all(tail == np.complex64(7.0 + 3.0j)), fn
+        assert np.all(np.isfinite(x.cpu().numpy()))
+    ctx.close()

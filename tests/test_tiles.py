@@ -1,0 +1,228 @@
+"""Tile (block-Jacobi) preconditioner: the oracle restatement pinned on the
+CPU, the HIP factorisation/application against it, and flexible GMRES with
+it against the FP64 oracle (no reference counterpart: the reference's GMRES,
+B/CUDA_GMRES/GMRES.cuh:22-446, has no preconditioner)."""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import rel
+
+
+def _problem(N, seed, eps_hi=12.1):
+    from cugmres_amd import problems
+    rng = problems.SplitMix64(seed)
+    theta = 2 * math.pi * rng.uniform()
+    eps = problems.block_eps_map(N, 8 if N >= 64 else 4, 0.5, 0.75, rng=rng, eps_hi=eps_hi)
+    return (eps - 1.0).reshape(-1), problems.plane_wave(N, problems.KH, theta).reshape(-1)
+
+
+# ---- oracle pins (CPU) -------------------------------------------------------
+def test_oracle_single_tile_is_the_exact_inverse(orc):
+    """With one tile covering the grid, P = A^{-1}: A P v = v with A the
+    oracle's FFT matvec (pinned by the reference's MKL dumps and dense brute
+    force)."""
+    from cugmres_amd import problems
+    from oracle import tiles
+    N = 16
+    chi, b = _problem(N, 3)
+    blk = orc.baseline_green_block(N, problems.KH)
+    K = tiles.tile_inverses(blk, chi, N, N)
+    z = tiles.tile_apply(K, b, N, N)
+    op = orc.Op(N, blk)
+    assert rel(op.matvec(chi, z), b) < 1e-11
+
+
+def test_oracle_zero_contrast_is_identity_and_tiles_are_local(orc):
+    from cugmres_amd import problems
+    from oracle import tiles
+    N, T = 32, 8
+    blk = orc.baseline_green_block(N, problems.KH)
+    rng = np.random.default_rng(0)
+    v = rng.standard_normal(N * N) + 1j * rng.standard_normal(N * N)
+    K0 = tiles.tile_inverses(blk, np.zeros(N * N), N, T)
+    assert np.allclose(tiles.tile_apply(K0, v, N, T), v, rtol=0, atol=1e-13)
+    chi, _ = _problem(N, 5)
+    K = tiles.tile_inverses(blk, chi, N, T)
+    # a vector on tile (1, 2) only stays on tile (1, 2), and there it equals
+    # the dense inverse of that tile's operator
+    mask = np.zeros((N, N), bool)
+    mask[T:2 * T, 2 * T:3 * T] = True
+    w = np.where(mask.reshape(-1), v, 0)
+    y = tiles.tile_apply(K, w, N, T)
+    assert np.abs(y[~mask.reshape(-1)]).max() == 0.0
+    i, j = np.divmod(np.arange(T * T), T)
+    G = blk[np.abs(i[:, None] - i[None, :]), np.abs(j[:, None] - j[None, :])]
+    ct = chi.reshape(N, N)[T:2 * T, 2 * T:3 * T].reshape(-1)
+    At = np.eye(T * T) - G * ct[None, :]
+    yt = np.linalg.solve(At, v.reshape(N, N)[T:2 * T, 2 * T:3 * T].reshape(-1))
+    assert rel(y.reshape(N, N)[T:2 * T, 2 * T:3 * T].reshape(-1), yt) < 1e-12
+
+
+def test_oracle_tile_layout_round_trip():
+    from oracle import tiles
+    N, T = 24, 8
+    v = np.arange(N * N) + 0j
+    t = tiles.to_tiles(v, N, T)
+    assert t[1, 2, 3 * T + 4] == v[(1 * T + 3) * N + 2 * T + 4]
+    assert np.array_equal(tiles.from_tiles(t, N, T), v)
+
+
+# ---- HIP path against the oracle ------------------------------------------------
+@pytest.mark.gpu
+@pytest.mark.parametrize("N,T", [(64, 8), (64, 16), (96, 16), (128, 16)])
+def test_tiles_apply_matches_oracle(orc, N, T):
+    import torch
+    from cugmres_amd import problems, solver, tiles as gt
+    from oracle import tiles
+    print(f"tiles apply N={N} T={T}", flush=True)
+    ctx = solver.Context(N, 0)
+    chi, b = _problem(N, N + T)
+    blk = ctx.block_helmholtz(problems.KH)
+    ref_blk = orc.baseline_green_block(N, problems.KH)
+    assert rel(blk.cpu().numpy(), ref_blk.reshape(-1)) < 1e-13
+    tl = gt.Tiles(ctx, T, 1).factor(blk, ctx.to_device(chi, "f32"))
+    y = tl.apply(ctx.to_device(b)).cpu().numpy()
+    yr = tiles.tile_apply(tiles.tile_inverses(ref_blk, chi, N, T), b, N, T)
+    assert rel(y, yr) < 1e-4   # FP32 LU of the tile operators
+    tl.close()
+    ctx.close()
+
+
+@pytest.mark.gpu
+def test_tiles_several_maps_and_shared_map(orc):
+    """Own maps (one inverse set per system) and one map with five right-hand
+    sides (the shared-inverse kernel, four vectors per work-group, a ragged
+    last group)."""
+    from cugmres_amd import problems, solver, tiles as gt
+    from oracle import tiles
+    import torch
+    N, T = 64, 16
+    ctx = solver.Context(N, 0)
+    blk = ctx.block_helmholtz(problems.KH)
+    ref_blk = orc.baseline_green_block(N, problems.KH)
+    probs = [_problem(N, 40 + k) for k in range(3)]
+    chis = torch.stack([ctx.to_device(c, "f32") for c, _ in probs])
+    bs = torch.stack([ctx.to_device(b) for _, b in probs])
+    tl = gt.Tiles(ctx, T, 3).factor(blk, chis)
+    y = tl.apply(bs).cpu().numpy()
+    for k, (c, b) in enumerate(probs):
+        assert rel(y[k], tiles.tile_apply(tiles.tile_inverses(ref_blk, c, N, T), b, N, T)) < 1e-4
+    tl1 = gt.Tiles(ctx, T, 1).factor(blk, chis[0])
+    rng = np.random.default_rng(2)
+    V = rng.standard_normal((5, N * N)) + 1j * rng.standard_normal((5, N * N))
+    y1 = tl1.apply(torch.as_tensor(V.astype(np.complex64)).to(ctx.device)).cpu().numpy()
+    K0 = tiles.tile_inverses(ref_blk, probs[0][0], N, T)
+    for k in range(5):
+        assert rel(y1[k], tiles.tile_apply(K0, V[k], N, T)) < 1e-4
+    tl.close()
+    tl1.close()
+    ctx.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("restart", [200, 400])
+def test_fgmres_with_tiles_converges_to_the_oracle_solution(orc, restart):
+    """On the benchmark physics (eps in {1, 12.1}), where unpreconditioned
+    GMRES(30) stalls near 1e-2, FGMRES with the tile inverses and cycles long
+    enough to keep the Krylov space (restarts of 30 or 64 still stall) reaches
+    the true relative residual 1e-6 (checked with the oracle's FP64 matvec),
+    and its solution matches the oracle's converged FP64 solve to 1e-5."""
+    from cugmres_amd import problems, solver, tiles as gt
+    from oracle import tiles
+    N, T = 64, 16
+    print(f"fgmres tiles N={N} restart={restart}", flush=True)
+    ctx = solver.Context(N, 0)
+    g, g64 = ctx.spectrum_helmholtz(problems.KH, fp64=True)
+    ctx.set_flexible_fp64(g64)
+    chi, b = _problem(N, 11)
+    dchi, db = ctx.to_device(chi, "f32"), ctx.to_device(b)
+    tl = gt.Tiles(ctx, T, 1).factor(ctx.block_helmholtz(problems.KH), dchi)
+    x, info = ctx.fgmres(g, dchi, db, precond=tl, restart=restart, tol=1e-6, max_cycles=4, stall_ratio=0.0)
+    assert info["converged"], info
+    op = orc.Op(N, orc.baseline_green_block(N, problems.KH), ctx.M)
+    xs = x.cpu().numpy().astype(np.complex128)
+    r = b - op.matvec(chi, xs)
+    assert np.linalg.norm(r) / np.linalg.norm(b) < 1.5e-6
+    # the oracle's own converged solve (FP64 GMRES preconditioned by the
+    # oracle's tile inverses, so that it finishes in seconds)
+    K = tiles.tile_inverses(orc.baseline_green_block(N, problems.KH), chi, N, T)
+    y, _ = _oracle_right_pc_gmres(lambda v: op.matvec(chi, v), lambda v: tiles.tile_apply(K, v, N, T), b, 1e-12)
+    assert rel(xs, y) < 1e-5
+    tl.close()
+    ctx.close()
+
+
+@pytest.mark.gpu
+def test_fused_and_two_pass_fgmres_with_tiles_agree():
+    """The tile inverses inside the fused flexible steps (restart <= 64) and
+    inside the two-pass CGS2 steps build the same Krylov space: the estimate
+    histories of two GMRES(30) cycles agree to 1e-4 of ||r0||."""
+    from cugmres_amd import problems, solver, tiles as gt
+    N, T = 64, 16
+    ctx = solver.Context(N, 0)
+    g = ctx.spectrum_helmholtz(problems.KH)
+    chi, b = _problem(N, 12)
+    dchi, db = ctx.to_device(chi, "f32"), ctx.to_device(b)
+    tl = gt.Tiles(ctx, T, 1).factor(ctx.block_helmholtz(problems.KH), dchi)
+    hs = []
+    for path in (0, 1):
+        ctx.set_arnoldi_path(path)
+        _, info = ctx.fgmres(g, dchi, db, precond=tl, restart=30, tol=1e-9, max_cycles=2, stall_ratio=0.0,
+                             history=True)
+        hs.append(info["history"])
+    assert len(hs[0]) == len(hs[1]) == 61
+    assert np.max(np.abs(hs[0] - hs[1])) < 1e-4 * hs[1][0]
+    assert hs[0][60] < 0.5 * hs[0][0]
+    tl.close()
+    ctx.close()
+
+
+@pytest.mark.gpu
+def test_fgmres_batched_with_tiles_equals_single():
+    import torch
+    from cugmres_amd import problems, solver, tiles as gt
+    N, T = 64, 16
+    ctx = solver.Context(N, 0)
+    ctx.set_arnoldi_path(2)   # one work-group shape for batch and single (bitwise)
+    g = ctx.spectrum_helmholtz(problems.KH)
+    probs = [_problem(N, 60 + k) for k in range(3)]
+    chis = torch.stack([ctx.to_device(c, "f32") for c, _ in probs])
+    bs = torch.stack([ctx.to_device(b) for _, b in probs])
+    tl = gt.Tiles(ctx, T, 3).factor(ctx.block_helmholtz(problems.KH), chis)
+    xb, infos = ctx.fgmres_batched(g, chis, bs, precond=tl, restart=30, tol=1e-6, max_cycles=3)
+    for k in range(3):
+        tk = gt.Tiles(ctx, T, 1).factor(ctx.block_helmholtz(problems.KH), chis[k])
+        x1, i1 = ctx.fgmres(g, chis[k], bs[k], precond=tk, restart=30, tol=1e-6, max_cycles=3, stall_ratio=0.0)
+        assert infos[k]["iterations"] == i1["iterations"]
+        assert rel(xb[k].cpu().numpy(), x1.cpu().numpy()) < 1e-6
+        tk.close()
+    tl.close()
+    ctx.close()
+
+
+def _oracle_right_pc_gmres(A, P, b, tol, m=400):
+    """Unrestarted right-preconditioned GMRES in FP64 numpy (CGS2), for the
+    reference solutions of the tests above: x = P y, min ||b - A P y||."""
+    n = b.size
+    bn = np.linalg.norm(b)
+    V = np.zeros((m + 1, n), complex)
+    Z = np.zeros((m, n), complex)
+    H = np.zeros((m + 1, m), complex)
+    V[0] = b / bn
+    for j in range(m):
+        Z[j] = P(V[j])
+        w = A(Z[j])
+        for _ in range(2):
+            h = V[:j + 1].conj() @ w
+            w = w - h @ V[:j + 1]
+            H[:j + 1, j] += h
+        H[j + 1, j] = np.linalg.norm(w)
+        V[j + 1] = w / H[j + 1, j]
+        e1 = np.zeros(j + 2, complex)
+        e1[0] = bn
+        y = np.linalg.lstsq(H[:j + 2, :j + 1], e1, rcond=None)[0]
+        if np.linalg.norm(e1 - H[:j + 2, :j + 1] @ y) <= tol * bn:
+            break
+    return y @ Z[:j + 1], j + 1
