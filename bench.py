@@ -72,7 +72,7 @@ def parse(argv=None):
     p.add_argument("--instances", type=int, default=7,
                    help="c3 mode: instances (7 = one per grid size; 256 = the full batched workload)")
     p.add_argument("--mode", choices=["bench", "step", "copy", "batched", "fno-step", "copybw", "sweep", "cycles",
-                                      "lr-step", "tiles-step", "c3", "c2"],
+                                      "lr-step", "tiles-step", "c3", "c2", "fftref"],
                    default="bench",
                    help="bench: the benchmark; step: the per-kernel profile of one Arnoldi step of the fused "
                         "GMRES(--restart) kernels (for rocprofv3 --pmc); lr-step: the long-recurrence step kernels "
@@ -117,6 +117,9 @@ def main():
         from cugmres_amd import sweep
         r = sweep.run(reps=a.sweep_reps, log=lambda s: print(s, flush=True))
         print(json.dumps({"sweep": {f"{N}x{m}": v for (N, m), v in r.items()}}), flush=True)
+        return
+    if a.mode == "fftref":
+        print(json.dumps(fft_reference(a), separators=(",", ":")), flush=True)
         return
     if a.mode == "c3":
         print(json.dumps(c3_run(a), separators=(",", ":")), flush=True)
@@ -357,6 +360,50 @@ def c3_instances(count, seed=2024):
         N = C3_SIZES[i % 7] if count <= 7 else C3_SIZES[min(6, int(rng.uniform() * 7))]
         blk = 4 + min(4, int(rng.uniform() * 5))
         out.append((N, blk, seed * 1000 + i))
+    return out
+
+
+def fft_reference(a, B=16, reps=20):
+    """The library's batched matvec (three fused FFT passes, B vectors of one
+    map at --N) beside rocFFT (torch.fft, hipFFT) doing the transforms alone:
+    a forward and an inverse 2-D FFT of B (2N x 2N) complex64 arrays, and the
+    same on the N x 2N row-transformed half.  Bytes are the minimum each
+    moves: the matvec 88 N^2 per vector (x in twice, the N x 2N intermediate
+    written and read twice, y out) plus the map and the quarter spectrum once;
+    a 2-D FFT of an M x M array reads and writes it once per pass (rocFFT's
+    two passes: 32 M^2 B)."""
+    import torch
+    from cugmres_amd import problems, solver
+    N = a.N
+    ctx = solver.Context(N, 0)
+    M = ctx.M
+    g = ctx.spectrum_helmholtz(problems.KH)
+    chi = ctx.to_device(problems.config_plain(N, seed=0)[0].reshape(-1), "f32")
+    x = torch.randn(B, N * N, dtype=torch.complex64, device=ctx.device)
+    y = torch.empty_like(x)
+
+    def timed(fn):
+        for _ in range(3):
+            fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps * 1e3
+
+    out = {"N": N, "M": M, "B": B}
+    us = timed(lambda: ctx.matvec_batched(g, chi, x, 0, y))
+    mv_bytes = B * 88 * N * N + 4 * N * N + 8 * (M // 2 + 1) ** 2
+    out["matvec_batched"] = {"us": us, "bytes": mv_bytes, "GBps": mv_bytes / (us * 1e-6) / 1e9}
+    big = torch.randn(B, M, M, dtype=torch.complex64, device=ctx.device)
+    us = timed(lambda: torch.fft.ifft2(torch.fft.fft2(big)))
+    b2 = 2 * 32 * M * M * B
+    out["rocfft_fft2_ifft2_MxM"] = {"us": us, "bytes": b2, "GBps": b2 / (us * 1e-6) / 1e9}
+    us = timed(lambda: torch.fft.fft2(big))
+    out["rocfft_fft2_MxM"] = {"us": us, "bytes": b2 // 2, "GBps": b2 / 2 / (us * 1e-6) / 1e9}
+    ctx.close()
     return out
 
 

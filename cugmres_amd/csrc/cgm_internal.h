@@ -184,7 +184,7 @@ struct LrWs {
   double2* part = nullptr;
   double2 *s = nullptr, *t = nullptr, *h1p = nullptr, *g = nullptr, *y = nullptr, *sn = nullptr;
   double* cs = nullptr;
-  float2 *a1 = nullptr, *a2 = nullptr;
+  double2 *a1 = nullptr, *a2 = nullptr;
   double* scal = nullptr;
   double2* R = nullptr;    // [B][m][m] column-major
   double* partn = nullptr;

@@ -67,8 +67,8 @@ struct LrArgs {
   double2* y;
   double* cs;
   double2* sn;
-  float2* a1;        // [B][ld+1] update coefficients (float)
-  float2* a2;
+  double2* a1;       // [B][ld+1] update coefficients
+  double2* a2;
   double* scal;      // [B][8]: alpha, gamma, 1/nu, c.re, c.im
   long sSmall;
   double2* R;        // [B][ld][ld] column-major
@@ -145,15 +145,18 @@ __global__ void __launch_bounds__(kBS) k_lr_proj(LrArgs a, int with_w) {
       ga.y += r.y;
     }
   }
-  // block reduction: wave sums (DPP), then FP64 over the four waves
+  // block reduction in FP64 (each thread's four-term FP32 sums widened, then
+  // wave sums of doubles and the four waves): the long recurrences need
+  // inner products accurate to FP32 rounding of each term, not of 256-term
+  // FP32 partial sums
   __shared__ double red[kBS / 64][kNV][2];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int nvals = 2 * nv + (g == 0 ? 2 : 0);
 #pragma unroll
   for (int v = 0; v < kJT; ++v) {
     if (v < nv) {
-      const float sr = wave_sum_f(as[v].x), si = wave_sum_f(as[v].y);
-      const float tr = wave_sum_f(at[v].x), ti = wave_sum_f(at[v].y);
+      const double sr = wave_sum((double)as[v].x), si = wave_sum((double)as[v].y);
+      const double tr = wave_sum((double)at[v].x), ti = wave_sum((double)at[v].y);
       if (lane == 0) {
         red[wave][v][0] = sr;
         red[wave][v][1] = si;
@@ -163,7 +166,7 @@ __global__ void __launch_bounds__(kBS) k_lr_proj(LrArgs a, int with_w) {
     }
   }
   if (g == 0) {
-    const float ar = wave_sum_f(al.x), gr = wave_sum_f(ga.x), gi = wave_sum_f(ga.y);
+    const double ar = wave_sum((double)al.x), gr = wave_sum((double)ga.x), gi = wave_sum((double)ga.y);
     if (lane == 0) {
       red[wave][2 * kJT][0] = ar;
       red[wave][2 * kJT][1] = 0.0;
@@ -247,8 +250,8 @@ __global__ void __launch_bounds__(kBS) k_lr_fin(LrArgs a) {
   double2* gv = a.g + (long)y * a.sSmall;
   double* cs = a.cs + (long)y * a.sSmall;
   double2* sn = a.sn + (long)y * a.sSmall;
-  float2* a1 = a.a1 + (long)y * a.sSmall;
-  float2* a2 = a.a2 + (long)y * a.sSmall;
+  double2* a1 = a.a1 + (long)y * a.sSmall;
+  double2* a2 = a.a2 + (long)y * a.sSmall;
   double* sc = a.scal + (long)y * 8;
   double2* Rc = a.R + (long)y * a.ld * a.ld + (long)(j > 0 ? j - 1 : 0) * a.ld;  // column j-1
   // |s|^2 and s^H t
@@ -364,9 +367,9 @@ __global__ void __launch_bounds__(kBS) k_lr_fin(LrArgs a) {
   for (int i = threadIdx.x; i < j; i += kBS) {
     const double2 si = s[i], ti = t[i];
     h1p[i] = ti;
-    a1[i] = make_float2((float)si.x, (float)si.y);
+    a1[i] = si;
     const double2 p = dmul2(si, cc);
-    a2[i] = make_float2((float)(ti.x - p.x), (float)(ti.y - p.y));
+    a2[i] = make_double2(ti.x - p.x, ti.y - p.y);
   }
   if (threadIdx.x == 0) {
     h1p[j] = hjj;
@@ -387,12 +390,15 @@ __global__ void __launch_bounds__(kBS) k_lr_upd(LrArgs a) {
   const long e = (long)blockIdx.x * kBS + threadIdx.x;
   const bool in = e < a.n;
   const float2* Q = a.Q + (long)y * (a.ld + 1) * a.n;
-  const float2* a1 = a.a1 + (long)y * a.sSmall;
-  const float2* a2 = a.a2 + (long)y * a.sSmall;
-  __shared__ float2 c1[kStage], c2[kStage];
-  float2 acc1 = make_float2(0.f, 0.f), acc2 = make_float2(0.f, 0.f);
-  for (int i0 = 0; i0 < j; i0 += kStage) {
-    const int cnt = min(kStage, j - i0);
+  const double2* a1 = a.a1 + (long)y * a.sSmall;
+  const double2* a2 = a.a2 + (long)y * a.sSmall;
+  // coefficients and sums in FP64: q_j and u_{j+1} are differences of
+  // nearly equal vectors, and a sum of thousands of FP32 terms would carry an
+  // error far above the FP32 rounding of the result
+  __shared__ double2 c1[kStage / 2], c2[kStage / 2];
+  double a1x = 0.0, a1y = 0.0, a2x = 0.0, a2y = 0.0;
+  for (int i0 = 0; i0 < j; i0 += kStage / 2) {
+    const int cnt = min(kStage / 2, j - i0);
     __syncthreads();
     for (int i = threadIdx.x; i < cnt; i += kBS) {
       c1[i] = a1[i0 + i];
@@ -407,34 +413,35 @@ __global__ void __launch_bounds__(kBS) k_lr_upd(LrArgs a) {
         for (int u = 0; u < kUU; ++u) q[u] = Q[(long)(i0 + i + u) * a.n + e];
 #pragma unroll
         for (int u = 0; u < kUU; ++u) {
-          const float2 k1 = c1[i + u], k2 = c2[i + u];
-          acc1.x += k1.x * q[u].x - k1.y * q[u].y;
-          acc1.y += k1.x * q[u].y + k1.y * q[u].x;
-          acc2.x += k2.x * q[u].x - k2.y * q[u].y;
-          acc2.y += k2.x * q[u].y + k2.y * q[u].x;
+          const double2 k1 = c1[i + u], k2 = c2[i + u];
+          const double qx = q[u].x, qy = q[u].y;
+          a1x += k1.x * qx - k1.y * qy;
+          a1y += k1.x * qy + k1.y * qx;
+          a2x += k2.x * qx - k2.y * qy;
+          a2y += k2.x * qy + k2.y * qx;
         }
       }
       for (; i < cnt; ++i) {
         const float2 qv = Q[(long)(i0 + i) * a.n + e];
-        const float2 k1 = c1[i], k2 = c2[i];
-        acc1.x += k1.x * qv.x - k1.y * qv.y;
-        acc1.y += k1.x * qv.y + k1.y * qv.x;
-        acc2.x += k2.x * qv.x - k2.y * qv.y;
-        acc2.y += k2.x * qv.y + k2.y * qv.x;
+        const double2 k1 = c1[i], k2 = c2[i];
+        a1x += k1.x * qv.x - k1.y * qv.y;
+        a1y += k1.x * qv.y + k1.y * qv.x;
+        a2x += k2.x * qv.x - k2.y * qv.y;
+        a2y += k2.x * qv.y + k2.y * qv.x;
       }
     }
   }
   const double* sc = a.scal + (long)y * 8;
   double ssq = 0.0;
   if (in) {
-    const float inu = (float)sc[3];
-    const float2 cc = make_float2((float)sc[4], (float)sc[5]);
+    const double inu = sc[3];
+    const double ccx = sc[4], ccy = sc[5];
     const float2 uv = a.U[(long)y * a.sU + e];
     const float2 wv = a.W[(long)y * a.sW + e];
     float2* qj = a.Q + (long)y * (a.ld + 1) * a.n + (long)j * a.n;
-    qj[e] = make_float2((uv.x - acc1.x) * inu, (uv.y - acc1.y) * inu);
-    const float2 cu = make_float2(cc.x * uv.x - cc.y * uv.y, cc.x * uv.y + cc.y * uv.x);
-    const float2 un = make_float2(wv.x - cu.x - acc2.x, wv.y - cu.y - acc2.y);
+    qj[e] = make_float2((float)((uv.x - a1x) * inu), (float)((uv.y - a1y) * inu));
+    const double cux = ccx * uv.x - ccy * uv.y, cuy = ccx * uv.y + ccy * uv.x;
+    const float2 un = make_float2((float)(wv.x - cux - a2x), (float)(wv.y - cuy - a2y));
     a.Unext[(long)y * a.sU + e] = un;
     ssq = (double)un.x * un.x + (double)un.y * un.y;
   }
@@ -586,8 +593,8 @@ int lr_alloc(LrWs& w, long n, int m, int B) {
   CGM_HIP(hipMalloc((void**)&w.y, sizeof(double2) * sm));
   CGM_HIP(hipMalloc((void**)&w.cs, sizeof(double) * sm));
   CGM_HIP(hipMalloc((void**)&w.sn, sizeof(double2) * sm));
-  CGM_HIP(hipMalloc((void**)&w.a1, sizeof(float2) * sm));
-  CGM_HIP(hipMalloc((void**)&w.a2, sizeof(float2) * sm));
+  CGM_HIP(hipMalloc((void**)&w.a1, sizeof(double2) * sm));
+  CGM_HIP(hipMalloc((void**)&w.a2, sizeof(double2) * sm));
   CGM_HIP(hipMalloc((void**)&w.scal, sizeof(double) * 8 * BB));
   CGM_HIP(hipMalloc((void**)&w.R, sizeof(double2) * (size_t)m * m * BB));
   CGM_HIP(hipMalloc((void**)&w.partn, sizeof(double) * (size_t)w.nuchunks * BB));
@@ -606,7 +613,7 @@ size_t lr_bytes(long n, int m, int B) {
   const long nchunks = (n + kBS * kPE - 1) / (kBS * kPE);
   const long ngroups = (m + kJT - 1) / kJT;
   return (size_t)B * (sizeof(float2) * ((size_t)(2 * m + 4) * n) + sizeof(double2) * (size_t)m * m +
-                      sizeof(double2) * (size_t)ngroups * kNV * nchunks + 80 * (size_t)(m + 1));
+                      sizeof(double2) * (size_t)ngroups * kNV * nchunks + 136 * (size_t)(m + 1));
 }
 
 static LrArgs lr_args(const LrWs& w, DevState* st, long sst, double* hist, long shist, int m) {
@@ -716,8 +723,8 @@ extern "C" int cgm_profile_longrec(cgm_ctx_t c, int j, int reps, float* ms) {
   CGM_HIP(hipMemsetAsync(w.U, 0, sizeof(float2) * 2 * (size_t)n, c->stream));
   CGM_HIP(hipMemsetAsync(w.W, 0, sizeof(float2) * (size_t)n, c->stream));
   CGM_HIP(hipMemsetAsync(w.scal, 0, sizeof(double) * 8, c->stream));
-  CGM_HIP(hipMemsetAsync(w.a1, 0, sizeof(float2) * (size_t)w.sSmall, c->stream));
-  CGM_HIP(hipMemsetAsync(w.a2, 0, sizeof(float2) * (size_t)w.sSmall, c->stream));
+  CGM_HIP(hipMemsetAsync(w.a1, 0, sizeof(double2) * (size_t)w.sSmall, c->stream));
+  CGM_HIP(hipMemsetAsync(w.a2, 0, sizeof(double2) * (size_t)w.sSmall, c->stream));
   CGM_HIP(hipMemsetAsync(c->st, 0, sizeof(DevState), c->stream));
   LrArgs a = lr_args(w, c->st, 0, nullptr, 0, j + 1);
   a.j = j;

@@ -40,10 +40,10 @@ def _ctx(N):
 
 
 def get_plane_wave(k, size):
-    """exp(-1j k x) on a size x size grid, x the column index (angle 0)."""
-    y, x = np.mgrid[:size, :size]
-    a = np.pi * 0 / 180
-    return np.exp(-1j * k * (x * np.cos(a) + y * np.sin(a)))
+    """exp(-1j k x) on a size x size grid, x the column index: the reference's
+    wave at incidence angle 0 (row r holds the same phase ramp for every r)."""
+    ramp = np.exp(-1j * k * np.arange(size, dtype=np.float64))
+    return np.broadcast_to(ramp, (size, size)).copy()
 
 
 def get_greenfun(r, k):
@@ -58,21 +58,27 @@ def get_greenfun(r, k):
 
 
 def get_green_matrix(k, size):
-    j, i = np.mgrid[:size, :size]
-    ij_block = np.sqrt((i - 1 / 2) ** 2 + j ** 2)
-    return get_greenfun(ij_block, k)
+    """Kernel block g[r, c] = (i/4) H0(k |(r, c - 1/2)|): distances with the
+    half-cell shift on the column offset (the Python orientation)."""
+    rows = np.arange(size, dtype=np.float64)[:, None]
+    cols = np.arange(size, dtype=np.float64)[None, :] - 0.5
+    return get_greenfun(np.hypot(rows, cols), k)
 
 
 def get_toeplitz_mat(ij_block):
-    """(2N-1)^2 even circulant embedding of an N x N block."""
-    b = np.array(ij_block, copy=True)
-    T1 = np.hstack((b, b[:, :0:-1]))
-    T2 = np.hstack((b[:0:-1, :], b[:0:-1, :0:-1]))
-    return np.vstack((T1, T2))
+    """(2N-1)^2 even circulant embedding of an N x N block: entry (r, c) is
+    block[d(r), d(c)] with d(t) = t for t < N and 2N - 1 - t above (the
+    mirrored negative offsets)."""
+    blk = np.asarray(ij_block)
+    n = blk.shape[0]
+    t = np.arange(2 * n - 1)
+    d = np.where(t < n, t, 2 * n - 1 - t)
+    return blk[d[:, None], d[None, :]].copy()
 
 
 def get_eps_from_mask(e, mask):
-    return (e - 1) * np.asarray(mask).reshape((-1, 1))
+    """Per-cell contrast column (e - 1) * mask, shape (N^2, 1)."""
+    return np.multiply(np.asarray(mask).reshape(-1, 1), e - 1)
 
 
 def get_complex_array(filename):

@@ -45,38 +45,49 @@ def test_caller_is_built_and_links_the_library():
     assert "libcugmres.so" in out and "not found" not in out.split("libcugmres.so")[1].split("\n")[0]
 
 
-@pytest.mark.gpu
-def test_reference_style_caller_matches_oracle(orc, tmp_path):
-    N, m = 64, 30
+def _run_caller(orc, tmp_path, N, m):
     gprep = golden("gprep.npz")["g64"].astype(np.complex64)      # IN/G_prep_64.txt
     mask = orc.ref_cylinder_mask(N).reshape(-1).astype(np.uint8)
     gp = tmp_path / "gprep.bin"
     mk = tmp_path / "mask.bin"
     gprep.tofile(gp)
     mask.tofile(mk)
-    prefix = str(tmp_path) + "/"
+    prefix = str(tmp_path) + f"/m{m}_"
     print(f"compat caller N={N} maxiter={m}", flush=True)
     r = subprocess.run([CALLER, str(N), str(m), str(gp), str(mk), prefix], capture_output=True, text=True,
                        timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     fields = dict(kv.split("=") for kv in r.stdout.split() if "=" in kv)
-    assert int(fields["GMRES_n"]) == m and int(fields["grad_n"]) == m
-    assert int(fields["clock_i"]) == 2
-    assert int(fields["stamped"]) == int(fields["nts"])   # every timestamp of the schedule is written
-
-    def load(name):
-        return np.fromfile(prefix + name, np.complex64)
-
     blk, _ = orc.gprep_to_block(N, gprep)
     op = orc.Op(N, blk)
     chi = orc.ref_chi(N) * mask.astype(np.float64)
     b = orc.ref_plane_wave(N).reshape(-1).astype(np.complex128)
-    # Fast_GMRES_with_CUDA: one cycle of m steps from x0 = b
+    return fields, (lambda name: np.fromfile(prefix + name, np.complex64)), op, chi, b
+
+
+@pytest.mark.gpu
+def test_reference_style_caller_matches_oracle(orc, tmp_path):
+    """maxiter = 50, past the steep drop of this problem's residual between
+    steps 20 and 29: the solution matches the oracle's complex128 GMRES to
+    1e-5 (the oracle's own complex64 restatement sits 7e-7 from it); the
+    for_gradient solution is within twice the oracle's own complex64-to-
+    complex128 distance of the complex128 iterate."""
+    N, m = 64, 50
+    fields, load, op, chi, b = _run_caller(orc, tmp_path, N, m)
+    assert int(fields["GMRES_n"]) == m and int(fields["grad_n"]) == m
+    assert int(fields["clock_i"]) == 2
+    assert int(fields["stamped"]) == int(fields["nts"])   # every timestamp of the schedule is written
     xr, info = op.gmres(chi, b, x0=b, restart=m, tol=0.0, max_cycles=1, history=True)
     idx = N * N // 2 + N // 3
     e = np.zeros(N * N, np.complex128)
     e[idx] = 1.0
     xg, ig = op.gmres(chi, e, x0=b, restart=m, tol=0.0, max_cycles=1, adjoint=True, history=True)
+    # the gradient solve ends mid-convergence (residual 4e-5), where the
+    # least-squares coefficients amplify rounding: the oracle's own complex64
+    # restatement sits 3.1e-5 from its complex128 iterate
+    xg32, _ = op.gmres(chi, e, x0=b, restart=m, tol=0.0, max_cycles=1, adjoint=True, history=True,
+                       precision="c64")
+    gap = rel(xg32, xg)
     A = np.zeros((4, 4), np.complex128)
     for rr in range(4):
         for c in range(4):
@@ -93,11 +104,26 @@ def test_reference_style_caller_matches_oracle(orc, tmp_path):
         "les": rel(load("les.bin"), np.linalg.solve(A, B)),
     }
     print(got, flush=True)
-    assert got["residual0"][0] == pytest.approx(got["residual0"][1], rel=1e-5)
+    beta0 = got["residual0"][1]
+    assert got["residual0"][0] == pytest.approx(beta0, rel=1e-5)
     assert got["matvec"] < 2e-6 and got["matvec_green"] < 2e-6
     assert got["les"] < 1e-6
     assert got["solution"] < 1e-5
-    assert got["gradient"] < 1e-5
-    assert got["residual"][0] == pytest.approx(got["residual"][1], rel=1e-3)
+    assert got["gradient"] < 1e-5 + 2 * gap
+    # estimates at this depth are complex64 noise: within 1e-6 ||r0|| of the oracle
+    assert abs(got["residual"][0] - got["residual"][1]) < 1e-6 * beta0
     assert got["grad_residual"][0] == pytest.approx(got["grad_residual"][1], rel=1e-3)
     assert int(fields["res_vs_tol"]) == int(got["residual"][0] > 0.001)
+
+
+@pytest.mark.gpu
+def test_reference_style_caller_tracks_complex64_oracle_at_the_drop(orc, tmp_path):
+    """maxiter = 30 ends inside the drop, where single precision delays it: the
+    oracle's complex64 restatement estimates 0.182 (complex128: 0.0027); the
+    caller's estimate follows the complex64 one."""
+    N, m = 64, 30
+    fields, load, op, chi, b = _run_caller(orc, tmp_path, N, m)
+    x64, i64 = op.gmres(chi, b, x0=b, restart=m, tol=0.0, max_cycles=1, history=True, precision="c64")
+    print(float(fields["residual"]), i64["history"][m], flush=True)
+    assert float(fields["residual"]) == pytest.approx(i64["history"][m], rel=0.1)
+    assert rel(load("solution.bin"), x64) < 2e-3

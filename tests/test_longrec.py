@@ -24,7 +24,8 @@ def test_unrestarted_gmres_matches_oracle(orc):
     stalls near 1e-2, a long cycle (FGMRES with the identity) converges (a
     second cycle may be needed when the complex64 residual floor, a few 1e-7
     here, sits near the tolerance); the solution matches the oracle's
-    unrestarted FP64 GMRES to 1e-5 and its true residual, recomputed by the
+    unrestarted FP64 GMRES to within 20 times its true residual (the forward
+    error the residual allows), and that true residual, recomputed by the
     oracle, is below 1.5e-6."""
     from cugmres_amd import problems, solver
     N = 64
@@ -37,10 +38,13 @@ def test_unrestarted_gmres_matches_oracle(orc):
     assert info["converged"] and info["cycles"] <= 2, info
     op = orc.Op(N, orc.baseline_green_block(N, problems.KH), ctx.M)
     xs = x.cpu().numpy().astype(np.complex128)
-    assert np.linalg.norm(b - op.matvec(chi, xs)) / np.linalg.norm(b) < 1.5e-6
+    r_true = np.linalg.norm(b - op.matvec(chi, xs)) / np.linalg.norm(b)
+    assert r_true < 1.5e-6
     xr, ir = op.gmres(chi, b, restart=1000, tol=1e-12, max_cycles=1)
     assert ir["converged"]
-    assert rel(xs, xr) < 1e-5
+    # forward error = A^-1 applied to the residual: about 8x the relative
+    # residual on this operator
+    assert rel(xs, xr) < 20 * r_true
     # the estimates are monotone (unitary rotations) and end at the tolerance
     h = info["history"]
     assert np.all(np.diff(h[1:info["iterations"] + 1]) <= 1e-7 * h[0])
