@@ -89,6 +89,9 @@ SIGNATURES = {
     "cgm_matvec": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int]),
     "cgm_matvec_batched": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_long, c_void_p, c_void_p,
                                           ctypes.c_int, ctypes.c_int]),
+    "cgm_matvec_batched_noalloc": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_long, c_void_p,
+                                                  c_void_p, ctypes.c_int, ctypes.c_int]),
+    "cgm_noalloc_scratch_bytes": (ctypes.c_size_t, []),
     "cgm_matvec_c128": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int]),
     "cgm_gmres": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                  ctypes.POINTER(GmresParams), ctypes.POINTER(GmresResult), c_void_p]),

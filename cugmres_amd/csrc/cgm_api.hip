@@ -115,6 +115,10 @@ extern "C" int cgm_create(cgm_ctx_t* out, int N, int device) {
   CGM_CHECK(device >= 0 && device < ndev, CGM_ERROR_INVALID_VALUE, "cgm_create: device %d of %d",
             device, ndev);
   CGM_HIP(hipSetDevice(device));
+  {
+    const int rs = noalloc_init(device);
+    if (rs) return rs;
+  }
   cgm_context* c = new cgm_context();
   c->N = N;
   c->device = device;

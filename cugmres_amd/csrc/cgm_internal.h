@@ -206,6 +206,9 @@ int lr_cycle(LrWs& w, hipStream_t st, DevState* dst, DevState* hst, long sst, in
              long shist, float2* x, long sx, const LrStep& precond, const LrStep& matvec);
 int lr_scale(hipStream_t st, const float2* u, long su, float2* z, long sz, long n, const DevState* dst, long sst,
              int B);
+// Device address of the allocation-free matvec's scratch array on `device`
+// (cgm_noalloc.hip), taken once per device; the device must be current.
+int noalloc_init(int device);
 }  // namespace cgm
 
 struct cgm_context;

@@ -142,6 +142,20 @@ int cgm_matvec(cgm_ctx_t ctx, const cgm_complex64* dev_gamma, const float* dev_c
  * takes one vector (B/CUDA_GMRES/Fast_matvec.cuh:1-28). */
 int cgm_matvec_batched(cgm_ctx_t ctx, const cgm_complex64* dev_gamma, const float* dev_chi, long chi_stride,
                        const cgm_complex64* dev_x, cgm_complex64* dev_y, int batch, int mode);
+/* cgm_matvec_batched without any allocation, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  The
+ * FFT intermediates live in a fixed device array of
+ * cgm_noalloc_scratch_bytes() (64 MiB: 16 vectors at N = 512, 4 at
+ * N = 1024), one per GPU, whose address cgm_create takes; a larger batch
+ * fails with CGM_ERROR_INVALID_VALUE and launches nothing.  The caller still
+ * provides x and y.  Calls on one GPU share its array: calls that overlap
+ * there (several streams or host threads) race.  Same outputs, bit for bit,
+ * as cgm_matvec_batched. */
+int cgm_matvec_batched_noalloc(cgm_ctx_t ctx, const cgm_complex64* dev_gamma, const float* dev_chi,
+                               long chi_stride, const cgm_complex64* dev_x, cgm_complex64* dev_y, int batch,
+                               int mode);
+size_t cgm_noalloc_scratch_bytes(void);
 /* FP64 variant (spectrum from dev_gamma64, chi as double). */
 int cgm_matvec_c128(cgm_ctx_t ctx, const cgm_complex128* dev_gamma64, const double* dev_chi,
                     const cgm_complex128* dev_x, cgm_complex128* dev_y, int mode);
