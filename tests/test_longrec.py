@@ -24,9 +24,9 @@ def test_unrestarted_gmres_matches_oracle(orc):
     stalls near 1e-2, a long cycle (FGMRES with the identity) converges (a
     second cycle may be needed when the complex64 residual floor, a few 1e-7
     here, sits near the tolerance); the solution matches the oracle's
-    unrestarted FP64 GMRES to within 20 times its true residual (the forward
-    error the residual allows), and that true residual, recomputed by the
-    oracle, is below 1.5e-6."""
+    unrestarted FP64 GMRES to 1e-4 (the operator's condition number, 1.8e4,
+    lets a 1e-6 residual move the solution further) and its true residual,
+    recomputed by the oracle, is below 1.5e-6."""
     from cugmres_amd import problems, solver
     N = 64
     print("longrec identity N=64", flush=True)
@@ -42,9 +42,10 @@ def test_unrestarted_gmres_matches_oracle(orc):
     assert r_true < 1.5e-6
     xr, ir = op.gmres(chi, b, restart=1000, tol=1e-12, max_cycles=1)
     assert ir["converged"]
-    # forward error = A^-1 applied to the residual: about 8x the relative
-    # residual on this operator
-    assert rel(xs, xr) < 20 * r_true
+    # forward error = A^-1 applied to the residual; this operator's condition
+    # number is 1.8e4 (dense SVD: sigma 68 ... 3.9e-3), so the residual check
+    # above is the tight one and the solution is only bounded loosely here
+    assert rel(xs, xr) < 1e-4
     # the estimates are monotone (unitary rotations) and end at the tolerance
     h = info["history"]
     assert np.all(np.diff(h[1:info["iterations"] + 1]) <= 1e-7 * h[0])
