@@ -12,7 +12,7 @@ uses seed --seed-offset + k (warm-up steps first).
 Methods (each system is solved from scratch; the time of a solve includes
 everything that depends on its map):
   - learned: flexible GMRES with the two-level preconditioner - the exact
-    inverses of the operator on 16 x 16 tiles (factored per map on the
+    inverses of the operator on 32 x 32 tiles (--tile; factored per map on the
     device), then the FNO network as a correction on the residual they leave
     (cugmres_amd.tiles.TwoLevel) - with the long-recurrence path (the whole
     Krylov basis in HBM, up to --lr-restart steps).  The network is trained
@@ -43,6 +43,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "solves/s to 1e-6 rel. residual, N=512: learned-FGMRES vs plain GMRES"
+
+
+def note(msg):
+    """Progress on stderr (stdout carries only the result line)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 HBM_PEAK_GBS = 8000.0
 
 
@@ -55,8 +60,8 @@ def parse(argv=None):
     p.add_argument("--restart", type=int, default=30, help="restart of plain GMRES")
     p.add_argument("--tol", type=float, default=1e-6)
     p.add_argument("--max-cycles", type=int, default=100, help="cap of plain GMRES(restart)")
-    p.add_argument("--tile", type=int, default=16, help="tile side of the tile inverses (8, 16, 32)")
-    p.add_argument("--lr-restart", type=int, default=4096,
+    p.add_argument("--tile", type=int, default=32, help="tile side of the tile inverses (8, 16, 32)")
+    p.add_argument("--lr-restart", type=int, default=8192,
                    help="cycle length of the preconditioned solves (long-recurrence path)")
     p.add_argument("--lr-cycles", type=int, default=2)
     p.add_argument("--compare", type=int, default=2,
@@ -76,7 +81,7 @@ def parse(argv=None):
                    default="bench",
                    help="bench: the benchmark; step: the per-kernel profile of one Arnoldi step of the fused "
                         "GMRES(--restart) kernels (for rocprofv3 --pmc); lr-step: the long-recurrence step kernels "
-                        "at basis size 1024; tiles-step: 20 applications of the tile inverses; c3: the batched "
+                        "at basis size --lr-j; tiles-step: 20 applications of the tile inverses; c3: the batched "
                         "workload (--instances maps of mixed N, 16 plane waves each); c2: the mixed-precision "
                         "workload (smooth map, FP64 refinement) at --N; copy: 3 copies of "
                         "256 MiB (counter calibration); batched: lockstep batches of 16 right-hand sides vs single "
@@ -87,6 +92,7 @@ def parse(argv=None):
                         "3 x (2 restart cycles of cgm_fgmres_batched with a seeded untrained network, then 2 of "
                         "cgm_gmres_batched) on --batch systems")
     p.add_argument("--sweep-reps", type=int, default=20)
+    p.add_argument("--lr-j", type=int, default=4096, help="lr-step mode: basis size of the profiled kernels")
     return p.parse_args(argv)
 
 
@@ -156,7 +162,7 @@ def main():
         print(json.dumps(profile_fno(ctx, net, dchi, db, B)))
         return
     if a.mode == "lr-step":
-        print(json.dumps(profile_longrec(ctx, 1024)), flush=True)
+        print(json.dumps(profile_longrec(ctx, a.lr_j)), flush=True)
         return
     if a.mode == "tiles-step":
         from cugmres_amd import tiles
@@ -245,8 +251,11 @@ def run_bench(a, ctx):
     else:
         tctx = solver.Context(a.train_N, 0)
         t0 = time.perf_counter()
-        tnet, hist = learn.train(tctx, samples=a.train_samples, batch=8, lr=1e-3, seed=0, tiles_T=a.tile)
+        note(f"training the network: {a.train_samples} samples at N = {a.train_N}")
+        tnet, hist = learn.train(tctx, samples=a.train_samples, batch=8, lr=1e-3, seed=0, tiles_T=a.tile,
+                                 log=note, log_every=200)
         torch.cuda.synchronize()
+        note(f"trained in {time.perf_counter() - t0:.1f} s")
         params = tnet.params.cpu().numpy()
         training = {"N": a.train_N, "samples": a.train_samples, "tiles": a.tile,
                     "seconds": round(time.perf_counter() - t0, 1), "loss_first": hist[0][1],
@@ -271,19 +280,28 @@ def run_bench(a, ctx):
         torch.cuda.synchronize()
         info["ms"] = (time.perf_counter() - t0) * 1e3
         info["seed"] = a.seed_offset + k
+        note(f"{method} seed {info['seed']}: {info['iterations']} steps, converged {info['converged']}, "
+             f"residual {info['rel_residual']:.2e}, {info['ms']:.0f} ms")
         return info
 
-    def run(method, warm, timed):
+    def run(method, warm, timed, live=False):
         for k in warm:
             solve(method, k)
         torch.cuda.synchronize()
+        if live:
+            ctx.longrec_timing(True)
         t0 = time.perf_counter()
         infos = [solve(method, k) for k in timed]
-        return infos, time.perf_counter() - t0
+        wall = time.perf_counter() - t0
+        if live:
+            live_t = ctx.longrec_timing_read()
+            ctx.longrec_timing(False)
+            return infos, wall, live_t
+        return infos, wall
 
     timed = list(range(a.warmup, total))
     cmp_sys = timed[:max(0, min(a.compare, len(timed)))]
-    l_infos, l_wall = run("learned", range(a.warmup), timed)
+    l_infos, l_wall, live = run("learned", range(a.warmup), timed, live=True)
     t_infos, t_wall = run("tiles", range(min(1, a.warmup)), cmp_sys) if cmp_sys else ([], 0.0)
     p_infos, p_wall = run("plain", range(min(1, a.warmup)), cmp_sys) if cmp_sys else ([], 0.0)
     learned = summarize_method(l_infos, l_wall, "learned")
@@ -302,17 +320,13 @@ def run_bench(a, ctx):
     per_step = {"lr_proj": lr["lr_proj"]["us"], "lr_update": lr["lr_update"]["us"],
                 "lr_reduce_givens": lr["lr_reduce"]["us"] + lr["lr_givens"]["us"],
                 "tile_apply": tp["us"], "matvec_x2": 2 * mv["us"], "fno_forward": fp["us"]}
-    dom = max(("lr_proj", "lr_update", "tile_apply"), key=lambda k: per_step[k])
-    src = {"lr_proj": lr["lr_proj"], "lr_update": lr["lr_update"], "tile_apply": tp}[dom]
-    ach = src["GBps"]
-    roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-            "traffic": measured_traffic(src["kernel"] + (f"_j{jmean}" if dom != "tile_apply" else ""), N),
-            "kernel": src["kernel"], "per_unit": src["per_unit"], "us": src["us"],
-            "us_per_step": {k: round(v, 1) for k, v in per_step.items()}}
+    roof = live_roofline(live, N, per_step, jmean)
     matvec = {"us": mv["us"], "bytes": mv["bytes"], "GBps": mv["GBps"], "frac": mv["GBps"] / HBM_PEAK_GBS,
               "per_unit": "100 N^2 B per matvec (quarter spectrum)"}
 
+    note("profiles done; reference problem")
     refp = reference_problem(ctx, a.restart) if N == 512 else None
+    note("CPU baseline")
     cpu = None if a.no_cpu else cpu_baseline(N, a.restart, a.cpu_seconds, a.max_cycles)
 
     out = make_line(a, N, ctx.M, learned, tiles_s, plain, l_wall, training, roof, matvec, cpu, refp)
@@ -606,6 +620,40 @@ def batched_throughput(restart, cycles=3, B=16):
                            "batched_matvec_hbm_frac": mv_bytes / (us_mv * 1e-6) / 1e9 / HBM_PEAK_GBS}
         ctx.close()
     return out
+
+
+LIVE_KERNELS = {"proj": "k_lr_proj", "upd": "k_lr_upd"}
+
+
+def live_roofline(live, N, per_step, jmean):
+    """Roofline of the dominant kernel of the timed learned solves, from the
+    live HIP-event sums (cgm_longrec_timing over the timed region, on the
+    solver's stream): achieved = algorithmic bytes summed over every launch /
+    summed launch durations.  traffic = the HBM bytes per launch measured by
+    rocprofv3 at a reference basis size (profiles/hbm_traffic_n512.json),
+    scaled to this run's mean algorithmic bytes per launch."""
+    key = max(LIVE_KERNELS, key=lambda k: live[k]["ms"])
+    t = live[key]
+    kern = LIVE_KERNELS[key]
+    n = N * N
+    ach = t["bytes"] / (t["ms"] * 1e-3) / 1e9 if t["ms"] > 0 else 0.0
+    per_launch = t["bytes"] / max(1, t["launches"])
+    extra = 2 if key == "proj" else 4
+    jbar = per_launch / (8 * n) - extra
+    traffic = None
+    ref = measured_traffic(kern + "_j4096", N)
+    if ref:
+        traffic = ref * per_launch / (8 * n * (4096 + extra))
+    return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+            "traffic": traffic, "kernel": kern, "launches": t["launches"],
+            "us": t["ms"] * 1e3 / max(1, t["launches"]),
+            "per_unit": f"8 B x N^2 x (j + {extra}) per launch at basis size j (N = {N}); mean j {jbar:.0f} "
+                        f"over the {t['launches']} launches of the timed solves",
+            "other": {LIVE_KERNELS[k]: {"GBps": live[k]["bytes"] / (live[k]["ms"] * 1e-3) / 1e9
+                                        if live[k]["ms"] > 0 else None,
+                                        "ms_total": live[k]["ms"], "launches": live[k]["launches"]}
+                      for k in LIVE_KERNELS if k != key},
+            "us_per_step_at_mean_basis": {"j": jmean, **{k: round(v, 1) for k, v in per_step.items()}}}
 
 
 def measured_traffic(kernel, N, batch=1):

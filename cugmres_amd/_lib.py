@@ -150,6 +150,8 @@ SIGNATURES = {
     "cgm_synchronize": (ctypes.c_int, [c_void_p]),
     "cgm_bessel01": (None, [ctypes.c_double] + [c_double_p] * 4),
     "cgm_green_values": (ctypes.c_int, [c_void_p, ctypes.c_size_t, ctypes.c_double, c_void_p]),
+    "cgm_longrec_timing": (ctypes.c_int, [c_void_p, ctypes.c_int]),
+    "cgm_longrec_timing_read": (ctypes.c_int, [c_void_p, c_void_p]),
     "cgm_profile_longrec": (ctypes.c_int, [c_void_p, ctypes.c_int, ctypes.c_int, c_float_p]),
     "cgm_profile_step": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, ctypes.c_int,
                                         c_float_p]),

@@ -95,6 +95,8 @@ extern "C" int cgm_destroy(cgm_ctx_t c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   krylov_free(c->ws);
   lr_free(c->lr);
+  if (c->lrt.created)
+    for (hipEvent_t e : c->lrt.ev) (void)hipEventDestroy(e);
   void* ptrs[] = {c->tw32, c->tw64, c->T32, c->T64, c->st, c->chi_tmp, c->vec_tmp, c->v64a, c->chi32, c->zb,
                   c->stB, c->T32B, c->les_ws, c->pc_v, c->pc_r, c->pc_z, c->chi64, c->z64, c->w64, c->T64B};
   for (void* p : ptrs)
@@ -1348,7 +1350,7 @@ static int lr_solve(cgm_context* c, const float2* gq, const float* chi, long chi
     beta_prev[0] = hst->beta;
     last_pc = L.use_pc;
     if ((rc = lr_cycle(c->lr, c->stream, dst, hst, sst, B, m, hist ? hist + (long)cycles * m : nullptr, hstride, x, n,
-                       pre, mv)))
+                       pre, mv, c->lrt.on ? &c->lrt : nullptr)))
       return rc;
     CGM_HIP(hipMemcpyAsync(hst, dst, sizeof(DevState) * (batched ? B : 1), hipMemcpyDeviceToHost, c->stream));
     CGM_HIP(hipStreamSynchronize(c->stream));

@@ -171,7 +171,7 @@ int launch_norm64(hipStream_t st, const double2* v, long n, double* partn, int G
 // Long-recurrence flexible GMRES (cgm_longrec.hip): restart up to
 // kMaxRestartLR, the whole basis in HBM, lagged CGS2 with two basis passes
 // per step.
-constexpr int kMaxRestartLR = 4096;
+constexpr int kMaxRestartLR = 16384;
 struct LrWs {
   int m = 0, B = 0;
   long n = 0;
@@ -189,6 +189,18 @@ struct LrWs {
   double2* R = nullptr;    // [B][m][m] column-major
   double* partn = nullptr;
 };
+// Live timing of the two streaming step kernels (projection, update) over
+// a solve: HIP events around each launch on the solver's stream, read back
+// every kRing steps; summed durations, algorithmic bytes and launch counts.
+struct LrTiming {
+  static constexpr int kRing = 32;
+  bool on = false, created = false;
+  hipEvent_t ev[4 * kRing];
+  double slot_bytes[2][kRing];
+  bool slot_upd[kRing];
+  double ms[2] = {0.0, 0.0}, bytes[2] = {0.0, 0.0};
+  long launches[2] = {0, 0};
+};
 int lr_alloc(LrWs& w, long n, int m, int B);
 void lr_free(LrWs& w);
 // Device bytes of a long-recurrence workspace.
@@ -203,7 +215,7 @@ struct LrStep {
   }
 };
 int lr_cycle(LrWs& w, hipStream_t st, DevState* dst, DevState* hst, long sst, int B, int m, double* hist,
-             long shist, float2* x, long sx, const LrStep& precond, const LrStep& matvec);
+             long shist, float2* x, long sx, const LrStep& precond, const LrStep& matvec, LrTiming* tm = nullptr);
 int lr_scale(hipStream_t st, const float2* u, long su, float2* z, long sz, long n, const DevState* dst, long sst,
              int B);
 // Device address of the allocation-free matvec's scratch array on `device`
@@ -255,6 +267,7 @@ struct Precond {
 
 struct cgm_context {
   int N = 0, M = 0, device = 0;
+  cgm::LrTiming lrt;   // live timing of the long-recurrence kernels (off by default)
   hipStream_t stream = nullptr;
   cgm::Plan plan{};    // complex64 passes (autotuned tiles)
   cgm::Plan plan64{};  // complex128 passes and the spectrum precompute (default tiles)

@@ -40,7 +40,8 @@ namespace cgm {
 namespace {
 
 constexpr int kBS = 256;     // threads per streaming work-group
-constexpr int kPE = 4;       // elements per thread in the projection (chunk = 1024)
+constexpr int kPE = 16;      // elements per thread in the projection (chunk = 4096)
+constexpr int kPP = kPE / 2; // element pairs per thread (one 16-byte load each)
 constexpr int kJT = 16;      // basis vectors per projection work-group
 constexpr int kNV = 2 * kJT + 2;  // values per projection work-group: s, t, (alpha, gamma)
 constexpr int kUU = 16;      // basis loads in flight per thread in the update
@@ -89,10 +90,40 @@ __device__ inline float2 cj_mul(float2 a, float2 b) {  // conj(a) b
   return make_float2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
 }
 
+// Elements 2p and 2p+1 of a complex vector of n elements: one 16-byte load
+// when n is even (every row of the basis then starts 16-byte aligned), two
+// 8-byte loads otherwise; zero beyond n.
+template <bool kVec>
+__device__ inline void load_pair(const float2* __restrict__ v, long p, long n, float2& lo, float2& hi) {
+  const long e = 2 * p;
+  if (kVec) {
+    if (e < n) {
+      const float4 q = *reinterpret_cast<const float4*>(v + e);
+      lo = make_float2(q.x, q.y);
+      hi = make_float2(q.z, q.w);
+    } else {
+      lo = hi = make_float2(0.f, 0.f);
+    }
+  } else {
+    lo = e < n ? v[e] : make_float2(0.f, 0.f);
+    hi = e + 1 < n ? v[e + 1] : make_float2(0.f, 0.f);
+  }
+}
+
 // PROJ: partial sums of s_v = q_v^H u, t_v = q_v^H w for the kJT vectors of
-// group g over chunk c, and (group 0) alpha = u^H u, gamma = u^H w.  Stored
-// value-major, part[sys][g][value][chunk], so the reduction reads contiguous
-// runs.  with_w = 0 at the cycle's last step (u_m only).
+// group g over chunk c (4096 elements, 16 per thread), and (group 0)
+// alpha = u^H u, gamma = u^H w.  Stored value-major,
+// part[sys][g][value][chunk], so the reduction reads contiguous runs.
+// with_w = 0 at the cycle's last step (u_m only).
+//
+// Precision: each thread forms FP32 sums of four terms and accumulates them
+// in FP64; the work-group sums are FP64 (the long recurrences need inner
+// products accurate to the FP32 rounding of each term).  The 32 FP64 sums
+// of eight vectors (s, t; re, im) are reduced across a wave by a transposing
+// butterfly: at each of five exchange steps a lane keeps half of its values
+// and adds the partner lane's copy of that half, so lane L ends with value
+// L / 2 (one exchange per value instead of six).
+template <bool kVec>
 __global__ void __launch_bounds__(kBS) k_lr_proj(LrArgs a, int with_w) {
   const DevState* st;
   if (sys_done(a, st)) return;
@@ -100,93 +131,117 @@ __global__ void __launch_bounds__(kBS) k_lr_proj(LrArgs a, int with_w) {
   const int g = blockIdx.x / a.nchunks, c = blockIdx.x % a.nchunks;
   const int v0 = g * kJT;
   const int nv = min(kJT, a.j - v0);
-  const long base = (long)c * (kBS * kPE) + threadIdx.x;
+  const long pbase = (long)c * (kBS * kPP) + threadIdx.x;
   const float2* u = a.U + (long)y * a.sU;
   const float2* w = a.W + (long)y * a.sW;
   const float2* Q = a.Q + (long)y * (a.ld + 1) * a.n;
   float2 ue[kPE], we[kPE];
 #pragma unroll
-  for (int k = 0; k < kPE; ++k) {
-    const long e = base + k * kBS;
-    const bool in = e < a.n;
-    ue[k] = in ? u[e] : make_float2(0.f, 0.f);
-    we[k] = (in && with_w) ? w[e] : make_float2(0.f, 0.f);
-  }
-  float2 as[kJT], at[kJT];
-#pragma unroll
-  for (int v = 0; v < kJT; ++v) {
-    as[v] = make_float2(0.f, 0.f);
-    at[v] = make_float2(0.f, 0.f);
-    if (v < nv) {
-      const float2* q = Q + (long)(v0 + v) * a.n;
-      float2 qe[kPE];
-#pragma unroll
-      for (int k = 0; k < kPE; ++k) {
-        const long e = base + k * kBS;
-        qe[k] = e < a.n ? q[e] : make_float2(0.f, 0.f);
-      }
-#pragma unroll
-      for (int k = 0; k < kPE; ++k) {
-        const float2 p = cj_mul(qe[k], ue[k]), r = cj_mul(qe[k], we[k]);
-        as[v].x += p.x;
-        as[v].y += p.y;
-        at[v].x += r.x;
-        at[v].y += r.y;
-      }
+  for (int k = 0; k < kPP; ++k) {
+    load_pair<kVec>(u, pbase + (long)k * kBS, a.n, ue[2 * k], ue[2 * k + 1]);
+    if (with_w) {
+      load_pair<kVec>(w, pbase + (long)k * kBS, a.n, we[2 * k], we[2 * k + 1]);
+    } else {
+      we[2 * k] = we[2 * k + 1] = make_float2(0.f, 0.f);
     }
   }
-  float2 al = make_float2(0.f, 0.f), ga = make_float2(0.f, 0.f);
-  if (g == 0) {
-#pragma unroll
-    for (int k = 0; k < kPE; ++k) {
-      al.x += ue[k].x * ue[k].x + ue[k].y * ue[k].y;
-      const float2 r = cj_mul(ue[k], we[k]);
-      ga.x += r.x;
-      ga.y += r.y;
-    }
-  }
-  // block reduction in FP64 (each thread's four-term FP32 sums widened, then
-  // wave sums of doubles and the four waves): the long recurrences need
-  // inner products accurate to FP32 rounding of each term, not of 256-term
-  // FP32 partial sums
-  __shared__ double red[kBS / 64][kNV][2];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int nvals = 2 * nv + (g == 0 ? 2 : 0);
+  __shared__ double red[kBS / 64][32];
+  double* P = reinterpret_cast<double*>(a.part + (long)y * a.sPart + (long)g * kNV * a.nchunks + c);
 #pragma unroll
-  for (int v = 0; v < kJT; ++v) {
-    if (v < nv) {
-      const double sr = wave_sum((double)as[v].x), si = wave_sum((double)as[v].y);
-      const double tr = wave_sum((double)at[v].x), ti = wave_sum((double)at[v].y);
-      if (lane == 0) {
-        red[wave][v][0] = sr;
-        red[wave][v][1] = si;
-        red[wave][kJT + v][0] = tr;
-        red[wave][kJT + v][1] = ti;
+  for (int h = 0; h < kJT / 8; ++h) {
+    double acc[32];  // [vector 0..7][s.re, s.im, t.re, t.im]
+#pragma unroll
+    for (int i = 0; i < 32; ++i) acc[i] = 0.0;
+#pragma unroll
+    for (int vv = 0; vv < 8; ++vv) {
+      const int v = h * 8 + vv;
+      if (v < nv) {
+        const float2* q = Q + (long)(v0 + v) * a.n;
+        float2 qe[kPE];
+#pragma unroll
+        for (int k = 0; k < kPP; ++k) load_pair<kVec>(q, pbase + (long)k * kBS, a.n, qe[2 * k], qe[2 * k + 1]);
+#pragma unroll
+        for (int b4 = 0; b4 < kPE; b4 += 4) {
+          float sr = 0.f, si = 0.f, tr = 0.f, ti = 0.f;
+#pragma unroll
+          for (int k = b4; k < b4 + 4; ++k) {
+            const float2 p = cj_mul(qe[k], ue[k]), r = cj_mul(qe[k], we[k]);
+            sr += p.x;
+            si += p.y;
+            tr += r.x;
+            ti += r.y;
+          }
+          acc[vv * 4 + 0] += sr;
+          acc[vv * 4 + 1] += si;
+          acc[vv * 4 + 2] += tr;
+          acc[vv * 4 + 3] += ti;
+        }
       }
     }
+#pragma unroll
+    for (int stp = 0; stp < 5; ++stp) {
+      const int off = 32 >> stp, half = 16 >> stp;
+      const bool upper = (lane & off) != 0;
+#pragma unroll
+      for (int i = 0; i < half; ++i) {
+        const double keep = upper ? acc[i + half] : acc[i];
+        const double send = upper ? acc[i] : acc[i + half];
+        acc[i] = keep + __shfl_xor(send, off, 64);
+      }
+    }
+    const double tot = acc[0] + __shfl_xor(acc[0], 1, 64);
+    if ((lane & 1) == 0) red[wave][lane >> 1] = tot;
+    __syncthreads();
+    if (threadIdx.x < 32) {
+      const int vv = threadIdx.x >> 2, comp = threadIdx.x & 3, v = h * 8 + vv;
+      if (v < nv) {
+        double sum = 0.0;
+#pragma unroll
+        for (int wv = 0; wv < kBS / 64; ++wv) sum += red[wv][threadIdx.x];
+        const int val = (comp < 2 ? v : kJT + v);
+        P[2 * (long)val * a.nchunks + (comp & 1)] = sum;
+      }
+    }
+    __syncthreads();
   }
   if (g == 0) {
-    const double ar = wave_sum((double)al.x), gr = wave_sum((double)ga.x), gi = wave_sum((double)ga.y);
-    if (lane == 0) {
-      red[wave][2 * kJT][0] = ar;
-      red[wave][2 * kJT][1] = 0.0;
-      red[wave][2 * kJT + 1][0] = gr;
-      red[wave][2 * kJT + 1][1] = gi;
-    }
-  }
-  __syncthreads();
-  (void)nvals;
-  const int idx = threadIdx.x;
-  const bool valid = idx < kNV && ((idx < kJT && idx < nv) || (idx >= kJT && idx < 2 * kJT && idx - kJT < nv) ||
-                                   (idx >= 2 * kJT && g == 0));
-  if (valid) {
-    double re = 0.0, im = 0.0;
+    double al = 0.0, gr = 0.0, gi = 0.0;
 #pragma unroll
-    for (int wv = 0; wv < kBS / 64; ++wv) {
-      re += red[wv][idx][0];
-      im += red[wv][idx][1];
+    for (int b4 = 0; b4 < kPE; b4 += 4) {
+      float a4 = 0.f, r4 = 0.f, i4 = 0.f;
+#pragma unroll
+      for (int k = b4; k < b4 + 4; ++k) {
+        a4 += ue[k].x * ue[k].x + ue[k].y * ue[k].y;
+        const float2 r = cj_mul(ue[k], we[k]);
+        r4 += r.x;
+        i4 += r.y;
+      }
+      al += a4;
+      gr += r4;
+      gi += i4;
     }
-    a.part[(long)y * a.sPart + ((long)g * kNV + idx) * a.nchunks + c] = make_double2(re, im);
+    al = wave_sum(al);
+    gr = wave_sum(gr);
+    gi = wave_sum(gi);
+    if (lane == 0) {
+      red[wave][0] = al;
+      red[wave][1] = gr;
+      red[wave][2] = gi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+#pragma unroll
+      for (int wv = 0; wv < kBS / 64; ++wv) {
+        s0 += red[wv][0];
+        s1 += red[wv][1];
+        s2 += red[wv][2];
+      }
+      double2* Pd = reinterpret_cast<double2*>(P);
+      Pd[(long)(2 * kJT) * a.nchunks] = make_double2(s0, 0.0);
+      Pd[(long)(2 * kJT + 1) * a.nchunks] = make_double2(s1, s2);
+    }
   }
 }
 
@@ -288,36 +343,68 @@ __global__ void __launch_bounds__(kBS) k_lr_fin(LrArgs a) {
     // column j-1: rows 0..j-1 = h1p + s, row j = nu, then rotations 0..j-2
     for (int i = threadIdx.x; i < j; i += kBS) Rc[i] = make_double2(h1p[i].x + s[i].x, h1p[i].y + s[i].y);
     __syncthreads();
-    __shared__ double2 lcol[kStage + 1];
-    __shared__ double2 lsn[kStage];
-    __shared__ double lcs[kStage];
+    // The previous rotations 0..j-2 act on the pairs (i, i+1) in turn; the
+    // entry carried down the column obeys carry_{i+1} = f_i(carry_i) with
+    // the affine map f_i(a) = -conj(s_i) a + c_i col_{i+1}, and the rotated
+    // row i is c_i carry_i + s_i col_{i+1}.  Affine maps compose
+    // associatively, so the chain is a block-wide scan: each thread composes
+    // the maps of its contiguous segment, the segments' maps are scanned in
+    // LDS (Kogge-Stone), and each thread then replays its segment from its
+    // incoming carry.  O(j / 256 + log 256) steps instead of j.
+    __shared__ double2 scA[2][kBS], scB[2][kBS];
     __shared__ double2 carry;
-    if (threadIdx.x == 0) carry = Rc[0];
-    for (int i0 = 0; i0 < j - 1; i0 += kStage) {
-      const int cnt = min(kStage, j - 1 - i0);
-      for (int i = threadIdx.x; i < cnt; i += kBS) {
-        lcol[i + 1] = Rc[i0 + i + 1];
-        lsn[i] = sn[i0 + i];
-        lcs[i] = cs[i0 + i];
+    const int nrot = j - 1;
+    const int per = (nrot + kBS - 1) / kBS;
+    const int r0 = min(nrot, (int)threadIdx.x * per), r1 = min(nrot, r0 + per);
+    const double2 carry0 = Rc[0];
+    double2 A = make_double2(1.0, 0.0), Bm = make_double2(0.0, 0.0);
+    for (int i = r0; i < r1; ++i) {
+      const double c = cs[i];
+      const double2 sv = sn[i], col = Rc[i + 1];
+      const double2 al = make_double2(-sv.x, sv.y);   // -conj(s_i)
+      A = dmul2(al, A);
+      Bm = dmul2(al, Bm);
+      Bm.x += c * col.x;
+      Bm.y += c * col.y;
+    }
+    // the first entry of the next segment, which its thread overwrites below
+    const double2 bnd = (r1 > r0) ? Rc[r1] : make_double2(0.0, 0.0);
+    int buf = 0;
+    scA[0][threadIdx.x] = A;
+    scB[0][threadIdx.x] = Bm;
+    __syncthreads();
+    for (int d = 1; d < kBS; d <<= 1) {
+      double2 a2 = scA[buf][threadIdx.x], b2 = scB[buf][threadIdx.x];
+      if ((int)threadIdx.x >= d) {
+        // (this segment's prefix) after (the prefix d segments earlier)
+        const double2 ae = scA[buf][threadIdx.x - d], be = scB[buf][threadIdx.x - d];
+        const double2 nb = dmul2(a2, be);
+        b2 = make_double2(nb.x + b2.x, nb.y + b2.y);
+        a2 = dmul2(a2, ae);
       }
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        double2 av = carry;
-        for (int i = 0; i < cnt; ++i) {
-          const double c = lcs[i];
-          const double2 sv = lsn[i], bv = lcol[i + 1];
-          // [a; b] <- [c a + s b; -conj(s) a + c b]
-          const double2 na = make_double2(c * av.x + (sv.x * bv.x - sv.y * bv.y), c * av.y + (sv.x * bv.y + sv.y * bv.x));
-          const double2 nb = make_double2(-(sv.x * av.x + sv.y * av.y) + c * bv.x, -(sv.x * av.y - sv.y * av.x) + c * bv.y);
-          lcol[i] = na;
-          av = nb;
-        }
-        carry = av;
-      }
-      __syncthreads();
-      for (int i = threadIdx.x; i < cnt; i += kBS) Rc[i0 + i] = lcol[i];
+      scA[buf ^ 1][threadIdx.x] = a2;
+      scB[buf ^ 1][threadIdx.x] = b2;
+      buf ^= 1;
       __syncthreads();
     }
+    double2 av = carry0;
+    if (threadIdx.x > 0) {
+      const double2 ae = scA[buf][threadIdx.x - 1], be = scB[buf][threadIdx.x - 1];
+      const double2 t = dmul2(ae, carry0);
+      av = make_double2(t.x + be.x, t.y + be.y);
+    }
+    for (int i = r0; i < r1; ++i) {
+      const double c = cs[i];
+      const double2 sv = sn[i], col = (i + 1 < r1) ? Rc[i + 1] : bnd;
+      const double2 sc2 = dmul2(sv, col);
+      const double2 al = make_double2(-sv.x, sv.y);
+      const double2 nb = dmul2(al, av);
+      Rc[i] = make_double2(c * av.x + sc2.x, c * av.y + sc2.y);
+      av = make_double2(nb.x + c * col.x, nb.y + c * col.y);
+    }
+    if (r1 == nrot && r1 > r0) carry = av;
+    if (nrot == 0 && threadIdx.x == 0) carry = carry0;
+    __syncthreads();
     if (threadIdx.x == 0) {
       const double2 av = carry;
       const double aa = hypot(av.x, av.y);
@@ -657,9 +744,35 @@ static LrArgs lr_args(const LrWs& w, DevState* st, long sst, double* hist, long 
 // Every `poll` steps the host reads the states and stops enqueueing once
 // every system is done.
 int lr_cycle(LrWs& w, hipStream_t st, DevState* dst, DevState* hst, long sst, int B, int m, double* hist,
-             long shist, float2* x, long sx, const LrStep& precond, const LrStep& matvec) {
+             long shist, float2* x, long sx, const LrStep& precond, const LrStep& matvec, LrTiming* tm) {
   const long n = w.n;
   LrArgs a = lr_args(w, dst, sst, hist, shist, m);
+  const bool vec = (n % 2) == 0;   // 16-byte loads: every vector starts 16-byte aligned
+  if (tm && !tm->created) {
+    for (int e = 0; e < 4 * LrTiming::kRing; ++e) CGM_HIP(hipEventCreate(&tm->ev[e]));
+    tm->created = true;
+  }
+  int pending = 0;  // ring slots recorded since the last read-back
+  auto harvest = [&]() -> int {
+    if (!tm || pending == 0) return CGM_SUCCESS;
+    const int last = pending - 1;
+    CGM_HIP(hipEventSynchronize(tm->ev[4 * last + (tm->slot_upd[last] ? 3 : 1)]));
+    for (int sl = 0; sl < pending; ++sl) {
+      float t = 0.f;
+      CGM_HIP(hipEventElapsedTime(&t, tm->ev[4 * sl], tm->ev[4 * sl + 1]));
+      tm->ms[0] += t;
+      tm->bytes[0] += tm->slot_bytes[0][sl];
+      tm->launches[0] += 1;
+      if (tm->slot_upd[sl]) {
+        CGM_HIP(hipEventElapsedTime(&t, tm->ev[4 * sl + 2], tm->ev[4 * sl + 3]));
+        tm->ms[1] += t;
+        tm->bytes[1] += tm->slot_bytes[1][sl];
+        tm->launches[1] += 1;
+      }
+    }
+    pending = 0;
+    return CGM_SUCCESS;
+  };
   hipLaunchKernelGGL(k_lr_init, dim3(1, B), dim3(64), 0, st, a);
   CGM_HIP(hipGetLastError());
   const int poll = 32;
@@ -675,11 +788,32 @@ int lr_cycle(LrWs& w, hipStream_t st, DevState* dst, DevState* hst, long sst, in
       if (rc) return rc;
     }
     const int gx = (a.ngroups > 0 ? a.ngroups : 1) * w.nchunks;
-    hipLaunchKernelGGL(k_lr_proj, dim3(gx, B), dim3(kBS), 0, st, a, with_w);
+    const int sl = pending;
+    if (tm) CGM_HIP(hipEventRecord(tm->ev[4 * sl], st));
+    if (vec)
+      hipLaunchKernelGGL(k_lr_proj<true>, dim3(gx, B), dim3(kBS), 0, st, a, with_w);
+    else
+      hipLaunchKernelGGL(k_lr_proj<false>, dim3(gx, B), dim3(kBS), 0, st, a, with_w);
+    if (tm) CGM_HIP(hipEventRecord(tm->ev[4 * sl + 1], st));
     hipLaunchKernelGGL(k_lr_red, dim3(a.ngroups > 0 ? a.ngroups : 1, B), dim3(kBS), 0, st, a);
     hipLaunchKernelGGL(k_lr_fin, dim3(1, B), dim3(kBS), 0, st, a);
-    if (with_w) hipLaunchKernelGGL(k_lr_upd, dim3(w.nuchunks, B), dim3(kBS), 0, st, a);
+    if (with_w) {
+      if (tm) CGM_HIP(hipEventRecord(tm->ev[4 * sl + 2], st));
+      hipLaunchKernelGGL(k_lr_upd, dim3(w.nuchunks, B), dim3(kBS), 0, st, a);
+      if (tm) CGM_HIP(hipEventRecord(tm->ev[4 * sl + 3], st));
+    }
     CGM_HIP(hipGetLastError());
+    if (tm) {
+      // algorithmic bytes: PROJ reads q_0..q_{j-1}, u (and w); UPD reads
+      // q_0..q_{j-1}, u, w and writes q_j, u_{j+1} (8 B per cell each)
+      tm->slot_bytes[0][sl] = 8.0 * (double)n * (j + (with_w ? 2 : 1)) * B;
+      tm->slot_bytes[1][sl] = with_w ? 8.0 * (double)n * (j + 4) * B : 0.0;
+      tm->slot_upd[sl] = with_w != 0;
+      if (++pending == LrTiming::kRing) {
+        const int rc = harvest();
+        if (rc) return rc;
+      }
+    }
     if (j >= 2 * poll && j % poll == 0 && j < m) {
       CGM_HIP(hipMemcpyAsync(hst, dst, sizeof(DevState) * (sst ? B : 1), hipMemcpyDeviceToHost, st));
       CGM_HIP(hipStreamSynchronize(st));
@@ -688,6 +822,7 @@ int lr_cycle(LrWs& w, hipStream_t st, DevState* dst, DevState* hst, long sst, in
       if (all) break;
     }
   }
+  if (const int rc = harvest()) return rc;
   hipLaunchKernelGGL(k_lr_backsub, dim3(1, B), dim3(1024), 0, st, a);
   hipLaunchKernelGGL(k_lr_xupd, dim3(w.nuchunks, B), dim3(kBS), 0, st, a, x, sx);
   CGM_HIP(hipGetLastError());
@@ -744,7 +879,13 @@ extern "C" int cgm_profile_longrec(cgm_ctx_t c, int j, int reps, float* ms) {
     (void)hipEventElapsedTime(&t, e0, e1);
     return t / reps;
   };
-  ms[0] = time_it([&] { hipLaunchKernelGGL(k_lr_proj, dim3(a.ngroups * w.nchunks, 1), dim3(kBS), 0, c->stream, a, 1); });
+  const bool vec = (n % 2) == 0;
+  ms[0] = time_it([&] {
+    if (vec)
+      hipLaunchKernelGGL(k_lr_proj<true>, dim3(a.ngroups * w.nchunks, 1), dim3(kBS), 0, c->stream, a, 1);
+    else
+      hipLaunchKernelGGL(k_lr_proj<false>, dim3(a.ngroups * w.nchunks, 1), dim3(kBS), 0, c->stream, a, 1);
+  });
   ms[1] = time_it([&] { hipLaunchKernelGGL(k_lr_red, dim3(a.ngroups, 1), dim3(kBS), 0, c->stream, a); });
   ms[3] = time_it([&] { hipLaunchKernelGGL(k_lr_upd, dim3(w.nuchunks, 1), dim3(kBS), 0, c->stream, a); });
   ms[2] = time_it([&] {
@@ -756,5 +897,28 @@ extern "C" int cgm_profile_longrec(cgm_ctx_t c, int j, int reps, float* ms) {
   (void)hipEventDestroy(e1);
   CGM_HIP(err);
   CGM_HIP(hipStreamSynchronize(c->stream));
+  return CGM_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------
+// live timing of the projection and update kernels inside solves
+// ---------------------------------------------------------------------------
+extern "C" int cgm_longrec_timing(cgm_ctx_t c, int enable) {
+  CGM_CHECK(c, CGM_ERROR_INVALID_VALUE, "null context");
+  cgm::LrTiming& t = c->lrt;
+  t.on = enable != 0;
+  t.ms[0] = t.ms[1] = t.bytes[0] = t.bytes[1] = 0.0;
+  t.launches[0] = t.launches[1] = 0;
+  return CGM_SUCCESS;
+}
+
+extern "C" int cgm_longrec_timing_read(cgm_ctx_t c, double* out) {
+  CGM_CHECK(c && out, CGM_ERROR_INVALID_VALUE, "null argument");
+  const cgm::LrTiming& t = c->lrt;
+  for (int k = 0; k < 2; ++k) {
+    out[3 * k] = t.ms[k];
+    out[3 * k + 1] = t.bytes[k];
+    out[3 * k + 2] = (double)t.launches[k];
+  }
   return CGM_SUCCESS;
 }

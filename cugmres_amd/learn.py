@@ -138,15 +138,17 @@ class Adam:
 
 
 def train(ctx, samples=20000, batch=8, lr=1e-3, seed=0, params=None, log=None, log_every=100,
-          clip=1.0, identity_weight=0.0, chain=3, tiles_T=0):
+          clip=1.0, identity_weight=0.0, chain=3, tiles_T=0, map_pool=8):
     """Trains a Preconditioner on `samples` seeded (map, vector) pairs at the
     context's N, starting from `params` or from init_params(seed) with the
     identity path weighted by identity_weight.  Step k trains on vectors
     taken k % (chain + 1) steps down the residual chain.  With tiles_T > 0
-    the network is the second level of tiles.TwoLevel: every step factors the
-    T x T tile inverses of its maps and trains on the normalised residuals
-    r = v - A T v they leave.  Returns (Preconditioner, history of (step,
-    loss))."""
+    the network is the second level of tiles.TwoLevel and trains on the
+    normalised residuals r = v - A T v that the T x T tile inverses of its
+    maps leave; the maps then come from a pool of map_pool batches whose
+    inverses are factored once (step k uses batch k % map_pool; 0 = new maps
+    factored at every step), while the vectors stay new at every step.
+    Returns (Preconditioner, history of (step, loss))."""
     t = ctx.torch
     gamma = ctx.spectrum_helmholtz(problems.KH)
     if params is None:
@@ -159,18 +161,27 @@ def train(ctx, samples=20000, batch=8, lr=1e-3, seed=0, params=None, log=None, l
     opt = Adam(p, lr)
     sampler = Sampler(ctx, gamma)
     tl = None
+    pool = []
     if tiles_T:
         from .tiles import Tiles, tile_residual
         blk = ctx.block_helmholtz(problems.KH)
-        tl = Tiles(ctx, tiles_T, batch)
+        for q in range(map_pool):
+            seeds = [seed * 1_000_003 + q * batch + b for b in range(batch)]
+            chi_q = t.from_numpy(sample_maps(ctx.N, seeds).reshape(batch, -1)).to(ctx.device)
+            pool.append((chi_q, Tiles(ctx, tiles_T, batch).factor(blk, chi_q)))
+        if not pool:
+            tl = Tiles(ctx, tiles_T, batch)
     steps = max(1, samples // batch)
     hist = []
     t0 = time.time()
     for k in range(steps):
         seeds = [seed * 1_000_003 + k * batch + b for b in range(batch)]
-        chi_d = t.from_numpy(sample_maps(ctx.N, seeds).reshape(batch, -1)).to(ctx.device)
+        if pool:
+            chi_d, tl = pool[k % len(pool)]
+        else:
+            chi_d = t.from_numpy(sample_maps(ctx.N, seeds).reshape(batch, -1)).to(ctx.device)
         v = sampler.batch(chi_d, seeds[0])
-        if tl is not None:
+        if tiles_T and not pool:
             tl.factor(blk, chi_d)
         if chain:
             v = residual_chain(ctx, gamma, net, chi_d, v, k % (chain + 1), tl)
@@ -190,7 +201,9 @@ def train(ctx, samples=20000, batch=8, lr=1e-3, seed=0, params=None, log=None, l
         hist.append((k, L))
         if log and (k % log_every == 0 or k == steps - 1):
             log(f"step {k:6d}/{steps} loss {L:.4e} |g| {gn:.3e} lr {cur:.2e} ({time.time() - t0:.1f} s)")
-    if tl is not None:
+    for _, tq in pool:
+        tq.close()
+    if tl is not None and not pool:
         tl.close()
     return net, hist
 
@@ -208,12 +221,15 @@ def main():
     ap.add_argument("--chain", type=int, default=3)
     ap.add_argument("--tiles", type=int, default=0,
                     help="train the network as the second level after T x T tile inverses (0: alone)")
+    ap.add_argument("--map-pool", type=int, default=8,
+                    help="with --tiles: batches of maps whose tile inverses are factored once and reused")
     ap.add_argument("--out", default="fno_params.npz")
     a = ap.parse_args()
     from .solver import Context
     ctx = Context(a.N)
     net, hist = train(ctx, a.samples, a.batch, a.lr, a.seed, log=lambda s: print(s, flush=True),
-                      identity_weight=a.identity_weight, chain=a.chain, tiles_T=a.tiles)
+                      identity_weight=a.identity_weight, chain=a.chain, tiles_T=a.tiles,
+                      map_pool=a.map_pool)
     fno.save(a.out, net.params.cpu().numpy(), N=a.N, samples=a.samples, seed=a.seed,
              loss=np.array([h[1] for h in hist], np.float32))
     print(f"saved {a.out}")

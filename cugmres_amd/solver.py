@@ -71,6 +71,18 @@ class Context:
         self._flex_g64 = gamma64
         check(lib().cgm_set_flexible_fp64(self._ptr, _ptr(gamma64)), "cgm_set_flexible_fp64")
 
+    def longrec_timing(self, enable=True):
+        """Live HIP-event timing of the long-recurrence projection and update
+        kernels in the solves that follow (sums zeroed)."""
+        check(lib().cgm_longrec_timing(self._ptr, int(bool(enable))), "cgm_longrec_timing")
+
+    def longrec_timing_read(self):
+        """{'proj'|'upd': {'ms', 'bytes', 'launches'}} summed since longrec_timing()."""
+        out = (_c.c_double * 6)()
+        check(lib().cgm_longrec_timing_read(self._ptr, out), "cgm_longrec_timing_read")
+        return {k: {"ms": out[3 * i], "bytes": out[3 * i + 1], "launches": int(out[3 * i + 2])}
+                for i, k in enumerate(("proj", "upd"))}
+
     def set_arnoldi_path(self, path):
         """0: fused lagged-CGS2 steps (default), 1: separate two-pass CGS2 kernels."""
         check(lib().cgm_set_arnoldi_path(self._ptr, int(path)), "cgm_set_arnoldi_path")
@@ -220,7 +232,7 @@ class Context:
         `precond`: the learned network (cugmres_amd.fno.Preconditioner), the
         tile inverses (cugmres_amd.tiles.Tiles), both (cugmres_amd.tiles.TwoLevel)
         or None (the identity).  A restart above 64 runs the long-recurrence
-        path (the whole basis in device memory, up to 4096 steps per cycle).  Plain GMRES takes over when a preconditioned cycle leaves
+        path (the whole basis in device memory, up to 16384 steps per cycle).  Plain GMRES takes over when a preconditioned cycle leaves
         more than stall_ratio of the residual.  Returns (x, info dict)."""
         t = self.torch
         x0_zero = x is None

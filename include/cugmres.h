@@ -265,7 +265,8 @@ int cgm_fno_forward(cgm_fno_t net, const float* dev_chi, const cgm_complex64* de
 int cgm_fno_backward(cgm_fno_t net, const cgm_complex64* dev_dz, float* dev_grad);
 
 typedef struct {
-  int restart;        /* Krylov dimension per cycle m (1..512)                     */
+  int restart;        /* Krylov dimension per cycle m (1..16384; above 64 the
+                         long-recurrence path keeps the whole basis in HBM)      */
   int max_cycles;     /* restart cap                                              */
   double tol;         /* stop when the true ||b - A x|| <= tol ||b||              */
   int x0_zero;        /* 1: start from x = 0                                      */
@@ -374,7 +375,7 @@ int cgm_fgmres_pc_batched(cgm_ctx_t ctx, const cgm_complex64* dev_gamma, const f
                           const cgm_fgmres_params* params, cgm_fgmres_result* results, double* dev_hist);
 
 /* cgm_gmres_refine with the correction solved by flexible GMRES with the
- * preconditioner pc (inner_restart up to 4096; above 64 the whole basis stays
+ * preconditioner pc (inner_restart up to 16384; above 64 the whole basis stays
  * in device memory).  The tiles of pc must have been factored for the map
  * chi rounded to float.  No counterpart in the reference. */
 int cgm_gmres_refine_pc(cgm_ctx_t ctx, const cgm_complex64* dev_gamma,
@@ -471,6 +472,15 @@ int cgm_fno_profile(cgm_fno_t net, const float* dev_chi, const cgm_complex64* de
  * memset per launch), [3] basis update forming q_j and u_{j+1} (ms holds 4
  * entries).  Overwrites the context's long-recurrence workspace. */
 int cgm_profile_longrec(cgm_ctx_t ctx, int j, int reps, float* ms);
+/* Live timing of the long-recurrence projection and update kernels inside
+ * the solves that follow (HIP events around each launch on the context's
+ * stream, read back every 32 steps; about one host synchronisation per 32
+ * steps while on).  enable = 0 turns it off; either call zeroes the sums.
+ * out[0..2] = projection: summed milliseconds, algorithmic bytes (8 B per
+ * cell for each basis vector read and each vector read or written),
+ * launches; out[3..5] = the same for the update. */
+int cgm_longrec_timing(cgm_ctx_t ctx, int enable);
+int cgm_longrec_timing_read(cgm_ctx_t ctx, double* out);
 
 /* ---- memory and host helpers ---------------------------------------------- */
 int cgm_malloc(void** ptr, size_t bytes);

@@ -24,7 +24,7 @@ def bench():
 def test_defaults_are_the_benchmark_configuration(bench):
     a = bench.parse([])
     assert (a.gpus, a.N, a.restart, a.tol, a.max_cycles) == (1, 512, 30, 1e-6, 100)
-    assert (a.tile, a.lr_restart, a.train_N) == (16, 4096, 256)
+    assert (a.tile, a.lr_restart, a.train_N) == (32, 8192, 256)
     assert a.seed_offset == 0 and a.mode == "bench"
 
 
@@ -40,8 +40,11 @@ def test_line_is_compact_at_320_systems(bench):
     learned = bench.summarize_method(_infos(320, lambda i: i % 5 != 0), 800.0, "learned")
     tiles = bench.summarize_method(_infos(320, lambda i: i % 7 != 0), 900.0, "tiles")
     plain = bench.summarize_method(_infos(320, lambda i: False), 50.0, "plain")
-    roof = {"bound": "hbm", "achieved": 4000.0, "peak": 8000.0, "unit": "GB/s", "frac": 0.5, "traffic": None,
-            "kernel": "k_lr_upd", "per_unit": "x" * 80, "us": 900.0, "us_per_step": {"lr_proj": 1.0}}
+    live = {"proj": {"ms": 9000.0, "bytes": 5.2e13, "launches": 33000},
+            "upd": {"ms": 9500.0, "bytes": 5.3e13, "launches": 32990}}
+    roof = bench.live_roofline(live, 512, {"lr_proj": 1500.0, "lr_update": 1600.0, "tile_apply": 360.0,
+                                            "matvec_x2": 41.0, "fno_forward": 103.0, "lr_reduce_givens": 30.0},
+                               4300)
     cpu = {"value": 0.004, "unit": "capped solves/s", "cores": 1, "kind": "port", "sample": "y" * 120,
            "all_cores": {"cores": 16, "value": 0.05}, "cpu_model": "z" * 60, "nproc": 16,
            "C0_N64_c128_m200": {"seeds": 10, "s_per_solve": [1, 2, 3], "rel_residual": [1, 2, 3]}}
@@ -62,3 +65,20 @@ def test_line_is_compact_at_320_systems(bench):
 
 def test_traffic_lookup_of_unmeasured_kernels_is_null(bench):
     assert bench.measured_traffic("no_such_kernel", 512, 1) is None
+
+
+def test_live_roofline_takes_the_longer_kernel_and_its_own_bytes(bench):
+    """The dominant kernel is the one with the larger summed duration over the
+    timed solves; achieved = its summed algorithmic bytes / summed time, and
+    the mean basis size follows from its bytes per launch."""
+    n = 512 * 512
+    live = {"proj": {"ms": 1000.0, "bytes": 8.0 * n * (3000 + 2) * 400, "launches": 400},
+            "upd": {"ms": 1200.0, "bytes": 8.0 * n * (3000 + 4) * 400, "launches": 400}}
+    r = bench.live_roofline(live, 512, {"lr_proj": 1.0}, 3000)
+    assert r["kernel"] == "k_lr_upd"
+    assert r["achieved"] == pytest.approx(8.0 * n * 3004 * 400 / 1.2 / 1e9)
+    assert r["frac"] == pytest.approx(r["achieved"] / 8000.0)
+    assert r["us"] == pytest.approx(1200.0 * 1e3 / 400)
+    assert "mean j 3000" in r["per_unit"]
+    assert r["other"]["k_lr_proj"]["launches"] == 400
+

@@ -166,5 +166,5 @@ def test_restart_beyond_the_limit_is_rejected():
     g = ctx.spectrum_helmholtz(problems.KH)
     chi, b = _problem(N, 2)
     with pytest.raises(_lib.CgmError):
-        ctx.fgmres(g, ctx.to_device(chi, "f32"), ctx.to_device(b), restart=4097, max_cycles=1, stall_ratio=0.0)
+        ctx.fgmres(g, ctx.to_device(chi, "f32"), ctx.to_device(b), restart=16385, max_cycles=1, stall_ratio=0.0)
     ctx.close()
