@@ -60,7 +60,7 @@ def parse(argv=None):
     p.add_argument("--restart", type=int, default=30, help="restart of plain GMRES")
     p.add_argument("--tol", type=float, default=1e-6)
     p.add_argument("--max-cycles", type=int, default=100, help="cap of plain GMRES(restart)")
-    p.add_argument("--tile", type=int, default=32, help="tile side of the tile inverses (8, 16, 32)")
+    p.add_argument("--tile", type=int, default=64, help="tile side of the tile inverses (8, 16, 32, 64)")
     p.add_argument("--lr-restart", type=int, default=8192,
                    help="cycle length of the preconditioned solves (long-recurrence path)")
     p.add_argument("--lr-cycles", type=int, default=2)
@@ -93,6 +93,9 @@ def parse(argv=None):
                         "cgm_gmres_batched) on --batch systems")
     p.add_argument("--sweep-reps", type=int, default=20)
     p.add_argument("--lr-j", type=int, default=4096, help="lr-step mode: basis size of the profiled kernels")
+    p.add_argument("--c2-seconds", type=float, default=600.0, help="c2 mode: no new outer step after this time")
+    p.add_argument("--tiles-synthetic", action="store_true",
+                   help="tiles-step mode: time the application on synthetic inverses (no factorisation)")
     return p.parse_args(argv)
 
 
@@ -166,6 +169,14 @@ def main():
         return
     if a.mode == "tiles-step":
         from cugmres_amd import tiles
+        if a.tiles_synthetic:
+            # the application alone, on synthetic inverses (no rocSOLVER call)
+            from cugmres_amd._lib import lib
+            tl = tiles.Tiles(ctx, a.tile, 1)
+            ms = ctypes.c_float()
+            lib().cgm_profile_tiles(tl.handle, 20, ctypes.byref(ms))
+            print(json.dumps({"tile": a.tile, "us": ms.value * 1e3}), flush=True)
+            return
         chi, b, _ = problems.config_plain(N, seed=0)
         tl = tiles.Tiles(ctx, a.tile, 1).factor(ctx.block_helmholtz(problems.KH), ctx.to_device(chi.reshape(-1), "f32"))
         print(json.dumps(profile_tiles(ctx, tl, ctx.to_device(b.reshape(-1)))), flush=True)
@@ -252,8 +263,10 @@ def run_bench(a, ctx):
         tctx = solver.Context(a.train_N, 0)
         t0 = time.perf_counter()
         note(f"training the network: {a.train_samples} samples at N = {a.train_N}")
+        # pooled maps: 8 batches of 8 (64 maps) up to 32 x 32 tiles, 2 batches with
+        # 64 x 64 tiles (13 GB of inverses per batch at N = 256)
         tnet, hist = learn.train(tctx, samples=a.train_samples, batch=8, lr=1e-3, seed=0, tiles_T=a.tile,
-                                 log=note, log_every=200)
+                                 map_pool=8 if a.tile <= 32 else 2, log=note, log_every=200)
         torch.cuda.synchronize()
         note(f"trained in {time.perf_counter() - t0:.1f} s")
         params = tnet.params.cpu().numpy()
@@ -320,7 +333,7 @@ def run_bench(a, ctx):
     per_step = {"lr_proj": lr["lr_proj"]["us"], "lr_update": lr["lr_update"]["us"],
                 "lr_reduce_givens": lr["lr_reduce"]["us"] + lr["lr_givens"]["us"],
                 "tile_apply": tp["us"], "matvec_x2": 2 * mv["us"], "fno_forward": fp["us"]}
-    roof = live_roofline(live, N, per_step, jmean)
+    roof = live_roofline(live, N, per_step, jmean, a.tile)
     matvec = {"us": mv["us"], "bytes": mv["bytes"], "GBps": mv["GBps"], "frac": mv["GBps"] / HBM_PEAK_GBS,
               "per_unit": "100 N^2 B per matvec (quarter spectrum)"}
 
@@ -421,14 +434,16 @@ def fft_reference(a, B=16, reps=20):
     return out
 
 
-def c3_run(a, rhs=16, mem_budget=160e9):
+def c3_run(a, rhs=16, mem_budget=150e9):
     """Batched workload: per instance one map and `rhs` plane waves at angles
     2 pi k / rhs, solved in lockstep by flexible GMRES with the tile inverses
     of that map (factored once, each tile's inverse streamed once per four
-    right-hand sides) on the long-recurrence path; the cycle length is capped
-    so the 16 bases fit mem_budget bytes.  Also plain GMRES(30) on the first
-    instance of each size (capped at 100 cycles).  Reports RHS-solves/s
-    counting only right-hand sides that reached the tolerance."""
+    right-hand sides) on the long-recurrence path with cycles of --lr-restart
+    steps (two at most).  The right-hand sides run in lockstep groups as large
+    as mem_budget bytes of Krylov bases allow (all 16 up to N = 256, fewer
+    above).  Also plain GMRES(30) on the first instance of each size (capped
+    at 100 cycles).  Reports RHS-solves/s counting only right-hand sides that
+    reached the tolerance."""
     import numpy as np
     import torch
     from cugmres_amd import problems, solver, tiles
@@ -437,8 +452,13 @@ def c3_run(a, rhs=16, mem_budget=160e9):
     t_all = 0.0
     conv_all = 0
     total = 0
-    for (N, blk_cells, seed) in c3_instances(a.instances):
+    # grouped by N, one context at a time: each context keeps its Krylov
+    # workspace (up to mem_budget bytes) until it is closed
+    for (N, blk_cells, seed) in sorted(c3_instances(a.instances), key=lambda t: t[0]):
         if N not in ctxs:
+            for c_old, _, _, _ in ctxs.values():
+                c_old.close()
+            ctxs.clear()
             c = solver.Context(N, 0)
             g32, g64 = c.spectrum_helmholtz(problems.KH, fp64=True)
             c.set_flexible_fp64(g64)
@@ -448,17 +468,26 @@ def c3_run(a, rhs=16, mem_budget=160e9):
         chi = ctx.to_device((eps - 1.0).reshape(-1), "f32")
         b = torch.stack([ctx.to_device(problems.plane_wave(N, problems.KH, 2 * math.pi * k / rhs).reshape(-1))
                          for k in range(rhs)])
-        m = int(min(4096, mem_budget / (rhs * 2 * 8 * N * N)))
+        m = a.lr_restart
+        per_rhs = 2 * 8 * N * N * (m + 1) + 16 * m * m   # basis, preconditioned vectors, R
+        grp = int(max(1, min(rhs, mem_budget // per_rhs)))
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         tl = tiles.Tiles(ctx, a.tile if N % a.tile == 0 else 16, 1).factor(gb, chi)
-        _, infos = ctx.fgmres_batched(g, chi, b, precond=tl, restart=m, tol=a.tol, max_cycles=1)
+        infos = []
+        for g0 in range(0, rhs, grp):
+            bg = b[g0:g0 + grp].contiguous()
+            _, ig = ctx.fgmres_batched(g, chi, bg, precond=tl, restart=m, tol=a.tol, max_cycles=2)
+            infos += ig
+            if N >= 384:
+                note(f"c3 N={N}: right-hand sides {g0}-{g0 + len(ig) - 1} done, "
+                     f"{sum(1 for i in ig if i['converged'])} converged, {time.perf_counter() - t0:.1f} s")
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         tl.close()
         conv = sum(1 for i in infos if i["converged"])
         e = per_n.setdefault(N, {"instances": 0, "rhs": 0, "converged": 0, "seconds": 0.0, "restart": m,
-                                 "iterations": [], "residuals": []})
+                                 "lockstep_group": grp, "iterations": [], "residuals": []})
         e["instances"] += 1
         e["rhs"] += rhs
         e["converged"] += conv
@@ -475,8 +504,8 @@ def c3_run(a, rhs=16, mem_budget=160e9):
         t_all += dt
         conv_all += conv
         total += rhs
-        print(f"c3 N={N} block={blk_cells} converged {conv}/{rhs} in {dt:.2f} s "
-              f"(iterations {_stats([i['iterations'] for i in infos])})", flush=True)
+        note(f"c3 N={N} block={blk_cells} converged {conv}/{rhs} in {dt:.2f} s (groups of {grp}; "
+             f"iterations {_stats([i['iterations'] for i in infos])})")
     for e in per_n.values():
         e["iterations"] = _stats(e["iterations"])
         e["residuals"] = _stats(e["residuals"])
@@ -512,8 +541,22 @@ def c2_run(a):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         tl = tiles.Tiles(ctx, a.tile, 1).factor(blk, chi32)
-        x, info = ctx.gmres_refine_pc(g32, g64, chi64, b64, tl, inner_restart=a.lr_restart, inner_tol=1e-4,
-                                      inner_max_cycles=1, outer_tol=1e-10, max_outer=12)
+        # one correction per call, so that each outer step reports itself;
+        # no new step starts once --c2-seconds have passed
+        x = torch.zeros_like(b64)
+        info = {"outer": 0, "inner_iterations": 0, "converged": False, "rel_residual": 1.0, "steps": []}
+        for _ in range(12):
+            x, st = ctx.gmres_refine_pc(g32, g64, chi64, b64, tl, x=x, inner_restart=a.lr_restart, inner_tol=1e-4,
+                                        inner_max_cycles=1, outer_tol=1e-10, max_outer=1)
+            info["outer"] += st["outer"]
+            info["inner_iterations"] += st["inner_iterations"]
+            info["converged"] = st["converged"]
+            info["rel_residual"] = st["rel_residual"]
+            info["steps"].append([st["inner_iterations"], st["rel_residual"]])
+            note(f"c2 N={N} seed {a.seed_offset + s}: outer step {info['outer']}, inner {st['inner_iterations']} "
+                 f"steps, FP64 residual {st['rel_residual']:.2e}, {time.perf_counter() - t0:.1f} s")
+            if st["converged"] or st["outer"] == 0 or time.perf_counter() - t0 > a.c2_seconds:
+                break
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         # FP64 true residual of the result, recomputed independently
@@ -521,7 +564,7 @@ def c2_run(a):
         rel_fp64 = float(torch.linalg.vector_norm(r) / torch.linalg.vector_norm(b64))
         tl.close()
         out["solves"].append({"seed": a.seed_offset + s, "seconds": dt, "fp64_rel_residual": rel_fp64, **info})
-        print(f"c2 N={N} seed {a.seed_offset + s}: {info} fp64 residual {rel_fp64:.2e} {dt:.2f} s", flush=True)
+        note(f"c2 N={N} seed {a.seed_offset + s}: {info} fp64 residual {rel_fp64:.2e} {dt:.2f} s")
     ctx.close()
     return out
 
@@ -622,37 +665,47 @@ def batched_throughput(restart, cycles=3, B=16):
     return out
 
 
-LIVE_KERNELS = {"proj": "k_lr_proj", "upd": "k_lr_upd"}
+LIVE_KERNELS = {"proj": "k_lr_proj", "upd": "k_lr_upd", "tile": "k_tile_apply"}
 
 
-def live_roofline(live, N, per_step, jmean):
+def live_roofline(live, N, per_step, jmean, tile=64):
     """Roofline of the dominant kernel of the timed learned solves, from the
     live HIP-event sums (cgm_longrec_timing over the timed region, on the
     solver's stream): achieved = algorithmic bytes summed over every launch /
     summed launch durations.  traffic = the HBM bytes per launch measured by
-    rocprofv3 at a reference basis size (profiles/hbm_traffic_n512.json),
-    scaled to this run's mean algorithmic bytes per launch."""
-    key = max(LIVE_KERNELS, key=lambda k: live[k]["ms"])
+    rocprofv3 (profiles/hbm_traffic_n512.json: the basis kernels at basis
+    size 4096, scaled to this run's mean algorithmic bytes per launch; the
+    tile application at this tile size)."""
+    key = max((k for k in LIVE_KERNELS if k in live), key=lambda k: live[k]["ms"])
     t = live[key]
-    kern = LIVE_KERNELS[key]
     n = N * N
     ach = t["bytes"] / (t["ms"] * 1e-3) / 1e9 if t["ms"] > 0 else 0.0
     per_launch = t["bytes"] / max(1, t["launches"])
-    extra = 2 if key == "proj" else 4
-    jbar = per_launch / (8 * n) - extra
     traffic = None
-    ref = measured_traffic(kern + "_j4096", N)
-    if ref:
-        traffic = ref * per_launch / (8 * n * (4096 + extra))
+    if key == "tile":
+        kern = "k_tile_apply_h" if tile == 64 else "k_tile_apply"
+        ref = measured_traffic(f"{kern}_T{tile}", N)
+        traffic = ref
+        ent = 4 if tile == 64 else 8
+        per_unit = (f"(N/T)^2 T^4 x {ent} B of inverses + 16 B x N^2 per launch (T = {tile}, N = {N}): "
+                    f"{per_launch:.4g} B, {t['launches']} launches of the timed solves")
+    else:
+        kern = LIVE_KERNELS[key]
+        extra = 2 if key == "proj" else 4
+        jbar = per_launch / (8 * n) - extra
+        ref = measured_traffic(kern + "_j4096", N)
+        if ref:
+            traffic = ref * per_launch / (8 * n * (4096 + extra))
+        per_unit = (f"8 B x N^2 x (j + {extra}) per launch at basis size j (N = {N}); mean j {jbar:.0f} "
+                    f"over the {t['launches']} launches of the timed solves")
     return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
             "traffic": traffic, "kernel": kern, "launches": t["launches"],
             "us": t["ms"] * 1e3 / max(1, t["launches"]),
-            "per_unit": f"8 B x N^2 x (j + {extra}) per launch at basis size j (N = {N}); mean j {jbar:.0f} "
-                        f"over the {t['launches']} launches of the timed solves",
-            "other": {LIVE_KERNELS[k]: {"GBps": live[k]["bytes"] / (live[k]["ms"] * 1e-3) / 1e9
+            "per_unit": per_unit,
+            "other": {LIVE_KERNELS[k]: {"GBps": round(live[k]["bytes"] / (live[k]["ms"] * 1e-3) / 1e9)
                                         if live[k]["ms"] > 0 else None,
-                                        "ms_total": live[k]["ms"], "launches": live[k]["launches"]}
-                      for k in LIVE_KERNELS if k != key},
+                                        "ms_total": round(live[k]["ms"], 1), "launches": live[k]["launches"]}
+                      for k in LIVE_KERNELS if k != key and k in live},
             "us_per_step_at_mean_basis": {"j": jmean, **{k: round(v, 1) for k, v in per_step.items()}}}
 
 

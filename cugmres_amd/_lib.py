@@ -152,6 +152,7 @@ SIGNATURES = {
     "cgm_green_values": (ctypes.c_int, [c_void_p, ctypes.c_size_t, ctypes.c_double, c_void_p]),
     "cgm_longrec_timing": (ctypes.c_int, [c_void_p, ctypes.c_int]),
     "cgm_longrec_timing_read": (ctypes.c_int, [c_void_p, c_void_p]),
+    "cgm_profile_tiles": (ctypes.c_int, [c_void_p, ctypes.c_int, c_float_p]),
     "cgm_profile_longrec": (ctypes.c_int, [c_void_p, ctypes.c_int, ctypes.c_int, c_float_p]),
     "cgm_profile_step": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, ctypes.c_int,
                                         c_float_p]),

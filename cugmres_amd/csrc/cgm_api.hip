@@ -1189,7 +1189,7 @@ static int ensure_flex64(cgm_context* c, int B, int maps) {
 // is widened into z64, the FP64 passes write w64, which is rounded into y
 // (stride sy).  Exact for the complex64 input up to FP64 rounding.
 static int matvec64_from32(cgm_context* c, const float2* x, long sx, const float2* b, long sb, float2* y, long sy,
-                           bool resid, long chi_stride, int B, const int* done, long sst) {
+                           bool resid, long chi_stride, int B, const int* done, long sst, bool keep64 = false) {
   const long n = (long)c->N * c->N;
   const dim3 grid((unsigned)((n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024), B);
   hipLaunchKernelGGL(k_c64_to_c128, grid, dim3(256), 0, c->stream, x, sx, c->z64, n, n);
@@ -1209,6 +1209,8 @@ static int matvec64_from32(cgm_context* c, const float2* x, long sx, const float
   if (rc) return rc;
   if (resid)
     hipLaunchKernelGGL(k_resid_mixed, grid, dim3(256), 0, c->stream, b, sb, c->w64, n, y, sy, n);
+  else if (keep64)
+    return CGM_SUCCESS;  // the caller reads w64
   else
     hipLaunchKernelGGL(k_c128_to_c64, grid, dim3(256), 0, c->stream, c->w64, n, y, sy, n);
   CGM_HIP(hipGetLastError());
@@ -1218,7 +1220,9 @@ static int matvec64_from32(cgm_context* c, const float2* x, long sx, const float
 static int lr_matvec_step(void* vp, int, const float2* z, long sz, float2* w, long sw) {
   const LrCall& L = *static_cast<const LrCall*>(vp);
   cgm_context* c = L.c;
-  if (c->flex_g64) return matvec64_from32(c, z, sz, nullptr, 0, w, sw, false, L.chi_stride, L.B, &L.dst->done, L.sst);
+  // FP64 products stay in w64, which the step kernels read (c->lr.W64)
+  if (c->flex_g64)
+    return matvec64_from32(c, z, sz, nullptr, 0, w, sw, false, L.chi_stride, L.B, &L.dst->done, L.sst, true);
   BatchStrides bs;
   bs.x = sz;
   bs.chi = L.chi_stride;
@@ -1282,6 +1286,7 @@ static int lr_solve(cgm_context* c, const float2* gq, const float* chi, long chi
     hipLaunchKernelGGL(k_f32_to_f64, grid, dim3(256), 0, c->stream, chi, chi_stride, c->chi64, n, n);
     CGM_HIP(hipGetLastError());
   }
+  c->lr.W64 = c->flex_g64 ? c->w64 : nullptr;
   const long hstride = (long)m * (p->max_cycles > 0 ? p->max_cycles : 1) + 1;
   CGM_HIP(hipMemsetAsync(dst, 0, sizeof(DevState) * (batched ? B : 1), c->stream));
   if (hist) CGM_HIP(hipMemsetAsync(hist, 0, sizeof(double) * hstride * B, c->stream));

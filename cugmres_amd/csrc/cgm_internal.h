@@ -188,18 +188,26 @@ struct LrWs {
   double* scal = nullptr;
   double2* R = nullptr;    // [B][m][m] column-major
   double* partn = nullptr;
+  // FP64 products w = A z_j of the current step (the context's buffer, n per
+  // system) when the products are FP64; null: w is the complex64 W.
+  const double2* W64 = nullptr;
 };
-// Live timing of the two streaming step kernels (projection, update) over
-// a solve: HIP events around each launch on the solver's stream, read back
-// every kRing steps; summed durations, algorithmic bytes and launch counts.
+// Live timing of the streaming step kernels over a solve (0 projection,
+// 1 update, 2 tile-inverse application): HIP events around each launch on
+// the solver's stream, read back every kRing steps; summed durations,
+// algorithmic bytes and launch counts.  `cur` is the ring slot of the step
+// whose preconditioner is being enqueued (-1 outside a long cycle); the tile
+// application records into it.
 struct LrTiming {
   static constexpr int kRing = 32;
+  static constexpr int kKernels = 3;
   bool on = false, created = false;
-  hipEvent_t ev[4 * kRing];
-  double slot_bytes[2][kRing];
-  bool slot_upd[kRing];
-  double ms[2] = {0.0, 0.0}, bytes[2] = {0.0, 0.0};
-  long launches[2] = {0, 0};
+  int cur = -1;
+  hipEvent_t ev[2 * kKernels * kRing];
+  double slot_bytes[kKernels][kRing];
+  bool slot_has[kKernels][kRing];
+  double ms[kKernels] = {0.0, 0.0, 0.0}, bytes[kKernels] = {0.0, 0.0, 0.0};
+  long launches[kKernels] = {0, 0, 0};
 };
 int lr_alloc(LrWs& w, long n, int m, int B);
 void lr_free(LrWs& w);

@@ -59,6 +59,7 @@ struct LrArgs {
   long sU;
   const float2* W;   // w = A z_j (stride sW)
   long sW;
+  const double2* W64;  // the same in FP64 (stride n) when the products are FP64, else null
   double2* part;     // [B][ngroups_max][kNV][nchunks]
   long sPart;
   double2* s;        // [B][ld+1]
@@ -135,14 +136,27 @@ __global__ void __launch_bounds__(kBS) k_lr_proj(LrArgs a, int with_w) {
   const float2* u = a.U + (long)y * a.sU;
   const float2* w = a.W + (long)y * a.sW;
   const float2* Q = a.Q + (long)y * (a.ld + 1) * a.n;
-  float2 ue[kPE], we[kPE];
+  // w in FP64: the FP64 product when there is one (rounding A z_j to
+  // complex64 would put an error of the size of ||A z_j|| times the
+  // complex64 unit roundoff into every column of the Arnoldi relation), else
+  // the complex64 w widened
+  float2 ue[kPE];
+  double2 we[kPE];
+  const double2* w64 = a.W64 ? a.W64 + (long)y * a.n : nullptr;
 #pragma unroll
   for (int k = 0; k < kPP; ++k) {
     load_pair<kVec>(u, pbase + (long)k * kBS, a.n, ue[2 * k], ue[2 * k + 1]);
-    if (with_w) {
-      load_pair<kVec>(w, pbase + (long)k * kBS, a.n, we[2 * k], we[2 * k + 1]);
+    const long e = 2 * (pbase + (long)k * kBS);
+    if (with_w && w64) {
+      we[2 * k] = e < a.n ? w64[e] : make_double2(0.0, 0.0);
+      we[2 * k + 1] = e + 1 < a.n ? w64[e + 1] : make_double2(0.0, 0.0);
+    } else if (with_w) {
+      float2 lo, hi;
+      load_pair<kVec>(w, pbase + (long)k * kBS, a.n, lo, hi);
+      we[2 * k] = make_double2(lo.x, lo.y);
+      we[2 * k + 1] = make_double2(hi.x, hi.y);
     } else {
-      we[2 * k] = we[2 * k + 1] = make_float2(0.f, 0.f);
+      we[2 * k] = we[2 * k + 1] = make_double2(0.0, 0.0);
     }
   }
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -163,14 +177,16 @@ __global__ void __launch_bounds__(kBS) k_lr_proj(LrArgs a, int with_w) {
         for (int k = 0; k < kPP; ++k) load_pair<kVec>(q, pbase + (long)k * kBS, a.n, qe[2 * k], qe[2 * k + 1]);
 #pragma unroll
         for (int b4 = 0; b4 < kPE; b4 += 4) {
-          float sr = 0.f, si = 0.f, tr = 0.f, ti = 0.f;
+          float sr = 0.f, si = 0.f;
+          double tr = 0.0, ti = 0.0;
 #pragma unroll
           for (int k = b4; k < b4 + 4; ++k) {
-            const float2 p = cj_mul(qe[k], ue[k]), r = cj_mul(qe[k], we[k]);
+            const float2 p = cj_mul(qe[k], ue[k]);
             sr += p.x;
             si += p.y;
-            tr += r.x;
-            ti += r.y;
+            const double qx = qe[k].x, qy = qe[k].y;
+            tr += qx * we[k].x + qy * we[k].y;
+            ti += qx * we[k].y - qy * we[k].x;
           }
           acc[vv * 4 + 0] += sr;
           acc[vv * 4 + 1] += si;
@@ -209,17 +225,15 @@ __global__ void __launch_bounds__(kBS) k_lr_proj(LrArgs a, int with_w) {
     double al = 0.0, gr = 0.0, gi = 0.0;
 #pragma unroll
     for (int b4 = 0; b4 < kPE; b4 += 4) {
-      float a4 = 0.f, r4 = 0.f, i4 = 0.f;
+      float a4 = 0.f;
 #pragma unroll
       for (int k = b4; k < b4 + 4; ++k) {
         a4 += ue[k].x * ue[k].x + ue[k].y * ue[k].y;
-        const float2 r = cj_mul(ue[k], we[k]);
-        r4 += r.x;
-        i4 += r.y;
+        const double ux = ue[k].x, uy = ue[k].y;
+        gr += ux * we[k].x + uy * we[k].y;
+        gi += ux * we[k].y - uy * we[k].x;
       }
       al += a4;
-      gr += r4;
-      gi += i4;
     }
     al = wave_sum(al);
     gr = wave_sum(gr);
@@ -524,7 +538,13 @@ __global__ void __launch_bounds__(kBS) k_lr_upd(LrArgs a) {
     const double inu = sc[3];
     const double ccx = sc[4], ccy = sc[5];
     const float2 uv = a.U[(long)y * a.sU + e];
-    const float2 wv = a.W[(long)y * a.sW + e];
+    double2 wv;
+    if (a.W64) {
+      wv = a.W64[(long)y * a.n + e];
+    } else {
+      const float2 w32 = a.W[(long)y * a.sW + e];
+      wv = make_double2(w32.x, w32.y);
+    }
     float2* qj = a.Q + (long)y * (a.ld + 1) * a.n + (long)j * a.n;
     qj[e] = make_float2((float)((uv.x - a1x) * inu), (float)((uv.y - a1y) * inu));
     const double cux = ccx * uv.x - ccy * uv.y, cuy = ccx * uv.y + ccy * uv.x;
@@ -715,6 +735,7 @@ static LrArgs lr_args(const LrWs& w, DevState* st, long sst, double* hist, long 
   a.sU = 2 * w.n;
   a.sW = w.n;
   a.W = w.W;
+  a.W64 = w.W64;
   a.part = w.part;
   a.sPart = w.sPart;
   a.s = w.s;
@@ -748,28 +769,27 @@ int lr_cycle(LrWs& w, hipStream_t st, DevState* dst, DevState* hst, long sst, in
   const long n = w.n;
   LrArgs a = lr_args(w, dst, sst, hist, shist, m);
   const bool vec = (n % 2) == 0;   // 16-byte loads: every vector starts 16-byte aligned
+  constexpr int KK = LrTiming::kKernels;
   if (tm && !tm->created) {
-    for (int e = 0; e < 4 * LrTiming::kRing; ++e) CGM_HIP(hipEventCreate(&tm->ev[e]));
+    for (int e = 0; e < 2 * KK * LrTiming::kRing; ++e) CGM_HIP(hipEventCreate(&tm->ev[e]));
     tm->created = true;
   }
   int pending = 0;  // ring slots recorded since the last read-back
   auto harvest = [&]() -> int {
     if (!tm || pending == 0) return CGM_SUCCESS;
     const int last = pending - 1;
-    CGM_HIP(hipEventSynchronize(tm->ev[4 * last + (tm->slot_upd[last] ? 3 : 1)]));
-    for (int sl = 0; sl < pending; ++sl) {
-      float t = 0.f;
-      CGM_HIP(hipEventElapsedTime(&t, tm->ev[4 * sl], tm->ev[4 * sl + 1]));
-      tm->ms[0] += t;
-      tm->bytes[0] += tm->slot_bytes[0][sl];
-      tm->launches[0] += 1;
-      if (tm->slot_upd[sl]) {
-        CGM_HIP(hipEventElapsedTime(&t, tm->ev[4 * sl + 2], tm->ev[4 * sl + 3]));
-        tm->ms[1] += t;
-        tm->bytes[1] += tm->slot_bytes[1][sl];
-        tm->launches[1] += 1;
+    // the step's last event: the update's end, or the projection's at the
+    // cycle's last step
+    CGM_HIP(hipEventSynchronize(tm->ev[2 * KK * last + (tm->slot_has[1][last] ? 3 : 1)]));
+    for (int sl = 0; sl < pending; ++sl)
+      for (int k = 0; k < KK; ++k) {
+        if (!tm->slot_has[k][sl]) continue;
+        float t = 0.f;
+        CGM_HIP(hipEventElapsedTime(&t, tm->ev[2 * KK * sl + 2 * k], tm->ev[2 * KK * sl + 2 * k + 1]));
+        tm->ms[k] += t;
+        tm->bytes[k] += tm->slot_bytes[k][sl];
+        tm->launches[k] += 1;
       }
-    }
     pending = 0;
     return CGM_SUCCESS;
   };
@@ -782,25 +802,29 @@ int lr_cycle(LrWs& w, hipStream_t st, DevState* dst, DevState* hst, long sst, in
     a.Unext = w.U + (long)((j + 1) & 1) * n;
     a.ngroups = (j + kJT - 1) / kJT;
     const int with_w = j < m;
+    const int sl = pending;
+    if (tm)
+      for (int k = 0; k < KK; ++k) tm->slot_has[k][sl] = false;
     if (with_w) {
+      if (tm) tm->cur = sl;  // the tile application records into this slot
       int rc = precond(j, a.U, 2 * n, w.Z + (long)j * n, (long)w.m * n);
+      if (tm) tm->cur = -1;
       if (!rc) rc = matvec(j, w.Z + (long)j * n, (long)w.m * n, w.W, n);
       if (rc) return rc;
     }
     const int gx = (a.ngroups > 0 ? a.ngroups : 1) * w.nchunks;
-    const int sl = pending;
-    if (tm) CGM_HIP(hipEventRecord(tm->ev[4 * sl], st));
+    if (tm) CGM_HIP(hipEventRecord(tm->ev[2 * KK * sl], st));
     if (vec)
       hipLaunchKernelGGL(k_lr_proj<true>, dim3(gx, B), dim3(kBS), 0, st, a, with_w);
     else
       hipLaunchKernelGGL(k_lr_proj<false>, dim3(gx, B), dim3(kBS), 0, st, a, with_w);
-    if (tm) CGM_HIP(hipEventRecord(tm->ev[4 * sl + 1], st));
+    if (tm) CGM_HIP(hipEventRecord(tm->ev[2 * KK * sl + 1], st));
     hipLaunchKernelGGL(k_lr_red, dim3(a.ngroups > 0 ? a.ngroups : 1, B), dim3(kBS), 0, st, a);
     hipLaunchKernelGGL(k_lr_fin, dim3(1, B), dim3(kBS), 0, st, a);
     if (with_w) {
-      if (tm) CGM_HIP(hipEventRecord(tm->ev[4 * sl + 2], st));
+      if (tm) CGM_HIP(hipEventRecord(tm->ev[2 * KK * sl + 2], st));
       hipLaunchKernelGGL(k_lr_upd, dim3(w.nuchunks, B), dim3(kBS), 0, st, a);
-      if (tm) CGM_HIP(hipEventRecord(tm->ev[4 * sl + 3], st));
+      if (tm) CGM_HIP(hipEventRecord(tm->ev[2 * KK * sl + 3], st));
     }
     CGM_HIP(hipGetLastError());
     if (tm) {
@@ -808,7 +832,8 @@ int lr_cycle(LrWs& w, hipStream_t st, DevState* dst, DevState* hst, long sst, in
       // q_0..q_{j-1}, u, w and writes q_j, u_{j+1} (8 B per cell each)
       tm->slot_bytes[0][sl] = 8.0 * (double)n * (j + (with_w ? 2 : 1)) * B;
       tm->slot_bytes[1][sl] = with_w ? 8.0 * (double)n * (j + 4) * B : 0.0;
-      tm->slot_upd[sl] = with_w != 0;
+      tm->slot_has[0][sl] = true;
+      tm->slot_has[1][sl] = with_w != 0;
       if (++pending == LrTiming::kRing) {
         const int rc = harvest();
         if (rc) return rc;
@@ -851,6 +876,7 @@ extern "C" int cgm_profile_longrec(cgm_ctx_t c, int j, int reps, float* ms) {
   int rc = lr_alloc(c->lr, n, j + 1, 1);
   if (rc) return rc;
   LrWs& w = c->lr;
+  w.W64 = nullptr;  // the complex64 w (timing only)
   // zero data: the kernels' cost does not depend on the values; the state
   // keeps done = 0 so every launch does its full work (k_lr_fin, which
   // would end the cycle on a zero vector, is timed last)
@@ -907,15 +933,18 @@ extern "C" int cgm_longrec_timing(cgm_ctx_t c, int enable) {
   CGM_CHECK(c, CGM_ERROR_INVALID_VALUE, "null context");
   cgm::LrTiming& t = c->lrt;
   t.on = enable != 0;
-  t.ms[0] = t.ms[1] = t.bytes[0] = t.bytes[1] = 0.0;
-  t.launches[0] = t.launches[1] = 0;
+  t.cur = -1;
+  for (int k = 0; k < cgm::LrTiming::kKernels; ++k) {
+    t.ms[k] = t.bytes[k] = 0.0;
+    t.launches[k] = 0;
+  }
   return CGM_SUCCESS;
 }
 
 extern "C" int cgm_longrec_timing_read(cgm_ctx_t c, double* out) {
   CGM_CHECK(c && out, CGM_ERROR_INVALID_VALUE, "null argument");
   const cgm::LrTiming& t = c->lrt;
-  for (int k = 0; k < 2; ++k) {
+  for (int k = 0; k < cgm::LrTiming::kKernels; ++k) {
     out[3 * k] = t.ms[k];
     out[3 * k + 1] = t.bytes[k];
     out[3 * k + 2] = (double)t.launches[k];

@@ -21,6 +21,14 @@
 // tile's input staged in LDS.  A map shared by several right-hand sides is
 // factored once, and each work-group then applies its tile's inverse to all
 // of them while it streams it.
+//
+// 64 x 64 tiles (4,096 unknowns each) store their inverses in FP16 complex
+// (half the bytes per step), each tile scaled by its largest magnitude so no
+// entry leaves FP16's range; the scale is applied to the FP32 sums.  Their
+// application splits every tile's rows over 8 work-groups.  On the benchmark
+// maps at N = 512 they halve the Krylov steps of 32 x 32 tiles (2,391 against
+// 4,573 on seed 0), and FP16 storage leaves the step count unchanged.
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
@@ -38,6 +46,8 @@ struct cgm_tiles {
   int nt = 0;               // tiles per grid side
   int M2 = 0;               // unknowns per tile (T * T)
   float2* K = nullptr;      // [B][nt * nt][M2 * M2] inverses, column-major
+  __half2* K16 = nullptr;   // FP16 copy (T = 64): [B][nt * nt][M2 * M2] of K / kscale
+  float* kscale = nullptr;  // [B][nt * nt] largest |K| entry of each tile (T = 64)
   int* ipiv = nullptr;      // [B][nt * nt][M2]
   int* info = nullptr;      // [B][nt * nt]
   int maps = 0;             // maps factored by the last cgm_tiles_factor (1: shared)
@@ -46,17 +56,22 @@ struct cgm_tiles {
 
 namespace {
 
-// A_t of tile blockIdx.x of map blockIdx.y, column-major into K.
+constexpr int kBuildChunk = 65536;  // matrix entries per work-group of k_tile_build
+
+// A_t of map blockIdx.y, column-major into K: work-group blockIdx.x covers
+// chunk (blockIdx.x % chunks) of tile (blockIdx.x / chunks).
 __global__ void __launch_bounds__(256)
 k_tile_build(const double2* __restrict__ g, const float* __restrict__ chi, long chi_stride, float2* __restrict__ K,
-             int N, int T) {
+             int N, int T, int chunks) {
   const int nt = N / T;
   const int M2 = T * T;
-  const int t = blockIdx.x;
+  const int t = blockIdx.x / chunks;
+  const long e0 = (long)(blockIdx.x % chunks) * kBuildChunk;
+  const long e1 = e0 + kBuildChunk < (long)M2 * M2 ? e0 + kBuildChunk : (long)M2 * M2;
   const int a = t / nt, b = t % nt;
   chi += (long)blockIdx.y * chi_stride;
   float2* A = K + ((long)blockIdx.y * nt * nt + t) * (long)M2 * M2;
-  for (int e = threadIdx.x; e < M2 * M2; e += blockDim.x) {
+  for (long e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
     const int p = e % M2, q = e / M2;  // row p, column q
     const int pr = p / T, pc = p % T, qr = q / T, qc = q % T;
     const int dr = pr > qr ? pr - qr : qr - pr;
@@ -130,6 +145,103 @@ k_tile_apply(const float2* __restrict__ K, long sK, FnoIO io, const int* __restr
     if (live[v]) io.z[(long)(s0 + v) * io.sZ + cell] = acc[v];
 }
 
+// Largest |K| entry of each tile (one work-group per tile of one map).
+__global__ void __launch_bounds__(1024) k_tile_maxabs(const float2* __restrict__ K, long m22, float* __restrict__ sc) {
+  const float2* Kt = K + (long)blockIdx.x * m22;
+  float mx = 0.f;
+  for (long e = threadIdx.x; e < m22; e += blockDim.x) {
+    const float2 v = Kt[e];
+    mx = fmaxf(mx, fmaxf(fabsf(v.x), fabsf(v.y)));
+  }
+  __shared__ float red[16];
+  for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float m = 0.f;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) m = fmaxf(m, red[w]);
+    sc[blockIdx.x] = m > 0.f ? m : 1.f;
+  }
+}
+
+// K16 = K / scale of its tile, rounded to FP16.
+__global__ void __launch_bounds__(256) k_tile_to_half(const float2* __restrict__ K, __half2* __restrict__ K16,
+                                                      const float* __restrict__ sc, long m22, long total) {
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+    const float r = 1.f / sc[e / m22];
+    const float2 v = K[e];
+    K16[e] = __floats2half2_rn(v.x * r, v.y * r);
+  }
+}
+
+// y_t = scale_t K16_t x_t for 64 x 64 tiles: work-group (tile, part) of
+// system blockIdx.y computes the kRowsH rows [part kRowsH, (part+1) kRowsH)
+// of its tile, two consecutive rows per thread (one 8-byte load of the
+// column-major FP16 inverse per column), FP32 sums; the tile's input is
+// staged in LDS by every part (part 0 also writes the scaled input back to
+// io.vw).  Same io / done conventions as k_tile_apply with NV = 1.
+constexpr int kTileH = 4096;           // unknowns of a 64 x 64 tile
+constexpr int kRowsH = 512;            // rows per work-group
+constexpr int kPartsH = kTileH / kRowsH;
+__global__ void __launch_bounds__(kRowsH / 2)
+k_tile_apply_h(const __half2* __restrict__ K16, const float* __restrict__ ksc, long sK, int maps1, FnoIO io,
+               const int* __restrict__ done, long sd, int N) {
+  constexpr int T = 64;
+  constexpr int kU = 16;  // columns loaded ahead
+  __shared__ float2 xs[kTileH];
+  const int nt = N / T;
+  const int t = blockIdx.x / kPartsH, part = blockIdx.x % kPartsH;
+  const int a = t / nt, b = t % nt;
+  const int s = blockIdx.y;
+  if (done && *boff(done, (long)s * sd)) return;
+  float sc = 1.f;
+  if (io.nrm) {
+    const double d = *boff(io.nrm, (long)s * io.sNrm);
+    sc = d > 0.0 ? (float)(1.0 / d) : 0.f;
+  }
+  for (int r = threadIdx.x; r < kTileH; r += blockDim.x) {
+    const long cell = (long)(a * T + r / T) * N + b * T + r % T;
+    float2 xv = io.v[(long)s * io.sV + cell];
+    xv.x *= sc;
+    xv.y *= sc;
+    if (io.nrm && io.vw && part == 0) io.vw[(long)s * io.sV + cell] = xv;
+    xs[r] = xv;
+  }
+  __syncthreads();
+  const long ko = (long)(maps1 ? 0 : s) * sK + (long)t * kTileH * kTileH;
+  const int p = part * kRowsH + 2 * threadIdx.x;  // rows p, p + 1
+  const uint2* Kt = reinterpret_cast<const uint2*>(K16 + ko + p);
+  float ax = 0.f, ay = 0.f, bx = 0.f, by = 0.f;
+  for (int q0 = 0; q0 < kTileH; q0 += kU) {
+    uint2 k[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) k[u] = Kt[(long)(q0 + u) * (kTileH / 2)];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const float2 xv = xs[q0 + u];
+      const float2 k0 = __half22float2(__builtin_bit_cast(__half2, k[u].x));
+      const float2 k1 = __half22float2(__builtin_bit_cast(__half2, k[u].y));
+      ax += k0.x * xv.x - k0.y * xv.y;
+      ay += k0.x * xv.y + k0.y * xv.x;
+      bx += k1.x * xv.x - k1.y * xv.y;
+      by += k1.x * xv.y + k1.y * xv.x;
+    }
+  }
+  const float m = ksc[(maps1 ? 0 : s) * (long)nt * nt + t];
+  const long c0 = (long)(a * T + p / T) * N + b * T + p % T;   // p and p + 1 share a tile row (p even)
+  io.z[(long)s * io.sZ + c0] = make_float2(ax * m, ay * m);
+  io.z[(long)s * io.sZ + c0 + 1] = make_float2(bx * m, by * m);
+}
+
+int launch_apply_h(const cgm_tiles* tl, hipStream_t st, const FnoIO& io, const int* done, long sd, int nsys) {
+  const long sK = (long)tl->nt * tl->nt * kTileH * kTileH;
+  const int ntiles = tl->nt * tl->nt;
+  hipLaunchKernelGGL(k_tile_apply_h, dim3(ntiles * kPartsH, nsys), dim3(kRowsH / 2), 0, st, tl->K16, tl->kscale,
+                     sK, tl->maps == 1 ? 1 : 0, io, done, sd, tl->N);
+  CGM_HIP(hipGetLastError());
+  return CGM_SUCCESS;
+}
+
 template <int M2>
 int launch_apply_m(const cgm_tiles* tl, hipStream_t st, const FnoIO& io, const int* done, long sd, int nsys) {
   const long sK = (long)tl->nt * tl->nt * M2 * M2;
@@ -163,14 +275,32 @@ int tiles_apply_io(const cgm_tiles* tl, hipStream_t st, const FnoIO& io, int nsy
     cgm_set_last_error("%d systems but %d factored maps", nsys, tl->maps);
     return CGM_ERROR_INVALID_VALUE;
   }
+  // live timing inside a long cycle (cgm_longrec_timing): events around the
+  // launch, algorithmic bytes = the inverses streamed (once per system, or
+  // once per group of four right-hand sides of one map) + v in, z out
+  LrTiming* tm = tl->ctx->lrt.on ? &tl->ctx->lrt : nullptr;
+  const int sl = tm ? tm->cur : -1;
+  constexpr int KK = LrTiming::kKernels;
+  if (sl >= 0) CGM_HIP(hipEventRecord(tm->ev[2 * KK * sl + 4], st));
+  int rc;
   switch (tl->M2) {
-    case 64: return launch_apply_m<64>(tl, st, io, done, sd, nsys);
-    case 256: return launch_apply_m<256>(tl, st, io, done, sd, nsys);
-    case 1024: return launch_apply_m<1024>(tl, st, io, done, sd, nsys);
+    case 64: rc = launch_apply_m<64>(tl, st, io, done, sd, nsys); break;
+    case 256: rc = launch_apply_m<256>(tl, st, io, done, sd, nsys); break;
+    case 1024: rc = launch_apply_m<1024>(tl, st, io, done, sd, nsys); break;
+    case 4096: rc = launch_apply_h(tl, st, io, done, sd, nsys); break;
     default:
       cgm_set_last_error("tile side %d unsupported", tl->T);
       return CGM_ERROR_UNSUPPORTED;
   }
+  if (sl >= 0 && !rc) {
+    CGM_HIP(hipEventRecord(tm->ev[2 * KK * sl + 5], st));
+    const double ntiles = (double)tl->nt * tl->nt;
+    const double per_entry = tl->K16 ? 4.0 : 8.0;
+    const int streams = (tl->maps == 1 && nsys > 1 && tl->M2 < 4096) ? (nsys + 3) / 4 : nsys;
+    tm->slot_bytes[2][sl] = ntiles * tl->M2 * (double)tl->M2 * per_entry * streams + 16.0 * tl->N * tl->N * nsys;
+    tm->slot_has[2][sl] = true;
+  }
+  return rc;
 }
 
 }  // namespace cgm
@@ -178,7 +308,8 @@ int tiles_apply_io(const cgm_tiles* tl, hipStream_t st, const FnoIO& io, int nsy
 extern "C" int cgm_tiles_create(cgm_ctx_t c, int T, int batch, cgm_tiles_t* out) {
   CGM_CHECK(c && out, CGM_ERROR_INVALID_VALUE, "null argument");
   *out = nullptr;
-  CGM_CHECK(T == 8 || T == 16 || T == 32, CGM_ERROR_UNSUPPORTED, "tile side %d: 8, 16 or 32 supported", T);
+  CGM_CHECK(T == 8 || T == 16 || T == 32 || T == 64, CGM_ERROR_UNSUPPORTED, "tile side %d: 8, 16, 32 or 64 supported",
+            T);
   CGM_CHECK(c->N % T == 0, CGM_ERROR_UNSUPPORTED, "N=%d is not a multiple of the tile side %d", c->N, T);
   CGM_CHECK(batch >= 1, CGM_ERROR_INVALID_VALUE, "batch %d < 1", batch);
   CGM_HIP(hipSetDevice(c->device));
@@ -192,9 +323,12 @@ extern "C" int cgm_tiles_create(cgm_ctx_t c, int T, int batch, cgm_tiles_t* out)
   t->M2 = T * T;
   const size_t ntiles = (size_t)t->nt * t->nt;
   const size_t M2 = (size_t)t->M2;
+  const bool half = T == 64;
   if (hipMalloc((void**)&t->K, sizeof(float2) * M2 * M2 * ntiles * batch) != hipSuccess ||
       hipMalloc((void**)&t->ipiv, sizeof(int) * M2 * ntiles * batch) != hipSuccess ||
       hipMalloc((void**)&t->info, sizeof(int) * ntiles * batch) != hipSuccess ||
+      (half && hipMalloc((void**)&t->K16, sizeof(__half2) * M2 * M2 * ntiles * batch) != hipSuccess) ||
+      (half && hipMalloc((void**)&t->kscale, sizeof(float) * ntiles * batch) != hipSuccess) ||
       rocblas_create_handle(&t->h) != rocblas_status_success) {
     cgm_tiles_destroy(t);
     cgm_set_last_error("cgm_tiles_create: allocation failed (%zu MiB of tile inverses)",
@@ -208,7 +342,7 @@ extern "C" int cgm_tiles_create(cgm_ctx_t c, int T, int batch, cgm_tiles_t* out)
 extern "C" int cgm_tiles_destroy(cgm_tiles_t t) {
   if (!t) return CGM_SUCCESS;
   (void)hipSetDevice(t->device);
-  void* ptrs[] = {t->K, t->ipiv, t->info};
+  void* ptrs[] = {t->K, t->ipiv, t->info, t->K16, t->kscale};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (t->h) (void)rocblas_destroy_handle(t->h);
@@ -219,7 +353,8 @@ extern "C" int cgm_tiles_destroy(cgm_tiles_t t) {
 extern "C" int cgm_tiles_memory(cgm_tiles_t t, size_t* bytes) {
   CGM_CHECK(t && bytes, CGM_ERROR_INVALID_VALUE, "null argument");
   const size_t ntiles = (size_t)t->nt * t->nt;
-  *bytes = (sizeof(float2) * t->M2 * t->M2 + sizeof(int) * (t->M2 + 1)) * ntiles * t->B;
+  const size_t k16 = t->K16 ? sizeof(__half2) * t->M2 * t->M2 + sizeof(float) : 0;
+  *bytes = (sizeof(float2) * t->M2 * t->M2 + sizeof(int) * (t->M2 + 1) + k16) * ntiles * t->B;
   return CGM_SUCCESS;
 }
 
@@ -234,8 +369,9 @@ extern "C" int cgm_tiles_factor(cgm_tiles_t t, const cgm_complex128* block, cons
   const int ntiles = t->nt * t->nt;
   const int M2 = t->M2;
   t->maps = 0;
-  hipLaunchKernelGGL(k_tile_build, dim3(ntiles, maps), dim3(256), 0, c->stream, (const double2*)block, chi,
-                     chi_stride, t->K, t->N, t->T);
+  const int chunks = (int)(((long)M2 * M2 + kBuildChunk - 1) / kBuildChunk);
+  hipLaunchKernelGGL(k_tile_build, dim3(ntiles * chunks, maps), dim3(256), 0, c->stream, (const double2*)block, chi,
+                     chi_stride, t->K, t->N, t->T, chunks);
   CGM_HIP(hipGetLastError());
   CGM_CHECK(rocblas_set_stream(t->h, c->stream) == rocblas_status_success, CGM_ERROR_DEVICE,
             "rocblas_set_stream failed");
@@ -252,6 +388,13 @@ extern "C" int cgm_tiles_factor(cgm_tiles_t t, const cgm_complex128* block, cons
               k / ntiles, info[k]);
   s = rocsolver_cgetri_strided_batched(t->h, M2, (rocblas_float_complex*)t->K, M2, sA, t->ipiv, M2, t->info, nb);
   CGM_CHECK(s == rocblas_status_success, CGM_ERROR_DEVICE, "rocsolver getri: status %d", (int)s);
+  if (t->K16) {
+    const long m22 = (long)M2 * M2;
+    hipLaunchKernelGGL(k_tile_maxabs, dim3(nb), dim3(1024), 0, c->stream, t->K, m22, t->kscale);
+    hipLaunchKernelGGL(k_tile_to_half, dim3(4096), dim3(256), 0, c->stream, t->K, t->K16, t->kscale, m22,
+                       m22 * nb);
+    CGM_HIP(hipGetLastError());
+  }
   CGM_HIP(hipStreamSynchronize(c->stream));
   t->maps = maps;
   return CGM_SUCCESS;
@@ -271,3 +414,67 @@ extern "C" int cgm_tiles_apply(cgm_tiles_t t, const cgm_complex64* x, cgm_comple
   io.sZ = io.sV;
   return tiles_apply_io(t, t->ctx->stream, io, batch, nullptr, 0);
 }
+
+// ---------------------------------------------------------------------------
+// diagnostics: the application alone on synthetic inverses
+// ---------------------------------------------------------------------------
+namespace {
+__global__ void k_tile_fill(float2* __restrict__ K, __half2* __restrict__ K16, float* __restrict__ sc, long total,
+                            long ntiles) {
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+    const float v = 1e-3f * (float)(e % 7);
+    if (K16)
+      K16[e] = __floats2half2_rn(v, -v);
+    else
+      K[e] = make_float2(v, -v);
+    if (sc && e < ntiles) sc[e] = 1.f;
+  }
+}
+}  // namespace
+
+extern "C" int cgm_profile_tiles(cgm_tiles_t t, int reps, float* ms) {
+  CGM_CHECK(t && ms, CGM_ERROR_INVALID_VALUE, "null argument");
+  CGM_CHECK(reps >= 1, CGM_ERROR_INVALID_VALUE, "reps < 1");
+  cgm_context* c = t->ctx;
+  CGM_HIP(hipSetDevice(c->device));
+  const long ntiles = (long)t->nt * t->nt;
+  const long total = (long)t->M2 * t->M2 * ntiles;  // one map
+  hipLaunchKernelGGL(k_tile_fill, dim3(4096), dim3(256), 0, c->stream, t->K, t->K16, t->kscale, total, ntiles);
+  CGM_HIP(hipGetLastError());
+  const long n = (long)t->N * t->N;
+  float2 *x = nullptr, *y = nullptr;
+  CGM_HIP(hipMalloc((void**)&x, sizeof(float2) * n));
+  if (hipMalloc((void**)&y, sizeof(float2) * n) != hipSuccess) {
+    (void)hipFree(x);
+    cgm_set_last_error("cgm_profile_tiles: allocation failed");
+    return CGM_ERROR_ALLOC;
+  }
+  int rc = CGM_SUCCESS;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipMemsetAsync(x, 0, sizeof(float2) * n, c->stream) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+      hipEventCreate(&e1) != hipSuccess) {
+    rc = CGM_ERROR_DEVICE;
+  } else {
+    t->maps = 1;
+    FnoIO io;
+    io.v = x;
+    io.sV = n;
+    io.z = y;
+    io.sZ = n;
+    rc = tiles_apply_io(t, c->stream, io, 1, nullptr, 0);
+    (void)hipEventRecord(e0, c->stream);
+    for (int r = 0; !rc && r < reps; ++r) rc = tiles_apply_io(t, c->stream, io, 1, nullptr, 0);
+    (void)hipEventRecord(e1, c->stream);
+    (void)hipEventSynchronize(e1);
+    (void)hipEventElapsedTime(ms, e0, e1);
+    *ms /= reps;
+    t->maps = 0;  // the synthetic inverses must be factored before use
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(x);
+  (void)hipFree(y);
+  return rc;
+}
+

@@ -77,11 +77,11 @@ class Context:
         check(lib().cgm_longrec_timing(self._ptr, int(bool(enable))), "cgm_longrec_timing")
 
     def longrec_timing_read(self):
-        """{'proj'|'upd': {'ms', 'bytes', 'launches'}} summed since longrec_timing()."""
-        out = (_c.c_double * 6)()
+        """{'proj'|'upd'|'tile': {'ms', 'bytes', 'launches'}} summed since longrec_timing()."""
+        out = (_c.c_double * 9)()
         check(lib().cgm_longrec_timing_read(self._ptr, out), "cgm_longrec_timing_read")
         return {k: {"ms": out[3 * i], "bytes": out[3 * i + 1], "launches": int(out[3 * i + 2])}
-                for i, k in enumerate(("proj", "upd"))}
+                for i, k in enumerate(("proj", "upd", "tile"))}
 
     def set_arnoldi_path(self, path):
         """0: fused lagged-CGS2 steps (default), 1: separate two-pass CGS2 kernels."""

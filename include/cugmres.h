@@ -313,12 +313,14 @@ int cgm_fgmres_batched(cgm_ctx_t ctx, const cgm_complex64* dev_gamma, const floa
 /* ---- tile preconditioner ------------------------------------------------------ */
 /* No counterpart in the reference, which has no preconditioner
  * (B/CUDA_GMRES/GMRES.cuh:22-446).  The grid is cut into T x T tiles
- * (T = 8, 16 or 32, N % T == 0) and P applies the exact inverse of the operator
- * restricted to each tile,
+ * (T = 8, 16, 32 or 64, N % T == 0) and P applies the exact inverse of the
+ * operator restricted to each tile,
  *     (P v)_t = (I - G_t diag(chi_t))^{-1} v_t,  G_t[p, q] = g[|p_r-q_r|, |p_c-q_c|],
  * with g the N x N generator block of the operator.  The inverses are formed
  * once per map in FP32 complex (batched LU with partial pivoting) and take
- * T^4 * 8 bytes per tile: (N/T)^2 T^4 * 8 = 512 MiB per map at N = 512, T = 16. */
+ * T^4 * 8 bytes per tile: (N/T)^2 T^4 * 8 = 512 MiB per map at N = 512, T = 16.
+ * With T = 64 they are applied from an FP16 copy scaled per tile (half the
+ * bytes per application; 4 GiB more per map at N = 512, 12 GiB in all). */
 typedef struct cgm_tiles* cgm_tiles_t;
 
 /* The generator block g[|di|, |dj|] (N*N complex128, device memory) of the
@@ -341,6 +343,11 @@ int cgm_tiles_factor(cgm_tiles_t tiles, const cgm_complex128* dev_block, const f
 /* y[b] = P x[b] for b < batch ([batch][N*N]; map b, or the one map).
  * Asynchronous on the context stream. */
 int cgm_tiles_apply(cgm_tiles_t tiles, const cgm_complex64* dev_x, cgm_complex64* dev_y, int batch);
+/* Diagnostics: fills the inverses of one map with synthetic values (no
+ * factorisation) and returns in *ms the average device time of `reps`
+ * applications to one vector (HIP events on the context stream).  The
+ * workspace must be factored again before it is used. */
+int cgm_profile_tiles(cgm_tiles_t tiles, int reps, float* ms);
 
 /* The right preconditioner of cgm_fgmres_pc / cgm_fgmres_pc_batched: the tile
  * inverses (tiles), the network (net), both, or the identity (both NULL).
@@ -478,7 +485,9 @@ int cgm_profile_longrec(cgm_ctx_t ctx, int j, int reps, float* ms);
  * steps while on).  enable = 0 turns it off; either call zeroes the sums.
  * out[0..2] = projection: summed milliseconds, algorithmic bytes (8 B per
  * cell for each basis vector read and each vector read or written),
- * launches; out[3..5] = the same for the update. */
+ * launches; out[3..5] = the same for the update; out[6..8] for the tile
+ * inverses' application (the inverses streamed, 8 B per entry, 4 B with
+ * T = 64, plus the vector in and out). */
 int cgm_longrec_timing(cgm_ctx_t ctx, int enable);
 int cgm_longrec_timing_read(cgm_ctx_t ctx, double* out);
 

@@ -176,8 +176,10 @@ def test_fno_forward_batch_equals_single_vectors(gpu, N, B):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("N,B", [(64, 2), (128, 1), (128, 4), (256, 8), (256, 16)])
+@pytest.mark.parametrize("N,B", [(64, 2), (128, 1), (128, 4), (256, 8), (256, 16), (512, 16)])
 def test_fno_backward_matches_oracle(gpu, N, B):
+    """(512, 16) runs 8,192 grid rows, where the training forward pass takes the
+    eight-row cell GEMM that writes the pre-activations the backward reads."""
     solver, fno = gpu
     chi, v, p = _problem(N, B, 21)
     rng = np.random.default_rng(3)

@@ -71,8 +71,10 @@ def test_oracle_tile_layout_round_trip():
 
 # ---- HIP path against the oracle ------------------------------------------------
 @pytest.mark.gpu
-@pytest.mark.parametrize("N,T", [(64, 8), (64, 16), (96, 16), (128, 16)])
+@pytest.mark.parametrize("N,T", [(64, 8), (64, 16), (96, 16), (128, 16), (128, 64)])
 def test_tiles_apply_matches_oracle(orc, N, T):
+    """FP32 inverses to 1e-4; 64 x 64 tiles apply an FP16 copy of theirs
+    (scaled per tile), to 5e-3."""
     import torch
     from cugmres_amd import problems, solver, tiles as gt
     from oracle import tiles
@@ -85,7 +87,7 @@ def test_tiles_apply_matches_oracle(orc, N, T):
     tl = gt.Tiles(ctx, T, 1).factor(blk, ctx.to_device(chi, "f32"))
     y = tl.apply(ctx.to_device(b)).cpu().numpy()
     yr = tiles.tile_apply(tiles.tile_inverses(ref_blk, chi, N, T), b, N, T)
-    assert rel(y, yr) < 1e-4   # FP32 LU of the tile operators
+    assert rel(y, yr) < (5e-3 if T == 64 else 1e-4)   # FP32 LU of the tile operators (FP16 copy at T = 64)
     tl.close()
     ctx.close()
 
@@ -151,6 +153,34 @@ def test_fgmres_with_tiles_converges_to_the_oracle_solution(orc, restart):
     y, _ = _oracle_right_pc_gmres(lambda v: op.matvec(chi, v), lambda v: tiles.tile_apply(K, v, N, T), b, 1e-12)
     assert rel(xs, y) < 1e-5
     tl.close()
+    ctx.close()
+
+
+@pytest.mark.gpu
+def test_fgmres_with_64_tiles_converges(orc):
+    """64 x 64 tiles (FP16 inverses) at N = 128 on the benchmark physics: one
+    long cycle reaches a true relative residual below 1e-6 (oracle FP64
+    matvec), in fewer steps than with 32 x 32 tiles."""
+    from cugmres_amd import problems, solver, tiles as gt
+    N = 128
+    print("fgmres tiles N=128 T=64", flush=True)
+    ctx = solver.Context(N, 0)
+    g, g64 = ctx.spectrum_helmholtz(problems.KH, fp64=True)
+    ctx.set_flexible_fp64(g64)
+    chi, b = _problem(N, 31)
+    dchi, db = ctx.to_device(chi, "f32"), ctx.to_device(b)
+    blk = ctx.block_helmholtz(problems.KH)
+    steps = {}
+    for T in (32, 64):
+        tl = gt.Tiles(ctx, T, 1).factor(blk, dchi)
+        x, info = ctx.fgmres(g, dchi, db, precond=tl, restart=2000, tol=1e-6, max_cycles=2, stall_ratio=0.0)
+        assert info["converged"], (T, info)
+        steps[T] = info["iterations"]
+        tl.close()
+    op = orc.Op(N, orc.baseline_green_block(N, problems.KH), ctx.M)
+    r = b - op.matvec(chi, x.cpu().numpy().astype(np.complex128))
+    assert np.linalg.norm(r) / np.linalg.norm(b) < 1.5e-6
+    assert steps[64] < steps[32], steps
     ctx.close()
 
 
