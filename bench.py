@@ -420,10 +420,17 @@ def fft_reference(a, B=16, reps=20):
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / reps * 1e3
 
-    out = {"N": N, "M": M, "B": B}
+    out = {"N": N, "M": M, "B": B, "radix_plan": ctx.radix_plan()}
     us = timed(lambda: ctx.matvec_batched(g, chi, x, 0, y))
     mv_bytes = B * 88 * N * N + 4 * N * N + 8 * (M // 2 + 1) ** 2
     out["matvec_batched"] = {"us": us, "bytes": mv_bytes, "GBps": mv_bytes / (us * 1e-6) / 1e9}
+    if M == 1024:   # both pass-2 kernels, whatever the autotuner chose
+        auto = out["radix_plan"]["wide_from_batch"]
+        for name, wide in (("radix8", 0), ("radix32", 1)):
+            ctx.set_radix_plan(wide)
+            u = timed(lambda: ctx.matvec_batched(g, chi, x, 0, y))
+            out["matvec_batched_" + name] = {"us": u, "GBps": mv_bytes / (u * 1e-6) / 1e9}
+        ctx.set_radix_plan(auto)
     big = torch.randn(B, M, M, dtype=torch.complex64, device=ctx.device)
     us = timed(lambda: torch.fft.ifft2(torch.fft.fft2(big)))
     b2 = 2 * 32 * M * M * B

@@ -80,6 +80,8 @@ SIGNATURES = {
     "cgm_set_arnoldi_path": (ctypes.c_int, [c_void_p, ctypes.c_int]),
     "cgm_autotune": (ctypes.c_int, [c_void_p]),
     "cgm_get_tiles": (ctypes.c_int, [c_void_p, ctypes.POINTER(ctypes.c_int), c_float_p]),
+    "cgm_get_radix_plan": (ctypes.c_int, [c_void_p, ctypes.POINTER(ctypes.c_int), c_float_p, c_float_p]),
+    "cgm_set_radix_plan": (ctypes.c_int, [c_void_p, ctypes.POINTER(ctypes.c_int)]),
     "cgm_spectrum_size": (ctypes.c_int, [c_void_p, ctypes.POINTER(ctypes.c_size_t)]),
     "cgm_spectrum_from_gprep": (ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
     "cgm_spectrum_from_block": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),

@@ -198,6 +198,49 @@ extern "C" int cgm_autotune(cgm_ctx_t c) {
       if (t2 < best2) { best2 = t2; c2 = k; }
     }
   }
+  // M = 1024: the register-resident radix-32 x 32 kernel of each pass
+  // against the tuned LDS-staged radix-8 one, at batches of 1, 4 and 16
+  // vectors; a pass takes the register-resident kernel from the smallest batch
+  // at which it is faster at that batch and at every larger one
+  int wide_from[3] = {0, 0, 0};
+  float wide_us[3][3] = {}, lds_us[3][3] = {};
+  if (!rc && M == 1024) {
+    const int batches[3] = {1, 4, 16};
+    const size_t nb = 16;
+    float2 *Tb = nullptr, *xb = nullptr, *yb = nullptr;
+    bool ok = hipMalloc((void**)&Tb, sizeof(float2) * (size_t)N * M * nb) == hipSuccess &&
+              hipMalloc((void**)&xb, sizeof(float2) * (size_t)n * nb) == hipSuccess &&
+              hipMalloc((void**)&yb, sizeof(float2) * (size_t)n * nb) == hipSuccess &&
+              hipMemsetAsync(Tb, 0, sizeof(float2) * (size_t)N * M * nb, c->stream) == hipSuccess &&
+              hipMemsetAsync(xb, 0, sizeof(float2) * (size_t)n * nb, c->stream) == hipSuccess;
+    if (ok) {
+      Plan Q = P;
+      Q.rows1 = r1;
+      Q.rows3 = r3;
+      Q.cols2 = c2;
+      BatchStrides bs;
+      bs.x = n;
+      bs.chi = 0;
+      bs.y = n;
+      bs.T = (long)N * M;
+      for (int k = 0; k < 3; ++k) {
+        const int B = batches[k];
+        for (int wide = 0; wide < 2; ++wide) {
+          Q.wide1_batch = Q.wide2_batch = Q.wide3_batch = wide;
+          float(&dst)[3][3] = wide ? wide_us : lds_us;
+          dst[0][k] = 1e3f * time_it([&] { rc |= launch_pass1<float>(Q, c->stream, xb, nullptr, chi, nullptr, kModeForward, Tb, c->tw32, nullptr, B, bs); });
+          dst[1][k] = 1e3f * time_it([&] { rc |= launch_pass2<float, float2>(Q, c->stream, Tb, gq, c->tw32, nullptr, B, bs); });
+          dst[2][k] = 1e3f * time_it([&] { rc |= launch_pass3<float>(Q, c->stream, Tb, xb, chi, nullptr, yb, kModeForward, c->tw32, nullptr, B, bs); });
+        }
+      }
+      for (int q = 0; q < 3; ++q)
+        for (int k = 2; k >= 0 && wide_us[q][k] < lds_us[q][k]; --k) wide_from[q] = batches[k];
+    }
+    (void)hipStreamSynchronize(c->stream);
+    if (Tb) (void)hipFree(Tb);
+    if (xb) (void)hipFree(xb);
+    if (yb) (void)hipFree(yb);
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   (void)hipStreamSynchronize(c->stream);
@@ -208,6 +251,14 @@ extern "C" int cgm_autotune(cgm_ctx_t c) {
   c->plan.rows1 = r1;
   c->plan.rows3 = r3;
   c->plan.cols2 = c2;
+  c->plan.wide1_batch = wide_from[0];
+  c->plan.wide2_batch = wide_from[1];
+  c->plan.wide3_batch = wide_from[2];
+  for (int q = 0; q < 3; ++q)
+    for (int k = 0; k < 3; ++k) {
+      c->tuned_wide_us[q][k] = wide_us[q][k];
+      c->tuned_lds_us[q][k] = lds_us[q][k];
+    }
   c->tuned_us[0] = best1 * 1e3f;
   c->tuned_us[1] = best2 * 1e3f;
   c->tuned_us[2] = best3 * 1e3f;
@@ -221,6 +272,30 @@ extern "C" int cgm_get_tiles(cgm_ctx_t c, int* tiles, float* us) {
   tiles[2] = c->plan.rows3;
   if (us)
     for (int k = 0; k < 3; ++k) us[k] = c->tuned_us[k];
+  return CGM_SUCCESS;
+}
+
+extern "C" int cgm_set_radix_plan(cgm_ctx_t c, const int* wide_from_batch) {
+  CGM_CHECK(c && wide_from_batch, CGM_ERROR_INVALID_VALUE, "null argument");
+  for (int q = 0; q < 3; ++q)
+    CGM_CHECK(wide_from_batch[q] >= 0, CGM_ERROR_INVALID_VALUE, "wide_from_batch[%d] < 0", q);
+  const bool m1024 = c->M == 1024;
+  c->plan.wide1_batch = m1024 ? wide_from_batch[0] : 0;
+  c->plan.wide2_batch = m1024 ? wide_from_batch[1] : 0;
+  c->plan.wide3_batch = m1024 ? wide_from_batch[2] : 0;
+  return CGM_SUCCESS;
+}
+
+extern "C" int cgm_get_radix_plan(cgm_ctx_t c, int* wide_from_batch, float* wide_us, float* lds_us) {
+  CGM_CHECK(c && wide_from_batch, CGM_ERROR_INVALID_VALUE, "null argument");
+  wide_from_batch[0] = c->plan.wide1_batch;
+  wide_from_batch[1] = c->plan.wide2_batch;
+  wide_from_batch[2] = c->plan.wide3_batch;
+  for (int q = 0; q < 3; ++q)
+    for (int k = 0; k < 3; ++k) {
+      if (wide_us) wide_us[3 * q + k] = c->tuned_wide_us[q][k];
+      if (lds_us) lds_us[3 * q + k] = c->tuned_lds_us[q][k];
+    }
   return CGM_SUCCESS;
 }
 

@@ -41,6 +41,9 @@ struct Plan {
   int N, M;
   int rows1, rows3;  // rows per work-group in passes 1 and 3
   int cols2;         // columns per work-group in pass 2
+  int wide1_batch, wide2_batch, wide3_batch;  // M = 1024, complex64: launches of at
+                     // least this many vectors take the register-resident
+                     // pass 1 / 2 / 3 (0: never)
 };
 
 int plan_init(Plan& P, int N);
@@ -285,6 +288,7 @@ struct cgm_context {
   double2* T64 = nullptr;  // N x M scratch of one complex128 matvec (lazy)
   float mask_chi = 0.f;
   float tuned_us[3] = {0.f, 0.f, 0.f};  // autotuned pass 1/2/3 times
+  float tuned_wide_us[3][3] = {}, tuned_lds_us[3][3] = {};  // [pass][batch 1/4/16], microseconds
   int path = 0;  // Arnoldi kernels: 0 = fused lagged CGS2 where supported, 1 = two-pass CGS2
   cgm::KrylovWs ws;
   cgm::LrWs lr;                  // long-recurrence FGMRES workspace (lazy)

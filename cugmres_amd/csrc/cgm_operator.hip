@@ -168,6 +168,233 @@ k_pass2(cplx<T>* __restrict__ Tm, const G* __restrict__ gq, int N, int cols_per_
   }
 }
 
+// ---------------------------------------------------------------------------
+// Pass 2 for M = 1024 in registers (complex64; chosen by the autotuner for
+// batches).  1024 = 32 x 32: 32 lanes transform one column, each lane holding
+// 32 values; a wave takes two columns and a work-group of four waves one
+// panel of kPW = 8 columns.  Forward transform, element n = n1 + 32 n2 in
+// lane n1, register n2:
+//   A: Y[n1][k1] = DFT32 over n2 (registers), times W1024^(n1 k1);
+//   exchange through LDS (lane k1 gathers Y[.][k1]);
+//   B: X[k1 + 32 k2] = DFT32 over n1 (registers) in lane k1, register k2.
+// The spectrum multiply runs on that layout, and the inverse retraces the
+// steps with conjugate twiddles, ending with x[n1 + 32 n2] in lane n1.  One
+// LDS exchange per direction (8 KB per column, XOR-swizzled so that both its
+// writes and its reads are free of bank conflicts) replaces the eight
+// LDS-staged radix stages of k_pass2.
+// ---------------------------------------------------------------------------
+__device__ __constant__ float2 kW32[22] = {
+    {1.000000000f, 0.000000000f},   {0.980785280f, -0.195090322f},  {0.923879533f, -0.382683432f},
+    {0.831469612f, -0.555570233f},  {0.707106781f, -0.707106781f},  {0.555570233f, -0.831469612f},
+    {0.382683432f, -0.923879533f},  {0.195090322f, -0.980785280f},  {0.000000000f, -1.000000000f},
+    {-0.195090322f, -0.980785280f}, {-0.382683432f, -0.923879533f}, {-0.555570233f, -0.831469612f},
+    {-0.707106781f, -0.707106781f}, {-0.831469612f, -0.555570233f}, {-0.923879533f, -0.382683432f},
+    {-0.980785280f, -0.195090322f}, {-1.000000000f, 0.000000000f},  {-0.980785280f, 0.195090322f},
+    {-0.923879533f, 0.382683432f},  {-0.831469612f, 0.555570233f},  {-0.707106781f, 0.707106781f},
+    {-0.555570233f, 0.831469612f}};  // exp(-2 pi i q / 32), q = 0..21
+
+// v[k] <- sum_n v[n] exp(SIGN 2 pi i n k / 32): with n = 4 m + p, four 8-point
+// DFTs over m, the twiddles W32^(p k1), then eight 4-point DFTs over p give
+// X[k1 + 8 k2].
+template <int SIGN>
+__device__ inline void dft32(float2* v) {
+  float2 a[4][8];
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+#pragma unroll
+    for (int m = 0; m < 8; ++m) a[p][m] = v[4 * m + p];
+    dft8<float, SIGN>(a[p]);
+  }
+#pragma unroll
+  for (int p = 1; p < 4; ++p)
+#pragma unroll
+    for (int k1 = 1; k1 < 8; ++k1) {
+      float2 w = kW32[p * k1];
+      if (SIGN > 0) w.y = -w.y;
+      a[p][k1] = cmul(a[p][k1], w);
+    }
+#pragma unroll
+  for (int k1 = 0; k1 < 8; ++k1) {
+    float2 b[4] = {a[0][k1], a[1][k1], a[2][k1], a[3][k1]};
+    dft4<float, SIGN>(b);
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) v[k1 + 8 * k2] = b[k2];
+  }
+}
+
+// Lane `lane` of a 32-lane column group: registers v[r] -> X[lane][r] in the
+// group's LDS region L, then reads back v[r] = X[r][lane] (the transpose).
+__device__ inline void xchg32(float2* __restrict__ L, float2* v, int lane) {
+#pragma unroll
+  for (int r = 0; r < 32; ++r) L[r * 32 + (lane ^ r)] = v[r];
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 32; ++r) v[r] = L[lane * 32 + (r ^ lane)];
+  __syncthreads();
+}
+
+template <typename G>
+__global__ void __launch_bounds__(256)
+k_pass2w(float2* __restrict__ Tm, const G* __restrict__ gq, int N, const float2* __restrict__ tw,
+         const int* __restrict__ done, BatchStrides bs) {
+  constexpr int M = 1024;
+  constexpr int H = M / 2 + 1;
+  __shared__ float2 lds[8][1024];
+  Tm += (long)blockIdx.y * bs.T;
+  done = boff(done, (long)blockIdx.y * bs.st);
+  const int panel = xcd_tile(blockIdx.x, gridDim.x);
+  const int grp = threadIdx.x >> 5;  // column of the panel
+  const int lane = threadIdx.x & 31;
+  const int col = panel * kPW + grp;
+  float2* L = lds[grp];
+  // The panel (N rows x 8 columns, contiguous in tpos() layout) is staged in
+  // LDS with a row stride of 9 values (coalesced global loads; the column
+  // reads that follow fall in distinct banks), then each lane takes rows
+  // n = lane + 32 r of its column (zero from N on).
+  float2* S = &lds[0][0];
+  constexpr int kRS = kPW + 1;
+  const float2* src = Tm + ((long)panel * N) * kPW;
+  for (int e = threadIdx.x; e < N * kPW; e += 256) S[(e / kPW) * kRS + e % kPW] = src[e];
+  __syncthreads();
+  if (done && *done) return;  // uniform over the work-group
+  float2 v[32];
+#pragma unroll
+  for (int r = 0; r < 32; ++r) {
+    const int n = lane + 32 * r;
+    v[r] = n < N ? S[n * kRS + grp] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();  // the staging area is reused by the exchanges
+  // forward: A over registers, twiddle, exchange, B
+  dft32<-1>(v);
+#pragma unroll
+  for (int r = 1; r < 32; ++r) v[r] = cmul(v[r], tw[lane * r]);
+  xchg32(L, v, lane);
+  dft32<-1>(v);
+  // lane k1 holds X[k1 + 32 k2] in v[k2]: multiply by the spectrum column
+  const int cq = col <= M - col ? col : M - col;
+  const G* grow = gq + (long)cq * H;
+#pragma unroll
+  for (int r = 0; r < 32; ++r) {
+    const int k = lane + 32 * r;
+    const int fr = k <= M - k ? k : M - k;
+    const G g = grow[fr];
+    v[r] = cmul(v[r], make_float2((float)g.x, (float)g.y));
+  }
+  // inverse: A' over registers (k2), conjugate twiddle, exchange, B' (k1)
+  dft32<1>(v);
+#pragma unroll
+  for (int r = 1; r < 32; ++r) {
+    float2 w = tw[lane * r];
+    w.y = -w.y;
+    v[r] = cmul(v[r], w);
+  }
+  xchg32(L, v, lane);
+  dft32<1>(v);
+  // back through the same staging layout for coalesced stores
+#pragma unroll
+  for (int r = 0; r < 32; ++r) {
+    const int n = lane + 32 * r;
+    if (n < N) S[n * kRS + grp] = v[r];
+  }
+  __syncthreads();
+  float2* dst = Tm + ((long)panel * N) * kPW;
+  for (int e = threadIdx.x; e < N * kPW; e += 256) dst[e] = S[(e / kPW) * kRS + e % kPW];
+}
+
+// Passes 1 and 3 for M = 1024 in registers, with the same 32 x 32 scheme:
+// 32 lanes per row, eight rows per work-group.  Pass 1 loads row i of
+// chi .* x (lane n1, register n2: element n1 + 32 n2), transforms it forward
+// and writes T[i][k1 + 32 k2] from lane k1, register k2.  Pass 3 reads that
+// layout, transforms it back, and lane n1 combines element n1 + 32 n2 of the
+// row with x.  Only the plain operator modes (y = x - G(chi x), x - chi G(x),
+// G x) without input scaling take these kernels.
+__global__ void __launch_bounds__(256)
+k_pass1w(const float2* __restrict__ x, const float* __restrict__ chi, int mul, float2* __restrict__ Tm, int N,
+         const float2* __restrict__ tw, const int* __restrict__ done, BatchStrides bs) {
+  __shared__ float2 lds[8][1024];
+  {
+    const long kb = blockIdx.y;
+    x += kb * bs.x;
+    if (chi) chi += kb * bs.chi;
+    Tm += kb * bs.T;
+    done = boff(done, kb * bs.st);
+  }
+  const int grp = threadIdx.x >> 5, lane = threadIdx.x & 31;
+  const int row = blockIdx.x * 8 + grp;
+  const bool live = row < N;
+  const long rb = (long)row * N;
+  float2 v[32];
+#pragma unroll
+  for (int r = 0; r < 32; ++r) {
+    const int n = lane + 32 * r;
+    float2 e = make_float2(0.f, 0.f);
+    if (live && n < N) {
+      e = x[rb + n];
+      if (mul) {
+        const float c = chi[rb + n];
+        e.x *= c;
+        e.y *= c;
+      }
+    }
+    v[r] = e;
+  }
+  if (done && *done) return;  // uniform over the work-group
+  dft32<-1>(v);
+#pragma unroll
+  for (int r = 1; r < 32; ++r) v[r] = cmul(v[r], tw[lane * r]);
+  xchg32(lds[grp], v, lane);
+  dft32<-1>(v);
+  if (live) {
+#pragma unroll
+    for (int r = 0; r < 32; ++r) Tm[tpos(row, lane + 32 * r, N)] = v[r];
+  }
+}
+
+__global__ void __launch_bounds__(256)
+k_pass3w(const float2* __restrict__ Tm, const float2* __restrict__ x, const float* __restrict__ chi,
+         float2* __restrict__ y, int mode, int N, float inv, const float2* __restrict__ tw,
+         const int* __restrict__ done, BatchStrides bs) {
+  __shared__ float2 lds[8][1024];
+  {
+    const long kb = blockIdx.y;
+    Tm += kb * bs.T;
+    x += kb * bs.x;
+    if (chi) chi += kb * bs.chi;
+    y += kb * bs.y;
+    done = boff(done, kb * bs.st);
+  }
+  const int grp = threadIdx.x >> 5, lane = threadIdx.x & 31;
+  const int row = blockIdx.x * 8 + grp;
+  const bool live = row < N;
+  float2 v[32];
+#pragma unroll
+  for (int r = 0; r < 32; ++r) v[r] = live ? Tm[tpos(row, lane + 32 * r, N)] : make_float2(0.f, 0.f);
+  if (done && *done) return;
+  dft32<1>(v);
+#pragma unroll
+  for (int r = 1; r < 32; ++r) {
+    float2 w = tw[lane * r];
+    w.y = -w.y;
+    v[r] = cmul(v[r], w);
+  }
+  xchg32(lds[grp], v, lane);
+  dft32<1>(v);
+  if (!live) return;
+  const long rb = (long)row * N;
+#pragma unroll
+  for (int r = 0; r < 32; ++r) {
+    const int n = lane + 32 * r;
+    if (n < N) {
+      const float2 g = cscale(v[r], inv);
+      float2 o;
+      if (mode == kModeForward) o = csub(x[rb + n], g);
+      else if (mode == kModeAdjoint) o = csub(x[rb + n], cscale(g, chi[rb + n]));
+      else o = g;
+      y[rb + n] = o;
+    }
+  }
+}
+
 template <typename T, int M>
 __global__ void __launch_bounds__((PassWG<T, M>::value))
 k_pass3(const cplx<T>* __restrict__ Tm, const cplx<T>* __restrict__ x, const T* __restrict__ chi,
@@ -372,6 +599,14 @@ int launch_pass1(const Plan& P, hipStream_t st, const cplx<T>* x, cplx<T>* xw, c
                  const double* norm_ptr, int mode, cplx<T>* Tm, const cplx<T>* tw, const int* done,
                  int B, const BatchStrides& bs) {
   const int M = P.M, N = P.N;
+  if constexpr (sizeof(T) == 4) {
+    if (M == 1024 && P.wide1_batch > 0 && B >= P.wide1_batch && !xw && !norm_ptr &&
+        (mode == kModeForward || mode == kModeAdjoint || mode == kModeGreen)) {
+      hipLaunchKernelGGL(k_pass1w, dim3((N + 7) / 8, B), dim3(256), 0, st, (const float2*)x, (const float*)chi,
+                         (int)(mode == kModeForward), (float2*)Tm, N, (const float2*)tw, done, bs);
+      return hipGetLastError() == hipSuccess ? CGM_SUCCESS : CGM_ERROR_DEVICE;
+    }
+  }
   const int rows = P.rows1;
   const int grid = (N + rows - 1) / rows;
   const int threads = rows * fft_threads(M);
@@ -394,6 +629,13 @@ template <typename T, typename G>
 int launch_pass2(const Plan& P, hipStream_t st, cplx<T>* Tm, const G* gq, const cplx<T>* tw,
                  const int* done, int B, const BatchStrides& bs) {
   const int M = P.M, N = P.N;
+  if constexpr (sizeof(T) == 4) {
+    if (M == 1024 && P.wide2_batch > 0 && B >= P.wide2_batch) {
+      hipLaunchKernelGGL((k_pass2w<G>), dim3(M / kPW, B), dim3(256), 0, st, (float2*)Tm, gq, N, (const float2*)tw,
+                         done, bs);
+      return hipGetLastError() == hipSuccess ? CGM_SUCCESS : CGM_ERROR_DEVICE;
+    }
+  }
   const int cols = P.cols2;
   const int grid = M / cols;
   const int threads = cols * fft_threads(M);
@@ -417,6 +659,15 @@ int launch_pass3(const Plan& P, hipStream_t st, const cplx<T>* Tm, const cplx<T>
                  const cplx<T>* b, cplx<T>* y, int mode, const cplx<T>* tw, const int* done,
                  int B, const BatchStrides& bs) {
   const int M = P.M, N = P.N;
+  if constexpr (sizeof(T) == 4) {
+    if (M == 1024 && P.wide3_batch > 0 && B >= P.wide3_batch && !b &&
+        (mode == kModeForward || mode == kModeAdjoint || mode == kModeGreen)) {
+      hipLaunchKernelGGL(k_pass3w, dim3((N + 7) / 8, B), dim3(256), 0, st, (const float2*)Tm, (const float2*)x,
+                         (const float*)chi, (float2*)y, mode, N, (float)(1.0 / ((double)M * (double)M)),
+                         (const float2*)tw, done, bs);
+      return hipGetLastError() == hipSuccess ? CGM_SUCCESS : CGM_ERROR_DEVICE;
+    }
+  }
   const int rows = P.rows3;
   const int grid = (N + rows - 1) / rows;
   const int threads = rows * fft_threads(M);
@@ -463,6 +714,7 @@ int plan_init(Plan& P, int N) {
   P.cols2 = groups_for(P.M, 256);
   while (P.M % P.cols2) P.cols2 /= 2;
   if (P.cols2 < 1) P.cols2 = 1;
+  P.wide1_batch = P.wide2_batch = P.wide3_batch = 0;
   (void)nt;
   return CGM_SUCCESS;
 }

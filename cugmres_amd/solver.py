@@ -64,6 +64,21 @@ class Context:
     def autotune(self):
         check(lib().cgm_autotune(self._ptr), "cgm_autotune")
 
+    def radix_plan(self):
+        """The autotuner's radix plan at M = 1024: {'wide_from_batch': [pass 1, 2, 3],
+        'radix32_us' / 'radix8_us': [pass][batch 1, 4, 16]}."""
+        wb = (_c.c_int * 3)()
+        w = (_c.c_float * 9)()
+        l_ = (_c.c_float * 9)()
+        check(lib().cgm_get_radix_plan(self._ptr, wb, w, l_), "cgm_get_radix_plan")
+        return {"wide_from_batch": list(wb), "radix32_us": [[round(w[3 * q + k], 2) for k in range(3)] for q in range(3)],
+                "radix8_us": [[round(l_[3 * q + k], 2) for k in range(3)] for q in range(3)]}
+
+    def set_radix_plan(self, wide_from_batch):
+        """wide_from_batch: one int for all passes or a list of three."""
+        v = wide_from_batch if isinstance(wide_from_batch, (list, tuple)) else [wide_from_batch] * 3
+        check(lib().cgm_set_radix_plan(self._ptr, (_c.c_int * 3)(*[int(x) for x in v])), "cgm_set_radix_plan")
+
     def set_flexible_fp64(self, gamma64):
         """Long-recurrence flexible solves form A z_j and their residuals with
         the FP64 passes and this complex128 spectrum (None: complex64 passes).

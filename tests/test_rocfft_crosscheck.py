@@ -57,3 +57,37 @@ def test_matvec_matches_rocfft(N):
         err = rel(y.cpu().numpy(), ref.cpu().numpy())
         assert err < 2e-6, (N, mode, err)
     ctx.close()
+
+
+@pytest.mark.parametrize("wide", [0, 1])
+def test_batched_matvec_matches_rocfft_with_both_radix_plans(wide):
+    """N = 512 (M = 1024): the LDS-staged radix-8 passes and the
+    register-resident radix-32 x 32 passes (forced by cgm_set_radix_plan), in
+    batches of 1, 4 and 16 vectors with their own maps, all three modes,
+    against the same product formed by torch.fft in complex128.  Also prints
+    the autotuner's own choice."""
+    import numpy as np
+    import torch
+    from cugmres_amd import problems, solver
+    N = 512
+    ctx = solver.Context(N, 0)
+    print("radix plan", ctx.radix_plan(), flush=True)
+    ctx.set_radix_plan(wide)
+    g = ctx.spectrum_helmholtz(problems.KH)
+    blk = ctx.block_helmholtz(problems.KH)
+    for B in (1, 4, 16):
+        print(f"batched matvec N={N} B={B} wide={wide}", flush=True)
+        chis = np.stack([problems.config_plain(N, seed=70 + k)[0].reshape(-1) for k in range(B)])
+        c32 = ctx.to_device(chis, "f32").reshape(B, N * N)
+        rng = np.random.default_rng(B)
+        x32 = torch.as_tensor((rng.standard_normal((B, N * N)) + 1j * rng.standard_normal((B, N * N)))
+                              .astype(np.complex64)).to(ctx.device)
+        for mode in (0, 1, 2):
+            y = ctx.matvec_batched(g, c32, x32, mode).cpu().numpy()
+            for k in (0, B - 1):
+                ref = _torch_matvec(torch, blk, c32[k].to(torch.float64), x32[k].to(torch.complex128), N, ctx.M,
+                                    mode)
+                err = rel(y[k], ref.cpu().numpy())
+                assert err < 2e-6, (B, mode, k, err)
+    ctx.close()
+
