@@ -266,13 +266,15 @@ def run_bench(a, ctx):
         # pooled maps: 8 batches of 8 (64 maps) up to 32 x 32 tiles, 2 batches with
         # 64 x 64 tiles (13 GB of inverses per batch at N = 256)
         tnet, hist = learn.train(tctx, samples=a.train_samples, batch=8, lr=1e-3, seed=0, tiles_T=a.tile,
-                                 map_pool=8 if a.tile <= 32 else 2, log=note, log_every=200)
+                                 map_pool=8 if a.tile <= 32 else 2, log=note, log_every=200, val_every=50)
         torch.cuda.synchronize()
         note(f"trained in {time.perf_counter() - t0:.1f} s")
         params = tnet.params.cpu().numpy()
         training = {"N": a.train_N, "samples": a.train_samples, "tiles": a.tile,
                     "seconds": round(time.perf_counter() - t0, 1), "loss_first": hist[0][1],
-                    "loss_last100": float(np.mean([h[1] for h in hist[-100:]]))}
+                    "loss_last100": float(np.mean([h[1] for h in hist[-100:]])),
+                    "validation_best": min(c[1] for c in tnet.validation), "kept_step": tnet.best_step,
+                    "validation_last": tnet.validation[-1][1]}
         tnet.close()
         tctx.close()
     tl = tiles.Tiles(ctx, a.tile, 1)

@@ -137,8 +137,36 @@ class Adam:
         p.addcdiv_(mh, vh.sqrt_().add_(self.eps), value=-lr)
 
 
+def validation_set(ctx, gamma, batch, tiles_T=0, seed=0, chain=3):
+    """Held-out (maps, vectors) for validation: maps and vectors from seeds
+    that training never draws (2e9 + ...).  With tiles_T the vectors are the
+    normalised residuals the tile inverses leave, sample b taken b % (chain + 1)
+    steps down the Richardson chain of the tile inverses alone (no network),
+    so the set stays fixed while the network trains.  Returns (chi, v, tiles)."""
+    t = ctx.torch
+    seeds = [2_000_000_000 + seed * 1_000_003 + b for b in range(batch)]
+    chi_d = t.from_numpy(sample_maps(ctx.N, seeds).reshape(batch, -1)).to(ctx.device)
+    v = Sampler(ctx, gamma).batch(chi_d, seeds[0])
+    tl = None
+    if tiles_T:
+        from .tiles import Tiles, tile_residual
+        tl = Tiles(ctx, tiles_T, batch).factor(ctx.block_helmholtz(problems.KH), chi_d)
+        for d in range(chain):
+            r = tile_residual(ctx, gamma, tl, chi_d, v.contiguous())[1]
+            for b in range(batch):
+                if b % (chain + 1) > d:
+                    v[b] = r[b]
+        v = tile_residual(ctx, gamma, tl, chi_d, v.contiguous())[1].contiguous()
+    return chi_d, v, tl
+
+
+def validation_loss(ctx, gamma, net, chi_d, v):
+    z = net.forward(chi_d, v)
+    return residual_loss_grad(ctx, gamma, chi_d, v, z)[0]
+
+
 def train(ctx, samples=20000, batch=8, lr=1e-3, seed=0, params=None, log=None, log_every=100,
-          clip=1.0, identity_weight=0.0, chain=3, tiles_T=0, map_pool=8):
+          clip=1.0, identity_weight=0.0, chain=3, tiles_T=0, map_pool=8, val_every=0):
     """Trains a Preconditioner on `samples` seeded (map, vector) pairs at the
     context's N, starting from `params` or from init_params(seed) with the
     identity path weighted by identity_weight.  Step k trains on vectors
@@ -148,6 +176,9 @@ def train(ctx, samples=20000, batch=8, lr=1e-3, seed=0, params=None, log=None, l
     maps leave; the maps then come from a pool of map_pool batches whose
     inverses are factored once (step k uses batch k % map_pool; 0 = new maps
     factored at every step), while the vectors stay new at every step.
+    With val_every > 0 the loss on a fixed held-out set (validation_set) is
+    evaluated every val_every steps and the parameters of its lowest value are
+    the ones returned (net.validation holds the curve, net.best_step the step).
     Returns (Preconditioner, history of (step, loss))."""
     t = ctx.torch
     gamma = ctx.spectrum_helmholtz(problems.KH)
@@ -173,6 +204,11 @@ def train(ctx, samples=20000, batch=8, lr=1e-3, seed=0, params=None, log=None, l
             tl = Tiles(ctx, tiles_T, batch)
     steps = max(1, samples // batch)
     hist = []
+    val = None
+    best = (float("inf"), None, -1)
+    if val_every:
+        val = validation_set(ctx, gamma, batch, tiles_T, seed, chain)
+        net.validation = []
     t0 = time.time()
     for k in range(steps):
         seeds = [seed * 1_000_003 + k * batch + b for b in range(batch)]
@@ -199,8 +235,22 @@ def train(ctx, samples=20000, batch=8, lr=1e-3, seed=0, params=None, log=None, l
         opt.step(p, g, cur)
         net.set_params(p)
         hist.append((k, L))
+        if val is not None and ((k + 1) % val_every == 0 or k == steps - 1):
+            Lv = validation_loss(ctx, gamma, net, val[0], val[1])
+            net.validation.append((k + 1, Lv))
+            if Lv < best[0]:
+                best = (Lv, p.clone(), k + 1)
         if log and (k % log_every == 0 or k == steps - 1):
-            log(f"step {k:6d}/{steps} loss {L:.4e} |g| {gn:.3e} lr {cur:.2e} ({time.time() - t0:.1f} s)")
+            vtxt = f" validation {net.validation[-1][1]:.4e}" if val is not None and net.validation else ""
+            log(f"step {k:6d}/{steps} loss {L:.4e}{vtxt} |g| {gn:.3e} lr {cur:.2e} ({time.time() - t0:.1f} s)")
+    if best[1] is not None:
+        # the validation-based stop: the parameters of the best held-out loss
+        net.set_params(best[1])
+        net.best_step = best[2]
+        if log:
+            log(f"kept the parameters of step {best[2]} (validation loss {best[0]:.4e})")
+    if val is not None and val[2] is not None:
+        val[2].close()
     for _, tq in pool:
         tq.close()
     if tl is not None and not pool:
@@ -223,13 +273,15 @@ def main():
                     help="train the network as the second level after T x T tile inverses (0: alone)")
     ap.add_argument("--map-pool", type=int, default=8,
                     help="with --tiles: batches of maps whose tile inverses are factored once and reused")
+    ap.add_argument("--val-every", type=int, default=50,
+                    help="evaluate a held-out set every this many steps and keep the best parameters (0: off)")
     ap.add_argument("--out", default="fno_params.npz")
     a = ap.parse_args()
     from .solver import Context
     ctx = Context(a.N)
     net, hist = train(ctx, a.samples, a.batch, a.lr, a.seed, log=lambda s: print(s, flush=True),
                       identity_weight=a.identity_weight, chain=a.chain, tiles_T=a.tiles,
-                      map_pool=a.map_pool)
+                      map_pool=a.map_pool, val_every=a.val_every)
     fno.save(a.out, net.params.cpu().numpy(), N=a.N, samples=a.samples, seed=a.seed,
              loss=np.array([h[1] for h in hist], np.float32))
     print(f"saved {a.out}")

@@ -290,3 +290,28 @@ def test_trained_network_gains_over_plain_gmres(gpu, orc):
         r = b.reshape(-1) - op.matvec(chi.reshape(-1), x.cpu().numpy().astype(np.complex128))
         assert abs(np.linalg.norm(r) / np.linalg.norm(b) - il["rel_residual"]) < 1e-3 * il["rel_residual"] + 1e-6
     assert min(gains) > 2.0, gains
+
+
+@pytest.mark.gpu
+def test_validation_based_stop_keeps_the_best_held_out_parameters(gpu):
+    """Two-level training (16 x 16 tiles, N = 64) with a held-out set evaluated
+    every 20 steps: the returned network's held-out loss is the lowest of the
+    curve, it belongs to the recorded step, and training lowered it below the
+    untrained network's."""
+    solver, fno = gpu
+    from cugmres_amd import learn, problems
+    N = 64
+    ctx = solver.Context(N)
+    net, hist = learn.train(ctx, samples=800, batch=8, lr=1e-3, seed=5, tiles_T=16, map_pool=2, val_every=20)
+    curve = net.validation
+    assert len(curve) == 5, curve
+    g = ctx.spectrum_helmholtz(problems.KH)
+    chi_v, v_v, tl = learn.validation_set(ctx, g, 8, 16, 5, 3)
+    kept = learn.validation_loss(ctx, g, net, chi_v, v_v)
+    best_step, best_loss = min(curve, key=lambda c: c[1])
+    assert net.best_step == best_step
+    assert kept == pytest.approx(best_loss, rel=1e-5)
+    untrained = fno.Preconditioner(ctx, batch=8, params=fno.init_params(5, zero_out=True))
+    assert kept < 0.9 * learn.validation_loss(ctx, g, untrained, chi_v, v_v)
+    tl.close()
+
