@@ -91,23 +91,37 @@ __device__ inline float2 cj_mul(float2 a, float2 b) {  // conj(a) b
   return make_float2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
 }
 
+// Streaming loads of the Krylov basis: each basis value is read once per
+// pass and not again before the next step, so the loads carry the
+// non-temporal hint.
+typedef float f2v_t __attribute__((ext_vector_type(2)));
+typedef float f4v_t __attribute__((ext_vector_type(4)));
+__device__ inline float2 nt_load(const float2* p) {
+  const f2v_t v = __builtin_nontemporal_load(reinterpret_cast<const f2v_t*>(p));
+  return make_float2(v.x, v.y);
+}
+__device__ inline float4 nt_load(const float4* p) {
+  const f4v_t v = __builtin_nontemporal_load(reinterpret_cast<const f4v_t*>(p));
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+
 // Elements 2p and 2p+1 of a complex vector of n elements: one 16-byte load
 // when n is even (every row of the basis then starts 16-byte aligned), two
 // 8-byte loads otherwise; zero beyond n.
-template <bool kVec>
+template <bool kVec, bool kNT = false>
 __device__ inline void load_pair(const float2* __restrict__ v, long p, long n, float2& lo, float2& hi) {
   const long e = 2 * p;
   if (kVec) {
     if (e < n) {
-      const float4 q = *reinterpret_cast<const float4*>(v + e);
+      const float4 q = kNT ? nt_load(reinterpret_cast<const float4*>(v + e)) : *reinterpret_cast<const float4*>(v + e);
       lo = make_float2(q.x, q.y);
       hi = make_float2(q.z, q.w);
     } else {
       lo = hi = make_float2(0.f, 0.f);
     }
   } else {
-    lo = e < n ? v[e] : make_float2(0.f, 0.f);
-    hi = e + 1 < n ? v[e + 1] : make_float2(0.f, 0.f);
+    lo = e < n ? (kNT ? nt_load(v + e) : v[e]) : make_float2(0.f, 0.f);
+    hi = e + 1 < n ? (kNT ? nt_load(v + e + 1) : v[e + 1]) : make_float2(0.f, 0.f);
   }
 }
 
@@ -174,7 +188,7 @@ __global__ void __launch_bounds__(kBS) k_lr_proj(LrArgs a, int with_w) {
         const float2* q = Q + (long)(v0 + v) * a.n;
         float2 qe[kPE];
 #pragma unroll
-        for (int k = 0; k < kPP; ++k) load_pair<kVec>(q, pbase + (long)k * kBS, a.n, qe[2 * k], qe[2 * k + 1]);
+        for (int k = 0; k < kPP; ++k) load_pair<kVec, true>(q, pbase + (long)k * kBS, a.n, qe[2 * k], qe[2 * k + 1]);
 #pragma unroll
         for (int b4 = 0; b4 < kPE; b4 += 4) {
           float sr = 0.f, si = 0.f;
@@ -511,7 +525,7 @@ __global__ void __launch_bounds__(kBS) k_lr_upd(LrArgs a) {
       for (; i + kUU <= cnt; i += kUU) {
         float2 q[kUU];
 #pragma unroll
-        for (int u = 0; u < kUU; ++u) q[u] = Q[(long)(i0 + i + u) * a.n + e];
+        for (int u = 0; u < kUU; ++u) q[u] = nt_load(&Q[(long)(i0 + i + u) * a.n + e]);
 #pragma unroll
         for (int u = 0; u < kUU; ++u) {
           const double2 k1 = c1[i + u], k2 = c2[i + u];
@@ -523,7 +537,7 @@ __global__ void __launch_bounds__(kBS) k_lr_upd(LrArgs a) {
         }
       }
       for (; i < cnt; ++i) {
-        const float2 qv = Q[(long)(i0 + i) * a.n + e];
+        const float2 qv = nt_load(&Q[(long)(i0 + i) * a.n + e]);
         const double2 k1 = c1[i], k2 = c2[i];
         a1x += k1.x * qv.x - k1.y * qv.y;
         a1y += k1.x * qv.y + k1.y * qv.x;

@@ -215,7 +215,12 @@ k_tile_apply_h(const __half2* __restrict__ K16, const float* __restrict__ ksc, l
   for (int q0 = 0; q0 < kTileH; q0 += kU) {
     uint2 k[kU];
 #pragma unroll
-    for (int u = 0; u < kU; ++u) k[u] = Kt[(long)(q0 + u) * (kTileH / 2)];
+    for (int u = 0; u < kU; ++u) {
+      // the inverses stream once per application: non-temporal loads
+      typedef unsigned int u2v_t __attribute__((ext_vector_type(2)));
+      const u2v_t w = __builtin_nontemporal_load(reinterpret_cast<const u2v_t*>(Kt + (long)(q0 + u) * (kTileH / 2)));
+      k[u] = make_uint2(w.x, w.y);
+    }
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
       const float2 xv = xs[q0 + u];
