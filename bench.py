@@ -600,8 +600,8 @@ def profile_longrec(ctx, j, reps=20):
 
 def profile_tiles(ctx, tl, v, reps=20):
     """Duration of one application of the tile inverses (torch events on the
-    context's stream) with its algorithmic bytes: every inverse once (T^4 x 8
-    B per tile) plus the vector in and out."""
+    context's stream) with its algorithmic bytes (cgm_tiles_apply_bytes: the
+    inverse columns streamed plus the vector in and out)."""
     import torch
     y = torch.empty_like(v)
     for _ in range(3):
@@ -614,9 +614,11 @@ def profile_tiles(ctx, tl, v, reps=20):
     torch.cuda.synchronize()
     us = e0.elapsed_time(e1) / reps * 1e3
     N, T = ctx.N, tl.T
-    b = (N // T) ** 2 * T ** 4 * 8 + 16 * N * N
-    return {"kernel": "k_tile_apply", "us": us, "bytes": b, "GBps": b / (us * 1e-6) / 1e9,
-            "per_unit": f"{b} B per launch: (N/T)^2 T^4 x 8 B + 16 N^2 B, N = {N}, T = {T}"}
+    b = tl.apply_bytes(1)
+    full = (N // T) ** 2 * T ** 4 * (4 if T == 64 else 8)
+    return {"kernel": "k_tile_apply_h" if T == 64 else "k_tile_apply", "us": us, "bytes": b,
+            "GBps": b / (us * 1e-6) / 1e9, "kept_fraction": (b - 16 * N * N) / full,
+            "per_unit": f"{b:.0f} B per launch: kept inverse columns + 16 N^2 B, N = {N}, T = {T}"}
 
 
 def batched_throughput(restart, cycles=3, B=16):

@@ -125,6 +125,7 @@ SIGNATURES = {
     "cgm_tiles_create": (ctypes.c_int, [c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(c_void_p)]),
     "cgm_tiles_destroy": (ctypes.c_int, [c_void_p]),
     "cgm_tiles_memory": (ctypes.c_int, [c_void_p, ctypes.POINTER(ctypes.c_size_t)]),
+    "cgm_tiles_apply_bytes": (ctypes.c_int, [c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
     "cgm_tiles_factor": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_long, ctypes.c_int]),
     "cgm_tiles_apply": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_int]),
     "cgm_set_flexible_fp64": (ctypes.c_int, [c_void_p, c_void_p]),

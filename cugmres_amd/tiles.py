@@ -35,6 +35,12 @@ class Tiles:
         check(lib().cgm_tiles_memory(self._h, ctypes.byref(b)), "cgm_tiles_memory")
         return b.value
 
+    def apply_bytes(self, batch=1):
+        """Algorithmic bytes of one application to `batch` systems (after factor)."""
+        b = ctypes.c_double()
+        check(lib().cgm_tiles_apply_bytes(self._h, int(batch), ctypes.byref(b)), "cgm_tiles_apply_bytes")
+        return b.value
+
     def factor(self, block, chi):
         """Invert every tile's operator: block is the N x N complex128 generator
         block on the device (Context.block_helmholtz), chi is (N*N,) for one map

@@ -330,8 +330,11 @@ int cgm_fgmres_batched(cgm_ctx_t ctx, const cgm_complex64* dev_gamma, const floa
  * with g the N x N generator block of the operator.  The inverses are formed
  * once per map in FP32 complex (batched LU with partial pivoting) and take
  * T^4 * 8 bytes per tile: (N/T)^2 T^4 * 8 = 512 MiB per map at N = 512, T = 16.
- * With T = 64 they are applied from an FP16 copy scaled per tile (half the
- * bytes per application; 4 GiB more per map at N = 512, 12 GiB in all). */
+ * With T = 64 they are applied from an FP16 copy scaled per tile (4 GiB more
+ * per map at N = 512, 12 GiB in all) that holds only the columns of cells with
+ * chi != 0: a cell with chi = 0 has the unit vector as its column of the tile
+ * operator and therefore of its inverse, so the application adds x there and
+ * streams the other columns (about 3/8 of them on the benchmark maps). */
 typedef struct cgm_tiles* cgm_tiles_t;
 
 /* The generator block g[|di|, |dj|] (N*N complex128, device memory) of the
@@ -354,6 +357,10 @@ int cgm_tiles_factor(cgm_tiles_t tiles, const cgm_complex128* dev_block, const f
 /* y[b] = P x[b] for b < batch ([batch][N*N]; map b, or the one map).
  * Asynchronous on the context stream. */
 int cgm_tiles_apply(cgm_tiles_t tiles, const cgm_complex64* dev_x, cgm_complex64* dev_y, int batch);
+/* Algorithmic bytes of one cgm_tiles_apply over `batch` systems: the inverse
+ * entries it streams (every column of every tile in FP32; for T = 64 the kept
+ * columns in FP16, padded to 16 per tile) plus 16 * N*N per system. */
+int cgm_tiles_apply_bytes(cgm_tiles_t tiles, int batch, double* bytes);
 /* Diagnostics: fills the inverses of one map with synthetic values (no
  * factorisation) and returns in *ms the average device time of `reps`
  * applications to one vector (HIP events on the context stream).  The

@@ -92,6 +92,60 @@ def test_tiles_apply_matches_oracle(orc, N, T):
     ctx.close()
 
 
+def _sparse_pattern_map(N, seed):
+    """A map whose 64 x 64 tiles exercise the packed-column storage: one tile
+    with chi = 0 everywhere (no kept column), one with chi != 0 everywhere
+    (all 4,096 kept), one with a ragged count (37 cells, padded to 48) and one
+    cut from the benchmark's block map."""
+    from cugmres_amd import problems
+    rng = np.random.default_rng(seed)
+    chi = np.zeros((N, N))
+    chi[0:64, 64:128] = rng.uniform(0.1, 0.5, (64, 64))
+    r = rng.choice(4096, 37, replace=False)
+    t = np.zeros(4096)
+    t[r] = rng.uniform(0.5, 11.1, 37)
+    chi[64:128, 0:64] = t.reshape(64, 64)
+    eps = problems.block_eps_map(64, 8, 0.5, 0.75, rng=problems.SplitMix64(seed), eps_hi=12.1)
+    chi[64:128, 64:128] = eps - 1.0
+    return chi.reshape(-1)
+
+
+@pytest.mark.gpu
+def test_64_tiles_keep_only_nonzero_contrast_columns(orc):
+    """64 x 64 tiles stream only the inverse columns of cells with chi != 0:
+    two maps of the sparse-pattern kind against the oracle's dense inverses
+    (5e-3, FP16 storage); on the tile with chi = 0 everywhere the output is
+    the input bit for bit; the bytes per application are the padded kept
+    columns (counted here from chi) x 4096 x 4 B + 16 N^2 B per system."""
+    import torch
+    from cugmres_amd import problems, solver, tiles as gt
+    from oracle import tiles
+    N, T = 128, 64
+    print("tiles packed columns N=128 T=64", flush=True)
+    ctx = solver.Context(N, 0)
+    blk = ctx.block_helmholtz(problems.KH)
+    ref_blk = orc.baseline_green_block(N, problems.KH)
+    chis = [_sparse_pattern_map(N, s) for s in (1, 2)]
+    rng = np.random.default_rng(3)
+    V = (rng.standard_normal((2, N * N)) + 1j * rng.standard_normal((2, N * N))).astype(np.complex64)
+    tl = gt.Tiles(ctx, T, 2).factor(blk, torch.stack([ctx.to_device(c, "f32") for c in chis]))
+    y = tl.apply(torch.as_tensor(V).to(ctx.device)).cpu().numpy()
+    want = 0.0
+    for k in range(2):
+        yr = tiles.tile_apply(tiles.tile_inverses(ref_blk, chis[k], N, T), V[k].astype(np.complex128), N, T)
+        assert rel(y[k], yr) < 5e-3
+        assert np.array_equal(y[k].reshape(N, N)[0:64, 0:64], V[k].reshape(N, N)[0:64, 0:64])
+        c = chis[k].reshape(N, N)
+        for a in range(2):
+            for b in range(2):
+                nz = int(np.count_nonzero(c[64 * a:64 * a + 64, 64 * b:64 * b + 64]))
+                want += -(-nz // 16) * 16 * 4096 * 4.0
+        want += 16.0 * N * N
+    assert tl.apply_bytes(2) == want
+    tl.close()
+    ctx.close()
+
+
 @pytest.mark.gpu
 def test_tiles_several_maps_and_shared_map(orc):
     """Own maps (one inverse set per system) and one map with five right-hand
