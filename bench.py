@@ -60,7 +60,7 @@ def parse(argv=None):
     p.add_argument("--restart", type=int, default=30, help="restart of plain GMRES")
     p.add_argument("--tol", type=float, default=1e-6)
     p.add_argument("--max-cycles", type=int, default=100, help="cap of plain GMRES(restart)")
-    p.add_argument("--tile", type=int, default=64, help="tile side of the tile inverses (8, 16, 32, 64)")
+    p.add_argument("--tile", type=int, default=64, help="tile side of the tile inverses (8, 16, 32, 64, 128)")
     p.add_argument("--lr-restart", type=int, default=8192,
                    help="cycle length of the preconditioned solves (long-recurrence path)")
     p.add_argument("--lr-cycles", type=int, default=2)
@@ -616,7 +616,8 @@ def profile_tiles(ctx, tl, v, reps=20):
     N, T = ctx.N, tl.T
     b = tl.apply_bytes(1)
     full = (N // T) ** 2 * T ** 4 * (4 if T == 64 else 8)
-    return {"kernel": "k_tile_apply_h" if T == 64 else "k_tile_apply", "us": us, "bytes": b,
+    return {"kernel": "tile application (k_tile_w, FFT convolution, combine)" if T >= 64 else "k_tile_apply",
+            "us": us, "bytes": b,
             "GBps": b / (us * 1e-6) / 1e9, "kept_fraction": (b - 16 * N * N) / full,
             "per_unit": f"{b:.0f} B per launch: kept inverse columns + 16 N^2 B, N = {N}, T = {T}"}
 
@@ -694,19 +695,27 @@ def live_roofline(live, N, per_step, jmean, tile=64):
     per_launch = t["bytes"] / max(1, t["launches"])
     traffic = None
     if key == "tile":
-        kern = "k_tile_apply_h" if tile == 64 else "k_tile_apply"
-        ref = measured_traffic(f"{kern}_T{tile}", N)
-        traffic = ref
-        ent = 4 if tile == 64 else 8
-        per_unit = (f"(N/T)^2 T^4 x {ent} B of inverses + 16 B x N^2 per launch (T = {tile}, N = {N}): "
-                    f"{per_launch:.4g} B, {t['launches']} launches of the timed solves")
+        if tile >= 64:
+            kern = "k_tile_w"
+            ref = measured_traffic(f"{kern}_T{tile}", N)   # synthetic full tiles: scale by the bytes
+            if ref:
+                traffic = ref["bytes_per_launch"] * per_launch / ref["algorithmic_bytes"]
+            per_unit = (f"sum over tiles of n_t^2 x 4 B of W (FP16, n_t = cells with chi != 0 padded to 16) "
+                        f"+ 16 B x N^2 per launch (T = {tile}, N = {N}): {per_launch:.4g} B, "
+                        f"{t['launches']} launches of the timed solves")
+        else:
+            kern = "k_tile_apply"
+            ref = measured_traffic(f"{kern}_T{tile}", N)
+            traffic = ref["bytes_per_launch"] if ref else None
+            per_unit = (f"(N/T)^2 T^4 x 8 B of inverses + 16 B x N^2 per launch (T = {tile}, N = {N}): "
+                        f"{per_launch:.4g} B, {t['launches']} launches of the timed solves")
     else:
         kern = LIVE_KERNELS[key]
         extra = 2 if key == "proj" else 4
         jbar = per_launch / (8 * n) - extra
         ref = measured_traffic(kern + "_j4096", N)
         if ref:
-            traffic = ref * per_launch / (8 * n * (4096 + extra))
+            traffic = ref["bytes_per_launch"] * per_launch / (8 * n * (4096 + extra))
         per_unit = (f"8 B x N^2 x (j + {extra}) per launch at basis size j (N = {N}); mean j {jbar:.0f} "
                     f"over the {t['launches']} launches of the timed solves")
     return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
@@ -724,7 +733,8 @@ def measured_traffic(kernel, N, batch=1):
     """HBM bytes per launch of `kernel` at grid size N and `batch` vectors per
     launch as measured by rocprofv3 (FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00,
     copy-calibrated) and committed in profiles/hbm_traffic_n512.json (key
-    `kernel`, or `kernel`_b<batch> for batches); None when none matches."""
+    `kernel`, or `kernel`_b<batch> for batches): the entry (bytes_per_launch,
+    algorithmic_bytes, ...), or None when none matches."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic_n512.json")
     key = kernel if batch == 1 else f"{kernel}_b{batch}"
     try:
@@ -734,7 +744,7 @@ def measured_traffic(kernel, N, batch=1):
         return None
     if not t or t.get("N") != N:
         return None
-    return t["bytes_per_launch"]
+    return t
 
 
 def profile_fno(ctx, net, dchi, db, batch=1):

@@ -12,34 +12,44 @@
 // tile) the local operator is
 //     A_t = I - G_t diag(chi_t),   G_t[p, q] = g[|p_r - q_r|, |p_c - q_c|],
 // with g the N x N generator block of the operator (the block whose circulant
-// embedding gives the kernel spectrum).  Factorisation, once per map: every
-// A_t is formed in FP32 complex (column-major, T^2 x T^2) and inverted in
-// place by a batched LU with partial pivoting (rocSOLVER getrf + getri).
-// Application, once per Krylov step: y_t = A_t^{-1} x_t, one work-group per
-// (tile, system), one thread per output cell streaming its row of the
-// inverse column by column (coalesced: the inverse is column-major), the
-// tile's input staged in LDS.  A map shared by several right-hand sides is
-// factored once, and each work-group then applies its tile's inverse to all
-// of them while it streams it.
+// embedding gives the kernel spectrum).
 //
-// 64 x 64 tiles (4,096 unknowns each) store their inverses in FP16 complex
-// (half the bytes per step), each tile scaled by its largest magnitude so no
-// entry leaves FP16's range; the scale is applied to the FP32 sums.  A cell q
-// with chi_q = 0 makes column q of A_t the unit vector e_q, so column q of the
-// inverse is e_q as well: only the columns of cells with chi != 0 are stored
-// (packed in order) and streamed, and the others add x_q to y_q.  On the
-// benchmark maps that is about three columns in eight.  The application
-// splits every tile's rows over 8 work-groups.  On the benchmark
-// maps at N = 512 they halve the Krylov steps of 32 x 32 tiles (2,391 against
-// 4,573 on seed 0), and FP16 storage leaves the step count unchanged.
+// Tiles of side 8, 16, 32 (dense form): every A_t is formed in FP32 complex
+// (column-major, T^2 x T^2) and inverted in place by a batched LU with
+// partial pivoting (rocSOLVER getrf + getri), once per map.  Application, once
+// per Krylov step: y_t = A_t^{-1} x_t, one work-group per (tile, system), one
+// thread per output cell streaming its row of the inverse column by column
+// (coalesced: the inverse is column-major), the tile's input staged in LDS.
+// A map shared by several right-hand sides is factored once, and each
+// work-group then applies its tile's inverse to all of them while it streams
+// it.
+//
+// Tiles of side 64 and 128 (reduced form).  With S the cells of the tile
+// where chi != 0 and D_S = diag(chi_S), the columns of A_t outside S are unit
+// vectors, and
+//     A_t^{-1} = I + G_t[:, S] W_t R_S,   W_t = (I - D_S G_SS)^{-1} D_S
+// (R_S restricts to S).  Factorisation inverts only the |S| x |S| matrices
+// I - D_S G_SS (rocSOLVER, batched over tiles of one size class) and keeps
+// W_t in FP16 complex, scaled per tile by its largest magnitude.  Application:
+// u_t = W_t x_t[S] (the only large stream: |S|^2 entries per tile), then
+// G_t u_t for all tiles at once as a batch of T x T convolutions through the
+// operator's own FFT passes at size 2T with the tile's spectrum, then
+// y = x + G_t u_t.  On the benchmark maps 3/8 of the cells carry chi != 0, so
+// a 64 x 64 tile streams 14 % of the entries of its dense inverse, and the
+// LU work drops by about 20x; that makes 128 x 128 tiles affordable, which
+// take 1,032 Krylov steps on seed 0 at N = 512 against 2,380 for 64 x 64 and
+// 4,573 for 32 x 32.
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
+#include <algorithm>
+#include <map>
 #include <vector>
 
 #include "cgm_internal.h"
+#include "cgm_special.h"
 
 using namespace cgm;
 
@@ -49,23 +59,71 @@ struct cgm_tiles {
   int N = 0, T = 0, B = 0;  // grid, tile side, systems (maps) the workspace holds
   int nt = 0;               // tiles per grid side
   int M2 = 0;               // unknowns per tile (T * T)
-  float2* K = nullptr;      // [B][nt * nt][M2 * M2] inverses, column-major
-  __half2* K16 = nullptr;   // FP16 copy (T = 64): [B][nt * nt][column slot][M2] of K / kscale,
-                            // only the columns of cells with chi != 0 (the others are unit vectors)
-  float* kscale = nullptr;  // [B][nt * nt] largest |K| entry of each tile's kept columns (T = 64)
-  int* ncol = nullptr;      // [B][nt * nt] kept columns, padded to a multiple of 16 (T = 64)
-  unsigned short* cols = nullptr;  // [B][nt * nt][M2] column of each slot (T = 64)
-  unsigned int* zmask = nullptr;   // [B][nt * nt][M2 / 32] cells with chi = 0 (T = 64)
-  std::vector<double> kept;        // per map: kept columns over all tiles (host; bytes per application)
-  int* ipiv = nullptr;      // [B][nt * nt][M2]
-  int* info = nullptr;      // [B][nt * nt]
+  bool reduced = false;     // T >= 64: reduced form
   int maps = 0;             // maps factored by the last cgm_tiles_factor (1: shared)
   rocblas_handle h = nullptr;
+  size_t bytes = 0;         // device memory held
+  // dense form (T <= 32)
+  float2* K = nullptr;       // [B][nt * nt][M2 * M2] inverses, column-major
+  int* ipiv = nullptr;      // [B][nt * nt][M2]   (reduced form: factor workspace)
+  int* info = nullptr;      // [B][nt * nt]       (reduced form: factor workspace)
+  // reduced form (T >= 64)
+  int* ncol = nullptr;             // [B][nt * nt] |S_t| padded to a multiple of kPadTo
+  unsigned short* cols = nullptr;  // [B][nt * nt][M2] the cells of S_t in order, then kPadCol
+  unsigned int* zmask = nullptr;   // [B][nt * nt][M2 / 32] cells with chi = 0
+  long* woff = nullptr;            // [B][nt * nt] offset of W_t in W16 (half2 elements)
+  float* kscale = nullptr;         // [B][nt * nt] largest |W_t| entry
+  int* ids = nullptr;              // [B][nt * nt] tile indices of one size class (factor)
+  __half2* W16 = nullptr;          // packed W_t / kscale_t, column-major n_t x n_t
+  size_t w16_cap = 0;              // elements
+  float2* Kw = nullptr;            // FP32 matrices of one size class (factor)
+  size_t kw_cap = 0;               // elements
+  size_t piv_cap = 0;              // elements of ipiv (reduced form)
+  float2* U = nullptr;             // [sys][nt * nt][M2] u_t scattered on the tile (apply)
+  float2* Y = nullptr;             // [sys][nt * nt][M2] G_t u_t (apply)
+  float2* Tm = nullptr;            // [sys][nt * nt][T * 2T] FFT scratch (apply)
+  int scratch_sys = 0;             // systems the apply scratch holds
+  Plan P;                          // tile convolution: N' = T, M' = 2T
+  float2* tw32 = nullptr;          // twiddles of size M'
+  double2* tw64 = nullptr;
+  double2* blkT = nullptr;         // T x T corner of the generator block
+  float2* gq = nullptr;            // quarter spectrum of the tile convolution
+  std::vector<double> kept;        // per map: inverse entries streamed per application
+  int nmax = 0;                    // largest n_t over the factored maps
 };
 
 namespace {
 
-constexpr int kBuildChunk = 65536;  // matrix entries per work-group of k_tile_build
+constexpr int kBuildChunk = 65536;           // matrix entries per work-group of the build kernels
+constexpr int kPadTo = 16;                   // n_t is padded to a multiple of this
+constexpr unsigned short kPadCol = 0xFFFF;   // cell index of a padding slot
+constexpr int kClass = 256;                  // size classes of the reduced LU: multiples of this
+constexpr size_t kFactorBytes = 8ull << 30;  // FP32 workspace per batched LU call (reduced form)
+constexpr int kRowsW = 512;                  // rows of W per work-group of k_tile_w
+constexpr int kChunkW = 4096;                // columns of W staged per pass of k_tile_w
+
+template <typename P>
+int grow(cgm_tiles* t, P** p, size_t* cap, size_t need) {
+  if (need <= *cap) return CGM_SUCCESS;
+  if (*p) {
+    (void)hipFree(*p);
+    t->bytes -= *cap * sizeof(P);
+    *p = nullptr;
+    *cap = 0;
+  }
+  if (hipMalloc((void**)p, need * sizeof(P)) != hipSuccess) {
+    *p = nullptr;
+    cgm_set_last_error("tile preconditioner: allocation of %zu MiB failed", need * sizeof(P) >> 20);
+    return CGM_ERROR_ALLOC;
+  }
+  *cap = need;
+  t->bytes += need * sizeof(P);
+  return CGM_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------
+// dense form (T <= 32)
+// ---------------------------------------------------------------------------
 
 // A_t of map blockIdx.y, column-major into K: work-group blockIdx.x covers
 // chunk (blockIdx.x % chunks) of tile (blockIdx.x / chunks).
@@ -135,6 +193,7 @@ k_tile_apply(const float2* __restrict__ K, long sK, FnoIO io, const int* __restr
   float2 acc[NV];
 #pragma unroll
   for (int v = 0; v < NV; ++v) acc[v] = make_float2(0.f, 0.f);
+#pragma unroll 2
   for (int q0 = 0; q0 < M2; q0 += kU) {
     float2 k[kU];
 #pragma unroll
@@ -154,189 +213,6 @@ k_tile_apply(const float2* __restrict__ K, long sK, FnoIO io, const int* __restr
     if (live[v]) io.z[(long)(s0 + v) * io.sZ + cell] = acc[v];
 }
 
-constexpr int kTileH = 4096;           // unknowns of a 64 x 64 tile
-constexpr int kRowsH = 512;            // rows per work-group of the application
-constexpr int kPartsH = kTileH / kRowsH;
-constexpr int kColPad = 16;            // kept columns are padded to a multiple of this
-
-constexpr unsigned short kPadCol = 0xFFFF;  // column index of a padding slot
-
-// Kept columns of every 64 x 64 tile of map blockIdx.y (tile blockIdx.x):
-// the cells with chi != 0 in order, their count padded to kColPad (padding
-// slots hold kPadCol and zero columns), and the chi = 0 bitmask.
-__global__ void __launch_bounds__(1024)
-k_tile_cols(const float* __restrict__ chi, long chi_stride, int N, int* __restrict__ ncol,
-            unsigned short* __restrict__ cols, unsigned int* __restrict__ zmask) {
-  constexpr int T = 64;
-  const int nt = N / T;
-  const int t = blockIdx.x, a = t / nt, b = t % nt;
-  const long tb = (long)blockIdx.y * nt * nt + t;
-  chi += (long)blockIdx.y * chi_stride;
-  __shared__ unsigned int zm[kTileH / 32];
-  __shared__ int wsum[16];
-  for (int w = threadIdx.x; w < kTileH / 32; w += blockDim.x) zm[w] = 0u;
-  __syncthreads();
-  bool nz[4];
-  int cnt = 0;
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int c = 4 * threadIdx.x + i;
-    nz[i] = chi[(long)(a * T + c / T) * N + b * T + c % T] != 0.f;
-    cnt += nz[i];
-    if (!nz[i]) atomicOr(&zm[c >> 5], 1u << (c & 31));
-  }
-  // exclusive scan of the counts over the work-group (waves of 64)
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  int inc = cnt;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const int y = __shfl_up(inc, o, 64);
-    if (lane >= o) inc += y;
-  }
-  if (lane == 63) wsum[wave] = inc;
-  __syncthreads();
-  int base = 0;
-  for (int w = 0; w < wave; ++w) base += wsum[w];
-  int slot = base + inc - cnt;
-  unsigned short* ct = cols + tb * kTileH;
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-    if (nz[i]) ct[slot++] = (unsigned short)(4 * threadIdx.x + i);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int total = 0;
-    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) total += wsum[w];
-    const int padded = (total + kColPad - 1) / kColPad * kColPad;
-    for (int e = total; e < padded; ++e) ct[e] = kPadCol;
-    ncol[tb] = padded;
-  }
-  for (int w = threadIdx.x; w < kTileH / 32; w += blockDim.x) zmask[tb * (kTileH / 32) + w] = zm[w];
-}
-
-// Largest |K| entry of each tile's kept columns (one work-group per tile of one map).
-__global__ void __launch_bounds__(1024) k_tile_maxabs(const float2* __restrict__ K, const int* __restrict__ ncol,
-                                                      const unsigned short* __restrict__ cols, float* __restrict__ sc) {
-  const long tb = blockIdx.x;
-  const float2* Kt = K + tb * (long)kTileH * kTileH;
-  const int nc = ncol[tb];
-  float mx = 0.f;
-  for (long e = threadIdx.x; e < (long)nc * kTileH; e += blockDim.x) {
-    const int s = (int)(e / kTileH), p = (int)(e % kTileH);
-    const int q = cols[tb * kTileH + s];
-    if (q == kPadCol) continue;
-    const float2 v = Kt[(long)q * kTileH + p];
-    mx = fmaxf(mx, fmaxf(fabsf(v.x), fabsf(v.y)));
-  }
-  __shared__ float red[16];
-  for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    float m = 0.f;
-    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) m = fmaxf(m, red[w]);
-    sc[tb] = m > 0.f ? m : 1.f;
-  }
-}
-
-// K16 slot s of tile tb = column cols[s] of K / scale (zero for padding slots).
-__global__ void __launch_bounds__(256) k_tile_to_half(const float2* __restrict__ K, __half2* __restrict__ K16,
-                                                      const float* __restrict__ sc, const int* __restrict__ ncol,
-                                                      const unsigned short* __restrict__ cols, long ntile_total) {
-  const long m22 = (long)kTileH * kTileH;
-  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < m22 * ntile_total; e += (long)gridDim.x * blockDim.x) {
-    const long tb = e / m22;
-    const long w = e % m22;
-    const int s = (int)(w / kTileH), p = (int)(w % kTileH);
-    if (s >= ncol[tb]) continue;
-    const int q = cols[tb * kTileH + s];
-    const float r = 1.f / sc[tb];
-    const float2 v = q == kPadCol ? make_float2(0.f, 0.f) : K[tb * m22 + (long)q * kTileH + p];
-    K16[e] = __floats2half2_rn(v.x * r, v.y * r);
-  }
-}
-
-// y_t = scale_t K16_t x_t + (x_q on the cells with chi = 0) for 64 x 64
-// tiles: work-group (tile, part) of system blockIdx.y computes the kRowsH rows
-// [part kRowsH, (part+1) kRowsH) of its tile, two consecutive rows per thread
-// (one 8-byte load of a packed FP16 column per kept column), FP32 sums; the
-// tile's input and its column list are staged in LDS by every part (part 0
-// also writes the scaled input back to io.vw).  Same io / done conventions as
-// k_tile_apply with NV = 1.
-__global__ void __launch_bounds__(kRowsH / 2)
-k_tile_apply_h(const __half2* __restrict__ K16, const float* __restrict__ ksc, const int* __restrict__ ncol,
-               const unsigned short* __restrict__ cols, const unsigned int* __restrict__ zmask, int maps1, FnoIO io,
-               const int* __restrict__ done, long sd, int N) {
-  constexpr int T = 64;
-  constexpr int kU = 16;  // columns loaded ahead (kColPad divides every padded count)
-  __shared__ float2 xs[kTileH];
-  __shared__ unsigned short cs[kTileH];
-  const int nt = N / T;
-  const int t = blockIdx.x / kPartsH, part = blockIdx.x % kPartsH;
-  const int a = t / nt, b = t % nt;
-  const int s = blockIdx.y;
-  if (done && *boff(done, (long)s * sd)) return;
-  const long tb = (long)(maps1 ? 0 : s) * nt * nt + t;
-  float sc = 1.f;
-  if (io.nrm) {
-    const double d = *boff(io.nrm, (long)s * io.sNrm);
-    sc = d > 0.0 ? (float)(1.0 / d) : 0.f;
-  }
-  const int nc = ncol[tb];
-  for (int r = threadIdx.x; r < kTileH; r += blockDim.x) {
-    const long cell = (long)(a * T + r / T) * N + b * T + r % T;
-    float2 xv = io.v[(long)s * io.sV + cell];
-    xv.x *= sc;
-    xv.y *= sc;
-    if (io.nrm && io.vw && part == 0) io.vw[(long)s * io.sV + cell] = xv;
-    xs[r] = xv;
-    if (r < nc) {
-      const unsigned short c = cols[tb * kTileH + r];
-      cs[r] = c == kPadCol ? 0 : c;  // a padding slot's column is zero: any input works
-    }
-  }
-  __syncthreads();
-  const int p = part * kRowsH + 2 * threadIdx.x;  // rows p, p + 1
-  const uint2* Kt = reinterpret_cast<const uint2*>(K16 + tb * (long)kTileH * kTileH + p);
-  float ax = 0.f, ay = 0.f, bx = 0.f, by = 0.f;
-  for (int q0 = 0; q0 < nc; q0 += kU) {
-    uint2 k[kU];
-#pragma unroll
-    for (int u = 0; u < kU; ++u) {
-      // the inverses stream once per application: non-temporal loads
-      typedef unsigned int u2v_t __attribute__((ext_vector_type(2)));
-      const u2v_t w = __builtin_nontemporal_load(reinterpret_cast<const u2v_t*>(Kt + (long)(q0 + u) * (kTileH / 2)));
-      k[u] = make_uint2(w.x, w.y);
-    }
-#pragma unroll
-    for (int u = 0; u < kU; ++u) {
-      const float2 xv = xs[cs[q0 + u]];
-      const float2 k0 = __half22float2(__builtin_bit_cast(__half2, k[u].x));
-      const float2 k1 = __half22float2(__builtin_bit_cast(__half2, k[u].y));
-      ax += k0.x * xv.x - k0.y * xv.y;
-      ay += k0.x * xv.y + k0.y * xv.x;
-      bx += k1.x * xv.x - k1.y * xv.y;
-      by += k1.x * xv.y + k1.y * xv.x;
-    }
-  }
-  const float m = ksc[tb];
-  const unsigned int zw = zmask[tb * (kTileH / 32) + (p >> 5)];
-  const bool z0 = (zw >> (p & 31)) & 1u, z1 = (zw >> ((p + 1) & 31)) & 1u;   // p even: same word
-  float2 y0 = make_float2(ax * m, ay * m), y1 = make_float2(bx * m, by * m);
-  if (z0) y0 = cadd(y0, xs[p]);
-  if (z1) y1 = cadd(y1, xs[p + 1]);
-  const long c0 = (long)(a * T + p / T) * N + b * T + p % T;   // p and p + 1 share a tile row (p even)
-  io.z[(long)s * io.sZ + c0] = y0;
-  io.z[(long)s * io.sZ + c0 + 1] = y1;
-}
-
-int launch_apply_h(const cgm_tiles* tl, hipStream_t st, const FnoIO& io, const int* done, long sd, int nsys) {
-  const int ntiles = tl->nt * tl->nt;
-  hipLaunchKernelGGL(k_tile_apply_h, dim3(ntiles * kPartsH, nsys), dim3(kRowsH / 2), 0, st, tl->K16, tl->kscale,
-                     tl->ncol, tl->cols, tl->zmask, tl->maps == 1 ? 1 : 0, io, done, sd, tl->N);
-  CGM_HIP(hipGetLastError());
-  return CGM_SUCCESS;
-}
-
 template <int M2>
 int launch_apply_m(const cgm_tiles* tl, hipStream_t st, const FnoIO& io, const int* done, long sd, int nsys) {
   const long sK = (long)tl->nt * tl->nt * M2 * M2;
@@ -354,6 +230,445 @@ int launch_apply_m(const cgm_tiles* tl, hipStream_t st, const FnoIO& io, const i
   return CGM_SUCCESS;
 }
 
+// ---------------------------------------------------------------------------
+// reduced form (T = 64, 128): factorisation
+// ---------------------------------------------------------------------------
+
+// S_t of every tile of map blockIdx.y (tile blockIdx.x): the cells with
+// chi != 0 in row-major order, their count padded to kPadTo (padding slots
+// hold kPadCol), and the chi = 0 bitmask.
+template <int T>
+__global__ void __launch_bounds__(1024)
+k_tile_cols(const float* __restrict__ chi, long chi_stride, int N, int* __restrict__ ncol,
+            unsigned short* __restrict__ cols, unsigned int* __restrict__ zmask) {
+  constexpr int M2 = T * T;
+  constexpr int CPT = M2 / 1024;  // cells per thread
+  const int nt = N / T;
+  const int t = blockIdx.x, a = t / nt, b = t % nt;
+  const long tb = (long)blockIdx.y * nt * nt + t;
+  chi += (long)blockIdx.y * chi_stride;
+  __shared__ unsigned int zm[M2 / 32];
+  __shared__ int wsum[16];
+  for (int w = threadIdx.x; w < M2 / 32; w += blockDim.x) zm[w] = 0u;
+  __syncthreads();
+  bool nz[CPT];
+  int cnt = 0;
+#pragma unroll
+  for (int i = 0; i < CPT; ++i) {
+    const int c = CPT * threadIdx.x + i;
+    nz[i] = chi[(long)(a * T + c / T) * N + b * T + c % T] != 0.f;
+    cnt += nz[i];
+    if (!nz[i]) atomicOr(&zm[c >> 5], 1u << (c & 31));
+  }
+  // exclusive scan of the counts over the work-group (16 waves of 64)
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int inc = cnt;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += y;
+  }
+  if (lane == 63) wsum[wave] = inc;
+  __syncthreads();
+  int base = 0;
+  for (int w = 0; w < wave; ++w) base += wsum[w];
+  int slot = base + inc - cnt;
+  unsigned short* ct = cols + tb * M2;
+#pragma unroll
+  for (int i = 0; i < CPT; ++i)
+    if (nz[i]) ct[slot++] = (unsigned short)(CPT * threadIdx.x + i);
+  int total = 0;
+  for (int w = 0; w < 16; ++w) total += wsum[w];
+  const int padded = (total + kPadTo - 1) / kPadTo * kPadTo;
+  if ((int)threadIdx.x < padded - total) ct[total + threadIdx.x] = kPadCol;
+  if (threadIdx.x == 0) ncol[tb] = padded;
+  __syncthreads();
+  for (int w = threadIdx.x; w < M2 / 32; w += blockDim.x) zmask[tb * (M2 / 32) + w] = zm[w];
+}
+
+// Grid index of cell c (row-major inside the tile) of tile t.
+__device__ inline long tile_cell(int t, int c, int nt, int T, int N) {
+  const int a = t / nt, b = t % nt;
+  return (long)(a * T + c / T) * N + b * T + c % T;
+}
+
+// I - D_S G_SS of the tiles ids[0 .. count) (one size class, n x n,
+// column-major; the identity on padding slots and beyond n_t): work-group
+// blockIdx.x covers chunk (blockIdx.x % chunks) of member blockIdx.x / chunks.
+__global__ void __launch_bounds__(256)
+k_tile_build_red(const double2* __restrict__ g, const float* __restrict__ chi, long chi_stride,
+                 const int* __restrict__ ncol, const unsigned short* __restrict__ cols, const int* __restrict__ ids,
+                 float2* __restrict__ Kw, int n, int N, int T, int chunks) {
+  const int nt = N / T;
+  const int i = blockIdx.x / chunks;
+  const long nn = (long)n * n;
+  const long e0 = (long)(blockIdx.x % chunks) * kBuildChunk;
+  const long e1 = e0 + kBuildChunk < nn ? e0 + kBuildChunk : nn;
+  const int tb = ids[i];
+  const int map = tb / (nt * nt), t = tb % (nt * nt);
+  const int nk = ncol[tb];
+  const unsigned short* ct = cols + (long)tb * T * T;
+  chi += (long)map * chi_stride;
+  float2* A = Kw + (long)i * nn;
+  for (long e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+    const int p = (int)(e % n), q = (int)(e / n);
+    const int cp = p < nk ? ct[p] : kPadCol, cq = q < nk ? ct[q] : kPadCol;
+    float2 v = make_float2(p == q ? 1.f : 0.f, 0.f);
+    if (cp != kPadCol && cq != kPadCol) {
+      const int pr = cp / T, pc = cp % T, qr = cq / T, qc = cq % T;
+      const int dr = pr > qr ? pr - qr : qr - pr;
+      const int dc = pc > qc ? pc - qc : qc - pc;
+      const double2 gv = g[(long)dr * N + dc];
+      const double cv = chi[tile_cell(t, cp, nt, T, N)];
+      v = make_float2((float)((p == q ? 1.0 : 0.0) - gv.x * cv), (float)(-gv.y * cv));
+    }
+    A[e] = v;
+  }
+}
+
+// chi of slot q of tile tb (0 for padding slots).
+__device__ inline float slot_chi(const float* chi, long chi_stride, const unsigned short* ct, int q, int tb, int nt,
+                                 int T, int N) {
+  const int cq = ct[q];
+  if (cq == kPadCol) return 0.f;
+  return chi[(long)(tb / (nt * nt)) * chi_stride + tile_cell(tb % (nt * nt), cq, nt, T, N)];
+}
+
+// kscale of every member set to 0 before k_tile_red_max.
+__global__ void k_tile_red_zero(const int* __restrict__ ids, int cnt, float* __restrict__ sc) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) sc[ids[i]] = 0.f;
+}
+
+// Largest |W| entry of each member (W = Minv D_S): work-group blockIdx.x
+// covers chunk (blockIdx.x % chunks) of member blockIdx.x / chunks and folds
+// its maximum into the member's kscale with an atomic max on the bits (the
+// values are >= 0, whose order the bits keep).
+__global__ void __launch_bounds__(256)
+k_tile_red_max(const float2* __restrict__ Kw, const float* __restrict__ chi, long chi_stride,
+               const int* __restrict__ ncol, const unsigned short* __restrict__ cols, const int* __restrict__ ids,
+               int n, int N, int T, int chunks, float* __restrict__ sc) {
+  const int nt = N / T;
+  const int i = blockIdx.x / chunks;
+  const int tb = ids[i];
+  const int nk = ncol[tb];
+  const long nn = (long)nk * nk;
+  const long e0 = (long)(blockIdx.x % chunks) * kBuildChunk;
+  const long e1 = e0 + kBuildChunk < nn ? e0 + kBuildChunk : nn;
+  const unsigned short* ct = cols + (long)tb * T * T;
+  const float2* A = Kw + (long)i * n * n;
+  float mx = 0.f;
+  for (long e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+    const int p = (int)(e % nk), q = (int)(e / nk);
+    const float cq = fabsf(slot_chi(chi, chi_stride, ct, q, tb, nt, T, N));
+    const float2 v = A[(long)q * n + p];
+    mx = fmaxf(mx, fmaxf(fabsf(v.x), fabsf(v.y)) * cq);
+  }
+  __shared__ float red[4];
+  for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const float m = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+    if (m > 0.f) atomicMax(reinterpret_cast<unsigned int*>(sc + tb), __float_as_uint(m));
+  }
+}
+
+// W16 of every member: (Minv D_S) / scale in FP16, n_t x n_t column-major at
+// woff[tb]; work-group blockIdx.x covers chunk (blockIdx.x % chunks).
+__global__ void __launch_bounds__(256)
+k_tile_red_pack(const float2* __restrict__ Kw, const float* __restrict__ chi, long chi_stride,
+                const int* __restrict__ ncol, const unsigned short* __restrict__ cols, const int* __restrict__ ids,
+                const long* __restrict__ woff, const float* __restrict__ sc, __half2* __restrict__ W16, int n, int N,
+                int T, int chunks) {
+  const int nt = N / T;
+  const int i = blockIdx.x / chunks;
+  const int tb = ids[i];
+  const int nk = ncol[tb];
+  const long nn = (long)nk * nk;
+  const long e0 = (long)(blockIdx.x % chunks) * kBuildChunk;
+  const long e1 = e0 + kBuildChunk < nn ? e0 + kBuildChunk : nn;
+  const unsigned short* ct = cols + (long)tb * T * T;
+  const float2* A = Kw + (long)i * n * n;
+  const float r = sc[tb] > 0.f ? 1.f / sc[tb] : 0.f;  // 0: W_t = 0
+  __half2* Wt = W16 + woff[tb];
+  for (long e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+    const int p = (int)(e % nk), q = (int)(e / nk);
+    const float cq = slot_chi(chi, chi_stride, ct, q, tb, nt, T, N) * r;
+    const float2 v = A[(long)q * n + p];
+    Wt[e] = __floats2half2_rn(v.x * cq, v.y * cq);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// reduced form: application
+// ---------------------------------------------------------------------------
+
+// u_t = scale_t W16_t x_t[S] for system blockIdx.y, written to U at the cells
+// of S (and zero at the cells with chi = 0).  Work-group (tile, part) computes
+// rows [part kRowsW, (part+1) kRowsW) of W_t, two consecutive rows per thread
+// (one 8-byte load per column), FP32 sums, x[S] staged in LDS kChunkW columns
+// at a time; every part also clears its share of the chi = 0 cells.  x is
+// io.v scaled by 1 / *(io.nrm + s sNrm bytes) when io.nrm is given; systems
+// with *(done + s sd bytes) set are skipped.
+template <int T>
+__global__ void __launch_bounds__(kRowsW / 2)
+k_tile_w(const __half2* __restrict__ W16, const long* __restrict__ woff, const float* __restrict__ ksc,
+         const int* __restrict__ ncol, const unsigned short* __restrict__ cols,
+         const unsigned int* __restrict__ zmask, int maps1, int parts, FnoIO io, float2* __restrict__ U,
+         const int* __restrict__ done, long sd, int N) {
+  constexpr int M2 = T * T;
+  constexpr int kU = 16;  // columns loaded ahead (kPadTo divides every n_t)
+  __shared__ float2 xs[kChunkW];
+  const int nt = N / T;
+  const int t = blockIdx.x / parts, part = blockIdx.x % parts;
+  const int s = blockIdx.y;
+  if (done && *boff(done, (long)s * sd)) return;
+  const long tb = (long)(maps1 ? 0 : s) * nt * nt + t;
+  float sc = 1.f;
+  if (io.nrm) {
+    const double d = *boff(io.nrm, (long)s * io.sNrm);
+    sc = d > 0.0 ? (float)(1.0 / d) : 0.f;
+  }
+  const int n = ncol[tb];
+  const unsigned short* ct = cols + tb * M2;
+  float2* Ut = U + ((long)s * nt * nt + t) * M2;
+  {
+    const unsigned int* zm = zmask + tb * (M2 / 32);
+    const int share = (M2 + parts - 1) / parts;
+    const int c1 = (part + 1) * share < M2 ? (part + 1) * share : M2;
+    for (int c = part * share + threadIdx.x; c < c1; c += blockDim.x)
+      if ((zm[c >> 5] >> (c & 31)) & 1u) Ut[c] = make_float2(0.f, 0.f);
+  }
+  if (part * kRowsW >= n) return;  // uniform over the work-group
+  const float2* v = io.v + (long)s * io.sV;
+  const int r = part * kRowsW + 2 * threadIdx.x;  // rows r, r + 1 (n even)
+  const bool act = r < n;
+  typedef unsigned int u2v_t __attribute__((ext_vector_type(2)));
+  const u2v_t* Wt = reinterpret_cast<const u2v_t*>(W16 + woff[tb] + (act ? r : 0));
+  const long colw = n / 2;  // one column in 8-byte units
+  float ax = 0.f, ay = 0.f, bx = 0.f, by = 0.f;
+  for (int k0 = 0; k0 < n; k0 += kChunkW) {
+    const int kn = n - k0 < kChunkW ? n - k0 : kChunkW;
+    __syncthreads();  // the previous chunk is consumed
+    for (int k = threadIdx.x; k < kn; k += blockDim.x) {
+      const int c = ct[k0 + k];
+      float2 xv = make_float2(0.f, 0.f);
+      if (c != kPadCol) {
+        xv = v[tile_cell(t, c, nt, T, N)];
+        xv.x *= sc;
+        xv.y *= sc;
+      }
+      xs[k] = xv;
+    }
+    __syncthreads();
+    if (act) {
+      for (int q0 = 0; q0 < kn; q0 += kU) {
+        uint2 w[kU];  // each vector load is copied out at once (rows r and r + 1)
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+          const u2v_t l = __builtin_nontemporal_load(Wt + (long)(k0 + q0 + u) * colw);
+          w[u] = make_uint2(l.x, l.y);
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+          const float2 xv = xs[q0 + u];
+          const float2 k0v = __half22float2(__builtin_bit_cast(__half2, w[u].x));
+          const float2 k1v = __half22float2(__builtin_bit_cast(__half2, w[u].y));
+          ax += k0v.x * xv.x - k0v.y * xv.y;
+          ay += k0v.x * xv.y + k0v.y * xv.x;
+          bx += k1v.x * xv.x - k1v.y * xv.y;
+          by += k1v.x * xv.y + k1v.y * xv.x;
+        }
+      }
+    }
+  }
+  if (!act) return;
+  const float m = ksc[tb];
+  const int c0 = ct[r], c1 = ct[r + 1];
+  if (c0 != kPadCol) Ut[c0] = make_float2(ax * m, ay * m);
+  if (c1 != kPadCol) Ut[c1] = make_float2(bx * m, by * m);
+}
+
+// z = x + Y (Y in tile-major order) for system blockIdx.y; x is io.v scaled
+// as in k_tile_w and is written to io.vw when io.nrm and io.vw are given.
+template <int T>
+__global__ void __launch_bounds__(256)
+k_tile_combine(FnoIO io, const float2* __restrict__ Y, const int* __restrict__ done, long sd, int N) {
+  const int s = blockIdx.y;
+  if (done && *boff(done, (long)s * sd)) return;
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)N * N) return;
+  float sc = 1.f;
+  if (io.nrm) {
+    const double d = *boff(io.nrm, (long)s * io.sNrm);
+    sc = d > 0.0 ? (float)(1.0 / d) : 0.f;
+  }
+  const int i = (int)(e / N), j = (int)(e % N);
+  const int nt = N / T;
+  float2 xv = io.v[(long)s * io.sV + e];
+  xv.x *= sc;
+  xv.y *= sc;
+  if (io.nrm && io.vw) io.vw[(long)s * io.sV + e] = xv;
+  const float2 g = Y[((long)s * nt * nt + (i / T) * nt + j / T) * (T * T) + (i % T) * T + j % T];
+  io.z[(long)s * io.sZ + e] = cadd(xv, g);
+}
+
+int ensure_scratch(cgm_tiles* t, int nsys) {
+  if (nsys <= t->scratch_sys) return CGM_SUCCESS;
+  const size_t n = (size_t)t->N * t->N;
+  const size_t tm = (size_t)t->nt * t->nt * t->T * t->P.M;
+  void* ptrs[] = {t->U, t->Y, t->Tm};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  t->bytes -= (2 * n + tm) * sizeof(float2) * t->scratch_sys;
+  t->U = t->Y = t->Tm = nullptr;
+  t->scratch_sys = 0;
+  if (hipMalloc((void**)&t->U, sizeof(float2) * n * nsys) != hipSuccess ||
+      hipMalloc((void**)&t->Y, sizeof(float2) * n * nsys) != hipSuccess ||
+      hipMalloc((void**)&t->Tm, sizeof(float2) * tm * nsys) != hipSuccess) {
+    cgm_set_last_error("tile preconditioner: scratch allocation for %d systems failed", nsys);
+    return CGM_ERROR_ALLOC;
+  }
+  t->scratch_sys = nsys;
+  t->bytes += (2 * n + tm) * sizeof(float2) * nsys;
+  return CGM_SUCCESS;
+}
+
+// ev, when given: events recorded around k_tile_w (live timing).
+int launch_apply_red(const cgm_tiles* tl, hipStream_t st, const FnoIO& io, const int* done, long sd, int nsys,
+                     hipEvent_t* ev) {
+  cgm_tiles* t = const_cast<cgm_tiles*>(tl);  // the apply scratch grows with the batch
+  int rc = ensure_scratch(t, nsys);
+  if (rc) return rc;
+  const int ntiles = tl->nt * tl->nt;
+  const int B = nsys * ntiles;
+  if (B > 65535) {
+    cgm_set_last_error("%d systems exceed the tile convolution's batch limit", nsys);
+    return CGM_ERROR_UNSUPPORTED;
+  }
+  const int parts = tl->nmax > kRowsW ? (tl->nmax + kRowsW - 1) / kRowsW : 1;
+  const int maps1 = tl->maps == 1 ? 1 : 0;
+  if (ev) CGM_HIP(hipEventRecord(ev[0], st));
+  if (tl->T == 64)
+    hipLaunchKernelGGL((k_tile_w<64>), dim3(ntiles * parts, nsys), dim3(kRowsW / 2), 0, st, tl->W16, tl->woff,
+                       tl->kscale, tl->ncol, tl->cols, tl->zmask, maps1, parts, io, tl->U, done, sd, tl->N);
+  else
+    hipLaunchKernelGGL((k_tile_w<128>), dim3(ntiles * parts, nsys), dim3(kRowsW / 2), 0, st, tl->W16, tl->woff,
+                       tl->kscale, tl->ncol, tl->cols, tl->zmask, maps1, parts, io, tl->U, done, sd, tl->N);
+  CGM_HIP(hipGetLastError());
+  if (ev) CGM_HIP(hipEventRecord(ev[1], st));
+  BatchStrides bs;
+  bs.x = tl->M2;
+  bs.y = tl->M2;
+  bs.T = (long)tl->T * tl->P.M;
+  rc = launch_pass1<float>(tl->P, st, tl->U, nullptr, nullptr, nullptr, kModeGreen, tl->Tm, tl->tw32, nullptr, B, bs);
+  if (!rc) rc = launch_pass2<float, float2>(tl->P, st, tl->Tm, tl->gq, tl->tw32, nullptr, B, bs);
+  if (!rc)
+    rc = launch_pass3<float>(tl->P, st, tl->Tm, tl->U, nullptr, nullptr, tl->Y, kModeGreen, tl->tw32, nullptr, B, bs);
+  if (rc) {
+    cgm_set_last_error("tile convolution at T = %d failed", tl->T);
+    return rc;
+  }
+  const long n = (long)tl->N * tl->N;
+  if (tl->T == 64)
+    hipLaunchKernelGGL((k_tile_combine<64>), dim3((n + 255) / 256, nsys), dim3(256), 0, st, io, tl->Y, done, sd,
+                       tl->N);
+  else
+    hipLaunchKernelGGL((k_tile_combine<128>), dim3((n + 255) / 256, nsys), dim3(256), 0, st, io, tl->Y, done, sd,
+                       tl->N);
+  CGM_HIP(hipGetLastError());
+  return CGM_SUCCESS;
+}
+
+int upload_twiddles(cgm_tiles* t) {
+  const int M = t->P.M;
+  std::vector<float2> t32(M);
+  std::vector<double2> t64(M);
+  for (int q = 0; q < M; ++q) {
+    const double a = -2.0 * CGM_PI * (double)q / (double)M;
+    t64[q] = make_double2(cos(a), sin(a));
+    t32[q] = make_float2((float)t64[q].x, (float)t64[q].y);
+  }
+  CGM_HIP(hipMemcpy(t->tw32, t32.data(), sizeof(float2) * M, hipMemcpyHostToDevice));
+  CGM_HIP(hipMemcpy(t->tw64, t64.data(), sizeof(double2) * M, hipMemcpyHostToDevice));
+  return CGM_SUCCESS;
+}
+
+int factor_reduced(cgm_tiles* t, const double2* blk, const float* chi, long chi_stride, int maps) {
+  cgm_context* c = t->ctx;
+  hipStream_t st = c->stream;
+  const int ntiles = t->nt * t->nt;
+  const int nb = ntiles * maps;
+  const int T = t->T, N = t->N;
+  // spectrum of the tile convolution from the T x T corner of the block
+  CGM_HIP(hipMemcpy2DAsync(t->blkT, sizeof(double2) * T, blk, sizeof(double2) * N, sizeof(double2) * T, T,
+                           hipMemcpyDeviceToDevice, st));
+  int rc = spectrum_from_block(t->P, st, t->blkT, t->tw64, t->gq, nullptr);
+  if (rc) return rc;
+  if (T == 64)
+    hipLaunchKernelGGL((k_tile_cols<64>), dim3(ntiles, maps), dim3(1024), 0, st, chi, chi_stride, N, t->ncol, t->cols,
+                       t->zmask);
+  else
+    hipLaunchKernelGGL((k_tile_cols<128>), dim3(ntiles, maps), dim3(1024), 0, st, chi, chi_stride, N, t->ncol,
+                       t->cols, t->zmask);
+  CGM_HIP(hipGetLastError());
+  std::vector<int> nc(nb);
+  CGM_HIP(hipMemcpyAsync(nc.data(), t->ncol, sizeof(int) * nb, hipMemcpyDeviceToHost, st));
+  CGM_HIP(hipStreamSynchronize(st));
+  std::vector<long> off(nb);
+  long total = 0;
+  std::map<int, std::vector<int>> classes;  // n rounded up to kClass -> tiles
+  t->nmax = 0;
+  for (int k = 0; k < nb; ++k) {
+    off[k] = total;
+    total += (long)nc[k] * nc[k];
+    t->kept[k / ntiles] += (double)nc[k] * nc[k];
+    t->nmax = std::max(t->nmax, nc[k]);
+    if (nc[k] > 0) classes[(nc[k] + kClass - 1) / kClass * kClass].push_back(k);
+  }
+  if ((rc = grow(t, &t->W16, &t->w16_cap, (size_t)std::max(total, 1L)))) return rc;
+  CGM_HIP(hipMemcpyAsync(t->woff, off.data(), sizeof(long) * nb, hipMemcpyHostToDevice, st));
+  for (auto& cl : classes) {
+    const int n = cl.first;
+    const std::vector<int>& all = cl.second;
+    const long nn = (long)n * n;
+    const size_t per = std::max<size_t>(1, std::min<size_t>(all.size(), kFactorBytes / (sizeof(float2) * nn)));
+    for (size_t i0 = 0; i0 < all.size(); i0 += per) {
+      const int cnt = (int)std::min<size_t>(per, all.size() - i0);
+      if ((rc = grow(t, &t->Kw, &t->kw_cap, (size_t)cnt * nn))) return rc;
+      if ((rc = grow(t, &t->ipiv, &t->piv_cap, (size_t)cnt * n))) return rc;
+      CGM_HIP(hipMemcpyAsync(t->ids, all.data() + i0, sizeof(int) * cnt, hipMemcpyHostToDevice, st));
+      const int chunks = (int)((nn + kBuildChunk - 1) / kBuildChunk);
+      hipLaunchKernelGGL(k_tile_build_red, dim3(cnt * chunks), dim3(256), 0, st, blk, chi, chi_stride, t->ncol, t->cols,
+                         t->ids, t->Kw, n, N, T, chunks);
+      CGM_HIP(hipGetLastError());
+      const rocblas_stride sA = (rocblas_stride)nn;
+      rocblas_status s = rocsolver_cgetrf_strided_batched(t->h, n, n, (rocblas_float_complex*)t->Kw, n, sA, t->ipiv,
+                                                          n, t->info, cnt);
+      CGM_CHECK(s == rocblas_status_success, CGM_ERROR_DEVICE, "rocsolver getrf: status %d", (int)s);
+      std::vector<int> info(cnt);
+      CGM_HIP(hipMemcpyAsync(info.data(), t->info, sizeof(int) * cnt, hipMemcpyDeviceToHost, st));
+      CGM_HIP(hipStreamSynchronize(st));
+      for (int k = 0; k < cnt; ++k) {
+        const int tb = all[i0 + k];
+        CGM_CHECK(info[k] == 0, CGM_ERROR_NONFINITE, "tile %d of map %d is singular (pivot %d)", tb % ntiles,
+                  tb / ntiles, info[k]);
+      }
+      s = rocsolver_cgetri_strided_batched(t->h, n, (rocblas_float_complex*)t->Kw, n, sA, t->ipiv, n, t->info, cnt);
+      CGM_CHECK(s == rocblas_status_success, CGM_ERROR_DEVICE, "rocsolver getri: status %d", (int)s);
+      hipLaunchKernelGGL(k_tile_red_zero, dim3(1), dim3(256), 0, st, t->ids, cnt, t->kscale);
+      hipLaunchKernelGGL(k_tile_red_max, dim3(cnt * chunks), dim3(256), 0, st, t->Kw, chi, chi_stride, t->ncol,
+                         t->cols, t->ids, n, N, T, chunks, t->kscale);
+      hipLaunchKernelGGL(k_tile_red_pack, dim3(cnt * chunks), dim3(256), 0, st, t->Kw, chi, chi_stride, t->ncol,
+                         t->cols, t->ids, t->woff, t->kscale, t->W16, n, N, T, chunks);
+      CGM_HIP(hipGetLastError());
+      CGM_HIP(hipStreamSynchronize(st));  // the next member batch reuses Kw and ids
+    }
+  }
+  CGM_HIP(hipStreamSynchronize(st));  // off is read by the copy above
+  return CGM_SUCCESS;
+}
+
 }  // namespace
 
 namespace cgm {
@@ -361,18 +676,22 @@ namespace cgm {
 cgm_context* tiles_context(const cgm_tiles* t) { return t->ctx; }
 int tiles_maps(const cgm_tiles* t) { return t->maps; }
 
-// Algorithmic bytes of one application to nsys systems: the columns streamed
-// (every column of every tile in FP32, once per system or once per group of
-// four right-hand sides of one map; the kept FP16 columns of the map each
-// system uses for 64 x 64 tiles) plus v in and z out.
+// Algorithmic bytes of one application to nsys systems: the inverse entries
+// streamed (dense form: every entry of every tile in FP32, once per system or
+// once per group of four right-hand sides of one map; reduced form: the FP16
+// entries of W of the map each system uses) plus v in and z out.
 double apply_bytes(const cgm_tiles* tl, int nsys) {
-  const double per_entry = tl->K16 ? 4.0 : 8.0;
-  double cols = 0.0;
-  if (tl->maps == 1)
-    cols = tl->kept[0] * ((nsys > 1 && tl->M2 < 4096) ? (nsys + 3) / 4 : nsys);
-  else
-    for (int k = 0; k < nsys; ++k) cols += tl->kept[k];
-  return cols * tl->M2 * per_entry + 16.0 * tl->N * tl->N * nsys;
+  double e = 0.0;
+  if (tl->reduced) {
+    for (int s = 0; s < nsys; ++s) e += tl->kept[tl->maps == 1 ? 0 : s] * 4.0;
+  } else {
+    const int streams = (tl->maps == 1 && nsys > 1) ? (nsys + 3) / 4 : nsys;
+    if (tl->maps == 1)
+      e = tl->kept[0] * 8.0 * streams;
+    else
+      for (int s = 0; s < nsys; ++s) e += tl->kept[s] * 8.0;
+  }
+  return e + 16.0 * tl->N * tl->N * nsys;
 }
 
 int tiles_apply_io(const cgm_tiles* tl, hipStream_t st, const FnoIO& io, int nsys, const int* done, long sd) {
@@ -385,23 +704,26 @@ int tiles_apply_io(const cgm_tiles* tl, hipStream_t st, const FnoIO& io, int nsy
     return CGM_ERROR_INVALID_VALUE;
   }
   // live timing inside a long cycle (cgm_longrec_timing): events around the
-  // launch and its algorithmic bytes
+  // streaming kernel (k_tile_apply, or k_tile_w of the reduced form) and the
+  // application's algorithmic bytes
   LrTiming* tm = tl->ctx->lrt.on ? &tl->ctx->lrt : nullptr;
   const int sl = tm ? tm->cur : -1;
   constexpr int KK = LrTiming::kKernels;
-  if (sl >= 0) CGM_HIP(hipEventRecord(tm->ev[2 * KK * sl + 4], st));
+  hipEvent_t* ev = sl >= 0 ? &tm->ev[2 * KK * sl + 4] : nullptr;
+  if (ev && !tl->reduced) CGM_HIP(hipEventRecord(ev[0], st));
   int rc;
   switch (tl->M2) {
     case 64: rc = launch_apply_m<64>(tl, st, io, done, sd, nsys); break;
     case 256: rc = launch_apply_m<256>(tl, st, io, done, sd, nsys); break;
     case 1024: rc = launch_apply_m<1024>(tl, st, io, done, sd, nsys); break;
-    case 4096: rc = launch_apply_h(tl, st, io, done, sd, nsys); break;
+    case 4096:
+    case 16384: rc = launch_apply_red(tl, st, io, done, sd, nsys, ev); break;
     default:
       cgm_set_last_error("tile side %d unsupported", tl->T);
       return CGM_ERROR_UNSUPPORTED;
   }
-  if (sl >= 0 && !rc) {
-    CGM_HIP(hipEventRecord(tm->ev[2 * KK * sl + 5], st));
+  if (ev && !rc) {
+    if (!tl->reduced) CGM_HIP(hipEventRecord(ev[1], st));
     tm->slot_bytes[2][sl] = apply_bytes(tl, nsys);
     tm->slot_has[2][sl] = true;
   }
@@ -413,8 +735,8 @@ int tiles_apply_io(const cgm_tiles* tl, hipStream_t st, const FnoIO& io, int nsy
 extern "C" int cgm_tiles_create(cgm_ctx_t c, int T, int batch, cgm_tiles_t* out) {
   CGM_CHECK(c && out, CGM_ERROR_INVALID_VALUE, "null argument");
   *out = nullptr;
-  CGM_CHECK(T == 8 || T == 16 || T == 32 || T == 64, CGM_ERROR_UNSUPPORTED, "tile side %d: 8, 16, 32 or 64 supported",
-            T);
+  CGM_CHECK(T == 8 || T == 16 || T == 32 || T == 64 || T == 128, CGM_ERROR_UNSUPPORTED,
+            "tile side %d: 8, 16, 32, 64 or 128 supported", T);
   CGM_CHECK(c->N % T == 0, CGM_ERROR_UNSUPPORTED, "N=%d is not a multiple of the tile side %d", c->N, T);
   CGM_CHECK(batch >= 1, CGM_ERROR_INVALID_VALUE, "batch %d < 1", batch);
   CGM_HIP(hipSetDevice(c->device));
@@ -426,21 +748,41 @@ extern "C" int cgm_tiles_create(cgm_ctx_t c, int T, int batch, cgm_tiles_t* out)
   t->B = batch;
   t->nt = c->N / T;
   t->M2 = T * T;
+  t->reduced = T >= 64;
   const size_t ntiles = (size_t)t->nt * t->nt;
   const size_t M2 = (size_t)t->M2;
-  const bool half = T == 64;
-  if (hipMalloc((void**)&t->K, sizeof(float2) * M2 * M2 * ntiles * batch) != hipSuccess ||
-      hipMalloc((void**)&t->ipiv, sizeof(int) * M2 * ntiles * batch) != hipSuccess ||
-      hipMalloc((void**)&t->info, sizeof(int) * ntiles * batch) != hipSuccess ||
-      (half && hipMalloc((void**)&t->K16, sizeof(__half2) * M2 * M2 * ntiles * batch) != hipSuccess) ||
-      (half && hipMalloc((void**)&t->kscale, sizeof(float) * ntiles * batch) != hipSuccess) ||
-      (half && hipMalloc((void**)&t->ncol, sizeof(int) * ntiles * batch) != hipSuccess) ||
-      (half && hipMalloc((void**)&t->cols, sizeof(unsigned short) * M2 * ntiles * batch) != hipSuccess) ||
-      (half && hipMalloc((void**)&t->zmask, sizeof(unsigned int) * (M2 / 32) * ntiles * batch) != hipSuccess) ||
-      rocblas_create_handle(&t->h) != rocblas_status_success) {
+  bool ok = rocblas_create_handle(&t->h) == rocblas_status_success;
+  size_t want = 0;
+  auto alloc = [&](void** p, size_t b) {
+    if (ok && hipMalloc(p, b) != hipSuccess) ok = false;
+    want += b;
+  };
+  if (!t->reduced) {
+    alloc((void**)&t->K, sizeof(float2) * M2 * M2 * ntiles * batch);
+    alloc((void**)&t->ipiv, sizeof(int) * M2 * ntiles * batch);
+    alloc((void**)&t->info, sizeof(int) * ntiles * batch);
+  } else {
+    const size_t nb = ntiles * batch;
+    alloc((void**)&t->ncol, sizeof(int) * nb);
+    alloc((void**)&t->cols, sizeof(unsigned short) * M2 * nb);
+    alloc((void**)&t->zmask, sizeof(unsigned int) * (M2 / 32) * nb);
+    alloc((void**)&t->woff, sizeof(long) * nb);
+    alloc((void**)&t->kscale, sizeof(float) * nb);
+    alloc((void**)&t->ids, sizeof(int) * nb);
+    alloc((void**)&t->info, sizeof(int) * nb);
+    if (ok && plan_init(t->P, T) != CGM_SUCCESS) ok = false;
+    const size_t M = (size_t)t->P.M;
+    alloc((void**)&t->tw32, sizeof(float2) * M);
+    alloc((void**)&t->tw64, sizeof(double2) * M);
+    alloc((void**)&t->blkT, sizeof(double2) * M2);
+    alloc((void**)&t->gq, sizeof(float2) * (M / 2 + 1) * (M / 2 + 1));
+    if (ok && upload_twiddles(t) != CGM_SUCCESS) ok = false;
+  }
+  t->bytes = want;
+  if (ok && t->reduced && ensure_scratch(t, batch) != CGM_SUCCESS) ok = false;
+  if (!ok) {
     cgm_tiles_destroy(t);
-    cgm_set_last_error("cgm_tiles_create: allocation failed (%zu MiB of tile inverses)",
-                       sizeof(float2) * M2 * M2 * ntiles * batch >> 20);
+    cgm_set_last_error("cgm_tiles_create: allocation failed (%zu MiB)", want >> 20);
     return CGM_ERROR_ALLOC;
   }
   *out = t;
@@ -450,7 +792,8 @@ extern "C" int cgm_tiles_create(cgm_ctx_t c, int T, int batch, cgm_tiles_t* out)
 extern "C" int cgm_tiles_destroy(cgm_tiles_t t) {
   if (!t) return CGM_SUCCESS;
   (void)hipSetDevice(t->device);
-  void* ptrs[] = {t->K, t->ipiv, t->info, t->K16, t->kscale, t->ncol, t->cols, t->zmask};
+  void* ptrs[] = {t->K,   t->ipiv, t->info, t->ncol, t->cols, t->zmask, t->woff, t->kscale, t->ids,
+                  t->W16, t->Kw,   t->U,    t->Y,    t->Tm,   t->tw32,  t->tw64, t->blkT,   t->gq};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (t->h) (void)rocblas_destroy_handle(t->h);
@@ -460,11 +803,7 @@ extern "C" int cgm_tiles_destroy(cgm_tiles_t t) {
 
 extern "C" int cgm_tiles_memory(cgm_tiles_t t, size_t* bytes) {
   CGM_CHECK(t && bytes, CGM_ERROR_INVALID_VALUE, "null argument");
-  const size_t ntiles = (size_t)t->nt * t->nt;
-  const size_t k16 = t->K16 ? sizeof(__half2) * t->M2 * t->M2 + sizeof(float) + sizeof(int) +
-                                  sizeof(unsigned short) * t->M2 + sizeof(unsigned int) * (t->M2 / 32)
-                            : 0;
-  *bytes = (sizeof(float2) * t->M2 * t->M2 + sizeof(int) * (t->M2 + 1) + k16) * ntiles * t->B;
+  *bytes = t->bytes;
   return CGM_SUCCESS;
 }
 
@@ -479,12 +818,20 @@ extern "C" int cgm_tiles_factor(cgm_tiles_t t, const cgm_complex128* block, cons
   const int ntiles = t->nt * t->nt;
   const int M2 = t->M2;
   t->maps = 0;
+  t->kept.assign(maps, 0.0);
+  CGM_CHECK(rocblas_set_stream(t->h, c->stream) == rocblas_status_success, CGM_ERROR_DEVICE,
+            "rocblas_set_stream failed");
+  if (t->reduced) {
+    const int rc = factor_reduced(t, (const double2*)block, chi, maps == 1 ? 0 : chi_stride, maps);
+    if (rc) return rc;
+    CGM_HIP(hipStreamSynchronize(c->stream));
+    t->maps = maps;
+    return CGM_SUCCESS;
+  }
   const int chunks = (int)(((long)M2 * M2 + kBuildChunk - 1) / kBuildChunk);
   hipLaunchKernelGGL(k_tile_build, dim3(ntiles * chunks, maps), dim3(256), 0, c->stream, (const double2*)block, chi,
                      chi_stride, t->K, t->N, t->T, chunks);
   CGM_HIP(hipGetLastError());
-  CGM_CHECK(rocblas_set_stream(t->h, c->stream) == rocblas_status_success, CGM_ERROR_DEVICE,
-            "rocblas_set_stream failed");
   const rocblas_stride sA = (rocblas_stride)M2 * M2;
   const int nb = ntiles * maps;
   rocblas_status s = rocsolver_cgetrf_strided_batched(t->h, M2, M2, (rocblas_float_complex*)t->K, M2, sA, t->ipiv,
@@ -498,22 +845,8 @@ extern "C" int cgm_tiles_factor(cgm_tiles_t t, const cgm_complex128* block, cons
               k / ntiles, info[k]);
   s = rocsolver_cgetri_strided_batched(t->h, M2, (rocblas_float_complex*)t->K, M2, sA, t->ipiv, M2, t->info, nb);
   CGM_CHECK(s == rocblas_status_success, CGM_ERROR_DEVICE, "rocsolver getri: status %d", (int)s);
-  t->kept.assign(maps, 0.0);
-  if (t->K16) {
-    hipLaunchKernelGGL(k_tile_cols, dim3(ntiles, maps), dim3(1024), 0, c->stream, chi, chi_stride, t->N, t->ncol,
-                       t->cols, t->zmask);
-    hipLaunchKernelGGL(k_tile_maxabs, dim3(nb), dim3(1024), 0, c->stream, t->K, t->ncol, t->cols, t->kscale);
-    hipLaunchKernelGGL(k_tile_to_half, dim3(4096), dim3(256), 0, c->stream, t->K, t->K16, t->kscale, t->ncol,
-                       t->cols, (long)nb);
-    CGM_HIP(hipGetLastError());
-    std::vector<int> nc(nb);
-    CGM_HIP(hipMemcpyAsync(nc.data(), t->ncol, sizeof(int) * nb, hipMemcpyDeviceToHost, c->stream));
-    CGM_HIP(hipStreamSynchronize(c->stream));
-    for (int k = 0; k < nb; ++k) t->kept[k / ntiles] += nc[k];
-  } else {
-    for (int m = 0; m < maps; ++m) t->kept[m] = (double)ntiles * M2;
-  }
   CGM_HIP(hipStreamSynchronize(c->stream));
+  t->kept.assign(maps, (double)ntiles * M2 * M2);
   t->maps = maps;
   return CGM_SUCCESS;
 }
@@ -546,22 +879,29 @@ extern "C" int cgm_tiles_apply(cgm_tiles_t t, const cgm_complex64* x, cgm_comple
 // diagnostics: the application alone on synthetic inverses
 // ---------------------------------------------------------------------------
 namespace {
-// Synthetic inverses for one map; 64 x 64 tiles keep every column.
-__global__ void k_tile_fill(float2* __restrict__ K, __half2* __restrict__ K16, float* __restrict__ sc,
-                            int* __restrict__ ncol, unsigned short* __restrict__ cols, unsigned int* __restrict__ zmask,
-                            int M2, long total, long ntiles) {
+// Synthetic dense inverses of one map.
+__global__ void k_tile_fill(float2* __restrict__ K, long total) {
   for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
     const float v = 1e-3f * (float)(e % 7);
-    if (K16)
-      K16[e] = __floats2half2_rn(v, -v);
-    else
-      K[e] = make_float2(v, -v);
-    if (sc && e < ntiles) {
+    K[e] = make_float2(v, -v);
+  }
+}
+
+// Synthetic reduced form of one map in which every cell carries chi != 0
+// (n_t = T^2 for every tile).
+__global__ void k_tile_fill_red(__half2* __restrict__ W16, long total, float* __restrict__ sc, int* __restrict__ ncol,
+                                unsigned short* __restrict__ cols, unsigned int* __restrict__ zmask,
+                                long* __restrict__ woff, int M2, long ntiles) {
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+    const float v = 1e-3f * (float)(e % 7);
+    W16[e] = __floats2half2_rn(v, -v);
+    if (e < ntiles) {
       sc[e] = 1.f;
       ncol[e] = M2;
+      woff[e] = e * (long)M2 * M2;
     }
-    if (cols && e < ntiles * M2) cols[e] = (unsigned short)(e % M2);
-    if (zmask && e < ntiles * (M2 / 32)) zmask[e] = 0u;
+    if (e < ntiles * M2) cols[e] = (unsigned short)(e % M2);
+    if (e < ntiles * (M2 / 32)) zmask[e] = 0u;
   }
 }
 }  // namespace
@@ -573,9 +913,18 @@ extern "C" int cgm_profile_tiles(cgm_tiles_t t, int reps, float* ms) {
   CGM_HIP(hipSetDevice(c->device));
   const long ntiles = (long)t->nt * t->nt;
   const long total = (long)t->M2 * t->M2 * ntiles;  // one map
-  hipLaunchKernelGGL(k_tile_fill, dim3(4096), dim3(256), 0, c->stream, t->K, t->K16, t->kscale, t->ncol, t->cols,
-                     t->zmask, t->M2, total, ntiles);
+  t->maps = 0;
+  if (t->reduced) {
+    const int rc = grow(t, &t->W16, &t->w16_cap, (size_t)total);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_tile_fill_red, dim3(4096), dim3(256), 0, c->stream, t->W16, total, t->kscale, t->ncol,
+                       t->cols, t->zmask, t->woff, t->M2, ntiles);
+    t->nmax = t->M2;
+  } else {
+    hipLaunchKernelGGL(k_tile_fill, dim3(4096), dim3(256), 0, c->stream, t->K, total);
+  }
   CGM_HIP(hipGetLastError());
+  t->kept.assign(1, (double)total);
   const long n = (long)t->N * t->N;
   float2 *x = nullptr, *y = nullptr;
   CGM_HIP(hipMalloc((void**)&x, sizeof(float2) * n));
@@ -591,7 +940,6 @@ extern "C" int cgm_profile_tiles(cgm_tiles_t t, int reps, float* ms) {
     rc = CGM_ERROR_DEVICE;
   } else {
     t->maps = 1;
-    t->kept.assign(1, (double)ntiles * t->M2);
     FnoIO io;
     io.v = x;
     io.sV = n;
@@ -613,4 +961,3 @@ extern "C" int cgm_profile_tiles(cgm_tiles_t t, int reps, float* ms) {
   (void)hipFree(y);
   return rc;
 }
-

@@ -76,11 +76,12 @@ def test_live_roofline_takes_the_longer_kernel_and_its_own_bytes(bench):
             "upd": {"ms": 1200.0, "bytes": 8.0 * n * (3000 + 4) * 400, "launches": 400}}
     r = bench.live_roofline(live, 512, {"lr_proj": 1.0}, 3000)
     assert r["kernel"] == "k_lr_upd"
-    # the tile application when it takes longer: 64 x 64 tiles, FP16 inverses
-    live["tile"] = {"ms": 1500.0, "bytes": 400 * (64 * 4096 * 4096 * 4.0 + 16.0 * n), "launches": 400}
-    rt = bench.live_roofline(live, 512, {"lr_proj": 1.0}, 3000, tile=64)
-    assert rt["kernel"] == "k_tile_apply_h" and rt["frac"] == pytest.approx(rt["achieved"] / 8000.0)
-    assert rt["achieved"] == pytest.approx(400 * (64 * 4096 * 4096 * 4.0 + 16.0 * n) / 1.5 / 1e9)
+    # the tile application when it takes longer: 128 x 128 tiles, reduced form
+    w = 16 * 6144 ** 2 * 4.0
+    live["tile"] = {"ms": 1500.0, "bytes": 400 * (w + 16.0 * n), "launches": 400}
+    rt = bench.live_roofline(live, 512, {"lr_proj": 1.0}, 3000, tile=128)
+    assert rt["kernel"] == "k_tile_w" and rt["frac"] == pytest.approx(rt["achieved"] / 8000.0)
+    assert rt["achieved"] == pytest.approx(400 * (w + 16.0 * n) / 1.5 / 1e9)
     del live["tile"]
     assert r["achieved"] == pytest.approx(8.0 * n * 3004 * 400 / 1.2 / 1e9)
     assert r["frac"] == pytest.approx(r["achieved"] / 8000.0)
