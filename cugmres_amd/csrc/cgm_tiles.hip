@@ -65,20 +65,27 @@ struct cgm_tiles {
   size_t bytes = 0;         // device memory held
   // dense form (T <= 32)
   float2* K = nullptr;       // [B][nt * nt][M2 * M2] inverses, column-major
-  int* ipiv = nullptr;      // [B][nt * nt][M2]   (reduced form: factor workspace)
-  int* info = nullptr;      // [B][nt * nt]       (reduced form: factor workspace)
+  int* ipiv = nullptr;      // [B][nt * nt][M2]
+  int* info = nullptr;      // [B][nt * nt]       (reduced form: singular-pivot flags)
   // reduced form (T >= 64)
   int* ncol = nullptr;             // [B][nt * nt] |S_t| padded to a multiple of kPadTo
   unsigned short* cols = nullptr;  // [B][nt * nt][M2] the cells of S_t in order, then kPadCol
   unsigned int* zmask = nullptr;   // [B][nt * nt][M2 / 32] cells with chi = 0
   long* woff = nullptr;            // [B][nt * nt] offset of W_t in W16 (half2 elements)
   float* kscale = nullptr;         // [B][nt * nt] largest |W_t| entry
-  int* ids = nullptr;              // [B][nt * nt] tile indices of one size class (factor)
+  int* ids = nullptr;              // [B][nt * nt] factor members (tiles with n_t > 0), by size
+  long* koff = nullptr;            // [B][nt * nt] offset of each member's matrix in Kw
+  int* nl = nullptr;               // [B][nt * nt] each member's matrix side (n_t padded to kBlk)
   __half2* W16 = nullptr;          // packed W_t / kscale_t, column-major n_t x n_t
   size_t w16_cap = 0;              // elements
-  float2* Kw = nullptr;            // FP32 matrices of one size class (factor)
+  float2* Kw = nullptr;            // FP32 matrices of the members (factor)
   size_t kw_cap = 0;               // elements
-  size_t piv_cap = 0;              // elements of ipiv (reduced form)
+  float2* Pb = nullptr;            // [member][kBlk][kBlk] pivot blocks (factor)
+  size_t pb_cap = 0;
+  float2* Rw = nullptr;            // kBlk x n_max row block (factor)
+  size_t rw_cap = 0;
+  float2* Cw = nullptr;            // n_max x kBlk column block (factor)
+  size_t cw_cap = 0;
   float2* U = nullptr;             // [sys][nt * nt][M2] u_t scattered on the tile (apply)
   float2* Y = nullptr;             // [sys][nt * nt][M2] G_t u_t (apply)
   float2* Tm = nullptr;            // [sys][nt * nt][T * 2T] FFT scratch (apply)
@@ -97,8 +104,7 @@ namespace {
 constexpr int kBuildChunk = 65536;           // matrix entries per work-group of the build kernels
 constexpr int kPadTo = 16;                   // n_t is padded to a multiple of this
 constexpr unsigned short kPadCol = 0xFFFF;   // cell index of a padding slot
-constexpr int kClass = 256;                  // size classes of the reduced LU: multiples of this
-constexpr size_t kFactorBytes = 8ull << 30;  // FP32 workspace per batched LU call (reduced form)
+constexpr int kBlk = 128;                    // block of the Gauss-Jordan inversion (reduced form)
 constexpr int kRowsW = 512;                  // rows of W per work-group of k_tile_w
 constexpr int kChunkW = 4096;                // columns of W staged per pass of k_tile_w
 
@@ -292,24 +298,26 @@ __device__ inline long tile_cell(int t, int c, int nt, int T, int N) {
   return (long)(a * T + c / T) * N + b * T + c % T;
 }
 
-// I - D_S G_SS of the tiles ids[0 .. count) (one size class, n x n,
-// column-major; the identity on padding slots and beyond n_t): work-group
-// blockIdx.x covers chunk (blockIdx.x % chunks) of member blockIdx.x / chunks.
+// I - D_S G_SS of member blockIdx.y (tile ids[.]) into Kw + koff[.],
+// nl x nl column-major, the identity on padding slots and beyond n_t;
+// work-group blockIdx.x covers one chunk of the entries.
 __global__ void __launch_bounds__(256)
 k_tile_build_red(const double2* __restrict__ g, const float* __restrict__ chi, long chi_stride,
                  const int* __restrict__ ncol, const unsigned short* __restrict__ cols, const int* __restrict__ ids,
-                 float2* __restrict__ Kw, int n, int N, int T, int chunks) {
+                 const long* __restrict__ koff, const int* __restrict__ nl, float2* __restrict__ Kw, int N, int T) {
   const int nt = N / T;
-  const int i = blockIdx.x / chunks;
+  const int i = blockIdx.y;
+  const int n = nl[i];
   const long nn = (long)n * n;
-  const long e0 = (long)(blockIdx.x % chunks) * kBuildChunk;
+  const long e0 = (long)blockIdx.x * kBuildChunk;
+  if (e0 >= nn) return;
   const long e1 = e0 + kBuildChunk < nn ? e0 + kBuildChunk : nn;
   const int tb = ids[i];
   const int map = tb / (nt * nt), t = tb % (nt * nt);
   const int nk = ncol[tb];
   const unsigned short* ct = cols + (long)tb * T * T;
   chi += (long)map * chi_stride;
-  float2* A = Kw + (long)i * nn;
+  float2* A = Kw + koff[i];
   for (long e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
     const int p = (int)(e % n), q = (int)(e / n);
     const int cp = p < nk ? ct[p] : kPadCol, cq = q < nk ? ct[q] : kPadCol;
@@ -326,6 +334,103 @@ k_tile_build_red(const double2* __restrict__ g, const float* __restrict__ chi, l
   }
 }
 
+// Pb[i] = the kBlk x kBlk diagonal block at k0 of member i (column-major).
+__global__ void __launch_bounds__(256)
+k_gj_gather(const float2* __restrict__ Kw, const long* __restrict__ koff, const int* __restrict__ nl, int k0,
+            float2* __restrict__ Pb) {
+  const int i = blockIdx.x;
+  const int n = nl[i];
+  const float2* M = Kw + koff[i];
+  float2* P = Pb + (long)i * kBlk * kBlk;
+  for (int e = threadIdx.x; e < kBlk * kBlk; e += blockDim.x) {
+    const int r = e % kBlk, c = e / kBlk;
+    P[e] = M[(long)(k0 + c) * n + k0 + r];
+  }
+}
+
+// In-place inverse of the kBlk x kBlk block Pb[blockIdx.x] (column-major) by
+// Gauss-Jordan elimination with partial (row) pivoting in LDS: at step j the
+// largest |A(i, j)|, i >= j, is swapped into row j, row j is divided by the
+// pivot (its diagonal entry taking the pivot's inverse), every other row i
+// loses A(i, j) times row j (its column-j entry taking -A(i, j) / pivot), and
+// the row swaps are undone as column swaps in reverse order at the end.
+// info[blockIdx.x] is set when a pivot is zero.
+__global__ void __launch_bounds__(1024) k_gj_inv(float2* __restrict__ Pb, int* __restrict__ info) {
+  constexpr int L = kBlk, LP = kBlk + 1;  // LDS column stride padded against bank conflicts
+  __shared__ float2 A[L * LP];
+  __shared__ float2 f[L];
+  __shared__ int perm[L];
+  __shared__ int sp;
+  float2* P = Pb + (long)blockIdx.x * L * L;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int e = tid; e < L * L; e += blockDim.x) A[(e / L) * LP + e % L] = P[e];
+  __syncthreads();
+  bool bad = false;
+  for (int j = 0; j < L; ++j) {
+    if (wave == 0) {
+      float best = -1.f;
+      int bi = j;
+      for (int i = j + lane; i < L; i += 64) {
+        const float2 v = A[j * LP + i];
+        const float m = v.x * v.x + v.y * v.y;
+        if (m > best) {
+          best = m;
+          bi = i;
+        }
+      }
+      for (int o = 32; o > 0; o >>= 1) {
+        const float ob = __shfl_xor(best, o, 64);
+        const int oi = __shfl_xor(bi, o, 64);
+        if (ob > best || (ob == best && oi < bi)) {
+          best = ob;
+          bi = oi;
+        }
+      }
+      if (lane == 0) {
+        sp = bi;
+        perm[j] = bi;
+      }
+    }
+    __syncthreads();
+    const int p = sp;
+    if (p != j && tid < L) {  // swap rows j and p (column tid)
+      const float2 a = A[tid * LP + j];
+      A[tid * LP + j] = A[tid * LP + p];
+      A[tid * LP + p] = a;
+    }
+    __syncthreads();
+    const float2 piv = A[j * LP + j];
+    const float d = piv.x * piv.x + piv.y * piv.y;
+    bad |= !(d > 0.f);
+    const float2 ip = d > 0.f ? make_float2(piv.x / d, -piv.y / d) : make_float2(0.f, 0.f);
+    if (tid < L) f[tid] = A[j * LP + tid];  // column j
+    __syncthreads();
+    if (tid < L) {  // row j divided by the pivot
+      const float2 a = tid == j ? make_float2(1.f, 0.f) : A[tid * LP + j];
+      A[tid * LP + j] = cmul(a, ip);
+    }
+    __syncthreads();
+    for (int e = tid; e < L * L; e += blockDim.x) {
+      const int i = e % L, c = e / L;
+      if (i == j) continue;
+      const float2 a = c == j ? make_float2(0.f, 0.f) : A[c * LP + i];
+      A[c * LP + i] = csub(a, cmul(f[i], A[c * LP + j]));
+    }
+    __syncthreads();
+  }
+  for (int j = L - 1; j >= 0; --j) {
+    const int p = perm[j];
+    if (p != j && tid < L) {  // swap columns j and p (row tid)
+      const float2 a = A[j * LP + tid];
+      A[j * LP + tid] = A[p * LP + tid];
+      A[p * LP + tid] = a;
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < L * L; e += blockDim.x) P[e] = A[(e / L) * LP + e % L];
+  if (bad && tid == 0) info[blockIdx.x] = 1;
+}
+
 // chi of slot q of tile tb (0 for padding slots).
 __device__ inline float slot_chi(const float* chi, long chi_stride, const unsigned short* ct, int q, int tb, int nt,
                                  int T, int N) {
@@ -339,23 +444,25 @@ __global__ void k_tile_red_zero(const int* __restrict__ ids, int cnt, float* __r
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) sc[ids[i]] = 0.f;
 }
 
-// Largest |W| entry of each member (W = Minv D_S): work-group blockIdx.x
-// covers chunk (blockIdx.x % chunks) of member blockIdx.x / chunks and folds
-// its maximum into the member's kscale with an atomic max on the bits (the
-// values are >= 0, whose order the bits keep).
+// Largest |W| entry of member blockIdx.y (W = Minv D_S, Minv at Kw + koff,
+// side nl): work-group blockIdx.x covers one chunk of the n_t x n_t entries
+// and folds its maximum into the member's kscale with an atomic max on the
+// bits (the values are >= 0, whose order the bits keep).
 __global__ void __launch_bounds__(256)
 k_tile_red_max(const float2* __restrict__ Kw, const float* __restrict__ chi, long chi_stride,
                const int* __restrict__ ncol, const unsigned short* __restrict__ cols, const int* __restrict__ ids,
-               int n, int N, int T, int chunks, float* __restrict__ sc) {
+               const long* __restrict__ koff, const int* __restrict__ nl, int N, int T, float* __restrict__ sc) {
   const int nt = N / T;
-  const int i = blockIdx.x / chunks;
+  const int i = blockIdx.y;
   const int tb = ids[i];
   const int nk = ncol[tb];
   const long nn = (long)nk * nk;
-  const long e0 = (long)(blockIdx.x % chunks) * kBuildChunk;
+  const long e0 = (long)blockIdx.x * kBuildChunk;
+  if (e0 >= nn) return;
   const long e1 = e0 + kBuildChunk < nn ? e0 + kBuildChunk : nn;
   const unsigned short* ct = cols + (long)tb * T * T;
-  const float2* A = Kw + (long)i * n * n;
+  const float2* A = Kw + koff[i];
+  const int n = nl[i];
   float mx = 0.f;
   for (long e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
     const int p = (int)(e % nk), q = (int)(e / nk);
@@ -373,22 +480,24 @@ k_tile_red_max(const float2* __restrict__ Kw, const float* __restrict__ chi, lon
   }
 }
 
-// W16 of every member: (Minv D_S) / scale in FP16, n_t x n_t column-major at
-// woff[tb]; work-group blockIdx.x covers chunk (blockIdx.x % chunks).
+// W16 of member blockIdx.y: (Minv D_S) / scale in FP16, n_t x n_t
+// column-major at woff[tb]; work-group blockIdx.x covers one chunk.
 __global__ void __launch_bounds__(256)
 k_tile_red_pack(const float2* __restrict__ Kw, const float* __restrict__ chi, long chi_stride,
                 const int* __restrict__ ncol, const unsigned short* __restrict__ cols, const int* __restrict__ ids,
-                const long* __restrict__ woff, const float* __restrict__ sc, __half2* __restrict__ W16, int n, int N,
-                int T, int chunks) {
+                const long* __restrict__ koff, const int* __restrict__ nl, const long* __restrict__ woff,
+                const float* __restrict__ sc, __half2* __restrict__ W16, int N, int T) {
   const int nt = N / T;
-  const int i = blockIdx.x / chunks;
+  const int i = blockIdx.y;
   const int tb = ids[i];
   const int nk = ncol[tb];
   const long nn = (long)nk * nk;
-  const long e0 = (long)(blockIdx.x % chunks) * kBuildChunk;
+  const long e0 = (long)blockIdx.x * kBuildChunk;
+  if (e0 >= nn) return;
   const long e1 = e0 + kBuildChunk < nn ? e0 + kBuildChunk : nn;
   const unsigned short* ct = cols + (long)tb * T * T;
-  const float2* A = Kw + (long)i * n * n;
+  const float2* A = Kw + koff[i];
+  const int n = nl[i];
   const float r = sc[tb] > 0.f ? 1.f / sc[tb] : 0.f;  // 0: W_t = 0
   __half2* Wt = W16 + woff[tb];
   for (long e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
@@ -594,6 +703,62 @@ int upload_twiddles(cgm_tiles* t) {
   return CGM_SUCCESS;
 }
 
+// Block Gauss-Jordan inversion, in place, of the members' matrices M (side
+// nl, column-major at Kw + koff): for every block column k0 (kBlk wide) the
+// pivot block is inverted (k_gj_inv, batched over the members still that
+// large), then with P = M_kk^{-1}, R = P M[k0 rows, :] and C = M[:, k0 cols]:
+//     M -= C R;  M[k0 rows, :] = R;  M[:, k0 cols] = -C P;  M_kk = P,
+// three GEMMs and three copies per member and block.  No pivoting between
+// blocks: the pivot blocks are Schur complements of the tile operator's
+// restrictions to groups of cells, which the near-resonant but physical
+// operators keep invertible (checked: rocsolver's info flag is replaced by a
+// zero-pivot flag per member).  In complex64 on the 128 x 128 tiles of the
+// benchmark maps the inverse is within 3e-5 - 7e-5 of a complex128 one (LU
+// with partial pivoting: 1e-5 - 2e-5), far below the FP16 storage of W.
+// *bad: the first member with a zero pivot, or -1.
+int gj_invert(cgm_tiles* t, int members, const std::vector<int>& nlh, const std::vector<long>& koffh, int* bad) {
+  hipStream_t st = t->ctx->stream;
+  const int nmx = nlh[0];  // members are sorted by size, largest first
+  int rc;
+  if ((rc = grow(t, &t->Pb, &t->pb_cap, (size_t)members * kBlk * kBlk))) return rc;
+  if ((rc = grow(t, &t->Rw, &t->rw_cap, (size_t)kBlk * nmx))) return rc;
+  if ((rc = grow(t, &t->Cw, &t->cw_cap, (size_t)kBlk * nmx))) return rc;
+  CGM_HIP(hipMemsetAsync(t->info, 0, sizeof(int) * members, st));
+  const rocblas_float_complex one{1.f, 0.f}, mone{-1.f, 0.f}, zero{0.f, 0.f};
+  const rocblas_operation nn = rocblas_operation_none;
+  for (int k0 = 0; k0 < nmx; k0 += kBlk) {
+    int cnt = 0;
+    while (cnt < members && nlh[cnt] > k0) ++cnt;
+    hipLaunchKernelGGL(k_gj_gather, dim3(cnt), dim3(256), 0, st, t->Kw, t->koff, t->nl, k0, t->Pb);
+    hipLaunchKernelGGL(k_gj_inv, dim3(cnt), dim3(1024), 0, st, t->Pb, t->info);
+    CGM_HIP(hipGetLastError());
+    for (int i = 0; i < cnt; ++i) {
+      const int n = nlh[i];
+      rocblas_float_complex* M = reinterpret_cast<rocblas_float_complex*>(t->Kw + koffh[i]);
+      rocblas_float_complex* P = reinterpret_cast<rocblas_float_complex*>(t->Pb + (long)i * kBlk * kBlk);
+      rocblas_float_complex* R = reinterpret_cast<rocblas_float_complex*>(t->Rw);
+      rocblas_float_complex* C = reinterpret_cast<rocblas_float_complex*>(t->Cw);
+      rocblas_status s = rocblas_cgemm(t->h, nn, nn, kBlk, n, kBlk, &one, P, kBlk, M + k0, n, &zero, R, kBlk);
+      CGM_HIP(hipMemcpyAsync(C, M + (long)k0 * n, sizeof(float2) * (size_t)n * kBlk, hipMemcpyDeviceToDevice, st));
+      if (s == rocblas_status_success) s = rocblas_cgemm(t->h, nn, nn, n, n, kBlk, &mone, C, n, R, kBlk, &one, M, n);
+      CGM_HIP(hipMemcpy2DAsync(M + k0, sizeof(float2) * n, R, sizeof(float2) * kBlk, sizeof(float2) * kBlk, n,
+                               hipMemcpyDeviceToDevice, st));
+      if (s == rocblas_status_success)
+        s = rocblas_cgemm(t->h, nn, nn, n, kBlk, kBlk, &mone, C, n, P, kBlk, &zero, M + (long)k0 * n, n);
+      CGM_CHECK(s == rocblas_status_success, CGM_ERROR_DEVICE, "rocblas cgemm: status %d", (int)s);
+      CGM_HIP(hipMemcpy2DAsync(M + k0 + (long)k0 * n, sizeof(float2) * n, P, sizeof(float2) * kBlk,
+                               sizeof(float2) * kBlk, kBlk, hipMemcpyDeviceToDevice, st));
+    }
+  }
+  std::vector<int> info(members);
+  CGM_HIP(hipMemcpyAsync(info.data(), t->info, sizeof(int) * members, hipMemcpyDeviceToHost, st));
+  CGM_HIP(hipStreamSynchronize(st));
+  *bad = -1;
+  for (int i = 0; i < members && *bad < 0; ++i)
+    if (info[i]) *bad = i;
+  return CGM_SUCCESS;
+}
+
 int factor_reduced(cgm_tiles* t, const double2* blk, const float* chi, long chi_stride, int maps) {
   cgm_context* c = t->ctx;
   hipStream_t st = c->stream;
@@ -615,57 +780,58 @@ int factor_reduced(cgm_tiles* t, const double2* blk, const float* chi, long chi_
   std::vector<int> nc(nb);
   CGM_HIP(hipMemcpyAsync(nc.data(), t->ncol, sizeof(int) * nb, hipMemcpyDeviceToHost, st));
   CGM_HIP(hipStreamSynchronize(st));
+  // W16 layout (tile order) and the factor members (tiles with n_t > 0, largest first)
   std::vector<long> off(nb);
+  std::vector<int> mem;
   long total = 0;
-  std::map<int, std::vector<int>> classes;  // n rounded up to kClass -> tiles
   t->nmax = 0;
   for (int k = 0; k < nb; ++k) {
     off[k] = total;
     total += (long)nc[k] * nc[k];
     t->kept[k / ntiles] += (double)nc[k] * nc[k];
     t->nmax = std::max(t->nmax, nc[k]);
-    if (nc[k] > 0) classes[(nc[k] + kClass - 1) / kClass * kClass].push_back(k);
+    if (nc[k] > 0) mem.push_back(k);
   }
+  std::stable_sort(mem.begin(), mem.end(), [&](int x, int y) { return nc[x] > nc[y]; });
+  const int members = (int)mem.size();
   if ((rc = grow(t, &t->W16, &t->w16_cap, (size_t)std::max(total, 1L)))) return rc;
   CGM_HIP(hipMemcpyAsync(t->woff, off.data(), sizeof(long) * nb, hipMemcpyHostToDevice, st));
-  for (auto& cl : classes) {
-    const int n = cl.first;
-    const std::vector<int>& all = cl.second;
-    const long nn = (long)n * n;
-    const size_t per = std::max<size_t>(1, std::min<size_t>(all.size(), kFactorBytes / (sizeof(float2) * nn)));
-    for (size_t i0 = 0; i0 < all.size(); i0 += per) {
-      const int cnt = (int)std::min<size_t>(per, all.size() - i0);
-      if ((rc = grow(t, &t->Kw, &t->kw_cap, (size_t)cnt * nn))) return rc;
-      if ((rc = grow(t, &t->ipiv, &t->piv_cap, (size_t)cnt * n))) return rc;
-      CGM_HIP(hipMemcpyAsync(t->ids, all.data() + i0, sizeof(int) * cnt, hipMemcpyHostToDevice, st));
-      const int chunks = (int)((nn + kBuildChunk - 1) / kBuildChunk);
-      hipLaunchKernelGGL(k_tile_build_red, dim3(cnt * chunks), dim3(256), 0, st, blk, chi, chi_stride, t->ncol, t->cols,
-                         t->ids, t->Kw, n, N, T, chunks);
-      CGM_HIP(hipGetLastError());
-      const rocblas_stride sA = (rocblas_stride)nn;
-      rocblas_status s = rocsolver_cgetrf_strided_batched(t->h, n, n, (rocblas_float_complex*)t->Kw, n, sA, t->ipiv,
-                                                          n, t->info, cnt);
-      CGM_CHECK(s == rocblas_status_success, CGM_ERROR_DEVICE, "rocsolver getrf: status %d", (int)s);
-      std::vector<int> info(cnt);
-      CGM_HIP(hipMemcpyAsync(info.data(), t->info, sizeof(int) * cnt, hipMemcpyDeviceToHost, st));
-      CGM_HIP(hipStreamSynchronize(st));
-      for (int k = 0; k < cnt; ++k) {
-        const int tb = all[i0 + k];
-        CGM_CHECK(info[k] == 0, CGM_ERROR_NONFINITE, "tile %d of map %d is singular (pivot %d)", tb % ntiles,
-                  tb / ntiles, info[k]);
-      }
-      s = rocsolver_cgetri_strided_batched(t->h, n, (rocblas_float_complex*)t->Kw, n, sA, t->ipiv, n, t->info, cnt);
-      CGM_CHECK(s == rocblas_status_success, CGM_ERROR_DEVICE, "rocsolver getri: status %d", (int)s);
-      hipLaunchKernelGGL(k_tile_red_zero, dim3(1), dim3(256), 0, st, t->ids, cnt, t->kscale);
-      hipLaunchKernelGGL(k_tile_red_max, dim3(cnt * chunks), dim3(256), 0, st, t->Kw, chi, chi_stride, t->ncol,
-                         t->cols, t->ids, n, N, T, chunks, t->kscale);
-      hipLaunchKernelGGL(k_tile_red_pack, dim3(cnt * chunks), dim3(256), 0, st, t->Kw, chi, chi_stride, t->ncol,
-                         t->cols, t->ids, t->woff, t->kscale, t->W16, n, N, T, chunks);
-      CGM_HIP(hipGetLastError());
-      CGM_HIP(hipStreamSynchronize(st));  // the next member batch reuses Kw and ids
-    }
+  if (members == 0) {
+    CGM_HIP(hipStreamSynchronize(st));  // off is read by the copy above
+    return CGM_SUCCESS;
   }
-  CGM_HIP(hipStreamSynchronize(st));  // off is read by the copy above
+  std::vector<int> nlh(members);
+  std::vector<long> koffh(members);
+  long ktot = 0;
+  int chunks = 1;
+  for (int i = 0; i < members; ++i) {
+    nlh[i] = (nc[mem[i]] + kBlk - 1) / kBlk * kBlk;
+    koffh[i] = ktot;
+    ktot += (long)nlh[i] * nlh[i];
+    chunks = std::max(chunks, (int)(((long)nlh[i] * nlh[i] + kBuildChunk - 1) / kBuildChunk));
+  }
+  if ((rc = grow(t, &t->Kw, &t->kw_cap, (size_t)ktot))) return rc;
+  CGM_HIP(hipMemcpyAsync(t->ids, mem.data(), sizeof(int) * members, hipMemcpyHostToDevice, st));
+  CGM_HIP(hipMemcpyAsync(t->nl, nlh.data(), sizeof(int) * members, hipMemcpyHostToDevice, st));
+  CGM_HIP(hipMemcpyAsync(t->koff, koffh.data(), sizeof(long) * members, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_tile_build_red, dim3(chunks, members), dim3(256), 0, st, blk, chi, chi_stride, t->ncol, t->cols,
+                     t->ids, t->koff, t->nl, t->Kw, N, T);
+  CGM_HIP(hipGetLastError());
+  int bad = -1;
+  if ((rc = gj_invert(t, members, nlh, koffh, &bad))) return rc;
+  if (bad >= 0) {
+    const int tb = mem[bad];
+    cgm_set_last_error("tile %d of map %d: a pivot block of its reduced operator is singular", tb % ntiles,
+                       tb / ntiles);
+    return CGM_ERROR_NONFINITE;
+  }
+  hipLaunchKernelGGL(k_tile_red_zero, dim3(1), dim3(256), 0, st, t->ids, members, t->kscale);
+  hipLaunchKernelGGL(k_tile_red_max, dim3(chunks, members), dim3(256), 0, st, t->Kw, chi, chi_stride, t->ncol, t->cols,
+                     t->ids, t->koff, t->nl, N, T, t->kscale);
+  hipLaunchKernelGGL(k_tile_red_pack, dim3(chunks, members), dim3(256), 0, st, t->Kw, chi, chi_stride, t->ncol,
+                     t->cols, t->ids, t->koff, t->nl, t->woff, t->kscale, t->W16, N, T);
+  CGM_HIP(hipGetLastError());
+  CGM_HIP(hipStreamSynchronize(st));  // the host vectors are read by the copies above
   return CGM_SUCCESS;
 }
 
@@ -769,6 +935,8 @@ extern "C" int cgm_tiles_create(cgm_ctx_t c, int T, int batch, cgm_tiles_t* out)
     alloc((void**)&t->woff, sizeof(long) * nb);
     alloc((void**)&t->kscale, sizeof(float) * nb);
     alloc((void**)&t->ids, sizeof(int) * nb);
+    alloc((void**)&t->koff, sizeof(long) * nb);
+    alloc((void**)&t->nl, sizeof(int) * nb);
     alloc((void**)&t->info, sizeof(int) * nb);
     if (ok && plan_init(t->P, T) != CGM_SUCCESS) ok = false;
     const size_t M = (size_t)t->P.M;
@@ -792,8 +960,9 @@ extern "C" int cgm_tiles_create(cgm_ctx_t c, int T, int batch, cgm_tiles_t* out)
 extern "C" int cgm_tiles_destroy(cgm_tiles_t t) {
   if (!t) return CGM_SUCCESS;
   (void)hipSetDevice(t->device);
-  void* ptrs[] = {t->K,   t->ipiv, t->info, t->ncol, t->cols, t->zmask, t->woff, t->kscale, t->ids,
-                  t->W16, t->Kw,   t->U,    t->Y,    t->Tm,   t->tw32,  t->tw64, t->blkT,   t->gq};
+  void* ptrs[] = {t->K,    t->ipiv, t->info, t->ncol, t->cols, t->zmask, t->woff, t->kscale,
+                  t->ids,  t->koff, t->nl,   t->W16,  t->Kw,   t->Pb,    t->Rw,   t->Cw,
+                  t->U,    t->Y,    t->Tm,   t->tw32, t->tw64, t->blkT,  t->gq};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (t->h) (void)rocblas_destroy_handle(t->h);
