@@ -60,7 +60,7 @@ def parse(argv=None):
     p.add_argument("--restart", type=int, default=30, help="restart of plain GMRES")
     p.add_argument("--tol", type=float, default=1e-6)
     p.add_argument("--max-cycles", type=int, default=100, help="cap of plain GMRES(restart)")
-    p.add_argument("--tile", type=int, default=64, help="tile side of the tile inverses (8, 16, 32, 64, 128)")
+    p.add_argument("--tile", type=int, default=128, help="tile side of the tile inverses (8, 16, 32, 64, 128)")
     p.add_argument("--lr-restart", type=int, default=8192,
                    help="cycle length of the preconditioned solves (long-recurrence path)")
     p.add_argument("--lr-cycles", type=int, default=2)
@@ -263,10 +263,9 @@ def run_bench(a, ctx):
         tctx = solver.Context(a.train_N, 0)
         t0 = time.perf_counter()
         note(f"training the network: {a.train_samples} samples at N = {a.train_N}")
-        # pooled maps: 8 batches of 8 (64 maps) up to 32 x 32 tiles, 2 batches with
-        # 64 x 64 tiles (13 GB of inverses per batch at N = 256)
+        # pooled maps: 8 batches of 8 (64 maps), each batch's tile inverses factored once
         tnet, hist = learn.train(tctx, samples=a.train_samples, batch=8, lr=1e-3, seed=0, tiles_T=a.tile,
-                                 map_pool=8 if a.tile <= 32 else 2, log=note, log_every=200, val_every=50)
+                                 map_pool=8, log=note, log_every=200, val_every=50)
         torch.cuda.synchronize()
         note(f"trained in {time.perf_counter() - t0:.1f} s")
         params = tnet.params.cpu().numpy()

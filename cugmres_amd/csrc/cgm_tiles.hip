@@ -70,10 +70,10 @@ struct cgm_tiles {
   // reduced form (T >= 64)
   int* ncol = nullptr;             // [B][nt * nt] |S_t| padded to a multiple of kPadTo
   unsigned short* cols = nullptr;  // [B][nt * nt][M2] the cells of S_t in order, then kPadCol
-  unsigned int* zmask = nullptr;   // [B][nt * nt][M2 / 32] cells with chi = 0
   long* woff = nullptr;            // [B][nt * nt] offset of W_t in W16 (half2 elements)
   float* kscale = nullptr;         // [B][nt * nt] largest |W_t| entry
   int* ids = nullptr;              // [B][nt * nt] factor members (tiles with n_t > 0), by size
+  int* tord = nullptr;             // [B][nt * nt] each map's tiles, largest n_t first
   long* koff = nullptr;            // [B][nt * nt] offset of each member's matrix in Kw
   int* nl = nullptr;               // [B][nt * nt] each member's matrix side (n_t padded to kBlk)
   __half2* W16 = nullptr;          // packed W_t / kscale_t, column-major n_t x n_t
@@ -105,7 +105,7 @@ constexpr int kBuildChunk = 65536;           // matrix entries per work-group of
 constexpr int kPadTo = 16;                   // n_t is padded to a multiple of this
 constexpr unsigned short kPadCol = 0xFFFF;   // cell index of a padding slot
 constexpr int kBlk = 128;                    // block of the Gauss-Jordan inversion (reduced form)
-constexpr int kRowsW = 512;                  // rows of W per work-group of k_tile_w
+constexpr int kRowsW = 128;                  // rows of W per work-group of k_tile_w
 constexpr int kChunkW = 4096;                // columns of W staged per pass of k_tile_w
 
 template <typename P>
@@ -241,22 +241,19 @@ int launch_apply_m(const cgm_tiles* tl, hipStream_t st, const FnoIO& io, const i
 // ---------------------------------------------------------------------------
 
 // S_t of every tile of map blockIdx.y (tile blockIdx.x): the cells with
-// chi != 0 in row-major order, their count padded to kPadTo (padding slots
-// hold kPadCol), and the chi = 0 bitmask.
+// chi != 0 in row-major order and their count padded to kPadTo (padding
+// slots hold kPadCol).
 template <int T>
 __global__ void __launch_bounds__(1024)
 k_tile_cols(const float* __restrict__ chi, long chi_stride, int N, int* __restrict__ ncol,
-            unsigned short* __restrict__ cols, unsigned int* __restrict__ zmask) {
+            unsigned short* __restrict__ cols) {
   constexpr int M2 = T * T;
   constexpr int CPT = M2 / 1024;  // cells per thread
   const int nt = N / T;
   const int t = blockIdx.x, a = t / nt, b = t % nt;
   const long tb = (long)blockIdx.y * nt * nt + t;
   chi += (long)blockIdx.y * chi_stride;
-  __shared__ unsigned int zm[M2 / 32];
   __shared__ int wsum[16];
-  for (int w = threadIdx.x; w < M2 / 32; w += blockDim.x) zm[w] = 0u;
-  __syncthreads();
   bool nz[CPT];
   int cnt = 0;
 #pragma unroll
@@ -264,7 +261,6 @@ k_tile_cols(const float* __restrict__ chi, long chi_stride, int N, int* __restri
     const int c = CPT * threadIdx.x + i;
     nz[i] = chi[(long)(a * T + c / T) * N + b * T + c % T] != 0.f;
     cnt += nz[i];
-    if (!nz[i]) atomicOr(&zm[c >> 5], 1u << (c & 31));
   }
   // exclusive scan of the counts over the work-group (16 waves of 64)
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -288,8 +284,6 @@ k_tile_cols(const float* __restrict__ chi, long chi_stride, int N, int* __restri
   const int padded = (total + kPadTo - 1) / kPadTo * kPadTo;
   if ((int)threadIdx.x < padded - total) ct[total + threadIdx.x] = kPadCol;
   if (threadIdx.x == 0) ncol[tb] = padded;
-  __syncthreads();
-  for (int w = threadIdx.x; w < M2 / 32; w += blockDim.x) zmask[tb * (M2 / 32) + w] = zm[w];
 }
 
 // Grid index of cell c (row-major inside the tile) of tile t.
@@ -513,44 +507,41 @@ k_tile_red_pack(const float2* __restrict__ Kw, const float* __restrict__ chi, lo
 // ---------------------------------------------------------------------------
 
 // u_t = scale_t W16_t x_t[S] for system blockIdx.y, written to U at the cells
-// of S (and zero at the cells with chi = 0).  Work-group (tile, part) computes
-// rows [part kRowsW, (part+1) kRowsW) of W_t, two consecutive rows per thread
-// (one 8-byte load per column), FP32 sums, x[S] staged in LDS kChunkW columns
-// at a time; every part also clears its share of the chi = 0 cells.  x is
-// io.v scaled by 1 / *(io.nrm + s sNrm bytes) when io.nrm is given; systems
-// with *(done + s sd bytes) set are skipped.
+// of S (U holds zero at the cells with chi = 0 from the factorisation on).
+// Work-group (tile, part) computes rows [part kRowsW, (part+1) kRowsW) of
+// W_t: lane l of every wave holds rows 2l, 2l + 1 (one 8-byte load of two
+// FP16 entries per column), the four waves take alternate groups of 16
+// columns, x[S] is staged in LDS kChunkW columns at a time, and the waves'
+// FP32 sums are added in LDS at the end.  The tiles are visited largest
+// first (tord, per map), so the longest work-groups start first.  x is io.v
+// scaled by 1 / *(io.nrm + s sNrm bytes) when io.nrm is given; systems with
+// *(done + s sd bytes) set are skipped.
 template <int T>
-__global__ void __launch_bounds__(kRowsW / 2)
+__global__ void __launch_bounds__(256)
 k_tile_w(const __half2* __restrict__ W16, const long* __restrict__ woff, const float* __restrict__ ksc,
-         const int* __restrict__ ncol, const unsigned short* __restrict__ cols,
-         const unsigned int* __restrict__ zmask, int maps1, int parts, FnoIO io, float2* __restrict__ U,
-         const int* __restrict__ done, long sd, int N) {
+         const int* __restrict__ ncol, const unsigned short* __restrict__ cols, const int* __restrict__ tord,
+         int maps1, int parts, FnoIO io, float2* __restrict__ U, const int* __restrict__ done, long sd, int N) {
   constexpr int M2 = T * T;
-  constexpr int kU = 16;  // columns loaded ahead (kPadTo divides every n_t)
+  constexpr int kU = 16;  // columns per group (kPadTo divides every n_t)
   __shared__ float2 xs[kChunkW];
+  __shared__ float4 red[3][64];
   const int nt = N / T;
-  const int t = blockIdx.x / parts, part = blockIdx.x % parts;
   const int s = blockIdx.y;
   if (done && *boff(done, (long)s * sd)) return;
-  const long tb = (long)(maps1 ? 0 : s) * nt * nt + t;
+  const int map = maps1 ? 0 : s;
+  const int t = tord[map * nt * nt + blockIdx.x / parts], part = blockIdx.x % parts;
+  const long tb = (long)map * nt * nt + t;
+  const int n = ncol[tb];
+  if (part * kRowsW >= n) return;  // uniform over the work-group
   float sc = 1.f;
   if (io.nrm) {
     const double d = *boff(io.nrm, (long)s * io.sNrm);
     sc = d > 0.0 ? (float)(1.0 / d) : 0.f;
   }
-  const int n = ncol[tb];
   const unsigned short* ct = cols + tb * M2;
-  float2* Ut = U + ((long)s * nt * nt + t) * M2;
-  {
-    const unsigned int* zm = zmask + tb * (M2 / 32);
-    const int share = (M2 + parts - 1) / parts;
-    const int c1 = (part + 1) * share < M2 ? (part + 1) * share : M2;
-    for (int c = part * share + threadIdx.x; c < c1; c += blockDim.x)
-      if ((zm[c >> 5] >> (c & 31)) & 1u) Ut[c] = make_float2(0.f, 0.f);
-  }
-  if (part * kRowsW >= n) return;  // uniform over the work-group
   const float2* v = io.v + (long)s * io.sV;
-  const int r = part * kRowsW + 2 * threadIdx.x;  // rows r, r + 1 (n even)
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r = part * kRowsW + 2 * lane;  // rows r, r + 1 (n even)
   const bool act = r < n;
   typedef unsigned int u2v_t __attribute__((ext_vector_type(2)));
   const u2v_t* Wt = reinterpret_cast<const u2v_t*>(W16 + woff[tb] + (act ? r : 0));
@@ -571,7 +562,7 @@ k_tile_w(const __half2* __restrict__ W16, const long* __restrict__ woff, const f
     }
     __syncthreads();
     if (act) {
-      for (int q0 = 0; q0 < kn; q0 += kU) {
+      for (int q0 = wave * kU; q0 < kn; q0 += 4 * kU) {
         uint2 w[kU];  // each vector load is copied out at once (rows r and r + 1)
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
@@ -591,8 +582,19 @@ k_tile_w(const __half2* __restrict__ W16, const long* __restrict__ woff, const f
       }
     }
   }
-  if (!act) return;
+  if (wave > 0) red[wave - 1][lane] = make_float4(ax, ay, bx, by);
+  __syncthreads();
+  if (wave > 0 || !act) return;
+#pragma unroll
+  for (int w = 0; w < 3; ++w) {
+    const float4 o = red[w][lane];
+    ax += o.x;
+    ay += o.y;
+    bx += o.z;
+    by += o.w;
+  }
   const float m = ksc[tb];
+  float2* Ut = U + ((long)s * nt * nt + t) * M2;
   const int c0 = ct[r], c1 = ct[r + 1];
   if (c0 != kPadCol) Ut[c0] = make_float2(ax * m, ay * m);
   if (c1 != kPadCol) Ut[c1] = make_float2(bx * m, by * m);
@@ -638,6 +640,11 @@ int ensure_scratch(cgm_tiles* t, int nsys) {
     cgm_set_last_error("tile preconditioner: scratch allocation for %d systems failed", nsys);
     return CGM_ERROR_ALLOC;
   }
+  // the cells with chi = 0 keep u = 0: the application writes only the cells of S
+  if (hipMemset(t->U, 0, sizeof(float2) * n * nsys) != hipSuccess) {
+    cgm_set_last_error("tile preconditioner: scratch clear failed");
+    return CGM_ERROR_DEVICE;
+  }
   t->scratch_sys = nsys;
   t->bytes += (2 * n + tm) * sizeof(float2) * nsys;
   return CGM_SUCCESS;
@@ -659,11 +666,11 @@ int launch_apply_red(const cgm_tiles* tl, hipStream_t st, const FnoIO& io, const
   const int maps1 = tl->maps == 1 ? 1 : 0;
   if (ev) CGM_HIP(hipEventRecord(ev[0], st));
   if (tl->T == 64)
-    hipLaunchKernelGGL((k_tile_w<64>), dim3(ntiles * parts, nsys), dim3(kRowsW / 2), 0, st, tl->W16, tl->woff,
-                       tl->kscale, tl->ncol, tl->cols, tl->zmask, maps1, parts, io, tl->U, done, sd, tl->N);
+    hipLaunchKernelGGL((k_tile_w<64>), dim3(ntiles * parts, nsys), dim3(256), 0, st, tl->W16, tl->woff, tl->kscale,
+                       tl->ncol, tl->cols, tl->tord, maps1, parts, io, tl->U, done, sd, tl->N);
   else
-    hipLaunchKernelGGL((k_tile_w<128>), dim3(ntiles * parts, nsys), dim3(kRowsW / 2), 0, st, tl->W16, tl->woff,
-                       tl->kscale, tl->ncol, tl->cols, tl->zmask, maps1, parts, io, tl->U, done, sd, tl->N);
+    hipLaunchKernelGGL((k_tile_w<128>), dim3(ntiles * parts, nsys), dim3(256), 0, st, tl->W16, tl->woff, tl->kscale,
+                       tl->ncol, tl->cols, tl->tord, maps1, parts, io, tl->U, done, sd, tl->N);
   CGM_HIP(hipGetLastError());
   if (ev) CGM_HIP(hipEventRecord(ev[1], st));
   BatchStrides bs;
@@ -771,11 +778,11 @@ int factor_reduced(cgm_tiles* t, const double2* blk, const float* chi, long chi_
   int rc = spectrum_from_block(t->P, st, t->blkT, t->tw64, t->gq, nullptr);
   if (rc) return rc;
   if (T == 64)
-    hipLaunchKernelGGL((k_tile_cols<64>), dim3(ntiles, maps), dim3(1024), 0, st, chi, chi_stride, N, t->ncol, t->cols,
-                       t->zmask);
+    hipLaunchKernelGGL((k_tile_cols<64>), dim3(ntiles, maps), dim3(1024), 0, st, chi, chi_stride, N, t->ncol,
+                       t->cols);
   else
     hipLaunchKernelGGL((k_tile_cols<128>), dim3(ntiles, maps), dim3(1024), 0, st, chi, chi_stride, N, t->ncol,
-                       t->cols, t->zmask);
+                       t->cols);
   CGM_HIP(hipGetLastError());
   std::vector<int> nc(nb);
   CGM_HIP(hipMemcpyAsync(nc.data(), t->ncol, sizeof(int) * nb, hipMemcpyDeviceToHost, st));
@@ -794,10 +801,19 @@ int factor_reduced(cgm_tiles* t, const double2* blk, const float* chi, long chi_
   }
   std::stable_sort(mem.begin(), mem.end(), [&](int x, int y) { return nc[x] > nc[y]; });
   const int members = (int)mem.size();
+  std::vector<int> ord(nb);  // per map: its tiles, largest first
+  for (int m = 0; m < maps; ++m) {
+    int* o = ord.data() + (long)m * ntiles;
+    for (int k = 0; k < ntiles; ++k) o[k] = k;
+    std::stable_sort(o, o + ntiles, [&](int x, int y) { return nc[m * ntiles + x] > nc[m * ntiles + y]; });
+  }
   if ((rc = grow(t, &t->W16, &t->w16_cap, (size_t)std::max(total, 1L)))) return rc;
   CGM_HIP(hipMemcpyAsync(t->woff, off.data(), sizeof(long) * nb, hipMemcpyHostToDevice, st));
+  CGM_HIP(hipMemcpyAsync(t->tord, ord.data(), sizeof(int) * nb, hipMemcpyHostToDevice, st));
+  // u = 0 on the cells with chi = 0 of the new maps: the application writes only the cells of S
+  CGM_HIP(hipMemsetAsync(t->U, 0, sizeof(float2) * (size_t)N * N * t->scratch_sys, st));
   if (members == 0) {
-    CGM_HIP(hipStreamSynchronize(st));  // off is read by the copy above
+    CGM_HIP(hipStreamSynchronize(st));  // off and ord are read by the copies above
     return CGM_SUCCESS;
   }
   std::vector<int> nlh(members);
@@ -931,11 +947,11 @@ extern "C" int cgm_tiles_create(cgm_ctx_t c, int T, int batch, cgm_tiles_t* out)
     const size_t nb = ntiles * batch;
     alloc((void**)&t->ncol, sizeof(int) * nb);
     alloc((void**)&t->cols, sizeof(unsigned short) * M2 * nb);
-    alloc((void**)&t->zmask, sizeof(unsigned int) * (M2 / 32) * nb);
     alloc((void**)&t->woff, sizeof(long) * nb);
     alloc((void**)&t->kscale, sizeof(float) * nb);
     alloc((void**)&t->ids, sizeof(int) * nb);
     alloc((void**)&t->koff, sizeof(long) * nb);
+    alloc((void**)&t->tord, sizeof(int) * nb);
     alloc((void**)&t->nl, sizeof(int) * nb);
     alloc((void**)&t->info, sizeof(int) * nb);
     if (ok && plan_init(t->P, T) != CGM_SUCCESS) ok = false;
@@ -960,8 +976,8 @@ extern "C" int cgm_tiles_create(cgm_ctx_t c, int T, int batch, cgm_tiles_t* out)
 extern "C" int cgm_tiles_destroy(cgm_tiles_t t) {
   if (!t) return CGM_SUCCESS;
   (void)hipSetDevice(t->device);
-  void* ptrs[] = {t->K,    t->ipiv, t->info, t->ncol, t->cols, t->zmask, t->woff, t->kscale,
-                  t->ids,  t->koff, t->nl,   t->W16,  t->Kw,   t->Pb,    t->Rw,   t->Cw,
+  void* ptrs[] = {t->K,    t->ipiv, t->info, t->ncol, t->cols, t->woff, t->kscale,
+                  t->ids,  t->tord, t->koff, t->nl,   t->W16,  t->Kw,   t->Pb,    t->Rw,   t->Cw,
                   t->U,    t->Y,    t->Tm,   t->tw32, t->tw64, t->blkT,  t->gq};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -1059,7 +1075,7 @@ __global__ void k_tile_fill(float2* __restrict__ K, long total) {
 // Synthetic reduced form of one map in which every cell carries chi != 0
 // (n_t = T^2 for every tile).
 __global__ void k_tile_fill_red(__half2* __restrict__ W16, long total, float* __restrict__ sc, int* __restrict__ ncol,
-                                unsigned short* __restrict__ cols, unsigned int* __restrict__ zmask,
+                                unsigned short* __restrict__ cols, int* __restrict__ tord,
                                 long* __restrict__ woff, int M2, long ntiles) {
   for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
     const float v = 1e-3f * (float)(e % 7);
@@ -1068,9 +1084,9 @@ __global__ void k_tile_fill_red(__half2* __restrict__ W16, long total, float* __
       sc[e] = 1.f;
       ncol[e] = M2;
       woff[e] = e * (long)M2 * M2;
+      tord[e] = (int)e;
     }
     if (e < ntiles * M2) cols[e] = (unsigned short)(e % M2);
-    if (e < ntiles * (M2 / 32)) zmask[e] = 0u;
   }
 }
 }  // namespace
@@ -1087,7 +1103,7 @@ extern "C" int cgm_profile_tiles(cgm_tiles_t t, int reps, float* ms) {
     const int rc = grow(t, &t->W16, &t->w16_cap, (size_t)total);
     if (rc) return rc;
     hipLaunchKernelGGL(k_tile_fill_red, dim3(4096), dim3(256), 0, c->stream, t->W16, total, t->kscale, t->ncol,
-                       t->cols, t->zmask, t->woff, t->M2, ntiles);
+                       t->cols, t->tord, t->woff, t->M2, ntiles);
     t->nmax = t->M2;
   } else {
     hipLaunchKernelGGL(k_tile_fill, dim3(4096), dim3(256), 0, c->stream, t->K, total);
