@@ -106,6 +106,7 @@ constexpr int kPadTo = 16;                   // n_t is padded to a multiple of t
 constexpr unsigned short kPadCol = 0xFFFF;   // cell index of a padding slot
 constexpr int kBlk = 128;                    // block of the Gauss-Jordan inversion (reduced form)
 constexpr int kRowsW = 128;                  // rows of W per work-group of k_tile_w
+constexpr int kWavesW = 8;                   // waves per work-group of k_tile_w (column groups)
 constexpr int kChunkW = 4096;                // columns of W staged per pass of k_tile_w
 
 template <typename P>
@@ -510,21 +511,21 @@ k_tile_red_pack(const float2* __restrict__ Kw, const float* __restrict__ chi, lo
 // of S (U holds zero at the cells with chi = 0 from the factorisation on).
 // Work-group (tile, part) computes rows [part kRowsW, (part+1) kRowsW) of
 // W_t: lane l of every wave holds rows 2l, 2l + 1 (one 8-byte load of two
-// FP16 entries per column), the four waves take alternate groups of 16
+// FP16 entries per column), the kWavesW waves take alternate groups of 16
 // columns, x[S] is staged in LDS kChunkW columns at a time, and the waves'
 // FP32 sums are added in LDS at the end.  The tiles are visited largest
 // first (tord, per map), so the longest work-groups start first.  x is io.v
 // scaled by 1 / *(io.nrm + s sNrm bytes) when io.nrm is given; systems with
 // *(done + s sd bytes) set are skipped.
 template <int T>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(64 * kWavesW)
 k_tile_w(const __half2* __restrict__ W16, const long* __restrict__ woff, const float* __restrict__ ksc,
          const int* __restrict__ ncol, const unsigned short* __restrict__ cols, const int* __restrict__ tord,
          int maps1, int parts, FnoIO io, float2* __restrict__ U, const int* __restrict__ done, long sd, int N) {
   constexpr int M2 = T * T;
   constexpr int kU = 16;  // columns per group (kPadTo divides every n_t)
   __shared__ float2 xs[kChunkW];
-  __shared__ float4 red[3][64];
+  __shared__ float4 red[kWavesW - 1][64];
   const int nt = N / T;
   const int s = blockIdx.y;
   if (done && *boff(done, (long)s * sd)) return;
@@ -562,7 +563,7 @@ k_tile_w(const __half2* __restrict__ W16, const long* __restrict__ woff, const f
     }
     __syncthreads();
     if (act) {
-      for (int q0 = wave * kU; q0 < kn; q0 += 4 * kU) {
+      for (int q0 = wave * kU; q0 < kn; q0 += kWavesW * kU) {
         uint2 w[kU];  // each vector load is copied out at once (rows r and r + 1)
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
@@ -586,7 +587,7 @@ k_tile_w(const __half2* __restrict__ W16, const long* __restrict__ woff, const f
   __syncthreads();
   if (wave > 0 || !act) return;
 #pragma unroll
-  for (int w = 0; w < 3; ++w) {
+  for (int w = 0; w < kWavesW - 1; ++w) {
     const float4 o = red[w][lane];
     ax += o.x;
     ay += o.y;
@@ -666,10 +667,10 @@ int launch_apply_red(const cgm_tiles* tl, hipStream_t st, const FnoIO& io, const
   const int maps1 = tl->maps == 1 ? 1 : 0;
   if (ev) CGM_HIP(hipEventRecord(ev[0], st));
   if (tl->T == 64)
-    hipLaunchKernelGGL((k_tile_w<64>), dim3(ntiles * parts, nsys), dim3(256), 0, st, tl->W16, tl->woff, tl->kscale,
+    hipLaunchKernelGGL((k_tile_w<64>), dim3(ntiles * parts, nsys), dim3(64 * kWavesW), 0, st, tl->W16, tl->woff, tl->kscale,
                        tl->ncol, tl->cols, tl->tord, maps1, parts, io, tl->U, done, sd, tl->N);
   else
-    hipLaunchKernelGGL((k_tile_w<128>), dim3(ntiles * parts, nsys), dim3(256), 0, st, tl->W16, tl->woff, tl->kscale,
+    hipLaunchKernelGGL((k_tile_w<128>), dim3(ntiles * parts, nsys), dim3(64 * kWavesW), 0, st, tl->W16, tl->woff, tl->kscale,
                        tl->ncol, tl->cols, tl->tord, maps1, parts, io, tl->U, done, sd, tl->N);
   CGM_HIP(hipGetLastError());
   if (ev) CGM_HIP(hipEventRecord(ev[1], st));
