@@ -698,7 +698,7 @@ def live_roofline(live, N, per_step, jmean, tile=64):
             kern = "k_tile_w"
             ref = measured_traffic(f"{kern}_T{tile}", N)   # synthetic full tiles: scale by the bytes
             if ref:
-                traffic = ref["bytes_per_launch"] * per_launch / ref["algorithmic_bytes"]
+                traffic = ref["bytes_per_launch"] * per_launch / ref["algorithmic"]
             per_unit = (f"sum over tiles of n_t^2 x 4 B of W (FP16, n_t = cells with chi != 0 padded to 16) "
                         f"+ 16 B x N^2 per launch (T = {tile}, N = {N}): {per_launch:.4g} B, "
                         f"{t['launches']} launches of the timed solves")
@@ -733,7 +733,7 @@ def measured_traffic(kernel, N, batch=1):
     launch as measured by rocprofv3 (FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00,
     copy-calibrated) and committed in profiles/hbm_traffic_n512.json (key
     `kernel`, or `kernel`_b<batch> for batches): the entry (bytes_per_launch,
-    algorithmic_bytes, ...), or None when none matches."""
+    algorithmic, ...), or None when none matches."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic_n512.json")
     key = kernel if batch == 1 else f"{kernel}_b{batch}"
     try:

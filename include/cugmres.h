@@ -324,17 +324,23 @@ int cgm_fgmres_batched(cgm_ctx_t ctx, const cgm_complex64* dev_gamma, const floa
 /* ---- tile preconditioner ------------------------------------------------------ */
 /* No counterpart in the reference, which has no preconditioner
  * (B/CUDA_GMRES/GMRES.cuh:22-446).  The grid is cut into T x T tiles
- * (T = 8, 16, 32 or 64, N % T == 0) and P applies the exact inverse of the
- * operator restricted to each tile,
+ * (T = 8, 16, 32, 64 or 128, N % T == 0) and P applies the exact inverse of
+ * the operator restricted to each tile,
  *     (P v)_t = (I - G_t diag(chi_t))^{-1} v_t,  G_t[p, q] = g[|p_r-q_r|, |p_c-q_c|],
- * with g the N x N generator block of the operator.  The inverses are formed
- * once per map in FP32 complex (batched LU with partial pivoting) and take
- * T^4 * 8 bytes per tile: (N/T)^2 T^4 * 8 = 512 MiB per map at N = 512, T = 16.
- * With T = 64 they are applied from an FP16 copy scaled per tile (4 GiB more
- * per map at N = 512, 12 GiB in all) that holds only the columns of cells with
- * chi != 0: a cell with chi = 0 has the unit vector as its column of the tile
- * operator and therefore of its inverse, so the application adds x there and
- * streams the other columns (about 3/8 of them on the benchmark maps). */
+ * with g the N x N generator block of the operator.
+ * T <= 32: the inverses are formed once per map in FP32 complex (batched LU
+ * with partial pivoting) and take T^4 * 8 bytes per tile: (N/T)^2 T^4 * 8 =
+ * 512 MiB per map at N = 512, T = 16.
+ * T = 64, 128 (reduced form): with S the cells of the tile where chi != 0,
+ *     (P v)_t = v_t + G_t[:, S] W_t v_t[S],  W_t = (I - D_S G_SS)^{-1} D_S,
+ * D_S = diag(chi_S).  W_t (|S| x |S|, |S| padded to 16) is formed once per map
+ * by a block Gauss-Jordan inversion in FP32 complex and kept in FP16 scaled
+ * per tile; G_t acting on W_t v_t[S] is a convolution done with FFTs of size
+ * 2T.  On the benchmark maps (3/8 of the cells with chi != 0) W takes 0.72 GB
+ * at N = 512, T = 64 and 2.55 GB at T = 128; the factorisation holds an FP32
+ * workspace of twice that size until the tiles are destroyed.  The FP16
+ * storage makes P about 1e-2 less exact on the near-resonant tiles of the
+ * benchmark maps (1e-5 on low-contrast ones). */
 typedef struct cgm_tiles* cgm_tiles_t;
 
 /* The generator block g[|di|, |dj|] (N*N complex128, device memory) of the
@@ -346,20 +352,22 @@ int cgm_block_reference(cgm_ctx_t ctx, double wave_number, double row_shift, dou
 /* Workspace for up to `maps` contrast maps at the context's N. */
 int cgm_tiles_create(cgm_ctx_t ctx, int T, int maps, cgm_tiles_t* tiles);
 int cgm_tiles_destroy(cgm_tiles_t tiles);
-/* Device bytes the workspace holds. */
+/* Device bytes the workspace holds (grows with the largest factorisation). */
 int cgm_tiles_memory(cgm_tiles_t tiles, size_t* bytes);
 /* Forms and inverts every tile's operator for `maps` maps (chi is
  * [maps][N*N] with chi_stride >= N*N, or one map with maps = 1).  One map
  * serves any number of right-hand sides.  Synchronous.  Returns
- * CGM_ERROR_NONFINITE if a tile's operator is singular. */
+ * CGM_ERROR_NONFINITE if a tile's operator is singular (T <= 32) or a pivot
+ * block of its reduced matrix is (T >= 64). */
 int cgm_tiles_factor(cgm_tiles_t tiles, const cgm_complex128* dev_block, const float* dev_chi, long chi_stride,
                      int maps);
 /* y[b] = P x[b] for b < batch ([batch][N*N]; map b, or the one map).
  * Asynchronous on the context stream. */
 int cgm_tiles_apply(cgm_tiles_t tiles, const cgm_complex64* dev_x, cgm_complex64* dev_y, int batch);
 /* Algorithmic bytes of one cgm_tiles_apply over `batch` systems: the inverse
- * entries it streams (every column of every tile in FP32; for T = 64 the kept
- * columns in FP16, padded to 16 per tile) plus 16 * N*N per system. */
+ * entries it streams (every entry of every tile in FP32 for T <= 32, once per
+ * group of four right-hand sides of one map; the FP16 entries of W for
+ * T >= 64, once per system) plus 16 * N*N per system. */
 int cgm_tiles_apply_bytes(cgm_tiles_t tiles, int batch, double* bytes);
 /* Diagnostics: fills the inverses of one map with synthetic values (no
  * factorisation) and returns in *ms the average device time of `reps`
@@ -504,8 +512,8 @@ int cgm_profile_longrec(cgm_ctx_t ctx, int j, int reps, float* ms);
  * out[0..2] = projection: summed milliseconds, algorithmic bytes (8 B per
  * cell for each basis vector read and each vector read or written),
  * launches; out[3..5] = the same for the update; out[6..8] for the tile
- * inverses' application (the inverses streamed, 8 B per entry, 4 B with
- * T = 64, plus the vector in and out). */
+ * inverses' streaming kernel (k_tile_apply for T <= 32, k_tile_w for
+ * T >= 64) with the bytes of cgm_tiles_apply_bytes. */
 int cgm_longrec_timing(cgm_ctx_t ctx, int enable);
 int cgm_longrec_timing_read(cgm_ctx_t ctx, double* out);
 
