@@ -69,8 +69,8 @@ def parse(argv=None):
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU-baseline sample")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--fno-params", default=None, help="trained network (.npz from cugmres_amd.learn)")
-    p.add_argument("--train-samples", type=int, default=8000)
-    p.add_argument("--train-N", type=int, default=256)
+    p.add_argument("--train-samples", type=int, default=4000)
+    p.add_argument("--train-N", type=int, default=512)
     p.add_argument("--seed-offset", type=int, default=0)
     p.add_argument("--details", default=None, help="write the per-system figures to this JSON file")
     p.add_argument("--batch", type=int, default=16, help="vectors of the batched / fno-step modes")
@@ -263,9 +263,11 @@ def run_bench(a, ctx):
         tctx = solver.Context(a.train_N, 0)
         t0 = time.perf_counter()
         note(f"training the network: {a.train_samples} samples at N = {a.train_N}")
-        # pooled maps: 8 batches of 8 (64 maps), each batch's tile inverses factored once
+        # pooled maps: 4 batches of 8 (32 maps), each batch's tile inverses factored once (20 GB of
+        # W per batch at N = 512, T = 128); a short training at the solve's own N: longer ones lower
+        # the loss but raise the FGMRES step count
         tnet, hist = learn.train(tctx, samples=a.train_samples, batch=8, lr=1e-3, seed=0, tiles_T=a.tile,
-                                 map_pool=8, log=note, log_every=200, val_every=50)
+                                 map_pool=4, log=note, log_every=200, val_every=50)
         torch.cuda.synchronize()
         note(f"trained in {time.perf_counter() - t0:.1f} s")
         params = tnet.params.cpu().numpy()

@@ -723,6 +723,18 @@ int upload_twiddles(cgm_tiles* t) {
 // zero-pivot flag per member).  In complex64 on the 128 x 128 tiles of the
 // benchmark maps the inverse is within 3e-5 - 7e-5 of a complex128 one (LU
 // with partial pivoting: 1e-5 - 2e-5), far below the FP16 storage of W.
+// Frees the FP32 matrices and blocks of the block Gauss-Jordan inversion.
+void release_factor_workspace(cgm_tiles* t) {
+  float2** bufs[] = {&t->Kw, &t->Pb, &t->Rw, &t->Cw};
+  size_t* caps[] = {&t->kw_cap, &t->pb_cap, &t->rw_cap, &t->cw_cap};
+  for (int i = 0; i < 4; ++i) {
+    if (*bufs[i]) (void)hipFree(*bufs[i]);
+    t->bytes -= *caps[i] * sizeof(float2);
+    *bufs[i] = nullptr;
+    *caps[i] = 0;
+  }
+}
+
 // *bad: the first member with a zero pivot, or -1.
 int gj_invert(cgm_tiles* t, int members, const std::vector<int>& nlh, const std::vector<long>& koffh, int* bad) {
   hipStream_t st = t->ctx->stream;
@@ -849,6 +861,7 @@ int factor_reduced(cgm_tiles* t, const double2* blk, const float* chi, long chi_
                      t->cols, t->ids, t->koff, t->nl, t->woff, t->kscale, t->W16, N, T);
   CGM_HIP(hipGetLastError());
   CGM_HIP(hipStreamSynchronize(st));  // the host vectors are read by the copies above
+  release_factor_workspace(t);  // FP32 matrices: twice the bytes of W, needed only here
   return CGM_SUCCESS;
 }
 

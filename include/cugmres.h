@@ -338,7 +338,7 @@ int cgm_fgmres_batched(cgm_ctx_t ctx, const cgm_complex64* dev_gamma, const floa
  * per tile; G_t acting on W_t v_t[S] is a convolution done with FFTs of size
  * 2T.  On the benchmark maps (3/8 of the cells with chi != 0) W takes 0.72 GB
  * at N = 512, T = 64 and 2.55 GB at T = 128; the factorisation holds an FP32
- * workspace of twice that size until the tiles are destroyed.  The FP16
+ * workspace of twice that size while it runs.  The FP16
  * storage makes P about 1e-2 less exact on the near-resonant tiles of the
  * benchmark maps (1e-5 on low-contrast ones). */
 typedef struct cgm_tiles* cgm_tiles_t;

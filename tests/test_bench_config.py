@@ -24,7 +24,7 @@ def bench():
 def test_defaults_are_the_benchmark_configuration(bench):
     a = bench.parse([])
     assert (a.gpus, a.N, a.restart, a.tol, a.max_cycles) == (1, 512, 30, 1e-6, 100)
-    assert (a.tile, a.lr_restart, a.train_N) == (128, 8192, 256)
+    assert (a.tile, a.lr_restart, a.train_N, a.train_samples) == (128, 8192, 512, 4000)
     assert a.seed_offset == 0 and a.mode == "bench"
 
 
