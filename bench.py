@@ -617,7 +617,8 @@ def profile_tiles(ctx, tl, v, reps=20):
     N, T = ctx.N, tl.T
     b = tl.apply_bytes(1)
     full = (N // T) ** 2 * T ** 4 * (4 if T == 64 else 8)
-    return {"kernel": "tile application (k_tile_w, FFT convolution, combine)" if T >= 64 else "k_tile_apply",
+    return {"kernel": "tile application (k_tile_ws, k_tile_wsum, FFT convolution, combine)" if T >= 64
+            else "k_tile_apply",
             "us": us, "bytes": b,
             "GBps": b / (us * 1e-6) / 1e9, "kept_fraction": (b - 16 * N * N) / full,
             "per_unit": f"{b:.0f} B per launch: kept inverse columns + 16 N^2 B, N = {N}, T = {T}"}
@@ -697,11 +698,12 @@ def live_roofline(live, N, per_step, jmean, tile=64):
     traffic = None
     if key == "tile":
         if tile >= 64:
-            kern = "k_tile_w"
+            kern = "k_tile_ws"
             ref = measured_traffic(f"{kern}_T{tile}", N)   # synthetic full tiles: scale by the bytes
             if ref:
                 traffic = ref["bytes_per_launch"] * per_launch / ref["algorithmic"]
-            per_unit = (f"sum over tiles of n_t^2 x 4 B of W (FP16, n_t = cells with chi != 0 padded to 16) "
+            per_unit = (f"the stored lower triangle of every W_t, b(b+1)/2 blocks of 128 x 128 x 4 B (FP16, "
+                        f"b = ceil(n_t / 128), n_t = cells with chi != 0 padded to 16) "
                         f"+ 16 B x N^2 per launch (T = {tile}, N = {N}): {per_launch:.4g} B, "
                         f"{t['launches']} launches of the timed solves")
         else:

@@ -31,14 +31,15 @@
 // (R_S restricts to S).  Factorisation inverts only the |S| x |S| matrices
 // I - D_S G_SS (rocSOLVER, batched over tiles of one size class) and keeps
 // W_t in FP16 complex, scaled per tile by its largest magnitude.  Application:
-// u_t = W_t x_t[S] (the only large stream: |S|^2 entries per tile), then
-// G_t u_t for all tiles at once as a batch of T x T convolutions through the
-// operator's own FFT passes at size 2T with the tile's spectrum, then
-// y = x + G_t u_t.  On the benchmark maps 3/8 of the cells carry chi != 0, so
-// a 64 x 64 tile streams 14 % of the entries of its dense inverse, and the
-// LU work drops by about 20x; that makes 128 x 128 tiles affordable, which
-// take 1,032 Krylov steps on seed 0 at N = 512 against 2,380 for 64 x 64 and
-// 4,573 for 32 x 32.
+// u_t = W_t x_t[S] (the only large stream: W_t is complex symmetric, since
+// W_t = (D_S^{-1} - G_SS)^{-1}, and only the lower triangle of its 128 x 128
+// blocks is stored and streamed), then G_t u_t for all tiles at once as a
+// batch of T x T convolutions through the operator's own FFT passes at size
+// 2T with the tile's spectrum, then y = x + G_t u_t.  On the benchmark maps
+// 3/8 of the cells carry chi != 0, so a 128 x 128 tile streams about 4 % of
+// the entries of its dense inverse, and the inversion work drops by about
+// 20x; that makes 128 x 128 tiles affordable, which take 1,032 Krylov steps
+// on seed 0 at N = 512 against 2,380 for 64 x 64 and 4,573 for 32 x 32.
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
@@ -73,10 +74,19 @@ struct cgm_tiles {
   long* woff = nullptr;            // [B][nt * nt] offset of W_t in W16 (half2 elements)
   float* kscale = nullptr;         // [B][nt * nt] largest |W_t| entry
   int* ids = nullptr;              // [B][nt * nt] factor members (tiles with n_t > 0), by size
-  int* tord = nullptr;             // [B][nt * nt] each map's tiles, largest n_t first
+  int* eloc = nullptr;             // [B][nt * nt] index of each tile's first block in its map's block list
+  int* btab = nullptr;             // block lists of the maps: tile << 16 | block row << 8 | block column
+  size_t btab_cap = 0;
+  int* bstart = nullptr;           // [B] first entry of each map's block list in btab
+  int* bcount = nullptr;           // [B] blocks of each map
+  int bcount_max = 0;              // largest block count over the factored maps
+  int nbmax = 0;                   // largest number of block rows of a tile
+  float2* Pw = nullptr;            // [sys][bcount_max][2][kBlkW] partial products (apply)
+  size_t pw_cap = 0;
   long* koff = nullptr;            // [B][nt * nt] offset of each member's matrix in Kw
   int* nl = nullptr;               // [B][nt * nt] each member's matrix side (n_t padded to kBlk)
-  __half2* W16 = nullptr;          // packed W_t / kscale_t, column-major n_t x n_t
+  __half2* W16 = nullptr;          // W_t / kscale_t in FP16: the kBlkW x kBlkW blocks (i, j <= i) of the
+                                   // lower triangle, each column-major, row-major over the blocks
   size_t w16_cap = 0;              // elements
   float2* Kw = nullptr;            // FP32 matrices of the members (factor)
   size_t kw_cap = 0;               // elements
@@ -96,7 +106,6 @@ struct cgm_tiles {
   double2* blkT = nullptr;         // T x T corner of the generator block
   float2* gq = nullptr;            // quarter spectrum of the tile convolution
   std::vector<double> kept;        // per map: inverse entries streamed per application
-  int nmax = 0;                    // largest n_t over the factored maps
 };
 
 namespace {
@@ -105,9 +114,7 @@ constexpr int kBuildChunk = 65536;           // matrix entries per work-group of
 constexpr int kPadTo = 16;                   // n_t is padded to a multiple of this
 constexpr unsigned short kPadCol = 0xFFFF;   // cell index of a padding slot
 constexpr int kBlk = 128;                    // block of the Gauss-Jordan inversion (reduced form)
-constexpr int kRowsW = 128;                  // rows of W per work-group of k_tile_w
-constexpr int kWavesW = 8;                   // waves per work-group of k_tile_w (column groups)
-constexpr int kChunkW = 4096;                // columns of W staged per pass of k_tile_w
+constexpr int kBlkW = 128;                   // block of the symmetric storage of W (apply)
 
 template <typename P>
 int grow(cgm_tiles* t, P** p, size_t* cap, size_t need) {
@@ -475,31 +482,44 @@ k_tile_red_max(const float2* __restrict__ Kw, const float* __restrict__ chi, lon
   }
 }
 
-// W16 of member blockIdx.y: (Minv D_S) / scale in FP16, n_t x n_t
-// column-major at woff[tb]; work-group blockIdx.x covers one chunk.
+// W16 of member blockIdx.y: (Minv D_S) / scale in FP16, the blocks (bi, bj <=
+// bi) of its lower triangle (kBlkW x kBlkW each, column-major, zero beyond
+// n_t) in row-major block order at woff[tb]; work-group blockIdx.x covers one
+// chunk.  The matrix side nl is a multiple of kBlkW (= kBlk).
 __global__ void __launch_bounds__(256)
 k_tile_red_pack(const float2* __restrict__ Kw, const float* __restrict__ chi, long chi_stride,
                 const int* __restrict__ ncol, const unsigned short* __restrict__ cols, const int* __restrict__ ids,
                 const long* __restrict__ koff, const int* __restrict__ nl, const long* __restrict__ woff,
                 const float* __restrict__ sc, __half2* __restrict__ W16, int N, int T) {
+  constexpr int B = kBlkW;
   const int nt = N / T;
   const int i = blockIdx.y;
   const int tb = ids[i];
   const int nk = ncol[tb];
-  const long nn = (long)nk * nk;
+  const int n = nl[i];
+  const int nB = n / B;
+  const long total = (long)nB * (nB + 1) / 2 * B * B;
   const long e0 = (long)blockIdx.x * kBuildChunk;
-  if (e0 >= nn) return;
-  const long e1 = e0 + kBuildChunk < nn ? e0 + kBuildChunk : nn;
+  if (e0 >= total) return;
+  const long e1 = e0 + kBuildChunk < total ? e0 + kBuildChunk : total;
   const unsigned short* ct = cols + (long)tb * T * T;
   const float2* A = Kw + koff[i];
-  const int n = nl[i];
   const float r = sc[tb] > 0.f ? 1.f / sc[tb] : 0.f;  // 0: W_t = 0
   __half2* Wt = W16 + woff[tb];
   for (long e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
-    const int p = (int)(e % nk), q = (int)(e / nk);
-    const float cq = slot_chi(chi, chi_stride, ct, q, tb, nt, T, N) * r;
-    const float2 v = A[(long)q * n + p];
-    Wt[e] = __floats2half2_rn(v.x * cq, v.y * cq);
+    const int b = (int)(e / (B * B)), w = (int)(e % (B * B));
+    int bi = (int)((sqrtf(8.f * b + 1.f) - 1.f) * 0.5f);  // block row: bi (bi + 1) / 2 <= b
+    while (bi * (bi + 1) / 2 > b) --bi;
+    while ((bi + 1) * (bi + 2) / 2 <= b) ++bi;
+    const int bj = b - bi * (bi + 1) / 2;
+    const int p = bi * B + w % B, q = bj * B + w / B;
+    float2 v = make_float2(0.f, 0.f);
+    if (p < nk && q < nk) {
+      const float cq = slot_chi(chi, chi_stride, ct, q, tb, nt, T, N) * r;
+      const float2 a = A[(long)q * n + p];
+      v = make_float2(a.x * cq, a.y * cq);
+    }
+    Wt[e] = __floats2half2_rn(v.x, v.y);
   }
 }
 
@@ -507,102 +527,162 @@ k_tile_red_pack(const float2* __restrict__ Kw, const float* __restrict__ chi, lo
 // reduced form: application
 // ---------------------------------------------------------------------------
 
-// u_t = scale_t W16_t x_t[S] for system blockIdx.y, written to U at the cells
-// of S (U holds zero at the cells with chi = 0 from the factorisation on).
-// Work-group (tile, part) computes rows [part kRowsW, (part+1) kRowsW) of
-// W_t: lane l of every wave holds rows 2l, 2l + 1 (one 8-byte load of two
-// FP16 entries per column), the kWavesW waves take alternate groups of 16
-// columns, x[S] is staged in LDS kChunkW columns at a time, and the waves'
-// FP32 sums are added in LDS at the end.  The tiles are visited largest
-// first (tord, per map), so the longest work-groups start first.  x is io.v
-// scaled by 1 / *(io.nrm + s sNrm bytes) when io.nrm is given; systems with
-// *(done + s sd bytes) set are skipped.
+// Two-component shuffle across the wave.
+__device__ inline float2 shfl_xor2(float2 v, int o) {
+  return make_float2(__shfl_xor(v.x, o, 64), __shfl_xor(v.y, o, 64));
+}
+
+// One stage of the transposing butterfly: the 2H values of every lane become
+// H, each the sum of the lane's and its partner's (lane ^ o) value of the
+// same column, the upper half kept by the lanes with bit o set.
+template <int H>
+__device__ inline void bfly_stage(float2* c, int lane, int o) {
+  const bool up = lane & o;
+#pragma unroll
+  for (int k = 0; k < H; ++k) {
+    const float2 send = up ? c[k] : c[k + H];
+    const float2 keep = up ? c[k + H] : c[k];
+    c[k] = cadd(keep, shfl_xor2(send, o));
+  }
+}
+
+// W_t is complex symmetric (W_t = (D_S^{-1} - G_SS)^{-1}, G symmetric), so
+// only its lower triangle is stored, in kBlkW x kBlkW blocks.  Work-group e
+// of system blockIdx.y takes block (i, j) of the system's map block list and
+// writes two partial products of kBlkW values (scaled W16): B x_j (for block
+// row i) and, for j < i, B^T x_i (for block row j), x_i, x_j the tile's
+// input at the cells of S in block rows i and j (io.v scaled by
+// 1 / *(io.nrm + s sNrm bytes) when io.nrm is given).  Lane l of every wave
+// holds rows 2l, 2l + 1 of the block (one 8-byte load per column), the four
+// waves take 32 columns each; the row sums meet in LDS, and the 32 column
+// sums of each wave are reduced across its lanes by a transposing butterfly
+// (lane l ends with column l / 2).  Systems with *(done + s sd bytes) set
+// are skipped.
 template <int T>
-__global__ void __launch_bounds__(64 * kWavesW)
-k_tile_w(const __half2* __restrict__ W16, const long* __restrict__ woff, const float* __restrict__ ksc,
-         const int* __restrict__ ncol, const unsigned short* __restrict__ cols, const int* __restrict__ tord,
-         int maps1, int parts, FnoIO io, float2* __restrict__ U, const int* __restrict__ done, long sd, int N) {
-  constexpr int M2 = T * T;
-  constexpr int kU = 16;  // columns per group (kPadTo divides every n_t)
-  __shared__ float2 xs[kChunkW];
-  __shared__ float4 red[kWavesW - 1][64];
+__global__ void __launch_bounds__(256)
+k_tile_ws(const __half2* __restrict__ W16, const long* __restrict__ woff, const int* __restrict__ ncol,
+          const unsigned short* __restrict__ cols, const int* __restrict__ btab, const int* __restrict__ bstart,
+          const int* __restrict__ bcount, int maps1, FnoIO io, float2* __restrict__ Pw, long spw,
+          const int* __restrict__ done, long sd, int N) {
+  constexpr int M2 = T * T, B = kBlkW;
+  __shared__ float2 xj[B], xi[B];
+  __shared__ float4 red[3][64];
   const int nt = N / T;
-  const int s = blockIdx.y;
+  const int s = blockIdx.y, map = maps1 ? 0 : s, e = blockIdx.x;
+  if (e >= bcount[map]) return;
   if (done && *boff(done, (long)s * sd)) return;
-  const int map = maps1 ? 0 : s;
-  const int t = tord[map * nt * nt + blockIdx.x / parts], part = blockIdx.x % parts;
+  const int pk = btab[bstart[map] + e];
+  const int t = pk >> 16, i = (pk >> 8) & 255, j = pk & 255;
   const long tb = (long)map * nt * nt + t;
   const int n = ncol[tb];
-  if (part * kRowsW >= n) return;  // uniform over the work-group
+  const unsigned short* ct = cols + tb * M2;
   float sc = 1.f;
   if (io.nrm) {
     const double d = *boff(io.nrm, (long)s * io.sNrm);
     sc = d > 0.0 ? (float)(1.0 / d) : 0.f;
   }
-  const unsigned short* ct = cols + tb * M2;
-  const float2* v = io.v + (long)s * io.sV;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int r = part * kRowsW + 2 * lane;  // rows r, r + 1 (n even)
-  const bool act = r < n;
-  typedef unsigned int u2v_t __attribute__((ext_vector_type(2)));
-  const u2v_t* Wt = reinterpret_cast<const u2v_t*>(W16 + woff[tb] + (act ? r : 0));
-  const long colw = n / 2;  // one column in 8-byte units
-  float ax = 0.f, ay = 0.f, bx = 0.f, by = 0.f;
-  for (int k0 = 0; k0 < n; k0 += kChunkW) {
-    const int kn = n - k0 < kChunkW ? n - k0 : kChunkW;
-    __syncthreads();  // the previous chunk is consumed
-    for (int k = threadIdx.x; k < kn; k += blockDim.x) {
-      const int c = ct[k0 + k];
-      float2 xv = make_float2(0.f, 0.f);
+  {
+    const int h = threadIdx.x >> 7, r = threadIdx.x & (B - 1);
+    const int p = (h ? i : j) * B + r;
+    float2 xv = make_float2(0.f, 0.f);
+    if (p < n) {
+      const int c = ct[p];
       if (c != kPadCol) {
-        xv = v[tile_cell(t, c, nt, T, N)];
+        xv = io.v[(long)s * io.sV + tile_cell(t, c, nt, T, N)];
         xv.x *= sc;
         xv.y *= sc;
       }
-      xs[k] = xv;
     }
-    __syncthreads();
-    if (act) {
-      for (int q0 = wave * kU; q0 < kn; q0 += kWavesW * kU) {
-        uint2 w[kU];  // each vector load is copied out at once (rows r and r + 1)
+    (h ? xi : xj)[r] = xv;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r0 = 2 * lane, q0 = 32 * wave;
+  typedef unsigned int u2v_t __attribute__((ext_vector_type(2)));
+  const u2v_t* Bp =
+      reinterpret_cast<const u2v_t*>(W16 + woff[tb] + (long)(i * (i + 1) / 2 + j) * B * B + r0);
+  const float2 x0 = xi[r0], x1 = xi[r0 + 1];
+  const bool off = i != j;  // uniform over the work-group
+  float2* P = Pw + (long)s * spw + (long)e * 2 * B;
+  float ax = 0.f, ay = 0.f, bx = 0.f, by = 0.f;
+#pragma unroll 1
+  for (int g = 0; g < 2; ++g) {
+    uint2 w[16];  // each vector load is copied out at once (rows r0 and r0 + 1)
 #pragma unroll
-        for (int u = 0; u < kU; ++u) {
-          const u2v_t l = __builtin_nontemporal_load(Wt + (long)(k0 + q0 + u) * colw);
-          w[u] = make_uint2(l.x, l.y);
-        }
+    for (int u = 0; u < 16; ++u) {
+      const u2v_t l = __builtin_nontemporal_load(Bp + (long)(q0 + 16 * g + u) * (B / 2));
+      w[u] = make_uint2(l.x, l.y);
+    }
+    float2 c[16];
 #pragma unroll
-        for (int u = 0; u < kU; ++u) {
-          const float2 xv = xs[q0 + u];
-          const float2 k0v = __half22float2(__builtin_bit_cast(__half2, w[u].x));
-          const float2 k1v = __half22float2(__builtin_bit_cast(__half2, w[u].y));
-          ax += k0v.x * xv.x - k0v.y * xv.y;
-          ay += k0v.x * xv.y + k0v.y * xv.x;
-          bx += k1v.x * xv.x - k1v.y * xv.y;
-          by += k1v.x * xv.y + k1v.y * xv.x;
-        }
-      }
+    for (int u = 0; u < 16; ++u) {
+      const float2 xv = xj[q0 + 16 * g + u];
+      const float2 k0 = __half22float2(__builtin_bit_cast(__half2, w[u].x));
+      const float2 k1 = __half22float2(__builtin_bit_cast(__half2, w[u].y));
+      ax += k0.x * xv.x - k0.y * xv.y;
+      ay += k0.x * xv.y + k0.y * xv.x;
+      bx += k1.x * xv.x - k1.y * xv.y;
+      by += k1.x * xv.y + k1.y * xv.x;
+      c[u] = cadd(cmul(k0, x0), cmul(k1, x1));
+    }
+    if (off) {  // the 16 column sums of B^T x_i over the wave: lane l ends with column l / 4
+      bfly_stage<8>(c, lane, 32);
+      bfly_stage<4>(c, lane, 16);
+      bfly_stage<2>(c, lane, 8);
+      bfly_stage<1>(c, lane, 4);
+      c[0] = cadd(c[0], shfl_xor2(c[0], 2));
+      c[0] = cadd(c[0], shfl_xor2(c[0], 1));
+      if ((lane & 3) == 0) P[B + q0 + 16 * g + (lane >> 2)] = c[0];
     }
   }
   if (wave > 0) red[wave - 1][lane] = make_float4(ax, ay, bx, by);
   __syncthreads();
-  if (wave > 0 || !act) return;
+  if (wave > 0) return;
 #pragma unroll
-  for (int w = 0; w < kWavesW - 1; ++w) {
+  for (int w = 0; w < 3; ++w) {
     const float4 o = red[w][lane];
     ax += o.x;
     ay += o.y;
     bx += o.z;
     by += o.w;
   }
-  const float m = ksc[tb];
-  float2* Ut = U + ((long)s * nt * nt + t) * M2;
-  const int c0 = ct[r], c1 = ct[r + 1];
-  if (c0 != kPadCol) Ut[c0] = make_float2(ax * m, ay * m);
-  if (c1 != kPadCol) Ut[c1] = make_float2(bx * m, by * m);
+  P[r0] = make_float2(ax, ay);
+  P[r0 + 1] = make_float2(bx, by);
+}
+
+// u_t of block row blockIdx.x % nbmax of tile blockIdx.x / nbmax, system
+// blockIdx.y: the partial products of the blocks (i, j <= i) of the row and
+// (k > i, i) of the column, summed in that fixed order, times the tile's
+// scale, written to U at the cells of S.
+template <int T>
+__global__ void __launch_bounds__(kBlkW)
+k_tile_wsum(const float* __restrict__ ksc, const int* __restrict__ ncol, const unsigned short* __restrict__ cols,
+            const int* __restrict__ eloc, int maps1, int nbmax, const float2* __restrict__ Pw, long spw,
+            float2* __restrict__ U, const int* __restrict__ done, long sd, int N) {
+  constexpr int M2 = T * T, B = kBlkW;
+  const int nt = N / T;
+  const int s = blockIdx.y, map = maps1 ? 0 : s;
+  const int t = blockIdx.x / nbmax, i = blockIdx.x % nbmax;
+  const long tb = (long)map * nt * nt + t;
+  const int n = ncol[tb], nB = (n + B - 1) / B;
+  if (i >= nB) return;
+  if (done && *boff(done, (long)s * sd)) return;
+  const int r = threadIdx.x, p = i * B + r;
+  const float2* P = Pw + (long)s * spw + (long)eloc[tb] * 2 * B;
+  float2 acc = make_float2(0.f, 0.f);
+  for (int j = 0; j <= i; ++j) acc = cadd(acc, P[(long)(i * (i + 1) / 2 + j) * 2 * B + r]);
+  for (int k = i + 1; k < nB; ++k) acc = cadd(acc, P[(long)(k * (k + 1) / 2 + i) * 2 * B + B + r]);
+  if (p < n) {
+    const int c = cols[tb * M2 + p];
+    if (c != kPadCol) {
+      const float m = ksc[tb];
+      U[((long)s * nt * nt + t) * M2 + c] = make_float2(acc.x * m, acc.y * m);
+    }
+  }
 }
 
 // z = x + Y (Y in tile-major order) for system blockIdx.y; x is io.v scaled
-// as in k_tile_w and is written to io.vw when io.nrm and io.vw are given.
+// as in k_tile_ws and is written to io.vw when io.nrm and io.vw are given.
 template <int T>
 __global__ void __launch_bounds__(256)
 k_tile_combine(FnoIO io, const float2* __restrict__ Y, const int* __restrict__ done, long sd, int N) {
@@ -648,10 +728,10 @@ int ensure_scratch(cgm_tiles* t, int nsys) {
   }
   t->scratch_sys = nsys;
   t->bytes += (2 * n + tm) * sizeof(float2) * nsys;
-  return CGM_SUCCESS;
+  return grow(t, &t->Pw, &t->pw_cap, (size_t)nsys * t->bcount_max * 2 * kBlkW);
 }
 
-// ev, when given: events recorded around k_tile_w (live timing).
+// ev, when given: events recorded around k_tile_ws (live timing).
 int launch_apply_red(const cgm_tiles* tl, hipStream_t st, const FnoIO& io, const int* done, long sd, int nsys,
                      hipEvent_t* ev) {
   cgm_tiles* t = const_cast<cgm_tiles*>(tl);  // the apply scratch grows with the batch
@@ -663,17 +743,29 @@ int launch_apply_red(const cgm_tiles* tl, hipStream_t st, const FnoIO& io, const
     cgm_set_last_error("%d systems exceed the tile convolution's batch limit", nsys);
     return CGM_ERROR_UNSUPPORTED;
   }
-  const int parts = tl->nmax > kRowsW ? (tl->nmax + kRowsW - 1) / kRowsW : 1;
   const int maps1 = tl->maps == 1 ? 1 : 0;
-  if (ev) CGM_HIP(hipEventRecord(ev[0], st));
-  if (tl->T == 64)
-    hipLaunchKernelGGL((k_tile_w<64>), dim3(ntiles * parts, nsys), dim3(64 * kWavesW), 0, st, tl->W16, tl->woff, tl->kscale,
-                       tl->ncol, tl->cols, tl->tord, maps1, parts, io, tl->U, done, sd, tl->N);
-  else
-    hipLaunchKernelGGL((k_tile_w<128>), dim3(ntiles * parts, nsys), dim3(64 * kWavesW), 0, st, tl->W16, tl->woff, tl->kscale,
-                       tl->ncol, tl->cols, tl->tord, maps1, parts, io, tl->U, done, sd, tl->N);
-  CGM_HIP(hipGetLastError());
-  if (ev) CGM_HIP(hipEventRecord(ev[1], st));
+  const long spw = (long)tl->bcount_max * 2 * kBlkW;
+  if (tl->bcount_max > 0) {
+    if (ev) CGM_HIP(hipEventRecord(ev[0], st));
+    if (tl->T == 64)
+      hipLaunchKernelGGL((k_tile_ws<64>), dim3(tl->bcount_max, nsys), dim3(256), 0, st, tl->W16, tl->woff, tl->ncol,
+                         tl->cols, tl->btab, tl->bstart, tl->bcount, maps1, io, tl->Pw, spw, done, sd, tl->N);
+    else
+      hipLaunchKernelGGL((k_tile_ws<128>), dim3(tl->bcount_max, nsys), dim3(256), 0, st, tl->W16, tl->woff, tl->ncol,
+                         tl->cols, tl->btab, tl->bstart, tl->bcount, maps1, io, tl->Pw, spw, done, sd, tl->N);
+    CGM_HIP(hipGetLastError());
+    if (ev) CGM_HIP(hipEventRecord(ev[1], st));
+    if (tl->T == 64)
+      hipLaunchKernelGGL((k_tile_wsum<64>), dim3(ntiles * tl->nbmax, nsys), dim3(kBlkW), 0, st, tl->kscale, tl->ncol,
+                         tl->cols, tl->eloc, maps1, tl->nbmax, tl->Pw, spw, tl->U, done, sd, tl->N);
+    else
+      hipLaunchKernelGGL((k_tile_wsum<128>), dim3(ntiles * tl->nbmax, nsys), dim3(kBlkW), 0, st, tl->kscale, tl->ncol,
+                         tl->cols, tl->eloc, maps1, tl->nbmax, tl->Pw, spw, tl->U, done, sd, tl->N);
+    CGM_HIP(hipGetLastError());
+  } else if (ev) {  // no cell of chi != 0: nothing streamed
+    CGM_HIP(hipEventRecord(ev[0], st));
+    CGM_HIP(hipEventRecord(ev[1], st));
+  }
   BatchStrides bs;
   bs.x = tl->M2;
   bs.y = tl->M2;
@@ -694,6 +786,53 @@ int launch_apply_red(const cgm_tiles* tl, hipStream_t st, const FnoIO& io, const
     hipLaunchKernelGGL((k_tile_combine<128>), dim3((n + 255) / 256, nsys), dim3(256), 0, st, io, tl->Y, done, sd,
                        tl->N);
   CGM_HIP(hipGetLastError());
+  return CGM_SUCCESS;
+}
+
+// The storage of W of maps 0 .. maps-1 from n_t (nc, [maps][tiles]): block
+// lists, offsets, entries streamed per map; uploads them and sizes W16 and the
+// partial products.
+int layout_reduced(cgm_tiles* t, const std::vector<int>& nc, int maps) {
+  hipStream_t st = t->ctx->stream;
+  const int ntiles = t->nt * t->nt;
+  const int nb = ntiles * maps;
+  constexpr long BB = (long)kBlkW * kBlkW;
+  std::vector<long> off(nb);
+  std::vector<int> eloc(nb), btab, bstart(maps), bcount(maps);
+  long total = 0;
+  t->kept.assign(maps, 0.0);
+  t->nbmax = 0;
+  t->bcount_max = 0;
+  for (int m = 0; m < maps; ++m) {
+    bstart[m] = (int)btab.size();
+    int le = 0;
+    for (int tl = 0; tl < ntiles; ++tl) {
+      const int k = m * ntiles + tl;
+      const int nB = (nc[k] + kBlkW - 1) / kBlkW;
+      const int nblk = nB * (nB + 1) / 2;
+      off[k] = total;
+      total += nblk * BB;
+      t->kept[m] += (double)nblk * BB;
+      eloc[k] = le;
+      le += nblk;
+      t->nbmax = std::max(t->nbmax, nB);
+      for (int i = 0; i < nB; ++i)
+        for (int j = 0; j <= i; ++j) btab.push_back(tl << 16 | i << 8 | j);
+    }
+    bcount[m] = le;
+    t->bcount_max = std::max(t->bcount_max, le);
+  }
+  int rc;
+  if ((rc = grow(t, &t->W16, &t->w16_cap, (size_t)std::max(total, 1L)))) return rc;
+  if ((rc = grow(t, &t->btab, &t->btab_cap, std::max<size_t>(btab.size(), 1)))) return rc;
+  if ((rc = grow(t, &t->Pw, &t->pw_cap, (size_t)t->scratch_sys * t->bcount_max * 2 * kBlkW))) return rc;
+  CGM_HIP(hipMemcpyAsync(t->woff, off.data(), sizeof(long) * nb, hipMemcpyHostToDevice, st));
+  CGM_HIP(hipMemcpyAsync(t->eloc, eloc.data(), sizeof(int) * nb, hipMemcpyHostToDevice, st));
+  if (!btab.empty())
+    CGM_HIP(hipMemcpyAsync(t->btab, btab.data(), sizeof(int) * btab.size(), hipMemcpyHostToDevice, st));
+  CGM_HIP(hipMemcpyAsync(t->bstart, bstart.data(), sizeof(int) * maps, hipMemcpyHostToDevice, st));
+  CGM_HIP(hipMemcpyAsync(t->bcount, bcount.data(), sizeof(int) * maps, hipMemcpyHostToDevice, st));
+  CGM_HIP(hipStreamSynchronize(st));  // the host vectors are read by the copies above
   return CGM_SUCCESS;
 }
 
@@ -800,33 +939,17 @@ int factor_reduced(cgm_tiles* t, const double2* blk, const float* chi, long chi_
   std::vector<int> nc(nb);
   CGM_HIP(hipMemcpyAsync(nc.data(), t->ncol, sizeof(int) * nb, hipMemcpyDeviceToHost, st));
   CGM_HIP(hipStreamSynchronize(st));
-  // W16 layout (tile order) and the factor members (tiles with n_t > 0, largest first)
-  std::vector<long> off(nb);
+  // the storage of W and the factor members (tiles with n_t > 0, largest first)
+  if ((rc = layout_reduced(t, nc, maps))) return rc;
   std::vector<int> mem;
-  long total = 0;
-  t->nmax = 0;
-  for (int k = 0; k < nb; ++k) {
-    off[k] = total;
-    total += (long)nc[k] * nc[k];
-    t->kept[k / ntiles] += (double)nc[k] * nc[k];
-    t->nmax = std::max(t->nmax, nc[k]);
+  for (int k = 0; k < nb; ++k)
     if (nc[k] > 0) mem.push_back(k);
-  }
   std::stable_sort(mem.begin(), mem.end(), [&](int x, int y) { return nc[x] > nc[y]; });
   const int members = (int)mem.size();
-  std::vector<int> ord(nb);  // per map: its tiles, largest first
-  for (int m = 0; m < maps; ++m) {
-    int* o = ord.data() + (long)m * ntiles;
-    for (int k = 0; k < ntiles; ++k) o[k] = k;
-    std::stable_sort(o, o + ntiles, [&](int x, int y) { return nc[m * ntiles + x] > nc[m * ntiles + y]; });
-  }
-  if ((rc = grow(t, &t->W16, &t->w16_cap, (size_t)std::max(total, 1L)))) return rc;
-  CGM_HIP(hipMemcpyAsync(t->woff, off.data(), sizeof(long) * nb, hipMemcpyHostToDevice, st));
-  CGM_HIP(hipMemcpyAsync(t->tord, ord.data(), sizeof(int) * nb, hipMemcpyHostToDevice, st));
   // u = 0 on the cells with chi = 0 of the new maps: the application writes only the cells of S
   CGM_HIP(hipMemsetAsync(t->U, 0, sizeof(float2) * (size_t)N * N * t->scratch_sys, st));
   if (members == 0) {
-    CGM_HIP(hipStreamSynchronize(st));  // off and ord are read by the copies above
+    CGM_HIP(hipStreamSynchronize(st));
     return CGM_SUCCESS;
   }
   std::vector<int> nlh(members);
@@ -900,7 +1023,7 @@ int tiles_apply_io(const cgm_tiles* tl, hipStream_t st, const FnoIO& io, int nsy
     return CGM_ERROR_INVALID_VALUE;
   }
   // live timing inside a long cycle (cgm_longrec_timing): events around the
-  // streaming kernel (k_tile_apply, or k_tile_w of the reduced form) and the
+  // streaming kernel (k_tile_apply, or k_tile_ws of the reduced form) and the
   // application's algorithmic bytes
   LrTiming* tm = tl->ctx->lrt.on ? &tl->ctx->lrt : nullptr;
   const int sl = tm ? tm->cur : -1;
@@ -965,7 +1088,9 @@ extern "C" int cgm_tiles_create(cgm_ctx_t c, int T, int batch, cgm_tiles_t* out)
     alloc((void**)&t->kscale, sizeof(float) * nb);
     alloc((void**)&t->ids, sizeof(int) * nb);
     alloc((void**)&t->koff, sizeof(long) * nb);
-    alloc((void**)&t->tord, sizeof(int) * nb);
+    alloc((void**)&t->eloc, sizeof(int) * nb);
+    alloc((void**)&t->bstart, sizeof(int) * batch);
+    alloc((void**)&t->bcount, sizeof(int) * batch);
     alloc((void**)&t->nl, sizeof(int) * nb);
     alloc((void**)&t->info, sizeof(int) * nb);
     if (ok && plan_init(t->P, T) != CGM_SUCCESS) ok = false;
@@ -991,7 +1116,8 @@ extern "C" int cgm_tiles_destroy(cgm_tiles_t t) {
   if (!t) return CGM_SUCCESS;
   (void)hipSetDevice(t->device);
   void* ptrs[] = {t->K,    t->ipiv, t->info, t->ncol, t->cols, t->woff, t->kscale,
-                  t->ids,  t->tord, t->koff, t->nl,   t->W16,  t->Kw,   t->Pb,    t->Rw,   t->Cw,
+                  t->ids,  t->eloc, t->btab, t->bstart, t->bcount, t->Pw, t->koff, t->nl,
+                  t->W16,  t->Kw,   t->Pb,    t->Rw,   t->Cw,
                   t->U,    t->Y,    t->Tm,   t->tw32, t->tw64, t->blkT,  t->gq};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -1087,18 +1213,16 @@ __global__ void k_tile_fill(float2* __restrict__ K, long total) {
 }
 
 // Synthetic reduced form of one map in which every cell carries chi != 0
-// (n_t = T^2 for every tile).
+// (n_t = T^2 for every tile): W16 (total entries), unit scales, identity
+// cell lists.
 __global__ void k_tile_fill_red(__half2* __restrict__ W16, long total, float* __restrict__ sc, int* __restrict__ ncol,
-                                unsigned short* __restrict__ cols, int* __restrict__ tord,
-                                long* __restrict__ woff, int M2, long ntiles) {
+                                unsigned short* __restrict__ cols, int M2, long ntiles) {
   for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
     const float v = 1e-3f * (float)(e % 7);
     W16[e] = __floats2half2_rn(v, -v);
     if (e < ntiles) {
       sc[e] = 1.f;
       ncol[e] = M2;
-      woff[e] = e * (long)M2 * M2;
-      tord[e] = (int)e;
     }
     if (e < ntiles * M2) cols[e] = (unsigned short)(e % M2);
   }
@@ -1114,16 +1238,15 @@ extern "C" int cgm_profile_tiles(cgm_tiles_t t, int reps, float* ms) {
   const long total = (long)t->M2 * t->M2 * ntiles;  // one map
   t->maps = 0;
   if (t->reduced) {
-    const int rc = grow(t, &t->W16, &t->w16_cap, (size_t)total);
+    const int rc = layout_reduced(t, std::vector<int>(ntiles, t->M2), 1);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_tile_fill_red, dim3(4096), dim3(256), 0, c->stream, t->W16, total, t->kscale, t->ncol,
-                       t->cols, t->tord, t->woff, t->M2, ntiles);
-    t->nmax = t->M2;
+    hipLaunchKernelGGL(k_tile_fill_red, dim3(4096), dim3(256), 0, c->stream, t->W16, (long)t->kept[0], t->kscale,
+                       t->ncol, t->cols, t->M2, ntiles);
   } else {
     hipLaunchKernelGGL(k_tile_fill, dim3(4096), dim3(256), 0, c->stream, t->K, total);
+    t->kept.assign(1, (double)total);
   }
   CGM_HIP(hipGetLastError());
-  t->kept.assign(1, (double)total);
   const long n = (long)t->N * t->N;
   float2 *x = nullptr, *y = nullptr;
   CGM_HIP(hipMalloc((void**)&x, sizeof(float2) * n));

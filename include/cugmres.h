@@ -334,11 +334,12 @@ int cgm_fgmres_batched(cgm_ctx_t ctx, const cgm_complex64* dev_gamma, const floa
  * T = 64, 128 (reduced form): with S the cells of the tile where chi != 0,
  *     (P v)_t = v_t + G_t[:, S] W_t v_t[S],  W_t = (I - D_S G_SS)^{-1} D_S,
  * D_S = diag(chi_S).  W_t (|S| x |S|, |S| padded to 16) is formed once per map
- * by a block Gauss-Jordan inversion in FP32 complex and kept in FP16 scaled
- * per tile; G_t acting on W_t v_t[S] is a convolution done with FFTs of size
- * 2T.  On the benchmark maps (3/8 of the cells with chi != 0) W takes 0.72 GB
- * at N = 512, T = 64 and 2.55 GB at T = 128; the factorisation holds an FP32
- * workspace of twice that size while it runs.  The FP16
+ * by a block Gauss-Jordan inversion in FP32 complex; it is complex symmetric,
+ * and the lower triangle of its 128 x 128 blocks is kept in FP16 scaled per
+ * tile; G_t acting on W_t v_t[S] is a convolution done with FFTs of size 2T.
+ * On the benchmark maps (3/8 of the cells with chi != 0) W takes 0.41 GB at
+ * N = 512, T = 64 and 1.33 GB at T = 128; the factorisation holds an FP32
+ * workspace of the full matrices while it runs.  The FP16
  * storage makes P about 1e-2 less exact on the near-resonant tiles of the
  * benchmark maps (1e-5 on low-contrast ones). */
 typedef struct cgm_tiles* cgm_tiles_t;
@@ -366,7 +367,7 @@ int cgm_tiles_factor(cgm_tiles_t tiles, const cgm_complex128* dev_block, const f
 int cgm_tiles_apply(cgm_tiles_t tiles, const cgm_complex64* dev_x, cgm_complex64* dev_y, int batch);
 /* Algorithmic bytes of one cgm_tiles_apply over `batch` systems: the inverse
  * entries it streams (every entry of every tile in FP32 for T <= 32, once per
- * group of four right-hand sides of one map; the FP16 entries of W for
+ * group of four right-hand sides of one map; the stored FP16 blocks of W for
  * T >= 64, once per system) plus 16 * N*N per system. */
 int cgm_tiles_apply_bytes(cgm_tiles_t tiles, int batch, double* bytes);
 /* Diagnostics: fills the inverses of one map with synthetic values (no
@@ -512,7 +513,7 @@ int cgm_profile_longrec(cgm_ctx_t ctx, int j, int reps, float* ms);
  * out[0..2] = projection: summed milliseconds, algorithmic bytes (8 B per
  * cell for each basis vector read and each vector read or written),
  * launches; out[3..5] = the same for the update; out[6..8] for the tile
- * inverses' streaming kernel (k_tile_apply for T <= 32, k_tile_w for
+ * inverses' streaming kernel (k_tile_apply for T <= 32, k_tile_ws for
  * T >= 64) with the bytes of cgm_tiles_apply_bytes. */
 int cgm_longrec_timing(cgm_ctx_t ctx, int enable);
 int cgm_longrec_timing_read(cgm_ctx_t ctx, double* out);

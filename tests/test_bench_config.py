@@ -80,7 +80,7 @@ def test_live_roofline_takes_the_longer_kernel_and_its_own_bytes(bench):
     w = 16 * 6144 ** 2 * 4.0
     live["tile"] = {"ms": 1500.0, "bytes": 400 * (w + 16.0 * n), "launches": 400}
     rt = bench.live_roofline(live, 512, {"lr_proj": 1.0}, 3000, tile=128)
-    assert rt["kernel"] == "k_tile_w" and rt["frac"] == pytest.approx(rt["achieved"] / 8000.0)
+    assert rt["kernel"] == "k_tile_ws" and rt["frac"] == pytest.approx(rt["achieved"] / 8000.0)
     assert rt["achieved"] == pytest.approx(400 * (w + 16.0 * n) / 1.5 / 1e9)
     del live["tile"]
     assert r["achieved"] == pytest.approx(8.0 * n * 3004 * 400 / 1.2 / 1e9)

@@ -142,8 +142,9 @@ def test_reduced_tiles_on_empty_full_and_ragged_tiles(orc):
     """64 x 64 tiles in the reduced form: two maps of the sparse-pattern kind
     (own map per system) against the oracle's reduced form (1e-2, FP16 W);
     on the tile with chi = 0 everywhere the output is the input bit for bit;
-    the bytes per application are n_t^2 x 4 B over the tiles (n_t = |S_t|
-    padded to 16, counted here from chi) + 16 N^2 B per system."""
+    the bytes per application are the stored lower triangle of every W_t,
+    b(b + 1)/2 blocks of 128 x 128 x 4 B with b = ceil(n_t / 128) (n_t =
+    |S_t| padded to 16, counted here from chi), + 16 N^2 B per system."""
     import torch
     from cugmres_amd import problems, solver, tiles as gt
     from oracle import tiles
@@ -167,7 +168,8 @@ def test_reduced_tiles_on_empty_full_and_ragged_tiles(orc):
         for a in range(2):
             for b in range(2):
                 nz = int(np.count_nonzero(c[64 * a:64 * a + 64, 64 * b:64 * b + 64]))
-                want += (-(-nz // 16) * 16) ** 2 * 4.0
+                nb_ = -(-(-(-nz // 16) * 16) // 128)
+                want += nb_ * (nb_ + 1) // 2 * 128 * 128 * 4.0
         want += 16.0 * N * N
     assert tl.apply_bytes(2) == want
     tl.close()
