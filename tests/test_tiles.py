@@ -320,6 +320,50 @@ def test_fgmres_batched_with_tiles_equals_single():
     ctx.close()
 
 
+@pytest.mark.gpu
+def test_reduced_tiles_shared_map_many_rhs_and_batched_solves(orc):
+    """64 x 64 tiles in the reduced form on one map shared by several
+    right-hand sides: five vectors in one application (more systems than the
+    workspace was created for, so the apply scratch grows) against the
+    oracle's reduced form, the same as five single applications bit for bit,
+    and lockstep long-recurrence FGMRES of three plane waves (systems finish
+    at different steps, so the done flags skip the finished ones) equal to
+    their single solves."""
+    import torch
+    from cugmres_amd import problems, solver, tiles as gt
+    from oracle import tiles
+    N, T = 128, 64
+    print("tiles reduced form shared map N=128 T=64", flush=True)
+    ctx = solver.Context(N, 0)
+    g, g64 = ctx.spectrum_helmholtz(problems.KH, fp64=True)
+    ctx.set_flexible_fp64(g64)
+    blk = ctx.block_helmholtz(problems.KH)
+    ref_blk = orc.baseline_green_block(N, problems.KH)
+    chi, _ = _problem(N, 77)
+    dchi = ctx.to_device(chi, "f32")
+    tl = gt.Tiles(ctx, T, 1).factor(blk, dchi)
+    rng = np.random.default_rng(5)
+    V = (rng.standard_normal((5, N * N)) + 1j * rng.standard_normal((5, N * N))).astype(np.complex64)
+    dV = torch.as_tensor(V).to(ctx.device)
+    y = tl.apply(dV).cpu().numpy()
+    R = tiles.tile_reduced(ref_blk, chi, N, T)
+    for k in range(5):
+        assert rel(y[k], tiles.tile_apply_reduced(R, ref_blk, V[k].astype(np.complex128), N, T)) < 1e-2
+        assert np.array_equal(tl.apply(dV[k].contiguous()).cpu().numpy(), y[k])
+    bs = torch.stack([ctx.to_device(problems.plane_wave(N, problems.KH, th).reshape(-1)) for th in (0.3, 1.9, 4.0)])
+    xb, infos = ctx.fgmres_batched(g, dchi, bs, precond=tl, restart=600, tol=1e-6, max_cycles=2)
+    its = []
+    for k in range(3):
+        x1, i1 = ctx.fgmres(g, dchi, bs[k], precond=tl, restart=600, tol=1e-6, max_cycles=2, stall_ratio=0.0)
+        assert infos[k]["converged"] and i1["converged"], (infos[k], i1)
+        assert infos[k]["iterations"] == i1["iterations"]
+        assert rel(xb[k].cpu().numpy(), x1.cpu().numpy()) < 1e-5
+        its.append(i1["iterations"])
+    print("steps", its, flush=True)
+    tl.close()
+    ctx.close()
+
+
 def _oracle_right_pc_gmres(A, P, b, tol, m=400):
     """Unrestarted right-preconditioned GMRES in FP64 numpy (CGS2), for the
     reference solutions of the tests above: x = P y, min ||b - A P y||."""
