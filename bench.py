@@ -483,7 +483,8 @@ def c3_run(a, rhs=16, mem_budget=150e9):
         grp = int(max(1, min(rhs, mem_budget // per_rhs)))
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        tl = tiles.Tiles(ctx, a.tile if N % a.tile == 0 else 16, 1).factor(gb, chi)
+        T = max(t for t in (a.tile, 64, 32, 16) if t <= a.tile and N % t == 0)   # largest tile that divides N
+        tl = tiles.Tiles(ctx, T, 1).factor(gb, chi)
         infos = []
         for g0 in range(0, rhs, grp):
             bg = b[g0:g0 + grp].contiguous()
@@ -497,7 +498,7 @@ def c3_run(a, rhs=16, mem_budget=150e9):
         tl.close()
         conv = sum(1 for i in infos if i["converged"])
         e = per_n.setdefault(N, {"instances": 0, "rhs": 0, "converged": 0, "seconds": 0.0, "restart": m,
-                                 "lockstep_group": grp, "iterations": [], "residuals": []})
+                                 "lockstep_group": grp, "tile": T, "iterations": [], "residuals": []})
         e["instances"] += 1
         e["rhs"] += rhs
         e["converged"] += conv
