@@ -29,7 +29,8 @@
 // vectors, and
 //     A_t^{-1} = I + G_t[:, S] W_t R_S,   W_t = (I - D_S G_SS)^{-1} D_S
 // (R_S restricts to S).  Factorisation inverts only the |S| x |S| matrices
-// I - D_S G_SS (rocSOLVER, batched over tiles of one size class) and keeps
+// D_S^{-1} - G_SS = D_S^{-1} (I - D_S G_SS), whose inverse is W_t (a symmetric
+// block sweep with rocBLAS updates and an LDS pivot-block kernel), and keeps
 // W_t in FP16 complex, scaled per tile by its largest magnitude.  Application:
 // u_t = W_t x_t[S] (the only large stream: W_t is complex symmetric, since
 // W_t = (D_S^{-1} - G_SS)^{-1}, and only the lower triangle of its 128 x 128
@@ -300,9 +301,10 @@ __device__ inline long tile_cell(int t, int c, int nt, int T, int N) {
   return (long)(a * T + c / T) * N + b * T + c % T;
 }
 
-// I - D_S G_SS of member blockIdx.y (tile ids[.]) into Kw + koff[.],
-// nl x nl column-major, the identity on padding slots and beyond n_t;
-// work-group blockIdx.x covers one chunk of the entries.
+// A = D_S^{-1} - G_SS of member blockIdx.y (tile ids[.]) into Kw + koff[.],
+// nl x nl column-major, the identity on padding slots and beyond n_t (A is
+// complex symmetric: G is, D_S is real diagonal; W_t = A^{-1}); work-group
+// blockIdx.x covers one chunk of the entries.
 __global__ void __launch_bounds__(256)
 k_tile_build_red(const double2* __restrict__ g, const float* __restrict__ chi, long chi_stride,
                  const int* __restrict__ ncol, const unsigned short* __restrict__ cols, const int* __restrict__ ids,
@@ -329,14 +331,15 @@ k_tile_build_red(const double2* __restrict__ g, const float* __restrict__ chi, l
       const int dr = pr > qr ? pr - qr : qr - pr;
       const int dc = pc > qc ? pc - qc : qc - pc;
       const double2 gv = g[(long)dr * N + dc];
-      const double cv = chi[tile_cell(t, cp, nt, T, N)];
-      v = make_float2((float)((p == q ? 1.0 : 0.0) - gv.x * cv), (float)(-gv.y * cv));
+      const double d = p == q ? 1.0 / (double)chi[tile_cell(t, cp, nt, T, N)] : 0.0;
+      v = make_float2((float)(d - gv.x), (float)(-gv.y));
     }
     A[e] = v;
   }
 }
 
-// Pb[i] = the kBlk x kBlk diagonal block at k0 of member i (column-major).
+// Pb[i] = the symmetric kBlk x kBlk diagonal block at k0 of member i
+// (column-major), mirrored from the lower triangle where the sweep keeps it.
 __global__ void __launch_bounds__(256)
 k_gj_gather(const float2* __restrict__ Kw, const long* __restrict__ koff, const int* __restrict__ nl, int k0,
             float2* __restrict__ Pb) {
@@ -346,8 +349,58 @@ k_gj_gather(const float2* __restrict__ Kw, const long* __restrict__ koff, const 
   float2* P = Pb + (long)i * kBlk * kBlk;
   for (int e = threadIdx.x; e < kBlk * kBlk; e += blockDim.x) {
     const int r = e % kBlk, c = e / kBlk;
-    P[e] = M[(long)(k0 + c) * n + k0 + r];
+    const int hi = r > c ? r : c, lo = r > c ? c : r;
+    P[e] = M[(long)(k0 + lo) * n + k0 + hi];
   }
+}
+
+// Column block C (n x kBlk, column-major) of pivot block k0 of one member's
+// symmetric matrix M (lower triangle kept): rows i < k0 from the row block
+// (M(k0 + c, i)), rows in the pivot block mirrored, rows i >= k0 + kBlk from
+// the column block (M(i, k0 + c)).  32 x 32 tiles through LDS, so that the
+// transposed part reads and writes along columns.
+__global__ void __launch_bounds__(256)
+k_sw_gather(const float2* __restrict__ M, int n, int k0, float2* __restrict__ C) {
+  __shared__ float2 tile[32][33];
+  const int i0 = blockIdx.x * 32, c0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int k = ty; k < 32; k += 8) {
+    // tile[a][b] = C(i0 + a, c0 + b), every read along a column of M
+    if (i0 < k0) {  // C(i0 + k, c0 + tx) = M(k0 + c0 + tx, i0 + k): column i0 + k, contiguous in tx
+      tile[k][tx] = M[(long)(i0 + k) * n + k0 + c0 + tx];
+    } else {
+      const int i = i0 + tx, c = k0 + c0 + k;  // C(i, c - k0) = M(max, min) of (i, c)
+      const int hi = i > c ? i : c, lo = i > c ? c : i;
+      tile[tx][k] = M[(long)lo * n + hi];
+    }
+  }
+  __syncthreads();
+  for (int k = ty; k < 32; k += 8) C[(long)(c0 + k) * n + i0 + tx] = tile[tx][k];
+}
+
+// Writes back the swept column block: CP = C P (n x kBlk) goes to rows
+// i >= k0 + kBlk of the column block and, transposed, to the row block for
+// rows i < k0; the pivot block becomes -P (its lower triangle is used).
+__global__ void __launch_bounds__(256)
+k_sw_scatter(float2* __restrict__ M, int n, int k0, const float2* __restrict__ CP, const float2* __restrict__ P) {
+  __shared__ float2 tile[32][33];
+  const int i0 = blockIdx.x * 32, c0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  if (i0 >= k0 && i0 < k0 + kBlk) {  // the pivot block: -P
+    for (int k = ty; k < 32; k += 8) {
+      const float2 v = P[(long)(c0 + k) * kBlk + i0 - k0 + tx];
+      M[(long)(k0 + c0 + k) * n + i0 + tx] = make_float2(-v.x, -v.y);
+    }
+    return;
+  }
+  if (i0 >= k0) {  // below: column block, straight copy along columns
+    for (int k = ty; k < 32; k += 8) M[(long)(k0 + c0 + k) * n + i0 + tx] = CP[(long)(c0 + k) * n + i0 + tx];
+    return;
+  }
+  for (int k = ty; k < 32; k += 8) tile[k][tx] = CP[(long)(c0 + k) * n + i0 + tx];  // CP(i0 + tx, c0 + k)
+  __syncthreads();
+  // M(k0 + c0 + tx, i0 + k) = CP(i0 + k, c0 + tx): column i0 + k of M, contiguous in tx
+  for (int k = ty; k < 32; k += 8) M[(long)(i0 + k) * n + k0 + c0 + tx] = tile[tx][k];
 }
 
 // In-place inverse of the kBlk x kBlk block Pb[blockIdx.x] (column-major) by
@@ -433,28 +486,20 @@ __global__ void __launch_bounds__(1024) k_gj_inv(float2* __restrict__ Pb, int* _
   if (bad && tid == 0) info[blockIdx.x] = 1;
 }
 
-// chi of slot q of tile tb (0 for padding slots).
-__device__ inline float slot_chi(const float* chi, long chi_stride, const unsigned short* ct, int q, int tb, int nt,
-                                 int T, int N) {
-  const int cq = ct[q];
-  if (cq == kPadCol) return 0.f;
-  return chi[(long)(tb / (nt * nt)) * chi_stride + tile_cell(tb % (nt * nt), cq, nt, T, N)];
-}
-
 // kscale of every member set to 0 before k_tile_red_max.
 __global__ void k_tile_red_zero(const int* __restrict__ ids, int cnt, float* __restrict__ sc) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) sc[ids[i]] = 0.f;
 }
 
-// Largest |W| entry of member blockIdx.y (W = Minv D_S, Minv at Kw + koff,
-// side nl): work-group blockIdx.x covers one chunk of the n_t x n_t entries
-// and folds its maximum into the member's kscale with an atomic max on the
-// bits (the values are >= 0, whose order the bits keep).
+// Largest |W| entry of member blockIdx.y (the swept matrix at Kw + koff, side
+// nl, holds -W in its lower triangle): work-group blockIdx.x covers one chunk
+// of the n_t x n_t entries and folds its maximum into the member's kscale
+// with an atomic max on the bits (the values are >= 0, whose order the bits
+// keep).
 __global__ void __launch_bounds__(256)
-k_tile_red_max(const float2* __restrict__ Kw, const float* __restrict__ chi, long chi_stride,
-               const int* __restrict__ ncol, const unsigned short* __restrict__ cols, const int* __restrict__ ids,
-               const long* __restrict__ koff, const int* __restrict__ nl, int N, int T, float* __restrict__ sc) {
-  const int nt = N / T;
+k_tile_red_max(const float2* __restrict__ Kw, const int* __restrict__ ncol, const unsigned short* __restrict__ cols,
+               const int* __restrict__ ids, const long* __restrict__ koff, const int* __restrict__ nl, int T,
+               float* __restrict__ sc) {
   const int i = blockIdx.y;
   const int tb = ids[i];
   const int nk = ncol[tb];
@@ -468,9 +513,9 @@ k_tile_red_max(const float2* __restrict__ Kw, const float* __restrict__ chi, lon
   float mx = 0.f;
   for (long e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
     const int p = (int)(e % nk), q = (int)(e / nk);
-    const float cq = fabsf(slot_chi(chi, chi_stride, ct, q, tb, nt, T, N));
+    if (p < q || ct[p] == kPadCol || ct[q] == kPadCol) continue;  // the lower triangle holds -W
     const float2 v = A[(long)q * n + p];
-    mx = fmaxf(mx, fmaxf(fabsf(v.x), fabsf(v.y)) * cq);
+    mx = fmaxf(mx, fmaxf(fabsf(v.x), fabsf(v.y)));
   }
   __shared__ float red[4];
   for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
@@ -482,17 +527,15 @@ k_tile_red_max(const float2* __restrict__ Kw, const float* __restrict__ chi, lon
   }
 }
 
-// W16 of member blockIdx.y: (Minv D_S) / scale in FP16, the blocks (bi, bj <=
-// bi) of its lower triangle (kBlkW x kBlkW each, column-major, zero beyond
+// W16 of member blockIdx.y: W / scale in FP16 (W = minus the swept matrix),
+// the blocks (bi, bj <= bi) of its lower triangle (kBlkW x kBlkW each, column-major, zero beyond
 // n_t) in row-major block order at woff[tb]; work-group blockIdx.x covers one
 // chunk.  The matrix side nl is a multiple of kBlkW (= kBlk).
 __global__ void __launch_bounds__(256)
-k_tile_red_pack(const float2* __restrict__ Kw, const float* __restrict__ chi, long chi_stride,
-                const int* __restrict__ ncol, const unsigned short* __restrict__ cols, const int* __restrict__ ids,
-                const long* __restrict__ koff, const int* __restrict__ nl, const long* __restrict__ woff,
-                const float* __restrict__ sc, __half2* __restrict__ W16, int N, int T) {
+k_tile_red_pack(const float2* __restrict__ Kw, const int* __restrict__ ncol, const unsigned short* __restrict__ cols,
+                const int* __restrict__ ids, const long* __restrict__ koff, const int* __restrict__ nl,
+                const long* __restrict__ woff, const float* __restrict__ sc, __half2* __restrict__ W16, int T) {
   constexpr int B = kBlkW;
-  const int nt = N / T;
   const int i = blockIdx.y;
   const int tb = ids[i];
   const int nk = ncol[tb];
@@ -514,10 +557,9 @@ k_tile_red_pack(const float2* __restrict__ Kw, const float* __restrict__ chi, lo
     const int bj = b - bi * (bi + 1) / 2;
     const int p = bi * B + w % B, q = bj * B + w / B;
     float2 v = make_float2(0.f, 0.f);
-    if (p < nk && q < nk) {
-      const float cq = slot_chi(chi, chi_stride, ct, q, tb, nt, T, N) * r;
-      const float2 a = A[(long)q * n + p];
-      v = make_float2(a.x * cq, a.y * cq);
+    if (p < nk && q < nk && ct[p] != kPadCol && ct[q] != kPadCol) {
+      const float2 a = p >= q ? A[(long)q * n + p] : A[(long)p * n + q];  // the sweep leaves -W, lower
+      v = make_float2(-a.x * r, -a.y * r);
     }
     Wt[e] = __floats2half2_rn(v.x, v.y);
   }
@@ -850,19 +892,24 @@ int upload_twiddles(cgm_tiles* t) {
   return CGM_SUCCESS;
 }
 
-// Block Gauss-Jordan inversion, in place, of the members' matrices M (side
-// nl, column-major at Kw + koff): for every block column k0 (kBlk wide) the
-// pivot block is inverted (k_gj_inv, batched over the members still that
-// large), then with P = M_kk^{-1}, R = P M[k0 rows, :] and C = M[:, k0 cols]:
-//     M -= C R;  M[k0 rows, :] = R;  M[:, k0 cols] = -C P;  M_kk = P,
-// three GEMMs and three copies per member and block.  No pivoting between
-// blocks: the pivot blocks are Schur complements of the tile operator's
-// restrictions to groups of cells, which the near-resonant but physical
-// operators keep invertible (checked: rocsolver's info flag is replaced by a
-// zero-pivot flag per member).  In complex64 on the 128 x 128 tiles of the
-// benchmark maps the inverse is within 3e-5 - 7e-5 of a complex128 one (LU
-// with partial pivoting: 1e-5 - 2e-5), far below the FP16 storage of W.
-// Frees the FP32 matrices and blocks of the block Gauss-Jordan inversion.
+// Symmetric block sweep (Gauss-Jordan on a symmetric matrix), in place, of
+// the members' matrices A = D_S^{-1} - G_SS (side nl, column-major at
+// Kw + koff, only the lower triangle kept): for every block column k0
+// (kBlk wide) the pivot block is inverted (k_gj_inv, batched over the members
+// still that large), then with P = A_kk^{-1} and C the block column k0 of the
+// symmetric matrix,
+//     A -= (C P) C^T (lower triangle, rocBLAS csyrkx);  A[:, k0 cols] = C P
+//     (rows outside the pivot; the rows above land in the block row);  A_kk = -P.
+// The matrix stays symmetric at every step (the sweep operator), and after
+// the last block it holds -A^{-1} = -W.  No pivoting between blocks: the pivot
+// blocks are Schur complements of the tile operator's restrictions to groups
+// of cells, which the near-resonant but physical operators keep invertible (a
+// zero pivot is reported per member).  In complex64 on the 128 x 128 tiles of
+// the benchmark maps the result is within 3.5e-5 - 5.5e-5 of a complex128
+// inverse (the unsymmetric block Gauss-Jordan: 3e-5 - 7e-5; LU with partial
+// pivoting: 1e-5 - 2e-5), far below the FP16 storage of W, for half the
+// flops of the unsymmetric form.
+// Frees the FP32 matrices and blocks of the symmetric block sweep.
 void release_factor_workspace(cgm_tiles* t) {
   float2** bufs[] = {&t->Kw, &t->Pb, &t->Rw, &t->Cw};
   size_t* caps[] = {&t->kw_cap, &t->pb_cap, &t->rw_cap, &t->cw_cap};
@@ -893,20 +940,21 @@ int gj_invert(cgm_tiles* t, int members, const std::vector<int>& nlh, const std:
     CGM_HIP(hipGetLastError());
     for (int i = 0; i < cnt; ++i) {
       const int n = nlh[i];
-      rocblas_float_complex* M = reinterpret_cast<rocblas_float_complex*>(t->Kw + koffh[i]);
-      rocblas_float_complex* P = reinterpret_cast<rocblas_float_complex*>(t->Pb + (long)i * kBlk * kBlk);
-      rocblas_float_complex* R = reinterpret_cast<rocblas_float_complex*>(t->Rw);
+      float2* M = t->Kw + koffh[i];
+      float2* P = t->Pb + (long)i * kBlk * kBlk;
+      const dim3 tiles32(n / 32, kBlk / 32);
+      hipLaunchKernelGGL(k_sw_gather, tiles32, dim3(256), 0, st, M, n, k0, t->Cw);
+      CGM_HIP(hipGetLastError());
+      rocblas_float_complex* Mr = reinterpret_cast<rocblas_float_complex*>(M);
+      rocblas_float_complex* Pr = reinterpret_cast<rocblas_float_complex*>(P);
       rocblas_float_complex* C = reinterpret_cast<rocblas_float_complex*>(t->Cw);
-      rocblas_status s = rocblas_cgemm(t->h, nn, nn, kBlk, n, kBlk, &one, P, kBlk, M + k0, n, &zero, R, kBlk);
-      CGM_HIP(hipMemcpyAsync(C, M + (long)k0 * n, sizeof(float2) * (size_t)n * kBlk, hipMemcpyDeviceToDevice, st));
-      if (s == rocblas_status_success) s = rocblas_cgemm(t->h, nn, nn, n, n, kBlk, &mone, C, n, R, kBlk, &one, M, n);
-      CGM_HIP(hipMemcpy2DAsync(M + k0, sizeof(float2) * n, R, sizeof(float2) * kBlk, sizeof(float2) * kBlk, n,
-                               hipMemcpyDeviceToDevice, st));
-      if (s == rocblas_status_success)
-        s = rocblas_cgemm(t->h, nn, nn, n, kBlk, kBlk, &mone, C, n, P, kBlk, &zero, M + (long)k0 * n, n);
-      CGM_CHECK(s == rocblas_status_success, CGM_ERROR_DEVICE, "rocblas cgemm: status %d", (int)s);
-      CGM_HIP(hipMemcpy2DAsync(M + k0 + (long)k0 * n, sizeof(float2) * n, P, sizeof(float2) * kBlk,
-                               sizeof(float2) * kBlk, kBlk, hipMemcpyDeviceToDevice, st));
+      rocblas_float_complex* CP = reinterpret_cast<rocblas_float_complex*>(t->Rw);
+      rocblas_status s = rocblas_cgemm(t->h, nn, nn, n, kBlk, kBlk, &one, C, n, Pr, kBlk, &zero, CP, n);
+      if (s == rocblas_status_success)  // lower triangle of M -= (C P) C^T
+        s = rocblas_csyrkx(t->h, rocblas_fill_lower, nn, n, kBlk, &mone, CP, n, C, n, &one, Mr, n);
+      CGM_CHECK(s == rocblas_status_success, CGM_ERROR_DEVICE, "rocblas cgemm / csyrkx: status %d", (int)s);
+      hipLaunchKernelGGL(k_sw_scatter, tiles32, dim3(256), 0, st, M, n, k0, t->Rw, P);
+      CGM_HIP(hipGetLastError());
     }
   }
   std::vector<int> info(members);
@@ -978,10 +1026,10 @@ int factor_reduced(cgm_tiles* t, const double2* blk, const float* chi, long chi_
     return CGM_ERROR_NONFINITE;
   }
   hipLaunchKernelGGL(k_tile_red_zero, dim3(1), dim3(256), 0, st, t->ids, members, t->kscale);
-  hipLaunchKernelGGL(k_tile_red_max, dim3(chunks, members), dim3(256), 0, st, t->Kw, chi, chi_stride, t->ncol, t->cols,
-                     t->ids, t->koff, t->nl, N, T, t->kscale);
-  hipLaunchKernelGGL(k_tile_red_pack, dim3(chunks, members), dim3(256), 0, st, t->Kw, chi, chi_stride, t->ncol,
-                     t->cols, t->ids, t->koff, t->nl, t->woff, t->kscale, t->W16, N, T);
+  hipLaunchKernelGGL(k_tile_red_max, dim3(chunks, members), dim3(256), 0, st, t->Kw, t->ncol, t->cols, t->ids,
+                     t->koff, t->nl, T, t->kscale);
+  hipLaunchKernelGGL(k_tile_red_pack, dim3(chunks, members), dim3(256), 0, st, t->Kw, t->ncol, t->cols, t->ids,
+                     t->koff, t->nl, t->woff, t->kscale, t->W16, T);
   CGM_HIP(hipGetLastError());
   CGM_HIP(hipStreamSynchronize(st));  // the host vectors are read by the copies above
   release_factor_workspace(t);  // FP32 matrices: twice the bytes of W, needed only here

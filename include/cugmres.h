@@ -334,7 +334,8 @@ int cgm_fgmres_batched(cgm_ctx_t ctx, const cgm_complex64* dev_gamma, const floa
  * T = 64, 128 (reduced form): with S the cells of the tile where chi != 0,
  *     (P v)_t = v_t + G_t[:, S] W_t v_t[S],  W_t = (I - D_S G_SS)^{-1} D_S,
  * D_S = diag(chi_S).  W_t (|S| x |S|, |S| padded to 16) is formed once per map
- * by a block Gauss-Jordan inversion in FP32 complex; it is complex symmetric,
+ * by a symmetric block sweep of D_S^{-1} - G_SS in FP32 complex; it is complex
+ * symmetric,
  * and the lower triangle of its 128 x 128 blocks is kept in FP16 scaled per
  * tile; G_t acting on W_t v_t[S] is a convolution done with FFTs of size 2T.
  * On the benchmark maps (3/8 of the cells with chi != 0) W takes 0.41 GB at
