@@ -429,7 +429,7 @@ def fft_reference(a, B=16, reps=20):
     out["matvec_batched"] = {"us": us, "bytes": mv_bytes, "GBps": mv_bytes / (us * 1e-6) / 1e9}
     if M == 1024:   # both pass-2 kernels, whatever the autotuner chose
         auto = out["radix_plan"]["wide_from_batch"]
-        for name, wide in (("radix8", 0), ("radix32", 1)):
+        for name, wide in (("lds_radix8", 0), ("one_wave", 1)):
             ctx.set_radix_plan(wide)
             u = timed(lambda: ctx.matvec_batched(g, chi, x, 0, y))
             out["matvec_batched_" + name] = {"us": u, "GBps": mv_bytes / (u * 1e-6) / 1e9}

@@ -198,8 +198,8 @@ extern "C" int cgm_autotune(cgm_ctx_t c) {
       if (t2 < best2) { best2 = t2; c2 = k; }
     }
   }
-  // M = 1024: the register-resident radix-32 x 32 kernel of each pass
-  // against the tuned LDS-staged radix-8 one, at batches of 1, 4 and 16
+  // M = 1024: the register-resident one-wave kernel of each pass against
+  // the tuned LDS-staged radix-8 one, at batches of 1, 4 and 16
   // vectors; a pass takes the register-resident kernel from the smallest batch
   // at which it is faster at that batch and at every larger one
   int wide_from[3] = {0, 0, 0};

@@ -169,149 +169,232 @@ k_pass2(cplx<T>* __restrict__ Tm, const G* __restrict__ gq, int N, int cols_per_
 }
 
 // ---------------------------------------------------------------------------
-// Pass 2 for M = 1024 in registers (complex64; chosen by the autotuner for
-// batches).  1024 = 32 x 32: 32 lanes transform one column, each lane holding
-// 32 values; a wave takes two columns and a work-group of four waves one
-// panel of kPW = 8 columns.  Forward transform, element n = n1 + 32 n2 in
-// lane n1, register n2:
-//   A: Y[n1][k1] = DFT32 over n2 (registers), times W1024^(n1 k1);
-//   exchange through LDS (lane k1 gathers Y[.][k1]);
-//   B: X[k1 + 32 k2] = DFT32 over n1 (registers) in lane k1, register k2.
-// The spectrum multiply runs on that layout, and the inverse retraces the
-// steps with conjugate twiddles, ending with x[n1 + 32 n2] in lane n1.  One
-// LDS exchange per direction (8 KB per column, XOR-swizzled so that both its
-// writes and its reads are free of bank conflicts) replaces the eight
-// LDS-staged radix stages of k_pass2.
+// Register-resident 1024-point transforms, one wave per transform (complex64,
+// M = 1024; the autotuner chooses them per pass and batch size).  Lane l
+// holds element l + 64 q in register q (q < 16), and the transform is the
+// Stockham sequence of radices 16, 16, 4:
+//   stage 0: a 16-point DFT over the registers, written to LDS as x[16 l + r];
+//   stage 1: reads x[l + 64 r], twiddles exp(s 2 pi i r (l % 16) / 256),
+//            16-point DFT, writes x[256 (l / 16) + l % 16 + 16 r];
+//   stage 2: butterflies j = l + 64 b (b < 4) read x[j + 256 r], twiddles
+//            exp(s 2 pi i r j / 1024), 4-point DFT; X[j + 256 r] stays in
+//            register b + 4 r, i.e. X[l + 64 q] in register q again.
+// The forward output is therefore the inverse's input layout, and the column
+// pass multiplies by the spectrum in registers between the two.  Two LDS
+// exchanges per transform, in the wave's own region of kWL elements with one
+// pad element after every 16: every access pattern above then falls in
+// distinct banks within each group of 16 lanes.  A wave orders its own LDS
+// accesses, so a transform needs no work-group barrier.
 // ---------------------------------------------------------------------------
-__device__ __constant__ float2 kW32[22] = {
-    {1.000000000f, 0.000000000f},   {0.980785280f, -0.195090322f},  {0.923879533f, -0.382683432f},
-    {0.831469612f, -0.555570233f},  {0.707106781f, -0.707106781f},  {0.555570233f, -0.831469612f},
-    {0.382683432f, -0.923879533f},  {0.195090322f, -0.980785280f},  {0.000000000f, -1.000000000f},
-    {-0.195090322f, -0.980785280f}, {-0.382683432f, -0.923879533f}, {-0.555570233f, -0.831469612f},
-    {-0.707106781f, -0.707106781f}, {-0.831469612f, -0.555570233f}, {-0.923879533f, -0.382683432f},
-    {-0.980785280f, -0.195090322f}, {-1.000000000f, 0.000000000f},  {-0.980785280f, 0.195090322f},
-    {-0.923879533f, 0.382683432f},  {-0.831469612f, 0.555570233f},  {-0.707106781f, 0.707106781f},
-    {-0.555570233f, 0.831469612f}};  // exp(-2 pi i q / 32), q = 0..21
+constexpr int kWL = 1024 + 1024 / 16;  // LDS elements per wave
+__device__ inline int wpad(int e) { return e + (e >> 4); }
 
-// v[k] <- sum_n v[n] exp(SIGN 2 pi i n k / 32): with n = 4 m + p, four 8-point
-// DFTs over m, the twiddles W32^(p k1), then eight 4-point DFTs over p give
-// X[k1 + 8 k2].
-template <int SIGN>
-__device__ inline void dft32(float2* v) {
-  float2 a[4][8];
+// Orders one wave's LDS writes before its later reads (no instruction: the
+// wave's LDS accesses complete in issue order; this keeps the compiler from
+// moving them across).
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// a * exp(SIGN 2 pi i Q / 16)
+template <int SIGN, int Q>
+__device__ inline float2 rot16(float2 a) {
+  constexpr float h = 0.70710678118654752440f;
+  constexpr float c1 = 0.92387953251128675613f, s1 = 0.38268343236508977173f;  // cos, sin of pi/8
+  if constexpr (Q == 4) {
+    return mul_si<float, SIGN>(a);
+  } else if constexpr (Q == 2) {
+    return make_float2(h * (a.x - SIGN * a.y), h * (a.y + SIGN * a.x));
+  } else if constexpr (Q == 6) {
+    return make_float2(-h * (a.x + SIGN * a.y), h * (SIGN * a.x - a.y));
+  } else if constexpr (Q == 1) {
+    return cmul(a, make_float2(c1, SIGN * s1));
+  } else if constexpr (Q == 3) {
+    return cmul(a, make_float2(s1, SIGN * c1));
+  } else {
+    static_assert(Q == 9, "twiddles of the 16-point DFT");
+    return cmul(a, make_float2(-c1, -SIGN * s1));
+  }
+}
+
+// v[k] <- sum_n v[n] exp(SIGN 2 pi i n k / 16) with n = 4 m + p: four 4-point
+// DFTs over m, the twiddles exp(SIGN 2 pi i p k1 / 16), then four 4-point DFTs
+// over p give X[k1 + 4 k2].  LO: v[8 ..] are zero and not read.
+template <int SIGN, bool LO>
+__device__ inline void dft16(float2* v) {
+  float2 a[4][4];
 #pragma unroll
   for (int p = 0; p < 4; ++p) {
 #pragma unroll
-    for (int m = 0; m < 8; ++m) a[p][m] = v[4 * m + p];
-    dft8<float, SIGN>(a[p]);
+    for (int m = 0; m < (LO ? 2 : 4); ++m) a[p][m] = v[4 * m + p];
+    if constexpr (LO)
+      dft4_lo<float, SIGN>(a[p]);
+    else
+      dft4<float, SIGN>(a[p]);
   }
+  a[1][1] = rot16<SIGN, 1>(a[1][1]);
+  a[1][2] = rot16<SIGN, 2>(a[1][2]);
+  a[1][3] = rot16<SIGN, 3>(a[1][3]);
+  a[2][1] = rot16<SIGN, 2>(a[2][1]);
+  a[2][2] = rot16<SIGN, 4>(a[2][2]);
+  a[2][3] = rot16<SIGN, 6>(a[2][3]);
+  a[3][1] = rot16<SIGN, 3>(a[3][1]);
+  a[3][2] = rot16<SIGN, 6>(a[3][2]);
+  a[3][3] = rot16<SIGN, 9>(a[3][3]);
 #pragma unroll
-  for (int p = 1; p < 4; ++p)
-#pragma unroll
-    for (int k1 = 1; k1 < 8; ++k1) {
-      float2 w = kW32[p * k1];
-      if (SIGN > 0) w.y = -w.y;
-      a[p][k1] = cmul(a[p][k1], w);
-    }
-#pragma unroll
-  for (int k1 = 0; k1 < 8; ++k1) {
+  for (int k1 = 0; k1 < 4; ++k1) {
     float2 b[4] = {a[0][k1], a[1][k1], a[2][k1], a[3][k1]};
     dft4<float, SIGN>(b);
 #pragma unroll
-    for (int k2 = 0; k2 < 4; ++k2) v[k1 + 8 * k2] = b[k2];
+    for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = b[k2];
   }
 }
 
-// Lane `lane` of a 32-lane column group: registers v[r] -> X[lane][r] in the
-// group's LDS region L, then reads back v[r] = X[r][lane] (the transpose).
-__device__ inline void xchg32(float2* __restrict__ L, float2* v, int lane) {
+// The twiddles of lane l for stages 1 and 2 (forward sign; the inverse uses
+// their conjugates), from the table tw[q] = exp(-2 pi i q / 1024).
+struct WaveTw {
+  float2 s1[15];  // r = 1..15: exp(-2 pi i r (l % 16) / 256)
+  float2 s2[12];  // b = 0..3, r = 1..3 at 3 b + r - 1: exp(-2 pi i r (l + 64 b) / 1024)
+  __device__ inline void load(const float2* __restrict__ tw, int l) {
+    const int k = l & 15;
 #pragma unroll
-  for (int r = 0; r < 32; ++r) L[r * 32 + (lane ^ r)] = v[r];
-  __syncthreads();
+    for (int r = 1; r < 16; ++r) s1[r - 1] = tw[4 * r * k];
 #pragma unroll
-  for (int r = 0; r < 32; ++r) v[r] = L[lane * 32 + (r ^ lane)];
-  __syncthreads();
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int r = 1; r < 4; ++r) s2[3 * b + r - 1] = tw[r * (l + 64 * b)];
+  }
+};
+
+template <int SIGN>
+__device__ inline float2 twid(float2 a, float2 w) {
+  if (SIGN > 0) w.y = -w.y;
+  return cmul(a, w);
 }
 
+// Stages 0 and 1 of a transform of v (in the register layout above); on
+// return stage 1's output is in X.  LO: v[8 ..] are zero and not read.
+template <int SIGN, bool LO>
+__device__ inline void wfft_head(float2* v, float2* __restrict__ X, const WaveTw& w, int l) {
+  dft16<SIGN, LO>(v);
+#pragma unroll
+  for (int r = 0; r < 16; ++r) X[wpad(16 * l + r)] = v[r];
+  wave_lds_sync();
+#pragma unroll
+  for (int r = 0; r < 16; ++r) v[r] = X[wpad(l + 64 * r)];
+#pragma unroll
+  for (int r = 1; r < 16; ++r) v[r] = twid<SIGN>(v[r], w.s1[r - 1]);
+  dft16<SIGN, false>(v);
+  const int base = 256 * (l >> 4) + (l & 15);
+#pragma unroll
+  for (int r = 0; r < 16; ++r) X[wpad(base + 16 * r)] = v[r];
+  wave_lds_sync();
+}
+
+// Forward transform; v[q] = x[l + 64 q] with x[n] = 0 for n >= 512 (v[8 ..]
+// not read) becomes v[q] = X[l + 64 q] for all 16 q.
+__device__ inline void wfft_fwd_lo(float2* v, float2* __restrict__ X, const WaveTw& w, int l) {
+  wfft_head<-1, true>(v, X, w, l);
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    float2 u[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) u[r] = X[wpad(l + 64 * b + 256 * r)];
+#pragma unroll
+    for (int r = 1; r < 4; ++r) u[r] = twid<-1>(u[r], w.s2[3 * b + r - 1]);
+    dft4<float, -1>(u);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) v[b + 4 * r] = u[r];
+  }
+  wave_lds_sync();
+}
+
+// Inverse (unnormalised) transform of v[q] = Y[l + 64 q]; only the outputs
+// below 512 are formed: v[q] = y[l + 64 q] for q < 8.
+__device__ inline void wfft_inv_hi(float2* v, float2* __restrict__ X, const WaveTw& w, int l) {
+  wfft_head<1, false>(v, X, w, l);
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    float2 u[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) u[r] = X[wpad(l + 64 * b + 256 * r)];
+#pragma unroll
+    for (int r = 1; r < 4; ++r) u[r] = twid<1>(u[r], w.s2[3 * b + r - 1]);
+    dft4<float, 1>(u);
+    v[b] = u[0];
+    v[b + 4] = u[1];
+  }
+  wave_lds_sync();
+}
+
+// Pass 2, M = 1024: a work-group of four waves takes half a column panel
+// (N rows x 4 columns, kPW / 2 contiguous values per row), staged through LDS
+// with a row stride of 5 values for coalesced global accesses; wave g then
+// transforms column g of it forward, multiplies by the spectrum column and
+// transforms back, keeping rows < N.
 template <typename G>
 __global__ void __launch_bounds__(256)
 k_pass2w(float2* __restrict__ Tm, const G* __restrict__ gq, int N, const float2* __restrict__ tw,
          const int* __restrict__ done, BatchStrides bs) {
   constexpr int M = 1024;
   constexpr int H = M / 2 + 1;
-  __shared__ float2 lds[8][1024];
+  constexpr int kC = kPW / 2;  // columns per work-group
+  constexpr int kRS = kC + 1;  // staging row stride
+  static_assert(512 * kRS <= 4 * kWL, "the staged half panel fits the work-group's LDS");
+  __shared__ float2 lds[4 * kWL];
   Tm += (long)blockIdx.y * bs.T;
   done = boff(done, (long)blockIdx.y * bs.st);
-  const int panel = xcd_tile(blockIdx.x, gridDim.x);
-  const int grp = threadIdx.x >> 5;  // column of the panel
-  const int lane = threadIdx.x & 31;
-  const int col = panel * kPW + grp;
-  float2* L = lds[grp];
-  // The panel (N rows x 8 columns, contiguous in tpos() layout) is staged in
-  // LDS with a row stride of 9 values (coalesced global loads; the column
-  // reads that follow fall in distinct banks), then each lane takes rows
-  // n = lane + 32 r of its column (zero from N on).
-  float2* S = &lds[0][0];
-  constexpr int kRS = kPW + 1;
-  const float2* src = Tm + ((long)panel * N) * kPW;
-  for (int e = threadIdx.x; e < N * kPW; e += 256) S[(e / kPW) * kRS + e % kPW] = src[e];
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int panel = tile >> 1;
+  const int grp = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int col = tile * kC + grp;
+  WaveTw w;
+  w.load(tw, lane);
+  float2* S = lds;
+  float2* g0 = Tm + (long)panel * N * kPW + (tile & 1) * kC;
+  for (int e = threadIdx.x; e < N * kC; e += 256) S[(e >> 2) * kRS + (e & 3)] = g0[(long)(e >> 2) * kPW + (e & 3)];
   __syncthreads();
   if (done && *done) return;  // uniform over the work-group
-  float2 v[32];
+  float2 v[16];
 #pragma unroll
-  for (int r = 0; r < 32; ++r) {
-    const int n = lane + 32 * r;
-    v[r] = n < N ? S[n * kRS + grp] : make_float2(0.f, 0.f);
+  for (int q = 0; q < 8; ++q) {
+    const int n = lane + 64 * q;
+    v[q] = n < N ? S[n * kRS + grp] : make_float2(0.f, 0.f);
   }
-  __syncthreads();  // the staging area is reused by the exchanges
-  // forward: A over registers, twiddle, exchange, B
-  dft32<-1>(v);
-#pragma unroll
-  for (int r = 1; r < 32; ++r) v[r] = cmul(v[r], tw[lane * r]);
-  xchg32(L, v, lane);
-  dft32<-1>(v);
-  // lane k1 holds X[k1 + 32 k2] in v[k2]: multiply by the spectrum column
+  __syncthreads();  // the staging area is reused by the transforms
+  float2* X = lds + grp * kWL;
+  wfft_fwd_lo(v, X, w, lane);
   const int cq = col <= M - col ? col : M - col;
   const G* grow = gq + (long)cq * H;
 #pragma unroll
-  for (int r = 0; r < 32; ++r) {
-    const int k = lane + 32 * r;
-    const int fr = k <= M - k ? k : M - k;
-    const G g = grow[fr];
-    v[r] = cmul(v[r], make_float2((float)g.x, (float)g.y));
+  for (int q = 0; q < 16; ++q) {
+    const int k = lane + 64 * q;
+    const G g = grow[k <= M - k ? k : M - k];
+    v[q] = cmul(v[q], make_float2((float)g.x, (float)g.y));
   }
-  // inverse: A' over registers (k2), conjugate twiddle, exchange, B' (k1)
-  dft32<1>(v);
+  wfft_inv_hi(v, X, w, lane);
+  __syncthreads();
 #pragma unroll
-  for (int r = 1; r < 32; ++r) {
-    float2 w = tw[lane * r];
-    w.y = -w.y;
-    v[r] = cmul(v[r], w);
-  }
-  xchg32(L, v, lane);
-  dft32<1>(v);
-  // back through the same staging layout for coalesced stores
-#pragma unroll
-  for (int r = 0; r < 32; ++r) {
-    const int n = lane + 32 * r;
-    if (n < N) S[n * kRS + grp] = v[r];
+  for (int q = 0; q < 8; ++q) {
+    const int n = lane + 64 * q;
+    if (n < N) S[n * kRS + grp] = v[q];
   }
   __syncthreads();
-  float2* dst = Tm + ((long)panel * N) * kPW;
-  for (int e = threadIdx.x; e < N * kPW; e += 256) dst[e] = S[(e / kPW) * kRS + e % kPW];
+  for (int e = threadIdx.x; e < N * kC; e += 256) g0[(long)(e >> 2) * kPW + (e & 3)] = S[(e >> 2) * kRS + (e & 3)];
 }
 
-// Passes 1 and 3 for M = 1024 in registers, with the same 32 x 32 scheme:
-// 32 lanes per row, eight rows per work-group.  Pass 1 loads row i of
-// chi .* x (lane n1, register n2: element n1 + 32 n2), transforms it forward
-// and writes T[i][k1 + 32 k2] from lane k1, register k2.  Pass 3 reads that
-// layout, transforms it back, and lane n1 combines element n1 + 32 n2 of the
-// row with x.  Only the plain operator modes (y = x - G(chi x), x - chi G(x),
-// G x) without input scaling take these kernels.
+// Passes 1 and 3 for M = 1024 with the same transforms: one wave per row,
+// four rows per work-group.  Pass 1 loads row i of chi .* x (lane l, register
+// q: element l + 64 q), transforms it forward and writes T[i][l + 64 q].
+// Pass 3 reads that layout, transforms it back, and lane l combines element
+// l + 64 q of the row with x.  Only the plain operator modes (y = x - G(chi x),
+// x - chi G(x), G x) without input scaling take these kernels.
 __global__ void __launch_bounds__(256)
 k_pass1w(const float2* __restrict__ x, const float* __restrict__ chi, int mul, float2* __restrict__ Tm, int N,
          const float2* __restrict__ tw, const int* __restrict__ done, BatchStrides bs) {
-  __shared__ float2 lds[8][1024];
+  __shared__ float2 lds[4 * kWL];
   {
     const long kb = blockIdx.y;
     x += kb * bs.x;
@@ -319,16 +402,18 @@ k_pass1w(const float2* __restrict__ x, const float* __restrict__ chi, int mul, f
     Tm += kb * bs.T;
     done = boff(done, kb * bs.st);
   }
-  const int grp = threadIdx.x >> 5, lane = threadIdx.x & 31;
-  const int row = blockIdx.x * 8 + grp;
-  const bool live = row < N;
+  const int grp = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int row = blockIdx.x * 4 + grp;
+  if (row >= N) return;  // no work-group barriers below
   const long rb = (long)row * N;
-  float2 v[32];
+  WaveTw w;
+  w.load(tw, lane);
+  float2 v[16];
 #pragma unroll
-  for (int r = 0; r < 32; ++r) {
-    const int n = lane + 32 * r;
+  for (int q = 0; q < 8; ++q) {
+    const int n = lane + 64 * q;
     float2 e = make_float2(0.f, 0.f);
-    if (live && n < N) {
+    if (n < N) {
       e = x[rb + n];
       if (mul) {
         const float c = chi[rb + n];
@@ -336,25 +421,19 @@ k_pass1w(const float2* __restrict__ x, const float* __restrict__ chi, int mul, f
         e.y *= c;
       }
     }
-    v[r] = e;
+    v[q] = e;
   }
-  if (done && *done) return;  // uniform over the work-group
-  dft32<-1>(v);
+  if (done && *done) return;
+  wfft_fwd_lo(v, lds + grp * kWL, w, lane);
 #pragma unroll
-  for (int r = 1; r < 32; ++r) v[r] = cmul(v[r], tw[lane * r]);
-  xchg32(lds[grp], v, lane);
-  dft32<-1>(v);
-  if (live) {
-#pragma unroll
-    for (int r = 0; r < 32; ++r) Tm[tpos(row, lane + 32 * r, N)] = v[r];
-  }
+  for (int q = 0; q < 16; ++q) Tm[tpos(row, lane + 64 * q, N)] = v[q];
 }
 
 __global__ void __launch_bounds__(256)
 k_pass3w(const float2* __restrict__ Tm, const float2* __restrict__ x, const float* __restrict__ chi,
          float2* __restrict__ y, int mode, int N, float inv, const float2* __restrict__ tw,
          const int* __restrict__ done, BatchStrides bs) {
-  __shared__ float2 lds[8][1024];
+  __shared__ float2 lds[4 * kWL];
   {
     const long kb = blockIdx.y;
     Tm += kb * bs.T;
@@ -363,29 +442,22 @@ k_pass3w(const float2* __restrict__ Tm, const float2* __restrict__ x, const floa
     y += kb * bs.y;
     done = boff(done, kb * bs.st);
   }
-  const int grp = threadIdx.x >> 5, lane = threadIdx.x & 31;
-  const int row = blockIdx.x * 8 + grp;
-  const bool live = row < N;
-  float2 v[32];
+  const int grp = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int row = blockIdx.x * 4 + grp;
+  if (row >= N) return;  // no work-group barriers below
+  WaveTw w;
+  w.load(tw, lane);
+  float2 v[16];
 #pragma unroll
-  for (int r = 0; r < 32; ++r) v[r] = live ? Tm[tpos(row, lane + 32 * r, N)] : make_float2(0.f, 0.f);
+  for (int q = 0; q < 16; ++q) v[q] = Tm[tpos(row, lane + 64 * q, N)];
   if (done && *done) return;
-  dft32<1>(v);
-#pragma unroll
-  for (int r = 1; r < 32; ++r) {
-    float2 w = tw[lane * r];
-    w.y = -w.y;
-    v[r] = cmul(v[r], w);
-  }
-  xchg32(lds[grp], v, lane);
-  dft32<1>(v);
-  if (!live) return;
+  wfft_inv_hi(v, lds + grp * kWL, w, lane);
   const long rb = (long)row * N;
 #pragma unroll
-  for (int r = 0; r < 32; ++r) {
-    const int n = lane + 32 * r;
+  for (int q = 0; q < 8; ++q) {
+    const int n = lane + 64 * q;
     if (n < N) {
-      const float2 g = cscale(v[r], inv);
+      const float2 g = cscale(v[q], inv);
       float2 o;
       if (mode == kModeForward) o = csub(x[rb + n], g);
       else if (mode == kModeAdjoint) o = csub(x[rb + n], cscale(g, chi[rb + n]));
@@ -602,7 +674,7 @@ int launch_pass1(const Plan& P, hipStream_t st, const cplx<T>* x, cplx<T>* xw, c
   if constexpr (sizeof(T) == 4) {
     if (M == 1024 && P.wide1_batch > 0 && B >= P.wide1_batch && !xw && !norm_ptr &&
         (mode == kModeForward || mode == kModeAdjoint || mode == kModeGreen)) {
-      hipLaunchKernelGGL(k_pass1w, dim3((N + 7) / 8, B), dim3(256), 0, st, (const float2*)x, (const float*)chi,
+      hipLaunchKernelGGL(k_pass1w, dim3((N + 3) / 4, B), dim3(256), 0, st, (const float2*)x, (const float*)chi,
                          (int)(mode == kModeForward), (float2*)Tm, N, (const float2*)tw, done, bs);
       return hipGetLastError() == hipSuccess ? CGM_SUCCESS : CGM_ERROR_DEVICE;
     }
@@ -631,7 +703,7 @@ int launch_pass2(const Plan& P, hipStream_t st, cplx<T>* Tm, const G* gq, const 
   const int M = P.M, N = P.N;
   if constexpr (sizeof(T) == 4) {
     if (M == 1024 && P.wide2_batch > 0 && B >= P.wide2_batch) {
-      hipLaunchKernelGGL((k_pass2w<G>), dim3(M / kPW, B), dim3(256), 0, st, (float2*)Tm, gq, N, (const float2*)tw,
+      hipLaunchKernelGGL((k_pass2w<G>), dim3(M / (kPW / 2), B), dim3(256), 0, st, (float2*)Tm, gq, N, (const float2*)tw,
                          done, bs);
       return hipGetLastError() == hipSuccess ? CGM_SUCCESS : CGM_ERROR_DEVICE;
     }
@@ -662,7 +734,7 @@ int launch_pass3(const Plan& P, hipStream_t st, const cplx<T>* Tm, const cplx<T>
   if constexpr (sizeof(T) == 4) {
     if (M == 1024 && P.wide3_batch > 0 && B >= P.wide3_batch && !b &&
         (mode == kModeForward || mode == kModeAdjoint || mode == kModeGreen)) {
-      hipLaunchKernelGGL(k_pass3w, dim3((N + 7) / 8, B), dim3(256), 0, st, (const float2*)Tm, (const float2*)x,
+      hipLaunchKernelGGL(k_pass3w, dim3((N + 3) / 4, B), dim3(256), 0, st, (const float2*)Tm, (const float2*)x,
                          (const float*)chi, (float2*)y, mode, N, (float)(1.0 / ((double)M * (double)M)),
                          (const float2*)tw, done, bs);
       return hipGetLastError() == hipSuccess ? CGM_SUCCESS : CGM_ERROR_DEVICE;

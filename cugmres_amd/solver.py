@@ -66,13 +66,14 @@ class Context:
 
     def radix_plan(self):
         """The autotuner's radix plan at M = 1024: {'wide_from_batch': [pass 1, 2, 3],
-        'radix32_us' / 'radix8_us': [pass][batch 1, 4, 16]}."""
+        'wave_us' / 'lds_us': [pass][batch 1, 4, 16] for the one-wave register kernel
+        and the LDS-staged radix-8 kernel}."""
         wb = (_c.c_int * 3)()
         w = (_c.c_float * 9)()
         l_ = (_c.c_float * 9)()
         check(lib().cgm_get_radix_plan(self._ptr, wb, w, l_), "cgm_get_radix_plan")
-        return {"wide_from_batch": list(wb), "radix32_us": [[round(w[3 * q + k], 2) for k in range(3)] for q in range(3)],
-                "radix8_us": [[round(l_[3 * q + k], 2) for k in range(3)] for q in range(3)]}
+        return {"wide_from_batch": list(wb), "wave_us": [[round(w[3 * q + k], 2) for k in range(3)] for q in range(3)],
+                "lds_us": [[round(l_[3 * q + k], 2) for k in range(3)] for q in range(3)]}
 
     def set_radix_plan(self, wide_from_batch):
         """wide_from_batch: one int for all passes or a list of three."""

@@ -96,13 +96,14 @@ int cgm_autotune(cgm_ctx_t ctx);
 int cgm_get_tiles(cgm_ctx_t ctx, int* tiles, float* us);
 /* The autotuner's radix plan at M = 1024 (the N = 512 embedding): for each
  * pass q = 0, 1, 2 (passes 1-3), launches of at least wide_from_batch[q]
- * vectors use the register-resident radix-32 x 32 kernel instead of the
+ * vectors use the register-resident one-wave kernel (one 64-lane wave per
+ * 1024-point transform, radices 16, 16, 4, two LDS exchanges) instead of the
  * LDS-staged radix-8 stages (0: never; other M always use the radix-8
  * stages).  Passes 1 and 3 take it for the plain modes only (no residual,
  * no input scaling).  wide_us / lds_us (optional, 9 each, [pass][batch]) =
  * the two kernels' times at batches of 1, 4 and 16 vectors in microseconds. */
 int cgm_get_radix_plan(cgm_ctx_t ctx, int* wide_from_batch, float* wide_us, float* lds_us);
-/* Overrides that choice per pass (0: the radix-8 kernel; 1: the radix-32 x 32
+/* Overrides that choice per pass (0: the radix-8 kernel; 1: the one-wave
  * kernel for every launch it supports).  Ignored at other M. */
 int cgm_set_radix_plan(cgm_ctx_t ctx, const int* wide_from_batch);
 /* Arnoldi kernels of the solvers: 0 (default) = fused steps with lagged CGS2

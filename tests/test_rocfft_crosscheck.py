@@ -62,7 +62,8 @@ def test_matvec_matches_rocfft(N):
 @pytest.mark.parametrize("wide", [0, 1])
 def test_batched_matvec_matches_rocfft_with_both_radix_plans(wide):
     """N = 512 (M = 1024): the LDS-staged radix-8 passes and the
-    register-resident radix-32 x 32 passes (forced by cgm_set_radix_plan), in
+    register-resident one-wave passes (radices 16, 16, 4; forced by
+    cgm_set_radix_plan), in
     batches of 1, 4 and 16 vectors with their own maps, all three modes,
     against the same product formed by torch.fft in complex128.  Also prints
     the autotuner's own choice."""
