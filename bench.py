@@ -94,6 +94,11 @@ def parse(argv=None):
     p.add_argument("--sweep-reps", type=int, default=20)
     p.add_argument("--lr-j", type=int, default=4096, help="lr-step mode: basis size of the profiled kernels")
     p.add_argument("--c2-seconds", type=float, default=600.0, help="c2 mode: no new outer step after this time")
+    p.add_argument("--c3-method", choices=["block", "lockstep"], default="block",
+                   help="c3 mode: block GMRES over each map's right-hand sides (one shared block Krylov space), "
+                        "or flexible GMRES of every right-hand side in lockstep")
+    p.add_argument("--block-restart", type=int, default=256, help="c3 block method: block steps per cycle")
+    p.add_argument("--block-cycles", type=int, default=16, help="c3 block method: cap on the cycles")
     p.add_argument("--tiles-synthetic", action="store_true",
                    help="tiles-step mode: time the application on synthetic inverses (no factorisation)")
     return p.parse_args(argv)
@@ -446,7 +451,12 @@ def fft_reference(a, B=16, reps=20):
 
 def c3_run(a, rhs=16, mem_budget=150e9):
     """Batched workload: per instance one map and `rhs` plane waves at angles
-    2 pi k / rhs, solved in lockstep by flexible GMRES with the tile inverses
+    2 pi k / rhs.  --c3-method block (default): block GMRES over the map's
+    right-hand sides with its tile inverses (cugmres_amd.block.solve_map: one
+    shared block Krylov space, the projection V^H W as batched complex GEMMs,
+    cycles of --block-restart block steps that restart from the FP64 true
+    residual with the solution kept in complex128).  --c3-method lockstep: the
+    right-hand sides are solved in lockstep by flexible GMRES with the tile inverses
     of that map (factored once, each tile's inverse streamed once per four
     right-hand sides) on the long-recurrence path with cycles of --lr-restart
     steps (two at most).  The right-hand sides run in lockstep groups as large
@@ -460,7 +470,7 @@ def c3_run(a, rhs=16, mem_budget=150e9):
     the tolerance (the FP64 true residual for the refined ones)."""
     import numpy as np
     import torch
-    from cugmres_amd import problems, solver, tiles
+    from cugmres_amd import block, problems, solver, tiles
     ctxs = {}
     per_n = {}
     t_all = 0.0
@@ -491,7 +501,16 @@ def c3_run(a, rhs=16, mem_budget=150e9):
         tl = tiles.Tiles(ctx, T, 1).factor(gb, chi)
         infos = []
         refined = 0
-        for g0 in range(0, rhs, grp):
+        bstat = None
+        if a.c3_method == "block":
+            chi64 = ctx.to_device((eps - 1.0).reshape(-1), "f64")
+            X, binfo = block.solve_map(ctx, g, chi, b, tiles=tl, restart=a.block_restart, tol=a.tol,
+                                       max_cycles=a.block_cycles, gamma64=ctxs[N][3], chi64=chi64)
+            infos = [{"converged": bool(cv), "rel_residual": float(rr), "iterations": binfo["block_steps"]}
+                     for cv, rr in zip(binfo["converged"], binfo["rel_residual"])]
+            bstat = {k: binfo[k] for k in ("block_steps", "cycles", "breakdowns")}
+            grp = rhs
+        for g0 in range(0, rhs if a.c3_method == "lockstep" else 0, grp):
             bg = b[g0:g0 + grp].contiguous()
             xg, ig = ctx.fgmres_batched(g, chi, bg, precond=tl, restart=m, tol=a.tol, max_cycles=2)
             # a system left above the tolerance at the complex64 floor of its solution (about 6e-8
@@ -520,6 +539,8 @@ def c3_run(a, rhs=16, mem_budget=150e9):
                                  "lockstep_group": grp, "tile": T, "refined_fp64": 0, "iterations": [],
                                  "residuals": []})
         e["refined_fp64"] += refined
+        if bstat is not None:
+            e.setdefault("block", []).append(bstat)
         e["instances"] += 1
         e["rhs"] += rhs
         e["converged"] += conv
@@ -536,18 +557,23 @@ def c3_run(a, rhs=16, mem_budget=150e9):
         t_all += dt
         conv_all += conv
         total += rhs
-        note(f"c3 N={N} block={blk_cells} converged {conv}/{rhs} in {dt:.2f} s ({refined} refined in FP64; "
-             f"groups of {grp}; "
-             f"iterations {_stats([i['iterations'] for i in infos])})")
+        note(f"c3 N={N} block={blk_cells} converged {conv}/{rhs} in {dt:.2f} s ({a.c3_method}; "
+             + (f"{bstat['block_steps']} block steps in {bstat['cycles']} cycles)" if bstat else
+                f"{refined} refined in FP64; groups of {grp}; iterations "
+                f"{_stats([i['iterations'] for i in infos])})"))
     for e in per_n.values():
         e["iterations"] = _stats(e["iterations"])
         e["residuals"] = _stats(e["residuals"])
     for c, _, _, _ in ctxs.values():
         c.close()
     return {"workload": f"{a.instances} instances x {rhs} plane waves, N in {list(C3_SIZES)}, blocks 4-8, "
-                        f"eps in {{1,12.1}}, tol {a.tol:g}", "method": "FGMRES with tile inverses, lockstep, "
-                        "one factorisation per map; FP64 refinement from where it stopped for a system left at "
-                        "the complex64 floor", "rhs_solves_per_s": conv_all / t_all if t_all else 0.0,
+                        f"eps in {{1,12.1}}, tol {a.tol:g}",
+            "method": ("block GMRES over each map's 16 right-hand sides with its tile inverses (one "
+                       "factorisation per map), projection V^H W as batched complex GEMMs, cycles restarting from "
+                       "the FP64 true residual" if a.c3_method == "block" else
+                       "FGMRES with tile inverses, lockstep, one factorisation per map; FP64 refinement from "
+                       "where it stopped for a system left at the complex64 floor"),
+            "rhs_solves_per_s": conv_all / t_all if t_all else 0.0,
             "converged": conv_all, "of": total, "seconds": t_all, "per_N": per_n}
 
 
