@@ -589,9 +589,9 @@ __device__ inline void bfly_stage(float2* c, int lane, int o) {
 }
 
 // W_t is complex symmetric (W_t = (D_S^{-1} - G_SS)^{-1}, G symmetric), so
-// only its lower triangle is stored, in kBlkW x kBlkW blocks.  Work-group e
-// of system blockIdx.y takes block (i, j) of the system's map block list and
-// writes two partial products of kBlkW values (scaled W16): B x_j (for block
+// only its lower triangle is stored, in kBlkW x kBlkW blocks.  A work-group
+// takes block e = (i, j) of its system s's map block list and writes two
+// partial products of kBlkW values (scaled W16): B x_j (for block
 // row i) and, for j < i, B^T x_i (for block row j), x_i, x_j the tile's
 // input at the cells of S in block rows i and j (io.v scaled by
 // 1 / *(io.nrm + s sNrm bytes) when io.nrm is given).  Lane l of every wave
@@ -599,7 +599,12 @@ __device__ inline void bfly_stage(float2* c, int lane, int o) {
 // waves take 32 columns each; the row sums meet in LDS, and the 32 column
 // sums of each wave are reduced across its lanes by a transposing butterfly
 // (lane l ends with column l / 2).  Systems with *(done + s sd bytes) set
-// are skipped.
+// are skipped.  The grid is (8 ceil(blocks / 8), systems): work-group L in
+// dispatch order (on XCD L % 8) takes system (L / 8) % systems and block
+// 8 (L / (8 systems)) + L % 8, so the systems of one block run back to back
+// on one XCD and, on a shared map, its L2 serves the block's later reads
+// (W is streamed from HBM about once, not once per system).  One system
+// keeps block e = L.
 template <int T>
 __global__ void __launch_bounds__(256)
 k_tile_ws(const __half2* __restrict__ W16, const long* __restrict__ woff, const int* __restrict__ ncol,
@@ -610,7 +615,10 @@ k_tile_ws(const __half2* __restrict__ W16, const long* __restrict__ woff, const 
   __shared__ float2 xj[B], xi[B];
   __shared__ float4 red[3][64];
   const int nt = N / T;
-  const int s = blockIdx.y, map = maps1 ? 0 : s, e = blockIdx.x;
+  const int nsys = gridDim.y;
+  const long L = (long)blockIdx.y * gridDim.x + blockIdx.x;
+  const long q = L >> 3;
+  const int s = (int)(q % nsys), map = maps1 ? 0 : s, e = (int)((q / nsys) * 8 + (L & 7));
   if (e >= bcount[map]) return;
   if (done && *boff(done, (long)s * sd)) return;
   const int pk = btab[bstart[map] + e];
@@ -790,10 +798,10 @@ int launch_apply_red(const cgm_tiles* tl, hipStream_t st, const FnoIO& io, const
   if (tl->bcount_max > 0) {
     if (ev) CGM_HIP(hipEventRecord(ev[0], st));
     if (tl->T == 64)
-      hipLaunchKernelGGL((k_tile_ws<64>), dim3(tl->bcount_max, nsys), dim3(256), 0, st, tl->W16, tl->woff, tl->ncol,
+      hipLaunchKernelGGL((k_tile_ws<64>), dim3((tl->bcount_max + 7) / 8 * 8, nsys), dim3(256), 0, st, tl->W16, tl->woff, tl->ncol,
                          tl->cols, tl->btab, tl->bstart, tl->bcount, maps1, io, tl->Pw, spw, done, sd, tl->N);
     else
-      hipLaunchKernelGGL((k_tile_ws<128>), dim3(tl->bcount_max, nsys), dim3(256), 0, st, tl->W16, tl->woff, tl->ncol,
+      hipLaunchKernelGGL((k_tile_ws<128>), dim3((tl->bcount_max + 7) / 8 * 8, nsys), dim3(256), 0, st, tl->W16, tl->woff, tl->ncol,
                          tl->cols, tl->btab, tl->bstart, tl->bcount, maps1, io, tl->Pw, spw, done, sd, tl->N);
     CGM_HIP(hipGetLastError());
     if (ev) CGM_HIP(hipEventRecord(ev[1], st));
