@@ -166,6 +166,7 @@ def block_gmres(torch, apply_A, apply_P, B, X=None, restart=64, tol=1e-6, max_cy
     bnorm = torch.linalg.vector_norm(B.to(torch.complex128), dim=1).cpu().numpy()
     bnorm = np.where(bnorm > 0, bnorm, 1.0)
     Vb = torch.empty(((restart + 1) * s, n), dtype=dt, device=dev)
+    on_device = dev.type == "cuda"
     steps = cycles = breaks = 0
     R = resid(X)
     rel = torch.linalg.vector_norm(R.to(torch.complex128), dim=1).cpu().numpy() / bnorm
@@ -191,10 +192,12 @@ def block_gmres(torch, apply_A, apply_P, B, X=None, restart=64, tol=1e-6, max_cy
         bn = bnorm[active]
         start = float(rel[active].max())
         hist = []
+        W_next = None
         for j in range(restart):
             k = (j + 1) * sa
             Vk = Vb[:k]
-            W = apply_A(apply_P(Vb[j * sa:k]))
+            W = W_next if W_next is not None else apply_A(apply_P(Vb[j * sa:k]))
+            W_next = None
             H1 = project(torch, Vk, W)                              # (k, sa) = V^H W
             W = update(torch, W, Vk, H1)
             H2 = project(torch, Vk, W)
@@ -209,7 +212,21 @@ def block_gmres(torch, apply_A, apply_P, B, X=None, restart=64, tol=1e-6, max_cy
                 qr.add(Hcol)
                 break
             Vb[k:k + sa] = Vn.to(dt)
-            Hcol = torch.cat([Hc, Sj]).cpu().numpy()
+            Hd = torch.cat([Hc, Sj])
+            if on_device:
+                # the Hessenberg column leaves through a pinned copy; the next
+                # step's operator and tile inverses are queued behind it, so
+                # the device computes them while the host updates the QR
+                Hh = torch.empty(Hd.shape, dtype=Hd.dtype, pin_memory=True)
+                Hh.copy_(Hd, non_blocking=True)
+                ev = torch.cuda.Event()
+                ev.record()
+                if j + 1 < restart:
+                    W_next = apply_A(apply_P(Vb[k:k + sa]))
+                ev.synchronize()
+                Hcol = Hh.numpy()
+            else:
+                Hcol = Hd.cpu().numpy()
             est = qr.add(Hcol) / bn
             if on_step is not None:
                 on_step(steps, est)
