@@ -343,7 +343,8 @@ def run_bench(a, ctx):
                 "tile_apply": tp["us"], "matvec_x2": 2 * mv["us"], "fno_forward": fp["us"]}
     roof = live_roofline(live, N, per_step, jmean, a.tile)
     matvec = {"us": mv["us"], "bytes": mv["bytes"], "GBps": mv["GBps"], "frac": mv["GBps"] / HBM_PEAK_GBS,
-              "per_unit": "100 N^2 B per matvec (quarter spectrum)"}
+              "per_unit": "100 N^2 B per matvec (quarter spectrum)",
+              "batched16": batched_matvec_rate(ctx, g, dchi0)}
 
     note("profiles done; reference problem")
     refp = reference_problem(ctx, a.restart) if N == 512 else None
@@ -396,6 +397,28 @@ def c3_instances(count, seed=2024):
         blk = 4 + min(4, int(rng.uniform() * 5))
         out.append((N, blk, seed * 1000 + i))
     return out
+
+
+def batched_matvec_rate(ctx, g, chi, B=16, reps=20):
+    """cgm_matvec_batched on B vectors of one map with the autotuner's radix
+    plan, timed with HIP events on the context's stream: microseconds per
+    launch and the fraction of HBM peak on 88 N^2 B per vector plus the map
+    and the quarter spectrum once (the bytes fft_reference counts)."""
+    import torch
+    N, M = ctx.N, ctx.M
+    x = torch.randn(B, N * N, dtype=torch.complex64, device=ctx.device)
+    y = torch.empty_like(x)
+    for _ in range(3):
+        ctx.matvec_batched(g, chi, x, 0, y)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        ctx.matvec_batched(g, chi, x, 0, y)
+    e1.record()
+    torch.cuda.synchronize()
+    us = e0.elapsed_time(e1) / reps * 1e3
+    nbytes = B * 88 * N * N + 4 * N * N + 8 * (M // 2 + 1) ** 2
+    return {"us": round(us, 1), "frac": round(nbytes / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 3)}
 
 
 def fft_reference(a, B=16, reps=20):

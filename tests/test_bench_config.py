@@ -49,7 +49,10 @@ def test_line_is_compact_at_320_systems(bench):
            "all_cores": {"cores": 16, "value": 0.05}, "cpu_model": "z" * 60, "nproc": 16,
            "C0_N64_c128_m200": {"seeds": 10, "s_per_solve": [1, 2, 3], "rel_residual": [1, 2, 3]}}
     out = bench.make_line(a, 512, 1024, learned, tiles, plain, 800.0, {"N": 256, "samples": 8000}, roof,
-                          {"us": 20.0, "GBps": 1300.0}, cpu, {"problem": "reference", "ms_per_solve": 3.0})
+                          {"us": 19.65290028601885, "bytes": 26222600, "GBps": 1334.2865235343845,
+                           "frac": 0.16678581544179807, "per_unit": "100 N^2 B per matvec (quarter spectrum)",
+                           "batched16": {"us": 119.2, "frac": 0.391}},
+                          cpu, {"problem": "reference", "ms_per_solve": 3.0})
     line = json.dumps(out, separators=(",", ":"))
     assert len(line) < 4096, len(line)
     back = json.loads(line)
