@@ -480,8 +480,9 @@ def c3_run(a, rhs=16, mem_budget=150e9):
     cycles of --block-restart block steps that restart from the FP64 true
     residual with the solution kept in complex128).  --c3-method lockstep: the
     right-hand sides are solved in lockstep by flexible GMRES with the tile inverses
-    of that map (factored once, each tile's inverse streamed once per four
-    right-hand sides) on the long-recurrence path with cycles of --lr-restart
+    of that map (factored once; the dense form streams each inverse once per
+    four right-hand sides, the reduced form runs the systems of one W block
+    back to back on one XCD) on the long-recurrence path with cycles of --lr-restart
     steps (two at most).  The right-hand sides run in lockstep groups as large
     as mem_budget bytes of Krylov bases allow (all 16 up to N = 256, fewer
     above).  A right-hand side left above the tolerance (on the hardest maps
