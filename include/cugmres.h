@@ -365,7 +365,10 @@ int cgm_tiles_memory(cgm_tiles_t tiles, size_t* bytes);
 int cgm_tiles_factor(cgm_tiles_t tiles, const cgm_complex128* dev_block, const float* dev_chi, long chi_stride,
                      int maps);
 /* y[b] = P x[b] for b < batch ([batch][N*N]; map b, or the one map).
- * Asynchronous on the context stream. */
+ * Asynchronous on the context stream.  With one map and several systems the
+ * reduced form (T >= 64) runs the systems of each W block back to back on one
+ * XCD, so W is read from HBM about once per call; the dense form (T <= 32)
+ * applies each inverse to four systems per pass. */
 int cgm_tiles_apply(cgm_tiles_t tiles, const cgm_complex64* dev_x, cgm_complex64* dev_y, int batch);
 /* Algorithmic bytes of one cgm_tiles_apply over `batch` systems: the inverse
  * entries it streams (every entry of every tile in FP32 for T <= 32, once per
