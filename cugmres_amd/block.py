@@ -271,8 +271,7 @@ def solve_map(ctx, gamma, chi, B, tiles=None, X=None, restart=64, tol=1e-6, max_
     residual = None
     if gamma64 is not None and chi64 is not None:
         B64 = B.to(torch.complex128)
-        if X is None:
-            X = torch.zeros_like(B64)
+        X = torch.zeros_like(B64) if X is None else X.to(torch.complex128)
 
         def residual(Xc):
             X64 = Xc.to(torch.complex128)
