@@ -603,8 +603,8 @@ __device__ inline void bfly_stage(float2* c, int lane, int o) {
 // dispatch order (on XCD L % 8) takes system (L / 8) % systems and block
 // 8 (L / (8 systems)) + L % 8, so the systems of one block run back to back
 // on one XCD and, on a shared map, its L2 serves the block's later reads
-// (W is streamed from HBM about once, not once per system).  One system
-// keeps block e = L.
+// (16 systems at N = 512 fetch 1.5 times W from HBM, not 16 times).  One
+// system keeps block e = L.
 template <int T>
 __global__ void __launch_bounds__(256)
 k_tile_ws(const __half2* __restrict__ W16, const long* __restrict__ woff, const int* __restrict__ ncol,

@@ -367,8 +367,9 @@ int cgm_tiles_factor(cgm_tiles_t tiles, const cgm_complex128* dev_block, const f
 /* y[b] = P x[b] for b < batch ([batch][N*N]; map b, or the one map).
  * Asynchronous on the context stream.  With one map and several systems the
  * reduced form (T >= 64) runs the systems of each W block back to back on one
- * XCD, so W is read from HBM about once per call; the dense form (T <= 32)
- * applies each inverse to four systems per pass. */
+ * XCD, whose L2 serves the repeated reads (16 systems fetch W from HBM about
+ * 1.5 times per call); the dense form (T <= 32) applies each inverse to four
+ * systems per pass. */
 int cgm_tiles_apply(cgm_tiles_t tiles, const cgm_complex64* dev_x, cgm_complex64* dev_y, int batch);
 /* Algorithmic bytes of one cgm_tiles_apply over `batch` systems: the inverse
  * entries it streams (every entry of every tile in FP32 for T <= 32, once per
