@@ -143,3 +143,22 @@ def test_block_gmres_on_one_map_matches_single_solves(orc):
         assert rel(Xs[c], y) < 2e-5, c
     tl.close()
     ctx.close()
+
+
+@pytest.mark.parametrize("chunk", [0, 64, 96])
+def test_gemm_forms_equal_their_definitions(chunk):
+    """The chunked batched projection, the real-GEMM update and the chunked
+    Gram matrix equal V^H W, W - (V H)^T and conj(W) W^T formed directly (n a
+    multiple of the chunk, and not: then one chunk)."""
+    import torch
+    from cugmres_amd.block import inner, project, update
+    g = torch.Generator().manual_seed(11)
+    n, k, s = 384, 9, 4
+    V = torch.complex(torch.randn(k, n, generator=g, dtype=torch.float64),
+                      torch.randn(k, n, generator=g, dtype=torch.float64))
+    W = torch.complex(torch.randn(s, n, generator=g, dtype=torch.float64),
+                      torch.randn(s, n, generator=g, dtype=torch.float64))
+    H = V.conj() @ W.T
+    assert rel(project(torch, V, W, chunk=chunk).resolve_conj().numpy(), H.numpy()) < 1e-13
+    assert rel(update(torch, W, V, H).numpy(), (W - (V.T @ H).T).numpy()) < 1e-13
+    assert rel(inner(torch, W, W, chunk=chunk).numpy(), (W.conj() @ W.T).numpy()) < 1e-13
