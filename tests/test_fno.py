@@ -259,15 +259,16 @@ def test_fgmres_with_network_reaches_true_tolerance(gpu, orc):
 
 
 @pytest.mark.gpu
-def test_trained_network_gains_over_plain_gmres(gpu, orc):
+@pytest.mark.parametrize("N,least_gain", [(128, 2.0), (256, 2.0)])
+def test_trained_network_gains_over_plain_gmres(gpu, orc, N, least_gain):
     """What training buys: a network trained on the GPU by the library's own
-    kernels (N = 128, 2,400 seeded samples) against plain GMRES(30) on
+    kernels (at the test's N, 2,400 seeded samples) against plain GMRES(30) on
     held-out seeds of the plain workload's kind, at equal iteration counts.
     The untrained network (s_id = 0, W_out = 0) is P = 0, so the baseline
     without it is plain GMRES."""
     solver, fno = gpu
     from cugmres_amd import learn, problems
-    N = 128
+    print(f"trained network gain N={N}", flush=True)
     ctx = solver.Context(N)
     net, hist = learn.train(ctx, samples=2400, batch=8, lr=1e-3, seed=3)
     first = np.mean([h[1] for h in hist[:10]])
@@ -289,7 +290,7 @@ def test_trained_network_gains_over_plain_gmres(gpu, orc):
         op = orc.Op(N, orc.baseline_green_block(N, problems.KH), ctx.M)
         r = b.reshape(-1) - op.matvec(chi.reshape(-1), x.cpu().numpy().astype(np.complex128))
         assert abs(np.linalg.norm(r) / np.linalg.norm(b) - il["rel_residual"]) < 1e-3 * il["rel_residual"] + 1e-6
-    assert min(gains) > 2.0, gains
+    assert min(gains) > least_gain, gains
 
 
 @pytest.mark.gpu
