@@ -258,6 +258,13 @@ def solve_map(ctx, gamma, chi, B, tiles=None, X=None, restart=64, tol=1e-6, max_
     Else everything is complex64.  Returns (X, info) as block_gmres."""
     torch = ctx.torch
     from . import _lib
+    n = ctx.N * ctx.N
+    if B.dim() != 2 or B.shape[1] != n or B.dtype != torch.complex64:
+        raise ValueError(f"solve_map: B must be (s, {n}) complex64, got {tuple(B.shape)} {B.dtype}")
+    if chi.dim() != 1 or chi.shape[0] != n or chi.dtype != torch.float32:
+        raise ValueError(f"solve_map: chi must be ({n},) float32 (one map), got {tuple(chi.shape)} {chi.dtype}")
+    if (gamma64 is None) != (chi64 is None):
+        raise ValueError("solve_map: give both gamma64 and chi64 (the FP64 path) or neither")
     s = int(B.shape[0])
     if tiles is not None and tiles.maps != 1:
         raise ValueError("solve_map: the tiles must be factored for one map")

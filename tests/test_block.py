@@ -162,3 +162,24 @@ def test_gemm_forms_equal_their_definitions(chunk):
     assert rel(project(torch, V, W, chunk=chunk).resolve_conj().numpy(), H.numpy()) < 1e-13
     assert rel(update(torch, W, V, H).numpy(), (W - (V.T @ H).T).numpy()) < 1e-13
     assert rel(inner(torch, W, W, chunk=chunk).numpy(), (W.conj() @ W.T).numpy()) < 1e-13
+
+
+def test_solve_map_rejects_wrong_arguments_before_any_device_work():
+    """Shapes and dtypes are checked first: a batch of maps, a complex128
+    right-hand side block, a wrong length or half of the FP64 pair raise
+    ValueError without touching the (here absent) device."""
+    import torch
+    from cugmres_amd.block import solve_map
+
+    class Ctx:
+        N = 8
+
+    Ctx.torch = torch
+    n = 64
+    B = torch.zeros(3, n, dtype=torch.complex64)
+    chi = torch.zeros(n, dtype=torch.float32)
+    for args, kw in [((B.to(torch.complex128), chi), {}), ((B[:, :10], chi), {}),
+                     ((B, torch.zeros(2, n, dtype=torch.float32)), {}), ((B, chi.double()), {}),
+                     ((B, chi), {"gamma64": object()})]:
+        with pytest.raises(ValueError):
+            solve_map(Ctx(), None, args[1], args[0], **kw)
