@@ -183,3 +183,34 @@ def test_solve_map_rejects_wrong_arguments_before_any_device_work():
                      ((B, chi), {"gamma64": object()})]:
         with pytest.raises(ValueError):
             solve_map(Ctx(), None, args[1], args[0], **kw)
+
+
+def test_block_gmres_on_the_helmholtz_operator_with_oracle_tiles(orc):
+    """The method on the physics, in FP64 on the CPU: the oracle's operator
+    (N = 16, 4-cell block map, eps in {1, 12.1}) and its 8 x 8 tile inverses
+    as P, 4 plane waves at 2 pi k / 4; block GMRES reaches 1e-10 and its
+    solutions equal a dense solve of the operator assembled column by column
+    from the oracle's matvec."""
+    import torch
+    from cugmres_amd import problems
+    from cugmres_amd.block import block_gmres
+    from oracle import tiles
+    N, T, s = 16, 8, 4
+    n = N * N
+    eps = problems.block_eps_map(N, 4, 0.5, 0.75, seed=1)
+    chi = (eps - 1.0).reshape(-1)
+    blk = orc.baseline_green_block(N, problems.KH)
+    op = orc.Op(N, blk)
+    K = tiles.tile_inverses(blk, chi, N, T)
+    B = np.stack([problems.plane_wave(N, problems.KH, 2 * math.pi * k / s).reshape(-1) for k in range(s)])
+
+    def rows(f):
+        return lambda Y: torch.as_tensor(np.stack([f(y) for y in Y.numpy()]))
+
+    X, info = block_gmres(torch, rows(lambda y: op.matvec(chi, y)), rows(lambda y: tiles.tile_apply(K, y, N, T)),
+                          torch.as_tensor(B.astype(np.complex128)), restart=40, tol=1e-10, max_cycles=3)
+    assert all(info["converged"]), info
+    A = np.stack([op.matvec(chi, e) for e in np.eye(n, dtype=np.complex128)], axis=1)
+    ref = np.linalg.solve(A, B.T).T
+    assert rel(X.numpy(), ref) < 1e-8
+    assert info["block_steps"] < n // s
