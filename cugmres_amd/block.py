@@ -181,7 +181,8 @@ def block_gmres(torch, apply_A, apply_P, B, X=None, restart=64, tol=1e-6, max_cy
         try:
             V0, S0 = cholqr2(torch, Ra)
         except BlockBreakdown:
-            # dependent residuals: solve the columns one block of one at a time
+            # dependent residuals: this cycle takes the first column alone, the
+            # others follow in later cycles from their updated residuals
             sa = 1
             idx = idx[:1]
             Ra = Ra[:1]
